@@ -1,0 +1,1128 @@
+/*
+ * vedo_oracle.c -- CPU FP64 restatement of VEDO's DEM step (TEST ORACLE).
+ *
+ * Test infrastructure: loaded only by tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg, as the checker.  Single-threaded like the
+ * reference.  Build with -ffp-contract=off so that every expression rounds
+ * exactly as the reference's x86-64 SSE2 build does.
+ *
+ * Reference quirks and the decisions taken here:
+ *  - Vector3d::direction() of a zero vector is uninitialised in the reference
+ *    (Vector3d.h:117-126); here it is the zero vector.
+ *  - ISwModels is reached through an uninitialised pointer (ImpactSolver.h:70);
+ *    its methods do not touch `this`, so they are free functions here.
+ *  - The reference pairs IactPairTab[i] with cIact[i] when saving histories
+ *    (NBSConsultant.cpp:384-390) although Initiate may skip pairs
+ *    (SimMediator.cpp:686-706).  Here histories are saved from the created
+ *    interactions themselves, which is what the reference does whenever no
+ *    pair is skipped (always, for scenes whose every group pair has a model
+ *    and no two density-0 DOs are paired).
+ *  - NBS rmax is the range of the *last* local DO and its bounding-box max
+ *    starts at numeric_limits<double>::min() (NBSConsultant.cpp:107-140);
+ *    both are kept as written.
+ */
+#include "vedo_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* aries::math::_PI = 4*atan(1) (Constants.h:22) */
+
+typedef struct { double x, y, z; } v3;
+
+static inline v3 mk(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static inline v3 ld(const double *p) { return mk(p[0], p[1], p[2]); }
+static inline void st3(double *p, v3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+static inline v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+/* operator*(T s, Vector3d v) -> (s*v.x, ...) (Vector3d.h:313-316) */
+static inline v3 smul(double s, v3 v) { return mk(s * v.x, s * v.y, s * v.z); }
+/* Vector3d::operator*(T s) -> (v.x*s, ...) (Vector3d.h:268-271) */
+static inline v3 muls(v3 v, double s) { return mk(v.x * s, v.y * s, v.z * s); }
+static inline v3 neg(v3 v) { return mk(-v.x, -v.y, -v.z); }
+/* Vector3d::dot (Vector3d.h:200-203) */
+static inline double dot(v3 a, v3 b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+/* Vector3d::cross (Vector3d.h:215-218) */
+static inline v3 cross(v3 a, v3 b)
+{
+    return mk((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x));
+}
+/* Vector3d::length (Vector3d.h:112-115) */
+static inline double len(v3 a) { return sqrt((a.x * a.x) + (a.y * a.y) + (a.z * a.z)); }
+/* Vector3d::direction (Vector3d.h:117-126); zero for a zero vector */
+static inline v3 dir(v3 a)
+{
+    double l = len(a);
+    if (l != 0) return mk(a.x / l, a.y / l, a.z / l);
+    return mk(0.0, 0.0, 0.0);
+}
+/* Vector3d::project_on (Vector3d.h:194-197) */
+static inline v3 project_on(v3 a, v3 v) { return smul(dot(a, v) / len(v), dir(v)); }
+
+/* Vector3d::rotate_around (Vector3d.h:152-163) */
+static v3 rotate_around(v3 p, v3 va)
+{
+    v3 u = dir(va);
+    double a = len(va);
+    double s = cos(0.5 * a);
+    double t = sin(0.5 * a);
+    v3 v = muls(u, t);
+    v3 t1 = smul(s * s, p);
+    v3 t2 = smul(dot(p, v), v);
+    v3 t3 = smul(2.0 * s, cross(v, p));
+    v3 t4 = cross(v, cross(v, p));
+    return add(add(add(t1, t2), t3), t4);
+}
+
+void vo_rotate_around(const double *p, const double *w, double *out)
+{
+    st3(out, rotate_around(ld(p), ld(w)));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Contact detectors                                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    v3 n, p;
+    double depth;
+    int active;
+} cinfo;
+
+/* CDSphere_Sphere::CalDistance + Detect (CDSphere2Sphere.cpp:14-60) */
+static void cd_ss(v3 xa, double ra, v3 xb, double rb, cinfo *c)
+{
+    v3 im = sub(xb, xa);
+    c->depth = ra + rb - len(im);
+    c->n = dir(im);
+    c->p = add(xa, smul(ra - 0.5 * c->depth, c->n));
+    c->active = c->depth > 0;
+}
+
+/* CDSphere_QuasiPlate::CalDistance + Detect (CDSphere2QuasiPlate.cpp:15-97) */
+static void cd_sp(v3 xa, double ra, v3 xb, v3 ox, v3 oz, double w, double h, double l, cinfo *c)
+{
+    v3 oy = cross(oz, ox);
+    v3 cap = sub(xa, project_on(sub(xa, xb), oz));
+    double hh = 0.5 * h, hw = 0.5 * w, hl = 0.5 * l;
+    double dx = dot(sub(cap, xb), ox);
+    if (dx < -hw) dx = -hw;
+    else if (dx > hw) dx = hw;
+    double dy = dot(sub(cap, xb), oy);
+    if (dy < -hl) dy = -hl;
+    else if (dy > hl) dy = hl;
+    v3 caps = add(add(muls(ox, dx), muls(oy, dy)), xb);
+    v3 im = sub(caps, xa);
+    c->depth = ra + hh - len(im);
+    c->n = dir(im);
+    c->p = add(xa, smul(ra - 0.5 * c->depth, c->n));
+    c->active = c->depth > 0;
+}
+
+/* CDSphere_QuasiCylinder::CalDistance + Detect (CDSphere2QuasiCylinder.cpp:16-80) */
+static void cd_sc(v3 xa, double ra, v3 xb, v3 ax, double rc, double hc, cinfo *c)
+{
+    double hh = 0.5 * hc;
+    double d = dot(sub(xa, xb), ax);
+    if (d < -hh) d = -hh;
+    else if (d > hh) d = hh;
+    v3 caps = add(xb, smul(d, ax));
+    v3 im = sub(caps, xa);
+    c->depth = ra + rc - len(im);
+    c->n = dir(im);
+    c->p = add(xa, smul(ra - 0.5 * c->depth, c->n));
+    c->active = c->depth > 0.0;
+}
+
+static void to_vo(const cinfo *ci, vo_contact *c)
+{
+    c->depth = ci->depth; c->active = ci->active;
+    st3(c->n, ci->n); st3(c->p, ci->p);
+}
+
+void vo_cd_sphere_sphere(const double *xa, double ra, const double *xb, double rb, vo_contact *c)
+{
+    cinfo ci; cd_ss(ld(xa), ra, ld(xb), rb, &ci); to_vo(&ci, c);
+}
+void vo_cd_sphere_plate(const double *xa, double ra, const double *xb, const double *oxb,
+                        const double *ozb, double w, double h, double l, vo_contact *c)
+{
+    cinfo ci; cd_sp(ld(xa), ra, ld(xb), ld(oxb), ld(ozb), w, h, l, &ci); to_vo(&ci, c);
+}
+void vo_cd_sphere_cylinder(const double *xa, double ra, const double *xb, const double *ozb,
+                           double rc, double hc, vo_contact *c)
+{
+    cinfo ci; cd_sc(ld(xa), ra, ld(xb), ld(ozb), rc, hc, &ci); to_vo(&ci, c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* ImpactStatus helpers (ImpactStatus.cpp:6-9, 248-274)                     */
+/* ------------------------------------------------------------------------ */
+
+static void status_default(vo_status *s)
+{
+    memset(s, 0, sizeof(*s));
+    s->last_contact = 1;   /* ImpactStatus() : bLastContact(true) */
+    s->last_bond = 1;      /*                  bLastBond(true)    */
+}
+
+static void status_clean(vo_status *s)   /* ImpactStatus::Clean */
+{
+    memset(s, 0, sizeof(*s));
+}
+
+static void status_clean_contact(vo_status *s)   /* CleanContactStatusAndImpact */
+{
+    s->last_contact = s->contact;
+    s->contact = 0;
+    s->last_bond = s->bond;
+    s->bond = 0;
+    memset(s->shear, 0, sizeof(s->shear));
+    memset(s->to_master, 0, sizeof(s->to_master));
+    memset(s->ang_to_master, 0, sizeof(s->ang_to_master));
+}
+
+/* ------------------------------------------------------------------------ */
+/* ISwModels helpers (ISwModels.cpp)                                         */
+/* ------------------------------------------------------------------------ */
+
+/* CriticalDamping(Ma, Mb, Kn, Ks) (ISwModels.cpp:88-108) */
+static void critical_damping(double ma, double mb, double kn, double ks, double *cn, double *cs)
+{
+    double me;
+    if (mb == 0) me = ma;
+    else if (ma == 0) me = mb;
+    else me = 0.5 * (ma + mb);
+    *cn = 2.0 * sqrt(me * kn);
+    *cs = 2.0 * sqrt(me * ks);
+}
+
+/* NormalBond (ISwModels.cpp:110-125) */
+static int normal_bond(double kn, double bn, double depth)
+{
+    return (depth <= 0.0) && (kn != 0.0) && (bn != 0.0) && (-depth * kn < bn);
+}
+
+/* NormalForce 5-arg (ISwModels.cpp:134-139) */
+static v3 normal_force(double kn, double cn, double depth, v3 n, v3 vn)
+{
+    return sub(smul(-kn * depth, n), smul(cn, vn));
+}
+
+/* NormalForce 6-arg (ISwModels.cpp:163-176) */
+static v3 normal_force_bond(double kn, double cn, double depth, v3 n, v3 vn)
+{
+    if (dot(vn, n) < 0.0) return sub(smul(-kn * depth, n), smul(cn, vn));
+    return smul(-cn, vn);
+}
+
+/* NormalForceHertzSpring (ISwModels.cpp:127-132) */
+static v3 normal_force_hertz(double kn, double cn, double depth, v3 n, v3 vn)
+{
+    return sub(smul(-kn * pow(depth, 1.5), n), smul(cn, vn));
+}
+
+/* ShearForceRotation (ISwModels.cpp:263-273) */
+static v3 shear_rotation(v3 f, v3 d)
+{
+    if (dot(f, d) <= 0.0) return smul(-len(f), dir(d));
+    return smul(len(f), dir(d));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Impact solvers                                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    v3 imp_a, ang_a, ang_b;   /* impulse on A; angular impulses on A and B */
+    int added;                /* AddImpact called on A and B */
+    int linear_only;          /* ISwHertz adds no angular impulse */
+} solve_out;
+
+typedef struct { v3 x, v, w; double m; } body;
+
+/* ISwBSDBF / ISwBSDB2F / ISwBtSDBF ::NextStep
+ * (ISwBSDBF.cpp:26-139, ISwBSDB2F.cpp:26-149, ISwBtSDBF.cpp:26-137) */
+static void solve_sd(int law, const double *p, const body *A, const body *B, const cinfo *ci,
+                     double dt, vo_status *st, solve_out *o)
+{
+    const double kn = p[0], ks = p[1], bn = p[4], bs = p[5], fc = p[6];
+    double cn, cs;
+    critical_damping(A->m, B->m, kn, ks, &cn, &cs);   /* ISwBSDBF.cpp:12-15 */
+    cn = p[2] * cn;
+    cs = p[3] * cs;
+    const double depth = ci->depth;
+    const v3 n = ci->n;
+    o->added = 0; o->linear_only = 0;
+    if (!ci->active && !normal_bond(kn, bn, depth)) {
+        status_clean(st);
+        o->imp_a = o->ang_a = o->ang_b = mk(0, 0, 0);
+        return;
+    }
+    st3(st->point, ci->p); st3(st->dir, ci->n); st->overlap = ci->depth;
+    v3 ra = mk(0, 0, 0), rb = mk(0, 0, 0), fn = mk(0, 0, 0), fs = mk(0, 0, 0);
+    if (depth > 0.0) {
+        ra = sub(ci->p, A->x);
+        rb = sub(ci->p, B->x);
+        v3 rv = sub(add(sub(A->v, B->v), cross(A->w, ra)), cross(B->w, rb));
+        v3 vn = smul(dot(rv, n), n);
+        v3 vs = sub(rv, vn);
+        fn = normal_force(kn, cn, depth, n, vn);
+        if (law == VO_BTSDBF)
+            fs = sub(smul(-ks * dt, vs), smul(cs, vs));
+        else
+            fs = sub(shear_rotation(ld(st->shear), vs), smul(ks * dt, vs));
+        double fmax = fc * kn * depth;
+        if (bs == 0.0) {
+            if (len(fs) > fmax) fs = smul(fmax, dir(fs));
+            else fs = sub(fs, smul(cs, vs));
+        } else {
+            if (len(fs) > bs) fs = smul(fmax, dir(fs));
+            else fs = sub(fs, smul(cs, vs));
+        }
+        if (law != VO_BTSDBF) st3(st->shear, fs);
+        st->contact = 1;
+    } else if (law != VO_BSDB2F || st->last_bond) {
+        v3 rv = sub(A->v, B->v);
+        v3 vn = smul(dot(rv, n), n);
+        if (law == VO_BTSDBF) fn = normal_force(kn, cn, depth, n, vn);
+        else fn = normal_force_bond(kn, cn, depth, n, vn);
+        fs = mk(0, 0, 0);
+        status_clean_contact(st);
+    }
+    v3 f = add(fn, fs);
+    v3 imp = muls(f, dt);
+    o->imp_a = imp;
+    o->ang_a = smul(dt, cross(ra, f));
+    o->ang_b = smul(-dt, cross(rb, f));
+    st3(st->to_master, smul(-1.0, imp));
+    st3(st->ang_to_master, o->ang_b);
+    o->added = 1;
+}
+
+/* ISwH::NextStep (ISwH.cpp:30-156) */
+static void solve_h(const double *p, const body *A, const body *B, const cinfo *ci, double dt,
+                    vo_status *st, solve_out *o)
+{
+    const double knh = p[0], meff = p[1], ksc1 = p[2], ksc2 = p[3], betan = p[4], fk = p[5];
+    const double depth = ci->depth;
+    const v3 n = ci->n;
+    o->added = 0; o->linear_only = 0;
+    if (!ci->active) {
+        status_clean(st);
+        o->imp_a = o->ang_a = o->ang_b = mk(0, 0, 0);
+        return;
+    }
+    st3(st->point, ci->p); st3(st->dir, ci->n); st->overlap = ci->depth;
+    v3 ra = mk(0, 0, 0), rb = mk(0, 0, 0), fn = mk(0, 0, 0), fs = mk(0, 0, 0);
+    if (depth > 0.0) {
+        ra = sub(ci->p, A->x);
+        rb = sub(ci->p, B->x);
+        v3 rv = sub(add(sub(A->v, B->v), cross(A->w, ra)), cross(B->w, rb));
+        v3 vn = smul(dot(rv, n), n);
+        v3 vs = sub(rv, vn);
+        double kn;
+        if (!st->contact) {
+            if (len(vn) == 0.0) kn = knh;
+            else if (betan < 0.99)
+                kn = 0.957832746294131 / (1.0 - betan * betan) * pow(meff, 0.2) * pow(knh, 0.8) *
+                     pow(len(vn), 0.4);
+            else
+                kn = 479.155951122633 * pow(meff, 0.2) * pow(knh, 0.8) * pow(len(vn), 0.4);
+            st->kn = kn;
+            st->contact = 1;
+        } else {
+            kn = st->kn;
+        }
+        double ncd = 2.0 * sqrt(meff * kn);
+        double ks = ksc1 * pow(ksc2 * fabs(kn * depth), 1.0 / 3.0);
+        fn = normal_force(kn, ncd * betan, depth, n, vn);
+        double ff = fk * kn * depth;
+        fs = sub(shear_rotation(ld(st->shear), vs), smul(ks * dt, vs));
+        if (len(fs) >= ff) fs = smul(ff, dir(fs));
+        st3(st->shear, fs);
+    } else {
+        status_clean_contact(st);
+    }
+    v3 f = add(fn, fs);
+    v3 imp = muls(f, dt);
+    o->imp_a = imp;
+    o->ang_a = smul(dt, cross(ra, f));
+    o->ang_b = smul(-dt, cross(rb, f));
+    st3(st->to_master, smul(-1.0, imp));
+    st3(st->ang_to_master, o->ang_b);
+    o->added = 1;
+}
+
+/* ISwHertz::NextStep (ISwHertz.cpp:20-94) */
+static void solve_hertz(const double *p, const body *A, const body *B, const cinfo *ci, double dt,
+                        vo_status *st, solve_out *o)
+{
+    const double kn = p[0], cn = p[1];
+    const double depth = ci->depth;
+    const v3 n = ci->n;
+    o->added = 0; o->linear_only = 1;
+    o->ang_a = o->ang_b = mk(0, 0, 0);
+    if (!ci->active) {
+        status_clean(st);
+        o->imp_a = mk(0, 0, 0);
+        return;
+    }
+    st3(st->point, ci->p); st3(st->dir, ci->n); st->overlap = ci->depth;
+    v3 fn = mk(0, 0, 0);
+    if (depth > 0.0) {
+        v3 ra = sub(ci->p, A->x);
+        v3 rb = sub(ci->p, B->x);
+        v3 rv = sub(add(sub(A->v, B->v), cross(A->w, ra)), cross(B->w, rb));
+        v3 vn = smul(dot(rv, n), n);
+        if (!st->contact) {
+            st->contact = 1;
+            st->init_vel = len(vn);
+        }
+        fn = normal_force_hertz(kn, cn, depth, n, vn);
+    } else {
+        status_clean_contact(st);
+    }
+    v3 imp = muls(fn, dt);
+    o->imp_a = imp;
+    st3(st->to_master, neg(imp));
+    memset(st->ang_to_master, 0, sizeof(st->ang_to_master));
+    o->added = 1;
+}
+
+static void solve(int law, const double *p, const body *A, const body *B, const cinfo *ci,
+                  double dt, vo_status *st, solve_out *o)
+{
+    switch (law) {
+    case VO_BSDBF: case VO_BSDB2F: case VO_BTSDBF: solve_sd(law, p, A, B, ci, dt, st, o); break;
+    case VO_H: solve_h(p, A, B, ci, dt, st, o); break;
+    case VO_HERTZ: solve_hertz(p, A, B, ci, dt, st, o); break;
+    default: memset(o, 0, sizeof(*o)); break;
+    }
+}
+
+int vo_solve(int law, const double *p, double ma, double mb, const vo_contact *c,
+             const double *xa, const double *va, const double *wa,
+             const double *xb, const double *vb, const double *wb,
+             double dt, vo_status *st, double *imp_a, double *ang_a, double *ang_b)
+{
+    body A = {ld(xa), ld(va), ld(wa), ma}, B = {ld(xb), ld(vb), ld(wb), mb};
+    cinfo ci; ci.depth = c->depth; ci.n = ld(c->n); ci.p = ld(c->p); ci.active = c->active;
+    solve_out o;
+    solve(law, p, &A, &B, &ci, dt, st, &o);
+    st3(imp_a, o.imp_a); st3(ang_a, o.ang_a); st3(ang_b, o.ang_b);
+    return o.added;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Simulation state                                                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    unsigned slave, master;   /* DO ids (Assembler::CreateInteraction ordering) */
+    unsigned lo, hi;          /* IactPairTab key */
+    int law;
+    const double *p;
+    cinfo ci;
+    vo_status st;
+} iact;
+
+typedef struct {
+    unsigned lo, hi;
+    vo_status st;
+} record;
+
+struct vo_sim {
+    int n_do, n_models, n_groups;
+    vo_model *models;
+    vo_iact *iact;
+    int *model_of;
+    /* derived model quantities (DOModel::Analysis, DOModel.cpp:17-149) */
+    double *range, *mass, *sudo;
+    /* per-DO state and accumulators (DOStatus + DiscreteObject) */
+    v3 *pos, *vel, *ox, *oz, *av, *imp, *aimp, *last_imp, *last_aimp;
+    double *do_sudo;          /* DiscreteObject::dSudoMass (0 for fixed/orbital) */
+    double dt, t_start, t_stop, t_current;
+    v3 g;
+    double sf;
+    unsigned up_iact, record_every, round;
+    int consultant;
+    /* candidates (IactPairTab) and interactions (cIact) */
+    unsigned *cand_lo, *cand_hi;
+    int n_cand, cap_cand;
+    iact *ia;
+    int n_ia;
+    /* IactRecordTab */
+    record *rec;
+    int n_rec;
+};
+
+static double model_range(const vo_model *m)
+{
+    switch (m->shape) {
+    case VO_SPHERE: return m->attr[0];
+    case VO_QUASICYLINDER:
+        if (m->scope == VO_LOCAL) return fmax(m->attr[0], 0.5 * m->attr[1]);
+        return m->attr[0];
+    case VO_QUASIPLATE:
+        if (m->scope == VO_LOCAL) {
+            double a = m->attr[0] > m->attr[1] ? m->attr[0] : m->attr[1];
+            a = a > m->attr[2] ? a : m->attr[2];
+            return 0.5 * a;
+        }
+        return 0.5 * m->attr[1];
+    default: return 0.0;
+    }
+}
+
+static void model_mass(const vo_model *m, double *mass, double *sudo)
+{
+    const double PI = 4.0 * atan(1.0);
+    double vol = 0.0, r, h, w, l;
+    switch (m->shape) {
+    case VO_SPHERE:
+        r = m->attr[0];
+        vol = (4.0 / 3.0 * PI) * r * r * r;   /* _FourthThirdsPI * r*r*r */
+        break;
+    case VO_QUASICYLINDER:
+        r = m->attr[0]; h = m->attr[1];
+        vol = PI * r * r * h;
+        break;
+    case VO_QUASIPLATE:
+        w = m->attr[0]; h = m->attr[1]; l = m->attr[2];
+        vol = w * h * l;
+        break;
+    }
+    *mass = m->density * vol;
+    *sudo = m->density_factor * *mass;
+}
+
+static const vo_iact *iact_of(const vo_sim *s, unsigned a, unsigned b)
+{
+    int ga = s->models[s->model_of[a]].group, gb = s->models[s->model_of[b]].group;
+    return &s->iact[ga * s->n_groups + gb];
+}
+
+static int push_cand(vo_sim *s, unsigned a, unsigned b)
+{
+    if (s->n_cand == s->cap_cand) {
+        int nc = s->cap_cand ? 2 * s->cap_cand : 1024;
+        unsigned *lo = realloc(s->cand_lo, sizeof(unsigned) * nc);
+        unsigned *hi = realloc(s->cand_hi, sizeof(unsigned) * nc);
+        if (!lo || !hi) { free(lo ? lo : NULL); return -1; }
+        s->cand_lo = lo; s->cand_hi = hi; s->cap_cand = nc;
+    }
+    s->cand_lo[s->n_cand] = a < b ? a : b;
+    s->cand_hi[s->n_cand] = a < b ? b : a;
+    s->n_cand++;
+    return 0;
+}
+
+/* DOMap::CalDistance (DOMap.cpp:164-414), no periodic boundary */
+static double domap_distance(const vo_sim *s, unsigned a, unsigned b)
+{
+    const vo_model *ma = &s->models[s->model_of[a]], *mb = &s->models[s->model_of[b]];
+    if (ma->shape > mb->shape) { unsigned t = a; a = b; b = t; const vo_model *tm = ma; ma = mb; mb = tm; }
+    if (ma->shape == VO_SPHERE && mb->shape == VO_SPHERE) {
+        return len(sub(s->pos[a], s->pos[b]));
+    }
+    if (ma->shape == VO_SPHERE && mb->shape == VO_QUASIPLATE) {
+        v3 ca = s->pos[a], cb = s->pos[b], oz = s->oz[b], ox = s->ox[b];
+        v3 oy = cross(oz, ox);
+        v3 cap = sub(ca, project_on(sub(ca, cb), oz));
+        double hw = 0.5 * mb->attr[0], hl = 0.5 * mb->attr[2];
+        double dx = dot(sub(cap, cb), ox);
+        if (dx < -hw) dx = -hw; else if (dx > hw) dx = hw;
+        double dy = dot(sub(cap, cb), oy);
+        if (dy < -hl) dy = -hl; else if (dy > hl) dy = hl;
+        v3 caps = add(add(muls(ox, dx), muls(oy, dy)), cb);
+        return len(sub(caps, ca));
+    }
+    if (ma->shape == VO_SPHERE && mb->shape == VO_QUASICYLINDER) {
+        double hh = 0.5 * mb->attr[1];
+        v3 ca = s->pos[a], cb = s->pos[b], ax = s->oz[b];
+        double d = dot(sub(ca, cb), ax);
+        v3 cap = add(cb, muls(ax, d));
+        v3 im;
+        if (d < hh && d > -hh) im = sub(cap, ca);
+        else if (d >= hh) im = sub(add(cb, muls(ax, hh)), ca);
+        else im = sub(sub(cb, muls(ax, hh)), ca);
+        return len(im);
+    }
+    return INFINITY;
+}
+
+/* BuildIactTab(v1, v2) and BuildIactTab(v) (LeapConsultant.cpp:198-262,
+ * NBSConsultant.cpp:318-374) */
+static int build_pairs(vo_sim *s, const unsigned *v1, int n1, const unsigned *v2, int n2,
+                       const double *safe)
+{
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j) {
+            unsigned a = v1[i], b = v2[j];
+            if (iact_of(s, a, b)->law == VO_LAW_NONE) continue;
+            if (domap_distance(s, a, b) > safe[a] + safe[b]) continue;
+            if (push_cand(s, a, b)) return -1;
+        }
+    return 0;
+}
+static int build_pairs_self(vo_sim *s, const unsigned *v, int n, const double *safe)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            unsigned a = v[i], b = v[j];
+            if (iact_of(s, a, b)->law == VO_LAW_NONE) continue;
+            if (domap_distance(s, a, b) > safe[a] + safe[b]) continue;
+            if (push_cand(s, a, b)) return -1;
+        }
+    return 0;
+}
+
+typedef struct { unsigned k[3]; unsigned id; } cellkey;   /* k in comparator order */
+
+static int cmp_cell(const void *pa, const void *pb)
+{
+    const cellkey *a = pa, *b = pb;
+    for (int d = 0; d < 3; ++d)
+        if (a->k[d] != b->k[d]) return a->k[d] < b->k[d] ? -1 : 1;
+    return a->id < b->id ? -1 : (a->id > b->id);
+}
+
+static int cmp_double(const void *pa, const void *pb)
+{
+    double a = *(const double *)pa, b = *(const double *)pb;
+    return (a > b) - (a < b);
+}
+
+/* Binary search for the cell (ix,iy,iz) in the sorted key array; returns the
+ * first index or -1.  perm maps (x,y,z) to comparator order. */
+static int find_cell(const cellkey *ck, int n, const int *perm, unsigned x, unsigned y, unsigned z)
+{
+    unsigned xyz[3] = {x, y, z};
+    unsigned key[3] = {xyz[perm[0]], xyz[perm[1]], xyz[perm[2]]};
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        int c = 0;
+        for (int d = 0; d < 3 && !c; ++d)
+            if (ck[mid].k[d] != key[d]) c = ck[mid].k[d] < key[d] ? -1 : 1;
+        if (c < 0) lo = mid + 1; else hi = mid;
+    }
+    if (lo < n && ck[lo].k[0] == key[0] && ck[lo].k[1] == key[1] && ck[lo].k[2] == key[2]) return lo;
+    return -1;
+}
+
+/* Shared grid traversal. offsets: 13 neighbour (dx,dy,dz) in reference order;
+ * self/global inserted at position `self_at`. */
+typedef struct {
+    cellkey *ck; int nck;
+    int *cell_start, *cell_len; int ncell;   /* per distinct cell (sorted) */
+    unsigned *ids;                          /* DO ids grouped by cell */
+    int perm[3];
+} grid;
+
+static void grid_free(grid *g) { free(g->ck); free(g->cell_start); free(g->cell_len); free(g->ids); }
+
+static int grid_build(const vo_sim *s, grid *g, const unsigned *loc, int nloc, double xmin, double ymin,
+                      double zmin, double zr, const int perm[3])
+{
+    memset(g, 0, sizeof(*g));
+    g->perm[0] = perm[0]; g->perm[1] = perm[1]; g->perm[2] = perm[2];
+    g->ck = malloc(sizeof(cellkey) * (nloc ? nloc : 1));
+    g->ids = malloc(sizeof(unsigned) * (nloc ? nloc : 1));
+    g->cell_start = malloc(sizeof(int) * (nloc ? nloc : 1));
+    g->cell_len = malloc(sizeof(int) * (nloc ? nloc : 1));
+    if (!g->ck || !g->ids || !g->cell_start || !g->cell_len) return -1;
+    for (int i = 0; i < nloc; ++i) {
+        v3 p = s->pos[loc[i]];
+        unsigned xyz[3] = {(unsigned)((p.x - xmin) / zr), (unsigned)((p.y - ymin) / zr),
+                           (unsigned)((p.z - zmin) / zr)};
+        g->ck[i].k[0] = xyz[perm[0]]; g->ck[i].k[1] = xyz[perm[1]]; g->ck[i].k[2] = xyz[perm[2]];
+        g->ck[i].id = loc[i];
+    }
+    qsort(g->ck, nloc, sizeof(cellkey), cmp_cell);
+    int nc = 0;
+    for (int i = 0; i < nloc; ++i) {
+        g->ids[i] = g->ck[i].id;
+        if (i == 0 || memcmp(g->ck[i].k, g->ck[i - 1].k, sizeof(g->ck[i].k))) {
+            g->cell_start[nc] = i; g->cell_len[nc] = 0;
+            g->ck[nc] = g->ck[i];   /* compact distinct keys in front (i >= nc) */
+            nc++;
+        }
+        g->cell_len[nc - 1]++;
+    }
+    g->ncell = nc; g->nck = nc;
+    return 0;
+}
+
+static void unperm(const grid *g, int c, unsigned *x, unsigned *y, unsigned *z)
+{
+    unsigned xyz[3];
+    for (int d = 0; d < 3; ++d) xyz[g->perm[d]] = g->ck[c].k[d];
+    *x = xyz[0]; *y = xyz[1]; *z = xyz[2];
+}
+
+static const int NB13[13][3] = {
+    {-1, -1, -1}, {-1, -1, 0}, {-1, -1, 1}, {0, -1, -1}, {0, -1, 0}, {0, -1, 1},
+    {1, -1, -1}, {1, -1, 0}, {1, -1, 1}, {-1, 0, -1}, {-1, 0, 0}, {0, 0, -1}, {1, 0, -1}};
+
+/* safe distance per DO: NBSConsultant.cpp:133-135 / LeapConsultant.h:117-119 */
+static void safe_lengths(const vo_sim *s, double *safe)
+{
+    double dtu = s->up_iact * s->dt;
+    double glen = len(s->g);
+    for (int i = 0; i < s->n_do; ++i) {
+        double rng = s->range[s->model_of[i]];
+        safe[i] = s->sf * (2.0 * rng + dtu * len(s->vel[i]) + 0.5 * dtu * dtu * glen);
+    }
+}
+
+/* LeapConsultant::Reset + subReset (LeapConsultant.cpp:108-195, LeapConsultant.h:57-514),
+ * non-periodic branch. */
+static int reset_leap(vo_sim *s)
+{
+    s->n_cand = 0;
+    int n = s->n_do, nloc = 0, nglob = 0;
+    unsigned *loc = malloc(sizeof(unsigned) * (n ? n : 1));
+    unsigned *glob = malloc(sizeof(unsigned) * (n ? n : 1));
+    double *safe = malloc(sizeof(double) * (n ? n : 1));
+    double *vx = malloc(sizeof(double) * (n ? n : 1)), *vy = malloc(sizeof(double) * (n ? n : 1));
+    double *vz = malloc(sizeof(double) * (n ? n : 1)), *vv = malloc(sizeof(double) * (n ? n : 1));
+    double *vr = malloc(sizeof(double) * (n ? n : 1));
+    int rc = -1;
+    grid g; memset(&g, 0, sizeof(g));
+    if (!loc || !glob || !safe || !vx || !vy || !vz || !vv || !vr) goto out;
+    for (int i = 0; i < n; ++i) {
+        const vo_model *m = &s->models[s->model_of[i]];
+        if (m->scope == VO_LOCAL) {
+            vx[nloc] = s->pos[i].x; vy[nloc] = s->pos[i].y; vz[nloc] = s->pos[i].z;
+            vv[nloc] = len(s->vel[i]); vr[nloc] = s->range[s->model_of[i]];
+            loc[nloc++] = (unsigned)i;
+        } else {
+            glob[nglob++] = (unsigned)i;
+        }
+    }
+    if (nloc == 0) { rc = 0; goto out; }   /* the reference indexes an empty vector here */
+    qsort(vx, nloc, sizeof(double), cmp_double); qsort(vy, nloc, sizeof(double), cmp_double);
+    qsort(vz, nloc, sizeof(double), cmp_double); qsort(vv, nloc, sizeof(double), cmp_double);
+    qsort(vr, nloc, sizeof(double), cmp_double);
+    double xmin = vx[0], xmax = vx[nloc - 1], ymin = vy[0], ymax = vy[nloc - 1];
+    double zmin = vz[0], zmax = vz[nloc - 1], vmax = vv[nloc - 1], rmax = vr[nloc - 1];
+    const double discard = 0.02, keep = 1.0 - discard;
+    size_t ihi = (size_t)(keep * nloc - 1), ilo = (size_t)(discard * nloc);
+    double xs = vx[ihi] - vx[ilo], ys = vy[ihi] - vy[ilo], zs = vz[ihi] - vz[ilo];
+    int perm[3];
+    /* TrirLeX: x,y,z  TrirLeY: y,x,z  TrirLeZ: z,x,y (NBSConsultant.h:52-118) */
+    if (xs >= ys) {
+        if (xs >= zs) { perm[0] = 0; perm[1] = 1; perm[2] = 2; }
+        else { perm[0] = 2; perm[1] = 0; perm[2] = 1; }
+    } else {
+        if (ys >= zs) { perm[0] = 1; perm[1] = 0; perm[2] = 2; }
+        else { perm[0] = 2; perm[1] = 0; perm[2] = 1; }
+    }
+    (void)xmax; (void)ymax; (void)zmax;
+    safe_lengths(s, safe);
+    double dtu = s->up_iact * s->dt;
+    double zr = s->sf * (2.0 * rmax + dtu * vmax + 0.5 * dtu * dtu * len(s->g));
+    if (grid_build(s, &g, loc, nloc, xmin, ymin, zmin, zr, perm)) goto out;
+    for (int c = 0; c < g.ncell; ++c) {
+        unsigned ix, iy, iz;
+        unperm(&g, c, &ix, &iy, &iz);
+        const unsigned *v1 = g.ids + g.cell_start[c];
+        int n1 = g.cell_len[c];
+        for (int k = 0; k < 13; ++k) {
+            int c2 = find_cell(g.ck, g.ncell, g.perm, ix + NB13[k][0], iy + NB13[k][1], iz + NB13[k][2]);
+            if (c2 >= 0 && build_pairs(s, v1, n1, g.ids + g.cell_start[c2], g.cell_len[c2], safe)) goto out;
+        }
+        if (build_pairs_self(s, v1, n1, safe)) goto out;
+        if (build_pairs(s, v1, n1, glob, nglob, safe)) goto out;
+    }
+    rc = 0;
+out:
+    grid_free(&g);
+    free(loc); free(glob); free(safe); free(vx); free(vy); free(vz); free(vv); free(vr);
+    return rc;
+}
+
+/* NBSConsultant::Reset (NBSConsultant.cpp:85-316) */
+static int reset_nbs(vo_sim *s)
+{
+    s->n_cand = 0;
+    int n = s->n_do, nloc = 0, nglob = 0;
+    unsigned *loc = malloc(sizeof(unsigned) * (n ? n : 1));
+    unsigned *glob = malloc(sizeof(unsigned) * (n ? n : 1));
+    double *safe = malloc(sizeof(double) * (n ? n : 1));
+    int rc = -1;
+    grid g; memset(&g, 0, sizeof(g));
+    if (!loc || !glob || !safe) goto out;
+    double xmin = DBL_MAX, xmax = DBL_MIN, ymin = DBL_MAX, ymax = DBL_MIN, zmin = DBL_MAX, zmax = DBL_MIN;
+    double vmax = 0.0, rmax = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const vo_model *m = &s->models[s->model_of[i]];
+        if (m->scope == VO_LOCAL) {
+            v3 p = s->pos[i];
+            xmin = fmin(xmin, p.x); xmax = fmax(xmax, p.x);
+            ymin = fmin(ymin, p.y); ymax = fmax(ymax, p.y);
+            zmin = fmin(zmin, p.z); zmax = fmax(zmax, p.z);
+            double vl = len(s->vel[i]);
+            vmax = vmax > vl ? vmax : vl;
+            rmax = s->range[s->model_of[i]];
+            loc[nloc++] = (unsigned)i;
+        } else {
+            glob[nglob++] = (unsigned)i;
+        }
+    }
+    safe_lengths(s, safe);
+    double dtu = s->up_iact * s->dt;
+    double zr = s->sf * (2.0 * rmax + dtu * vmax + 0.5 * dtu * dtu * len(s->g));
+    unsigned ncx = (unsigned)ceil((xmax - xmin) / zr); if (ncx < 1) ncx = 1;
+    unsigned ncy = (unsigned)ceil((ymax - ymin) / zr); if (ncy < 1) ncy = 1;
+    unsigned ncz = (unsigned)ceil((zmax - zmin) / zr); if (ncz < 1) ncz = 1;
+    int perm[3] = {1, 0, 2};   /* TrirLeY; only affects lookup, not traversal */
+    if (grid_build(s, &g, loc, nloc, xmin, ymin, zmin, zr, perm)) goto out;
+    for (unsigned ix = 0; ix < ncx; ++ix)
+        for (unsigned iy = 0; iy < ncy; ++iy)
+            for (unsigned iz = 0; iz < ncz; ++iz) {
+                int c = find_cell(g.ck, g.ncell, g.perm, ix, iy, iz);
+                if (c < 0) continue;
+                const unsigned *v1 = g.ids + g.cell_start[c];
+                int n1 = g.cell_len[c];
+                for (int k = 0; k < 12; ++k) {
+                    int c2 = find_cell(g.ck, g.ncell, g.perm, ix + NB13[k][0], iy + NB13[k][1], iz + NB13[k][2]);
+                    if (c2 >= 0 && build_pairs(s, v1, n1, g.ids + g.cell_start[c2], g.cell_len[c2], safe)) goto out;
+                }
+                if (build_pairs_self(s, v1, n1, safe)) goto out;
+                if (build_pairs(s, v1, n1, glob, nglob, safe)) goto out;
+                int c2 = find_cell(g.ck, g.ncell, g.perm, ix + 1, iy, iz - 1);
+                if (c2 >= 0 && build_pairs(s, v1, n1, g.ids + g.cell_start[c2], g.cell_len[c2], safe)) goto out;
+            }
+    rc = 0;
+out:
+    grid_free(&g);
+    free(loc); free(glob); free(safe);
+    return rc;
+}
+
+static int cmp_rec(const void *pa, const void *pb)
+{
+    const record *a = pa, *b = pb;
+    if (a->lo != b->lo) return a->lo < b->lo ? -1 : 1;
+    return (a->hi > b->hi) - (a->hi < b->hi);
+}
+
+static const record *find_rec(const vo_sim *s, unsigned lo, unsigned hi)
+{
+    int a = 0, b = s->n_rec;
+    while (a < b) {
+        int m = (a + b) / 2;
+        if (s->rec[m].lo < lo || (s->rec[m].lo == lo && s->rec[m].hi < hi)) a = m + 1; else b = m;
+    }
+    if (a < s->n_rec && s->rec[a].lo == lo && s->rec[a].hi == hi) return &s->rec[a];
+    return NULL;
+}
+
+static void detect(vo_sim *s, iact *it)
+{
+    unsigned a = it->slave, b = it->master;
+    const vo_model *ma = &s->models[s->model_of[a]], *mb = &s->models[s->model_of[b]];
+    if (mb->shape == VO_SPHERE)
+        cd_ss(s->pos[a], ma->attr[0], s->pos[b], mb->attr[0], &it->ci);
+    else if (mb->shape == VO_QUASIPLATE)
+        cd_sp(s->pos[a], ma->attr[0], s->pos[b], s->ox[b], s->oz[b], mb->attr[0], mb->attr[1], mb->attr[2], &it->ci);
+    else if (mb->shape == VO_QUASICYLINDER)
+        cd_sc(s->pos[a], ma->attr[0], s->pos[b], s->oz[b], mb->attr[0], mb->attr[1], &it->ci);
+}
+
+/* SimMediator::Initiate (SimMediator.cpp:640-712) + Assembler::CreateInteraction
+ * (Assembler.cpp:15-87). */
+static int initiate(vo_sim *s)
+{
+    free(s->ia);
+    s->ia = malloc(sizeof(iact) * (s->n_cand ? s->n_cand : 1));
+    if (!s->ia) return -1;
+    s->n_ia = 0;
+    for (int k = 0; k < s->n_cand; ++k) {
+        unsigned p1 = s->cand_lo[k], p2 = s->cand_hi[k];
+        const vo_iact *im = iact_of(s, p1, p2);
+        if (im->law == VO_LAW_NONE) continue;
+        const vo_model *m1 = &s->models[s->model_of[p1]], *m2 = &s->models[s->model_of[p2]];
+        if (p1 == p2) continue;
+        if (m2->density < 0.0 || m1->density < 0.0) continue;
+        if (m2->density == 0.0 && m1->density == 0.0) continue;
+        unsigned sl, ms;
+        if (m1->shape > m2->shape) { sl = p2; ms = p1; }
+        else if (m1->shape < m2->shape) { sl = p1; ms = p2; }
+        else if (m1->density == 0.0) { sl = p2; ms = p1; }
+        else { sl = p1; ms = p2; }
+        const vo_model *msl = &s->models[s->model_of[sl]], *mms = &s->models[s->model_of[ms]];
+        /* CDcf: only the three sphere-slave detectors are on this path */
+        if (msl->shape != VO_SPHERE ||
+            (mms->shape != VO_SPHERE && mms->shape != VO_QUASIPLATE && mms->shape != VO_QUASICYLINDER))
+            continue;
+        iact *it = &s->ia[s->n_ia++];
+        it->slave = sl; it->master = ms; it->lo = p1; it->hi = p2;
+        it->law = im->law; it->p = im->p;
+        const record *r = find_rec(s, p1, p2);
+        if (r) it->st = r->st; else status_default(&it->st);
+        detect(s, it);
+    }
+    return 0;
+}
+
+/* RebuildIactRecordTab (LeapConsultant.cpp:264-279 / NBSConsultant.cpp:376-395) */
+static int rebuild_records(vo_sim *s)
+{
+    int n = 0;
+    for (int k = 0; k < s->n_ia; ++k) n += s->ia[k].ci.active;
+    free(s->rec);
+    s->rec = malloc(sizeof(record) * (n ? n : 1));
+    if (!s->rec) return -1;
+    s->n_rec = 0;
+    for (int k = 0; k < s->n_ia; ++k)
+        if (s->ia[k].ci.active) {
+            record *r = &s->rec[s->n_rec++];
+            r->lo = s->ia[k].lo; r->hi = s->ia[k].hi; r->st = s->ia[k].st;
+        }
+    /* std::map keeps the first insertion of a key; keys are unique here */
+    qsort(s->rec, s->n_rec, sizeof(record), cmp_rec);
+    return 0;
+}
+
+static int consultant_reset(vo_sim *s)
+{
+    return s->consultant == 1 ? reset_nbs(s) : reset_leap(s);
+}
+
+vo_sim *vo_create(const vo_scene *sc, char *err, int errlen)
+{
+    vo_sim *s = calloc(1, sizeof(vo_sim));
+    if (!s) return NULL;
+    int n = sc->n_do;
+    s->n_do = n; s->n_models = sc->n_models; s->n_groups = sc->n_groups;
+    s->models = malloc(sizeof(vo_model) * sc->n_models);
+    s->iact = malloc(sizeof(vo_iact) * sc->n_groups * sc->n_groups);
+    s->model_of = malloc(sizeof(int) * (n ? n : 1));
+    s->range = malloc(sizeof(double) * sc->n_models);
+    s->mass = malloc(sizeof(double) * sc->n_models);
+    s->sudo = malloc(sizeof(double) * sc->n_models);
+    v3 **vs[] = {&s->pos, &s->vel, &s->ox, &s->oz, &s->av, &s->imp, &s->aimp, &s->last_imp, &s->last_aimp};
+    for (unsigned k = 0; k < sizeof(vs) / sizeof(vs[0]); ++k) *vs[k] = calloc(n ? n : 1, sizeof(v3));
+    s->do_sudo = malloc(sizeof(double) * (n ? n : 1));
+    if (!s->models || !s->iact || !s->model_of || !s->range || !s->mass || !s->sudo || !s->do_sudo ||
+        !s->pos || !s->vel || !s->ox || !s->oz || !s->av || !s->imp || !s->aimp || !s->last_imp ||
+        !s->last_aimp) {
+        snprintf(err, errlen, "out of memory");
+        vo_destroy(s);
+        return NULL;
+    }
+    memcpy(s->models, sc->models, sizeof(vo_model) * sc->n_models);
+    memcpy(s->iact, sc->iact, sizeof(vo_iact) * sc->n_groups * sc->n_groups);
+    memcpy(s->model_of, sc->model_of, sizeof(int) * n);
+    for (int m = 0; m < sc->n_models; ++m) {
+        s->range[m] = model_range(&s->models[m]);
+        model_mass(&s->models[m], &s->mass[m], &s->sudo[m]);
+    }
+    for (int i = 0; i < n; ++i) {
+        s->pos[i] = ld(sc->pos + 3 * i); s->vel[i] = ld(sc->vel + 3 * i);
+        s->ox[i] = ld(sc->ox + 3 * i); s->oz[i] = ld(sc->oz + 3 * i); s->av[i] = ld(sc->av + 3 * i);
+        const vo_model *m = &s->models[s->model_of[i]];
+        /* fixed/orbital wall ctors set dSudoMass = 0 (DOFixedQuasiPlate.cpp:12) */
+        if (m->behavior == VO_FIXED && m->shape != VO_SPHERE) s->do_sudo[i] = 0.0;
+        else if (m->behavior == VO_ORBITAL) s->do_sudo[i] = 0.0;
+        else s->do_sudo[i] = s->sudo[s->model_of[i]];
+    }
+    s->dt = sc->dt; s->t_start = sc->t_start; s->t_stop = sc->t_stop; s->t_current = sc->t_current;
+    s->g = ld(sc->g); s->sf = sc->safety_factor;
+    s->up_iact = sc->up_iact ? sc->up_iact : 1; s->record_every = sc->record ? sc->record : 0xffffffffu;
+    s->consultant = sc->consultant;
+    if (consultant_reset(s) || initiate(s)) {
+        snprintf(err, errlen, "out of memory in consultant reset");
+        vo_destroy(s);
+        return NULL;
+    }
+    return s;
+}
+
+void vo_destroy(vo_sim *s)
+{
+    if (!s) return;
+    free(s->models); free(s->iact); free(s->model_of); free(s->range); free(s->mass); free(s->sudo);
+    free(s->pos); free(s->vel); free(s->ox); free(s->oz); free(s->av); free(s->imp); free(s->aimp);
+    free(s->last_imp); free(s->last_aimp); free(s->do_sudo);
+    free(s->cand_lo); free(s->cand_hi); free(s->ia); free(s->rec);
+    free(s);
+}
+
+int vo_set_history(vo_sim *s, int n, const unsigned *lo, const unsigned *hi, const vo_status *st)
+{
+    free(s->rec);
+    s->rec = malloc(sizeof(record) * (n ? n : 1));
+    if (!s->rec) return -1;
+    for (int k = 0; k < n; ++k) { s->rec[k].lo = lo[k]; s->rec[k].hi = hi[k]; s->rec[k].st = st[k]; }
+    s->n_rec = n;
+    qsort(s->rec, n, sizeof(record), cmp_rec);
+    return initiate(s);
+}
+
+/* DOSphere::Response (DOSphere.cpp:21-48) and the wall responses
+ * (DOFixedQuasiPlate.h:16-19, DOOrbitalQuasiPlate.cpp:18-34,
+ * DOConstrainedQuasiPlate.cpp:18-35, DOOrbitalQuasiCylinder.cpp:17-33). */
+static void response(vo_sim *s, int i)
+{
+    const vo_model *m = &s->models[s->model_of[i]];
+    double dt = s->dt;
+    if (m->shape == VO_SPHERE && m->behavior == VO_MOBILE) {
+        v3 V = s->vel[i], AV = s->av[i], P = s->pos[i], Ox = s->ox[i], Oz = s->oz[i];
+        double r = m->attr[0], sm = s->do_sudo[i];
+        v3 dv = smul(1.0 / sm, s->imp[i]);
+        v3 dav = smul(2.5 / sm / r / r, s->aimp[i]);
+        v3 dp = smul(dt, add(V, smul(0.5, dv)));
+        v3 dw = smul(dt, add(AV, smul(0.5, dav)));
+        s->pos[i] = add(P, dp);
+        s->ox[i] = rotate_around(Ox, dw);
+        s->oz[i] = rotate_around(Oz, dw);
+        s->vel[i] = add(V, dv);
+        s->av[i] = add(AV, dav);
+    } else if (m->behavior == VO_ORBITAL && m->shape != VO_SPHERE) {
+        v3 dp = muls(s->vel[i], dt), dw = muls(s->av[i], dt);
+        s->pos[i] = add(s->pos[i], dp);
+        s->ox[i] = rotate_around(s->ox[i], dw);
+        s->oz[i] = rotate_around(s->oz[i], dw);
+    } else if (m->behavior == VO_CONSTRAINED && m->shape != VO_SPHERE) {
+        v3 V = s->vel[i], Oz = s->oz[i];
+        v3 dv = smul(1.0 / s->do_sudo[i], smul(dot(s->imp[i], Oz), dir(Oz)));
+        v3 dp = smul(dt, add(V, smul(0.5, dv)));
+        v3 dw = muls(s->av[i], dt);
+        s->pos[i] = add(s->pos[i], dp);
+        s->ox[i] = rotate_around(s->ox[i], dw);
+        s->oz[i] = rotate_around(Oz, dw);
+        s->vel[i] = add(V, dv);
+    }
+    /* fixed: ClearImpact only */
+    s->last_imp[i] = s->imp[i]; s->last_aimp[i] = s->aimp[i];
+    s->imp[i] = mk(0, 0, 0); s->aimp[i] = mk(0, 0, 0);
+}
+
+static int step_impl(vo_sim *s, int nas, const unsigned *ids, const double *apos, const double *avel,
+                     const double *aox, const double *aoz, const double *aav)
+{
+    double dt = s->dt;
+    /* 1. IactContainer::CalculateImpact (IactContainer.cpp:33-37) */
+    for (int k = 0; k < s->n_ia; ++k) {
+        iact *it = &s->ia[k];
+        unsigned a = it->slave, b = it->master;
+        body A = {s->pos[a], s->vel[a], s->av[a], s->do_sudo[a]};
+        body B = {s->pos[b], s->vel[b], s->av[b], s->do_sudo[b]};
+        solve_out o;
+        solve(it->law, it->p, &A, &B, &it->ci, dt, &it->st, &o);
+        if (o.added) {
+            /* DiscreteObject::AddImpact (DiscreteObject.cpp:44-53) */
+            s->imp[a] = add(s->imp[a], o.imp_a);
+            s->imp[b] = add(s->imp[b], o.linear_only ? neg(o.imp_a) : smul(-1.0, o.imp_a));
+            if (!o.linear_only) {
+                s->aimp[a] = add(s->aimp[a], o.ang_a);
+                s->aimp[b] = add(s->aimp[b], o.ang_b);
+            }
+        }
+    }
+    /* 3. AddFieldImpact + AddConstrainedImpact (DOContainer.cpp:20-50) */
+    v3 gdt = muls(s->g, dt);
+    for (int i = 0; i < s->n_do; ++i) s->imp[i] = add(s->imp[i], muls(gdt, s->do_sudo[i]));
+    for (int i = 0; i < s->n_do; ++i) {
+        const vo_model *m = &s->models[s->model_of[i]];
+        s->imp[i] = add(s->imp[i], smul(dt, ld(m->fext)));
+    }
+    /* 4. Response */
+    for (int i = 0; i < s->n_do; ++i) response(s, i);
+    for (int k = 0; k < nas; ++k) {
+        unsigned i = ids[k];
+        if ((int)i >= s->n_do) return -1;
+        s->pos[i] = ld(apos + 3 * k); s->vel[i] = ld(avel + 3 * k);
+        s->ox[i] = ld(aox + 3 * k); s->oz[i] = ld(aoz + 3 * k); s->av[i] = ld(aav + 3 * k);
+        /* DOContainer::SetDOStatus replaces the whole DOStatus, including the
+         * stored last impulses, which an assigned status carries as zero */
+        s->last_imp[i] = mk(0, 0, 0); s->last_aimp[i] = mk(0, 0, 0);
+    }
+    /* 5. CheckContactStatus */
+    for (int k = 0; k < s->n_ia; ++k) detect(s, &s->ia[k]);
+    /* 6. Consultant::NextStep (LeapConsultant.cpp:49-97 / NBSConsultant.cpp:35-78) */
+    s->round++;
+    s->t_current += dt;   /* SystemParameter::NextStep */
+    int rebuild = 0, reset = (s->round % s->up_iact) == 0;
+    if (reset) {
+        if (rebuild_records(s)) return -1;
+        rebuild = 1;
+        if (consultant_reset(s)) return -1;
+    }
+    if ((s->round % s->record_every) == 0 && !rebuild) {
+        if (rebuild_records(s)) return -1;
+        rebuild = 1;
+    }
+    /* IactContainer::BackupImpactStatus (IactContainer.cpp:128-131) */
+    for (int k = 0; k < s->n_ia; ++k) {
+        s->ia[k].st.last_contact = s->ia[k].st.contact;
+        s->ia[k].st.last_bond = s->ia[k].st.bond;
+    }
+    if (s->t_current >= s->t_stop && !rebuild) {
+        if (rebuild_records(s)) return -1;
+    }
+    /* 8. Initiate on reset */
+    if (reset && initiate(s)) return -1;
+    return s->t_current < s->t_stop;
+}
+
+int vo_step(vo_sim *s) { return step_impl(s, 0, NULL, NULL, NULL, NULL, NULL, NULL); }
+
+int vo_step_assigned(vo_sim *s, int n, const unsigned *ids, const double *pos, const double *vel,
+                     const double *ox, const double *oz, const double *av)
+{
+    return step_impl(s, n, ids, pos, vel, ox, oz, av);
+}
+
+void vo_get_state(const vo_sim *s, double *pos, double *vel, double *ox, double *oz, double *av,
+                  double *impact, double *ang_impact)
+{
+    for (int i = 0; i < s->n_do; ++i) {
+        if (pos) st3(pos + 3 * i, s->pos[i]);
+        if (vel) st3(vel + 3 * i, s->vel[i]);
+        if (ox) st3(ox + 3 * i, s->ox[i]);
+        if (oz) st3(oz + 3 * i, s->oz[i]);
+        if (av) st3(av + 3 * i, s->av[i]);
+        if (impact) st3(impact + 3 * i, s->last_imp[i]);
+        if (ang_impact) st3(ang_impact + 3 * i, s->last_aimp[i]);
+    }
+}
+
+double vo_time(const vo_sim *s) { return s->t_current; }
+
+int vo_num_interactions(const vo_sim *s) { return s->n_ia; }
+void vo_get_interactions(const vo_sim *s, unsigned *slave, unsigned *master)
+{
+    for (int k = 0; k < s->n_ia; ++k) { slave[k] = s->ia[k].slave; master[k] = s->ia[k].master; }
+}
+int vo_num_candidates(const vo_sim *s) { return s->n_cand; }
+void vo_get_candidates(const vo_sim *s, unsigned *lo, unsigned *hi)
+{
+    for (int k = 0; k < s->n_cand; ++k) { lo[k] = s->cand_lo[k]; hi[k] = s->cand_hi[k]; }
+}
+
+int vo_num_active(const vo_sim *s)
+{
+    int n = 0;
+    for (int k = 0; k < s->n_ia; ++k) n += s->ia[k].ci.active;
+    return n;
+}
+void vo_get_active(const vo_sim *s, unsigned *lo, unsigned *hi, vo_status *st)
+{
+    int n = 0;
+    for (int k = 0; k < s->n_ia; ++k)
+        if (s->ia[k].ci.active) {
+            lo[n] = s->ia[k].lo; hi[n] = s->ia[k].hi;
+            if (st) st[n] = s->ia[k].st;
+            n++;
+        }
+}

@@ -1,0 +1,34 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
+
+
+def load_golden(name):
+    """Load a golden fixture as (npz, WorldArrays)."""
+    from vedo_amd.world import WorldArrays
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    sc = z["scalars"]
+    wa = WorldArrays(
+        model_shape=z["model_shape"], model_behavior=z["model_behavior"], model_scope=z["model_scope"],
+        model_group=z["model_group"], model_density=z["model_density"],
+        model_density_factor=z["model_density_factor"], model_fext=z["model_fext"],
+        model_attr=z["model_attr"], iact_law=z["iact_law"], iact_param=z["iact_param"],
+        model_of=z["model_of"], pos=z["pos"], vel=z["vel"], ox=z["ox"], oz=z["oz"], av=z["av"],
+        dt=float(sc[0]), t_start=float(sc[1]), t_stop=float(sc[2]), t_current=float(sc[3]), g=z["g"],
+        safety_factor=float(sc[4]), up_iact=int(z["cadence"][0]), record=int(z["cadence"][1]))
+    return z, wa
+
+
+GOLDEN_CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_nbs", "drum_kin"]

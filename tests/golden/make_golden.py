@@ -1,0 +1,139 @@
+"""Generate golden vectors from the compiled reference (oracle/_ref).
+
+Run in the build container, where /root/reference exists:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+For every case in CASES it writes the scene exchange file, runs
+oracle/_ref/vedo_ref_harness (the reference's own SimMediator/Consultant/
+ImpactSolver code) in a scratch directory, and stores the dumps as
+tests/golden/<case>.npz: inputs (the scene arrays) and expected outputs
+(per-DO state and impulses, interaction list, contact depths, ImpactStatus of
+every interaction) at the listed steps.  The fixtures are data only.
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from vedo_amd import scenes  # noqa: E402
+from vedo_amd.world import write_vscene  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "vedo_ref_harness")
+
+STATUS_FIELDS = ("kn", "init_vel", "shear", "point", "dir", "to_master", "ang_to_master", "overlap")
+
+
+def parse_dumps(path):
+    out = []
+    with open(path, "rb") as f:
+        buf = f.read()
+    o = 0
+    while o < len(buf):
+        assert buf[o:o + 4] == b"DUMP", "bad dump magic"
+        o += 4
+        step, = struct.unpack_from("<i", buf, o); o += 4
+        t, = struct.unpack_from("<d", buf, o); o += 8
+        n, = struct.unpack_from("<i", buf, o); o += 4
+        st = np.frombuffer(buf, "<f8", n * 21, o).reshape(n, 7, 3).copy(); o += n * 21 * 8
+        ni, = struct.unpack_from("<i", buf, o); o += 4
+        rec = np.frombuffer(buf, np.dtype([("slave", "<u4"), ("master", "<u4"), ("active", "<u4"),
+                                           ("depth", "<f8")]), ni, o).copy(); o += ni * 20
+        sdt = np.dtype([("flags", "u1", 4), ("kn", "<f8"), ("init_vel", "<f8"), ("shear", "<f8", 3),
+                        ("point", "<f8", 3), ("dir", "<f8", 3), ("to_master", "<f8", 3),
+                        ("ang_to_master", "<f8", 3), ("overlap", "<f8")])
+        ist = np.frombuffer(buf, sdt, ni, o).copy(); o += ni * sdt.itemsize
+        out.append(dict(step=step, t=t, state=st, iact=rec, status=ist))
+    return out
+
+
+def _wall_box_ids(wa):
+    return np.nonzero(wa.model_shape[wa.model_of] != 1)[0]
+
+
+def make_cases():
+    """(name, WorldArrays, consultant, nsteps, dumps, assigned-statuses or None)."""
+    cases = []
+    common = dict(spacing=2.02, jitter=0.05, vmax=0.2, offset=3.0)
+    for law, name, extra in (("ISwBSDBF", "box_bsdbf", dict(kn=2e3, e=0.6, mu=0.4)),
+                             ("ISwH", "box_h", dict(E=1e6, nu=0.3, mu=0.5, e=0.5)),
+                             ("ISwHertz", "box_hertz", dict(E=1e6, nu=0.3)),
+                             ("ISwBtSDBF", "box_btsdbf", dict(kn=2e3, e=0.6, mu=0.4)),
+                             ("ISwBSDB2F", "box_b2f", dict(kn=2e3, e=0.6, mu=0.4))):
+        w = scenes.box_scene(64, seed=11, law=law, dt=2e-5, steps=400, omega_max=5.0, **common, **extra)
+        wa = w.to_arrays(up_iact=7, record=1000)
+        cases.append((name, wa, 0, 150, [0, 1, 2, 6, 7, 8, 50, 150], None))
+    # NBS consultant on the same kind of scene
+    w = scenes.box_scene(64, seed=12, law="ISwBSDBF", dt=2e-5, steps=400, kn=2e3, e=0.6, mu=0.4, **common)
+    cases.append(("box_nbs", w.to_arrays(up_iact=5, record=1000), 1, 100, [0, 1, 5, 6, 100], None))
+    # polydisperse drum section with kinematic walls (prescribed rigid rotation)
+    w, wall_ids, c, a = scenes.drum_scene(300, seed=13, R=0.02, L=0.012, omega=20.0, dt=1e-5, steps=500,
+                                          sides=16, rmin=0.6e-3, rmax=1.4e-3, classes=5, kn=2e3)
+    wa = w.to_arrays(up_iact=9, record=1000)
+    nsteps = 120
+    ids = np.array(wall_ids, np.uint32)
+    p0, v0, ox0, oz0 = wa.pos[ids], wa.vel[ids], wa.ox[ids], wa.oz[ids]
+    assigned = []
+    for k in range(1, nsteps + 1):
+        p, v, ox, oz, av = scenes.rigid_rotation_status(p0, v0, ox0, oz0, c, a, 20.0, k * wa.dt)
+        assigned.append(np.concatenate([p, v, ox, oz, av], axis=1))
+    cases.append(("drum_kin", wa, 0, nsteps, [0, 1, 2, 9, 10, 60, 120], (ids, np.stack(assigned))))
+    return cases
+
+
+def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE):
+    with tempfile.TemporaryDirectory() as td:
+        scene = os.path.join(td, "scene.txt")
+        write_vscene(wa, scene, consultant)
+        args = [HARNESS, scene, os.path.join(td, "out.bin"), str(nsteps), ",".join(map(str, dumps))]
+        if assigned is not None:
+            ids, st = assigned
+            with open(os.path.join(td, "assigned.bin"), "wb") as f:
+                f.write(struct.pack("<i", len(ids)))
+                f.write(np.asarray(ids, "<u4").tobytes())
+                f.write(np.ascontiguousarray(st, "<f8").tobytes())
+            args += ["--assigned", os.path.join(td, "assigned.bin")]
+        subprocess.run(args, cwd=td, check=True, stdout=subprocess.DEVNULL)
+        d = parse_dumps(os.path.join(td, "out.bin"))
+    arrays = dict(
+        consultant=np.int32(consultant), nsteps=np.int32(nsteps), dumps=np.array(dumps, np.int32),
+        model_shape=wa.model_shape, model_behavior=wa.model_behavior, model_scope=wa.model_scope,
+        model_group=wa.model_group, model_density=wa.model_density,
+        model_density_factor=wa.model_density_factor, model_fext=wa.model_fext, model_attr=wa.model_attr,
+        iact_law=wa.iact_law, iact_param=wa.iact_param, model_of=wa.model_of, pos=wa.pos, vel=wa.vel,
+        ox=wa.ox, oz=wa.oz, av=wa.av,
+        scalars=np.array([wa.dt, wa.t_start, wa.t_stop, wa.t_current, wa.safety_factor]),
+        g=wa.g, cadence=np.array([wa.up_iact, wa.record], np.int64))
+    if assigned is not None:
+        arrays["assigned_ids"] = np.asarray(assigned[0], np.uint32)
+        arrays["assigned"] = assigned[1]
+    for k, rec in enumerate(d):
+        arrays[f"d{k}_step"] = np.int32(rec["step"])
+        arrays[f"d{k}_t"] = np.float64(rec["t"])
+        arrays[f"d{k}_state"] = rec["state"]
+        arrays[f"d{k}_iact"] = rec["iact"]
+        arrays[f"d{k}_status"] = rec["status"]
+    np.savez_compressed(os.path.join(outdir, f"{name}.npz"), **arrays)
+    return d
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        raise SystemExit("build the reference harness first: make -C oracle ref")
+    for name, wa, cons, nsteps, dumps, assigned in make_cases():
+        d = run_case(name, wa, cons, nsteps, dumps, assigned)
+        nact = [int(r["iact"]["active"].sum()) for r in d]
+        print(f"{name}: {wa.n_do} DOs, {len(d)} dumps, interactions {[len(r['iact']) for r in d]}, active {nact}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,76 @@
+"""The oracle against golden vectors produced by the compiled reference.
+
+Fixtures come from tests/golden/make_golden.py, which runs the reference's
+own SimMediator/LeapConsultant/NBSConsultant/ISw*/CD*/DO* code.  The oracle
+must reproduce every FP64 bit: positions, velocities, orientations, angular
+velocities, last impulses, interaction (slave, master) lists in reference
+order, contact depths and ImpactStatus records.
+
+One documented exception: where a sphere's rotation vector is exactly zero
+the reference's Vector3d::direction() returns an uninitialised vector
+(Vector3d.h:117-126) and its orientation becomes NaN; the oracle defines the
+direction of a zero vector as zero and keeps the orientation.  Those entries
+(NaN in the fixture) are excluded; every other entry is compared bitwise.
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES, load_golden
+from oracle.oracle import OracleSim
+
+FIELDS = ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")
+
+
+def _run(name):
+    z, wa = load_golden(name)
+    sim = OracleSim(wa, int(z["consultant"]))
+    step = 0
+    for k, d in enumerate(z["dumps"]):
+        while step < d:
+            if "assigned" in z.files:
+                a = z["assigned"][step]
+                sim.step_assigned(z["assigned_ids"], a[:, 0:3], a[:, 3:6], a[:, 6:9], a[:, 9:12], a[:, 12:15])
+            else:
+                sim.step()
+            step += 1
+        yield k, int(d), z, sim
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_oracle_bit_exact_vs_reference(name):
+    checked = 0
+    for k, d, z, sim in _run(name):
+        st = sim.state()
+        mine = np.stack([st[f] for f in FIELDS], 1)
+        ref = z[f"d{k}_state"]
+        keep = ~np.isnan(ref)
+        # NaN only ever appears in orientations (fields 2, 3), never elsewhere
+        assert keep[:, [0, 1, 4, 5, 6]].all()
+        assert np.array_equal(mine[keep].view(np.uint64), ref[keep].view(np.uint64)), f"step {d}"
+        assert sim.time == float(z[f"d{k}_t"])
+        s, m = sim.interactions()
+        ri = z[f"d{k}_iact"]
+        np.testing.assert_array_equal(s, ri["slave"])
+        np.testing.assert_array_equal(m, ri["master"])
+        # active set and histories of the active contacts
+        lo, hi, stt = sim.active()
+        act = ri["active"].astype(bool)
+        rlo = np.minimum(ri["slave"], ri["master"])[act]
+        rhi = np.maximum(ri["slave"], ri["master"])[act]
+        np.testing.assert_array_equal(lo, rlo)
+        np.testing.assert_array_equal(hi, rhi)
+        rs = z[f"d{k}_status"][act]
+        for f in ("kn", "init_vel", "shear", "point", "dir", "to_master", "ang_to_master", "overlap"):
+            assert np.array_equal(np.asarray(stt[f]).view(np.uint64), np.asarray(rs[f]).view(np.uint64)), f
+        np.testing.assert_array_equal(stt["contact"], rs["flags"][:, 0])
+        np.testing.assert_array_equal(stt["last_contact"], rs["flags"][:, 1])
+        checked += 1
+    assert checked == len(load_golden(name)[0]["dumps"])
+
+
+def test_golden_cases_have_contacts():
+    """Each fixture exercises the force path (active contacts exist)."""
+    for name in GOLDEN_CASES:
+        z, _ = load_golden(name)
+        total = sum(int(z[f"d{k}_iact"]["active"].sum()) for k in range(len(z["dumps"])))
+        assert total > 0, name

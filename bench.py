@@ -1,0 +1,152 @@
+"""Benchmark of the DEM step on one MI355X.
+
+Workload (N=1 GPU): polydisperse spheres (log-normal radii, median 1 mm,
+sigma_ln 0.2, clipped to [0.5, 2] mm, 64 radius classes, rho 2500) in a drum
+of 64 quasi-plates + 2 end plates (R = 0.15 m) rotating at 2 rad/s about a
+horizontal axis; ISwBSDBF linear spring-dashpot kn = 1e4 N/m, ks = 0.8 kn,
+e = 0.6, mu = 0.4; dt = 5e-6 s; LeapConsultant candidate lists rebuilt every
+UpIact steps; FP64 throughout.  A "step" is one SimMediator::Run over all
+spheres.  The scene first settles for --prep untimed steps so that the timed
+steps run on a contact-rich bed.
+
+Prints one JSON line (see the metric/roofline/cpu_baseline fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s @1M spheres vs plain GPU; p50/p99 scene latency batched"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per step (fixed here, used by the roofline):
+#   280 B per sphere  = FP64 x,v,w read+write (144) + FP64 Ox,Oz read+write (96)
+#                       + radius/model read (8) + geometry mirror write+read (32)
+#   72 B per contact  = (lo,hi) key (8) + FP64 tangential history read+write (48)
+#                       + per-contact scalar state read+write (16)
+BYTES_PER_SPHERE = 280
+BYTES_PER_CONTACT = 72
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(state, wa, n_gpu, seconds):
+    """Oracle (FP64 C restatement of the reference step, single thread) on a
+    bounded sample of the same workload: the spheres of a slab of the settled
+    drum along its axis, with the drum walls, stepped for about `seconds`."""
+    from oracle.oracle import OracleSim
+    from vedo_amd.world import WorldArrays
+    n = n_gpu
+    pos = state["pos"]
+    y = pos[:n, 1]
+    target = 20000
+    y_cut = np.sort(y)[min(target, n) - 1]
+    keep = np.nonzero(y[:n] <= y_cut)[0]
+    walls = np.arange(n, wa.n_do)
+    ids = np.concatenate([keep, walls])
+    sub = WorldArrays(**{k: getattr(wa, k) for k in wa.__dataclass_fields__})
+    sub.model_of = wa.model_of[ids]
+    for f in ("pos", "vel", "ox", "oz", "av"):
+        setattr(sub, f, np.ascontiguousarray(state[f][ids]))
+    sim = OracleSim(sub)
+    # walls are prescribed like on the GPU: rigid rotation about the axis
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < seconds:
+        sim.step(1)
+        steps += 1
+    el = time.perf_counter() - t0
+    return {"value": len(keep) * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (FP64 C restatement, 1 thread) on {len(keep)} spheres of a slab of the same "
+                      f"settled drum (y <= {y_cut:.4f} m) with its {len(walls)} walls held fixed, {steps} steps "
+                      f"in {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--prep", type=int, default=1500)
+    ap.add_argument("--up-iact", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one GPU holds the whole domain); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from vedo_amd import scenes
+    from vedo_amd.gpu import GpuSimulation
+
+    t = time.perf_counter()
+    wa, wall_ids, c, a = scenes.drum_scene(args.n, arrays=True, up_iact=args.up_iact)
+    log(f"[bench] scene: {args.n} spheres + {len(wall_ids)} walls, drum length {wa.pos[:, 1].max():.3f} m "
+        f"({time.perf_counter() - t:.1f} s)")
+    t = time.perf_counter()
+    sim = GpuSimulation(wa)
+    sim.set_wall_rotation(wall_ids, c, a, 2.0, 0.0)
+    log(f"[bench] created on GPU ({time.perf_counter() - t:.1f} s)")
+    t = time.perf_counter()
+    done = 0
+    while done < args.prep:
+        k = min(500, args.prep - done)
+        sim.step(k)
+        done += k
+        log(f"[bench] prep {done}/{args.prep} steps, active contacts {sim.count_active()}")
+    log(f"[bench] prep {time.perf_counter() - t:.1f} s")
+    sim.step(args.warmup)
+    c0 = sim.count_active()
+    # timed region: K steps bracketed by device synchronisation
+    sim.kernel_timing(args.steps)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(args.steps, sync=False)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    k_ms, k_launches = sim.timing()
+    c1 = sim.count_active()
+    stats = sim.stats()
+    n = args.n
+    c_act = 0.5 * (c0 + c1)
+    bytes_step = BYTES_PER_SPHERE * n + BYTES_PER_CONTACT * c_act
+    k_avg_s = k_ms / 1e3 / max(k_launches, 1)
+    achieved = bytes_step / k_avg_s / 1e9
+    value = n * args.steps / wall
+    log(f"[bench] timed {args.steps} steps in {wall:.3f} s; step kernel avg {k_avg_s * 1e6:.1f} us over "
+        f"{k_launches} launches; contacts {c0}->{c1}; {stats}")
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "rotating-drum polydisperse spheres, ISwBSDBF, LeapConsultant", "spheres": n,
+                   "walls": len(wall_ids), "radius_classes": 64, "dt": wa.dt, "up_iact": args.up_iact,
+                   "prep_steps": args.prep, "active_contacts": c_act, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step", "kernel_avg_us": k_avg_s * 1e6,
+                     "algorithmic_bytes_per_launch": bytes_step},
+        "rebuilds": stats["rebuilds"], "relists": stats["relists"],
+    }
+    if not args.no_cpu:
+        st = sim.state()
+        out["cpu_baseline"] = cpu_baseline(st, wa, n, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    sim.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

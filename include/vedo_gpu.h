@@ -1,0 +1,166 @@
+/*
+ * vedo_gpu.h -- C-ABI of the MI355X DEM step library (libvedo_hip.so).
+ *
+ * The library replaces the inner work of VEDO's SimMediator::Run()
+ * (SimMediator.cpp:714-842) for scenes of spheres against quasi-plates and
+ * quasi-cylinders: the Consultant's candidate list and rebuild cadence
+ * (LeapConsultant.cpp:108-195, NBSConsultant.cpp:85-316), the contact
+ * detectors (CDSphere2Sphere.cpp:14-60, CDSphere2QuasiPlate.cpp:15-97,
+ * CDSphere2QuasiCylinder.cpp:16-80), the five impact solvers (ISwBSDBF.cpp,
+ * ISwBSDB2F.cpp, ISwBtSDBF.cpp, ISwH.cpp, ISwHertz.cpp), the per-contact
+ * ImpactStatus history and its IactRecordTab remap (NBSConsultant.cpp:376-395,
+ * SimMediator.cpp:640-712), and DOSphere::Response (DOSphere.cpp:21-48).
+ *
+ * Conventions:
+ *  - every call returns 0 on success and a negative VG_E* code on error; the
+ *    message is available from vg_last_error(h).  Reference exit() paths
+ *    (missing IactModel mechanism, unknown shape pairing) become errors.
+ *  - host buffers are caller-owned; device state is owned by the handle.
+ *  - a handle owns one HIP stream and is not thread-safe.
+ *  - DO ids are the reference's DOWorld indices (0..n_do-1).
+ *  - enumerations use the reference's C++11 values (DOModel.h:26-28).
+ */
+#ifndef VEDO_GPU_H
+#define VEDO_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VG_OK 0
+#define VG_EINVAL (-1)
+#define VG_ENOMEM (-2)
+#define VG_EHIP (-3)
+#define VG_EUNSUPPORTED (-4)
+#define VG_ECAPACITY (-5)
+
+/* DOShapeType, DOBehaviorType, DOScopeType (DOModel.h:26-28) */
+#define VG_SPHERE 1
+#define VG_QUASICYLINDER 6
+#define VG_QUASIPLATE 7
+#define VG_CONSTRAINED 1
+#define VG_FIXED 2
+#define VG_MOBILE 3
+#define VG_ORBITAL 4
+#define VG_GLOBAL 1
+#define VG_LOCAL 2
+
+/* Impact solvers (ModuleList.h:52-56).  Mechanism slots per law follow the
+ * order in which the reference constructors read them:
+ *  ISwBSDBF/ISwBSDB2F/ISwBtSDBF (ISwBSDBF.cpp:10-18): NormalStiffness,
+ *    ShearStiffness, NormalDampingRatio, ShearDampingRatio,
+ *    NormalBondStrength, ShearBondStrength, FrictionCoefficient
+ *  ISwH (ISwH.cpp:10-21): HertzNormalStiffness, EquivalentMass,
+ *    ShearStiffnessCoefficient1, ShearStiffnessCoefficient2, DampingRatio,
+ *    KineticFrictionCoefficient
+ *  ISwHertz (ISwHertz.cpp:11-12): NormalStiffness, NormalDampingCoefficient */
+#define VG_LAW_NONE 0
+#define VG_BSDBF 1
+#define VG_BSDB2F 2
+#define VG_BTSDBF 3
+#define VG_H 4
+#define VG_HERTZ 5
+#define VG_NPARAM 8
+
+/* DOModel (DOModel.h:132-137) after DOModel::Analysis (DOModel.cpp:17-149). */
+typedef struct {
+    int shape, behavior, scope, group;
+    double density, density_factor;
+    double fext[3];
+    double attr[3]; /* Sphere: radius | QuasiCylinder: radius, height | QuasiPlate: width, height, length */
+} vg_model;
+
+/* One entry of the dense group x group IactModel table (IactModel.cpp:16-30,
+ * DOWorld.cpp:297-302: symmetric, first match). */
+typedef struct {
+    int law;
+    double p[VG_NPARAM];
+} vg_iact;
+
+/* A DOWorld in structure-of-arrays form (replaces the DOWorld/SystemParameter
+ * the Consultant reads, DOWorld.h:32, SystemParameter.h:17-25). */
+typedef struct {
+    int n_do, n_models, n_groups;
+    const vg_model *models;
+    const vg_iact *iact;       /* n_groups*n_groups */
+    const int *model_of;       /* n_do */
+    const double *pos, *vel, *ox, *oz, *av;   /* 3*n_do each */
+    double dt, t_start, t_stop, t_current;
+    double g[3];
+    double safety_factor;      /* aries::Constants SafetyFactor (Constants.h:184) */
+    unsigned up_iact;          /* candidate-list rebuild period (anne's UpIact) */
+    unsigned record;           /* record period (anne's Record) */
+    int consultant;            /* 0: LeapConsultant rules (the only one on the GPU) */
+    double inner_skin;         /* <= 0: automatic */
+} vg_scene_desc;
+
+/* ImpactStatus payload of one contact (ImpactStatus.h:208-222). */
+typedef struct {
+    unsigned char contact, last_contact, bond, last_bond;
+    double kn, init_vel;
+    double shear[3];
+    double point[3], dir[3], to_master[3], ang_to_master[3];
+    double overlap;
+} vg_status;
+
+typedef struct vg_scene vg_scene;
+
+/* Create a scene on the current HIP device: builds the candidate list
+ * (Consultant::Reset) and the interactions (SimMediator::Initiate).
+ * Replaces `new LeapConsultant(...)` + `SimMediator sm(...)` (Anne.cpp:90-118). */
+int vg_create(const vg_scene_desc *desc, vg_scene **out);
+int vg_destroy(vg_scene *h);
+const char *vg_last_error(const vg_scene *h);
+/* Error text of the last failed vg_create (no handle exists then). */
+const char *vg_create_error(void);
+
+/* Restore histories before the first step (IactRecordTab restore,
+ * SimMediator.cpp:690-707). lo < hi are DO ids. */
+int vg_set_history(vg_scene *h, int n, const unsigned *lo, const unsigned *hi, const vg_status *st);
+
+/* Enqueue nsteps of SimMediator::Run() (SimMediator.cpp:714-842) on the
+ * handle's stream.  Asynchronous; vg_sync waits.  Rebuilds every up_iact
+ * steps as the Consultant's NextStep does (LeapConsultant.cpp:49-97). */
+int vg_step(vg_scene *h, int nsteps);
+int vg_sync(vg_scene *h);
+
+/* One step with prescribed statuses for kinematic DOs (walls), the
+ * equivalent of SimMediator::Run(const std::map<_uint_t,DOStatus>&)
+ * (SimMediator.cpp:1121-1163).  Arrays hold 3 doubles per listed DO. */
+int vg_step_assigned(vg_scene *h, int n, const unsigned *ids, const double *pos, const double *vel,
+                     const double *ox, const double *oz, const double *av);
+
+/* Prescribe a rigid rotation of walls about an axis, evaluated on the device
+ * every step (a kinematic drum), instead of host-assigned statuses.  The
+ * listed walls' current statuses are the t = t_ref configuration. */
+int vg_set_wall_rotation(vg_scene *h, int n, const unsigned *ids, const double *axis_point,
+                         const double *axis_dir, double omega, double t_ref);
+
+/* DOStatus of every DO in DO-id order, as DOWorld::UpdateDOStatus sees it
+ * (DOWorld.cpp:304-318); impact/ang_impact are the impulses of the last
+ * Response.  Any pointer may be NULL. */
+int vg_get_state(vg_scene *h, double *pos, double *vel, double *ox, double *oz, double *av,
+                 double *impact, double *ang_impact);
+double vg_time(const vg_scene *h);
+
+/* Number of interactions in the candidate list (Consultant::GetIactNum). */
+int vg_num_interactions(vg_scene *h, long long *n);
+/* Active contacts (ContactInfo.bActive of the last detect) with their
+ * ImpactStatus, keyed (lo, hi) -- the IactRecordTab payload that
+ * RebuildIactRecordTab saves (NBSConsultant.cpp:376-395).  Call with
+ * st == NULL to get the count in *n; otherwise *n is the capacity on input. */
+int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *hi, vg_status *st);
+
+/* Timing on the handle's stream.  After vg_kernel_timing(h, k) the next k
+ * step-kernel launches are bracketed by device events and vg_step_timing
+ * returns their summed duration and count; otherwise it returns the duration
+ * of the last vg_step call (all its kernels) and its step count. */
+int vg_kernel_timing(vg_scene *h, int max_launches);
+int vg_step_timing(vg_scene *h, double *ms, int *launches);
+/* Counters: rebuilds, relists, max neighbour-list fill. */
+int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, int *max_fill, int *capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,327 @@
+// vg_device.h -- device data layout and per-contact physics of the DEM step.
+//
+// Layout in HBM (one scene, spheres in cell order, re-sorted at every
+// candidate-list rebuild):
+//   pos  double4 [N]  x, y, z, radius
+//   vel  double4 [N]  vx, vy, vz, path length since the last neighbour relist
+//   avl  double4 [N]  wx, wy, wz, sudo-mass m* (DOModel.cpp:22-30)
+//   ori  double  [6][N]  Ox.xyz, Oz.xyz (orientation, SoA)
+//   oid  u32 [N]  sphere rank (order of DO ids);  mdl i32 [N]  DOModel index
+// Neighbour ("inner") list, ELL transposed so that slot k of 64 consecutive
+// spheres is one contiguous 256-byte run:
+//   hdr  u8  [N]          count (7 bits) | current half (bit 7)
+//   ent  u32 [2][KI][N]   partner (26 bits) | status flags (6 bits)
+//   hist f64 [2][KI][4][N] tangential force history xyz + Kn / initial velocity
+// Every sphere keeps its own copy of each pair's ImpactStatus; the two copies
+// are updated by the same instruction sequence from the same inputs, so they
+// stay bit-identical and no thread ever reads another thread's history.
+#pragma once
+
+#include "vg_math.h"
+
+namespace vg {
+
+constexpr unsigned PARTNER_MASK = (1u << 26) - 1;
+constexpr unsigned F_CONTACT = 1u << 26;
+constexpr unsigned F_LASTCONTACT = 1u << 27;
+constexpr unsigned F_BOND = 1u << 28;
+constexpr unsigned F_LASTBOND = 1u << 29;
+constexpr unsigned F_SLAVE = 1u << 30;   // this sphere is the pair's slave (lower DO id)
+constexpr unsigned F_NOBACKUP = 1u << 31;   // status restored at a rebuild: skip the pending Backup
+constexpr unsigned F_STATUS = F_CONTACT | F_LASTCONTACT | F_BOND | F_LASTBOND;
+// ImpactStatus() defaults: bLastContact(true), bLastBond(true) (ImpactStatus.cpp:6-9)
+constexpr unsigned F_DEFAULT = F_LASTCONTACT | F_LASTBOND | F_NOBACKUP;
+
+enum { LAW_NONE = 0, BSDBF = 1, BSDB2F = 2, BTSDBF = 3, LAW_H = 4, LAW_HERTZ = 5 };
+enum { SH_SPHERE = 1, SH_CYL = 6, SH_PLATE = 7 };
+
+struct LawDev {
+    int law;
+    int pad;
+    double p[8];
+};
+
+struct ModelDev {
+    int group;
+    int pad;
+    double fext[3];
+};
+
+struct WallDev {
+    d3 pos, vel, ox, oz, av;
+    double a0, a1, a2;   // plate: w, h, l ; cylinder: r, h
+    double range;        // DOModel::Analysis range (global scope)
+    int shape, group;
+    int behavior, kin;   // kin: 0 none, 1 host-assigned, 2 device rotation
+};
+
+struct Contact {
+    d3 n, p;
+    double depth;
+    bool active;
+};
+
+struct Hist {
+    unsigned f;   // status flags
+    d3 s;         // shear force
+    double aux;   // Kn (ISwH) or initial velocity (ISwHertz)
+};
+
+struct PairOut {
+    d3 imp, ang_a, ang_b;
+    bool added, linear_only;
+};
+
+struct RecOut {   // ImpactStatus fields only the record path stores
+    d3 point, dir, to_master, ang_to_master;
+    double overlap;
+};
+
+// CDSphere_Sphere::CalDistance + Detect (CDSphere2Sphere.cpp:14-60)
+__device__ inline void cd_ss(d3 xa, double ra, d3 xb, double rb, Contact &c)
+{
+    d3 im = xb - xa;
+    c.depth = ra + rb - len(im);
+    c.n = dir(im);
+    c.p = xa + smul(ra - 0.5 * c.depth, c.n);
+    c.active = c.depth > 0;
+}
+
+// CDSphere_QuasiPlate::CalDistance + Detect (CDSphere2QuasiPlate.cpp:15-97)
+__device__ inline void cd_sp(d3 xa, double ra, const WallDev &w, Contact &c)
+{
+    d3 oy = cross(w.oz, w.ox);
+    d3 cap = xa - project_on(xa - w.pos, w.oz);
+    double hh = 0.5 * w.a1, hw = 0.5 * w.a0, hl = 0.5 * w.a2;
+    double dx = dot(cap - w.pos, w.ox);
+    if (dx < -hw) dx = -hw;
+    else if (dx > hw) dx = hw;
+    double dy = dot(cap - w.pos, oy);
+    if (dy < -hl) dy = -hl;
+    else if (dy > hl) dy = hl;
+    d3 caps = (muls(w.ox, dx) + muls(oy, dy)) + w.pos;
+    d3 im = caps - xa;
+    c.depth = ra + hh - len(im);
+    c.n = dir(im);
+    c.p = xa + smul(ra - 0.5 * c.depth, c.n);
+    c.active = c.depth > 0;
+}
+
+// CDSphere_QuasiCylinder::CalDistance + Detect (CDSphere2QuasiCylinder.cpp:16-80)
+__device__ inline void cd_sc(d3 xa, double ra, const WallDev &w, Contact &c)
+{
+    double hh = 0.5 * w.a1;
+    double d = dot(xa - w.pos, w.oz);
+    if (d < -hh) d = -hh;
+    else if (d > hh) d = hh;
+    d3 caps = w.pos + smul(d, w.oz);
+    d3 im = caps - xa;
+    c.depth = ra + w.a0 - len(im);
+    c.n = dir(im);
+    c.p = xa + smul(ra - 0.5 * c.depth, c.n);
+    c.active = c.depth > 0.0;
+}
+
+__device__ inline void cd_wall(d3 xa, double ra, const WallDev &w, Contact &c)
+{
+    if (w.shape == SH_PLATE) cd_sp(xa, ra, w, c);
+    else cd_sc(xa, ra, w, c);
+}
+
+// DOMap::CalDistance for sphere-plate / sphere-cylinder (DOMap.cpp:353-412)
+__device__ inline double domap_wall_distance(d3 ca, const WallDev &w)
+{
+    if (w.shape == SH_PLATE) {
+        d3 oy = cross(w.oz, w.ox);
+        d3 cap = ca - project_on(ca - w.pos, w.oz);
+        double hw = 0.5 * w.a0, hl = 0.5 * w.a2;
+        double dx = dot(cap - w.pos, w.ox);
+        if (dx < -hw) dx = -hw;
+        else if (dx > hw) dx = hw;
+        double dy = dot(cap - w.pos, oy);
+        if (dy < -hl) dy = -hl;
+        else if (dy > hl) dy = hl;
+        d3 caps = (muls(w.ox, dx) + muls(oy, dy)) + w.pos;
+        return len(caps - ca);
+    }
+    double hh = 0.5 * w.a1;
+    double d = dot(ca - w.pos, w.oz);
+    d3 cap = w.pos + muls(w.oz, d);
+    d3 im;
+    if (d < hh && d > -hh) im = cap - ca;
+    else if (d >= hh) im = (w.pos + muls(w.oz, hh)) - ca;
+    else im = (w.pos - muls(w.oz, hh)) - ca;
+    return len(im);
+}
+
+// ---- ImpactStatus transitions (ImpactStatus.cpp:248-274) ----
+__device__ inline void st_clean(Hist &h)
+{
+    h.f &= ~F_STATUS;
+    h.s = mk(0, 0, 0);
+    h.aux = 0.0;
+}
+__device__ inline void st_clean_contact(Hist &h)
+{
+    unsigned f = h.f & ~F_STATUS;
+    if (h.f & F_CONTACT) f |= F_LASTCONTACT;
+    if (h.f & F_BOND) f |= F_LASTBOND;
+    h.f = f;
+    h.s = mk(0, 0, 0);
+}
+
+// ---- ISwModels (ISwModels.cpp) ----
+__device__ inline d3 normal_force(double kn, double cn, double depth, d3 n, d3 vn)
+{
+    return smul(-kn * depth, n) - smul(cn, vn);
+}
+__device__ inline d3 normal_force_bond(double kn, double cn, double depth, d3 n, d3 vn)
+{
+    if (dot(vn, n) < 0.0) return smul(-kn * depth, n) - smul(cn, vn);
+    return smul(-cn, vn);
+}
+__device__ inline d3 shear_rotation(d3 f, d3 d)
+{
+    if (dot(f, d) <= 0.0) return smul(-len(f), dir(d));
+    return smul(len(f), dir(d));
+}
+
+// One ImpactSolver::NextStep for slave A and master B.
+// ISwBSDBF/ISwBSDB2F/ISwBtSDBF (ISwBSDBF.cpp:26-139, ISwBSDB2F.cpp:26-149,
+// ISwBtSDBF.cpp:26-137), ISwH (ISwH.cpp:30-156), ISwHertz (ISwHertz.cpp:20-94).
+template <bool REC>
+__device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va, d3 wa, d3 xb, d3 vb,
+                             d3 wb, const Contact &c, double dt, Hist &h, PairOut &o, RecOut &r)
+{
+    o.added = false;
+    o.linear_only = false;
+    const int law = L.law;
+    const double depth = c.depth;
+    const d3 n = c.n;
+    if (law == LAW_HERTZ) {
+        o.linear_only = true;
+        if (!c.active) {
+            st_clean(h);
+            return;
+        }
+        d3 fn = mk(0, 0, 0);
+        if (depth > 0.0) {
+            d3 ra = c.p - xa, rb = c.p - xb;
+            d3 rv = ((va - vb) + cross(wa, ra)) - cross(wb, rb);
+            d3 vn = smul(dot(rv, n), n);
+            if (!(h.f & F_CONTACT)) {
+                h.f |= F_CONTACT;
+                h.aux = len(vn);
+            }
+            fn = smul(-L.p[0] * pow(depth, 1.5), n) - smul(L.p[1], vn);
+        } else {
+            st_clean_contact(h);
+        }
+        o.imp = muls(fn, dt);
+        o.ang_a = o.ang_b = mk(0, 0, 0);
+        o.added = true;
+        if (REC) {
+            r.point = c.p; r.dir = c.n; r.overlap = c.depth;
+            r.to_master = -o.imp; r.ang_to_master = mk(0, 0, 0);
+        }
+        return;
+    }
+    if (law == LAW_H) {
+        const double knh = L.p[0], meff = L.p[1], ksc1 = L.p[2], ksc2 = L.p[3], betan = L.p[4], fk = L.p[5];
+        if (!c.active) {
+            st_clean(h);
+            return;
+        }
+        d3 ra = mk(0, 0, 0), rb = mk(0, 0, 0), fn = mk(0, 0, 0), fs = mk(0, 0, 0);
+        if (depth > 0.0) {
+            ra = c.p - xa;
+            rb = c.p - xb;
+            d3 rv = ((va - vb) + cross(wa, ra)) - cross(wb, rb);
+            d3 vn = smul(dot(rv, n), n);
+            d3 vs = rv - vn;
+            double kn;
+            if (!(h.f & F_CONTACT)) {
+                if (len(vn) == 0.0) kn = knh;
+                else if (betan < 0.99)
+                    kn = 0.957832746294131 / (1.0 - betan * betan) * pow(meff, 0.2) * pow(knh, 0.8) *
+                         pow(len(vn), 0.4);
+                else
+                    kn = 479.155951122633 * pow(meff, 0.2) * pow(knh, 0.8) * pow(len(vn), 0.4);
+                h.aux = kn;
+                h.f |= F_CONTACT;
+            } else {
+                kn = h.aux;
+            }
+            double ncd = 2.0 * sqrt(meff * kn);
+            double ks = ksc1 * pow(ksc2 * fabs(kn * depth), 1.0 / 3.0);
+            fn = normal_force(kn, ncd * betan, depth, n, vn);
+            double ff = fk * kn * depth;
+            fs = shear_rotation(h.s, vs) - smul(ks * dt, vs);
+            if (len(fs) >= ff) fs = smul(ff, dir(fs));
+            h.s = fs;
+        } else {
+            st_clean_contact(h);
+        }
+        d3 f = fn + fs;
+        o.imp = muls(f, dt);
+        o.ang_a = smul(dt, cross(ra, f));
+        o.ang_b = smul(-dt, cross(rb, f));
+        o.added = true;
+        if (REC) {
+            r.point = c.p; r.dir = c.n; r.overlap = c.depth;
+            r.to_master = smul(-1.0, o.imp); r.ang_to_master = o.ang_b;
+        }
+        return;
+    }
+    // spring-dashpot family
+    const double kn = L.p[0], ks = L.p[1], bn = L.p[4], bs = L.p[5], fc = L.p[6];
+    double me;   // ISwModels::CriticalDamping (ISwModels.cpp:88-108)
+    if (mb == 0) me = ma;
+    else if (ma == 0) me = mb;
+    else me = 0.5 * (ma + mb);
+    const double cn = L.p[2] * (2.0 * sqrt(me * kn));
+    const double cs = L.p[3] * (2.0 * sqrt(me * ks));
+    const bool nbond = (depth <= 0.0) && (kn != 0.0) && (bn != 0.0) && (-depth * kn < bn);
+    if (!c.active && !nbond) {
+        st_clean(h);
+        return;
+    }
+    d3 ra = mk(0, 0, 0), rb = mk(0, 0, 0), fn = mk(0, 0, 0), fs = mk(0, 0, 0);
+    if (depth > 0.0) {
+        ra = c.p - xa;
+        rb = c.p - xb;
+        d3 rv = ((va - vb) + cross(wa, ra)) - cross(wb, rb);
+        d3 vn = smul(dot(rv, n), n);
+        d3 vs = rv - vn;
+        fn = normal_force(kn, cn, depth, n, vn);
+        if (law == BTSDBF) fs = smul(-ks * dt, vs) - smul(cs, vs);
+        else fs = shear_rotation(h.s, vs) - smul(ks * dt, vs);
+        double fmax = fc * kn * depth;
+        if (bs == 0.0) {
+            if (len(fs) > fmax) fs = smul(fmax, dir(fs));
+            else fs = fs - smul(cs, vs);
+        } else {
+            if (len(fs) > bs) fs = smul(fmax, dir(fs));
+            else fs = fs - smul(cs, vs);
+        }
+        if (law != BTSDBF) h.s = fs;
+        h.f |= F_CONTACT;
+    } else if (law != BSDB2F || (h.f & F_LASTBOND)) {
+        d3 rv = va - vb;
+        d3 vn = smul(dot(rv, n), n);
+        if (law == BTSDBF) fn = normal_force(kn, cn, depth, n, vn);
+        else fn = normal_force_bond(kn, cn, depth, n, vn);
+        st_clean_contact(h);
+    }
+    d3 f = fn + fs;
+    o.imp = muls(f, dt);
+    o.ang_a = smul(dt, cross(ra, f));
+    o.ang_b = smul(-dt, cross(rb, f));
+    o.added = true;
+    if (REC) {
+        r.point = c.p; r.dir = c.n; r.overlap = c.depth;
+        r.to_master = smul(-1.0, o.imp); r.ang_to_master = o.ang_b;
+    }
+}
+
+}  // namespace vg

@@ -1,0 +1,611 @@
+// vg_kernels.hip -- gfx950 kernels of the DEM step.
+//
+// Per step one fused kernel (k_step) does, for every sphere: contact detection
+// against its neighbour list, the impact solver of every pair (both sides of a
+// sphere pair evaluate the pair identically), the field/external impulse and
+// DOSphere::Response.  It reads state buffer A and writes buffer B, so no
+// thread sees a half-updated neighbour.  Per-sphere impulse sums follow the
+// neighbour-list order, which the rebuild lays down in the reference's
+// IactPairTab order (LeapConsultant.h:207-444), so the FP64 sums round exactly
+// as the reference's DiscreteObject::AddImpact sequence does.
+//
+// Every up_iact steps the rebuild kernels (k_bounds ... k_inner_build)
+// restate LeapConsultant::Reset: a cell grid of size ZoneRange over the local
+// DOs, candidate pairs within safe_i + safe_j in the 13-cell half shell, self
+// cell and global DOs, and IactRecordTab's history carry for pairs active at
+// the rebuild.
+#include <hip/hip_runtime.h>
+
+#include "vg_kernels.h"
+
+namespace vg {
+
+__constant__ int NB13[13][3] = {{-1, -1, -1}, {-1, -1, 0}, {-1, -1, 1}, {0, -1, -1}, {0, -1, 0},
+                                {0, -1, 1},   {1, -1, -1}, {1, -1, 0},  {1, -1, 1},  {-1, 0, -1},
+                                {-1, 0, 0},   {0, 0, -1},  {1, 0, -1}};
+
+__device__ inline size_t eidx(int h, int k, int ki, int n, int i)
+{
+    return ((size_t)(h * ki + k)) * (size_t)n + (size_t)i;
+}
+__device__ inline size_t hidx(int h, int k, int c, int ki, int n, int i)
+{
+    return (((size_t)(h * ki + k)) * 4 + c) * (size_t)n + (size_t)i;
+}
+
+__device__ inline void load_hist(const double *hist, int h, int k, int ki, int n, int i, Hist &hs)
+{
+    hs.s.x = hist[hidx(h, k, 0, ki, n, i)];
+    hs.s.y = hist[hidx(h, k, 1, ki, n, i)];
+    hs.s.z = hist[hidx(h, k, 2, ki, n, i)];
+    hs.aux = hist[hidx(h, k, 3, ki, n, i)];
+}
+__device__ inline void store_hist(double *hist, int h, int k, int ki, int n, int i, const Hist &hs)
+{
+    hist[hidx(h, k, 0, ki, n, i)] = hs.s.x;
+    hist[hidx(h, k, 1, ki, n, i)] = hs.s.y;
+    hist[hidx(h, k, 2, ki, n, i)] = hs.s.z;
+    hist[hidx(h, k, 3, ki, n, i)] = hs.aux;
+}
+
+// ---------------------------------------------------------------------------
+// Step
+// ---------------------------------------------------------------------------
+
+// Rebuild sphere i's neighbour list from its candidate list: keep partners
+// within `skin` of contact, carry flags and history of partners already
+// listed (both lists are ordered subsequences of the candidate list).
+__device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_old, int c_old, int h_new)
+{
+    const unsigned b = a.o_off[i], e = a.o_off[i + 1];
+    int p = 0, q = 0;
+    unsigned old_partner = c_old > 0 ? (a.ell.ent[eidx(h_old, 0, a.ki, a.n, i)] & PARTNER_MASK) : 0xffffffffu;
+    for (unsigned t = b; t < e; ++t) {
+        const unsigned j = a.o_nbr[t];
+        Contact c;
+        if (j < (unsigned)a.n) {
+            const double4 pj = a.in.pos[j];
+            cd_ss(xi, ri, ld3(pj), pj.w, c);
+        } else {
+            cd_wall(xi, ri, a.walls[j - a.n], c);
+        }
+        const bool carried = (p < c_old) && (old_partner == j);
+        unsigned f = 0;
+        Hist hs;
+        if (carried) {
+            f = a.ell.ent[eidx(h_old, p, a.ki, a.n, i)] & ~PARTNER_MASK;
+            load_hist(a.ell.hist, h_old, p, a.ki, a.n, i, hs);
+            ++p;
+            old_partner = p < c_old ? (a.ell.ent[eidx(h_old, p, a.ki, a.n, i)] & PARTNER_MASK) : 0xffffffffu;
+        }
+        if (c.depth >= -a.skin) {
+            if (q >= a.ki) {
+                atomicOr(&a.flags[3], ERR_INNER_CAP);
+                break;
+            }
+            if (!carried) {
+                // A candidate outside the list has been solved as inactive
+                // every step: its status is Clean() (ImpactStatus.cpp:259-274).
+                f = 0;
+                if (j >= (unsigned)a.n || a.in.oid[i] < a.in.oid[j]) f |= F_SLAVE;
+                hs.s = mk(0, 0, 0);
+                hs.aux = 0.0;
+            }
+            a.ell.ent[eidx(h_new, q, a.ki, a.n, i)] = j | f;
+            store_hist(a.ell.hist, h_new, q, a.ki, a.n, i, hs);
+            ++q;
+        }
+    }
+    return q;
+}
+
+template <bool REC>
+__global__ __launch_bounds__(256) void k_step(StepArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        a.flags[(a.step + 2) % 3] = 0;
+        if (a.flags[a.step % 3]) atomicAdd(a.relists, 1ull);
+    }
+    if (i >= a.n) return;
+    const bool do_relist = a.flags[a.step % 3] != 0;
+
+    const double4 p4 = a.in.pos[i];
+    double4 v4 = a.in.vel[i];
+    const double4 w4 = a.in.avl[i];
+    const d3 xi = ld3(p4), vi = ld3(v4), wi = ld3(w4);
+    const double ri = p4.w, mi = w4.w;
+    const int gi = a.t.models[a.in.mdl[i]].group;
+
+    unsigned hdr = a.ell.hdr[i];
+    int h = hdr >> 7, cnt = hdr & 127;
+    if (do_relist) {
+        const int hn = h ^ 1;
+        cnt = relist(a, i, xi, ri, h, cnt, hn);
+        h = hn;
+        a.ell.hdr[i] = (unsigned char)((h << 7) | cnt);
+        v4.w = 0.0;
+    }
+
+    d3 I = mk(0, 0, 0), L = mk(0, 0, 0);
+    for (int k = 0; k < cnt; ++k) {
+        const size_t ei = eidx(h, k, a.ki, a.n, i);
+        const unsigned ent = a.ell.ent[ei];
+        const unsigned j = ent & PARTNER_MASK;
+        Hist hs;
+        hs.f = ent & ~PARTNER_MASK;
+        load_hist(a.ell.hist, h, k, a.ki, a.n, i, hs);
+        // IactContainer::BackupImpactStatus of the previous step (IactContainer.cpp:128-131)
+        if (!(hs.f & F_NOBACKUP)) {
+            unsigned f = hs.f & ~(F_LASTCONTACT | F_LASTBOND);
+            if (hs.f & F_CONTACT) f |= F_LASTCONTACT;
+            if (hs.f & F_BOND) f |= F_LASTBOND;
+            hs.f = f;
+        }
+        hs.f &= ~F_NOBACKUP;
+        PairOut o;
+        RecOut r;
+        if (REC) {
+            r.point = r.dir = r.to_master = r.ang_to_master = mk(0, 0, 0);
+            r.overlap = 0.0;
+        }
+        Contact c;
+        if (j < (unsigned)a.n) {
+            const double4 pj = a.in.pos[j], vj = a.in.vel[j], wj = a.in.avl[j];
+            const int gj = a.t.models[a.in.mdl[j]].group;
+            const LawDev &law = a.t.laws[gi * a.t.n_groups + gj];
+            if (hs.f & F_SLAVE) {
+                cd_ss(xi, ri, ld3(pj), pj.w, c);
+                solve<REC>(law, mi, wj.w, xi, vi, wi, ld3(pj), ld3(vj), ld3(wj), c, a.dt, hs, o, r);
+                if (o.added) {
+                    I = I + o.imp;
+                    if (!o.linear_only) L = L + o.ang_a;
+                }
+            } else {
+                cd_ss(ld3(pj), pj.w, xi, ri, c);
+                solve<REC>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), xi, vi, wi, c, a.dt, hs, o, r);
+                if (o.added) {
+                    I = I + (o.linear_only ? -o.imp : smul(-1.0, o.imp));
+                    if (!o.linear_only) L = L + o.ang_b;
+                }
+            }
+        } else {
+            const WallDev &w = a.walls[j - a.n];
+            const LawDev &law = a.t.laws[gi * a.t.n_groups + w.group];
+            cd_wall(xi, ri, w, c);
+            solve<REC>(law, mi, 0.0, xi, vi, wi, w.pos, w.vel, w.av, c, a.dt, hs, o, r);
+            if (o.added) {
+                I = I + o.imp;
+                if (!o.linear_only) L = L + o.ang_a;
+            }
+        }
+        a.ell.ent[ei] = j | hs.f;
+        store_hist(a.ell.hist, h, k, a.ki, a.n, i, hs);
+        if (REC) {
+            double *rp = a.rec + ((size_t)k * 13) * a.n + i;
+            const double vals[13] = {r.point.x, r.point.y, r.point.z, r.dir.x, r.dir.y, r.dir.z,
+                                     r.to_master.x, r.to_master.y, r.to_master.z, r.ang_to_master.x,
+                                     r.ang_to_master.y, r.ang_to_master.z, r.overlap};
+            for (int q = 0; q < 13; ++q) rp[(size_t)q * a.n] = vals[q];
+        }
+    }
+    // DOContainer::AddFieldImpact + AddConstrainedImpact (DOContainer.cpp:20-50)
+    const d3 gdt = muls(mk(a.gx, a.gy, a.gz), a.dt);
+    I = I + muls(gdt, mi);
+    const ModelDev &md = a.t.models[a.in.mdl[i]];
+    I = I + smul(a.dt, mk(md.fext[0], md.fext[1], md.fext[2]));
+    // DOSphere::Response (DOSphere.cpp:21-48)
+    const d3 dv = smul(1.0 / mi, I);
+    const d3 dav = smul(2.5 / mi / ri / ri, L);
+    const d3 dp = smul(a.dt, vi + smul(0.5, dv));
+    const d3 dw = smul(a.dt, wi + smul(0.5, dav));
+    const d3 xn = xi + dp;
+    const d3 ox = mk(a.in.ori[i], a.in.ori[a.n + i], a.in.ori[2 * a.n + i]);
+    const d3 oz = mk(a.in.ori[3 * a.n + i], a.in.ori[4 * a.n + i], a.in.ori[5 * a.n + i]);
+    const d3 oxn = rotate_around(ox, dw), ozn = rotate_around(oz, dw);
+    const d3 vn = vi + dv, wn = wi + dav;
+    const double path = v4.w + len(dp);
+    if (path > a.relist_half) atomicOr(&a.flags[(a.step + 1) % 3], 1);
+    a.out.pos[i] = make_double4(xn.x, xn.y, xn.z, ri);
+    a.out.vel[i] = make_double4(vn.x, vn.y, vn.z, path);
+    a.out.avl[i] = make_double4(wn.x, wn.y, wn.z, mi);
+    a.out.ori[i] = oxn.x; a.out.ori[a.n + i] = oxn.y; a.out.ori[2 * a.n + i] = oxn.z;
+    a.out.ori[3 * a.n + i] = ozn.x; a.out.ori[4 * a.n + i] = ozn.y; a.out.ori[5 * a.n + i] = ozn.z;
+    a.out.oid[i] = a.in.oid[i];
+    a.out.mdl[i] = a.in.mdl[i];
+    if (REC) {
+        a.rec_imp[i] = I.x; a.rec_imp[a.n + i] = I.y; a.rec_imp[2 * a.n + i] = I.z;
+        a.rec_imp[3 * a.n + i] = L.x; a.rec_imp[4 * a.n + i] = L.y; a.rec_imp[5 * a.n + i] = L.z;
+    }
+}
+
+template __global__ void k_step<false>(StepArgs);
+template __global__ void k_step<true>(StepArgs);
+
+// Wall responses after the spheres' (DOFixedQuasiPlate.h:16-19,
+// DOOrbitalQuasiPlate.cpp:18-34) and kinematic prescriptions
+// (SimMediator.cpp:1121-1163).  kin 1: host-assigned status in `assigned`;
+// kin 2: rigid rotation about (c, a) at omega from the reference status.
+__global__ void k_walls(WallDev *walls, int nw, const double *assigned, const WallDev *ref, const WallKin *kin,
+                        double dt, double t_new)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    WallDev W = walls[w];
+    if (W.kin == 1) {
+        const double *s = assigned + 15 * w;
+        W.pos = mk(s[0], s[1], s[2]); W.vel = mk(s[3], s[4], s[5]);
+        W.ox = mk(s[6], s[7], s[8]); W.oz = mk(s[9], s[10], s[11]); W.av = mk(s[12], s[13], s[14]);
+    } else if (W.kin == 2) {
+        const WallKin &K = kin[w];
+        const WallDev &R = ref[w];
+        const double th = K.omega * (t_new - K.t_ref);
+        const double ct = cos(th), st = sin(th);
+        auto rot = [&](d3 v) { return (smul(ct, v) + smul(st, cross(K.a, v))) + smul(dot(v, K.a) * (1.0 - ct), K.a); };
+        W.pos = K.c + rot(R.pos - K.c);
+        const d3 om = smul(K.omega, K.a);
+        W.vel = cross(om, W.pos - K.c);
+        W.ox = rot(R.ox);
+        W.oz = rot(R.oz);
+        W.av = om;
+    } else if (W.behavior == 4) {   // orbital
+        const d3 dp = muls(W.vel, dt), dw = muls(W.av, dt);
+        W.pos = W.pos + dp;
+        W.ox = rotate_around(W.ox, dw);
+        W.oz = rotate_around(W.oz, dw);
+    }
+    walls[w] = W;
+}
+
+// ---------------------------------------------------------------------------
+// Rebuild
+// ---------------------------------------------------------------------------
+
+__device__ inline unsigned long long ord64(double d)
+{
+    unsigned long long u = __double_as_longlong(d);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// Bounds of the local DOs (LeapConsultant.cpp:119-164): min/max position,
+// max |v|, max range; and the oid -> current-index map.
+__global__ void k_bounds(Geo g, int n, unsigned long long *red, unsigned *inv, double *xs, double *ys, double *zs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ unsigned long long sm[8][256];
+    unsigned long long v[8];
+    if (i < n) {
+        const double4 p = g.pos[i], u = g.vel[i];
+        v[0] = ord64(p.x); v[1] = ord64(p.y); v[2] = ord64(p.z);
+        v[3] = ~ord64(p.x); v[4] = ~ord64(p.y); v[5] = ~ord64(p.z);
+        v[6] = ord64(len(mk(u.x, u.y, u.z)));
+        v[7] = ord64(p.w);
+        inv[g.oid[i]] = i;
+        xs[i] = p.x; ys[i] = p.y; zs[i] = p.z;
+    } else {
+        for (int k = 0; k < 8; ++k) v[k] = 0;
+    }
+    for (int k = 0; k < 8; ++k) sm[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 8; ++k) sm[k][threadIdx.x] = max(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 8; ++k) atomicMax(&red[k], sm[k][0]);
+}
+
+__device__ inline void cell_of(const GridDev &G, d3 p, unsigned &cx, unsigned &cy, unsigned &cz)
+{
+    cx = (unsigned)((p.x - G.xmin) / G.zr);
+    cy = (unsigned)((p.y - G.ymin) / G.zr);
+    cz = (unsigned)((p.z - G.zmin) / G.zr);
+}
+__device__ inline unsigned long long cmp_key(const GridDev &G, unsigned cx, unsigned cy, unsigned cz)
+{
+    const unsigned c[3] = {cx, cy, cz};
+    return ((unsigned long long)c[G.perm[0]] << 42) | ((unsigned long long)c[G.perm[1]] << 21) |
+           (unsigned long long)c[G.perm[2]];
+}
+__device__ inline long long lin_of(const GridDev &G, long long cx, long long cy, long long cz)
+{
+    return ((cz + 1) * G.ny + (cy + 1)) * G.nx + (cx + 1);
+}
+
+// Sort keys in DO-id order: the stable sort then keeps each cell's DOs in id
+// order, as the reference's per-cell vectors do (LeapConsultant.h:170-188).
+__global__ void k_cellkey(Geo g, int n, GridDev G, const unsigned *inv, unsigned long long *keys, unsigned *vals)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const unsigned i = inv[t];
+    const double4 p = g.pos[i];
+    unsigned cx, cy, cz;
+    cell_of(G, mk(p.x, p.y, p.z), cx, cy, cz);
+    keys[t] = cmp_key(G, cx, cy, cz);
+    vals[t] = i;
+}
+
+// Gather state into cell order; per-sphere safe distance
+// SafetyFactor*(2*range + dt_up*|v| + 0.5*dt_up^2*|g|) (LeapConsultant.h:117-119).
+__global__ void k_reorder(Geo src, Geo dst, int n, const unsigned *perm, GridDev G, double *safe, int *lin)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const unsigned p = perm[s];
+    const double4 x = src.pos[p];
+    double4 v = src.vel[p];
+    dst.pos[s] = x;
+    safe[s] = G.sf * (2.0 * x.w + G.safe_v * len(mk(v.x, v.y, v.z)) + G.safe_g);
+    v.w = 0.0;
+    dst.vel[s] = v;
+    dst.avl[s] = src.avl[p];
+    for (int c = 0; c < 6; ++c) dst.ori[(size_t)c * n + s] = src.ori[(size_t)c * n + p];
+    dst.oid[s] = src.oid[p];
+    dst.mdl[s] = src.mdl[p];
+    unsigned cx, cy, cz;
+    cell_of(G, mk(x.x, x.y, x.z), cx, cy, cz);
+    lin[s] = (int)lin_of(G, cx, cy, cz);
+}
+
+__global__ void k_fill_int(int *p, long long n, int v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_cellstart(const int *lin, int n, int *cstart, int *cend)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    if (s == 0 || lin[s] != lin[s - 1]) cstart[lin[s]] = s;
+    if (s == n - 1 || lin[s] != lin[s + 1]) cend[lin[s]] = s + 1;
+}
+
+__device__ inline bool accept_sphere(const OuterArgs &a, int s, d3 xs, double safe_s, int gs, int j)
+{
+    const int gj = a.t.models[a.g.mdl[j]].group;
+    if (a.t.laws[gs * a.t.n_groups + gj].law == LAW_NONE) return false;
+    const double4 pj = a.g.pos[j];
+    return !(len(xs - mk(pj.x, pj.y, pj.z)) > safe_s + a.safe[j]);
+}
+
+__device__ inline bool accept_wall(const OuterArgs &a, d3 xs, double safe_s, int gs, int w)
+{
+    const WallDev &W = a.walls[w];
+    if (a.t.laws[gs * a.t.n_groups + W.group].law == LAW_NONE) return false;
+    const double sw = a.G.sf * (2.0 * W.range + a.G.safe_v * len(W.vel) + a.G.safe_g);
+    return !(domap_wall_distance(xs, W) > safe_s + sw);
+}
+
+// Candidate partners of sphere s in the reference's IactPairTab order: the
+// groups "current cell = c" of the Leap traversal are visited in comparator
+// order; within the own cell the 13 half-shell cells, the self cell and the
+// global DOs follow in reference order (LeapConsultant.h:207-444).
+template <bool FILL>
+__global__ void k_outer(OuterArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.n) return;
+    const double4 p = a.g.pos[s];
+    const d3 xs = mk(p.x, p.y, p.z);
+    const double safe_s = a.safe[s];
+    const int gs = a.t.models[a.g.mdl[s]].group;
+    unsigned cx, cy, cz;
+    cell_of(a.G, xs, cx, cy, cz);
+    // 14 groups: own cell (slot -1) and the 13 cells that hold s's cell in their half shell
+    unsigned long long gkey[14];
+    int gslot[14];
+    int ng = 0;
+    gkey[ng] = cmp_key(a.G, cx, cy, cz);
+    gslot[ng++] = -1;
+    for (int k = 0; k < 13; ++k) {
+        const long long ox = (long long)cx - NB13[k][0], oy = (long long)cy - NB13[k][1],
+                        oz = (long long)cz - NB13[k][2];
+        if (ox < 0 || oy < 0 || oz < 0 || ox >= a.G.nx - 2 || oy >= a.G.ny - 2 || oz >= a.G.nz - 2) continue;
+        const long long l = lin_of(a.G, ox, oy, oz);
+        if (a.cend[l] <= a.cstart[l]) continue;
+        gkey[ng] = cmp_key(a.G, (unsigned)ox, (unsigned)oy, (unsigned)oz);
+        gslot[ng++] = k;
+    }
+    for (int u = 1; u < ng; ++u) {   // insertion sort by cell key
+        const unsigned long long kk = gkey[u];
+        const int ks = gslot[u];
+        int v = u - 1;
+        while (v >= 0 && gkey[v] > kk) {
+            gkey[v + 1] = gkey[v];
+            gslot[v + 1] = gslot[v];
+            --v;
+        }
+        gkey[v + 1] = kk;
+        gslot[v + 1] = ks;
+    }
+    unsigned cnt = 0;
+    unsigned base = FILL ? a.off[s] : 0;
+    auto emit = [&](unsigned j) {
+        if (FILL) a.nbr[base + cnt] = j;
+        ++cnt;
+    };
+    auto scan_cell = [&](long long x, long long y, long long z, bool self) {
+        if (x < -1 || y < -1 || z < -1 || x > a.G.nx - 3 || y > a.G.ny - 3 || z > a.G.nz - 3) return;
+        const long long l = lin_of(a.G, x, y, z);
+        const int b = a.cstart[l], e = a.cend[l];
+        for (int j = b; j < e; ++j) {
+            if (self && j == s) continue;
+            if (accept_sphere(a, s, xs, safe_s, gs, j)) emit((unsigned)j);
+        }
+    };
+    for (int q = 0; q < ng; ++q) {
+        if (gslot[q] < 0) {
+            for (int k = 0; k < 13; ++k)
+                scan_cell((long long)cx + NB13[k][0], (long long)cy + NB13[k][1], (long long)cz + NB13[k][2], false);
+            scan_cell(cx, cy, cz, true);
+            for (int w = 0; w < a.nw; ++w)
+                if (accept_wall(a, xs, safe_s, gs, w)) emit((unsigned)(a.n + w));
+        } else {
+            const int k = gslot[q];
+            scan_cell((long long)cx - NB13[k][0], (long long)cy - NB13[k][1], (long long)cz - NB13[k][2], false);
+        }
+    }
+    if (!FILL) a.count[s] = cnt;
+}
+
+template __global__ void k_outer<false>(OuterArgs);
+template __global__ void k_outer<true>(OuterArgs);
+
+// Neighbour lists at a rebuild, with IactRecordTab's history carry: a pair
+// keeps its ImpactStatus if it was listed and is active now
+// (RebuildIactRecordTab + Initiate, NBSConsultant.cpp:376-395,
+// SimMediator.cpp:690-707); everything else starts from ImpactStatus().
+__global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ unsigned old_oid[];   // [ki][blockDim]
+    if (s >= a.n) return;
+    const unsigned p = a.perm[s];
+    const unsigned hdr = a.eold.hdr[p];
+    const int h_old = hdr >> 7, c_old = hdr & 127;
+    for (int k = 0; k < c_old; ++k) {
+        const unsigned j = a.eold.ent[eidx(h_old, k, a.ki, a.n, p)] & PARTNER_MASK;
+        old_oid[k * blockDim.x + threadIdx.x] = j < (unsigned)a.n ? a.gold.oid[j] : 0x80000000u + (j - a.n);
+    }
+    const double4 ps = a.g.pos[s];
+    const d3 xs = mk(ps.x, ps.y, ps.z);
+    const unsigned my_oid = a.g.oid[s];
+    int q = 0;
+    for (unsigned t = a.off[s]; t < a.off[s + 1]; ++t) {
+        const unsigned j = a.nbr[t];
+        Contact c;
+        unsigned key;
+        bool slave = true;
+        if (j < (unsigned)a.n) {
+            const double4 pj = a.g.pos[j];
+            key = a.g.oid[j];
+            slave = my_oid < key;
+            if (slave) cd_ss(xs, ps.w, mk(pj.x, pj.y, pj.z), pj.w, c);
+            else cd_ss(mk(pj.x, pj.y, pj.z), pj.w, xs, ps.w, c);
+        } else {
+            key = 0x80000000u + (j - a.n);
+            cd_wall(xs, ps.w, a.walls[j - a.n], c);
+        }
+        if (c.depth < -a.skin) continue;
+        if (q >= a.ki) {
+            atomicOr(&a.flags[3], ERR_INNER_CAP);
+            break;
+        }
+        unsigned f = F_DEFAULT;
+        Hist hs;
+        hs.s = mk(0, 0, 0);
+        hs.aux = 0.0;
+        if (c.active) {
+            for (int k = 0; k < c_old; ++k)
+                if (old_oid[k * blockDim.x + threadIdx.x] == key) {
+                    f = (a.eold.ent[eidx(h_old, k, a.ki, a.n, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
+                    load_hist(a.eold.hist, h_old, k, a.ki, a.n, p, hs);
+                    break;
+                }
+        }
+        if (slave) f |= F_SLAVE;
+        a.ell.ent[eidx(0, q, a.ki, a.n, s)] = j | f;
+        store_hist(a.ell.hist, 0, q, a.ki, a.n, s, hs);
+        ++q;
+    }
+    a.ell.hdr[s] = (unsigned char)q;
+    atomicMax(&a.flags[4], q);
+}
+
+// Fresh lists at scene creation (all statuses ImpactStatus()).
+__global__ void k_hdr_zero(unsigned char *hdr, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) hdr[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Export
+// ---------------------------------------------------------------------------
+
+__global__ void k_scatter_state(Geo g, int n, const double *rec_imp, double *out)
+{
+    // out: [n][21] in sphere-rank order: pos vel ox oz av impact ang_impact
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = g.pos[i], v = g.vel[i], w = g.avl[i];
+    double *d = out + (size_t)g.oid[i] * 21;
+    d[0] = p.x; d[1] = p.y; d[2] = p.z;
+    d[3] = v.x; d[4] = v.y; d[5] = v.z;
+    d[6] = g.ori[i]; d[7] = g.ori[n + i]; d[8] = g.ori[2 * n + i];
+    d[9] = g.ori[3 * n + i]; d[10] = g.ori[4 * n + i]; d[11] = g.ori[5 * n + i];
+    d[12] = w.x; d[13] = w.y; d[14] = w.z;
+    for (int c = 0; c < 6; ++c) d[15 + c] = rec_imp ? rec_imp[(size_t)c * n + i] : 0.0;
+}
+
+__global__ void k_export_active(Geo g, Ell ell, int n, int ki, const WallDev *walls, const unsigned *wall_id,
+                                const unsigned *sphere_id, const double *rec, ActiveOut *out,
+                                unsigned long long cap, unsigned long long *counter)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned hdr = ell.hdr[i];
+    const int h = hdr >> 7, cnt = hdr & 127;
+    const double4 p = g.pos[i];
+    const d3 xi = mk(p.x, p.y, p.z);
+    for (int k = 0; k < cnt; ++k) {
+        const unsigned ent = ell.ent[eidx(h, k, ki, n, i)];
+        if (!(ent & F_SLAVE)) continue;
+        const unsigned j = ent & PARTNER_MASK;
+        Contact c;
+        unsigned other;
+        if (j < (unsigned)n) {
+            const double4 pj = g.pos[j];
+            cd_ss(xi, p.w, mk(pj.x, pj.y, pj.z), pj.w, c);
+            other = sphere_id[g.oid[j]];
+        } else {
+            cd_wall(xi, p.w, walls[j - n], c);
+            other = wall_id[j - n];
+        }
+        if (!c.active) continue;
+        const unsigned long long slot = atomicAdd(counter, 1ull);
+        if (!out || slot >= cap) continue;
+        ActiveOut &A = out[slot];
+        const unsigned me = sphere_id[g.oid[i]];
+        A.lo = me < other ? me : other;
+        A.hi = me < other ? other : me;
+        A.flags = ent & ~PARTNER_MASK;
+        Hist hs;
+        load_hist(ell.hist, h, k, ki, n, i, hs);
+        A.hist[0] = hs.s.x; A.hist[1] = hs.s.y; A.hist[2] = hs.s.z; A.hist[3] = hs.aux;
+        for (int q = 0; q < 13; ++q) A.rec[q] = rec ? rec[((size_t)k * 13 + q) * n + i] : 0.0;
+    }
+}
+
+// Impulse on each wall from the last recorded step: sum of ImpactToMaster
+// and AngularImpactToMaster over its contacts (the DOStatus impact a wall's
+// ClearImpact stores), one block per wall, fixed-order reduction.
+__global__ void k_wall_impulse(Ell ell, int n, int ki, const double *rec, double *out)
+{
+    const int w = blockIdx.x;
+    __shared__ double sm[6][256];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned hdr = ell.hdr[i];
+        const int h = hdr >> 7, cnt = hdr & 127;
+        for (int k = 0; k < cnt; ++k) {
+            const unsigned j = ell.ent[eidx(h, k, ki, n, i)] & PARTNER_MASK;
+            if (j != (unsigned)(n + w)) continue;
+            for (int c = 0; c < 6; ++c) acc[c] += rec[((size_t)k * 13 + 6 + c) * n + i];
+        }
+    }
+    for (int c = 0; c < 6; ++c) sm[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 6; ++c) sm[c][threadIdx.x] += sm[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 6; ++c) out[w * 6 + c] = sm[c][0];
+}
+
+}  // namespace vg

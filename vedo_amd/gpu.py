@@ -1,0 +1,223 @@
+"""ctypes binding of libvedo_hip.so (include/vedo_gpu.h).
+
+``GpuSimulation`` is the GPU counterpart of running ``anne`` on a world:
+create (Consultant::Reset + SimMediator::Initiate), step
+(SimMediator::Run), read back DOStatus and the IactRecordTab payload.
+There is no CPU fallback: if the HIP library is missing or no GPU is
+present, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvedo_hip.so")
+
+
+class VgModel(C.Structure):
+    _fields_ = [("shape", C.c_int), ("behavior", C.c_int), ("scope", C.c_int), ("group", C.c_int),
+                ("density", C.c_double), ("density_factor", C.c_double),
+                ("fext", C.c_double * 3), ("attr", C.c_double * 3)]
+
+
+class VgIact(C.Structure):
+    _fields_ = [("law", C.c_int), ("p", C.c_double * 8)]
+
+
+class VgStatus(C.Structure):
+    _fields_ = [("contact", C.c_ubyte), ("last_contact", C.c_ubyte), ("bond", C.c_ubyte),
+                ("last_bond", C.c_ubyte), ("kn", C.c_double), ("init_vel", C.c_double),
+                ("shear", C.c_double * 3), ("point", C.c_double * 3), ("dir", C.c_double * 3),
+                ("to_master", C.c_double * 3), ("ang_to_master", C.c_double * 3),
+                ("overlap", C.c_double)]
+
+
+_D = C.POINTER(C.c_double)
+_U = C.POINTER(C.c_uint)
+
+
+class VgSceneDesc(C.Structure):
+    _fields_ = [("n_do", C.c_int), ("n_models", C.c_int), ("n_groups", C.c_int),
+                ("models", C.POINTER(VgModel)), ("iact", C.POINTER(VgIact)),
+                ("model_of", C.POINTER(C.c_int)),
+                ("pos", _D), ("vel", _D), ("ox", _D), ("oz", _D), ("av", _D),
+                ("dt", C.c_double), ("t_start", C.c_double), ("t_stop", C.c_double),
+                ("t_current", C.c_double), ("g", C.c_double * 3), ("safety_factor", C.c_double),
+                ("up_iact", C.c_uint), ("record", C.c_uint), ("consultant", C.c_int),
+                ("inner_skin", C.c_double)]
+
+
+STATUS_DTYPE = np.dtype([("contact", "u1"), ("last_contact", "u1"), ("bond", "u1"), ("last_bond", "u1"),
+                         ("pad", "u1", 4), ("kn", "f8"), ("init_vel", "f8"), ("shear", "f8", 3),
+                         ("point", "f8", 3), ("dir", "f8", 3), ("to_master", "f8", 3),
+                         ("ang_to_master", "f8", 3), ("overlap", "f8")])
+assert STATUS_DTYPE.itemsize == C.sizeof(VgStatus)
+
+EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_set_history", "vg_step",
+           "vg_sync", "vg_step_assigned", "vg_set_wall_rotation", "vg_get_state", "vg_time",
+           "vg_num_interactions", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats")
+
+_lib = None
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libvedo_hip.so; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GpuError(f"{LIB_PATH} is not built: run `python -m vedo_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.vg_create.argtypes = [C.POINTER(VgSceneDesc), C.POINTER(vp)]
+        L.vg_destroy.argtypes = [vp]
+        L.vg_last_error.restype = C.c_char_p
+        L.vg_last_error.argtypes = [vp]
+        L.vg_create_error.restype = C.c_char_p
+        L.vg_set_history.argtypes = [vp, C.c_int, _U, _U, C.POINTER(VgStatus)]
+        L.vg_step.argtypes = [vp, C.c_int]
+        L.vg_sync.argtypes = [vp]
+        L.vg_step_assigned.argtypes = [vp, C.c_int, _U, _D, _D, _D, _D, _D]
+        L.vg_set_wall_rotation.argtypes = [vp, C.c_int, _U, _D, _D, C.c_double, C.c_double]
+        L.vg_get_state.argtypes = [vp, _D, _D, _D, _D, _D, _D, _D]
+        L.vg_time.restype = C.c_double
+        L.vg_time.argtypes = [vp]
+        L.vg_num_interactions.argtypes = [vp, C.POINTER(C.c_longlong)]
+        L.vg_get_active.argtypes = [vp, C.POINTER(C.c_longlong), _U, _U, C.POINTER(VgStatus)]
+        L.vg_step_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.vg_kernel_timing.argtypes = [vp, C.c_int]
+        L.vg_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_int),
+                               C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _up(a):
+    return a.ctypes.data_as(_U)
+
+
+class GpuSimulation:
+    """One scene on the current GPU (a WorldArrays from vedo_amd.world)."""
+
+    def __init__(self, wa, inner_skin: float = 0.0):
+        L = lib()
+        self.L = L
+        self.n = wa.n_do
+        M = len(wa.model_shape)
+        G = wa.iact_law.shape[0]
+        self._models = (VgModel * M)()
+        for k in range(M):
+            m = self._models[k]
+            m.shape, m.behavior, m.scope, m.group = (int(wa.model_shape[k]), int(wa.model_behavior[k]),
+                                                     int(wa.model_scope[k]), int(wa.model_group[k]))
+            m.density, m.density_factor = float(wa.model_density[k]), float(wa.model_density_factor[k])
+            for d in range(3):
+                m.fext[d] = float(wa.model_fext[k, d])
+                m.attr[d] = float(wa.model_attr[k, d])
+        self._iact = (VgIact * (G * G))()
+        for i in range(G):
+            for j in range(G):
+                it = self._iact[i * G + j]
+                it.law = int(wa.iact_law[i, j])
+                for p in range(8):
+                    it.p[p] = float(wa.iact_param[i, j, p])
+        arrs = [np.ascontiguousarray(x, np.float64) for x in (wa.pos, wa.vel, wa.ox, wa.oz, wa.av)]
+        model_of = np.ascontiguousarray(wa.model_of, np.int32)
+        d = VgSceneDesc()
+        d.n_do, d.n_models, d.n_groups = self.n, M, G
+        d.models, d.iact = self._models, self._iact
+        d.model_of = model_of.ctypes.data_as(C.POINTER(C.c_int))
+        d.pos, d.vel, d.ox, d.oz, d.av = (_dp(a) for a in arrs)
+        d.dt, d.t_start, d.t_stop, d.t_current = wa.dt, wa.t_start, wa.t_stop, wa.t_current
+        for c in range(3):
+            d.g[c] = float(wa.g[c])
+        d.safety_factor = wa.safety_factor
+        d.up_iact, d.record, d.consultant = wa.up_iact, wa.record, 0
+        d.inner_skin = inner_skin
+        h = C.c_void_p()
+        rc = L.vg_create(C.byref(d), C.byref(h))
+        if rc:
+            raise GpuError(f"vg_create failed ({rc}): {L.vg_create_error().decode()}")
+        self.h = h
+
+    def _check(self, rc):
+        if rc:
+            raise GpuError(f"libvedo_hip error {rc}: {self.L.vg_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.vg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, k: int = 1, sync: bool = True):
+        self._check(self.L.vg_step(self.h, int(k)))
+        if sync:
+            self.sync()
+
+    def sync(self):
+        self._check(self.L.vg_sync(self.h))
+
+    def step_assigned(self, ids, pos, vel, ox, oz, av):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        arrs = [np.ascontiguousarray(x, np.float64) for x in (pos, vel, ox, oz, av)]
+        self._check(self.L.vg_step_assigned(self.h, len(ids), _up(ids), *(_dp(a) for a in arrs)))
+
+    def set_wall_rotation(self, ids, axis_point, axis_dir, omega: float, t_ref: float = 0.0):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        c = np.ascontiguousarray(axis_point, np.float64)
+        a = np.ascontiguousarray(axis_dir, np.float64)
+        self._check(self.L.vg_set_wall_rotation(self.h, len(ids), _up(ids), _dp(c), _dp(a), omega, t_ref))
+
+    def state(self):
+        out = {k: np.zeros((self.n, 3)) for k in ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")}
+        self._check(self.L.vg_get_state(self.h, *(_dp(out[k]) for k in
+                                                   ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact"))))
+        return out
+
+    @property
+    def time(self):
+        return self.L.vg_time(self.h)
+
+    def num_interactions(self):
+        n = C.c_longlong()
+        self._check(self.L.vg_num_interactions(self.h, C.byref(n)))
+        return n.value
+
+    def count_active(self) -> int:
+        n = C.c_longlong()
+        self._check(self.L.vg_get_active(self.h, C.byref(n), None, None, None))
+        return n.value
+
+    def active(self):
+        n = C.c_longlong(self.count_active())
+        lo, hi = np.zeros(n.value, np.uint32), np.zeros(n.value, np.uint32)
+        st = np.zeros(n.value, STATUS_DTYPE)
+        self._check(self.L.vg_get_active(self.h, C.byref(n), _up(lo), _up(hi),
+                                         st.ctypes.data_as(C.POINTER(VgStatus))))
+        return lo[:n.value], hi[:n.value], st[:n.value]
+
+    def kernel_timing(self, max_launches: int):
+        self._check(self.L.vg_kernel_timing(self.h, int(max_launches)))
+
+    def timing(self):
+        ms, k = C.c_double(), C.c_int()
+        self._check(self.L.vg_step_timing(self.h, C.byref(ms), C.byref(k)))
+        return ms.value, k.value
+
+    def stats(self):
+        rb, rl, mf, cap = C.c_longlong(), C.c_longlong(), C.c_int(), C.c_int()
+        self._check(self.L.vg_stats(self.h, C.byref(rb), C.byref(rl), C.byref(mf), C.byref(cap)))
+        return dict(rebuilds=rb.value, relists=rl.value, max_fill=mf.value, capacity=cap.value)

@@ -175,7 +175,8 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
                rmax: float = 2e-3, classes: int = 64, density: float = 2500.0, R: float = 0.15,
                L: Optional[float] = None, omega: float = 2.0, kn: float = 1e4, kt_ratio: float = 0.8,
                e: float = 0.6, mu: float = 0.4, dt: float = 5e-6, steps: int = 10000,
-               sides: int = 64, thick: float = 2e-3, gap: float = 1.01):
+               sides: int = 64, thick: float = 2e-3, gap: float = 1.01, arrays: bool = False,
+               up_iact: int = 20, record: int = 10 ** 9):
     """Polydisperse spheres in a rotating drum of quasi-plates.
 
     Radii are log-normal (median, sigma) clipped to [rmin, rmax] and quantised
@@ -186,7 +187,9 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
     each z-layer uses the spacing of its largest sphere times ``gap``.  When
     ``L`` is None the drum is made just long enough for that lattice.
 
-    Returns (world, wall_ids, axis_point, axis_dir)."""
+    Returns (world, wall_ids, axis_point, axis_dir); with ``arrays=True`` the
+    first item is a WorldArrays built without per-sphere Python objects (for
+    million-sphere scenes)."""
     rng = SplitMix64(seed)
     cls = lognormal_classes(classes, median, sigma, rmin, rmax)
     r_raw = np.clip(median * np.exp(sigma * rng.normal(n)), rmin, rmax)
@@ -224,12 +227,12 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
             s = 2.0 * gap * r_sorted[placed]
             nx, ny = layer_capacity(z, s, L_try)
             take = min(nx * ny, n - placed)
-            for _ in range(4):   # settle the layer spacing on its largest sphere
-                if take <= 0:
-                    break
+            while take > 0:   # shrink the layer until its largest sphere fits the spacing
                 s = 2.0 * gap * r_sorted[placed + take - 1]
                 nx, ny = layer_capacity(z, s, L_try)
-                take = min(nx * ny, n - placed)
+                if nx * ny >= take:
+                    break
+                take = nx * ny
             if take <= 0:
                 z += 0.25 * s
                 if z > 0.6 * rin:
@@ -261,7 +264,7 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
     for k, r in enumerate(cls):
         models.append(DOModel(f"S{k:02d}", "Granule", "mobile", "local", density, 1.0, (0, 0, 0),
                               "Sphere", (float(r),)))
-    statuses = [DOStatus(f"S{ci[i]:02d}", tuple(pos_final[i])) for i in range(n)]
+    statuses = [] if arrays else [DOStatus(f"S{ci[i]:02d}", tuple(pos_final[i])) for i in range(n)]
     # drum walls: mantle plates tangent to the circle of radius R, axis along y
     chord = 2.0 * R * math.sin(math.pi / sides) * 1.02
     models.append(DOModel("Mantle", "Wall", "orbital", "global", 0.0, 1.0, (0, 0, 0), "QuasiPlate",
@@ -284,6 +287,17 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
             IactModel("Wall", "Granule", "ISwBSDBF", spring_dashpot(kn, kt_ratio, e, mu, False))]
     sp = SystemParameter(0.0, steps * dt, dt, 0.0, (0.0, 0.0, -9.81))
     world = DOWorld(sp, models, iact, statuses)
+    if arrays:
+        wa = world.to_arrays(up_iact, record)
+        wall_model = wa.model_of
+        one = np.ones((n, 1))
+        wa.model_of = np.concatenate([ci.astype(np.int32), wall_model])
+        wa.pos = np.concatenate([pos_final, wa.pos])
+        wa.vel = np.concatenate([np.zeros((n, 3)), wa.vel])
+        wa.ox = np.concatenate([one * np.array([1.0, 0.0, 0.0]), wa.ox])
+        wa.oz = np.concatenate([one * np.array([0.0, 0.0, 1.0]), wa.oz])
+        wa.av = np.concatenate([np.zeros((n, 3)), wa.av])
+        return wa, wall_ids, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0)
     return world, wall_ids, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0)
 
 

@@ -55,10 +55,10 @@ def cpu_baseline(state, wa, n_gpu, seconds):
     ids = np.concatenate([keep, walls])
     sub = WorldArrays(**{k: getattr(wa, k) for k in wa.__dataclass_fields__})
     sub.model_of = wa.model_of[ids]
+    sub.model_behavior = np.where(wa.model_shape == 1, wa.model_behavior, 2).astype(np.int32)   # walls fixed
     for f in ("pos", "vel", "ox", "oz", "av"):
         setattr(sub, f, np.ascontiguousarray(state[f][ids]))
     sim = OracleSim(sub)
-    # walls are prescribed like on the GPU: rigid rotation about the axis
     t0 = time.perf_counter()
     steps = 0
     while time.perf_counter() - t0 < seconds:

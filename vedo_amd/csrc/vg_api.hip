@@ -75,8 +75,15 @@ struct vg_scene {
     size_t onbr_cap = 0;
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
-    double *rec = nullptr, *rec_imp = nullptr;
-    bool rec_valid = false;
+    double *rec = nullptr, *rec_imp = nullptr, *rec_imp2 = nullptr;
+    bool rec_valid = false;   // rec_imp holds the last step's impulses
+    bool ent_valid = false;   // rec (per-entry ImpactStatus fields) matches the current lists
+    // IactRecordTab snapshot taken before a rebuild re-sorts the spheres
+    ActiveOut *snap = nullptr;
+    size_t snap_cap = 0;
+    long long snap_count = 0;
+    double *snap_wimp = nullptr;
+    bool snap_valid = false;
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
@@ -208,7 +215,9 @@ static int rebuild(vg_scene *h)
     (void)rocprim::radix_sort_pairs(nullptr, need, h->keys, h->skeys, h->svals, h->perm, (size_t)n, 0, 63, st);
     if (ensure_tmp(h, need)) return VG_EHIP;
     HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, h->keys, h->skeys, h->svals, h->perm, (size_t)n, 0, 63, st));
-    hipLaunchKernelGGL(k_reorder, dim3(nblk(n, 256)), dim3(256), 0, st, G0, G1, n, h->perm, gd, h->safe, h->lin);
+    hipLaunchKernelGGL(k_reorder, dim3(nblk(n, 256)), dim3(256), 0, st, G0, G1, n, h->perm, gd, h->safe, h->lin,
+                       h->rec_valid ? (const double *)h->rec_imp : (const double *)nullptr, h->rec_imp2);
+    if (h->rec_valid) std::swap(h->rec_imp, h->rec_imp2);
     hipLaunchKernelGGL(k_cellstart, dim3(nblk(n, 256)), dim3(256), 0, st, h->lin, n, h->cstart, h->cend);
     // candidate list: count, scan, fill
     OuterArgs oa;
@@ -246,7 +255,7 @@ static int rebuild(vg_scene *h)
     HIPCHK(hipGetLastError());
     h->cur ^= 1;
     h->ecur ^= 1;
-    h->rec_valid = false;
+    h->ent_valid = false;
     h->rebuilds++;
     return 0;
 }
@@ -284,7 +293,7 @@ static void free_scene(vg_scene *h)
     }
     F(h->models); F(h->laws); F(h->walls); F(h->walls_ref); F(h->wkin); F(h->assigned); F(h->flags);
     F(h->relists); F(h->red); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->safe); F(h->lin);
-    F(h->cstart); F(h->cend); F(h->ocount); F(h->ooff); F(h->onbr); F(h->tmp); F(h->rec); F(h->rec_imp); F(h->d_sphere_ids); F(h->d_wall_ids);
+    F(h->cstart); F(h->cend); F(h->ocount); F(h->ooff); F(h->onbr); F(h->tmp); F(h->rec); F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->d_sphere_ids); F(h->d_wall_ids);
     for (auto e : h->kev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -443,6 +452,7 @@ extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
     rc |= dalloc(h, &h->safe, n) | dalloc(h, &h->lin, n);
     rc |= dalloc(h, &h->ocount, (size_t)n + 1) | dalloc(h, &h->ooff, (size_t)n + 1);
     rc |= dalloc(h, &h->rec, (size_t)h->ki * 13 * n) | dalloc(h, &h->rec_imp, (size_t)6 * n);
+    rc |= dalloc(h, &h->rec_imp2, (size_t)6 * n) | dalloc(h, &h->snap_wimp, (size_t)6 * std::max(h->nw, 1));
     rc |= dalloc(h, &h->d_sphere_ids, n) | dalloc(h, &h->d_wall_ids, std::max(h->nw, 1));
     if (rc) return bail(VG_ENOMEM);
     h->onbr_cap = (size_t)n * 64;
@@ -523,6 +533,45 @@ static StepArgs step_args(vg_scene *h)
     return a;
 }
 
+static int export_active(vg_scene *h, ActiveOut *out, size_t cap, unsigned long long *dcnt)
+{
+    HIPCHK(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), h->stream));
+    hipLaunchKernelGGL(k_export_active, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, h->geo[h->cur],
+                       h->ell[h->ecur], h->n, h->ki, (const WallDev *)h->walls, (const unsigned *)h->d_wall_ids,
+                       (const unsigned *)h->d_sphere_ids,
+                       h->ent_valid ? (const double *)h->rec : (const double *)nullptr, out,
+                       (unsigned long long)cap, dcnt);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// RebuildIactRecordTab before the candidate-list reset (NBSConsultant.cpp:40-47):
+// the active contacts and the wall impulses of the last step are kept, since
+// the rebuild re-sorts the spheres and their lists.
+static int snapshot_records(vg_scene *h)
+{
+    unsigned long long *dcnt = nullptr;
+    if (dalloc(h, &dcnt, 1)) return VG_ENOMEM;
+    if (export_active(h, nullptr, 0, dcnt)) return VG_EHIP;
+    unsigned long long cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (cnt > h->snap_cap) {
+        if (h->snap) HIPCHK(hipFree(h->snap));
+        h->snap_cap = cnt + cnt / 4 + 64;
+        if (dalloc(h, &h->snap, h->snap_cap)) return VG_ENOMEM;
+    }
+    if (export_active(h, h->snap, h->snap_cap, dcnt)) return VG_EHIP;
+    if (h->nw && h->ent_valid)
+        hipLaunchKernelGGL(k_wall_impulse, dim3(h->nw), dim3(256), 0, h->stream, h->ell[h->ecur], h->n, h->ki,
+                           (const double *)h->rec, h->snap_wimp);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(dcnt));
+    h->snap_count = (long long)cnt;
+    h->snap_valid = true;
+    return 0;
+}
+
 static int one_step(vg_scene *h, bool rec, bool assigned)
 {
     StepArgs a = step_args(h);
@@ -545,8 +594,13 @@ static int one_step(vg_scene *h, bool rec, bool assigned)
     }
     (void)assigned;
     h->rec_valid = rec;
+    h->ent_valid = rec;
+    h->snap_valid = false;
     HIPCHK(hipGetLastError());
-    if (h->round % h->up_iact == 0) return rebuild(h);
+    if (h->round % h->up_iact == 0) {
+        if (rec && snapshot_records(h)) return VG_EHIP;
+        return rebuild(h);
+    }
     return 0;
 }
 
@@ -665,7 +719,10 @@ extern "C" int vg_get_state(vg_scene *h, double *pos, double *vel, double *ox, d
                        h->rec_valid ? h->rec_imp : (const double *)nullptr, dout);
     std::vector<double> wimp((size_t)6 * std::max(h->nw, 1), 0.0);
     double *dw = nullptr;
-    if (h->nw && h->rec_valid) {
+    if (h->nw && h->snap_valid) {
+        if (dalloc(h, &dw, (size_t)6 * h->nw)) return VG_ENOMEM;
+        HIPCHK(hipMemcpyAsync(dw, h->snap_wimp, sizeof(double) * 6 * h->nw, hipMemcpyDeviceToDevice, h->stream));
+    } else if (h->nw && h->ent_valid) {
         if (dalloc(h, &dw, (size_t)6 * h->nw)) return VG_ENOMEM;
         hipLaunchKernelGGL(k_wall_impulse, dim3(h->nw), dim3(256), 0, h->stream, h->ell[h->ecur], n, h->ki,
                            (const double *)h->rec, dw);
@@ -713,42 +770,42 @@ extern "C" int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *
 {
     if (!h || !n) return VG_EINVAL;
     HIPCHK(hipStreamSynchronize(h->stream));
-    unsigned long long *dcnt = nullptr;
-    if (dalloc(h, &dcnt, 1)) return VG_ENOMEM;
-    HIPCHK(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), h->stream));
-    if (!st) {   // count only
-        hipLaunchKernelGGL(k_export_active, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, h->geo[h->cur],
-                           h->ell[h->ecur], h->n, h->ki, (const WallDev *)h->walls, (const unsigned *)h->d_wall_ids,
-                           (const unsigned *)h->d_sphere_ids, (const double *)nullptr, (ActiveOut *)nullptr,
-                           0ull, dcnt);
-        unsigned long long cnt = 0;
+    unsigned long long cnt = 0;
+    std::vector<ActiveOut> ho;
+    if (h->snap_valid) {
+        cnt = (unsigned long long)h->snap_count;
+        if (st) {
+            ho.resize(std::min<size_t>(cnt, (size_t)std::max(*n, 0ll)));
+            if (!ho.empty()) HIPCHK(hipMemcpy(ho.data(), h->snap, sizeof(ActiveOut) * ho.size(), hipMemcpyDeviceToHost));
+        }
+    } else {
+        unsigned long long *dcnt = nullptr;
+        if (dalloc(h, &dcnt, 1)) return VG_ENOMEM;
+        if (export_active(h, nullptr, 0, dcnt)) return VG_EHIP;
         HIPCHK(hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
+        if (st) {
+            const size_t cap = (size_t)std::max(*n, 0ll);
+            ActiveOut *dout = nullptr;
+            if (dalloc(h, &dout, cap)) return VG_ENOMEM;
+            if (export_active(h, dout, cap, dcnt)) return VG_EHIP;
+            ho.resize(std::min<size_t>(cnt, cap));
+            if (!ho.empty()) HIPCHK(hipMemcpyAsync(ho.data(), dout, sizeof(ActiveOut) * ho.size(), hipMemcpyDeviceToHost,
+                                                  h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            HIPCHK(hipFree(dout));
+        }
         HIPCHK(hipFree(dcnt));
+    }
+    if (!st) {
         *n = (long long)cnt;
         return 0;
     }
-    const size_t cap = (size_t)std::max(*n, 0ll);
-    ActiveOut *dout = nullptr;
-    if (dalloc(h, &dout, cap)) return VG_ENOMEM;
-    hipLaunchKernelGGL(k_export_active, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, h->geo[h->cur],
-                       h->ell[h->ecur], h->n, h->ki, (const WallDev *)h->walls, (const unsigned *)h->d_wall_ids,
-                       (const unsigned *)h->d_sphere_ids,
-                       h->rec_valid ? (const double *)h->rec : (const double *)nullptr, dout,
-                       (unsigned long long)cap, dcnt);
-    unsigned long long cnt = 0;
-    HIPCHK(hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    const size_t got = std::min((size_t)cnt, cap);
-    std::vector<ActiveOut> ho(got);
-    if (got) HIPCHK(hipMemcpy(ho.data(), dout, sizeof(ActiveOut) * got, hipMemcpyDeviceToHost));
-    HIPCHK(hipFree(dout));
-    HIPCHK(hipFree(dcnt));
-    if (cnt > cap) return fail(h, VG_ECAPACITY, "vg_get_active: capacity smaller than the active count");
+    if ((long long)cnt > *n) return fail(h, VG_ECAPACITY, "vg_get_active: capacity smaller than the active count");
     std::sort(ho.begin(), ho.end(), [](const ActiveOut &x, const ActiveOut &y) {
         return x.lo != y.lo ? x.lo < y.lo : x.hi < y.hi;
     });
-    for (size_t k = 0; k < got; ++k) {
+    for (size_t k = 0; k < ho.size(); ++k) {
         const ActiveOut &A = ho[k];
         lo[k] = A.lo;
         hi[k] = A.hi;
@@ -767,7 +824,7 @@ extern "C" int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *
         }
         S.overlap = A.rec[12];
     }
-    *n = (long long)got;
+    *n = (long long)ho.size();
     return 0;
 }
 

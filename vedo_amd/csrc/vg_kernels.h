@@ -93,7 +93,8 @@ __global__ void k_walls(WallDev *walls, int nw, const double *assigned, const Wa
                         double dt, double t_new);
 __global__ void k_bounds(Geo g, int n, unsigned long long *red, unsigned *inv, double *xs, double *ys, double *zs);
 __global__ void k_cellkey(Geo g, int n, GridDev G, const unsigned *inv, unsigned long long *keys, unsigned *vals);
-__global__ void k_reorder(Geo src, Geo dst, int n, const unsigned *perm, GridDev G, double *safe, int *lin);
+__global__ void k_reorder(Geo src, Geo dst, int n, const unsigned *perm, GridDev G, double *safe, int *lin,
+                          const double *imp_src, double *imp_dst);
 __global__ void k_fill_int(int *p, long long n, int v);
 __global__ void k_cellstart(const int *lin, int n, int *cstart, int *cend);
 template <bool FILL> __global__ void k_outer(OuterArgs a);

@@ -329,7 +329,8 @@ __global__ void k_cellkey(Geo g, int n, GridDev G, const unsigned *inv, unsigned
 
 // Gather state into cell order; per-sphere safe distance
 // SafetyFactor*(2*range + dt_up*|v| + 0.5*dt_up^2*|g|) (LeapConsultant.h:117-119).
-__global__ void k_reorder(Geo src, Geo dst, int n, const unsigned *perm, GridDev G, double *safe, int *lin)
+__global__ void k_reorder(Geo src, Geo dst, int n, const unsigned *perm, GridDev G, double *safe, int *lin,
+                          const double *imp_src, double *imp_dst)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -344,6 +345,8 @@ __global__ void k_reorder(Geo src, Geo dst, int n, const unsigned *perm, GridDev
     for (int c = 0; c < 6; ++c) dst.ori[(size_t)c * n + s] = src.ori[(size_t)c * n + p];
     dst.oid[s] = src.oid[p];
     dst.mdl[s] = src.mdl[p];
+    if (imp_src)   // last-step impulses follow their sphere
+        for (int c = 0; c < 6; ++c) imp_dst[(size_t)c * n + s] = imp_src[(size_t)c * n + p];
     unsigned cx, cy, cz;
     cell_of(G, mk(x.x, x.y, x.z), cx, cy, cz);
     lin[s] = (int)lin_of(G, cx, cy, cz);

@@ -1,13 +1,13 @@
 """Benchmark of the DEM step on one MI355X.
 
-Workload (N=1 GPU): polydisperse spheres (log-normal radii, median 1 mm,
+Workload (N=1 GPU): a settled bed of polydisperse spheres (log-normal radii, median 1 mm,
 sigma_ln 0.2, clipped to [0.5, 2] mm, 64 radius classes, rho 2500) in a drum
 of 64 quasi-plates + 2 end plates (R = 0.15 m) rotating at 2 rad/s about a
 horizontal axis; ISwBSDBF linear spring-dashpot kn = 1e4 N/m, ks = 0.8 kn,
 e = 0.6, mu = 0.4; dt = 5e-6 s; LeapConsultant candidate lists rebuilt every
 UpIact steps; FP64 throughout.  A "step" is one SimMediator::Run over all
-spheres.  The scene first settles for --prep untimed steps so that the timed
-steps run on a contact-rich bed.
+spheres.  The bed first settles for --prep untimed steps in the resting drum,
+then the drum starts turning at 2 rad/s and the timed steps follow.
 
 Prints one JSON line (see the metric/roofline/cpu_baseline fields).
 """
@@ -48,9 +48,10 @@ def cpu_baseline(state, wa, n_gpu, seconds):
     n = n_gpu
     pos = state["pos"]
     y = pos[:n, 1]
-    target = 20000
-    y_cut = np.sort(y)[min(target, n) - 1]
-    keep = np.nonzero(y[:n] <= y_cut)[0]
+    target = min(20000, n)
+    y_mid = float(np.median(y))
+    keep = np.sort(np.argsort(np.abs(y - y_mid), kind="stable")[:target])
+    y_lo, y_hi = float(y[keep].min()), float(y[keep].max())
     walls = np.arange(n, wa.n_do)
     ids = np.concatenate([keep, walls])
     sub = WorldArrays(**{k: getattr(wa, k) for k in wa.__dataclass_fields__})
@@ -66,9 +67,9 @@ def cpu_baseline(state, wa, n_gpu, seconds):
         steps += 1
     el = time.perf_counter() - t0
     return {"value": len(keep) * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (FP64 C restatement, 1 thread) on {len(keep)} spheres of a slab of the same "
-                      f"settled drum (y <= {y_cut:.4f} m) with its {len(walls)} walls held fixed, {steps} steps "
-                      f"in {el:.1f} s"}
+            "sample": f"oracle (FP64 C restatement, 1 thread) on the {len(keep)} spheres of the same bed in "
+                      f"the axial slab {y_lo:.4f} <= y <= {y_hi:.4f} m (middle of the drum) with its {len(walls)} "
+                      f"walls held fixed, {steps} steps in {el:.1f} s"}
 
 
 def main():
@@ -77,8 +78,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--prep", type=int, default=1500)
-    ap.add_argument("--up-iact", type=int, default=50)
+    ap.add_argument("--prep", type=int, default=300)
+    ap.add_argument("--up-iact", type=int, default=100)
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -88,7 +90,10 @@ def main():
         sys.exit(2)
 
     from vedo_amd import scenes
-    from vedo_amd.gpu import GpuSimulation
+    from vedo_amd.gpu import GpuSimulation, copy_bandwidth
+
+    copy_gbps = copy_bandwidth(1 << 30, 10)
+    log(f"[bench] streaming copy kernel: {copy_gbps:.0f} GB/s")
 
     t = time.perf_counter()
     wa, wall_ids, c, a = scenes.drum_scene(args.n, arrays=True, up_iact=args.up_iact)
@@ -96,7 +101,8 @@ def main():
         f"({time.perf_counter() - t:.1f} s)")
     t = time.perf_counter()
     sim = GpuSimulation(wa)
-    sim.set_wall_rotation(wall_ids, c, a, 2.0, 0.0)
+    # the bed settles in a resting drum, then the drum starts turning
+    sim.set_wall_rotation(wall_ids, c, a, 0.0, 0.0)
     log(f"[bench] created on GPU ({time.perf_counter() - t:.1f} s)")
     t = time.perf_counter()
     done = 0
@@ -106,6 +112,14 @@ def main():
         done += k
         log(f"[bench] prep {done}/{args.prep} steps, active contacts {sim.count_active()}")
     log(f"[bench] prep {time.perf_counter() - t:.1f} s")
+    sim.set_wall_rotation(wall_ids, c, a, 2.0, sim.time)
+    skin = sim.skin
+    tune = None
+    if not args.no_tune:
+        from vedo_amd.autotune import tune_skin
+        skin, tune = tune_skin(sim, float(wa.model_attr[wa.model_shape == 1, 0].min()), args.up_iact)
+        log(f"[bench] skin autotune: {({f'{k * 1e3:.3f}mm': round(v * 1e6, 1) for k, v in tune.items()})} us/step "
+            f"-> {skin * 1e3:.3f} mm")
     sim.step(args.warmup)
     c0 = sim.count_active()
     # timed region: K steps bracketed by device synchronisation
@@ -132,9 +146,11 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "rotating-drum polydisperse spheres, ISwBSDBF, LeapConsultant", "spheres": n,
                    "walls": len(wall_ids), "radius_classes": 64, "dt": wa.dt, "up_iact": args.up_iact,
-                   "prep_steps": args.prep, "active_contacts": c_act, "parallelism": "single GPU"},
+                   "prep_steps": args.prep, "active_contacts": c_act, "neighbour_skin_m": skin,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "copy_kernel_gbps": copy_gbps, "frac_of_copy": achieved / copy_gbps,
                      "kernel": "k_step", "kernel_avg_us": k_avg_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_step},
         "rebuilds": stats["rebuilds"], "relists": stats["relists"],

@@ -145,6 +145,13 @@ double vg_time(const vg_scene *h);
 
 /* Number of interactions in the candidate list (Consultant::GetIactNum). */
 int vg_num_interactions(vg_scene *h, long long *n);
+
+/* Neighbour-list skin (distance beyond contact within which candidates are
+ * walked every step).  Any value gives the same results; it trades per-step
+ * list length against relist frequency.  No reference counterpart: the
+ * reference walks its whole candidate list every step. */
+int vg_set_skin(vg_scene *h, double skin);
+double vg_get_skin(const vg_scene *h);
 /* Active contacts (ContactInfo.bActive of the last detect) with their
  * ImpactStatus, keyed (lo, hi) -- the IactRecordTab payload that
  * RebuildIactRecordTab saves (NBSConsultant.cpp:376-395).  Call with
@@ -159,6 +166,12 @@ int vg_kernel_timing(vg_scene *h, int max_launches);
 int vg_step_timing(vg_scene *h, double *ms, int *launches);
 /* Counters: rebuilds, relists, max neighbour-list fill. */
 int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, int *max_fill, int *capacity);
+
+/* Streaming copy of `bytes` bytes, `iters` times, on the current device;
+ * returns the achieved read+write bandwidth in GB/s.  Diagnostics: measures
+ * the achievable HBM bandwidth and calibrates traffic counters.  No reference
+ * counterpart. */
+int vg_copy_bench(size_t bytes, int iters, double *gbps);
 
 #ifdef __cplusplus
 }

@@ -45,8 +45,10 @@ struct vg_scene {
     std::vector<WallDev> walls_h;
     std::vector<ModelDev> models_h;
     std::vector<LawDev> laws_h;
+    bool sd_only = false;      // every law is of the spring-dashpot family
+    int single_group = 0, no_bond = 1;
     // scalars
-    double dt = 0, t = 0, t_stop = 0, g[3] = {0, 0, 0}, sf = 1.1, skin = 0;
+    double dt = 0, t = 0, t_stop = 0, g[3] = {0, 0, 0}, sf = 1.1, skin = 0, bond_reach = 0;
     unsigned up_iact = 1, record = 0;
     unsigned long long round = 0, step_count = 0;
     long long rebuilds = 0;
@@ -63,7 +65,7 @@ struct vg_scene {
     double *assigned = nullptr;
     int *flags = nullptr;
     unsigned long long *relists = nullptr;
-    unsigned long long *red = nullptr;
+    unsigned long long *red = nullptr, *bpart = nullptr;
     unsigned *inv = nullptr, *perm = nullptr, *svals = nullptr;
     unsigned long long *keys = nullptr, *skeys = nullptr;
     double *xs[2] = {nullptr, nullptr}, *ys[2] = {nullptr, nullptr}, *zs[2] = {nullptr, nullptr};
@@ -71,8 +73,8 @@ struct vg_scene {
     int *lin = nullptr;
     int *cstart = nullptr, *cend = nullptr;
     long long cells_cap = 0;
-    unsigned *ocount = nullptr, *ooff = nullptr, *onbr = nullptr;
-    size_t onbr_cap = 0;
+    unsigned *ocount = nullptr, *onbr = nullptr;
+    int ko = 64;   // candidate-list capacity per sphere (grows)
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
     double *rec = nullptr, *rec_imp = nullptr, *rec_imp2 = nullptr;
@@ -148,10 +150,10 @@ static int rebuild(vg_scene *h)
     Geo &G0 = h->geo[h->cur];
     Geo &G1 = h->geo[h->cur ^ 1];
     // bounds
-    unsigned long long init[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    HIPCHK(hipMemcpyAsync(h->red, init, sizeof(init), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_bounds, dim3(nblk(n, 256)), dim3(256), 0, st, G0, n, h->red, h->inv, h->xs[0], h->ys[0],
+    const int nb = (int)nblk(n, 256);
+    hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(256), 0, st, G0, n, h->bpart, h->inv, h->xs[0], h->ys[0],
                        h->zs[0]);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, (const unsigned long long *)h->bpart, nb, h->red);
     // 2%-trimmed spans pick the comparator (LeapConsultant.cpp:139-191)
     size_t need = 0;
     (void)rocprim::radix_sort_keys(nullptr, need, h->xs[0], h->xs[1], (size_t)n, 0, 64, st);
@@ -219,34 +221,30 @@ static int rebuild(vg_scene *h)
                        h->rec_valid ? (const double *)h->rec_imp : (const double *)nullptr, h->rec_imp2);
     if (h->rec_valid) std::swap(h->rec_imp, h->rec_imp2);
     hipLaunchKernelGGL(k_cellstart, dim3(nblk(n, 256)), dim3(256), 0, st, h->lin, n, h->cstart, h->cend);
-    // candidate list: count, scan, fill
+    // candidate list in one pass; grow the capacity and redo if a sphere overflowed
     OuterArgs oa;
     oa.g = G1; oa.n = n; oa.nw = h->nw; oa.G = gd; oa.cstart = h->cstart; oa.cend = h->cend; oa.safe = h->safe;
-    oa.walls = h->walls; oa.t = tables(h); oa.count = h->ocount; oa.off = h->ooff; oa.nbr = h->onbr;
-    hipLaunchKernelGGL(k_outer<false>, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
-    HIPCHK(hipMemsetAsync(h->ocount + n, 0, sizeof(unsigned), st));
-    need = 0;
-    (void)rocprim::exclusive_scan(nullptr, need, h->ocount, h->ooff, 0u, (size_t)n + 1, rocprim::plus<unsigned>(), st);
-    if (ensure_tmp(h, need)) return VG_EHIP;
-    HIPCHK(rocprim::exclusive_scan(h->tmp, need, h->ocount, h->ooff, 0u, (size_t)n + 1, rocprim::plus<unsigned>(),
-                                   st));
-    unsigned total = 0;
-    HIPCHK(hipMemcpyAsync(&total, h->ooff + n, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (total > h->onbr_cap) {
-        if (h->onbr) HIPCHK(hipFree(h->onbr));
-        h->onbr_cap = (size_t)total + total / 4 + 1024;
-        if (dalloc(h, &h->onbr, h->onbr_cap)) return VG_ENOMEM;
+    oa.walls = h->walls; oa.t = tables(h); oa.count = h->ocount; oa.nbr = h->onbr; oa.ko = h->ko;
+    oa.single_group = h->single_group; oa.flags = h->flags;
+    for (;;) {
+        HIPCHK(hipMemsetAsync(h->flags + 5, 0, sizeof(int), st));
+        hipLaunchKernelGGL(k_outer, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
+        int maxc = 0;
+        HIPCHK(hipMemcpyAsync(&maxc, h->flags + 5, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (maxc <= h->ko) break;
+        HIPCHK(hipFree(h->onbr));
+        h->onbr = nullptr;
+        h->ko = maxc + maxc / 4 + 8;
+        if (dalloc(h, &h->onbr, (size_t)h->ko * n)) return VG_ENOMEM;
         oa.nbr = h->onbr;
+        oa.ko = h->ko;
     }
-    oa.off = h->ooff;
-    hipLaunchKernelGGL(k_outer<true>, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
-    // every sphere pair is listed from both sides; walls once
-    h->n_iact = (long long)total;   // refined below
+    h->n_iact = -1;   // counted on request (vg_num_interactions)
     // neighbour lists with history carry
     InnerArgs ia;
     ia.g = G1; ia.gold = G0; ia.ell = h->ell[h->ecur ^ 1]; ia.eold = h->ell[h->ecur];
-    ia.n = n; ia.nw = h->nw; ia.ki = h->ki; ia.perm = h->perm; ia.off = h->ooff; ia.nbr = h->onbr;
+    ia.n = n; ia.nw = h->nw; ia.ki = h->ki; ia.perm = h->perm; ia.cnt = h->ocount; ia.nbr = h->onbr; ia.ko = h->ko;
     ia.walls = h->walls; ia.skin = h->skin; ia.flags = h->flags;
     hipLaunchKernelGGL(k_inner_build, dim3(nblk(n, 128)), dim3(128), (size_t)h->ki * 128 * sizeof(unsigned), st,
                        ia);
@@ -276,7 +274,7 @@ static int alloc_geo(vg_scene *h, Geo &g, int n)
 
 static int alloc_ell(vg_scene *h, Ell &e, int n, int ki)
 {
-    if (dalloc(h, &e.hdr, n) || dalloc(h, &e.ent, (size_t)2 * ki * n) || dalloc(h, &e.hist, (size_t)8 * ki * n))
+    if (dalloc(h, &e.hdr, n) || dalloc(h, &e.ent, (size_t)2 * ki * n) || dalloc(h, &e.hist, (size_t)2 * ki * n))
         return VG_ENOMEM;
     return 0;
 }
@@ -292,8 +290,8 @@ static void free_scene(vg_scene *h)
         F(h->xs[b]); F(h->ys[b]); F(h->zs[b]);
     }
     F(h->models); F(h->laws); F(h->walls); F(h->walls_ref); F(h->wkin); F(h->assigned); F(h->flags);
-    F(h->relists); F(h->red); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->safe); F(h->lin);
-    F(h->cstart); F(h->cend); F(h->ocount); F(h->ooff); F(h->onbr); F(h->tmp); F(h->rec); F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->d_sphere_ids); F(h->d_wall_ids);
+    F(h->relists); F(h->red); F(h->bpart); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->safe); F(h->lin);
+    F(h->cstart); F(h->cend); F(h->ocount); F(h->onbr); F(h->tmp); F(h->rec); F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->d_sphere_ids); F(h->d_wall_ids);
     for (auto e : h->kev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -403,6 +401,20 @@ extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
         if ((L == VG_BSDBF || L == VG_BSDB2F) && d->iact[k].p[4] != 0.0 && d->iact[k].p[0] != 0.0)
             bond_reach = std::max(bond_reach, std::fabs(d->iact[k].p[4] / d->iact[k].p[0]));
     }
+    h->sd_only = true;
+    for (const LawDev &L : h->laws_h) {
+        if (L.law == VG_H || L.law == VG_HERTZ) h->sd_only = false;
+        if ((L.law == VG_BSDBF || L.law == VG_BSDB2F || L.law == VG_BTSDBF) && L.p[4] != 0.0) h->no_bond = 0;
+    }
+    {
+        int g0 = -1;
+        h->single_group = 1;
+        for (unsigned id : sphere_ids) {
+            const int g = d->models[d->model_of[id]].group;
+            if (g0 < 0) g0 = g;
+            else if (g != g0) h->single_group = 0;
+        }
+    }
     h->dt = d->dt; h->t = d->t_current; h->t_stop = d->t_stop;
     for (int c = 0; c < 3; ++c) h->g[c] = d->g[c];
     h->sf = d->safety_factor; h->up_iact = d->up_iact ? d->up_iact : 1; h->record = d->record;
@@ -410,8 +422,9 @@ extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
     for (unsigned id : sphere_ids) rmin = std::min(rmin, d->models[d->model_of[id]].attr[0]);
     h->skin = d->inner_skin > 0 ? d->inner_skin : 0.1 * rmin;
     h->skin = std::max(h->skin, bond_reach * 1.01);
+    h->bond_reach = bond_reach;
     const char *ki_env = getenv("VG_NEIGHBOUR_CAPACITY");
-    if (ki_env) h->ki = std::max(8, std::min(127, atoi(ki_env)));
+    if (ki_env) h->ki = std::max(8, std::min(64, atoi(ki_env)));   // the step kernel keeps a 64-bit entry mask
     // walls
     h->walls_h.resize(h->nw);
     for (int w = 0; w < h->nw; ++w) {
@@ -446,17 +459,17 @@ extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
     rc |= dalloc(h, &h->walls, h->nw) | dalloc(h, &h->walls_ref, h->nw) | dalloc(h, &h->wkin, h->nw);
     rc |= dalloc(h, &h->assigned, (size_t)15 * h->nw);
     rc |= dalloc(h, &h->flags, 8) | dalloc(h, &h->relists, 1) | dalloc(h, &h->red, 8);
+    rc |= dalloc(h, &h->bpart, (size_t)8 * nblk(n, 256));
     rc |= dalloc(h, &h->inv, n) | dalloc(h, &h->perm, n) | dalloc(h, &h->svals, n);
     rc |= dalloc(h, &h->keys, n) | dalloc(h, &h->skeys, n);
     for (int b = 0; b < 2; ++b) rc |= dalloc(h, &h->xs[b], n) | dalloc(h, &h->ys[b], n) | dalloc(h, &h->zs[b], n);
     rc |= dalloc(h, &h->safe, n) | dalloc(h, &h->lin, n);
-    rc |= dalloc(h, &h->ocount, (size_t)n + 1) | dalloc(h, &h->ooff, (size_t)n + 1);
+    rc |= dalloc(h, &h->ocount, (size_t)n + 1);
     rc |= dalloc(h, &h->rec, (size_t)h->ki * 13 * n) | dalloc(h, &h->rec_imp, (size_t)6 * n);
     rc |= dalloc(h, &h->rec_imp2, (size_t)6 * n) | dalloc(h, &h->snap_wimp, (size_t)6 * std::max(h->nw, 1));
     rc |= dalloc(h, &h->d_sphere_ids, n) | dalloc(h, &h->d_wall_ids, std::max(h->nw, 1));
     if (rc) return bail(VG_ENOMEM);
-    h->onbr_cap = (size_t)n * 64;
-    if (dalloc(h, &h->onbr, h->onbr_cap)) return bail(VG_ENOMEM);
+    if (dalloc(h, &h->onbr, (size_t)h->ko * n)) return bail(VG_ENOMEM);
     // upload spheres in DO-id order
     {
         std::vector<double4> pos(n), vel(n), avl(n);
@@ -521,7 +534,7 @@ static StepArgs step_args(vg_scene *h)
     a.n = h->n; a.nw = h->nw; a.ki = h->ki;
     a.in = h->geo[h->cur]; a.out = h->geo[h->cur ^ 1];
     a.ell = h->ell[h->ecur];
-    a.o_off = h->ooff; a.o_nbr = h->onbr;
+    a.o_cnt = h->ocount; a.o_nbr = h->onbr; a.ko = h->ko;
     a.t = tables(h);
     a.walls = h->walls;
     a.flags = h->flags; a.relists = h->relists;
@@ -530,6 +543,8 @@ static StepArgs step_args(vg_scene *h)
     a.relist_half = 0.5 * h->skin * (1.0 - 1e-9);
     a.step = (int)(h->step_count % 3);
     a.rec_imp = h->rec_imp; a.rec = h->rec;
+    a.single_group = h->single_group;
+    a.no_bond = h->no_bond;
     return a;
 }
 
@@ -577,8 +592,14 @@ static int one_step(vg_scene *h, bool rec, bool assigned)
     StepArgs a = step_args(h);
     const bool kt = 2 * (h->kev_used + 1) <= (int)h->kev.size();
     if (kt) HIPCHK(hipEventRecord(h->kev[2 * h->kev_used], h->stream));
-    if (rec) hipLaunchKernelGGL(k_step<true>, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, a);
-    else hipLaunchKernelGGL(k_step<false>, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, a);
+    const dim3 grid(nblk(h->n, 256)), blk(256);
+    if (h->sd_only) {
+        if (rec) hipLaunchKernelGGL((k_step<true, true>), grid, blk, 0, h->stream, a);
+        else hipLaunchKernelGGL((k_step<false, true>), grid, blk, 0, h->stream, a);
+    } else {
+        if (rec) hipLaunchKernelGGL((k_step<true, false>), grid, blk, 0, h->stream, a);
+        else hipLaunchKernelGGL((k_step<false, false>), grid, blk, 0, h->stream, a);
+    }
     if (kt) {
         HIPCHK(hipEventRecord(h->kev[2 * h->kev_used + 1], h->stream));
         h->kev_used++;
@@ -762,9 +783,33 @@ extern "C" double vg_time(const vg_scene *h) { return h ? h->t : 0.0; }
 extern "C" int vg_num_interactions(vg_scene *h, long long *n)
 {
     if (!h || !n) return VG_EINVAL;
+    if (h->n_iact < 0) {
+        unsigned long long *d = nullptr, c[2] = {0, 0};
+        if (dalloc(h, &d, 2)) return VG_ENOMEM;
+        HIPCHK(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), h->stream));
+        hipLaunchKernelGGL(k_count_iact, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, (const unsigned *)h->ocount,
+                           (const unsigned *)h->onbr, h->n, h->ko, d);
+        HIPCHK(hipMemcpyAsync(c, d, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipFree(d));
+        h->n_iact = (long long)(c[0] / 2 + c[1]);   // sphere pairs appear in both spheres' lists
+    }
     *n = h->n_iact;
     return 0;
 }
+
+extern "C" int vg_set_skin(vg_scene *h, double skin)
+{
+    if (!h || !(skin >= 0.0)) return VG_EINVAL;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->skin = std::max(skin, h->bond_reach * 1.01);
+    // every sphere rebuilds its neighbour list with the new skin at the next step
+    int one = 1;
+    HIPCHK(hipMemcpy(h->flags + (h->step_count % 3), &one, sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" double vg_get_skin(const vg_scene *h) { return h ? h->skin : 0.0; }
 
 extern "C" int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *hi, vg_status *st)
 {
@@ -884,4 +929,37 @@ extern "C" int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, in
     if (max_fill) *max_fill = flags[4];
     if (capacity) *capacity = h->ki;
     return 0;
+}
+
+// Streaming copy of `bytes` bytes, `iters` times, on the current device: the
+// achievable HBM bandwidth (read + write bytes / time) and the calibration
+// kernel for HBM traffic counters.  No reference counterpart (diagnostics).
+extern "C" int vg_copy_bench(size_t bytes, int iters, double *gbps)
+{
+    if (bytes < 1024 || iters < 1 || !gbps) return VG_EINVAL;
+    const size_t n = bytes / sizeof(double2);
+    double2 *s = nullptr, *d = nullptr;
+    if (hipMalloc((void **)&s, n * sizeof(double2)) != hipSuccess) return VG_ENOMEM;
+    if (hipMalloc((void **)&d, n * sizeof(double2)) != hipSuccess) {
+        (void)hipFree(s);
+        return VG_ENOMEM;
+    }
+    (void)hipMemset(s, 0, n * sizeof(double2));
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const unsigned blocks = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, s, d, n);   // warm-up
+    (void)hipEventRecord(e0, 0);
+    for (int k = 0; k < iters; ++k) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, s, d, n);
+    (void)hipEventRecord(e1, 0);
+    const hipError_t rc = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *gbps = 2.0 * (double)n * sizeof(double2) * iters / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(s);
+    (void)hipFree(d);
+    return rc == hipSuccess ? 0 : VG_EHIP;
 }

@@ -11,7 +11,8 @@
 // spheres is one contiguous 256-byte run:
 //   hdr  u8  [N]          count (7 bits) | current half (bit 7)
 //   ent  u32 [2][KI][N]   partner (26 bits) | status flags (6 bits)
-//   hist f64 [2][KI][4][N] tangential force history xyz + Kn / initial velocity
+//   hist double4 [2][KI][N] tangential force history xyz + Kn / initial velocity
+//        (one 32-byte sector per entry: divergent contacts touch one sector each)
 // Every sphere keeps its own copy of each pair's ImpactStatus; the two copies
 // are updated by the same instruction sequence from the same inputs, so they
 // stay bit-identical and no thread ever reads another thread's history.
@@ -186,10 +187,11 @@ __device__ inline d3 shear_rotation(d3 f, d3 d)
     return smul(len(f), dir(d));
 }
 
-// One ImpactSolver::NextStep for slave A and master B.
+// One ImpactSolver::NextStep for slave A and master B.  SD: only the
+// spring-dashpot family is present in the scene (smaller kernel).
 // ISwBSDBF/ISwBSDB2F/ISwBtSDBF (ISwBSDBF.cpp:26-139, ISwBSDB2F.cpp:26-149,
 // ISwBtSDBF.cpp:26-137), ISwH (ISwH.cpp:30-156), ISwHertz (ISwHertz.cpp:20-94).
-template <bool REC>
+template <bool REC, bool SD>
 __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va, d3 wa, d3 xb, d3 vb,
                              d3 wb, const Contact &c, double dt, Hist &h, PairOut &o, RecOut &r)
 {
@@ -198,7 +200,7 @@ __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va
     const int law = L.law;
     const double depth = c.depth;
     const d3 n = c.n;
-    if (law == LAW_HERTZ) {
+    if (!SD && law == LAW_HERTZ) {
         o.linear_only = true;
         if (!c.active) {
             st_clean(h);
@@ -226,7 +228,7 @@ __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va
         }
         return;
     }
-    if (law == LAW_H) {
+    if (!SD && law == LAW_H) {
         const double knh = L.p[0], meff = L.p[1], ksc1 = L.p[2], ksc2 = L.p[3], betan = L.p[4], fk = L.p[5];
         if (!c.active) {
             st_clean(h);

@@ -15,7 +15,7 @@ struct Geo {   // one buffer set of sphere state (see vg_device.h)
 struct Ell {
     unsigned char *hdr;
     unsigned *ent;    // [2][KI][N]
-    double *hist;     // [2][KI][4][N]
+    double4 *hist;    // [2][KI][N]: shear force xyz, Kn / initial velocity
 };
 
 struct Tables {
@@ -28,14 +28,17 @@ struct StepArgs {
     int n, nw, ki;
     Geo in, out;
     Ell ell;
-    const unsigned *o_off;   // outer list CSR offsets [N+1]
-    const unsigned *o_nbr;
+    const unsigned *o_cnt;   // candidate-list length per sphere [N]
+    const unsigned *o_nbr;   // candidate list, ELL [KO][N]
+    int ko;
     Tables t;
     const WallDev *walls;
     int *flags;              // [0..2] relist ring, [3] error bits, [4] max fill
     unsigned long long *relists;
     double dt, gx, gy, gz, skin, relist_half;
     int step;
+    int single_group;        // all sphere models share one group: no partner group lookup
+    int no_bond;             // no bonded law: surely separated pairs skip the contact geometry
     // record path
     double *rec_imp;         // [6][N] last impulse / angular impulse per sphere (DO order-free, sorted index)
     double *rec;             // [2 halves][KI][13][N] point, dir, to_master, ang_to_master, overlap
@@ -66,9 +69,11 @@ struct OuterArgs {
     const double *safe;
     const WallDev *walls;
     Tables t;
-    unsigned *count;         // count pass output [N]
-    const unsigned *off;     // fill pass input [N+1]
-    unsigned *nbr;
+    int single_group;        // all spheres in one group
+    unsigned *count;         // candidates per sphere [N]
+    unsigned *nbr;           // ELL [KO][N]
+    int ko;
+    int *flags;              // [5]: largest candidate count seen
 };
 
 struct InnerArgs {
@@ -76,7 +81,8 @@ struct InnerArgs {
     Ell ell, eold;
     int n, nw, ki;
     const unsigned *perm;    // new index -> previous index
-    const unsigned *off, *nbr;
+    const unsigned *cnt, *nbr;
+    int ko;
     const WallDev *walls;
     double skin;
     int *flags;
@@ -88,16 +94,18 @@ struct ActiveOut {
     double rec[13];
 };
 
-template <bool REC> __global__ void k_step(StepArgs a);
+template <bool REC, bool SD> __global__ void k_step(StepArgs a);
 __global__ void k_walls(WallDev *walls, int nw, const double *assigned, const WallDev *ref, const WallKin *kin,
                         double dt, double t_new);
-__global__ void k_bounds(Geo g, int n, unsigned long long *red, unsigned *inv, double *xs, double *ys, double *zs);
+__global__ void k_bounds(Geo g, int n, unsigned long long *part, unsigned *inv, double *xs, double *ys, double *zs);
+__global__ void k_bounds_final(const unsigned long long *part, int nblocks, unsigned long long *red);
 __global__ void k_cellkey(Geo g, int n, GridDev G, const unsigned *inv, unsigned long long *keys, unsigned *vals);
 __global__ void k_reorder(Geo src, Geo dst, int n, const unsigned *perm, GridDev G, double *safe, int *lin,
                           const double *imp_src, double *imp_dst);
 __global__ void k_fill_int(int *p, long long n, int v);
 __global__ void k_cellstart(const int *lin, int n, int *cstart, int *cend);
-template <bool FILL> __global__ void k_outer(OuterArgs a);
+__global__ void k_outer(OuterArgs a);
+__global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int n, int ko, unsigned long long *out);
 __global__ void k_inner_build(InnerArgs a);
 __global__ void k_hdr_zero(unsigned char *hdr, int n);
 __global__ void k_scatter_state(Geo g, int n, const double *rec_imp, double *out);
@@ -105,5 +113,6 @@ __global__ void k_export_active(Geo g, Ell ell, int n, int ki, const WallDev *wa
                                 const unsigned *sphere_id, const double *rec, ActiveOut *out,
                                 unsigned long long cap, unsigned long long *counter);
 __global__ void k_wall_impulse(Ell ell, int n, int ki, const double *rec, double *out);
+__global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n);
 
 }  // namespace vg

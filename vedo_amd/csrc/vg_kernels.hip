@@ -28,24 +28,15 @@ __device__ inline size_t eidx(int h, int k, int ki, int n, int i)
 {
     return ((size_t)(h * ki + k)) * (size_t)n + (size_t)i;
 }
-__device__ inline size_t hidx(int h, int k, int c, int ki, int n, int i)
+__device__ inline void load_hist(const double4 *hist, int h, int k, int ki, int n, int i, Hist &hs)
 {
-    return (((size_t)(h * ki + k)) * 4 + c) * (size_t)n + (size_t)i;
+    const double4 v = hist[eidx(h, k, ki, n, i)];
+    hs.s = mk(v.x, v.y, v.z);
+    hs.aux = v.w;
 }
-
-__device__ inline void load_hist(const double *hist, int h, int k, int ki, int n, int i, Hist &hs)
+__device__ inline void store_hist(double4 *hist, int h, int k, int ki, int n, int i, const Hist &hs)
 {
-    hs.s.x = hist[hidx(h, k, 0, ki, n, i)];
-    hs.s.y = hist[hidx(h, k, 1, ki, n, i)];
-    hs.s.z = hist[hidx(h, k, 2, ki, n, i)];
-    hs.aux = hist[hidx(h, k, 3, ki, n, i)];
-}
-__device__ inline void store_hist(double *hist, int h, int k, int ki, int n, int i, const Hist &hs)
-{
-    hist[hidx(h, k, 0, ki, n, i)] = hs.s.x;
-    hist[hidx(h, k, 1, ki, n, i)] = hs.s.y;
-    hist[hidx(h, k, 2, ki, n, i)] = hs.s.z;
-    hist[hidx(h, k, 3, ki, n, i)] = hs.aux;
+    hist[eidx(h, k, ki, n, i)] = make_double4(hs.s.x, hs.s.y, hs.s.z, hs.aux);
 }
 
 // ---------------------------------------------------------------------------
@@ -57,15 +48,20 @@ __device__ inline void store_hist(double *hist, int h, int k, int ki, int n, int
 // listed (both lists are ordered subsequences of the candidate list).
 __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_old, int c_old, int h_new)
 {
-    const unsigned b = a.o_off[i], e = a.o_off[i + 1];
+    const unsigned e = a.o_cnt[i];
     int p = 0, q = 0;
     unsigned old_partner = c_old > 0 ? (a.ell.ent[eidx(h_old, 0, a.ki, a.n, i)] & PARTNER_MASK) : 0xffffffffu;
-    for (unsigned t = b; t < e; ++t) {
-        const unsigned j = a.o_nbr[t];
+    for (unsigned t = 0; t < e; ++t) {
+        const unsigned j = a.o_nbr[(size_t)t * a.n + i];
         Contact c;
+        bool near = true;
         if (j < (unsigned)a.n) {
             const double4 pj = a.in.pos[j];
-            cd_ss(xi, ri, ld3(pj), pj.w, c);
+            const d3 dx = ld3(pj) - xi;
+            const double lim = ri + pj.w + a.skin;
+            // far beyond the skin: excluded without the exact geometry
+            if (dot(dx, dx) > lim * lim * (1.0 + 1e-9)) near = false;
+            else cd_ss(xi, ri, ld3(pj), pj.w, c);
         } else {
             cd_wall(xi, ri, a.walls[j - a.n], c);
         }
@@ -74,11 +70,11 @@ __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_o
         Hist hs;
         if (carried) {
             f = a.ell.ent[eidx(h_old, p, a.ki, a.n, i)] & ~PARTNER_MASK;
-            load_hist(a.ell.hist, h_old, p, a.ki, a.n, i, hs);
+            if (f & F_CONTACT) load_hist(a.ell.hist, h_old, p, a.ki, a.n, i, hs);
             ++p;
             old_partner = p < c_old ? (a.ell.ent[eidx(h_old, p, a.ki, a.n, i)] & PARTNER_MASK) : 0xffffffffu;
         }
-        if (c.depth >= -a.skin) {
+        if (near && c.depth >= -a.skin) {
             if (q >= a.ki) {
                 atomicOr(&a.flags[3], ERR_INNER_CAP);
                 break;
@@ -92,17 +88,32 @@ __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_o
                 hs.aux = 0.0;
             }
             a.ell.ent[eidx(h_new, q, a.ki, a.n, i)] = j | f;
-            store_hist(a.ell.hist, h_new, q, a.ki, a.n, i, hs);
+            if (f & F_CONTACT) store_hist(a.ell.hist, h_new, q, a.ki, a.n, i, hs);
             ++q;
         }
     }
     return q;
 }
 
-template <bool REC>
+// History words are meaningful only while the entry's Contact flag is set:
+// every other status of the five solvers has a zero shear force and a zero
+// Kn / initial velocity (Clean, CleanContactStatusAndImpact, ImpactStatus()).
+// So the step reads them only for contacts and writes them only when the
+// solve leaves the pair in contact.
+// Workgroups are dealt to the 8 XCDs round-robin; remap them so that each XCD
+// walks one contiguous eighth of the (spatially sorted) spheres and its L2
+// holds the neighbours its waves gather.
+__device__ inline int xcd_block(int b, int nb)
+{
+    const int per = (nb + 7) / 8, x = b % 8, q = b / 8;
+    const int full = nb - 8 * (per - 1);   // XCDs 0..full-1 get `per` blocks, the rest per-1
+    return x < full ? x * per + q : full * per + (x - full) * (per - 1) + q;
+}
+
+template <bool REC, bool SD>
 __global__ __launch_bounds__(256) void k_step(StepArgs a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i == 0) {
         a.flags[(a.step + 2) % 3] = 0;
         if (a.flags[a.step % 3]) atomicAdd(a.relists, 1ull);
@@ -115,7 +126,8 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
     const double4 w4 = a.in.avl[i];
     const d3 xi = ld3(p4), vi = ld3(v4), wi = ld3(w4);
     const double ri = p4.w, mi = w4.w;
-    const int gi = a.t.models[a.in.mdl[i]].group;
+    const int mdl_i = a.in.mdl[i];
+    const int gi = a.t.models[mdl_i].group;
 
     unsigned hdr = a.ell.hdr[i];
     int h = hdr >> 7, cnt = hdr & 127;
@@ -127,14 +139,44 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
         v4.w = 0.0;
     }
 
-    d3 I = mk(0, 0, 0), L = mk(0, 0, 0);
+    // Pass 1: surely separated sphere pairs are settled here (Clean status,
+    // no impulse); every other entry is marked for the full solve.  Pass 2
+    // runs the solves in list order, so a wavefront executes the contact
+    // physics as many times as its busiest lane has near contacts, not as
+    // many times as its longest list has entries.
+    unsigned long long todo = 0;
     for (int k = 0; k < cnt; ++k) {
+        const size_t ei = eidx(h, k, a.ki, a.n, i);
+        const unsigned ent = a.ell.ent[ei];
+        const unsigned j = ent & PARTNER_MASK;
+        bool full = true;
+        if (a.no_bond && j < (unsigned)a.n) {
+            const double4 pj = a.in.pos[j];
+            const d3 dx = ld3(pj) - xi;
+            const double rs = ri + pj.w;
+            if (dot(dx, dx) > rs * rs * (1.0 + 1e-9)) {
+                full = false;
+                // Backup, then Clean() of an inactive pair (ImpactStatus.cpp:259-274)
+                const unsigned f = ent & ~PARTNER_MASK & ~F_STATUS & ~F_NOBACKUP;
+                if (f != (ent & ~PARTNER_MASK)) a.ell.ent[ei] = j | f;
+                if (REC) {
+                    double *rp = a.rec + ((size_t)k * 13) * a.n + i;
+                    for (int q = 0; q < 13; ++q) rp[(size_t)q * a.n] = 0.0;
+                }
+            }
+        }
+        if (full) todo |= 1ull << k;
+    }
+    d3 I = mk(0, 0, 0), L = mk(0, 0, 0);
+    while (todo) {
+        const int k = __builtin_ctzll(todo);
+        todo &= todo - 1;
         const size_t ei = eidx(h, k, a.ki, a.n, i);
         const unsigned ent = a.ell.ent[ei];
         const unsigned j = ent & PARTNER_MASK;
         Hist hs;
         hs.f = ent & ~PARTNER_MASK;
-        load_hist(a.ell.hist, h, k, a.ki, a.n, i, hs);
+        const bool had_contact = (hs.f & F_CONTACT) != 0;
         // IactContainer::BackupImpactStatus of the previous step (IactContainer.cpp:128-131)
         if (!(hs.f & F_NOBACKUP)) {
             unsigned f = hs.f & ~(F_LASTCONTACT | F_LASTBOND);
@@ -143,7 +185,10 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             hs.f = f;
         }
         hs.f &= ~F_NOBACKUP;
+        if (had_contact) load_hist(a.ell.hist, h, k, a.ki, a.n, i, hs);
+        else { hs.s = mk(0, 0, 0); hs.aux = 0.0; }
         PairOut o;
+        o.added = false;
         RecOut r;
         if (REC) {
             r.point = r.dir = r.to_master = r.ang_to_master = mk(0, 0, 0);
@@ -152,18 +197,18 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
         Contact c;
         if (j < (unsigned)a.n) {
             const double4 pj = a.in.pos[j], vj = a.in.vel[j], wj = a.in.avl[j];
-            const int gj = a.t.models[a.in.mdl[j]].group;
+            const int gj = a.single_group ? gi : a.t.models[a.in.mdl[j]].group;
             const LawDev &law = a.t.laws[gi * a.t.n_groups + gj];
             if (hs.f & F_SLAVE) {
                 cd_ss(xi, ri, ld3(pj), pj.w, c);
-                solve<REC>(law, mi, wj.w, xi, vi, wi, ld3(pj), ld3(vj), ld3(wj), c, a.dt, hs, o, r);
+                solve<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(pj), ld3(vj), ld3(wj), c, a.dt, hs, o, r);
                 if (o.added) {
                     I = I + o.imp;
                     if (!o.linear_only) L = L + o.ang_a;
                 }
             } else {
                 cd_ss(ld3(pj), pj.w, xi, ri, c);
-                solve<REC>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), xi, vi, wi, c, a.dt, hs, o, r);
+                solve<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), xi, vi, wi, c, a.dt, hs, o, r);
                 if (o.added) {
                     I = I + (o.linear_only ? -o.imp : smul(-1.0, o.imp));
                     if (!o.linear_only) L = L + o.ang_b;
@@ -173,14 +218,14 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             const WallDev &w = a.walls[j - a.n];
             const LawDev &law = a.t.laws[gi * a.t.n_groups + w.group];
             cd_wall(xi, ri, w, c);
-            solve<REC>(law, mi, 0.0, xi, vi, wi, w.pos, w.vel, w.av, c, a.dt, hs, o, r);
+            solve<REC, SD>(law, mi, 0.0, xi, vi, wi, w.pos, w.vel, w.av, c, a.dt, hs, o, r);
             if (o.added) {
                 I = I + o.imp;
                 if (!o.linear_only) L = L + o.ang_a;
             }
         }
-        a.ell.ent[ei] = j | hs.f;
-        store_hist(a.ell.hist, h, k, a.ki, a.n, i, hs);
+        if (hs.f != (ent & ~PARTNER_MASK)) a.ell.ent[ei] = j | hs.f;
+        if (hs.f & F_CONTACT) store_hist(a.ell.hist, h, k, a.ki, a.n, i, hs);
         if (REC) {
             double *rp = a.rec + ((size_t)k * 13) * a.n + i;
             const double vals[13] = {r.point.x, r.point.y, r.point.z, r.dir.x, r.dir.y, r.dir.z,
@@ -192,7 +237,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
     // DOContainer::AddFieldImpact + AddConstrainedImpact (DOContainer.cpp:20-50)
     const d3 gdt = muls(mk(a.gx, a.gy, a.gz), a.dt);
     I = I + muls(gdt, mi);
-    const ModelDev &md = a.t.models[a.in.mdl[i]];
+    const ModelDev &md = a.t.models[mdl_i];
     I = I + smul(a.dt, mk(md.fext[0], md.fext[1], md.fext[2]));
     // DOSphere::Response (DOSphere.cpp:21-48)
     const d3 dv = smul(1.0 / mi, I);
@@ -200,9 +245,9 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
     const d3 dp = smul(a.dt, vi + smul(0.5, dv));
     const d3 dw = smul(a.dt, wi + smul(0.5, dav));
     const d3 xn = xi + dp;
-    const d3 ox = mk(a.in.ori[i], a.in.ori[a.n + i], a.in.ori[2 * a.n + i]);
-    const d3 oz = mk(a.in.ori[3 * a.n + i], a.in.ori[4 * a.n + i], a.in.ori[5 * a.n + i]);
-    const d3 oxn = rotate_around(ox, dw), ozn = rotate_around(oz, dw);
+    d3 oxn = mk(a.in.ori[i], a.in.ori[a.n + i], a.in.ori[2 * a.n + i]);
+    d3 ozn = mk(a.in.ori[3 * a.n + i], a.in.ori[4 * a.n + i], a.in.ori[5 * a.n + i]);
+    rotate_pair(oxn, ozn, dw);
     const d3 vn = vi + dv, wn = wi + dav;
     const double path = v4.w + len(dp);
     if (path > a.relist_half) atomicOr(&a.flags[(a.step + 1) % 3], 1);
@@ -212,15 +257,17 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
     a.out.ori[i] = oxn.x; a.out.ori[a.n + i] = oxn.y; a.out.ori[2 * a.n + i] = oxn.z;
     a.out.ori[3 * a.n + i] = ozn.x; a.out.ori[4 * a.n + i] = ozn.y; a.out.ori[5 * a.n + i] = ozn.z;
     a.out.oid[i] = a.in.oid[i];
-    a.out.mdl[i] = a.in.mdl[i];
+    a.out.mdl[i] = mdl_i;
     if (REC) {
         a.rec_imp[i] = I.x; a.rec_imp[a.n + i] = I.y; a.rec_imp[2 * a.n + i] = I.z;
         a.rec_imp[3 * a.n + i] = L.x; a.rec_imp[4 * a.n + i] = L.y; a.rec_imp[5 * a.n + i] = L.z;
     }
 }
 
-template __global__ void k_step<false>(StepArgs);
-template __global__ void k_step<true>(StepArgs);
+template __global__ void k_step<false, false>(StepArgs);
+template __global__ void k_step<true, false>(StepArgs);
+template __global__ void k_step<false, true>(StepArgs);
+template __global__ void k_step<true, true>(StepArgs);
 
 // Wall responses after the spheres' (DOFixedQuasiPlate.h:16-19,
 // DOOrbitalQuasiPlate.cpp:18-34) and kinematic prescriptions
@@ -268,8 +315,9 @@ __device__ inline unsigned long long ord64(double d)
 }
 
 // Bounds of the local DOs (LeapConsultant.cpp:119-164): min/max position,
-// max |v|, max range; and the oid -> current-index map.
-__global__ void k_bounds(Geo g, int n, unsigned long long *red, unsigned *inv, double *xs, double *ys, double *zs)
+// max |v|, max range; and the rank -> current-index map.  Stage 1 writes one
+// partial per block, stage 2 (k_bounds_final) reduces them.
+__global__ void k_bounds(Geo g, int n, unsigned long long *part, unsigned *inv, double *xs, double *ys, double *zs)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ unsigned long long sm[8][256];
@@ -292,8 +340,23 @@ __global__ void k_bounds(Geo g, int n, unsigned long long *red, unsigned *inv, d
             for (int k = 0; k < 8; ++k) sm[k][threadIdx.x] = max(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 8; ++k) atomicMax(&red[k], sm[k][0]);
+    if (threadIdx.x < 8) part[(size_t)blockIdx.x * 8 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ void k_bounds_final(const unsigned long long *part, int nblocks, unsigned long long *red)
+{
+    __shared__ unsigned long long sm[8][256];
+    unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int k = 0; k < 8; ++k) v[k] = max(v[k], part[(size_t)b * 8 + k]);
+    for (int k = 0; k < 8; ++k) sm[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 8; ++k) sm[k][threadIdx.x] = max(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) red[threadIdx.x] = sm[threadIdx.x][0];
 }
 
 __device__ inline void cell_of(const GridDev &G, d3 p, unsigned &cx, unsigned &cy, unsigned &cz)
@@ -313,8 +376,21 @@ __device__ inline long long lin_of(const GridDev &G, long long cx, long long cy,
     return ((cz + 1) * G.ny + (cy + 1)) * G.nx + (cx + 1);
 }
 
-// Sort keys in DO-id order: the stable sort then keeps each cell's DOs in id
-// order, as the reference's per-cell vectors do (LeapConsultant.h:170-188).
+// Memory order of the spheres: Morton (Z-order) of the cell, so that the
+// cells around a sphere sit close in memory; each sphere's list order is
+// independent of it (k_outer orders by the comparator key).  Keys are
+// produced in DO-id order and the sort is stable, so each cell's spheres stay
+// in id order as the reference's per-cell vectors do (LeapConsultant.h:170-188).
+__device__ inline unsigned long long spread3(unsigned v)
+{
+    unsigned long long x = v & 0x1fffffull;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
 __global__ void k_cellkey(Geo g, int n, GridDev G, const unsigned *inv, unsigned long long *keys, unsigned *vals)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -323,7 +399,7 @@ __global__ void k_cellkey(Geo g, int n, GridDev G, const unsigned *inv, unsigned
     const double4 p = g.pos[i];
     unsigned cx, cy, cz;
     cell_of(G, mk(p.x, p.y, p.z), cx, cy, cz);
-    keys[t] = cmp_key(G, cx, cy, cz);
+    keys[t] = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
     vals[t] = i;
 }
 
@@ -368,10 +444,21 @@ __global__ void k_cellstart(const int *lin, int n, int *cstart, int *cend)
 
 __device__ inline bool accept_sphere(const OuterArgs &a, int s, d3 xs, double safe_s, int gs, int j)
 {
-    const int gj = a.t.models[a.g.mdl[j]].group;
-    if (a.t.laws[gs * a.t.n_groups + gj].law == LAW_NONE) return false;
+    if (!a.single_group) {
+        const int gj = a.t.models[a.g.mdl[j]].group;
+        if (a.t.laws[gs * a.t.n_groups + gj].law == LAW_NONE) return false;
+    } else if (a.t.laws[gs * a.t.n_groups + gs].law == LAW_NONE) {
+        return false;
+    }
     const double4 pj = a.g.pos[j];
-    return !(len(xs - mk(pj.x, pj.y, pj.z)) > safe_s + a.safe[j]);
+    const d3 d = xs - mk(pj.x, pj.y, pj.z);
+    const double lim = safe_s + a.safe[j];
+    // the squared distance decides every pair away from the threshold; near it
+    // the reference's own test `len(d) > safe_i + safe_j` decides
+    const double d2 = dot(d, d), l2 = lim * lim;
+    if (d2 > l2 * (1.0 + 1e-9)) return false;
+    if (d2 < l2 * (1.0 - 1e-9)) return true;
+    return !(len(d) > lim);
 }
 
 __device__ inline bool accept_wall(const OuterArgs &a, d3 xs, double safe_s, int gs, int w)
@@ -386,8 +473,7 @@ __device__ inline bool accept_wall(const OuterArgs &a, d3 xs, double safe_s, int
 // groups "current cell = c" of the Leap traversal are visited in comparator
 // order; within the own cell the 13 half-shell cells, the self cell and the
 // global DOs follow in reference order (LeapConsultant.h:207-444).
-template <bool FILL>
-__global__ void k_outer(OuterArgs a)
+__global__ __launch_bounds__(128) void k_outer(OuterArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.n) return;
@@ -425,9 +511,8 @@ __global__ void k_outer(OuterArgs a)
         gslot[v + 1] = ks;
     }
     unsigned cnt = 0;
-    unsigned base = FILL ? a.off[s] : 0;
     auto emit = [&](unsigned j) {
-        if (FILL) a.nbr[base + cnt] = j;
+        if (cnt < (unsigned)a.ko) a.nbr[(size_t)cnt * a.n + s] = j;
         ++cnt;
     };
     auto scan_cell = [&](long long x, long long y, long long z, bool self) {
@@ -451,11 +536,23 @@ __global__ void k_outer(OuterArgs a)
             scan_cell((long long)cx - NB13[k][0], (long long)cy - NB13[k][1], (long long)cz - NB13[k][2], false);
         }
     }
-    if (!FILL) a.count[s] = cnt;
+    a.count[s] = cnt;
+    if (cnt > (unsigned)a.ko) atomicMax(&a.flags[5], (int)cnt);
 }
 
-template __global__ void k_outer<false>(OuterArgs);
-template __global__ void k_outer<true>(OuterArgs);
+// Consultant::GetIactNum: sphere pairs are listed from both sides, walls once.
+__global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int n, int ko, unsigned long long *out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    unsigned sph = 0, wal = 0;
+    for (unsigned t = 0; t < cnt[s] && t < (unsigned)ko; ++t) {
+        if (nbr[(size_t)t * n + s] < (unsigned)n) ++sph;
+        else ++wal;
+    }
+    atomicAdd(&out[0], (unsigned long long)sph);
+    atomicAdd(&out[1], (unsigned long long)wal);
+}
 
 // Neighbour lists at a rebuild, with IactRecordTab's history carry: a pair
 // keeps its ImpactStatus if it was listed and is active now
@@ -477,13 +574,16 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
     const d3 xs = mk(ps.x, ps.y, ps.z);
     const unsigned my_oid = a.g.oid[s];
     int q = 0;
-    for (unsigned t = a.off[s]; t < a.off[s + 1]; ++t) {
-        const unsigned j = a.nbr[t];
+    for (unsigned t = 0; t < a.cnt[s]; ++t) {
+        const unsigned j = a.nbr[(size_t)t * a.n + s];
         Contact c;
         unsigned key;
         bool slave = true;
         if (j < (unsigned)a.n) {
             const double4 pj = a.g.pos[j];
+            const d3 dx = mk(pj.x, pj.y, pj.z) - xs;
+            const double lim = ps.w + pj.w + a.skin;
+            if (dot(dx, dx) > lim * lim * (1.0 + 1e-9)) continue;   // surely beyond the skin
             key = a.g.oid[j];
             slave = my_oid < key;
             if (slave) cd_ss(xs, ps.w, mk(pj.x, pj.y, pj.z), pj.w, c);
@@ -505,13 +605,13 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
             for (int k = 0; k < c_old; ++k)
                 if (old_oid[k * blockDim.x + threadIdx.x] == key) {
                     f = (a.eold.ent[eidx(h_old, k, a.ki, a.n, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
-                    load_hist(a.eold.hist, h_old, k, a.ki, a.n, p, hs);
+                    if (f & F_CONTACT) load_hist(a.eold.hist, h_old, k, a.ki, a.n, p, hs);
                     break;
                 }
         }
         if (slave) f |= F_SLAVE;
         a.ell.ent[eidx(0, q, a.ki, a.n, s)] = j | f;
-        store_hist(a.ell.hist, 0, q, a.ki, a.n, s, hs);
+        if (f & F_CONTACT) store_hist(a.ell.hist, 0, q, a.ki, a.n, s, hs);
         ++q;
     }
     a.ell.hdr[s] = (unsigned char)q;
@@ -577,7 +677,9 @@ __global__ void k_export_active(Geo g, Ell ell, int n, int ki, const WallDev *wa
         A.hi = me < other ? other : me;
         A.flags = ent & ~PARTNER_MASK;
         Hist hs;
-        load_hist(ell.hist, h, k, ki, n, i, hs);
+        hs.s = mk(0, 0, 0);
+        hs.aux = 0.0;
+        if (ent & F_CONTACT) load_hist(ell.hist, h, k, ki, n, i, hs);
         A.hist[0] = hs.s.x; A.hist[1] = hs.s.y; A.hist[2] = hs.s.z; A.hist[3] = hs.aux;
         for (int q = 0; q < 13; ++q) A.rec[q] = rec ? rec[((size_t)k * 13 + q) * n + i] : 0.0;
     }
@@ -609,6 +711,14 @@ __global__ void k_wall_impulse(Ell ell, int n, int ki, const double *rec, double
     }
     if (threadIdx.x == 0)
         for (int c = 0; c < 6; ++c) out[w * 6 + c] = sm[c][0];
+}
+
+// Plain streaming copy used to calibrate HBM counters and measure achievable
+// bandwidth.
+__global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 }  // namespace vg

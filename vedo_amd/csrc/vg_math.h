@@ -54,6 +54,23 @@ __device__ inline d3 rotate_around(d3 p, d3 va)
     return ((t1 + t2) + t3) + t4;
 }
 
+// rotate_around applied to two vectors with the same rotation vector: the
+// shared terms (direction, half-angle cos/sin) are identical in both calls
+// of the reference (DOSphere.cpp:41), so computing them once is exact.
+__device__ inline void rotate_pair(d3 &p1, d3 &p2, d3 va)
+{
+    d3 u = dir(va);
+    double a = len(va);
+    double s = cos(0.5 * a);
+    double t = sin(0.5 * a);
+    d3 v = muls(u, t);
+    const double ss = s * s, s2 = 2.0 * s;
+    d3 t1 = smul(ss, p1), t2 = smul(dot(p1, v), v), t3 = smul(s2, cross(v, p1)), t4 = cross(v, cross(v, p1));
+    p1 = ((t1 + t2) + t3) + t4;
+    t1 = smul(ss, p2); t2 = smul(dot(p2, v), v); t3 = smul(s2, cross(v, p2)); t4 = cross(v, cross(v, p2));
+    p2 = ((t1 + t2) + t3) + t4;
+}
+
 __device__ inline d3 ld3(const double4 &v) { return mk(v.x, v.y, v.z); }
 
 }  // namespace vg

@@ -58,7 +58,7 @@ assert STATUS_DTYPE.itemsize == C.sizeof(VgStatus)
 
 EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_set_history", "vg_step",
            "vg_sync", "vg_step_assigned", "vg_set_wall_rotation", "vg_get_state", "vg_time",
-           "vg_num_interactions", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats")
+           "vg_num_interactions", "vg_set_skin", "vg_get_skin", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench")
 
 _lib = None
 
@@ -92,10 +92,23 @@ def lib():
         L.vg_get_active.argtypes = [vp, C.POINTER(C.c_longlong), _U, _U, C.POINTER(VgStatus)]
         L.vg_step_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.vg_kernel_timing.argtypes = [vp, C.c_int]
+        L.vg_set_skin.argtypes = [vp, C.c_double]
+        L.vg_get_skin.restype = C.c_double
+        L.vg_get_skin.argtypes = [vp]
         L.vg_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_int),
                                C.POINTER(C.c_int)]
+        L.vg_copy_bench.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 10) -> float:
+    """Achieved read+write GB/s of a plain streaming copy kernel."""
+    gbps = C.c_double()
+    rc = lib().vg_copy_bench(nbytes, iters, C.byref(gbps))
+    if rc:
+        raise GpuError(f"vg_copy_bench failed ({rc})")
+    return gbps.value
 
 
 def _dp(a):
@@ -208,6 +221,13 @@ class GpuSimulation:
         self._check(self.L.vg_get_active(self.h, C.byref(n), _up(lo), _up(hi),
                                          st.ctypes.data_as(C.POINTER(VgStatus))))
         return lo[:n.value], hi[:n.value], st[:n.value]
+
+    def set_skin(self, skin: float):
+        self._check(self.L.vg_set_skin(self.h, float(skin)))
+
+    @property
+    def skin(self) -> float:
+        return self.L.vg_get_skin(self.h)
 
     def kernel_timing(self, max_launches: int):
         self._check(self.L.vg_kernel_timing(self.h, int(max_launches)))

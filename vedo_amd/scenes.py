@@ -175,7 +175,7 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
                rmax: float = 2e-3, classes: int = 64, density: float = 2500.0, R: float = 0.15,
                L: Optional[float] = None, omega: float = 2.0, kn: float = 1e4, kt_ratio: float = 0.8,
                e: float = 0.6, mu: float = 0.4, dt: float = 5e-6, steps: int = 10000,
-               sides: int = 64, thick: float = 2e-3, gap: float = 1.01, arrays: bool = False,
+               sides: int = 64, thick: float = 2e-3, gap: float = 0.9999, arrays: bool = False,
                up_iact: int = 20, record: int = 10 ** 9):
     """Polydisperse spheres in a rotating drum of quasi-plates.
 
@@ -183,9 +183,10 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
     to ``classes`` log-spaced classes, one DOModel each in one group.  The drum
     axis is y; its mantle is a ``sides``-gon of quasi-plates plus two end
     plates, all prescribed to rotate rigidly at ``omega`` (kinematic walls).
-    Spheres start on a dense layered lattice in the lower part of the drum:
-    each z-layer uses the spacing of its largest sphere times ``gap``.  When
-    ``L`` is None the drum is made just long enough for that lattice.
+    Spheres start on a layered cubic lattice in the lower part of the drum:
+    each z-layer holds one radius class at spacing 2*gap*r (gap < 1: touching
+    neighbours start in slight contact, a pre-settled bed).  When ``L`` is
+    None the drum is made just long enough for that lattice.
 
     Returns (world, wall_ids, axis_point, axis_dir); with ``arrays=True`` the
     first item is a WorldArrays built without per-sphere Python objects (for
@@ -200,65 +201,63 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
     order = np.argsort(ci, kind="stable")
     ci_sorted = ci[order]
     r_sorted = cls[ci_sorted]
-    # Layered fill of the disc (x,z) times the drum length along y, bottom up.
-    margin = thick * 0.5 + 0.25 * rmax
+    # Fill of the lower half of the drum, one slab per radius class along the
+    # axis (y): each y-layer of a slab is a triangular lattice of equal spheres
+    # in the cross-section (spacing 2*gap*r, gap slightly below 1: neighbours
+    # start just touching), y-layers aligned, the last one filled bottom-up.
+    margin = thick * 0.5 + 1e-3 * rmin
     rin = R - margin
-    if L is None:
-        vol = float(np.sum((2.0 * gap * r_sorted) ** 3))
-        L_try = max(4.0 * rmax, vol / (0.5 * math.pi * rin * rin) * 1.2 + 2.0 * margin)
-    else:
-        L_try = L
 
-    def layer_capacity(z, s, Ld):
-        zz = max(abs(z), abs(z + s))
-        if zz >= rin:
-            return 0, 0
-        half = math.sqrt(rin * rin - zz * zz)
-        nx = int(2.0 * half / s)
-        ny = int((Ld - 2.0 * margin) / s)
-        return max(nx, 0), max(ny, 0)
-
-    while True:
-        pos = np.zeros((n, 3))
-        placed = 0
-        z = -rin
-        ok = True
-        while placed < n:
-            s = 2.0 * gap * r_sorted[placed]
-            nx, ny = layer_capacity(z, s, L_try)
-            take = min(nx * ny, n - placed)
-            while take > 0:   # shrink the layer until its largest sphere fits the spacing
-                s = 2.0 * gap * r_sorted[placed + take - 1]
-                nx, ny = layer_capacity(z, s, L_try)
-                if nx * ny >= take:
-                    break
-                take = nx * ny
-            if take <= 0:
-                z += 0.25 * s
-                if z > 0.6 * rin:
-                    ok = False
-                    break
-                continue
-            k = np.arange(take)
-            gx, gy = k % nx, k // nx
-            pos[placed:placed + take, 0] = -(nx - 1) * s * 0.5 + gx * s
-            pos[placed:placed + take, 1] = margin + s * 0.5 + gy * s
-            pos[placed:placed + take, 2] = z + 0.5 * s
-            placed += take
-            z += s
-            if placed < n and z > 0.6 * rin:
-                ok = False
+    def section(r, sp):
+        # triangular lattice in the cross-section: rows at pitch sp*sqrt(3)/2
+        # from the lowest point of the drum, alternate rows offset by sp/2, so
+        # every sphere rests in a pocket of the row below
+        lim = rin - r
+        xs, zs = [], []
+        m = 0
+        while True:
+            z = -lim + m * sp * math.sqrt(3.0) / 2.0
+            if z > 0.0:
                 break
-        if ok:
-            break
-        if L is not None:
-            raise ValueError("drum too small for the requested sphere count")
-        L_try *= 1.1
-    Ld = L_try
+            half = math.sqrt(max(0.0, lim * lim - z * z))
+            ph = 0.5 * (m % 2)
+            k = np.arange(-int(half / sp) - 1, int(half / sp) + 2)
+            x = (k + ph) * sp
+            x = x[np.abs(x) <= half]
+            xs.append(x)
+            zs.append(np.full(len(x), z))
+            m += 1
+        X, Z = np.concatenate(xs), np.concatenate(zs)
+        o = np.lexsort((X, Z))
+        return X[o], Z[o]
+
+    pos = np.zeros((n, 3))
+    y = margin
+    placed = 0
+    for c in np.unique(ci_sorted):
+        m = int(np.sum(ci_sorted == c))
+        r = float(cls[c])
+        sp = 2.0 * gap * r
+        X, Z = section(r, sp)
+        cap = len(X)
+        for start_k in range(0, m, cap):
+            take = min(cap, m - start_k)
+            sl = slice(placed, placed + take)
+            pos[sl, 0] = X[:take]
+            pos[sl, 1] = y + r
+            pos[sl, 2] = Z[:take]
+            placed += take
+            y += sp
+        y += 1e-3 * rmin
+    Ld = y + margin
+    if L is not None:
+        if Ld > L:
+            raise ValueError("drum too short for the requested sphere count")
+        Ld = L
     # undo the class sort: sphere i keeps its own radius
     pos_final = np.empty_like(pos)
     pos_final[order] = pos
-    jitter = rng.uniform(3 * n, -0.005, 0.005).reshape(n, 3) * radii[:, None]
+    jitter = rng.uniform(3 * n, -1e-6, 1e-6).reshape(n, 3) * radii[:, None]
     pos_final += jitter
     models = []
     for k, r in enumerate(cls):

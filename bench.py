@@ -6,8 +6,9 @@ of 64 quasi-plates + 2 end plates (R = 0.15 m) rotating at 2 rad/s about a
 horizontal axis; ISwBSDBF linear spring-dashpot kn = 1e4 N/m, ks = 0.8 kn,
 e = 0.6, mu = 0.4; dt = 5e-6 s; LeapConsultant candidate lists rebuilt every
 UpIact steps; FP64 throughout.  A "step" is one SimMediator::Run over all
-spheres.  The bed first settles for --prep untimed steps in the resting drum,
-then the drum starts turning at 2 rad/s and the timed steps follow.
+spheres.  The bed first settles untimed in the resting drum (at most --prep
+steps, quenched whenever its kinetic energy peaks, see settle()), then the
+drum starts turning at 2 rad/s and the timed steps follow.
 
 Prints one JSON line (see the metric/roofline/cpu_baseline fields).
 """
@@ -72,13 +73,44 @@ def cpu_baseline(state, wa, n_gpu, seconds):
                       f"walls held fixed, {steps} steps in {el:.1f} s"}
 
 
+def settle(sim, n, total_mass, max_steps, chunk=100, v_rms_stop=2e-3):
+    """Untimed preparation: the bed relaxes in the resting drum.  Each time
+    the spheres' kinetic energy has passed a maximum, all sphere velocities
+    are reset to zero through SimMediator::Run(map) (a quench): that removes
+    the slow elastic "breathing" of the deep bed, which contact damping alone
+    damps only weakly.  Stops when the rms speed is below v_rms_stop."""
+    ids = np.arange(n, dtype=np.uint32)
+    zeros = np.zeros((n, 3))
+    last, rising, done, quenches = None, False, 0, 0
+    while done < max_steps:
+        sim.step(chunk)
+        done += chunk
+        e = sim.system_energy()
+        ke = e["translation"] + e["rotation"]
+        v_rms = (2.0 * e["translation"] / total_mass) ** 0.5
+        log(f"[bench] settle {done} steps: v_rms {v_rms:.2e} m/s, v_max {e['velocity_max']:.2e} m/s, "
+            f"active contacts {sim.count_active()}, quenches {quenches}")
+        if v_rms < v_rms_stop:
+            break
+        if last is not None and rising and ke < last:
+            st = sim.state()
+            sim.step_assigned(ids, st["pos"][:n], zeros, st["ox"][:n], st["oz"][:n], zeros)
+            done += 1
+            quenches += 1
+            last, rising = None, False
+            continue
+        rising = last is None or ke > last
+        last = ke
+    return done, quenches
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--prep", type=int, default=300)
+    ap.add_argument("--prep", type=int, default=6000, help="maximum untimed settling steps")
     ap.add_argument("--up-iact", type=int, default=100)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -105,13 +137,10 @@ def main():
     sim.set_wall_rotation(wall_ids, c, a, 0.0, 0.0)
     log(f"[bench] created on GPU ({time.perf_counter() - t:.1f} s)")
     t = time.perf_counter()
-    done = 0
-    while done < args.prep:
-        k = min(500, args.prep - done)
-        sim.step(k)
-        done += k
-        log(f"[bench] prep {done}/{args.prep} steps, active contacts {sim.count_active()}")
-    log(f"[bench] prep {time.perf_counter() - t:.1f} s")
+    r_sph = wa.model_attr[wa.model_of[:args.n], 0]
+    total_mass = float(np.sum(wa.model_density[wa.model_of[:args.n]] * 4.0 / 3.0 * np.pi * r_sph ** 3))
+    done, quenches = settle(sim, args.n, total_mass, args.prep)
+    log(f"[bench] prep {done} steps, {quenches} quenches, {time.perf_counter() - t:.1f} s")
     sim.set_wall_rotation(wall_ids, c, a, 2.0, sim.time)
     skin = sim.skin
     tune = None
@@ -146,7 +175,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "rotating-drum polydisperse spheres, ISwBSDBF, LeapConsultant", "spheres": n,
                    "walls": len(wall_ids), "radius_classes": 64, "dt": wa.dt, "up_iact": args.up_iact,
-                   "prep_steps": args.prep, "active_contacts": c_act, "neighbour_skin_m": skin,
+                   "prep_steps": done, "prep_quenches": quenches, "active_contacts": c_act, "neighbour_skin_m": skin,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

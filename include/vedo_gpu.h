@@ -124,9 +124,11 @@ int vg_set_history(vg_scene *h, int n, const unsigned *lo, const unsigned *hi, c
 int vg_step(vg_scene *h, int nsteps);
 int vg_sync(vg_scene *h);
 
-/* One step with prescribed statuses for kinematic DOs (walls), the
- * equivalent of SimMediator::Run(const std::map<_uint_t,DOStatus>&)
- * (SimMediator.cpp:1121-1163).  Arrays hold 3 doubles per listed DO. */
+/* One step with assigned statuses, the equivalent of
+ * SimMediator::Run(const std::map<_uint_t,DOStatus>&) (SimMediator.cpp:
+ * 1121-1163, 843-889): after Response the listed DOs (spheres or walls) take
+ * the given status, with zero stored impulses, before contact detection and
+ * the rebuild.  Arrays hold 3 doubles per listed DO.  Synchronous. */
 int vg_step_assigned(vg_scene *h, int n, const unsigned *ids, const double *pos, const double *vel,
                      const double *ox, const double *oz, const double *av);
 
@@ -166,6 +168,14 @@ int vg_kernel_timing(vg_scene *h, int max_launches);
 int vg_step_timing(vg_scene *h, double *ms, int *launches);
 /* Counters: rebuilds, relists, max neighbour-list fill. */
 int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, int *max_fill, int *capacity);
+
+/* DOWorld::CalculateSystemEnergy (DOWorld.cpp:481-632) over the mobile DOs,
+ * computed on the device from the current state.  out[16]: potential,
+ * translational and rotational energy, average momentum (3), average
+ * "angular momentum" as the reference forms it, mmi.cross(w) (3), momentum
+ * norm, angular momentum norm, max |v|, min |v|, max |w|, min |w|, number of
+ * mobile DOs.  Sums are reduced in a fixed tree order. */
+int vg_system_energy(vg_scene *h, double *out);
 
 /* Streaming copy of `bytes` bytes, `iters` times, on the current device;
  * returns the achieved read+write bandwidth in GB/s.  Diagnostics: measures

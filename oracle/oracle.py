@@ -80,6 +80,7 @@ def lib():
         L.vo_get_state.argtypes = [C.c_void_p, D, D, D, D, D, D, D]
         L.vo_time.restype = C.c_double
         L.vo_time.argtypes = [C.c_void_p]
+        L.vo_system_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         for f in ("vo_num_interactions", "vo_num_candidates", "vo_num_active"):
             getattr(L, f).argtypes = [C.c_void_p]
         L.vo_get_interactions.argtypes = [C.c_void_p, U, U]
@@ -175,6 +176,13 @@ class OracleSim:
     @property
     def time(self):
         return self.L.vo_time(self.h)
+
+    def system_energy(self):
+        """DOWorld::CalculateSystemEnergy of the current state, as a dict."""
+        from vedo_amd.gpu import energy_dict
+        out = np.zeros(16)
+        self.L.vo_system_energy(self.h, _dp(out))
+        return energy_dict(out)
 
     def interactions(self):
         n = self.L.vo_num_interactions(self.h)

@@ -11,6 +11,9 @@
 //
 // usage: vedo_ref_harness scene.txt out.bin nsteps dump_step[,dump_step...]
 //                         [--assigned statuses.bin]
+// statuses.bin: int count, count DO ids, then per step count rows of 15
+// doubles (pos, vel, Ox, Oz, w); a row whose first value is NaN is not
+// assigned at that step.
 // The process writes the reference's side files (time_0.csv,
 // SystemStatus.csv, *.ido) into its working directory; run it in a scratch
 // directory.
@@ -212,11 +215,16 @@ int main(int argc, char **argv)
             for (int w = 0; w < nassigned; ++w) {
                 double b[15];
                 assigned.read((char *)b, sizeof(b));
+                if (b[0] != b[0]) continue;   // NaN row: not assigned at this step
                 DOStatus s = *world->GetDOStatus(assigned_ids[w]);
                 s.SetPosition(Vector3df(b[0], b[1], b[2]));
                 s.SetVelocity(Vector3df(b[3], b[4], b[5]));
                 s.SetOrientation(Vector3df(b[6], b[7], b[8]), Vector3df(b[9], b[10], b[11]));
                 s.SetAngularVelocity(Vector3df(b[12], b[13], b[14]));
+                // the caller's status carries no impulses (the world's copy
+                // holds those of its last sync, Consultant.cpp:240)
+                s.SetImpact(Vector3df());
+                s.SetAngularImpact(Vector3df());
                 m[assigned_ids[w]] = s;
             }
             sm->Run(m);

@@ -1099,6 +1099,82 @@ void vo_get_state(const vo_sim *s, double *pos, double *vel, double *ox, double 
 
 double vo_time(const vo_sim *s) { return s->t_current; }
 
+/* DOModel::Analysis vMassMomentInertia (DOModel.cpp:22-29, 56-89) */
+static v3 model_mmi(const vo_sim *s, int m)
+{
+    const vo_model *M = &s->models[m];
+    const double mass = s->mass[m];
+    double r, h, w, l, d;
+    switch (M->shape) {
+    case VO_SPHERE:
+        r = M->attr[0];
+        d = 0.4 * mass * r * r;
+        return mk(d, d, d);
+    case VO_QUASICYLINDER:
+        r = M->attr[0]; h = M->attr[1];
+        d = mass * (3.0 * r * r + h * h) / 12.0;
+        return mk(d, d, 0.5 * mass * r * r);
+    case VO_QUASIPLATE:
+        w = M->attr[0]; h = M->attr[1]; l = M->attr[2];
+        return smul(mass / 12.0, mk(h * h + l * l, w * w + l * l, h * h + w * w));
+    default: return mk(0, 0, 0);
+    }
+}
+
+/* DOWorld::CalculateSystemEnergy (DOWorld.cpp:481-632), statement by
+ * statement: the first mobile DO initialises the sums, the others add to
+ * them; the averages are 1.0/N times the sums (NaN when there is no mobile
+ * DO, as in the reference); the norms are a second sequential pass. */
+void vo_system_energy(const vo_sim *s, double *out)
+{
+    double pe = 0, et = 0, er = 0, vmax = 0, vmin = 0, wmax = 0, wmin = 0, mn = 0, ln = 0;
+    v3 P = mk(0, 0, 0), L = mk(0, 0, 0);
+    unsigned nm = 0;
+    int first = -1;
+    for (int i = 0; i < s->n_do; ++i) {
+        const int m = s->model_of[i];
+        if (s->models[m].behavior != VO_MOBILE) continue;
+        const double mass = s->mass[m];
+        const v3 mmi = model_mmi(s, m);
+        const double v = len(s->vel[i]);
+        const v3 av = s->av[i];
+        if (first < 0) {
+            first = i;
+            P = smul(mass, s->vel[i]);
+            L = cross(mmi, av);
+            pe = -mass * dot(s->g, s->pos[i]);
+            et = 0.5 * mass * v * v;
+            er = 0.5 * (mmi.x * av.x * av.x + mmi.y * av.y * av.y + mmi.z * av.z * av.z);
+            vmax = vmin = v;
+            wmax = wmin = len(av);
+            nm = 1;
+            continue;
+        }
+        const double dav = len(av);
+        nm++;
+        P = add(P, smul(mass, s->vel[i]));
+        L = add(L, cross(mmi, av));
+        pe -= mass * dot(s->g, s->pos[i]);
+        et += 0.5 * mass * v * v;
+        er += 0.5 * (mmi.x * av.x * av.x + mmi.y * av.y * av.y + mmi.z * av.z * av.z);
+        if (v > vmax) vmax = v;
+        else if (v < vmin) vmin = v;
+        if (dav > wmax) wmax = dav;
+        else if (dav < wmin) wmin = dav;
+    }
+    P = smul(1.0 / (double)nm, P);
+    L = smul(1.0 / (double)nm, L);
+    if (first >= 0)
+        for (int i = first; i < s->n_do; ++i) {
+            const int m = s->model_of[i];
+            if (s->models[m].behavior != VO_MOBILE) continue;
+            mn += len(sub(smul(s->mass[m], s->vel[i]), P));
+            ln += len(sub(cross(model_mmi(s, m), s->av[i]), L));
+        }
+    const double o[16] = {pe, et, er, P.x, P.y, P.z, L.x, L.y, L.z, mn, ln, vmax, vmin, wmax, wmin, (double)nm};
+    for (int k = 0; k < 16; ++k) out[k] = o[k];
+}
+
 int vo_num_interactions(const vo_sim *s) { return s->n_ia; }
 void vo_get_interactions(const vo_sim *s, unsigned *slave, unsigned *master)
 {

@@ -108,6 +108,13 @@ void vo_get_candidates(const vo_sim *s, unsigned *lo, unsigned *hi);
 int vo_num_active(const vo_sim *s);
 void vo_get_active(const vo_sim *s, unsigned *lo, unsigned *hi, vo_status *st);
 
+/* DOWorld::CalculateSystemEnergy (DOWorld.cpp:481-632) of the current state,
+ * sequentially over the mobile DOs in DO-id order.  out[16]: potential,
+ * translation, rotation, momentum avg (3), angular momentum avg (3) formed as
+ * mmi.cross(w), momentum norm, angular momentum norm, v max, v min, w max,
+ * w min, number of mobile DOs. */
+void vo_system_energy(const vo_sim *s, double *out);
+
 /* Unit-level entry points used by closed-form tests. */
 typedef struct { double depth; double n[3]; double p[3]; int active; } vo_contact;
 void vo_cd_sphere_sphere(const double *xa, double ra, const double *xb, double rb, vo_contact *c);

@@ -31,4 +31,18 @@ def load_golden(name):
     return z, wa
 
 
-GOLDEN_CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_nbs", "drum_kin"]
+GOLDEN_CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_nbs", "drum_kin", "box_assign"]
+
+
+def step_golden(sim, z, step):
+    """Advance `sim` by the fixture's step `step` (0-based): SimMediator::Run()
+    or, when the fixture has assigned statuses, Run(map) with the rows of that
+    step that are not NaN (a NaN row is not assigned at that step)."""
+    if "assigned" not in z.files:
+        sim.step(1)
+        return
+    a = z["assigned"][step]
+    keep = ~np.isnan(a[:, 0])
+    ids = np.asarray(z["assigned_ids"])[keep]
+    a = a[keep]
+    sim.step_assigned(ids, a[:, 0:3], a[:, 3:6], a[:, 6:9], a[:, 9:12], a[:, 12:15])

@@ -87,6 +87,29 @@ def make_cases():
         p, v, ox, oz, av = scenes.rigid_rotation_status(p0, v0, ox0, oz0, c, a, 20.0, k * wa.dt)
         assigned.append(np.concatenate([p, v, ox, oz, av], axis=1))
     cases.append(("drum_kin", wa, 0, nsteps, [0, 1, 2, 9, 10, 60, 120], (ids, np.stack(assigned))))
+    # Run(map) on spheres: a few spheres take new statuses (moved by up to
+    # 0.2 r, new velocities) at some steps, one of them a rebuild step; rows
+    # of NaN are "not assigned at this step"
+    w = scenes.box_scene(64, seed=14, law="ISwBSDBF", dt=2e-5, steps=400, kn=2e3, e=0.6, mu=0.4,
+                         omega_max=5.0, **common)
+    wa = w.to_arrays(up_iact=7, record=1000)
+    nsteps = 40
+    ids = np.array([5, 17, 40, 41], np.uint32)
+    rng = np.random.default_rng(14)
+    r = wa.model_attr[wa.model_of[ids], 0]
+    rows = np.full((nsteps, len(ids), 15), np.nan)
+    for step in (3, 7, 8, 21):
+        sel = rng.random(len(ids)) < 0.75
+        sel[0] = True
+        p = wa.pos[ids] + rng.uniform(-0.2, 0.2, (len(ids), 3)) * r[:, None]
+        v = rng.uniform(-0.3, 0.3, (len(ids), 3))
+        th = rng.uniform(0, 2 * np.pi, len(ids))
+        ox = np.stack([np.cos(th), np.sin(th), np.zeros_like(th)], 1)
+        oz = np.tile([0.0, 0.0, 1.0], (len(ids), 1))
+        av = rng.uniform(-5, 5, (len(ids), 3))
+        st = np.concatenate([p, v, ox, oz, av], 1)
+        rows[step - 1][sel] = st[sel]
+    cases.append(("box_assign", wa, 0, nsteps, [0, 2, 3, 4, 7, 8, 9, 21, 22, 40], (ids, rows)))
     return cases
 
 
@@ -127,9 +150,13 @@ def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE):
 
 
 def main():
+    """Writes every case, or only those named on the command line."""
     if not os.path.exists(HARNESS):
         raise SystemExit("build the reference harness first: make -C oracle ref")
+    only = set(sys.argv[1:])
     for name, wa, cons, nsteps, dumps, assigned in make_cases():
+        if only and name not in only:
+            continue
         d = run_case(name, wa, cons, nsteps, dumps, assigned)
         nact = [int(r["iact"]["active"].sum()) for r in d]
         print(f"{name}: {wa.n_do} DOs, {len(d)} dumps, interactions {[len(r['iact']) for r in d]}, active {nact}")

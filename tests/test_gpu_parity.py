@@ -16,7 +16,7 @@ The NBS fixture is oracle-only: the GPU runs LeapConsultant rules.
 import numpy as np
 import pytest
 
-from conftest import GOLDEN_CASES, load_golden
+from conftest import GOLDEN_CASES, load_golden, step_golden
 
 pytestmark = pytest.mark.gpu
 
@@ -47,11 +47,7 @@ def test_gpu_matches_reference(name):
     step = 0
     for k, d in enumerate(z["dumps"]):
         while step < d:
-            if "assigned" in z.files:
-                a = z["assigned"][step]
-                sim.step_assigned(z["assigned_ids"], a[:, 0:3], a[:, 3:6], a[:, 6:9], a[:, 9:12], a[:, 12:15])
-            else:
-                sim.step(1)
+            step_golden(sim, z, step)
             step += 1
         st = sim.state()
         ref = z[f"d{k}_state"]
@@ -91,3 +87,35 @@ def test_gpu_library_loads_and_rejects_bad_scene():
     wa.model_behavior[wa.model_shape == 1] = 2   # fixed spheres are not on the GPU path
     with pytest.raises(GpuError):
         GpuSimulation(wa)
+
+
+def test_gpu_system_energy_matches_oracle():
+    """vg_system_energy against the oracle's sequential restatement of
+    DOWorld::CalculateSystemEnergy on bit-identical states (ISwBSDBF): the
+    extremes and the count agree exactly, the sums to 1e-13 of the sum of
+    their terms' magnitudes (tree order vs sequential order)."""
+    from oracle.oracle import OracleSim
+    from vedo_amd.gpu import GpuSimulation
+    z, wa = load_golden("box_bsdbf")
+    gpu, cpu = GpuSimulation(wa), OracleSim(wa)
+    sph = wa.model_shape[wa.model_of] == 1
+    m = (wa.model_density * 4.0 / 3.0 * np.pi * wa.model_attr[:, 0] ** 3)[wa.model_of][sph]
+    inertia = 0.4 * m * wa.model_attr[wa.model_of][sph, 0] ** 2
+    for k in range(4):
+        eg, ec = gpu.system_energy(), cpu.system_energy()
+        st = cpu.state()
+        v, w, x = st["vel"][sph], st["av"][sph], st["pos"][sph]
+        n = sph.sum()
+        scales = {"potential": np.sum(m * np.abs(x @ wa.g)), "translation": ec["translation"],
+                  "rotation": ec["rotation"], "momentum_avg": np.sum(m * np.linalg.norm(v, axis=1)) / n,
+                  "angular_momentum_avg": np.sum(inertia * np.linalg.norm(w, axis=1)) * 2 / n,
+                  "momentum_norm": ec["momentum_norm"], "angular_momentum_norm": ec["angular_momentum_norm"]}
+        print(f"  step {20 * k}: " + " ".join(f"{f} {np.max(np.abs(eg[f] - ec[f])) / max(s, 1e-300):.1e}"
+                                            for f, s in scales.items()), flush=True)
+        for f, s in scales.items():
+            assert np.max(np.abs(eg[f] - ec[f])) <= 1e-13 * s, f
+        for f in ("velocity_max", "velocity_min", "angular_velocity_max", "angular_velocity_min", "mobile"):
+            assert eg[f] == ec[f], f
+        gpu.step(20)
+        cpu.step(20)
+    gpu.close()

@@ -235,3 +235,72 @@ def test_rebuild_cadence_is_transparent_to_active_sets():
         lb, hb, _ = b.active()
         assert sorted(zip(la.tolist(), ha.tolist())) == sorted(zip(lb.tolist(), hb.tolist()))
     np.testing.assert_allclose(a.state()["pos"], b.state()["pos"], rtol=1e-12)
+
+
+def _energy_world(statuses, g=(0.0, 0.0, -9.81)):
+    """Two sphere models (r 1 mm and 2 mm, rho 2500) and a fixed plate."""
+    models = [DOModel("A", "G", "mobile", "local", 2500.0, 1.0, (0, 0, 0), "Sphere", (1e-3,)),
+              DOModel("B", "G", "mobile", "local", 2500.0, 1.0, (0, 0, 0), "Sphere", (2e-3,)),
+              DOModel("W", "H", "fixed", "global", 0.0, 1.0, (0, 0, 0), "QuasiPlate", (0.1, 0.01, 0.1))]
+    mech = scenes.spring_dashpot(1e4, 0.8, 0.6, 0.4)
+    iact = [IactModel("G", "G", "ISwBSDBF", mech), IactModel("H", "G", "ISwBSDBF", mech)]
+    return DOWorld(SystemParameter(0.0, 1.0, 1e-5, 0.0, g), models, iact, statuses)
+
+
+def test_system_energy_closed_form():
+    """DOWorld::CalculateSystemEnergy (DOWorld.cpp:481-632) on three spheres
+    with chosen velocities: energies are sums of 1/2 m v^2, 1/2 I w^2 and
+    -m g.x; the "angular momentum" is the reference's mmi.cross(w), i.e.
+    I (1,1,1) x w; the fixed plate does not count."""
+    ma = 2500.0 * 4.0 / 3.0 * math.pi * 1e-9
+    mb = 2500.0 * 4.0 / 3.0 * math.pi * 8e-9
+    ia, ib = 0.4 * ma * 1e-6, 0.4 * mb * 4e-6
+    st = [DOStatus("A", (0.0, 0.0, 0.1), (1.0, 2.0, 2.0), angular_velocity=(0.0, 0.0, 2.0)),
+          DOStatus("W", (0.0, 0.0, -0.5)),
+          DOStatus("B", (0.3, 0.0, -0.2), (0.0, -3.0, 0.0), angular_velocity=(1.0, 0.0, 0.0)),
+          DOStatus("A", (0.0, 0.2, 0.3), (0.0, 0.0, 0.5), angular_velocity=(0.0, 4.0, 0.0))]
+    e = OracleSim(_energy_world(st).to_arrays(up_iact=10, record=10 ** 6)).system_energy()
+    assert e["mobile"] == 3
+    assert e["translation"] == pytest.approx(0.5 * ma * 9.0 + 0.5 * mb * 9.0 + 0.5 * ma * 0.25, rel=1e-14)
+    assert e["rotation"] == pytest.approx(0.5 * ia * 4.0 + 0.5 * ib * 1.0 + 0.5 * ia * 16.0, rel=1e-14)
+    assert e["potential"] == pytest.approx(9.81 * (ma * 0.1 + mb * -0.2 + ma * 0.3), rel=1e-13)
+    p = (ma * np.array([1.0, 2.0, 2.0]) + mb * np.array([0.0, -3.0, 0.0]) + ma * np.array([0.0, 0.0, 0.5])) / 3
+    np.testing.assert_allclose(e["momentum_avg"], p, rtol=1e-14)
+    one = np.ones(3)
+    ls = [ia * np.cross(one, [0, 0, 2.0]), ib * np.cross(one, [1.0, 0, 0]), ia * np.cross(one, [0, 4.0, 0])]
+    np.testing.assert_allclose(e["angular_momentum_avg"], sum(ls) / 3, rtol=1e-14)
+    moms = [ma * np.array([1.0, 2.0, 2.0]), mb * np.array([0.0, -3.0, 0.0]), ma * np.array([0.0, 0.0, 0.5])]
+    assert e["momentum_norm"] == pytest.approx(sum(np.linalg.norm(q - p) for q in moms), rel=1e-13)
+    assert e["angular_momentum_norm"] == pytest.approx(sum(np.linalg.norm(q - sum(ls) / 3) for q in ls), rel=1e-13)
+    assert (e["velocity_max"], e["velocity_min"]) == (3.0, 0.5)
+    assert (e["angular_velocity_max"], e["angular_velocity_min"]) == (4.0, 1.0)
+
+
+def test_system_energy_without_mobile_dos():
+    """No mobile DO: zero energies and extremes, NaN averages (1.0/0 * 0 in
+    the reference)."""
+    e = OracleSim(_energy_world([DOStatus("W", (0.0, 0.0, -0.5))]).to_arrays(up_iact=10, record=10)).system_energy()
+    assert e["mobile"] == 0 and e["translation"] == 0.0 and e["potential"] == 0.0 and e["velocity_max"] == 0.0
+    assert np.isnan(e["momentum_avg"]).all() and np.isnan(e["angular_momentum_avg"]).all()
+
+
+def test_energy_conserved_in_free_flight():
+    """Spheres far apart, no walls, gravity: the update is an exact parabola,
+    so potential + translational energy is conserved (a sign error in the
+    potential fails this), rotational energy is constant and the average
+    momentum changes by exactly m g t per sphere."""
+    w = scenes.box_scene(27, seed=5, law="ISwBSDBF", dt=1e-4, steps=400, spacing=40.0, jitter=0.05, vmax=0.3,
+                         offset=3.0, kn=2e3, e=0.6, mu=0.4, walls=False, omega_max=5.0)
+    sim = OracleSim(w.to_arrays(up_iact=7, record=1000))
+    e0 = sim.system_energy()
+    for _ in range(200):
+        sim.step(1)
+        assert sim.active()[0].size == 0
+    e1 = sim.system_energy()
+    tot0, tot1 = e0["potential"] + e0["translation"], e1["potential"] + e1["translation"]
+    assert abs(tot1 - tot0) <= 1e-12 * (abs(e0["potential"]) + e0["translation"] + e1["translation"])
+    assert e1["translation"] != pytest.approx(e0["translation"], rel=1e-3)
+    assert e1["rotation"] == pytest.approx(e0["rotation"], rel=1e-13)
+    m = 2500.0 * 4.0 / 3.0 * math.pi * 1e-9
+    np.testing.assert_allclose(e1["momentum_avg"] - e0["momentum_avg"], [0.0, 0.0, -9.81 * m * 0.02],
+                               rtol=1e-12, atol=1e-20)

@@ -15,7 +15,7 @@ direction of a zero vector as zero and keeps the orientation.  Those entries
 import numpy as np
 import pytest
 
-from conftest import GOLDEN_CASES, load_golden
+from conftest import GOLDEN_CASES, load_golden, step_golden
 from oracle.oracle import OracleSim
 
 FIELDS = ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")
@@ -27,11 +27,7 @@ def _run(name):
     step = 0
     for k, d in enumerate(z["dumps"]):
         while step < d:
-            if "assigned" in z.files:
-                a = z["assigned"][step]
-                sim.step_assigned(z["assigned_ids"], a[:, 0:3], a[:, 3:6], a[:, 6:9], a[:, 9:12], a[:, 12:15])
-            else:
-                sim.step()
+            step_golden(sim, z, step)
             step += 1
         yield k, int(d), z, sim
 

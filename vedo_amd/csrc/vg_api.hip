@@ -40,6 +40,7 @@ struct vg_scene {
     std::vector<int> do_is_sphere;         // DO id -> 1 sphere / 0 wall
     std::vector<unsigned> wall_ids;        // wall index -> DO id
     std::vector<unsigned> sphere_ids;      // sphere rank -> DO id
+    std::vector<int> do_to_sphere;         // DO id -> sphere rank or -1
     unsigned *d_sphere_ids = nullptr, *d_wall_ids = nullptr;
     std::vector<int> do_to_wall;           // DO id -> wall index or -1
     std::vector<WallDev> walls_h;
@@ -63,6 +64,9 @@ struct vg_scene {
     WallDev *walls = nullptr, *walls_ref = nullptr;
     WallKin *wkin = nullptr;
     double *assigned = nullptr;
+    double *assigned_sph = nullptr;   // [n][16] assigned sphere statuses by rank (allocated on first use)
+    double2 *mm = nullptr;            // per model: mass, mass moment of inertia (spheres)
+    double *en_part = nullptr, *en_out = nullptr;
     int *flags = nullptr;
     unsigned long long *relists = nullptr;
     unsigned long long *red = nullptr, *bpart = nullptr;
@@ -249,7 +253,9 @@ static int rebuild(vg_scene *h)
     hipLaunchKernelGGL(k_inner_build, dim3(nblk(n, 128)), dim3(128), (size_t)h->ki * 128 * sizeof(unsigned), st,
                        ia);
     // the fresh lists need no relist: clear the ring slot the next step reads
+    // and restart the wall displacement bounds
     HIPCHK(hipMemsetAsync(h->flags + ((h->step_count) % 3), 0, sizeof(int), st));
+    if (h->nw) hipLaunchKernelGGL(k_wall_path_reset, dim3(nblk(h->nw, 64)), dim3(64), 0, st, h->walls, h->nw);
     HIPCHK(hipGetLastError());
     h->cur ^= 1;
     h->ecur ^= 1;
@@ -292,6 +298,7 @@ static void free_scene(vg_scene *h)
     F(h->models); F(h->laws); F(h->walls); F(h->walls_ref); F(h->wkin); F(h->assigned); F(h->flags);
     F(h->relists); F(h->red); F(h->bpart); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->safe); F(h->lin);
     F(h->cstart); F(h->cend); F(h->ocount); F(h->onbr); F(h->tmp); F(h->rec); F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->d_sphere_ids); F(h->d_wall_ids);
+    F(h->assigned_sph); F(h->mm); F(h->en_part); F(h->en_out);
     for (auto e : h->kev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -438,6 +445,10 @@ extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
         W.av = mk(d->av[3 * id], d->av[3 * id + 1], d->av[3 * id + 2]);
         W.a0 = M.attr[0]; W.a1 = M.attr[1]; W.a2 = M.attr[2];
         W.range = model_range(M);
+        W.reach = M.shape == VG_QUASIPLATE ? 0.5 * std::sqrt(M.attr[0] * M.attr[0] + M.attr[1] * M.attr[1] +
+                                                             M.attr[2] * M.attr[2])
+                                           : std::sqrt(M.attr[0] * M.attr[0] + 0.25 * M.attr[1] * M.attr[1]);
+        W.path = 0.0;
         W.shape = M.shape; W.group = M.group; W.behavior = M.behavior; W.kin = 0;
     }
     // device
@@ -468,6 +479,8 @@ extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
     rc |= dalloc(h, &h->rec, (size_t)h->ki * 13 * n) | dalloc(h, &h->rec_imp, (size_t)6 * n);
     rc |= dalloc(h, &h->rec_imp2, (size_t)6 * n) | dalloc(h, &h->snap_wimp, (size_t)6 * std::max(h->nw, 1));
     rc |= dalloc(h, &h->d_sphere_ids, n) | dalloc(h, &h->d_wall_ids, std::max(h->nw, 1));
+    rc |= dalloc(h, &h->mm, d->n_models) | dalloc(h, &h->en_out, 24);
+    rc |= dalloc(h, &h->en_part, (size_t)16 * std::max(1u, std::min(1024u, nblk(n, 256))));
     if (rc) return bail(VG_ENOMEM);
     if (dalloc(h, &h->onbr, (size_t)h->ko * n)) return bail(VG_ENOMEM);
     // upload spheres in DO-id order
@@ -503,7 +516,18 @@ extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
             return bail(VG_EHIP);
         }
     }
-    if (hipMemcpy(h->models, h->models_h.data(), sizeof(ModelDev) * d->n_models, hipMemcpyHostToDevice) !=
+    // DOModel::Analysis mass and mass moment of inertia of spheres (DOModel.cpp:22-29)
+    std::vector<double2> mm(d->n_models, make_double2(0.0, 0.0));
+    for (int m = 0; m < d->n_models; ++m)
+        if (d->models[m].shape == VG_SPHERE) {
+            const double r = d->models[m].attr[0];
+            const double mass = d->models[m].density * ((4.0 / 3.0 * PI) * r * r * r);
+            mm[m] = make_double2(mass, 0.4 * mass * r * r);
+        }
+    h->do_to_sphere.assign(h->n_do, -1);
+    for (int s = 0; s < n; ++s) h->do_to_sphere[sphere_ids[s]] = s;
+    if (hipMemcpy(h->mm, mm.data(), sizeof(double2) * d->n_models, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->models, h->models_h.data(), sizeof(ModelDev) * d->n_models, hipMemcpyHostToDevice) !=
             hipSuccess ||
         hipMemcpy(h->laws, h->laws_h.data(), sizeof(LawDev) * h->laws_h.size(), hipMemcpyHostToDevice) !=
             hipSuccess ||
@@ -552,7 +576,8 @@ static int export_active(vg_scene *h, ActiveOut *out, size_t cap, unsigned long 
 {
     HIPCHK(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), h->stream));
     hipLaunchKernelGGL(k_export_active, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, h->geo[h->cur],
-                       h->ell[h->ecur], h->n, h->ki, (const WallDev *)h->walls, (const unsigned *)h->d_wall_ids,
+                       h->ell[h->ecur], h->n, h->ki, (const unsigned *)h->ocount, (const unsigned *)h->onbr,
+                       (const WallDev *)h->walls, (const unsigned *)h->d_wall_ids,
                        (const unsigned *)h->d_sphere_ids,
                        h->ent_valid ? (const double *)h->rec : (const double *)nullptr, out,
                        (unsigned long long)cap, dcnt);
@@ -587,7 +612,7 @@ static int snapshot_records(vg_scene *h)
     return 0;
 }
 
-static int one_step(vg_scene *h, bool rec, bool assigned)
+static int one_step(vg_scene *h, bool rec, bool assign_spheres)
 {
     StepArgs a = step_args(h);
     const bool kt = 2 * (h->kev_used + 1) <= (int)h->kev.size();
@@ -611,9 +636,16 @@ static int one_step(vg_scene *h, bool rec, bool assigned)
     h->t += h->dt;   // SystemParameter::NextStep
     if (h->nw) {
         hipLaunchKernelGGL(k_walls, dim3(nblk(h->nw, 64)), dim3(64), 0, h->stream, h->walls, h->nw,
-                           h->assigned, h->walls_ref, h->wkin, h->dt, h->t);
+                           h->assigned, h->walls_ref, h->wkin, h->dt, h->t, h->flags, a.step, a.relist_half);
     }
-    (void)assigned;
+    if (assign_spheres) {
+        // SimMediator::Run(map): the assigned statuses replace the integrated
+        // ones before contact detection and the rebuild; positions may jump,
+        // so every sphere relists at the next step
+        hipLaunchKernelGGL(k_assign_spheres, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, h->geo[h->cur], h->n,
+                           (const double *)h->assigned_sph, rec ? h->rec_imp : (double *)nullptr);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h->stream, h->flags + (h->step_count % 3), 1ll, 1);
+    }
     h->rec_valid = rec;
     h->ent_valid = rec;
     h->snap_valid = false;
@@ -658,12 +690,25 @@ extern "C" int vg_sync(vg_scene *h)
 extern "C" int vg_step_assigned(vg_scene *h, int n, const unsigned *ids, const double *pos, const double *vel,
                                 const double *ox, const double *oz, const double *av)
 {
-    if (!h || n < 0) return VG_EINVAL;
+    if (!h || n < 0 || (n > 0 && (!ids || !pos || !vel || !ox || !oz || !av))) return VG_EINVAL;
     std::vector<double> buf((size_t)15 * h->nw, 0.0);
     std::vector<char> set(h->nw, 0);
+    std::vector<double> sb;
+    bool any_sphere = false;
     for (int k = 0; k < n; ++k) {
-        if (ids[k] >= (unsigned)h->n_do || h->do_to_wall[ids[k]] < 0)
-            return fail(h, VG_EINVAL, "assigned statuses are supported for walls only");
+        if (ids[k] >= (unsigned)h->n_do)
+            return fail(h, VG_EINVAL, "assigned status of an unknown DO id");
+        if (h->do_to_sphere[ids[k]] >= 0) {
+            if (!any_sphere) sb.assign((size_t)16 * h->n, 0.0);
+            any_sphere = true;
+            double *b = &sb[(size_t)16 * h->do_to_sphere[ids[k]]];
+            for (int c = 0; c < 3; ++c) {
+                b[c] = pos[3 * k + c]; b[3 + c] = vel[3 * k + c]; b[6 + c] = ox[3 * k + c];
+                b[9 + c] = oz[3 * k + c]; b[12 + c] = av[3 * k + c];
+            }
+            b[15] = 1.0;
+            continue;
+        }
         const int w = h->do_to_wall[ids[k]];
         double *b = &buf[(size_t)15 * w];
         for (int c = 0; c < 3; ++c) {
@@ -671,6 +716,11 @@ extern "C" int vg_step_assigned(vg_scene *h, int n, const unsigned *ids, const d
             b[9 + c] = oz[3 * k + c]; b[12 + c] = av[3 * k + c];
         }
         set[w] = 1;
+    }
+    if (any_sphere) {
+        if (!h->assigned_sph && dalloc(h, &h->assigned_sph, (size_t)16 * h->n)) return VG_ENOMEM;
+        HIPCHK(hipMemcpyAsync(h->assigned_sph, sb.data(), sizeof(double) * sb.size(), hipMemcpyHostToDevice,
+                              h->stream));
     }
     bool changed = false;
     for (int w = 0; w < h->nw; ++w) {
@@ -692,9 +742,27 @@ extern "C" int vg_step_assigned(vg_scene *h, int n, const unsigned *ids, const d
         }
     }
     h->last_launches = 0;
-    int rc = one_step(h, true, true);
+    int rc = one_step(h, true, any_sphere);
     if (rc) return rc;
-    // synchronous: the host buffer `buf` is reused by the next call
+    // synchronous: the host buffers are reused by the next call
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int vg_system_energy(vg_scene *h, double *out)
+{
+    if (!h || !out) return VG_EINVAL;
+    const int n = h->n;
+    const int nb = (int)std::max(1u, std::min(1024u, nblk(n, 256)));
+    double *avg = h->en_out + 16;
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_energy_part, dim3(nb), dim3(256), 0, h->stream, h->geo[h->cur], n,
+                           (const double2 *)h->mm, h->g[0], h->g[1], h->g[2], (const double *)avg, pass, h->en_part);
+        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, h->stream, (const double *)h->en_part, nb, n, pass,
+                           avg, h->en_out);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, h->en_out, 16 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }

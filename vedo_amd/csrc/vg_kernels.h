@@ -96,7 +96,8 @@ struct ActiveOut {
 
 template <bool REC, bool SD> __global__ void k_step(StepArgs a);
 __global__ void k_walls(WallDev *walls, int nw, const double *assigned, const WallDev *ref, const WallKin *kin,
-                        double dt, double t_new);
+                        double dt, double t_new, int *flags, int step, double relist_half);
+__global__ void k_wall_path_reset(WallDev *walls, int nw);
 __global__ void k_bounds(Geo g, int n, unsigned long long *part, unsigned *inv, double *xs, double *ys, double *zs);
 __global__ void k_bounds_final(const unsigned long long *part, int nblocks, unsigned long long *red);
 __global__ void k_cellkey(Geo g, int n, GridDev G, const unsigned *inv, unsigned long long *keys, unsigned *vals);
@@ -109,10 +110,15 @@ __global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int n, in
 __global__ void k_inner_build(InnerArgs a);
 __global__ void k_hdr_zero(unsigned char *hdr, int n);
 __global__ void k_scatter_state(Geo g, int n, const double *rec_imp, double *out);
-__global__ void k_export_active(Geo g, Ell ell, int n, int ki, const WallDev *walls, const unsigned *wall_id,
-                                const unsigned *sphere_id, const double *rec, ActiveOut *out,
-                                unsigned long long cap, unsigned long long *counter);
+__global__ void k_export_active(Geo g, Ell ell, int n, int ki, const unsigned *o_cnt, const unsigned *o_nbr,
+                                const WallDev *walls, const unsigned *wall_id, const unsigned *sphere_id,
+                                const double *rec, ActiveOut *out, unsigned long long cap,
+                                unsigned long long *counter);
 __global__ void k_wall_impulse(Ell ell, int n, int ki, const double *rec, double *out);
+__global__ void k_assign_spheres(Geo g, int n, const double *buf, double *rec_imp);
+__global__ void k_energy_part(Geo g, int n, const double2 *mm, double gx, double gy, double gz, const double *avg,
+                              int pass, double *part);
+__global__ void k_energy_final(const double *part, int nb, int n, int pass, double *avg, double *out);
 __global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n);
 
 }  // namespace vg

@@ -274,11 +274,12 @@ template __global__ void k_step<true, true>(StepArgs);
 // (SimMediator.cpp:1121-1163).  kin 1: host-assigned status in `assigned`;
 // kin 2: rigid rotation about (c, a) at omega from the reference status.
 __global__ void k_walls(WallDev *walls, int nw, const double *assigned, const WallDev *ref, const WallKin *kin,
-                        double dt, double t_new)
+                        double dt, double t_new, int *flags, int step, double relist_half)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nw) return;
-    WallDev W = walls[w];
+    const WallDev old = walls[w];
+    WallDev W = old;
     if (W.kin == 1) {
         const double *s = assigned + 15 * w;
         W.pos = mk(s[0], s[1], s[2]); W.vel = mk(s[3], s[4], s[5]);
@@ -301,7 +302,25 @@ __global__ void k_walls(WallDev *walls, int nw, const double *assigned, const Wa
         W.ox = rotate_around(W.ox, dw);
         W.oz = rotate_around(W.oz, dw);
     }
+    // Relist guard for moving walls.  A wall point q = a Ox + b Oy + c Oz
+    // relative to the centre, |a|, |b|, |c| <= reach, moves by at most
+    // |dpos| + reach (|dOx| + |dOy| + |dOz|), so a sphere-wall depth changes
+    // by at most the sphere's path plus this accumulated bound: a wall whose
+    // bound exceeds half the skin makes every sphere relist at the next
+    // step, as a sphere's own path does.  The bound restarts when the step
+    // that just ran relisted (the lists were made at these wall positions).
+    const d3 oy0 = cross(old.oz, old.ox), oy1 = cross(W.oz, W.ox);
+    const double d = len(W.pos - old.pos) + W.reach * ((len(W.ox - old.ox) + len(oy1 - oy0)) + len(W.oz - old.oz));
+    W.path = (flags[step % 3] ? 0.0 : old.path) + d;
+    if (W.path > relist_half) atomicOr(&flags[(step + 1) % 3], 1);
     walls[w] = W;
+}
+
+// Fresh lists after a rebuild: the wall displacement bounds restart.
+__global__ void k_wall_path_reset(WallDev *walls, int nw)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nw) walls[w].path = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -644,9 +663,17 @@ __global__ void k_scatter_state(Geo g, int n, const double *rec_imp, double *out
     for (int c = 0; c < 6; ++c) d[15 + c] = rec_imp ? rec_imp[(size_t)c * n + i] : 0.0;
 }
 
-__global__ void k_export_active(Geo g, Ell ell, int n, int ki, const WallDev *walls, const unsigned *wall_id,
-                                const unsigned *sphere_id, const double *rec, ActiveOut *out,
-                                unsigned long long cap, unsigned long long *counter)
+// Active contacts at the current positions with their ImpactStatus.  The
+// candidate list is walked, not the neighbour list: a pair may come into
+// contact at the end of a step whose guard has just asked for a relist (or
+// after assigned statuses moved spheres), before the next step relists.  The
+// neighbour list is an ordered subsequence of the candidate list, so a merge
+// walk finds each listed pair's entry (flags, history, records); a candidate
+// outside the list has been solved as inactive every step: Clean status.
+__global__ void k_export_active(Geo g, Ell ell, int n, int ki, const unsigned *o_cnt, const unsigned *o_nbr,
+                                const WallDev *walls, const unsigned *wall_id, const unsigned *sphere_id,
+                                const double *rec, ActiveOut *out, unsigned long long cap,
+                                unsigned long long *counter)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -654,10 +681,23 @@ __global__ void k_export_active(Geo g, Ell ell, int n, int ki, const WallDev *wa
     const int h = hdr >> 7, cnt = hdr & 127;
     const double4 p = g.pos[i];
     const d3 xi = mk(p.x, p.y, p.z);
-    for (int k = 0; k < cnt; ++k) {
-        const unsigned ent = ell.ent[eidx(h, k, ki, n, i)];
+    const unsigned e = o_cnt[i];
+    int q = 0;
+    unsigned listed = cnt > 0 ? (ell.ent[eidx(h, 0, ki, n, i)] & PARTNER_MASK) : 0xffffffffu;
+    for (unsigned t = 0; t < e; ++t) {
+        const unsigned j = o_nbr[(size_t)t * n + i];
+        int k = -1;
+        unsigned ent;
+        if (listed == j) {
+            k = q;
+            ent = ell.ent[eidx(h, q, ki, n, i)];
+            ++q;
+            listed = q < cnt ? (ell.ent[eidx(h, q, ki, n, i)] & PARTNER_MASK) : 0xffffffffu;
+        } else {
+            ent = j;
+            if (j >= (unsigned)n || g.oid[i] < g.oid[j]) ent |= F_SLAVE;
+        }
         if (!(ent & F_SLAVE)) continue;
-        const unsigned j = ent & PARTNER_MASK;
         Contact c;
         unsigned other;
         if (j < (unsigned)n) {
@@ -679,9 +719,9 @@ __global__ void k_export_active(Geo g, Ell ell, int n, int ki, const WallDev *wa
         Hist hs;
         hs.s = mk(0, 0, 0);
         hs.aux = 0.0;
-        if (ent & F_CONTACT) load_hist(ell.hist, h, k, ki, n, i, hs);
+        if (k >= 0 && (ent & F_CONTACT)) load_hist(ell.hist, h, k, ki, n, i, hs);
         A.hist[0] = hs.s.x; A.hist[1] = hs.s.y; A.hist[2] = hs.s.z; A.hist[3] = hs.aux;
-        for (int q = 0; q < 13; ++q) A.rec[q] = rec ? rec[((size_t)k * 13 + q) * n + i] : 0.0;
+        for (int r = 0; r < 13; ++r) A.rec[r] = (rec && k >= 0) ? rec[((size_t)k * 13 + r) * n + i] : 0.0;
     }
 }
 
@@ -711,6 +751,134 @@ __global__ void k_wall_impulse(Ell ell, int n, int ki, const double *rec, double
     }
     if (threadIdx.x == 0)
         for (int c = 0; c < 6; ++c) out[w * 6 + c] = sm[c][0];
+}
+
+// Assigned sphere statuses of SimMediator::Run(map) (SimMediator.cpp:881-889):
+// after Response, the listed DOs take the given status, which carries zero
+// stored impulses.  buf holds 16 doubles per sphere rank: pos, vel, Ox, Oz,
+// w, and a nonzero last value for the listed ones.
+__global__ void k_assign_spheres(Geo g, int n, const double *buf, double *rec_imp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *b = buf + (size_t)g.oid[i] * 16;
+    if (b[15] == 0.0) return;
+    g.pos[i] = make_double4(b[0], b[1], b[2], g.pos[i].w);
+    g.vel[i] = make_double4(b[3], b[4], b[5], 0.0);   // the host forces a relist
+    for (int c = 0; c < 6; ++c) g.ori[(size_t)c * n + i] = b[6 + c];
+    g.avl[i] = make_double4(b[12], b[13], b[14], g.avl[i].w);
+    if (rec_imp)
+        for (int c = 0; c < 6; ++c) rec_imp[(size_t)c * n + i] = 0.0;
+}
+
+// DOWorld::CalculateSystemEnergy (DOWorld.cpp:481-632) over the spheres (the
+// mobile DOs of this path).  Every per-sphere term is formed as the reference
+// forms it; the sums are a fixed-shape tree, so they differ from the
+// reference's sequential sums only in rounding order.
+//
+// k_energy_part, pass 0, writes per block: [0] sum m (g.x), [1] sum 0.5 m v v,
+// [2] sum 0.5 (I wx wx + I wy wy + I wz wz), [3..5] sum m v, [6..8] sum
+// mmi x w (the reference's "angular momentum"), [9] max |v|, [10] min |v|,
+// [11] max |w|, [12] min |w|.  Pass 1 writes [0] sum |m v - P| and
+// [1] sum |mmi x w - L| for the averages P, L in avg[0..5].
+static constexpr int EN_K0 = 13, EN_K1 = 2, EN_BLOCK = 256;
+
+__device__ inline void en_terms(const Geo &g, int i, const double2 *mm, double gx, double gy, double gz, d3 &mv,
+                                d3 &Lw, double *t)
+{
+    const double4 p = g.pos[i], v4 = g.vel[i], w4 = g.avl[i];
+    const double2 M = mm[g.mdl[i]];
+    const double m = M.x, I = M.y;
+    const d3 vel = ld3(v4), av = ld3(w4);
+    const double v = len(vel);
+    mv = smul(m, vel);
+    Lw = cross(mk(I, I, I), av);
+    t[0] = m * (((gx * p.x) + (gy * p.y)) + (gz * p.z));
+    t[1] = 0.5 * m * v * v;
+    t[2] = 0.5 * ((((I * av.x) * av.x) + ((I * av.y) * av.y)) + ((I * av.z) * av.z));
+    t[3] = v;
+    t[4] = len(av);
+}
+
+template <int K>
+__device__ inline void en_block_reduce(double (*sh)[EN_BLOCK], double *acc, int nsum, double *out)
+{
+    const int t = threadIdx.x;
+    for (int k = 0; k < K; ++k) sh[k][t] = acc[k];
+    __syncthreads();
+    for (int s = EN_BLOCK / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int k = 0; k < K; ++k) {
+                const double a = sh[k][t], b = sh[k][t + s];
+                sh[k][t] = k < nsum ? a + b : (((k - nsum) & 1) ? fmin(a, b) : fmax(a, b));
+            }
+        __syncthreads();
+    }
+    if (t < K) out[t] = sh[t][0];
+}
+
+__global__ __launch_bounds__(EN_BLOCK) void k_energy_part(Geo g, int n, const double2 *mm, double gx, double gy,
+                                                          double gz, const double *avg, int pass, double *part)
+{
+    __shared__ double sh[EN_K0][EN_BLOCK];
+    double acc[EN_K0];
+    for (int k = 0; k < EN_K0; ++k) acc[k] = 0.0;
+    acc[9] = acc[11] = -INFINITY;
+    acc[10] = acc[12] = INFINITY;
+    for (int i = blockIdx.x * EN_BLOCK + threadIdx.x; i < n; i += gridDim.x * EN_BLOCK) {
+        d3 mv, Lw;
+        double t[5];
+        en_terms(g, i, mm, gx, gy, gz, mv, Lw, t);
+        if (pass == 0) {
+            acc[0] += t[0]; acc[1] += t[1]; acc[2] += t[2];
+            acc[3] += mv.x; acc[4] += mv.y; acc[5] += mv.z;
+            acc[6] += Lw.x; acc[7] += Lw.y; acc[8] += Lw.z;
+            acc[9] = fmax(acc[9], t[3]); acc[10] = fmin(acc[10], t[3]);
+            acc[11] = fmax(acc[11], t[4]); acc[12] = fmin(acc[12], t[4]);
+        } else {
+            acc[0] += len(mv - mk(avg[0], avg[1], avg[2]));
+            acc[1] += len(Lw - mk(avg[3], avg[4], avg[5]));
+        }
+    }
+    if (pass == 0) en_block_reduce<EN_K0>(sh, acc, 9, part + (size_t)blockIdx.x * EN_K0);
+    else en_block_reduce<EN_K1>(sh, acc, 2, part + (size_t)blockIdx.x * EN_K1);
+}
+
+// One block: folds the per-block partials in block order and writes the
+// results (out layout of vg_system_energy).  Pass 0 also leaves the averages
+// for pass 1 in avg[0..5].
+__global__ __launch_bounds__(EN_BLOCK) void k_energy_final(const double *part, int nb, int n, int pass, double *avg,
+                                                           double *out)
+{
+    __shared__ double sh[EN_K0][EN_BLOCK];
+    __shared__ double r[EN_K0];
+    const int K = pass == 0 ? EN_K0 : EN_K1, nsum = pass == 0 ? 9 : 2;
+    double acc[EN_K0];
+    for (int k = 0; k < EN_K0; ++k) acc[k] = 0.0;
+    acc[9] = acc[11] = -INFINITY;
+    acc[10] = acc[12] = INFINITY;
+    for (int b = threadIdx.x; b < nb; b += EN_BLOCK)
+        for (int k = 0; k < K; ++k) {
+            const double x = part[(size_t)b * K + k];
+            acc[k] = k < nsum ? acc[k] + x : (((k - nsum) & 1) ? fmin(acc[k], x) : fmax(acc[k], x));
+        }
+    if (pass == 0) en_block_reduce<EN_K0>(sh, acc, 9, r);
+    else en_block_reduce<EN_K1>(sh, acc, 2, r);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (pass == 0) {
+        // vMomentumAvg = 1.0 / NumMobileDO * vMomentumAvg (DOWorld.cpp:589-590)
+        const double f = 1.0 / (double)n;
+        for (int c = 0; c < 6; ++c) avg[c] = f * r[3 + c];
+        out[0] = n ? -r[0] : 0.0;
+        out[1] = r[1]; out[2] = r[2];
+        for (int c = 0; c < 6; ++c) out[3 + c] = avg[c];
+        out[11] = n ? r[9] : 0.0; out[12] = n ? r[10] : 0.0;
+        out[13] = n ? r[11] : 0.0; out[14] = n ? r[12] : 0.0;
+        out[15] = (double)n;
+    } else {
+        out[9] = r[0]; out[10] = r[1];
+    }
 }
 
 // Plain streaming copy used to calibrate HBM counters and measure achievable

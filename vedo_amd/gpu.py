@@ -58,9 +58,24 @@ assert STATUS_DTYPE.itemsize == C.sizeof(VgStatus)
 
 EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_set_history", "vg_step",
            "vg_sync", "vg_step_assigned", "vg_set_wall_rotation", "vg_get_state", "vg_time",
-           "vg_num_interactions", "vg_set_skin", "vg_get_skin", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench")
+           "vg_num_interactions", "vg_set_skin", "vg_get_skin", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
+           "vg_system_energy")
 
 _lib = None
+
+# Layout of vg_system_energy's output (DOWorld::CalculateSystemEnergy,
+# DOWorld.cpp:481-632, the values it stores in SystemParameter).
+ENERGY_FIELDS = ("potential", "translation", "rotation", "momentum_avg", "angular_momentum_avg", "momentum_norm",
+                 "angular_momentum_norm", "velocity_max", "velocity_min", "angular_velocity_max",
+                 "angular_velocity_min", "mobile")
+
+
+def energy_dict(out):
+    o = np.asarray(out, np.float64)
+    return {"potential": o[0], "translation": o[1], "rotation": o[2], "momentum_avg": o[3:6].copy(),
+            "angular_momentum_avg": o[6:9].copy(), "momentum_norm": o[9], "angular_momentum_norm": o[10],
+            "velocity_max": o[11], "velocity_min": o[12], "angular_velocity_max": o[13],
+            "angular_velocity_min": o[14], "mobile": int(o[15])}
 
 
 class GpuError(RuntimeError):
@@ -98,6 +113,7 @@ def lib():
         L.vg_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_int),
                                C.POINTER(C.c_int)]
         L.vg_copy_bench.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.vg_system_energy.argtypes = [vp, _D]
         _lib = L
     return _lib
 
@@ -187,6 +203,12 @@ class GpuSimulation:
         ids = np.ascontiguousarray(ids, np.uint32)
         arrs = [np.ascontiguousarray(x, np.float64) for x in (pos, vel, ox, oz, av)]
         self._check(self.L.vg_step_assigned(self.h, len(ids), _up(ids), *(_dp(a) for a in arrs)))
+
+    def system_energy(self):
+        """DOWorld::CalculateSystemEnergy of the current state (ENERGY_FIELDS)."""
+        out = np.zeros(16)
+        self._check(self.L.vg_system_energy(self.h, _dp(out)))
+        return energy_dict(out)
 
     def set_wall_rotation(self, ids, axis_point, axis_dir, omega: float, t_ref: float = 0.0):
         ids = np.ascontiguousarray(ids, np.uint32)

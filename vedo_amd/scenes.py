@@ -175,7 +175,7 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
                rmax: float = 2e-3, classes: int = 64, density: float = 2500.0, R: float = 0.15,
                L: Optional[float] = None, omega: float = 2.0, kn: float = 1e4, kt_ratio: float = 0.8,
                e: float = 0.6, mu: float = 0.4, dt: float = 5e-6, steps: int = 10000,
-               sides: int = 64, thick: float = 2e-3, gap: float = 0.9999, arrays: bool = False,
+               sides: int = 64, thick: float = 2e-3, gap: float = 1e-6, arrays: bool = False,
                up_iact: int = 20, record: int = 10 ** 9):
     """Polydisperse spheres in a rotating drum of quasi-plates.
 
@@ -183,10 +183,11 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
     to ``classes`` log-spaced classes, one DOModel each in one group.  The drum
     axis is y; its mantle is a ``sides``-gon of quasi-plates plus two end
     plates, all prescribed to rotate rigidly at ``omega`` (kinematic walls).
-    Spheres start on a layered cubic lattice in the lower part of the drum:
-    each z-layer holds one radius class at spacing 2*gap*r (gap < 1: touching
-    neighbours start in slight contact, a pre-settled bed).  When ``L`` is
-    None the drum is made just long enough for that lattice.
+    Spheres start as a settled bed in the lower half of the drum: per radius
+    class a slab of triangular lattices in the cross-section, pocket contacts
+    pre-compressed to the equilibrium of their load, in-row neighbours
+    2*(1+gap)*r apart.  When ``L`` is None the drum is made just long enough
+    for that bed.
 
     Returns (world, wall_ids, axis_point, axis_dir); with ``arrays=True`` the
     first item is a WorldArrays built without per-sphere Python objects (for
@@ -200,33 +201,41 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
     # Sort by class so that each lattice layer holds similar radii.
     order = np.argsort(ci, kind="stable")
     ci_sorted = ci[order]
-    r_sorted = cls[ci_sorted]
     # Fill of the lower half of the drum, one slab per radius class along the
-    # axis (y): each y-layer of a slab is a triangular lattice of equal spheres
-    # in the cross-section (spacing 2*gap*r, gap slightly below 1: neighbours
-    # start just touching), y-layers aligned, the last one filled bottom-up.
-    margin = thick * 0.5 + 1e-3 * rmin
+    # axis (y).  Each y-layer of a slab is a triangular lattice of equal
+    # spheres in the cross-section: rows at spacing a = 2r(1+gap), alternate
+    # rows offset by a/2, so that every sphere sits in the pocket of two
+    # spheres of the row below.  The pocket contacts are pre-compressed to
+    # the linear-spring equilibrium of the weight above them (W = rows above
+    # * m g shared by two contacts at 30 degrees from vertical), so the bed
+    # starts at rest and stays so; in-row and y neighbours start a hair
+    # apart and carry no load.
+    margin = thick * 0.5 + 1e-9
     rin = R - margin
+    g = 9.81
+    cos30 = math.sqrt(3.0) / 2.0
 
-    def section(r, sp):
-        # triangular lattice in the cross-section: rows at pitch sp*sqrt(3)/2
-        # from the lowest point of the drum, alternate rows offset by sp/2, so
-        # every sphere rests in a pocket of the row below
+    def rows_z(r, a, M):
         lim = rin - r
+        mg = density * 4.0 / 3.0 * math.pi * r ** 3 * g
+        z = [-lim]
+        for j in range(M - 1):
+            d = 2.0 * r - (M - 1 - j) * mg / (2.0 * cos30 * kn)
+            z.append(z[-1] + math.sqrt(d * d - 0.25 * a * a))
+        return lim, z
+
+    def section(r, a, M):
+        lim, zr = rows_z(r, a, M)
         xs, zs = [], []
-        m = 0
-        while True:
-            z = -lim + m * sp * math.sqrt(3.0) / 2.0
+        for m, z in enumerate(zr):
             if z > 0.0:
                 break
             half = math.sqrt(max(0.0, lim * lim - z * z))
-            ph = 0.5 * (m % 2)
-            k = np.arange(-int(half / sp) - 1, int(half / sp) + 2)
-            x = (k + ph) * sp
+            k = np.arange(-int(half / a) - 1, int(half / a) + 2)
+            x = (k + 0.5 * (m % 2)) * a
             x = x[np.abs(x) <= half]
             xs.append(x)
             zs.append(np.full(len(x), z))
-            m += 1
         X, Z = np.concatenate(xs), np.concatenate(zs)
         o = np.lexsort((X, Z))
         return X[o], Z[o]
@@ -237,17 +246,30 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
     for c in np.unique(ci_sorted):
         m = int(np.sum(ci_sorted == c))
         r = float(cls[c])
-        sp = 2.0 * gap * r
-        X, Z = section(r, sp)
+        a = 2.0 * (1.0 + gap) * r
+        m_full = int((rin - r) / math.sqrt(3.0 * r * r)) + 1
+        X, Z = section(r, a, m_full)
         cap = len(X)
-        for start_k in range(0, m, cap):
-            take = min(cap, m - start_k)
+        left = m
+        while left > 0:
+            take = min(cap, left)
+            if take < cap:
+                # last, partial layer: as many rows as it fills, loaded by those
+                lo, hi = 1, m_full
+                while lo < hi:
+                    mid = (lo + hi) // 2
+                    if len(section(r, a, mid)[0]) >= take:
+                        hi = mid
+                    else:
+                        lo = mid + 1
+                X, Z = section(r, a, lo)
             sl = slice(placed, placed + take)
             pos[sl, 0] = X[:take]
             pos[sl, 1] = y + r
             pos[sl, 2] = Z[:take]
             placed += take
-            y += sp
+            left -= take
+            y += a
         y += 1e-3 * rmin
     Ld = y + margin
     if L is not None:
@@ -257,7 +279,7 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
     # undo the class sort: sphere i keeps its own radius
     pos_final = np.empty_like(pos)
     pos_final[order] = pos
-    jitter = rng.uniform(3 * n, -1e-6, 1e-6).reshape(n, 3) * radii[:, None]
+    jitter = rng.uniform(3 * n, -1e-8, 1e-8).reshape(n, 3) * radii[:, None]
     pos_final += jitter
     models = []
     for k, r in enumerate(cls):
@@ -272,7 +294,7 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
                           (2.0 * R + 4.0 * thick, thick, 2.0 * R + 4.0 * thick)))
     wall_ids = []
     for k in range(sides):
-        th = 2.0 * math.pi * (k + 0.5) / sides
+        th = 2.0 * math.pi * k / sides           # plate 0 is the bottom of the drum
         c = (R * math.sin(th), 0.5 * Ld, -R * math.cos(th))
         oz = (math.sin(th), 0.0, -math.cos(th))         # outward normal
         ox = (math.cos(th), 0.0, math.sin(th))          # tangential

@@ -166,8 +166,12 @@ int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *hi, vg_stat
  * of the last vg_step call (all its kernels) and its step count. */
 int vg_kernel_timing(vg_scene *h, int max_launches);
 int vg_step_timing(vg_scene *h, double *ms, int *launches);
-/* Counters: rebuilds, relists, max neighbour-list fill. */
-int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, int *max_fill, int *capacity);
+/* Counters since vg_create: rebuilds, relists, max neighbour-list fill, the
+ * list capacity, and contact_steps, the sum over steps of the number of
+ * sphere-sphere and sphere-wall pairs left in contact by the step's solve.
+ * Any pointer may be NULL. */
+int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, int *max_fill, int *capacity,
+             long long *contact_steps);
 
 /* DOWorld::CalculateSystemEnergy (DOWorld.cpp:481-632) over the mobile DOs,
  * computed on the device from the current state.  out[16]: potential,
@@ -176,6 +180,48 @@ int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, int *max_fill
  * norm, angular momentum norm, max |v|, min |v|, max |w|, min |w|, number of
  * mobile DOs.  Sums are reduced in a fixed tree order. */
 int vg_system_energy(vg_scene *h, double *out);
+
+/* ---- Batched and served scenes ----
+ * A batch handle holds many independent scenes in one packed device layout
+ * and steps all of them with one kernel launch per step.  Each scene keeps
+ * its own DOWorld (models, IactModel table, walls, dt, gravity), grid and
+ * candidate order, so its result is the same alone and in a batch.  The
+ * scenes share UpIact; a scene is placed on the device, or leaves it, at a
+ * rebuild boundary (every UpIact batch steps), so every scene rebuilds at its
+ * own steps 0, UpIact, 2 UpIact, ... as it would alone.  No reference
+ * counterpart: the reference runs one DOWorld per process (Anne.cpp). */
+#define VG_SCENE_FREE 0
+#define VG_SCENE_QUEUED 1     /* admitted, placed at the next boundary */
+#define VG_SCENE_RUNNING 2
+#define VG_SCENE_FINISHED 3   /* reached its step count; state held until the next boundary */
+#define VG_SCENE_RETIRING 4   /* leaves the device at the next boundary */
+#define VG_SCENE_DONE 5       /* left the device; its final state is kept on the host */
+
+typedef struct {
+    long long max_spheres;     /* sphere capacity of the device layout */
+    int max_walls;             /* wall capacity */
+    int max_scenes;            /* scene slots */
+    int neighbour_capacity;    /* neighbour-list entries per sphere (<= 64; 0: 32) */
+    unsigned up_iact;          /* shared UpIact */
+} vg_batch_desc;
+
+int vg_batch_create(const vg_batch_desc *bd, vg_scene **out);
+/* Queue a scene for `steps` steps (< 0: unbounded); *scene is its slot. */
+int vg_admit(vg_scene *h, const vg_scene_desc *desc, long long steps, int *scene);
+/* Drop a scene (at the next boundary if it is on the device) or forget a
+ * kept result. */
+int vg_retire(vg_scene *h, int scene);
+int vg_scene_status(vg_scene *h, int scene, int *state, long long *steps_done, double *t);
+/* DOStatus of the scene's DOs in its DO-id order (kept on the host once the
+ * scene is done). */
+int vg_scene_get_state(vg_scene *h, int scene, double *pos, double *vel, double *ox, double *oz, double *av);
+int vg_scene_get_active(vg_scene *h, int scene, long long *n, unsigned *lo, unsigned *hi, vg_status *st);
+int vg_scene_energy(vg_scene *h, int scene, double *out);
+/* vg_set_wall_rotation for one scene of a batch (also before placement). */
+int vg_scene_set_wall_rotation(vg_scene *h, int scene, int n, const unsigned *ids, const double *axis_point,
+                               const double *axis_dir, double omega, double t_ref);
+int vg_batch_info(vg_scene *h, long long *spheres, int *scenes_on_device, int *scenes_queued,
+                  unsigned long long *steps);
 
 /* Streaming copy of `bytes` bytes, `iters` times, on the current device;
  * returns the achieved read+write bandwidth in GB/s.  Diagnostics: measures

@@ -119,3 +119,22 @@ def test_gpu_system_energy_matches_oracle():
         gpu.step(20)
         cpu.step(20)
     gpu.close()
+
+
+def test_gpu_contact_counter_matches_oracle():
+    """vg_stats' contact_steps: the pairs left in contact by each step's solve,
+    summed over the steps, equal the oracle's active counts at the start of
+    each step (ISwBSDBF: the solve keeps exactly the detected contacts)."""
+    from oracle.oracle import OracleSim
+    from vedo_amd.gpu import GpuSimulation
+    z, wa = load_golden("box_bsdbf")
+    gpu, cpu = GpuSimulation(wa), OracleSim(wa)
+    expect = 0
+    for _ in range(60):
+        expect += cpu.active()[0].size
+        cpu.step(1)
+    gpu.step(60)
+    got = gpu.stats()["contact_steps"]
+    print(f"\n  contact-steps over 60 steps: gpu {got} oracle {expect}", flush=True)
+    assert got == expect
+    gpu.close()

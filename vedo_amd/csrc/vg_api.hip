@@ -1,9 +1,15 @@
 // vg_api.hip -- host side of libvedo_hip.so: the C-ABI of include/vedo_gpu.h.
 //
 // Plays the role of VEDO's SimMediator (SimMediator.cpp:640-842) and
-// LeapConsultant (LeapConsultant.cpp:49-195) for one scene on one GPU:
-// holds the device state, orders the per-step kernels on its own stream, and
-// runs the candidate-list rebuild every up_iact steps.
+// LeapConsultant (LeapConsultant.cpp:49-195) on one GPU for a batch of
+// independent scenes: holds the device state of every scene in one packed
+// layout, orders the per-step kernels on its own stream (one step kernel for
+// all scenes), and runs the candidate-list rebuild every up_iact steps.
+// vg_create makes a batch of one scene; vg_batch_create / vg_admit make the
+// batched and served forms.  A scene's arithmetic does not depend on the
+// other scenes of its batch: its grid, IactModel table, walls, time step and
+// rebuild cadence are its own, and its per-sphere sums follow its own
+// candidate order, so its result is the same alone and in a batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,54 +31,83 @@ using namespace vg;
 
 static std::string g_create_error;
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
+// Host record of one scene slot.
+struct SceneHost {
+    int state = VG_SCENE_FREE;
+    // DO bookkeeping (DO ids are the scene's own)
+    int n_do = 0, n = 0, nw = 0;
+    std::vector<unsigned> sphere_ids;   // rank -> DO id
+    std::vector<unsigned> wall_ids;     // wall order -> DO id
+    std::vector<int> do_to_sphere;      // DO id -> rank or -1
+    std::vector<int> do_to_wall;        // DO id -> wall slot (batch-wide) or -1
+    int w_begin = 0, law_off = 0, n_groups = 0, model_off = 0, n_models = 0;
+    double dt = 0, t = 0, t_stop = 0, g[3] = {0, 0, 0}, sf = 1.1, skin = 0, bond_reach = 0, rmin = 0;
+    bool sd_only = true, single_group = true, no_bond = true;
+    long long steps_done = 0, steps_target = -1;
+    int seg_begin = 0, seg_end = 0;     // slots of the spheres in the current layout
+    // initial state until the scene is placed
+    std::vector<double4> pos0, vel0, avl0;
+    std::vector<double> ori0;
+    std::vector<int> mdl0;
+    std::vector<WallDev> walls0;
+    std::vector<WallKin> kin0;          // rotations prescribed before placement
+    std::vector<char> kin0_set;
+    // result kept on the host once a finished scene leaves the device
+    std::vector<double> result;         // [n_do][21]
 };
 
 struct vg_scene {
     std::string err;
     hipStream_t stream = nullptr;
     int device = 0;
-    // sizes
-    int n = 0, nw = 0, n_do = 0, ki = 48, n_groups = 0;
-    // host mirrors
-    std::vector<int> do_is_sphere;         // DO id -> 1 sphere / 0 wall
-    std::vector<unsigned> wall_ids;        // wall index -> DO id
-    std::vector<unsigned> sphere_ids;      // sphere rank -> DO id
-    std::vector<int> do_to_sphere;         // DO id -> sphere rank or -1
-    unsigned *d_sphere_ids = nullptr, *d_wall_ids = nullptr;
-    std::vector<int> do_to_wall;           // DO id -> wall index or -1
-    std::vector<WallDev> walls_h;
+    bool batch = false;                 // made by vg_batch_create
+    // capacities and sizes
+    int ld = 0, wcap = 0, scap = 0, ki = 48;
+    int n = 0;                          // occupied sphere slots (placed scenes)
+    int nw_used = 0;                    // wall slots up to the last one in use
+    std::vector<SceneHost> sc;          // scene slots
+    std::vector<SceneDev> sdev;         // host copy of the device scene table
     std::vector<ModelDev> models_h;
+    std::vector<double2> mm_h;          // per model: mass, mass moment of inertia (spheres)
     std::vector<LawDev> laws_h;
-    bool sd_only = false;      // every law is of the spring-dashpot family
-    int single_group = 0, no_bond = 1;
-    // scalars
-    double dt = 0, t = 0, t_stop = 0, g[3] = {0, 0, 0}, sf = 1.1, skin = 0, bond_reach = 0;
-    unsigned up_iact = 1, record = 0;
+    std::vector<WallDev> walls_h;       // host copy of wall kinematic modes (kin)
+    std::vector<char> wall_used;
+    // batch-wide properties of the placed scenes
+    bool sd_only = true;
+    int single_group = 1, no_bond = 1;
+    unsigned up_iact = 1;
     unsigned long long round = 0, step_count = 0;
     long long rebuilds = 0;
     long long n_iact = 0;
+    bool record = true;                 // per-entry ImpactStatus records (one-scene handles)
     // device
     Geo geo[2];
     int cur = 0;
     Ell ell[2];
     int ecur = 0;
     ModelDev *models = nullptr;
+    double2 *mm = nullptr;
+    int models_cap = 0;
     LawDev *laws = nullptr;
+    int laws_cap = 0;
+    SceneDev *scenes = nullptr;
+    double *stime = nullptr;            // [scap][2] scene times for the wall kernel
+    int *rflags = nullptr;              // [scap][3] relist rings
     WallDev *walls = nullptr, *walls_ref = nullptr;
     WallKin *wkin = nullptr;
     double *assigned = nullptr;
-    double *assigned_sph = nullptr;   // [n][16] assigned sphere statuses by rank (allocated on first use)
-    double2 *mm = nullptr;            // per model: mass, mass moment of inertia (spheres)
+    double *assigned_sph = nullptr;     // [ld][16] assigned sphere statuses by rank (allocated on first use)
     double *en_part = nullptr, *en_out = nullptr;
     int *flags = nullptr;
     unsigned long long *relists = nullptr;
-    unsigned long long *red = nullptr, *bpart = nullptr;
+    unsigned long long *contacts = nullptr;   // per-wavefront contact-pair counts, summed by vg_stats
+    unsigned long long *red = nullptr;  // [scap][8] bounds
     unsigned *inv = nullptr, *perm = nullptr, *svals = nullptr;
     unsigned long long *keys = nullptr, *skeys = nullptr;
     double *xs[2] = {nullptr, nullptr}, *ys[2] = {nullptr, nullptr}, *zs[2] = {nullptr, nullptr};
+    long long *qidx = nullptr, *segs = nullptr;
+    unsigned *sid[2] = {nullptr, nullptr};   // scene slot of each sphere (quantile sorts)
+    double *qval = nullptr;
     double *safe = nullptr;
     int *lin = nullptr;
     int *cstart = nullptr, *cend = nullptr;
@@ -84,7 +119,7 @@ struct vg_scene {
     double *rec = nullptr, *rec_imp = nullptr, *rec_imp2 = nullptr;
     bool rec_valid = false;   // rec_imp holds the last step's impulses
     bool ent_valid = false;   // rec (per-entry ImpactStatus fields) matches the current lists
-    // IactRecordTab snapshot taken before a rebuild re-sorts the spheres
+    // IactRecordTab snapshot taken before a rebuild re-sorts the spheres (one-scene handles)
     ActiveOut *snap = nullptr;
     size_t snap_cap = 0;
     long long snap_count = 0;
@@ -143,126 +178,588 @@ static int ensure_tmp(vg_scene *h, size_t bytes)
     return 0;
 }
 
-static Tables tables(vg_scene *h) { return Tables{h->models, h->laws, h->n_groups}; }
+static Tables tables(vg_scene *h) { return Tables{h->models, h->laws}; }
+
+static double model_range(const vg_model &m)
+{
+    // DOModel::Analysis (DOModel.cpp:17-149), global walls
+    if (m.shape == VG_SPHERE) return m.attr[0];
+    if (m.shape == VG_QUASICYLINDER) return m.scope == VG_LOCAL ? std::max(m.attr[0], 0.5 * m.attr[1]) : m.attr[0];
+    if (m.shape == VG_QUASIPLATE) {
+        if (m.scope == VG_LOCAL) return 0.5 * std::max(std::max(m.attr[0], m.attr[1]), m.attr[2]);
+        return 0.5 * m.attr[1];
+    }
+    return 0.0;
+}
+
+static const double PI = 4.0 * std::atan(1.0);
+
+// ---------------------------------------------------------------------------
+// Device tables
+// ---------------------------------------------------------------------------
+
+static int upload_scenes(vg_scene *h)
+{
+    HIPCHK(hipMemcpyAsync(h->scenes, h->sdev.data(), sizeof(SceneDev) * h->scap, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));   // the host table is reused right away
+    return 0;
+}
+
+template <class T> static int grow_table(vg_scene *h, T **d, int *cap, const std::vector<T> &host)
+{
+    if ((int)host.size() > *cap) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (*d) HIPCHK(hipFree(*d));
+        *d = nullptr;
+        *cap = (int)host.size() * 2 + 16;
+        if (dalloc(h, d, (size_t)*cap)) return VG_ENOMEM;
+    }
+    if (!host.empty()) HIPCHK(hipMemcpy(*d, host.data(), sizeof(T) * host.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// batch-wide kernel specialisations follow the placed scenes
+static void update_batch_flags(vg_scene *h)
+{
+    h->sd_only = true;
+    h->single_group = 1;
+    h->no_bond = 1;
+    for (const SceneHost &S : h->sc) {
+        if (S.state != VG_SCENE_RUNNING && S.state != VG_SCENE_FINISHED) continue;
+        h->sd_only = h->sd_only && S.sd_only;
+        h->single_group = h->single_group && S.single_group;
+        h->no_bond = h->no_bond && S.no_bond;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scene description -> host scene (validation as the reference's setup)
+// ---------------------------------------------------------------------------
+
+static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, std::string &err)
+{
+    if (d->consultant != 0) {
+        err = "only LeapConsultant candidate-list rules run on the GPU";
+        return VG_EUNSUPPORTED;
+    }
+    if (d->n_do <= 0 || d->n_models <= 0 || d->n_groups <= 0 || !d->models || !d->iact || !d->model_of ||
+        !d->pos || !d->vel || !d->ox || !d->oz || !d->av) {
+        err = "incomplete scene description";
+        return VG_EINVAL;
+    }
+    S = SceneHost();
+    S.n_do = d->n_do;
+    S.n_groups = d->n_groups;
+    S.n_models = d->n_models;
+    S.do_to_sphere.assign(d->n_do, -1);
+    S.do_to_wall.assign(d->n_do, -1);
+    for (int i = 0; i < d->n_do; ++i) {
+        const int m = d->model_of[i];
+        if (m < 0 || m >= d->n_models) {
+            err = "DOStatus names an unknown DOModel";
+            return VG_EINVAL;
+        }
+        const vg_model &M = d->models[m];
+        if (M.density < 0.0) {
+            err = "negative density";
+            return VG_EUNSUPPORTED;
+        }
+        if (M.group < 0 || M.group >= d->n_groups) {
+            err = "DOModel names an unknown DOGroup";
+            return VG_EINVAL;
+        }
+        if (M.shape == VG_SPHERE) {
+            if (M.scope != VG_LOCAL || M.behavior != VG_MOBILE || M.density == 0.0) {
+                err = "spheres must be local, mobile and of positive density";
+                return VG_EUNSUPPORTED;
+            }
+            S.do_to_sphere[i] = (int)S.sphere_ids.size();
+            S.sphere_ids.push_back((unsigned)i);
+        } else if (M.shape == VG_QUASIPLATE || M.shape == VG_QUASICYLINDER) {
+            if (M.scope != VG_GLOBAL || (M.behavior != VG_FIXED && M.behavior != VG_ORBITAL)) {
+                err = "walls must be global and fixed or orbital";
+                return VG_EUNSUPPORTED;
+            }
+            S.do_to_wall[i] = (int)S.wall_ids.size();   // wall order; the slot offset is added at admission
+            S.wall_ids.push_back((unsigned)i);
+        } else {
+            err = "shape is not on the GPU path";
+            return VG_EUNSUPPORTED;
+        }
+    }
+    S.n = (int)S.sphere_ids.size();
+    S.nw = (int)S.wall_ids.size();
+    if (S.n == 0) {
+        err = "scene has no spheres";
+        return VG_EINVAL;
+    }
+    double bond_reach = 0.0;
+    for (int k = 0; k < d->n_groups * d->n_groups; ++k) {
+        const int L = d->iact[k].law;
+        if (L < 0 || L > 5) {
+            err = "unknown impact solver";
+            return VG_EINVAL;
+        }
+        if ((L == VG_BSDBF || L == VG_BSDB2F) && d->iact[k].p[4] != 0.0 && d->iact[k].p[0] != 0.0)
+            bond_reach = std::max(bond_reach, std::fabs(d->iact[k].p[4] / d->iact[k].p[0]));
+        if (L == VG_H || L == VG_HERTZ) S.sd_only = false;
+        if ((L == VG_BSDBF || L == VG_BSDB2F || L == VG_BTSDBF) && d->iact[k].p[4] != 0.0) S.no_bond = false;
+    }
+    {
+        int g0 = -1;
+        for (unsigned id : S.sphere_ids) {
+            const int g = d->models[d->model_of[id]].group;
+            if (g0 < 0) g0 = g;
+            else if (g != g0) S.single_group = false;
+        }
+    }
+    S.dt = d->dt; S.t = d->t_current; S.t_stop = d->t_stop;
+    for (int c = 0; c < 3; ++c) S.g[c] = d->g[c];
+    S.sf = d->safety_factor;
+    double rmin = 1e300;
+    for (unsigned id : S.sphere_ids) rmin = std::min(rmin, d->models[d->model_of[id]].attr[0]);
+    S.rmin = rmin;
+    S.skin = d->inner_skin > 0 ? d->inner_skin : 0.1 * rmin;
+    S.skin = std::max(S.skin, bond_reach * 1.01);
+    S.bond_reach = bond_reach;
+    // initial state (model indices are made batch-wide at admission)
+    const int n = S.n;
+    S.pos0.resize(n); S.vel0.resize(n); S.avl0.resize(n); S.ori0.resize(6 * (size_t)n); S.mdl0.resize(n);
+    for (int s = 0; s < n; ++s) {
+        const unsigned id = S.sphere_ids[s];
+        const vg_model &M = d->models[d->model_of[id]];
+        const double r = M.attr[0];
+        const double mass = M.density * ((4.0 / 3.0 * PI) * r * r * r);   // DOModel.cpp:22-27
+        const double sudo = M.density_factor * mass;
+        S.pos0[s] = make_double4(d->pos[3 * id], d->pos[3 * id + 1], d->pos[3 * id + 2], r);
+        S.vel0[s] = make_double4(d->vel[3 * id], d->vel[3 * id + 1], d->vel[3 * id + 2], 0.0);
+        S.avl0[s] = make_double4(d->av[3 * id], d->av[3 * id + 1], d->av[3 * id + 2], sudo);
+        for (int c = 0; c < 3; ++c) {
+            S.ori0[(size_t)c * n + s] = d->ox[3 * id + c];
+            S.ori0[(size_t)(3 + c) * n + s] = d->oz[3 * id + c];
+        }
+        S.mdl0[s] = d->model_of[id];
+    }
+    S.walls0.resize(S.nw);
+    for (int w = 0; w < S.nw; ++w) {
+        const unsigned id = S.wall_ids[w];
+        const vg_model &M = d->models[d->model_of[id]];
+        WallDev &W = S.walls0[w];
+        W = WallDev();
+        W.pos = mk(d->pos[3 * id], d->pos[3 * id + 1], d->pos[3 * id + 2]);
+        W.vel = mk(d->vel[3 * id], d->vel[3 * id + 1], d->vel[3 * id + 2]);
+        W.ox = mk(d->ox[3 * id], d->ox[3 * id + 1], d->ox[3 * id + 2]);
+        W.oz = mk(d->oz[3 * id], d->oz[3 * id + 1], d->oz[3 * id + 2]);
+        W.av = mk(d->av[3 * id], d->av[3 * id + 1], d->av[3 * id + 2]);
+        W.a0 = M.attr[0]; W.a1 = M.attr[1]; W.a2 = M.attr[2];
+        W.range = model_range(M);
+        W.reach = M.shape == VG_QUASIPLATE ? 0.5 * std::sqrt(M.attr[0] * M.attr[0] + M.attr[1] * M.attr[1] +
+                                                             M.attr[2] * M.attr[2])
+                                           : std::sqrt(M.attr[0] * M.attr[0] + 0.25 * M.attr[1] * M.attr[1]);
+        W.path = 0.0;
+        W.shape = M.shape; W.group = M.group; W.behavior = M.behavior; W.kin = 0;
+    }
+    (void)h;
+    return 0;
+}
+
+// Reserve table entries, wall slots and a scene slot; the scene is placed on
+// the device at the next rebuild boundary.
+static int admit(vg_scene *h, const vg_scene_desc *d, long long steps, int *slot_out)
+{
+    SceneHost S;
+    std::string err;
+    int rc = scene_from_desc(h, d, S, err);
+    if (rc) return fail(h, rc, err);
+    if (d->up_iact != h->up_iact && !(d->up_iact == 0 && h->up_iact == 1))
+        return fail(h, VG_EUNSUPPORTED, "the scenes of a batch share one UpIact");
+    int slot = -1;
+    for (int k = 0; k < h->scap; ++k)
+        if (h->sc[k].state == VG_SCENE_FREE) {
+            slot = k;
+            break;
+        }
+    if (slot < 0) return fail(h, VG_ECAPACITY, "no free scene slot");
+    long long spheres = 0;
+    for (const SceneHost &T : h->sc)
+        if (T.state == VG_SCENE_QUEUED || T.state == VG_SCENE_RUNNING || T.state == VG_SCENE_FINISHED)
+            spheres += T.n;
+    if (spheres + S.n > h->ld) return fail(h, VG_ECAPACITY, "sphere capacity of the batch exceeded");
+    // a contiguous run of free wall slots (first fit)
+    int w_begin = -1;
+    for (int w = 0, run = 0; w < h->wcap && S.nw > 0; ++w) {
+        run = h->wall_used[w] ? 0 : run + 1;
+        if (run == S.nw) {
+            w_begin = w - S.nw + 1;
+            break;
+        }
+    }
+    if (S.nw > 0 && w_begin < 0) return fail(h, VG_ECAPACITY, "wall capacity of the batch exceeded");
+    if (S.nw == 0) w_begin = 0;
+    if ((long long)h->ld + h->wcap > (long long)PARTNER_MASK)
+        return fail(h, VG_EUNSUPPORTED, "too many DOs for the neighbour-list encoding");
+    // tables
+    S.model_off = (int)h->models_h.size();
+    for (int m = 0; m < d->n_models; ++m) {
+        ModelDev md;
+        md.group = d->models[m].group;
+        md.scene = slot;
+        for (int c = 0; c < 3; ++c) md.fext[c] = d->models[m].fext[c];
+        h->models_h.push_back(md);
+        double2 q = make_double2(0.0, 0.0);
+        if (d->models[m].shape == VG_SPHERE) {
+            const double r = d->models[m].attr[0];
+            const double mass = d->models[m].density * ((4.0 / 3.0 * PI) * r * r * r);
+            q = make_double2(mass, 0.4 * mass * r * r);
+        }
+        h->mm_h.push_back(q);
+    }
+    S.law_off = (int)h->laws_h.size();
+    for (int k = 0; k < d->n_groups * d->n_groups; ++k) {
+        LawDev L;
+        L.law = d->iact[k].law;
+        L.pad = 0;
+        for (int p = 0; p < 8; ++p) L.p[p] = d->iact[k].p[p];
+        h->laws_h.push_back(L);
+    }
+    for (int &m : S.mdl0) m += S.model_off;
+    S.w_begin = w_begin;
+    for (int w = 0; w < S.nw; ++w) {
+        S.walls0[w].scene = slot;
+        h->wall_used[w_begin + w] = 1;
+    }
+    for (int &w : S.do_to_wall)
+        if (w >= 0) w += w_begin;
+    // tables: models (with DOModel::Analysis mass / inertia, DOModel.cpp:22-29) and laws
+    const int models_cap = h->models_cap;
+    rc = grow_table(h, &h->models, &h->models_cap, h->models_h);
+    if (!rc && h->models_cap != models_cap) {
+        if (h->mm) HIPCHK(hipFree(h->mm));
+        h->mm = nullptr;
+        rc = dalloc(h, &h->mm, (size_t)h->models_cap);
+    }
+    if (!rc) HIPCHK(hipMemcpy(h->mm, h->mm_h.data(), sizeof(double2) * h->mm_h.size(), hipMemcpyHostToDevice));
+    if (!rc) rc = grow_table(h, &h->laws, &h->laws_cap, h->laws_h);
+    if (rc) return rc;
+    S.steps_target = steps;
+    S.state = VG_SCENE_QUEUED;
+    h->sc[slot] = std::move(S);
+    *slot_out = slot;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Rebuild boundary: place queued scenes, drop finished ones, rebuild
+// ---------------------------------------------------------------------------
+
+static int fetch_scene_state(vg_scene *h, int slot, std::vector<double> &hb);
+
+static int place_queued(vg_scene *h)
+{
+    Geo &G = h->geo[h->cur];
+    bool any = false;
+    for (int k = 0; k < h->scap; ++k) {
+        SceneHost &S = h->sc[k];
+        if (S.state != VG_SCENE_QUEUED) continue;
+        const int b = h->n, n = S.n, ld = h->ld;
+        std::vector<unsigned> oid(n);
+        for (int s = 0; s < n; ++s) oid[s] = (unsigned)s;
+        HIPCHK(hipMemcpy(G.pos + b, S.pos0.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(G.vel + b, S.vel0.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(G.avl + b, S.avl0.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+        for (int c = 0; c < 6; ++c)
+            HIPCHK(hipMemcpy(G.ori + (size_t)c * ld + b, S.ori0.data() + (size_t)c * n, sizeof(double) * n,
+                             hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(G.oid + b, oid.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(G.mdl + b, S.mdl0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_hdr_zero, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->ell[h->ecur].hdr, b, b + n);
+        if (S.nw) {
+            if (!S.kin0.empty()) {
+                for (int w = 0; w < S.nw; ++w)
+                    if (S.kin0_set[w]) S.walls0[w].kin = 2;
+                HIPCHK(hipMemcpy(h->wkin + S.w_begin, S.kin0.data(), sizeof(WallKin) * S.nw, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(h->walls_ref + S.w_begin, S.walls0.data(), sizeof(WallDev) * S.nw,
+                                 hipMemcpyHostToDevice));
+            }
+            HIPCHK(hipMemcpy(h->walls + S.w_begin, S.walls0.data(), sizeof(WallDev) * S.nw, hipMemcpyHostToDevice));
+            for (int w = 0; w < S.nw; ++w) h->walls_h[S.w_begin + w] = S.walls0[w];
+            h->nw_used = std::max(h->nw_used, S.w_begin + S.nw);
+        }
+        const double t2[2] = {S.t, S.t};
+        HIPCHK(hipMemcpy(h->stime + 2 * k, t2, sizeof(t2), hipMemcpyHostToDevice));
+        const int z3[3] = {0, 0, 0};
+        HIPCHK(hipMemcpy(h->rflags + 3 * k, z3, sizeof(z3), hipMemcpyHostToDevice));
+        SceneDev &D = h->sdev[k];
+        D = SceneDev();
+        D.dt = S.dt; D.gx = S.g[0]; D.gy = S.g[1]; D.gz = S.g[2];
+        D.skin = S.skin; D.relist_half = 0.5 * S.skin * (1.0 - 1e-9);
+        D.law_off = S.law_off; D.n_groups = S.n_groups;
+        D.w_begin = S.w_begin; D.w_end = S.w_begin + S.nw;
+        D.active = 1;
+        S.seg_begin = b;
+        S.seg_end = b + n;
+        S.state = VG_SCENE_RUNNING;
+        S.pos0.clear(); S.vel0.clear(); S.avl0.clear(); S.ori0.clear(); S.mdl0.clear();
+        S.pos0.shrink_to_fit(); S.vel0.shrink_to_fit(); S.avl0.shrink_to_fit(); S.ori0.shrink_to_fit();
+        S.mdl0.shrink_to_fit();
+        h->n += n;
+        any = true;
+    }
+    if (any) update_batch_flags(h);
+    return 0;
+}
 
 // LeapConsultant::Reset + subReset (LeapConsultant.cpp:108-195,
-// LeapConsultant.h:57-514) followed by SimMediator::Initiate's history restore.
+// LeapConsultant.h:57-514) for every placed scene, followed by
+// SimMediator::Initiate's history restore.  Finished scenes leave the device
+// here (their final state is kept on the host).
 static int rebuild(vg_scene *h)
 {
-    const int n = h->n;
     hipStream_t st = h->stream;
+    // finished scenes: keep the result, retire
+    bool retiring = false;
+    for (int k = 0; k < h->scap; ++k) {
+        SceneHost &S = h->sc[k];
+        if (S.state == VG_SCENE_FINISHED && h->batch) {
+            if (fetch_scene_state(h, k, S.result)) return VG_EHIP;
+            S.state = VG_SCENE_RETIRING;
+        }
+        if (S.state == VG_SCENE_RETIRING) {
+            h->sdev[k].active = -1;
+            retiring = true;
+        }
+    }
+    if (place_queued(h)) return VG_EHIP;
+    const int n_total = h->n;
+    // rank offsets: live scenes by slot, then the retiring ones
+    int n_live = 0, max_slot = -1;
+    std::vector<int> live;
+    for (int k = 0; k < h->scap; ++k) {
+        SceneHost &S = h->sc[k];
+        if (S.state == VG_SCENE_RUNNING || S.state == VG_SCENE_FINISHED) {
+            h->sdev[k].rank_off = n_live;
+            n_live += S.n;
+            live.push_back(k);
+            max_slot = k;
+        }
+    }
+    {
+        int r = n_live;
+        for (int k = 0; k < h->scap; ++k)
+            if (h->sc[k].state == VG_SCENE_RETIRING) {
+                h->sdev[k].rank_off = r;
+                r += h->sc[k].n;
+            }
+    }
+    if (upload_scenes(h)) return VG_EHIP;
     Geo &G0 = h->geo[h->cur];
     Geo &G1 = h->geo[h->cur ^ 1];
-    // bounds
-    const int nb = (int)nblk(n, 256);
-    hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(256), 0, st, G0, n, h->bpart, h->inv, h->xs[0], h->ys[0],
-                       h->zs[0]);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, (const unsigned long long *)h->bpart, nb, h->red);
-    // 2%-trimmed spans pick the comparator (LeapConsultant.cpp:139-191)
-    size_t need = 0;
-    (void)rocprim::radix_sort_keys(nullptr, need, h->xs[0], h->xs[1], (size_t)n, 0, 64, st);
-    if (ensure_tmp(h, need)) return VG_EHIP;
-    HIPCHK(rocprim::radix_sort_keys(h->tmp, need, h->xs[0], h->xs[1], (size_t)n, 0, 64, st));
-    HIPCHK(rocprim::radix_sort_keys(h->tmp, need, h->ys[0], h->ys[1], (size_t)n, 0, 64, st));
-    HIPCHK(rocprim::radix_sort_keys(h->tmp, need, h->zs[0], h->zs[1], (size_t)n, 0, 64, st));
-    unsigned long long red[8];
-    HIPCHK(hipMemcpyAsync(red, h->red, sizeof(red), hipMemcpyDeviceToHost, st));
-    const double discard = 0.02, keep = 1.0 - discard;
-    const size_t ihi = (size_t)(keep * n - 1), ilo = (size_t)(discard * n);
-    double q[6];
-    HIPCHK(hipMemcpyAsync(&q[0], h->xs[1] + ilo, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&q[1], h->xs[1] + ihi, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&q[2], h->ys[1] + ilo, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&q[3], h->ys[1] + ihi, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&q[4], h->zs[1] + ilo, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&q[5], h->zs[1] + ihi, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    const double xmax = from_ord(red[0]), ymax = from_ord(red[1]), zmax = from_ord(red[2]);
-    const double xmin = from_ord(~red[3]), ymin = from_ord(~red[4]), zmin = from_ord(~red[5]);
-    const double vmax = from_ord(red[6]), rmax = from_ord(red[7]);
-    const double xspan = q[1] - q[0], yspan = q[3] - q[2], zspan = q[5] - q[4];
-    GridDev gd;
-    // TrirLeX (x,y,z), TrirLeY (y,x,z), TrirLeZ (z,x,y) (NBSConsultant.h:52-118)
-    int perm[3];
-    if (xspan >= yspan) {
-        if (xspan >= zspan) { perm[0] = 0; perm[1] = 1; perm[2] = 2; }
-        else { perm[0] = 2; perm[1] = 0; perm[2] = 1; }
-    } else {
-        if (yspan >= zspan) { perm[0] = 1; perm[1] = 0; perm[2] = 2; }
-        else { perm[0] = 2; perm[1] = 0; perm[2] = 1; }
+    const int S_live = (int)live.size();
+    if (n_total > 0) {
+        // bounds and quantile sorts over the current layout
+        HIPCHK(hipMemsetAsync(h->red, 0, sizeof(unsigned long long) * 8 * h->scap, st));
+        hipLaunchKernelGGL(k_bounds, dim3(nblk(n_total, 256)), dim3(256), 0, st, G0, n_total,
+                           (const SceneDev *)h->scenes, (const ModelDev *)h->models, h->red, h->inv, h->xs[0],
+                           h->ys[0], h->zs[0], h->sid[0], (unsigned)(max_slot + 1));
     }
-    const double dtu = h->up_iact * h->dt;
-    const double glen = olen(h->g);
-    const double zr = h->sf * (2.0 * rmax + dtu * vmax + 0.5 * dtu * dtu * glen);
-    unsigned ncx = (unsigned)std::ceil((xmax - xmin) / zr); ncx = std::max(ncx, 1u);
-    unsigned ncy = (unsigned)std::ceil((ymax - ymin) / zr); ncy = std::max(ncy, 1u);
-    unsigned ncz = (unsigned)std::ceil((zmax - zmin) / zr); ncz = std::max(ncz, 1u);
-    if (ncx >= (1u << 21) || ncy >= (1u << 21) || ncz >= (1u << 21))
-        return fail(h, VG_EUNSUPPORTED, "cell grid exceeds 2^21 cells per axis");
-    gd.xmin = xmin; gd.ymin = ymin; gd.zmin = zmin; gd.zr = zr;
-    gd.safe_v = dtu;
-    gd.safe_g = 0.5 * dtu * dtu * glen;
-    gd.sf = h->sf;
-    gd.nx = (int)ncx + 3; gd.ny = (int)ncy + 3; gd.nz = (int)ncz + 3;
-    gd.perm[0] = perm[0]; gd.perm[1] = perm[1]; gd.perm[2] = perm[2];
-    const long long cells = (long long)gd.nx * gd.ny * gd.nz;
-    if (cells > (1ll << 31)) return fail(h, VG_EUNSUPPORTED, "cell grid too large (sparse domain)");
-    if (cells > h->cells_cap) {
-        if (h->cstart) HIPCHK(hipFree(h->cstart));
-        if (h->cend) HIPCHK(hipFree(h->cend));
-        h->cells_cap = cells + cells / 4;
-        if (dalloc(h, &h->cstart, h->cells_cap) || dalloc(h, &h->cend, h->cells_cap)) return VG_ENOMEM;
-    }
-    hipLaunchKernelGGL(k_fill_int, dim3(nblk(cells, 256)), dim3(256), 0, st, h->cstart, cells, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(nblk(cells, 256)), dim3(256), 0, st, h->cend, cells, 0);
-    // stable sort by (comparator cell key), DO-id order inside a cell
-    hipLaunchKernelGGL(k_cellkey, dim3(nblk(n, 256)), dim3(256), 0, st, G0, n, gd, h->inv, h->keys, h->svals);
-    need = 0;
-    (void)rocprim::radix_sort_pairs(nullptr, need, h->keys, h->skeys, h->svals, h->perm, (size_t)n, 0, 63, st);
-    if (ensure_tmp(h, need)) return VG_EHIP;
-    HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, h->keys, h->skeys, h->svals, h->perm, (size_t)n, 0, 63, st));
-    hipLaunchKernelGGL(k_reorder, dim3(nblk(n, 256)), dim3(256), 0, st, G0, G1, n, h->perm, gd, h->safe, h->lin,
-                       h->rec_valid ? (const double *)h->rec_imp : (const double *)nullptr, h->rec_imp2);
-    if (h->rec_valid) std::swap(h->rec_imp, h->rec_imp2);
-    hipLaunchKernelGGL(k_cellstart, dim3(nblk(n, 256)), dim3(256), 0, st, h->lin, n, h->cstart, h->cend);
-    // candidate list in one pass; grow the capacity and redo if a sphere overflowed
-    OuterArgs oa;
-    oa.g = G1; oa.n = n; oa.nw = h->nw; oa.G = gd; oa.cstart = h->cstart; oa.cend = h->cend; oa.safe = h->safe;
-    oa.walls = h->walls; oa.t = tables(h); oa.count = h->ocount; oa.nbr = h->onbr; oa.ko = h->ko;
-    oa.single_group = h->single_group; oa.flags = h->flags;
-    for (;;) {
-        HIPCHK(hipMemsetAsync(h->flags + 5, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_outer, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
-        int maxc = 0;
-        HIPCHK(hipMemcpyAsync(&maxc, h->flags + 5, sizeof(int), hipMemcpyDeviceToHost, st));
+    std::vector<GridDev> grids(h->scap);
+    std::vector<long long> cell_off(h->scap, 0);
+    long long cells = 0;
+    if (S_live > 0) {
+        // 2%-trimmed spans pick the comparator (LeapConsultant.cpp:139-191):
+        // each coordinate sorted over all spheres, then stably by scene slot,
+        // which leaves every live scene's coordinates sorted in its rank range
+        std::vector<long long> qi(2 * S_live);
+        const double discard = 0.02, keep = 1.0 - discard;
+        for (int q = 0; q < S_live; ++q) {
+            const SceneHost &S = h->sc[live[q]];
+            const double n = (double)S.n;
+            const long long base = h->sdev[live[q]].rank_off;
+            // (size_t)(discard*n) and (size_t)(keep*n - 1) as the reference indexes, clamped to the scene
+            const long long ilo = std::min((long long)(discard * n), (long long)S.n - 1);
+            const long long ihi = std::max(0ll, std::min((long long)std::floor(keep * n - 1.0), (long long)S.n - 1));
+            qi[2 * q] = base + ilo;
+            qi[2 * q + 1] = base + ihi;
+        }
+        HIPCHK(hipMemcpy(h->qidx, qi.data(), sizeof(long long) * qi.size(), hipMemcpyHostToDevice));
+        const bool by_scene = S_live > 1 || retiring;
+        int sbits = 1;
+        while ((1 << sbits) <= max_slot + 1) ++sbits;
+        size_t need = 0, need2 = 0;
+        (void)rocprim::radix_sort_pairs(nullptr, need, h->xs[0], h->xs[1], h->sid[0], h->sid[1], (size_t)n_total,
+                                        0, 64, st);
+        (void)rocprim::radix_sort_pairs(nullptr, need2, h->sid[1], h->sid[0], h->xs[1], h->xs[0], (size_t)n_total,
+                                        0, sbits, st);
+        if (ensure_tmp(h, std::max(need, need2))) return VG_EHIP;
+        double *coord[3][2] = {{h->xs[0], h->xs[1]}, {h->ys[0], h->ys[1]}, {h->zs[0], h->zs[1]}};
+        for (int c = 0; c < 3; ++c) {
+            if (by_scene) {
+                // sid[0] holds the slots in layout order; keep it for the next coordinate
+                HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, coord[c][0], coord[c][1], h->sid[0],
+                                                 (unsigned *)h->keys, (size_t)n_total, 0, 64, st));
+                HIPCHK(rocprim::radix_sort_pairs(h->tmp, need2, (unsigned *)h->keys, (unsigned *)h->skeys,
+                                                 coord[c][1], coord[c][0], (size_t)n_total, 0, sbits, st));
+            } else {
+                HIPCHK(rocprim::radix_sort_keys(h->tmp, need, coord[c][0], coord[c][1], (size_t)n_total, 0, 64, st));
+                HIPCHK(hipMemcpyAsync(coord[c][0], coord[c][1], sizeof(double) * n_total, hipMemcpyDeviceToDevice,
+                                      st));
+            }
+        }
+        hipLaunchKernelGGL(k_quantiles, dim3(nblk(S_live, 64)), dim3(64), 0, st, (const double *)h->xs[0],
+                           (const double *)h->ys[0], (const double *)h->zs[0], (const long long *)h->qidx, S_live,
+                           h->qval);
+        std::vector<unsigned long long> red((size_t)8 * h->scap);
+        std::vector<double> q6((size_t)6 * S_live);
+        HIPCHK(hipMemcpyAsync(red.data(), h->red, sizeof(unsigned long long) * red.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(q6.data(), h->qval, sizeof(double) * q6.size(), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (maxc <= h->ko) break;
-        HIPCHK(hipFree(h->onbr));
-        h->onbr = nullptr;
-        h->ko = maxc + maxc / 4 + 8;
-        if (dalloc(h, &h->onbr, (size_t)h->ko * n)) return VG_ENOMEM;
-        oa.nbr = h->onbr;
-        oa.ko = h->ko;
+        for (int q = 0; q < S_live; ++q) {
+            const int k = live[q];
+            const SceneHost &S = h->sc[k];
+            const unsigned long long *rr = &red[(size_t)8 * k];
+            const double xmax = from_ord(rr[0]), ymax = from_ord(rr[1]), zmax = from_ord(rr[2]);
+            const double xmin = from_ord(~rr[3]), ymin = from_ord(~rr[4]), zmin = from_ord(~rr[5]);
+            const double vmax = from_ord(rr[6]), rmax = from_ord(rr[7]);
+            const double *qq = &q6[(size_t)6 * q];
+            const double xspan = qq[1] - qq[0], yspan = qq[3] - qq[2], zspan = qq[5] - qq[4];
+            GridDev &gd = grids[k];
+            // TrirLeX (x,y,z), TrirLeY (y,x,z), TrirLeZ (z,x,y) (NBSConsultant.h:52-118)
+            int perm[3];
+            if (xspan >= yspan) {
+                if (xspan >= zspan) { perm[0] = 0; perm[1] = 1; perm[2] = 2; }
+                else { perm[0] = 2; perm[1] = 0; perm[2] = 1; }
+            } else {
+                if (yspan >= zspan) { perm[0] = 1; perm[1] = 0; perm[2] = 2; }
+                else { perm[0] = 2; perm[1] = 0; perm[2] = 1; }
+            }
+            const double dtu = h->up_iact * S.dt;
+            const double glen = olen(S.g);
+            const double zr = S.sf * (2.0 * rmax + dtu * vmax + 0.5 * dtu * dtu * glen);
+            unsigned ncx = (unsigned)std::ceil((xmax - xmin) / zr); ncx = std::max(ncx, 1u);
+            unsigned ncy = (unsigned)std::ceil((ymax - ymin) / zr); ncy = std::max(ncy, 1u);
+            unsigned ncz = (unsigned)std::ceil((zmax - zmin) / zr); ncz = std::max(ncz, 1u);
+            if (ncx >= (1u << 17) || ncy >= (1u << 17) || ncz >= (1u << 17))
+                return fail(h, VG_EUNSUPPORTED, "cell grid exceeds 2^17 cells per axis");
+            gd.xmin = xmin; gd.ymin = ymin; gd.zmin = zmin; gd.zr = zr;
+            gd.safe_v = dtu;
+            gd.safe_g = 0.5 * dtu * dtu * glen;
+            gd.sf = S.sf;
+            gd.nx = (int)ncx + 3; gd.ny = (int)ncy + 3; gd.nz = (int)ncz + 3;
+            gd.perm[0] = perm[0]; gd.perm[1] = perm[1]; gd.perm[2] = perm[2];
+            const long long c = (long long)gd.nx * gd.ny * gd.nz;
+            cell_off[k] = cells;
+            cells += c;
+        }
+        if (cells > (1ll << 31)) return fail(h, VG_EUNSUPPORTED, "cell grid too large (sparse domain)");
+        for (int q = 0; q < S_live; ++q) {
+            const int k = live[q];
+            h->sdev[k].G = grids[k];
+            h->sdev[k].cell_off = cell_off[k];
+        }
+        if (upload_scenes(h)) return VG_EHIP;
+        if (cells > h->cells_cap) {
+            if (h->cstart) HIPCHK(hipFree(h->cstart));
+            if (h->cend) HIPCHK(hipFree(h->cend));
+            h->cstart = h->cend = nullptr;
+            h->cells_cap = cells + cells / 4;
+            if (dalloc(h, &h->cstart, h->cells_cap) || dalloc(h, &h->cend, h->cells_cap)) return VG_ENOMEM;
+        }
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(cells, 256)), dim3(256), 0, st, h->cstart, cells, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(cells, 256)), dim3(256), 0, st, h->cend, cells, 0);
+    }
+    if (n_total > 0) {
+        // stable sort by (scene slot, Morton cell key): ranks are in DO-id order
+        // within each scene, so each cell's spheres stay in id order
+        int slot_bits = 0;
+        while ((1 << slot_bits) <= max_slot + 1) ++slot_bits;
+        const unsigned long long retired_key = (unsigned long long)(max_slot + 1) << 51;
+        const int end_bit = std::min(64, 51 + slot_bits);
+        hipLaunchKernelGGL(k_cellkey, dim3(nblk(n_total, 256)), dim3(256), 0, st, G0, n_total,
+                           (const SceneDev *)h->scenes, (const ModelDev *)h->models, (const unsigned *)h->inv,
+                           retired_key, h->keys, h->svals);
+        size_t need = 0;
+        (void)rocprim::radix_sort_pairs(nullptr, need, h->keys, h->skeys, h->svals, h->perm, (size_t)n_total, 0,
+                                        end_bit, st);
+        if (ensure_tmp(h, need)) return VG_EHIP;
+        HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, h->keys, h->skeys, h->svals, h->perm, (size_t)n_total, 0,
+                                         end_bit, st));
+    }
+    const int n = n_live;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_reorder, dim3(nblk(n, 256)), dim3(256), 0, st, G0, G1, n, h->ld, (const unsigned *)h->perm,
+                           (const SceneDev *)h->scenes, (const ModelDev *)h->models, h->safe, h->lin,
+                           h->rec_valid ? (const double *)h->rec_imp : (const double *)nullptr, h->rec_imp2);
+        if (h->rec_valid) std::swap(h->rec_imp, h->rec_imp2);
+        hipLaunchKernelGGL(k_cellstart, dim3(nblk(n, 256)), dim3(256), 0, st, h->lin, n, h->cstart, h->cend);
+        // candidate list in one pass; grow the capacity and redo if a sphere overflowed
+        OuterArgs oa;
+        oa.g = G1; oa.n = n; oa.ld = h->ld; oa.scenes = h->scenes; oa.cstart = h->cstart; oa.cend = h->cend;
+        oa.safe = h->safe; oa.walls = h->walls; oa.t = tables(h); oa.count = h->ocount; oa.nbr = h->onbr;
+        oa.ko = h->ko; oa.single_group = h->single_group; oa.flags = h->flags;
+        for (;;) {
+            HIPCHK(hipMemsetAsync(h->flags + 5, 0, sizeof(int), st));
+            hipLaunchKernelGGL(k_outer, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
+            int maxc = 0;
+            HIPCHK(hipMemcpyAsync(&maxc, h->flags + 5, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (maxc <= h->ko) break;
+            HIPCHK(hipFree(h->onbr));
+            h->onbr = nullptr;
+            h->ko = maxc + maxc / 4 + 8;
+            if (dalloc(h, &h->onbr, (size_t)h->ko * h->ld)) return VG_ENOMEM;
+            oa.nbr = h->onbr;
+            oa.ko = h->ko;
+        }
+        // neighbour lists with history carry
+        InnerArgs ia;
+        ia.g = G1; ia.gold = G0; ia.ell = h->ell[h->ecur ^ 1]; ia.eold = h->ell[h->ecur];
+        ia.n = n; ia.ld = h->ld; ia.ki = h->ki; ia.perm = h->perm; ia.cnt = h->ocount; ia.nbr = h->onbr;
+        ia.ko = h->ko; ia.scenes = h->scenes; ia.models = h->models; ia.walls = h->walls; ia.flags = h->flags;
+        hipLaunchKernelGGL(k_inner_build, dim3(nblk(n, 128)), dim3(128), (size_t)h->ki * 128 * sizeof(unsigned), st,
+                           ia);
     }
     h->n_iact = -1;   // counted on request (vg_num_interactions)
-    // neighbour lists with history carry
-    InnerArgs ia;
-    ia.g = G1; ia.gold = G0; ia.ell = h->ell[h->ecur ^ 1]; ia.eold = h->ell[h->ecur];
-    ia.n = n; ia.nw = h->nw; ia.ki = h->ki; ia.perm = h->perm; ia.cnt = h->ocount; ia.nbr = h->onbr; ia.ko = h->ko;
-    ia.walls = h->walls; ia.skin = h->skin; ia.flags = h->flags;
-    hipLaunchKernelGGL(k_inner_build, dim3(nblk(n, 128)), dim3(128), (size_t)h->ki * 128 * sizeof(unsigned), st,
-                       ia);
     // the fresh lists need no relist: clear the ring slot the next step reads
     // and restart the wall displacement bounds
-    HIPCHK(hipMemsetAsync(h->flags + ((h->step_count) % 3), 0, sizeof(int), st));
-    if (h->nw) hipLaunchKernelGGL(k_wall_path_reset, dim3(nblk(h->nw, 64)), dim3(64), 0, st, h->walls, h->nw);
+    hipLaunchKernelGGL(k_ring_clear, dim3(nblk(h->scap, 256)), dim3(256), 0, st, h->rflags, h->scap,
+                       (int)(h->step_count % 3));
+    if (h->nw_used) hipLaunchKernelGGL(k_wall_path_reset, dim3(nblk(h->nw_used, 64)), dim3(64), 0, st, h->walls, h->nw_used);
     HIPCHK(hipGetLastError());
+    // new layout: live scenes in slot order
+    for (int k : live) {
+        SceneHost &S = h->sc[k];
+        S.seg_begin = h->sdev[k].rank_off;
+        S.seg_end = S.seg_begin + S.n;
+    }
+    if (retiring) {
+        for (int k = 0; k < h->scap; ++k) {
+            SceneHost &S = h->sc[k];
+            if (S.state != VG_SCENE_RETIRING) continue;
+            for (int w = 0; w < S.nw; ++w) {
+                h->wall_used[S.w_begin + w] = 0;
+                h->walls_h[S.w_begin + w].scene = -1;
+            }
+            if (S.nw)
+                HIPCHK(hipMemcpy(h->walls + S.w_begin, h->walls_h.data() + S.w_begin, sizeof(WallDev) * S.nw,
+                                 hipMemcpyHostToDevice));
+            S.state = S.result.empty() ? VG_SCENE_FREE : VG_SCENE_DONE;
+            S.seg_begin = S.seg_end = 0;
+            h->sdev[k] = SceneDev();
+        }
+        if (upload_scenes(h)) return VG_EHIP;
+        update_batch_flags(h);
+    }
+    h->n = n;
     h->cur ^= 1;
     h->ecur ^= 1;
     h->ent_valid = false;
     h->rebuilds++;
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Allocation
+// ---------------------------------------------------------------------------
 
 static void set_err_create(const std::string &s) { g_create_error = s; }
 
@@ -295,10 +792,11 @@ static void free_scene(vg_scene *h)
         F(h->ell[b].hdr); F(h->ell[b].ent); F(h->ell[b].hist);
         F(h->xs[b]); F(h->ys[b]); F(h->zs[b]);
     }
-    F(h->models); F(h->laws); F(h->walls); F(h->walls_ref); F(h->wkin); F(h->assigned); F(h->flags);
-    F(h->relists); F(h->red); F(h->bpart); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->safe); F(h->lin);
-    F(h->cstart); F(h->cend); F(h->ocount); F(h->onbr); F(h->tmp); F(h->rec); F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->d_sphere_ids); F(h->d_wall_ids);
-    F(h->assigned_sph); F(h->mm); F(h->en_part); F(h->en_out);
+    F(h->models); F(h->mm); F(h->laws); F(h->scenes); F(h->stime); F(h->rflags); F(h->walls); F(h->walls_ref);
+    F(h->wkin); F(h->assigned); F(h->assigned_sph); F(h->en_part); F(h->en_out); F(h->flags); F(h->relists);
+    F(h->contacts); F(h->red); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->qidx); F(h->segs);
+    F(h->qval); F(h->safe); F(h->lin); F(h->cstart); F(h->cend); F(h->ocount); F(h->onbr); F(h->tmp); F(h->rec);
+    F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->sid[0]); F(h->sid[1]);
     for (auto e : h->kev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -313,16 +811,50 @@ extern "C" int vg_destroy(vg_scene *h)
     return 0;
 }
 
-static double model_range(const vg_model &m)
+// Device buffers for ld sphere slots, wcap wall slots and scap scene slots.
+static int alloc_batch(vg_scene *h, int ld, int wcap, int scap, int ki, bool record)
 {
-    // DOModel::Analysis (DOModel.cpp:17-149), global walls
-    if (m.shape == VG_SPHERE) return m.attr[0];
-    if (m.shape == VG_QUASICYLINDER) return m.scope == VG_LOCAL ? std::max(m.attr[0], 0.5 * m.attr[1]) : m.attr[0];
-    if (m.shape == VG_QUASIPLATE) {
-        if (m.scope == VG_LOCAL) return 0.5 * std::max(std::max(m.attr[0], m.attr[1]), m.attr[2]);
-        return 0.5 * m.attr[1];
-    }
-    return 0.0;
+    h->ld = ld; h->wcap = wcap; h->scap = scap; h->ki = ki; h->record = record;
+    h->sc.assign(scap, SceneHost());
+    h->sdev.assign(scap, SceneDev());
+    h->walls_h.assign(std::max(wcap, 1), WallDev());
+    h->wall_used.assign(std::max(wcap, 1), 0);
+    if (hipGetDevice(&h->device) != hipSuccess) return fail(h, VG_EHIP, "no HIP device");
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(h, VG_EHIP, "hipStreamCreate failed");
+    (void)hipEventCreate(&h->ev0);
+    (void)hipEventCreate(&h->ev1);
+    int rc = 0;
+    rc |= alloc_geo(h, h->geo[0], ld) | alloc_geo(h, h->geo[1], ld);
+    rc |= alloc_ell(h, h->ell[0], ld, ki) | alloc_ell(h, h->ell[1], ld, ki);
+    rc |= dalloc(h, &h->scenes, scap) | dalloc(h, &h->stime, 2 * (size_t)scap) | dalloc(h, &h->rflags, 3 * (size_t)scap);
+    rc |= dalloc(h, &h->walls, wcap) | dalloc(h, &h->walls_ref, wcap) | dalloc(h, &h->wkin, wcap);
+    rc |= dalloc(h, &h->assigned, (size_t)15 * wcap);
+    rc |= dalloc(h, &h->flags, 8) | dalloc(h, &h->relists, 1) | dalloc(h, &h->red, (size_t)8 * scap);
+    rc |= dalloc(h, &h->inv, ld) | dalloc(h, &h->perm, ld) | dalloc(h, &h->svals, ld);
+    rc |= dalloc(h, &h->keys, ld) | dalloc(h, &h->skeys, ld);
+    for (int b = 0; b < 2; ++b) rc |= dalloc(h, &h->xs[b], ld) | dalloc(h, &h->ys[b], ld) | dalloc(h, &h->zs[b], ld);
+    rc |= dalloc(h, &h->qidx, 2 * (size_t)scap) | dalloc(h, &h->segs, 2 * (size_t)scap) | dalloc(h, &h->qval, 6 * (size_t)scap);
+    rc |= dalloc(h, &h->sid[0], ld) | dalloc(h, &h->sid[1], ld);
+    rc |= dalloc(h, &h->safe, ld) | dalloc(h, &h->lin, ld);
+    rc |= dalloc(h, &h->ocount, (size_t)ld + 1);
+    rc |= dalloc(h, &h->rec_imp, (size_t)6 * ld) | dalloc(h, &h->rec_imp2, (size_t)6 * ld);
+    if (record) rc |= dalloc(h, &h->rec, (size_t)ki * 13 * ld) | dalloc(h, &h->snap_wimp, (size_t)6 * std::max(wcap, 1));
+    rc |= dalloc(h, &h->en_out, 24) | dalloc(h, &h->en_part, (size_t)16 * 1024);
+    rc |= dalloc(h, &h->contacts, nblk(ld, 64) + 1);
+    if (rc) return VG_ENOMEM;
+    if (dalloc(h, &h->onbr, (size_t)h->ko * ld)) return VG_ENOMEM;
+    if (hipMemset(h->flags, 0, 8 * sizeof(int)) != hipSuccess ||
+        hipMemset(h->relists, 0, sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(h->contacts, 0, sizeof(unsigned long long) * (nblk(ld, 64) + 1)) != hipSuccess ||
+        hipMemset(h->rflags, 0, sizeof(int) * 3 * scap) != hipSuccess ||
+        hipMemset(h->scenes, 0, sizeof(SceneDev) * scap) != hipSuccess)
+        return fail(h, VG_EHIP, "device initialisation failed");
+    // wall slots not in use belong to no scene (the wall kernel skips them)
+    for (WallDev &W : h->walls_h) W.scene = -1;
+    if (hipMemcpy(h->walls, h->walls_h.data(), sizeof(WallDev) * wcap, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(h, VG_EHIP, "device initialisation failed");
+    return 0;
 }
 
 extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
@@ -339,210 +871,23 @@ extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
         delete h;
         return code;
     };
-    if (d->consultant != 0) {
-        h->err = "only LeapConsultant candidate-list rules run on the GPU";
-        return bail(VG_EUNSUPPORTED);
+    SceneHost probe;
+    std::string err;
+    int rc = scene_from_desc(h, d, probe, err);
+    if (rc) {
+        h->err = err;
+        return bail(rc);
     }
-    const double PI = 4.0 * std::atan(1.0);
-    h->n_do = d->n_do;
-    h->n_groups = d->n_groups;
-    h->do_is_sphere.assign(d->n_do, 0);
-    h->do_to_wall.assign(d->n_do, -1);
-    std::vector<unsigned> &sphere_ids = h->sphere_ids;
-    for (int i = 0; i < d->n_do; ++i) {
-        const int m = d->model_of[i];
-        if (m < 0 || m >= d->n_models) {
-            h->err = "DOStatus names an unknown DOModel";
-            return bail(VG_EINVAL);
-        }
-        const vg_model &M = d->models[m];
-        if (M.density < 0.0) {
-            h->err = "negative density";
-            return bail(VG_EUNSUPPORTED);
-        }
-        if (M.shape == VG_SPHERE) {
-            if (M.scope != VG_LOCAL || M.behavior != VG_MOBILE || M.density == 0.0) {
-                h->err = "spheres must be local, mobile and of positive density";
-                return bail(VG_EUNSUPPORTED);
-            }
-            h->do_is_sphere[i] = 1;
-            sphere_ids.push_back((unsigned)i);
-        } else if (M.shape == VG_QUASIPLATE || M.shape == VG_QUASICYLINDER) {
-            if (M.scope != VG_GLOBAL || (M.behavior != VG_FIXED && M.behavior != VG_ORBITAL)) {
-                h->err = "walls must be global and fixed or orbital";
-                return bail(VG_EUNSUPPORTED);
-            }
-            h->do_to_wall[i] = (int)h->wall_ids.size();
-            h->wall_ids.push_back((unsigned)i);
-        } else {
-            h->err = "shape is not on the GPU path";
-            return bail(VG_EUNSUPPORTED);
-        }
-    }
-    h->n = (int)sphere_ids.size();
-    h->nw = (int)h->wall_ids.size();
-    if (h->n == 0) {
-        h->err = "scene has no spheres";
-        return bail(VG_EINVAL);
-    }
-    if (h->n + h->nw > (int)PARTNER_MASK) {
-        h->err = "too many DOs for the neighbour-list encoding";
-        return bail(VG_EUNSUPPORTED);
-    }
-    // tables
-    h->models_h.resize(d->n_models);
-    for (int m = 0; m < d->n_models; ++m) {
-        h->models_h[m].group = d->models[m].group;
-        for (int c = 0; c < 3; ++c) h->models_h[m].fext[c] = d->models[m].fext[c];
-    }
-    h->laws_h.resize((size_t)d->n_groups * d->n_groups);
-    double bond_reach = 0.0;
-    for (int k = 0; k < d->n_groups * d->n_groups; ++k) {
-        h->laws_h[k].law = d->iact[k].law;
-        for (int p = 0; p < 8; ++p) h->laws_h[k].p[p] = d->iact[k].p[p];
-        if (d->iact[k].law < 0 || d->iact[k].law > 5) {
-            h->err = "unknown impact solver";
-            return bail(VG_EINVAL);
-        }
-        const int L = d->iact[k].law;
-        if ((L == VG_BSDBF || L == VG_BSDB2F) && d->iact[k].p[4] != 0.0 && d->iact[k].p[0] != 0.0)
-            bond_reach = std::max(bond_reach, std::fabs(d->iact[k].p[4] / d->iact[k].p[0]));
-    }
-    h->sd_only = true;
-    for (const LawDev &L : h->laws_h) {
-        if (L.law == VG_H || L.law == VG_HERTZ) h->sd_only = false;
-        if ((L.law == VG_BSDBF || L.law == VG_BSDB2F || L.law == VG_BTSDBF) && L.p[4] != 0.0) h->no_bond = 0;
-    }
-    {
-        int g0 = -1;
-        h->single_group = 1;
-        for (unsigned id : sphere_ids) {
-            const int g = d->models[d->model_of[id]].group;
-            if (g0 < 0) g0 = g;
-            else if (g != g0) h->single_group = 0;
-        }
-    }
-    h->dt = d->dt; h->t = d->t_current; h->t_stop = d->t_stop;
-    for (int c = 0; c < 3; ++c) h->g[c] = d->g[c];
-    h->sf = d->safety_factor; h->up_iact = d->up_iact ? d->up_iact : 1; h->record = d->record;
-    double rmin = 1e300;
-    for (unsigned id : sphere_ids) rmin = std::min(rmin, d->models[d->model_of[id]].attr[0]);
-    h->skin = d->inner_skin > 0 ? d->inner_skin : 0.1 * rmin;
-    h->skin = std::max(h->skin, bond_reach * 1.01);
-    h->bond_reach = bond_reach;
+    int ki = 48;
     const char *ki_env = getenv("VG_NEIGHBOUR_CAPACITY");
-    if (ki_env) h->ki = std::max(8, std::min(64, atoi(ki_env)));   // the step kernel keeps a 64-bit entry mask
-    // walls
-    h->walls_h.resize(h->nw);
-    for (int w = 0; w < h->nw; ++w) {
-        const unsigned id = h->wall_ids[w];
-        const vg_model &M = d->models[d->model_of[id]];
-        WallDev &W = h->walls_h[w];
-        W.pos = mk(d->pos[3 * id], d->pos[3 * id + 1], d->pos[3 * id + 2]);
-        W.vel = mk(d->vel[3 * id], d->vel[3 * id + 1], d->vel[3 * id + 2]);
-        W.ox = mk(d->ox[3 * id], d->ox[3 * id + 1], d->ox[3 * id + 2]);
-        W.oz = mk(d->oz[3 * id], d->oz[3 * id + 1], d->oz[3 * id + 2]);
-        W.av = mk(d->av[3 * id], d->av[3 * id + 1], d->av[3 * id + 2]);
-        W.a0 = M.attr[0]; W.a1 = M.attr[1]; W.a2 = M.attr[2];
-        W.range = model_range(M);
-        W.reach = M.shape == VG_QUASIPLATE ? 0.5 * std::sqrt(M.attr[0] * M.attr[0] + M.attr[1] * M.attr[1] +
-                                                             M.attr[2] * M.attr[2])
-                                           : std::sqrt(M.attr[0] * M.attr[0] + 0.25 * M.attr[1] * M.attr[1]);
-        W.path = 0.0;
-        W.shape = M.shape; W.group = M.group; W.behavior = M.behavior; W.kin = 0;
-    }
-    // device
-    if (hipGetDevice(&h->device) != hipSuccess) {
-        h->err = "no HIP device";
-        return bail(VG_EHIP);
-    }
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-        h->err = "hipStreamCreate failed";
-        return bail(VG_EHIP);
-    }
-    (void)hipEventCreate(&h->ev0);
-    (void)hipEventCreate(&h->ev1);
-    const int n = h->n;
-    int rc = 0;
-    rc |= alloc_geo(h, h->geo[0], n) | alloc_geo(h, h->geo[1], n);
-    rc |= alloc_ell(h, h->ell[0], n, h->ki) | alloc_ell(h, h->ell[1], n, h->ki);
-    rc |= dalloc(h, &h->models, d->n_models) | dalloc(h, &h->laws, (size_t)d->n_groups * d->n_groups);
-    rc |= dalloc(h, &h->walls, h->nw) | dalloc(h, &h->walls_ref, h->nw) | dalloc(h, &h->wkin, h->nw);
-    rc |= dalloc(h, &h->assigned, (size_t)15 * h->nw);
-    rc |= dalloc(h, &h->flags, 8) | dalloc(h, &h->relists, 1) | dalloc(h, &h->red, 8);
-    rc |= dalloc(h, &h->bpart, (size_t)8 * nblk(n, 256));
-    rc |= dalloc(h, &h->inv, n) | dalloc(h, &h->perm, n) | dalloc(h, &h->svals, n);
-    rc |= dalloc(h, &h->keys, n) | dalloc(h, &h->skeys, n);
-    for (int b = 0; b < 2; ++b) rc |= dalloc(h, &h->xs[b], n) | dalloc(h, &h->ys[b], n) | dalloc(h, &h->zs[b], n);
-    rc |= dalloc(h, &h->safe, n) | dalloc(h, &h->lin, n);
-    rc |= dalloc(h, &h->ocount, (size_t)n + 1);
-    rc |= dalloc(h, &h->rec, (size_t)h->ki * 13 * n) | dalloc(h, &h->rec_imp, (size_t)6 * n);
-    rc |= dalloc(h, &h->rec_imp2, (size_t)6 * n) | dalloc(h, &h->snap_wimp, (size_t)6 * std::max(h->nw, 1));
-    rc |= dalloc(h, &h->d_sphere_ids, n) | dalloc(h, &h->d_wall_ids, std::max(h->nw, 1));
-    rc |= dalloc(h, &h->mm, d->n_models) | dalloc(h, &h->en_out, 24);
-    rc |= dalloc(h, &h->en_part, (size_t)16 * std::max(1u, std::min(1024u, nblk(n, 256))));
-    if (rc) return bail(VG_ENOMEM);
-    if (dalloc(h, &h->onbr, (size_t)h->ko * n)) return bail(VG_ENOMEM);
-    // upload spheres in DO-id order
-    {
-        std::vector<double4> pos(n), vel(n), avl(n);
-        std::vector<double> ori(6 * (size_t)n);
-        std::vector<unsigned> oid(n);
-        std::vector<int> mdl(n);
-        for (int s = 0; s < n; ++s) {
-            const unsigned id = sphere_ids[s];
-            const vg_model &M = d->models[d->model_of[id]];
-            const double r = M.attr[0];
-            const double mass = M.density * ((4.0 / 3.0 * PI) * r * r * r);   // DOModel.cpp:22-27
-            const double sudo = M.density_factor * mass;
-            pos[s] = make_double4(d->pos[3 * id], d->pos[3 * id + 1], d->pos[3 * id + 2], r);
-            vel[s] = make_double4(d->vel[3 * id], d->vel[3 * id + 1], d->vel[3 * id + 2], 0.0);
-            avl[s] = make_double4(d->av[3 * id], d->av[3 * id + 1], d->av[3 * id + 2], sudo);
-            for (int c = 0; c < 3; ++c) {
-                ori[(size_t)c * n + s] = d->ox[3 * id + c];
-                ori[(size_t)(3 + c) * n + s] = d->oz[3 * id + c];
-            }
-            oid[s] = (unsigned)s;
-            mdl[s] = d->model_of[id];
-        }
-        Geo &g = h->geo[0];
-        if (hipMemcpy(g.pos, pos.data(), sizeof(double4) * n, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(g.vel, vel.data(), sizeof(double4) * n, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(g.avl, avl.data(), sizeof(double4) * n, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(g.ori, ori.data(), sizeof(double) * 6 * n, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(g.oid, oid.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(g.mdl, mdl.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
-            h->err = "upload failed";
-            return bail(VG_EHIP);
-        }
-    }
-    // DOModel::Analysis mass and mass moment of inertia of spheres (DOModel.cpp:22-29)
-    std::vector<double2> mm(d->n_models, make_double2(0.0, 0.0));
-    for (int m = 0; m < d->n_models; ++m)
-        if (d->models[m].shape == VG_SPHERE) {
-            const double r = d->models[m].attr[0];
-            const double mass = d->models[m].density * ((4.0 / 3.0 * PI) * r * r * r);
-            mm[m] = make_double2(mass, 0.4 * mass * r * r);
-        }
-    h->do_to_sphere.assign(h->n_do, -1);
-    for (int s = 0; s < n; ++s) h->do_to_sphere[sphere_ids[s]] = s;
-    if (hipMemcpy(h->mm, mm.data(), sizeof(double2) * d->n_models, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->models, h->models_h.data(), sizeof(ModelDev) * d->n_models, hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(h->laws, h->laws_h.data(), sizeof(LawDev) * h->laws_h.size(), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        (h->nw && hipMemcpy(h->walls, h->walls_h.data(), sizeof(WallDev) * h->nw, hipMemcpyHostToDevice) !=
-                      hipSuccess) ||
-        hipMemcpy(h->d_sphere_ids, sphere_ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice) != hipSuccess ||
-        (h->nw && hipMemcpy(h->d_wall_ids, h->wall_ids.data(), sizeof(unsigned) * h->nw, hipMemcpyHostToDevice) !=
-                      hipSuccess) ||
-        hipMemset(h->flags, 0, 8 * sizeof(int)) != hipSuccess ||
-        hipMemset(h->relists, 0, sizeof(unsigned long long)) != hipSuccess) {
-        h->err = "upload failed";
-        return bail(VG_EHIP);
-    }
-    hipLaunchKernelGGL(k_hdr_zero, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->ell[0].hdr, n);
-    hipLaunchKernelGGL(k_hdr_zero, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->ell[1].hdr, n);
+    if (ki_env) ki = std::max(8, std::min(64, atoi(ki_env)));   // the step kernel keeps a 64-bit entry mask
+    h->up_iact = d->up_iact ? d->up_iact : 1;
+    rc = alloc_batch(h, probe.n, std::max(probe.nw, 1), 1, ki, true);
+    if (rc) return bail(rc);
+    int slot = -1;
+    rc = admit(h, d, -1, &slot);
+    if (rc) return bail(rc);
+    // Consultant::Reset + SimMediator::Initiate at creation
     if (rebuild(h)) return bail(VG_EHIP);
     if (hipStreamSynchronize(h->stream) != hipSuccess) {
         h->err = "device error during the first candidate-list build";
@@ -552,19 +897,56 @@ extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
     return 0;
 }
 
+extern "C" int vg_batch_create(const vg_batch_desc *bd, vg_scene **out)
+{
+    if (!bd || !out || bd->max_spheres <= 0 || bd->max_scenes <= 0 || bd->max_walls < 0) {
+        set_err_create("invalid batch description");
+        return VG_EINVAL;
+    }
+    *out = nullptr;
+    if (bd->max_spheres + (long long)bd->max_walls >= (long long)PARTNER_MASK) {
+        set_err_create("too many DOs for the neighbour-list encoding");
+        return VG_EUNSUPPORTED;
+    }
+    vg_scene *h = new vg_scene();
+    h->batch = true;
+    h->up_iact = bd->up_iact ? bd->up_iact : 1;
+    const int ki = bd->neighbour_capacity > 0 ? std::max(8, std::min(64, bd->neighbour_capacity)) : 32;
+    int rc = alloc_batch(h, (int)bd->max_spheres, std::max(bd->max_walls, 1), bd->max_scenes, ki, false);
+    if (rc) {
+        set_err_create(h->err);
+        free_scene(h);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return 0;
+}
+
+extern "C" int vg_admit(vg_scene *h, const vg_scene_desc *d, long long steps, int *scene)
+{
+    if (!h || !d || !scene) return VG_EINVAL;
+    if (!h->batch) return fail(h, VG_EINVAL, "vg_admit needs a batch handle (vg_batch_create)");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return admit(h, d, steps, scene);
+}
+
+// ---------------------------------------------------------------------------
+// Stepping
+// ---------------------------------------------------------------------------
+
 static StepArgs step_args(vg_scene *h)
 {
     StepArgs a;
-    a.n = h->n; a.nw = h->nw; a.ki = h->ki;
+    a.n = h->n; a.ld = h->ld; a.nw = h->nw_used; a.ki = h->ki;
     a.in = h->geo[h->cur]; a.out = h->geo[h->cur ^ 1];
     a.ell = h->ell[h->ecur];
     a.o_cnt = h->ocount; a.o_nbr = h->onbr; a.ko = h->ko;
     a.t = tables(h);
+    a.scenes = h->scenes;
+    a.n_scenes = h->scap;
     a.walls = h->walls;
-    a.flags = h->flags; a.relists = h->relists;
-    a.dt = h->dt; a.gx = h->g[0]; a.gy = h->g[1]; a.gz = h->g[2];
-    a.skin = h->skin;
-    a.relist_half = 0.5 * h->skin * (1.0 - 1e-9);
+    a.rflags = h->rflags; a.flags = h->flags; a.relists = h->relists; a.contacts = h->contacts;
     a.step = (int)(h->step_count % 3);
     a.rec_imp = h->rec_imp; a.rec = h->rec;
     a.single_group = h->single_group;
@@ -572,52 +954,85 @@ static StepArgs step_args(vg_scene *h)
     return a;
 }
 
-static int export_active(vg_scene *h, ActiveOut *out, size_t cap, unsigned long long *dcnt)
+static const SceneHost *scene0(const vg_scene *h)
 {
+    return (!h || h->batch || h->sc.empty()) ? nullptr : &h->sc[0];
+}
+
+static int export_active(vg_scene *h, int slot, ActiveOut *out, size_t cap, unsigned long long *dcnt,
+                         unsigned *d_sid, unsigned *d_wid)
+{
+    const SceneHost &S = h->sc[slot];
     HIPCHK(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), h->stream));
-    hipLaunchKernelGGL(k_export_active, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, h->geo[h->cur],
-                       h->ell[h->ecur], h->n, h->ki, (const unsigned *)h->ocount, (const unsigned *)h->onbr,
-                       (const WallDev *)h->walls, (const unsigned *)h->d_wall_ids,
-                       (const unsigned *)h->d_sphere_ids,
-                       h->ent_valid ? (const double *)h->rec : (const double *)nullptr, out,
+    hipLaunchKernelGGL(k_export_active, dim3(nblk(S.n, 256)), dim3(256), 0, h->stream, h->geo[h->cur],
+                       h->ell[h->ecur], S.seg_begin, S.seg_end, h->ld, h->ki, (const unsigned *)h->ocount,
+                       (const unsigned *)h->onbr, (const WallDev *)h->walls, (const unsigned *)d_wid,
+                       (const unsigned *)d_sid, h->ent_valid ? (const double *)h->rec : (const double *)nullptr, out,
                        (unsigned long long)cap, dcnt);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
+// DO-id maps of a scene on the device (for exports): sphere rank -> DO id,
+// wall slot -> DO id
+static int id_maps(vg_scene *h, int slot, unsigned **d_sid, unsigned **d_wid)
+{
+    const SceneHost &S = h->sc[slot];
+    std::vector<unsigned> wid(h->wcap, 0u);
+    for (int w = 0; w < S.nw; ++w) wid[S.w_begin + w] = S.wall_ids[w];
+    if (dalloc(h, d_sid, (size_t)S.n) || dalloc(h, d_wid, (size_t)h->wcap)) return VG_ENOMEM;
+    HIPCHK(hipMemcpy(*d_sid, S.sphere_ids.data(), sizeof(unsigned) * S.n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(*d_wid, wid.data(), sizeof(unsigned) * h->wcap, hipMemcpyHostToDevice));
+    return 0;
+}
+
 // RebuildIactRecordTab before the candidate-list reset (NBSConsultant.cpp:40-47):
 // the active contacts and the wall impulses of the last step are kept, since
-// the rebuild re-sorts the spheres and their lists.
+// the rebuild re-sorts the spheres and their lists (one-scene handles).
 static int snapshot_records(vg_scene *h)
 {
     unsigned long long *dcnt = nullptr;
-    if (dalloc(h, &dcnt, 1)) return VG_ENOMEM;
-    if (export_active(h, nullptr, 0, dcnt)) return VG_EHIP;
+    unsigned *d_sid = nullptr, *d_wid = nullptr;
+    if (dalloc(h, &dcnt, 1) || id_maps(h, 0, &d_sid, &d_wid)) return VG_ENOMEM;
+    if (export_active(h, 0, nullptr, 0, dcnt, d_sid, d_wid)) return VG_EHIP;
     unsigned long long cnt = 0;
     HIPCHK(hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (cnt > h->snap_cap) {
         if (h->snap) HIPCHK(hipFree(h->snap));
+        h->snap = nullptr;
         h->snap_cap = cnt + cnt / 4 + 64;
         if (dalloc(h, &h->snap, h->snap_cap)) return VG_ENOMEM;
     }
-    if (export_active(h, h->snap, h->snap_cap, dcnt)) return VG_EHIP;
-    if (h->nw && h->ent_valid)
-        hipLaunchKernelGGL(k_wall_impulse, dim3(h->nw), dim3(256), 0, h->stream, h->ell[h->ecur], h->n, h->ki,
-                           (const double *)h->rec, h->snap_wimp);
+    if (export_active(h, 0, h->snap, h->snap_cap, dcnt, d_sid, d_wid)) return VG_EHIP;
+    const SceneHost &S = h->sc[0];
+    if (S.nw && h->ent_valid)
+        hipLaunchKernelGGL(k_wall_impulse, dim3(S.nw), dim3(256), 0, h->stream, h->ell[h->ecur], S.seg_begin,
+                           S.seg_end, h->ld, h->ki, S.w_begin, (const double *)h->rec, h->snap_wimp);
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipFree(dcnt));
+    HIPCHK(hipFree(d_sid));
+    HIPCHK(hipFree(d_wid));
     h->snap_count = (long long)cnt;
     h->snap_valid = true;
     return 0;
 }
 
-static int one_step(vg_scene *h, bool rec, bool assign_spheres)
+static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>> *assign_spheres)
 {
+    if (h->n == 0) {
+        // nothing placed yet: the first boundary places the queued scenes
+        bool queued = false;
+        for (const SceneHost &S : h->sc) queued = queued || S.state == VG_SCENE_QUEUED;
+        if (!queued) return 0;
+        if (rebuild(h)) return VG_EHIP;
+        if (h->n == 0) return 0;
+    }
+    rec = rec && h->record;
     StepArgs a = step_args(h);
     const bool kt = 2 * (h->kev_used + 1) <= (int)h->kev.size();
     if (kt) HIPCHK(hipEventRecord(h->kev[2 * h->kev_used], h->stream));
-    const dim3 grid(nblk(h->n, 256)), blk(256);
+    const dim3 grid(nblk(std::max(h->n, h->scap), 256)), blk(256);
     if (h->sd_only) {
         if (rec) hipLaunchKernelGGL((k_step<true, true>), grid, blk, 0, h->stream, a);
         else hipLaunchKernelGGL((k_step<false, true>), grid, blk, 0, h->stream, a);
@@ -630,28 +1045,48 @@ static int one_step(vg_scene *h, bool rec, bool assign_spheres)
         h->kev_used++;
     }
     h->last_launches++;
+    if (h->nw_used)
+        hipLaunchKernelGGL(k_walls, dim3(nblk(h->nw_used, 64)), dim3(64), 0, h->stream, h->walls, h->nw_used,
+                           (const SceneDev *)h->scenes, h->stime, (const double *)h->assigned,
+                           (const WallDev *)h->walls_ref, (const WallKin *)h->wkin, h->rflags,
+                           (int)(h->step_count % 6));
     h->cur ^= 1;
     h->step_count++;
     h->round++;
-    h->t += h->dt;   // SystemParameter::NextStep
-    if (h->nw) {
-        hipLaunchKernelGGL(k_walls, dim3(nblk(h->nw, 64)), dim3(64), 0, h->stream, h->walls, h->nw,
-                           h->assigned, h->walls_ref, h->wkin, h->dt, h->t, h->flags, a.step, a.relist_half);
+    // SystemParameter::NextStep per scene; a scene that reaches its step
+    // count is frozen (its state is held until the next boundary)
+    bool frozen = false;
+    for (int k = 0; k < h->scap; ++k) {
+        SceneHost &S = h->sc[k];
+        if (S.state != VG_SCENE_RUNNING) continue;
+        S.t += S.dt;
+        S.steps_done++;
+        if (S.steps_target >= 0 && S.steps_done >= S.steps_target) {
+            S.state = VG_SCENE_FINISHED;
+            h->sdev[k].active = 0;
+            frozen = true;
+        }
     }
+    if (frozen && upload_scenes(h)) return VG_EHIP;
     if (assign_spheres) {
         // SimMediator::Run(map): the assigned statuses replace the integrated
         // ones before contact detection and the rebuild; positions may jump,
-        // so every sphere relists at the next step
-        hipLaunchKernelGGL(k_assign_spheres, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, h->geo[h->cur], h->n,
-                           (const double *)h->assigned_sph, rec ? h->rec_imp : (double *)nullptr);
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h->stream, h->flags + (h->step_count % 3), 1ll, 1);
+        // so the scene's spheres relist at the next step
+        for (const auto &pr : *assign_spheres) {
+            const SceneHost &S = h->sc[pr.first];
+            hipLaunchKernelGGL(k_assign_spheres, dim3(nblk(S.n, 256)), dim3(256), 0, h->stream, h->geo[h->cur],
+                               S.seg_begin, S.seg_end, h->ld, (const double *)h->assigned_sph + pr.second,
+                               rec ? h->rec_imp : (double *)nullptr);
+            hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h->stream,
+                               h->rflags + 3 * pr.first + (h->step_count % 3), 1ll, 1);
+        }
     }
     h->rec_valid = rec;
     h->ent_valid = rec;
     h->snap_valid = false;
     HIPCHK(hipGetLastError());
     if (h->round % h->up_iact == 0) {
-        if (rec && snapshot_records(h)) return VG_EHIP;
+        if (rec && !h->batch && snapshot_records(h)) return VG_EHIP;
         return rebuild(h);
     }
     return 0;
@@ -663,7 +1098,7 @@ extern "C" int vg_step(vg_scene *h, int nsteps)
     h->last_launches = 0;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
     for (int k = 0; k < nsteps; ++k) {
-        int rc = one_step(h, k == nsteps - 1, false);
+        int rc = one_step(h, k == nsteps - 1, nullptr);
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(h->ev1, h->stream));
@@ -691,79 +1126,91 @@ extern "C" int vg_step_assigned(vg_scene *h, int n, const unsigned *ids, const d
                                 const double *ox, const double *oz, const double *av)
 {
     if (!h || n < 0 || (n > 0 && (!ids || !pos || !vel || !ox || !oz || !av))) return VG_EINVAL;
-    std::vector<double> buf((size_t)15 * h->nw, 0.0);
-    std::vector<char> set(h->nw, 0);
+    const SceneHost *S0 = scene0(h);
+    if (!S0) return fail(h, VG_EINVAL, "vg_step_assigned needs a one-scene handle");
+    const SceneHost &S = *S0;
+    std::vector<double> buf((size_t)15 * h->wcap, 0.0);
+    std::vector<char> set(h->wcap, 0);
     std::vector<double> sb;
     bool any_sphere = false;
     for (int k = 0; k < n; ++k) {
-        if (ids[k] >= (unsigned)h->n_do)
-            return fail(h, VG_EINVAL, "assigned status of an unknown DO id");
-        if (h->do_to_sphere[ids[k]] >= 0) {
-            if (!any_sphere) sb.assign((size_t)16 * h->n, 0.0);
+        if (ids[k] >= (unsigned)S.n_do) return fail(h, VG_EINVAL, "assigned status of an unknown DO id");
+        const double *src[5] = {pos + 3 * k, vel + 3 * k, ox + 3 * k, oz + 3 * k, av + 3 * k};
+        if (S.do_to_sphere[ids[k]] >= 0) {
+            if (!any_sphere) sb.assign((size_t)16 * S.n, 0.0);
             any_sphere = true;
-            double *b = &sb[(size_t)16 * h->do_to_sphere[ids[k]]];
-            for (int c = 0; c < 3; ++c) {
-                b[c] = pos[3 * k + c]; b[3 + c] = vel[3 * k + c]; b[6 + c] = ox[3 * k + c];
-                b[9 + c] = oz[3 * k + c]; b[12 + c] = av[3 * k + c];
-            }
+            double *b = &sb[(size_t)16 * S.do_to_sphere[ids[k]]];
+            for (int f = 0; f < 5; ++f)
+                for (int c = 0; c < 3; ++c) b[3 * f + c] = src[f][c];
             b[15] = 1.0;
             continue;
         }
-        const int w = h->do_to_wall[ids[k]];
+        const int w = S.do_to_wall[ids[k]];
         double *b = &buf[(size_t)15 * w];
-        for (int c = 0; c < 3; ++c) {
-            b[c] = pos[3 * k + c]; b[3 + c] = vel[3 * k + c]; b[6 + c] = ox[3 * k + c];
-            b[9 + c] = oz[3 * k + c]; b[12 + c] = av[3 * k + c];
-        }
+        for (int f = 0; f < 5; ++f)
+            for (int c = 0; c < 3; ++c) b[3 * f + c] = src[f][c];
         set[w] = 1;
     }
     if (any_sphere) {
-        if (!h->assigned_sph && dalloc(h, &h->assigned_sph, (size_t)16 * h->n)) return VG_ENOMEM;
+        if (!h->assigned_sph && dalloc(h, &h->assigned_sph, (size_t)16 * h->ld)) return VG_ENOMEM;
         HIPCHK(hipMemcpyAsync(h->assigned_sph, sb.data(), sizeof(double) * sb.size(), hipMemcpyHostToDevice,
                               h->stream));
     }
     bool changed = false;
-    for (int w = 0; w < h->nw; ++w) {
+    for (int w = S.w_begin; w < S.w_begin + S.nw; ++w) {
         const int want = set[w] ? 1 : (h->walls_h[w].kin == 1 ? 0 : h->walls_h[w].kin);
         if (want != h->walls_h[w].kin) {
             h->walls_h[w].kin = want;
             changed = true;
         }
     }
-    if (h->nw) {
+    if (S.nw) {
         HIPCHK(hipMemcpyAsync(h->assigned, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice, h->stream));
         if (changed) {
             // only the kin field changes here; the device copy holds the live status
-            std::vector<WallDev> cur(h->nw);
-            HIPCHK(hipMemcpyAsync(cur.data(), h->walls, sizeof(WallDev) * h->nw, hipMemcpyDeviceToHost, h->stream));
+            std::vector<WallDev> cur(h->wcap);
+            HIPCHK(hipMemcpyAsync(cur.data(), h->walls, sizeof(WallDev) * h->wcap, hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
-            for (int w = 0; w < h->nw; ++w) cur[w].kin = h->walls_h[w].kin;
-            HIPCHK(hipMemcpyAsync(h->walls, cur.data(), sizeof(WallDev) * h->nw, hipMemcpyHostToDevice, h->stream));
+            for (int w = S.w_begin; w < S.w_begin + S.nw; ++w) cur[w].kin = h->walls_h[w].kin;
+            HIPCHK(hipMemcpyAsync(h->walls, cur.data(), sizeof(WallDev) * h->wcap, hipMemcpyHostToDevice, h->stream));
         }
     }
     h->last_launches = 0;
-    int rc = one_step(h, true, any_sphere);
+    std::vector<std::pair<int, int>> asg;
+    if (any_sphere) asg.push_back({0, 0});
+    int rc = one_step(h, true, any_sphere ? &asg : nullptr);
     if (rc) return rc;
     // synchronous: the host buffers are reused by the next call
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
-extern "C" int vg_system_energy(vg_scene *h, double *out)
+static int set_rotation(vg_scene *h, int slot, int n, const unsigned *ids, const double *c, const double *a,
+                        double omega, double t_ref)
 {
-    if (!h || !out) return VG_EINVAL;
-    const int n = h->n;
-    const int nb = (int)std::max(1u, std::min(1024u, nblk(n, 256)));
-    double *avg = h->en_out + 16;
-    for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(k_energy_part, dim3(nb), dim3(256), 0, h->stream, h->geo[h->cur], n,
-                           (const double2 *)h->mm, h->g[0], h->g[1], h->g[2], (const double *)avg, pass, h->en_part);
-        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, h->stream, (const double *)h->en_part, nb, n, pass,
-                           avg, h->en_out);
-    }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, h->en_out, 16 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    const SceneHost &S = h->sc[slot];
     HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<WallDev> cur(h->wcap), ref(h->wcap);
+    std::vector<WallKin> kin(h->wcap);
+    HIPCHK(hipMemcpy(cur.data(), h->walls, sizeof(WallDev) * h->wcap, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ref.data(), h->walls_ref, sizeof(WallDev) * h->wcap, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(kin.data(), h->wkin, sizeof(WallKin) * h->wcap, hipMemcpyDeviceToHost));
+    const double al = olen(a);
+    for (int k = 0; k < n; ++k) {
+        if (ids[k] >= (unsigned)S.n_do || S.do_to_wall[ids[k]] < 0)
+            return fail(h, VG_EINVAL, "wall rotation applies to walls only");
+        const int w = S.do_to_wall[ids[k]];
+        kin[w].c = mk(c[0], c[1], c[2]);
+        kin[w].a = mk(a[0] / al, a[1] / al, a[2] / al);
+        kin[w].omega = omega;
+        kin[w].t_ref = t_ref;
+        cur[w].kin = 2;
+        ref[w] = cur[w];
+        h->walls_h[w].kin = 2;
+    }
+    HIPCHK(hipMemcpy(h->walls_ref, ref.data(), sizeof(WallDev) * h->wcap, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->walls, cur.data(), sizeof(WallDev) * h->wcap, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->wkin, kin.data(), sizeof(WallKin) * h->wcap, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -771,65 +1218,51 @@ extern "C" int vg_set_wall_rotation(vg_scene *h, int n, const unsigned *ids, con
                                     double omega, double t_ref)
 {
     if (!h || n < 0) return VG_EINVAL;
-    HIPCHK(hipStreamSynchronize(h->stream));
-    std::vector<WallDev> cur(h->nw);
-    std::vector<WallKin> kin(h->nw);
-    if (h->nw) HIPCHK(hipMemcpy(cur.data(), h->walls, sizeof(WallDev) * h->nw, hipMemcpyDeviceToHost));
-    const double al = olen(a);
-    for (int k = 0; k < n; ++k) {
-        if (ids[k] >= (unsigned)h->n_do || h->do_to_wall[ids[k]] < 0)
-            return fail(h, VG_EINVAL, "wall rotation applies to walls only");
-        const int w = h->do_to_wall[ids[k]];
-        kin[w].c = mk(c[0], c[1], c[2]);
-        kin[w].a = mk(a[0] / al, a[1] / al, a[2] / al);
-        kin[w].omega = omega;
-        kin[w].t_ref = t_ref;
-        cur[w].kin = 2;
-        h->walls_h[w].kin = 2;
-    }
-    if (h->nw) {
-        HIPCHK(hipMemcpy(h->walls_ref, cur.data(), sizeof(WallDev) * h->nw, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->walls, cur.data(), sizeof(WallDev) * h->nw, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->wkin, kin.data(), sizeof(WallKin) * h->nw, hipMemcpyHostToDevice));
-    }
-    return 0;
+    if (!scene0(h)) return fail(h, VG_EINVAL, "vg_set_wall_rotation needs a one-scene handle");
+    return set_rotation(h, 0, n, ids, c, a, omega, t_ref);
 }
 
-extern "C" int vg_get_state(vg_scene *h, double *pos, double *vel, double *ox, double *oz, double *av,
-                            double *impact, double *ang_impact)
+// ---------------------------------------------------------------------------
+// State and records
+// ---------------------------------------------------------------------------
+
+// DOStatus of every DO of a placed scene, [n_do][21]: pos vel Ox Oz w impact
+// angular-impact, in the scene's DO-id order.
+static int fetch_scene_state(vg_scene *h, int slot, std::vector<double> &hb)
 {
-    if (!h) return VG_EINVAL;
+    const SceneHost &S = h->sc[slot];
     HIPCHK(hipStreamSynchronize(h->stream));
-    const int nd = h->n_do, n = h->n;
+    const int nd = S.n_do, n = S.n;
     double *dout = nullptr;
     if (dalloc(h, &dout, (size_t)21 * n)) return VG_ENOMEM;
     std::vector<double> sb((size_t)21 * n, 0.0);
-    hipLaunchKernelGGL(k_scatter_state, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->geo[h->cur], n,
-                       h->rec_valid ? h->rec_imp : (const double *)nullptr, dout);
-    std::vector<double> wimp((size_t)6 * std::max(h->nw, 1), 0.0);
+    hipLaunchKernelGGL(k_scatter_state, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->geo[h->cur], S.seg_begin,
+                       S.seg_end, h->ld, h->rec_valid ? (const double *)h->rec_imp : (const double *)nullptr, dout);
+    std::vector<double> wimp((size_t)6 * std::max(S.nw, 1), 0.0);
     double *dw = nullptr;
-    if (h->nw && h->snap_valid) {
-        if (dalloc(h, &dw, (size_t)6 * h->nw)) return VG_ENOMEM;
-        HIPCHK(hipMemcpyAsync(dw, h->snap_wimp, sizeof(double) * 6 * h->nw, hipMemcpyDeviceToDevice, h->stream));
-    } else if (h->nw && h->ent_valid) {
-        if (dalloc(h, &dw, (size_t)6 * h->nw)) return VG_ENOMEM;
-        hipLaunchKernelGGL(k_wall_impulse, dim3(h->nw), dim3(256), 0, h->stream, h->ell[h->ecur], n, h->ki,
-                           (const double *)h->rec, dw);
+    const bool one = !h->batch;
+    if (one && S.nw && h->snap_valid) {
+        if (dalloc(h, &dw, (size_t)6 * S.nw)) return VG_ENOMEM;
+        HIPCHK(hipMemcpyAsync(dw, h->snap_wimp, sizeof(double) * 6 * S.nw, hipMemcpyDeviceToDevice, h->stream));
+    } else if (one && S.nw && h->ent_valid) {
+        if (dalloc(h, &dw, (size_t)6 * S.nw)) return VG_ENOMEM;
+        hipLaunchKernelGGL(k_wall_impulse, dim3(S.nw), dim3(256), 0, h->stream, h->ell[h->ecur], S.seg_begin,
+                           S.seg_end, h->ld, h->ki, S.w_begin, (const double *)h->rec, dw);
     }
     HIPCHK(hipMemcpyAsync(sb.data(), dout, sizeof(double) * sb.size(), hipMemcpyDeviceToHost, h->stream));
-    if (dw) HIPCHK(hipMemcpyAsync(wimp.data(), dw, sizeof(double) * 6 * h->nw, hipMemcpyDeviceToHost, h->stream));
-    std::vector<WallDev> walls(h->nw);
-    if (h->nw)
-        HIPCHK(hipMemcpyAsync(walls.data(), h->walls, sizeof(WallDev) * h->nw, hipMemcpyDeviceToHost, h->stream));
+    if (dw) HIPCHK(hipMemcpyAsync(wimp.data(), dw, sizeof(double) * 6 * S.nw, hipMemcpyDeviceToHost, h->stream));
+    std::vector<WallDev> walls(std::max(S.nw, 1));
+    if (S.nw)
+        HIPCHK(hipMemcpyAsync(walls.data(), h->walls + S.w_begin, sizeof(WallDev) * S.nw, hipMemcpyDeviceToHost,
+                              h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipFree(dout));
     if (dw) HIPCHK(hipFree(dw));
-    std::vector<double> hb((size_t)21 * nd, 0.0);
+    hb.assign((size_t)21 * nd, 0.0);
     for (int s = 0; s < n; ++s)
-        std::memcpy(&hb[(size_t)21 * h->sphere_ids[s]], &sb[(size_t)21 * s], 21 * sizeof(double));
-    for (int w = 0; w < h->nw; ++w) {
-        const unsigned id = h->wall_ids[w];
-        double *d = &hb[(size_t)21 * id];
+        std::memcpy(&hb[(size_t)21 * S.sphere_ids[s]], &sb[(size_t)21 * s], 21 * sizeof(double));
+    for (int w = 0; w < S.nw; ++w) {
+        double *d = &hb[(size_t)21 * S.wall_ids[w]];
         const WallDev &W = walls[w];
         const d3 v[5] = {W.pos, W.vel, W.ox, W.oz, W.av};
         for (int k = 0; k < 5; ++k) { d[3 * k] = v[k].x; d[3 * k + 1] = v[k].y; d[3 * k + 2] = v[k].z; }
@@ -837,26 +1270,49 @@ extern "C" int vg_get_state(vg_scene *h, double *pos, double *vel, double *ox, d
         const bool zero = W.kin != 0;
         for (int c = 0; c < 6; ++c) d[15 + c] = zero ? 0.0 : wimp[(size_t)6 * w + c];
     }
+    return 0;
+}
+
+static void split_state(const std::vector<double> &hb, int nd, double *pos, double *vel, double *ox, double *oz,
+                        double *av, double *impact, double *ang_impact)
+{
     double *outs[7] = {pos, vel, ox, oz, av, impact, ang_impact};
     for (int f = 0; f < 7; ++f) {
         if (!outs[f]) continue;
         for (int i = 0; i < nd; ++i)
             for (int c = 0; c < 3; ++c) outs[f][3 * i + c] = hb[(size_t)21 * i + 3 * f + c];
     }
+}
+
+extern "C" int vg_get_state(vg_scene *h, double *pos, double *vel, double *ox, double *oz, double *av,
+                            double *impact, double *ang_impact)
+{
+    if (!h) return VG_EINVAL;
+    const SceneHost *S = scene0(h);
+    if (!S) return fail(h, VG_EINVAL, "vg_get_state needs a one-scene handle (vg_scene_get_state for batches)");
+    std::vector<double> hb;
+    if (fetch_scene_state(h, 0, hb)) return VG_EHIP;
+    split_state(hb, S->n_do, pos, vel, ox, oz, av, impact, ang_impact);
     return 0;
 }
 
-extern "C" double vg_time(const vg_scene *h) { return h ? h->t : 0.0; }
+extern "C" double vg_time(const vg_scene *h)
+{
+    const SceneHost *S = scene0(h);
+    return S ? S->t : 0.0;
+}
 
 extern "C" int vg_num_interactions(vg_scene *h, long long *n)
 {
     if (!h || !n) return VG_EINVAL;
+    const SceneHost *S = scene0(h);
+    if (!S) return fail(h, VG_EINVAL, "vg_num_interactions needs a one-scene handle");
     if (h->n_iact < 0) {
         unsigned long long *d = nullptr, c[2] = {0, 0};
         if (dalloc(h, &d, 2)) return VG_ENOMEM;
         HIPCHK(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), h->stream));
-        hipLaunchKernelGGL(k_count_iact, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, (const unsigned *)h->ocount,
-                           (const unsigned *)h->onbr, h->n, h->ko, d);
+        hipLaunchKernelGGL(k_count_iact, dim3(nblk(S->n, 256)), dim3(256), 0, h->stream, (const unsigned *)h->ocount,
+                           (const unsigned *)h->onbr, S->seg_begin, S->seg_end, h->ld, h->ko, d);
         HIPCHK(hipMemcpyAsync(c, d, sizeof(c), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         HIPCHK(hipFree(d));
@@ -870,22 +1326,53 @@ extern "C" int vg_set_skin(vg_scene *h, double skin)
 {
     if (!h || !(skin >= 0.0)) return VG_EINVAL;
     HIPCHK(hipStreamSynchronize(h->stream));
-    h->skin = std::max(skin, h->bond_reach * 1.01);
-    // every sphere rebuilds its neighbour list with the new skin at the next step
-    int one = 1;
-    HIPCHK(hipMemcpy(h->flags + (h->step_count % 3), &one, sizeof(int), hipMemcpyHostToDevice));
+    // every scene takes the skin (at least its bond reach) and relists at the next step
+    for (int k = 0; k < h->scap; ++k) {
+        SceneHost &S = h->sc[k];
+        if (S.state == VG_SCENE_FREE || S.state == VG_SCENE_DONE) continue;
+        S.skin = std::max(skin, S.bond_reach * 1.01);
+        h->sdev[k].skin = S.skin;
+        h->sdev[k].relist_half = 0.5 * S.skin * (1.0 - 1e-9);
+    }
+    if (upload_scenes(h)) return VG_EHIP;
+    std::vector<int> r(3 * (size_t)h->scap);
+    HIPCHK(hipMemcpy(r.data(), h->rflags, sizeof(int) * r.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < h->scap; ++k) r[3 * k + (h->step_count % 3)] = 1;
+    HIPCHK(hipMemcpy(h->rflags, r.data(), sizeof(int) * r.size(), hipMemcpyHostToDevice));
     return 0;
 }
 
-extern "C" double vg_get_skin(const vg_scene *h) { return h ? h->skin : 0.0; }
-
-extern "C" int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *hi, vg_status *st)
+extern "C" double vg_get_skin(const vg_scene *h)
 {
-    if (!h || !n) return VG_EINVAL;
+    if (!h) return 0.0;
+    for (const SceneHost &S : h->sc)
+        if (S.state == VG_SCENE_RUNNING || S.state == VG_SCENE_QUEUED || S.state == VG_SCENE_FINISHED) return S.skin;
+    return 0.0;
+}
+
+static void status_of(const ActiveOut &A, vg_status &S)
+{
+    S.contact = (A.flags & F_CONTACT) != 0;
+    S.last_contact = (A.flags & F_LASTCONTACT) != 0;
+    S.bond = (A.flags & F_BOND) != 0;
+    S.last_bond = (A.flags & F_LASTBOND) != 0;
+    S.shear[0] = A.hist[0]; S.shear[1] = A.hist[1]; S.shear[2] = A.hist[2];
+    // the fourth history word is Kn for ISwH and the initial velocity for ISwHertz
+    S.kn = A.hist[3];
+    S.init_vel = A.hist[3];
+    for (int c = 0; c < 3; ++c) {
+        S.point[c] = A.rec[c]; S.dir[c] = A.rec[3 + c]; S.to_master[c] = A.rec[6 + c];
+        S.ang_to_master[c] = A.rec[9 + c];
+    }
+    S.overlap = A.rec[12];
+}
+
+static int get_active(vg_scene *h, int slot, long long *n, unsigned *lo, unsigned *hi, vg_status *st)
+{
     HIPCHK(hipStreamSynchronize(h->stream));
     unsigned long long cnt = 0;
     std::vector<ActiveOut> ho;
-    if (h->snap_valid) {
+    if (!h->batch && h->snap_valid) {
         cnt = (unsigned long long)h->snap_count;
         if (st) {
             ho.resize(std::min<size_t>(cnt, (size_t)std::max(*n, 0ll)));
@@ -893,22 +1380,25 @@ extern "C" int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *
         }
     } else {
         unsigned long long *dcnt = nullptr;
-        if (dalloc(h, &dcnt, 1)) return VG_ENOMEM;
-        if (export_active(h, nullptr, 0, dcnt)) return VG_EHIP;
+        unsigned *d_sid = nullptr, *d_wid = nullptr;
+        if (dalloc(h, &dcnt, 1) || id_maps(h, slot, &d_sid, &d_wid)) return VG_ENOMEM;
+        if (export_active(h, slot, nullptr, 0, dcnt, d_sid, d_wid)) return VG_EHIP;
         HIPCHK(hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         if (st) {
             const size_t cap = (size_t)std::max(*n, 0ll);
             ActiveOut *dout = nullptr;
             if (dalloc(h, &dout, cap)) return VG_ENOMEM;
-            if (export_active(h, dout, cap, dcnt)) return VG_EHIP;
+            if (export_active(h, slot, dout, cap, dcnt, d_sid, d_wid)) return VG_EHIP;
             ho.resize(std::min<size_t>(cnt, cap));
-            if (!ho.empty()) HIPCHK(hipMemcpyAsync(ho.data(), dout, sizeof(ActiveOut) * ho.size(), hipMemcpyDeviceToHost,
-                                                  h->stream));
+            if (!ho.empty())
+                HIPCHK(hipMemcpyAsync(ho.data(), dout, sizeof(ActiveOut) * ho.size(), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
             HIPCHK(hipFree(dout));
         }
         HIPCHK(hipFree(dcnt));
+        HIPCHK(hipFree(d_sid));
+        HIPCHK(hipFree(d_wid));
     }
     if (!st) {
         *n = (long long)cnt;
@@ -919,26 +1409,19 @@ extern "C" int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *
         return x.lo != y.lo ? x.lo < y.lo : x.hi < y.hi;
     });
     for (size_t k = 0; k < ho.size(); ++k) {
-        const ActiveOut &A = ho[k];
-        lo[k] = A.lo;
-        hi[k] = A.hi;
-        vg_status &S = st[k];
-        S.contact = (A.flags & F_CONTACT) != 0;
-        S.last_contact = (A.flags & F_LASTCONTACT) != 0;
-        S.bond = (A.flags & F_BOND) != 0;
-        S.last_bond = (A.flags & F_LASTBOND) != 0;
-        S.shear[0] = A.hist[0]; S.shear[1] = A.hist[1]; S.shear[2] = A.hist[2];
-        // the fourth history word is Kn for ISwH and the initial velocity for ISwHertz
-        S.kn = A.hist[3];
-        S.init_vel = A.hist[3];
-        for (int c = 0; c < 3; ++c) {
-            S.point[c] = A.rec[c]; S.dir[c] = A.rec[3 + c]; S.to_master[c] = A.rec[6 + c];
-            S.ang_to_master[c] = A.rec[9 + c];
-        }
-        S.overlap = A.rec[12];
+        lo[k] = ho[k].lo;
+        hi[k] = ho[k].hi;
+        status_of(ho[k], st[k]);
     }
     *n = (long long)ho.size();
     return 0;
+}
+
+extern "C" int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *hi, vg_status *st)
+{
+    if (!h || !n) return VG_EINVAL;
+    if (!scene0(h)) return fail(h, VG_EINVAL, "vg_get_active needs a one-scene handle (vg_scene_get_active)");
+    return get_active(h, 0, n, lo, hi, st);
 }
 
 extern "C" int vg_set_history(vg_scene *h, int n, const unsigned *lo, const unsigned *hi, const vg_status *st)
@@ -947,6 +1430,35 @@ extern "C" int vg_set_history(vg_scene *h, int n, const unsigned *lo, const unsi
     if (!h) return VG_EINVAL;
     if (n == 0) return 0;
     return fail(h, VG_EUNSUPPORTED, "history restore is not implemented yet");
+}
+
+// ---------------------------------------------------------------------------
+// Metrics, timing, counters
+// ---------------------------------------------------------------------------
+
+static int system_energy(vg_scene *h, int slot, double *out)
+{
+    const SceneHost &S = h->sc[slot];
+    const int n = S.n;
+    const int nb = (int)std::max(1u, std::min(1024u, nblk(n, 256)));
+    double *avg = h->en_out + 16;
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_energy_part, dim3(nb), dim3(256), 0, h->stream, h->geo[h->cur], S.seg_begin, S.seg_end,
+                           (const double2 *)h->mm, S.g[0], S.g[1], S.g[2], (const double *)avg, pass, h->en_part);
+        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, h->stream, (const double *)h->en_part, nb, n, pass,
+                           avg, h->en_out);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, h->en_out, 16 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int vg_system_energy(vg_scene *h, double *out)
+{
+    if (!h || !out) return VG_EINVAL;
+    if (!scene0(h)) return fail(h, VG_EINVAL, "vg_system_energy needs a one-scene handle (vg_scene_energy)");
+    return system_energy(h, 0, out);
 }
 
 extern "C" int vg_step_timing(vg_scene *h, double *ms, int *launches)
@@ -984,7 +1496,8 @@ extern "C" int vg_kernel_timing(vg_scene *h, int max_launches)
     return 0;
 }
 
-extern "C" int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, int *max_fill, int *capacity)
+extern "C" int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, int *max_fill, int *capacity,
+                        long long *contact_steps)
 {
     if (!h) return VG_EINVAL;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -996,6 +1509,140 @@ extern "C" int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, in
     if (relists) *relists = (long long)r;
     if (max_fill) *max_fill = flags[4];
     if (capacity) *capacity = h->ki;
+    if (contact_steps) {
+        std::vector<unsigned long long> c(nblk(h->ld, 64));
+        if (!c.empty())
+            HIPCHK(hipMemcpy(c.data(), h->contacts, sizeof(unsigned long long) * c.size(), hipMemcpyDeviceToHost));
+        unsigned long long t = 0;
+        for (unsigned long long x : c) t += x;
+        *contact_steps = (long long)t;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batch: per-scene queries
+// ---------------------------------------------------------------------------
+
+static int check_slot(vg_scene *h, int scene)
+{
+    if (!h) return VG_EINVAL;
+    if (scene < 0 || scene >= h->scap || h->sc[scene].state == VG_SCENE_FREE)
+        return fail(h, VG_EINVAL, "unknown scene");
+    return 0;
+}
+
+extern "C" int vg_scene_status(vg_scene *h, int scene, int *state, long long *steps_done, double *t)
+{
+    if (int rc = check_slot(h, scene)) return rc;
+    const SceneHost &S = h->sc[scene];
+    if (state) *state = S.state == VG_SCENE_RETIRING ? VG_SCENE_FINISHED : S.state;
+    if (steps_done) *steps_done = S.steps_done;
+    if (t) *t = S.t;
+    return 0;
+}
+
+extern "C" int vg_scene_get_state(vg_scene *h, int scene, double *pos, double *vel, double *ox, double *oz,
+                                  double *av)
+{
+    if (int rc = check_slot(h, scene)) return rc;
+    SceneHost &S = h->sc[scene];
+    if (S.state == VG_SCENE_QUEUED) return fail(h, VG_EINVAL, "scene not placed yet");
+    if (S.state == VG_SCENE_DONE) {
+        split_state(S.result, S.n_do, pos, vel, ox, oz, av, nullptr, nullptr);
+        return 0;
+    }
+    std::vector<double> hb;
+    if (fetch_scene_state(h, scene, hb)) return VG_EHIP;
+    split_state(hb, S.n_do, pos, vel, ox, oz, av, nullptr, nullptr);
+    return 0;
+}
+
+extern "C" int vg_scene_get_active(vg_scene *h, int scene, long long *n, unsigned *lo, unsigned *hi, vg_status *st)
+{
+    if (int rc = check_slot(h, scene)) return rc;
+    if (!n) return VG_EINVAL;
+    const SceneHost &S = h->sc[scene];
+    if (S.state != VG_SCENE_RUNNING && S.state != VG_SCENE_FINISHED)
+        return fail(h, VG_EINVAL, "scene is not on the device");
+    return get_active(h, scene, n, lo, hi, st);
+}
+
+extern "C" int vg_scene_energy(vg_scene *h, int scene, double *out)
+{
+    if (int rc = check_slot(h, scene)) return rc;
+    if (!out) return VG_EINVAL;
+    const SceneHost &S = h->sc[scene];
+    if (S.state != VG_SCENE_RUNNING && S.state != VG_SCENE_FINISHED)
+        return fail(h, VG_EINVAL, "scene is not on the device");
+    return system_energy(h, scene, out);
+}
+
+extern "C" int vg_scene_set_wall_rotation(vg_scene *h, int scene, int n, const unsigned *ids, const double *c,
+                                          const double *a, double omega, double t_ref)
+{
+    if (int rc = check_slot(h, scene)) return rc;
+    SceneHost &S = h->sc[scene];
+    if (!ids || !c || !a || n < 0) return VG_EINVAL;
+    if (S.state == VG_SCENE_QUEUED) {
+        // not placed yet: the prescription is applied when the scene is placed
+        const double al = olen(a);
+        if (S.kin0.empty()) {
+            S.kin0.assign(S.nw, WallKin());
+            S.kin0_set.assign(S.nw, 0);
+        }
+        for (int k = 0; k < n; ++k) {
+            if (ids[k] >= (unsigned)S.n_do || S.do_to_wall[ids[k]] < 0)
+                return fail(h, VG_EINVAL, "wall rotation applies to walls only");
+            const int w = S.do_to_wall[ids[k]] - S.w_begin;
+            WallKin &K = S.kin0[w];
+            K.c = mk(c[0], c[1], c[2]);
+            K.a = mk(a[0] / al, a[1] / al, a[2] / al);
+            K.omega = omega;
+            K.t_ref = t_ref;
+            S.kin0_set[w] = 1;
+        }
+        return 0;
+    }
+    if (S.state != VG_SCENE_RUNNING && S.state != VG_SCENE_FINISHED)
+        return fail(h, VG_EINVAL, "scene is not on the device");
+    return set_rotation(h, scene, n, ids, c, a, omega, t_ref);
+}
+
+extern "C" int vg_retire(vg_scene *h, int scene)
+{
+    if (int rc = check_slot(h, scene)) return rc;
+    SceneHost &S = h->sc[scene];
+    if (S.state == VG_SCENE_DONE) {
+        S = SceneHost();   // forget the kept result
+        return 0;
+    }
+    if (S.state == VG_SCENE_QUEUED) {
+        for (int w = 0; w < S.nw; ++w) h->wall_used[S.w_begin + w] = 0;
+        S = SceneHost();
+        return 0;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // leaves the device at the next boundary, without a kept result
+    S.result.clear();
+    S.state = VG_SCENE_RETIRING;
+    h->sdev[scene].active = 0;
+    return upload_scenes(h);
+}
+
+extern "C" int vg_batch_info(vg_scene *h, long long *spheres, int *scenes_on_device, int *scenes_queued,
+                             unsigned long long *steps)
+{
+    if (!h) return VG_EINVAL;
+    int on = 0, q = 0;
+    for (const SceneHost &S : h->sc) {
+        if (S.state == VG_SCENE_RUNNING || S.state == VG_SCENE_FINISHED || S.state == VG_SCENE_RETIRING) ++on;
+        if (S.state == VG_SCENE_QUEUED) ++q;
+    }
+    if (spheres) *spheres = h->n;
+    if (scenes_on_device) *scenes_on_device = on;
+    if (scenes_queued) *scenes_queued = q;
+    if (steps) *steps = h->step_count;
     return 0;
 }
 

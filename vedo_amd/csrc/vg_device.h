@@ -1,17 +1,19 @@
 // vg_device.h -- device data layout and per-contact physics of the DEM step.
 //
-// Layout in HBM (one scene, spheres in cell order, re-sorted at every
-// candidate-list rebuild):
-//   pos  double4 [N]  x, y, z, radius
-//   vel  double4 [N]  vx, vy, vz, path length since the last neighbour relist
-//   avl  double4 [N]  wx, wy, wz, sudo-mass m* (DOModel.cpp:22-30)
-//   ori  double  [6][N]  Ox.xyz, Oz.xyz (orientation, SoA)
-//   oid  u32 [N]  sphere rank (order of DO ids);  mdl i32 [N]  DOModel index
+// Layout in HBM (the spheres of every scene of a batch, scene by scene, each
+// scene's in cell order, re-sorted at every candidate-list rebuild; arrays
+// have a fixed leading dimension LD >= N, the batch's sphere capacity):
+//   pos  double4 [LD]  x, y, z, radius
+//   vel  double4 [LD]  vx, vy, vz, path length since the last neighbour relist
+//   avl  double4 [LD]  wx, wy, wz, sudo-mass m* (DOModel.cpp:22-30)
+//   ori  double  [6][LD]  Ox.xyz, Oz.xyz (orientation, SoA)
+//   oid  u32 [LD]  sphere rank within its scene (order of DO ids)
+//   mdl  i32 [LD]  DOModel index (batch-wide table; the model names the scene)
 // Neighbour ("inner") list, ELL transposed so that slot k of 64 consecutive
 // spheres is one contiguous 256-byte run:
-//   hdr  u8  [N]          count (7 bits) | current half (bit 7)
-//   ent  u32 [2][KI][N]   partner (26 bits) | status flags (6 bits)
-//   hist double4 [2][KI][N] tangential force history xyz + Kn / initial velocity
+//   hdr  u8  [LD]          count (7 bits) | current half (bit 7)
+//   ent  u32 [2][KI][LD]   partner (26 bits: sphere slot, or LD + wall slot) | status flags
+//   hist double4 [2][KI][LD] tangential force history xyz + Kn / initial velocity
 //        (one 32-byte sector per entry: divergent contacts touch one sector each)
 // Every sphere keeps its own copy of each pair's ImpactStatus; the two copies
 // are updated by the same instruction sequence from the same inputs, so they
@@ -43,8 +45,8 @@ struct LawDev {
 };
 
 struct ModelDev {
-    int group;
-    int pad;
+    int group;   // DOGroup index within the model's scene
+    int scene;   // scene slot of the batch
     double fext[3];
 };
 
@@ -56,6 +58,7 @@ struct WallDev {
     double path;         // bound on any wall point's displacement since the last relist
     int shape, group;
     int behavior, kin;   // kin: 0 none, 1 host-assigned, 2 device rotation
+    int scene, pad;      // scene slot of the batch
 };
 
 struct Contact {

@@ -46,35 +46,36 @@ __device__ inline void store_hist(double4 *hist, int h, int k, int ki, int n, in
 // Rebuild sphere i's neighbour list from its candidate list: keep partners
 // within `skin` of contact, carry flags and history of partners already
 // listed (both lists are ordered subsequences of the candidate list).
-__device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_old, int c_old, int h_new)
+__device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_old, int c_old, int h_new,
+                             double skin)
 {
     const unsigned e = a.o_cnt[i];
     int p = 0, q = 0;
-    unsigned old_partner = c_old > 0 ? (a.ell.ent[eidx(h_old, 0, a.ki, a.n, i)] & PARTNER_MASK) : 0xffffffffu;
+    unsigned old_partner = c_old > 0 ? (a.ell.ent[eidx(h_old, 0, a.ki, a.ld, i)] & PARTNER_MASK) : 0xffffffffu;
     for (unsigned t = 0; t < e; ++t) {
-        const unsigned j = a.o_nbr[(size_t)t * a.n + i];
+        const unsigned j = a.o_nbr[(size_t)t * a.ld + i];
         Contact c;
         bool near = true;
-        if (j < (unsigned)a.n) {
+        if (j < (unsigned)a.ld) {
             const double4 pj = a.in.pos[j];
             const d3 dx = ld3(pj) - xi;
-            const double lim = ri + pj.w + a.skin;
+            const double lim = ri + pj.w + skin;
             // far beyond the skin: excluded without the exact geometry
             if (dot(dx, dx) > lim * lim * (1.0 + 1e-9)) near = false;
             else cd_ss(xi, ri, ld3(pj), pj.w, c);
         } else {
-            cd_wall(xi, ri, a.walls[j - a.n], c);
+            cd_wall(xi, ri, a.walls[j - a.ld], c);
         }
         const bool carried = (p < c_old) && (old_partner == j);
         unsigned f = 0;
         Hist hs;
         if (carried) {
-            f = a.ell.ent[eidx(h_old, p, a.ki, a.n, i)] & ~PARTNER_MASK;
-            if (f & F_CONTACT) load_hist(a.ell.hist, h_old, p, a.ki, a.n, i, hs);
+            f = a.ell.ent[eidx(h_old, p, a.ki, a.ld, i)] & ~PARTNER_MASK;
+            if (f & F_CONTACT) load_hist(a.ell.hist, h_old, p, a.ki, a.ld, i, hs);
             ++p;
-            old_partner = p < c_old ? (a.ell.ent[eidx(h_old, p, a.ki, a.n, i)] & PARTNER_MASK) : 0xffffffffu;
+            old_partner = p < c_old ? (a.ell.ent[eidx(h_old, p, a.ki, a.ld, i)] & PARTNER_MASK) : 0xffffffffu;
         }
-        if (near && c.depth >= -a.skin) {
+        if (near && c.depth >= -skin) {
             if (q >= a.ki) {
                 atomicOr(&a.flags[3], ERR_INNER_CAP);
                 break;
@@ -83,12 +84,12 @@ __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_o
                 // A candidate outside the list has been solved as inactive
                 // every step: its status is Clean() (ImpactStatus.cpp:259-274).
                 f = 0;
-                if (j >= (unsigned)a.n || a.in.oid[i] < a.in.oid[j]) f |= F_SLAVE;
+                if (j >= (unsigned)a.ld || a.in.oid[i] < a.in.oid[j]) f |= F_SLAVE;
                 hs.s = mk(0, 0, 0);
                 hs.aux = 0.0;
             }
-            a.ell.ent[eidx(h_new, q, a.ki, a.n, i)] = j | f;
-            if (f & F_CONTACT) store_hist(a.ell.hist, h_new, q, a.ki, a.n, i, hs);
+            a.ell.ent[eidx(h_new, q, a.ki, a.ld, i)] = j | f;
+            if (f & F_CONTACT) store_hist(a.ell.hist, h_new, q, a.ki, a.ld, i, hs);
             ++q;
         }
     }
@@ -114,26 +115,40 @@ template <bool REC, bool SD>
 __global__ __launch_bounds__(256) void k_step(StepArgs a)
 {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        a.flags[(a.step + 2) % 3] = 0;
-        if (a.flags[a.step % 3]) atomicAdd(a.relists, 1ull);
+    if (i < a.n_scenes) {
+        // scene i's relist ring: clear the slot of the step after next, count this step's relist
+        a.rflags[3 * i + (a.step + 2) % 3] = 0;
+        if (a.rflags[3 * i + a.step % 3]) atomicAdd(a.relists, 1ull);
     }
     if (i >= a.n) return;
-    const bool do_relist = a.flags[a.step % 3] != 0;
+    const int mdl_i = a.in.mdl[i];
+    const ModelDev &md = a.t.models[mdl_i];
+    const int sc = md.scene;
+    const SceneDev &S = a.scenes[sc];
+    if (S.active <= 0) {
+        // a frozen scene (finished, awaiting retirement) is carried over unchanged
+        a.out.pos[i] = a.in.pos[i];
+        a.out.vel[i] = a.in.vel[i];
+        a.out.avl[i] = a.in.avl[i];
+        for (int c = 0; c < 6; ++c) a.out.ori[(size_t)c * a.ld + i] = a.in.ori[(size_t)c * a.ld + i];
+        a.out.oid[i] = a.in.oid[i];
+        a.out.mdl[i] = mdl_i;
+        return;
+    }
+    const bool do_relist = a.rflags[3 * sc + a.step % 3] != 0;
 
     const double4 p4 = a.in.pos[i];
     double4 v4 = a.in.vel[i];
     const double4 w4 = a.in.avl[i];
     const d3 xi = ld3(p4), vi = ld3(v4), wi = ld3(w4);
     const double ri = p4.w, mi = w4.w;
-    const int mdl_i = a.in.mdl[i];
-    const int gi = a.t.models[mdl_i].group;
+    const int gi = md.group;
 
     unsigned hdr = a.ell.hdr[i];
     int h = hdr >> 7, cnt = hdr & 127;
     if (do_relist) {
         const int hn = h ^ 1;
-        cnt = relist(a, i, xi, ri, h, cnt, hn);
+        cnt = relist(a, i, xi, ri, h, cnt, hn, S.skin);
         h = hn;
         a.ell.hdr[i] = (unsigned char)((h << 7) | cnt);
         v4.w = 0.0;
@@ -146,11 +161,11 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
     // many times as its longest list has entries.
     unsigned long long todo = 0;
     for (int k = 0; k < cnt; ++k) {
-        const size_t ei = eidx(h, k, a.ki, a.n, i);
+        const size_t ei = eidx(h, k, a.ki, a.ld, i);
         const unsigned ent = a.ell.ent[ei];
         const unsigned j = ent & PARTNER_MASK;
         bool full = true;
-        if (a.no_bond && j < (unsigned)a.n) {
+        if (a.no_bond && j < (unsigned)a.ld) {
             const double4 pj = a.in.pos[j];
             const d3 dx = ld3(pj) - xi;
             const double rs = ri + pj.w;
@@ -160,18 +175,19 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
                 const unsigned f = ent & ~PARTNER_MASK & ~F_STATUS & ~F_NOBACKUP;
                 if (f != (ent & ~PARTNER_MASK)) a.ell.ent[ei] = j | f;
                 if (REC) {
-                    double *rp = a.rec + ((size_t)k * 13) * a.n + i;
-                    for (int q = 0; q < 13; ++q) rp[(size_t)q * a.n] = 0.0;
+                    double *rp = a.rec + ((size_t)k * 13) * a.ld + i;
+                    for (int q = 0; q < 13; ++q) rp[(size_t)q * a.ld] = 0.0;
                 }
             }
         }
         if (full) todo |= 1ull << k;
     }
     d3 I = mk(0, 0, 0), L = mk(0, 0, 0);
+    unsigned n_contact = 0;   // pairs left in contact, counted on their slave side
     while (todo) {
         const int k = __builtin_ctzll(todo);
         todo &= todo - 1;
-        const size_t ei = eidx(h, k, a.ki, a.n, i);
+        const size_t ei = eidx(h, k, a.ki, a.ld, i);
         const unsigned ent = a.ell.ent[ei];
         const unsigned j = ent & PARTNER_MASK;
         Hist hs;
@@ -185,7 +201,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             hs.f = f;
         }
         hs.f &= ~F_NOBACKUP;
-        if (had_contact) load_hist(a.ell.hist, h, k, a.ki, a.n, i, hs);
+        if (had_contact) load_hist(a.ell.hist, h, k, a.ki, a.ld, i, hs);
         else { hs.s = mk(0, 0, 0); hs.aux = 0.0; }
         PairOut o;
         o.added = false;
@@ -195,73 +211,81 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             r.overlap = 0.0;
         }
         Contact c;
-        if (j < (unsigned)a.n) {
+        if (j < (unsigned)a.ld) {
             const double4 pj = a.in.pos[j], vj = a.in.vel[j], wj = a.in.avl[j];
             const int gj = a.single_group ? gi : a.t.models[a.in.mdl[j]].group;
-            const LawDev &law = a.t.laws[gi * a.t.n_groups + gj];
+            const LawDev &law = a.t.laws[S.law_off + gi * S.n_groups + gj];
             if (hs.f & F_SLAVE) {
                 cd_ss(xi, ri, ld3(pj), pj.w, c);
-                solve<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(pj), ld3(vj), ld3(wj), c, a.dt, hs, o, r);
+                solve<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(pj), ld3(vj), ld3(wj), c, S.dt, hs, o, r);
                 if (o.added) {
                     I = I + o.imp;
                     if (!o.linear_only) L = L + o.ang_a;
                 }
             } else {
                 cd_ss(ld3(pj), pj.w, xi, ri, c);
-                solve<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), xi, vi, wi, c, a.dt, hs, o, r);
+                solve<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), xi, vi, wi, c, S.dt, hs, o, r);
                 if (o.added) {
                     I = I + (o.linear_only ? -o.imp : smul(-1.0, o.imp));
                     if (!o.linear_only) L = L + o.ang_b;
                 }
             }
         } else {
-            const WallDev &w = a.walls[j - a.n];
-            const LawDev &law = a.t.laws[gi * a.t.n_groups + w.group];
+            const WallDev &w = a.walls[j - a.ld];
+            const LawDev &law = a.t.laws[S.law_off + gi * S.n_groups + w.group];
             cd_wall(xi, ri, w, c);
-            solve<REC, SD>(law, mi, 0.0, xi, vi, wi, w.pos, w.vel, w.av, c, a.dt, hs, o, r);
+            solve<REC, SD>(law, mi, 0.0, xi, vi, wi, w.pos, w.vel, w.av, c, S.dt, hs, o, r);
             if (o.added) {
                 I = I + o.imp;
                 if (!o.linear_only) L = L + o.ang_a;
             }
         }
         if (hs.f != (ent & ~PARTNER_MASK)) a.ell.ent[ei] = j | hs.f;
-        if (hs.f & F_CONTACT) store_hist(a.ell.hist, h, k, a.ki, a.n, i, hs);
+        if (hs.f & F_CONTACT) {
+            store_hist(a.ell.hist, h, k, a.ki, a.ld, i, hs);
+            n_contact += (hs.f & F_SLAVE) ? 1u : 0u;
+        }
         if (REC) {
-            double *rp = a.rec + ((size_t)k * 13) * a.n + i;
+            double *rp = a.rec + ((size_t)k * 13) * a.ld + i;
             const double vals[13] = {r.point.x, r.point.y, r.point.z, r.dir.x, r.dir.y, r.dir.z,
                                      r.to_master.x, r.to_master.y, r.to_master.z, r.ang_to_master.x,
                                      r.ang_to_master.y, r.ang_to_master.z, r.overlap};
-            for (int q = 0; q < 13; ++q) rp[(size_t)q * a.n] = vals[q];
+            for (int q = 0; q < 13; ++q) rp[(size_t)q * a.ld] = vals[q];
         }
     }
     // DOContainer::AddFieldImpact + AddConstrainedImpact (DOContainer.cpp:20-50)
-    const d3 gdt = muls(mk(a.gx, a.gy, a.gz), a.dt);
+    const d3 gdt = muls(mk(S.gx, S.gy, S.gz), S.dt);
     I = I + muls(gdt, mi);
-    const ModelDev &md = a.t.models[mdl_i];
-    I = I + smul(a.dt, mk(md.fext[0], md.fext[1], md.fext[2]));
+    I = I + smul(S.dt, mk(md.fext[0], md.fext[1], md.fext[2]));
     // DOSphere::Response (DOSphere.cpp:21-48)
     const d3 dv = smul(1.0 / mi, I);
     const d3 dav = smul(2.5 / mi / ri / ri, L);
-    const d3 dp = smul(a.dt, vi + smul(0.5, dv));
-    const d3 dw = smul(a.dt, wi + smul(0.5, dav));
+    const d3 dp = smul(S.dt, vi + smul(0.5, dv));
+    const d3 dw = smul(S.dt, wi + smul(0.5, dav));
     const d3 xn = xi + dp;
-    d3 oxn = mk(a.in.ori[i], a.in.ori[a.n + i], a.in.ori[2 * a.n + i]);
-    d3 ozn = mk(a.in.ori[3 * a.n + i], a.in.ori[4 * a.n + i], a.in.ori[5 * a.n + i]);
+    d3 oxn = mk(a.in.ori[i], a.in.ori[a.ld + i], a.in.ori[2 * a.ld + i]);
+    d3 ozn = mk(a.in.ori[3 * a.ld + i], a.in.ori[4 * a.ld + i], a.in.ori[5 * a.ld + i]);
     rotate_pair(oxn, ozn, dw);
     const d3 vn = vi + dv, wn = wi + dav;
     const double path = v4.w + len(dp);
-    if (path > a.relist_half) atomicOr(&a.flags[(a.step + 1) % 3], 1);
+    if (path > S.relist_half) atomicOr(&a.rflags[3 * sc + (a.step + 1) % 3], 1);
     a.out.pos[i] = make_double4(xn.x, xn.y, xn.z, ri);
     a.out.vel[i] = make_double4(vn.x, vn.y, vn.z, path);
     a.out.avl[i] = make_double4(wn.x, wn.y, wn.z, mi);
-    a.out.ori[i] = oxn.x; a.out.ori[a.n + i] = oxn.y; a.out.ori[2 * a.n + i] = oxn.z;
-    a.out.ori[3 * a.n + i] = ozn.x; a.out.ori[4 * a.n + i] = ozn.y; a.out.ori[5 * a.n + i] = ozn.z;
+    a.out.ori[i] = oxn.x; a.out.ori[a.ld + i] = oxn.y; a.out.ori[2 * a.ld + i] = oxn.z;
+    a.out.ori[3 * a.ld + i] = ozn.x; a.out.ori[4 * a.ld + i] = ozn.y; a.out.ori[5 * a.ld + i] = ozn.z;
     a.out.oid[i] = a.in.oid[i];
     a.out.mdl[i] = mdl_i;
     if (REC) {
-        a.rec_imp[i] = I.x; a.rec_imp[a.n + i] = I.y; a.rec_imp[2 * a.n + i] = I.z;
-        a.rec_imp[3 * a.n + i] = L.x; a.rec_imp[4 * a.n + i] = L.y; a.rec_imp[5 * a.n + i] = L.z;
+        a.rec_imp[i] = I.x; a.rec_imp[a.ld + i] = I.y; a.rec_imp[2 * a.ld + i] = I.z;
+        a.rec_imp[3 * a.ld + i] = L.x; a.rec_imp[4 * a.ld + i] = L.y; a.rec_imp[5 * a.ld + i] = L.z;
     }
+    // Wavefront sum of the contact counts (each < 128) bit by bit over the
+    // live lanes; the first live lane adds it to the wavefront's own slot.
+    unsigned long long tot = 0;
+    for (int b = 0; b < 7; ++b) tot += (unsigned long long)__popcll(__ballot((n_contact >> b) & 1u)) << b;
+    const unsigned long long live = __ballot(1);
+    if (__lane_id() == (unsigned)(__ffsll((long long)live) - 1)) a.contacts[i >> 6] += tot;
 }
 
 template __global__ void k_step<false, false>(StepArgs);
@@ -273,12 +297,25 @@ template __global__ void k_step<true, true>(StepArgs);
 // DOOrbitalQuasiPlate.cpp:18-34) and kinematic prescriptions
 // (SimMediator.cpp:1121-1163).  kin 1: host-assigned status in `assigned`;
 // kin 2: rigid rotation about (c, a) at omega from the reference status.
-__global__ void k_walls(WallDev *walls, int nw, const double *assigned, const WallDev *ref, const WallKin *kin,
-                        double dt, double t_new, int *flags, int step, double relist_half)
+__global__ void k_walls(WallDev *walls, int nw, const SceneDev *scenes, double *stime, const double *assigned,
+                        const WallDev *ref, const WallKin *kin, int *rflags, int step)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nw) return;
     const WallDev old = walls[w];
+    if (old.scene < 0) return;   // slot not in use
+    const SceneDev &S = scenes[old.scene];
+    // scene time before / after this step, double-buffered by step parity
+    // (SystemParameter::NextStep's tCurrent += dt); the scene's first wall
+    // writes the next value
+    const int sc = old.scene, nsc_t = step & 1;
+    const double t_old = stime[2 * sc + nsc_t];
+    if (S.active <= 0) {
+        if (w == S.w_begin) stime[2 * sc + (nsc_t ^ 1)] = t_old;
+        return;
+    }
+    const double dt = S.dt, t_new = t_old + dt;
+    if (w == S.w_begin) stime[2 * sc + (nsc_t ^ 1)] = t_new;
     WallDev W = old;
     if (W.kin == 1) {
         const double *s = assigned + 15 * w;
@@ -306,17 +343,25 @@ __global__ void k_walls(WallDev *walls, int nw, const double *assigned, const Wa
     // relative to the centre, |a|, |b|, |c| <= reach, moves by at most
     // |dpos| + reach (|dOx| + |dOy| + |dOz|), so a sphere-wall depth changes
     // by at most the sphere's path plus this accumulated bound: a wall whose
-    // bound exceeds half the skin makes every sphere relist at the next
-    // step, as a sphere's own path does.  The bound restarts when the step
-    // that just ran relisted (the lists were made at these wall positions).
+    // bound exceeds half the skin makes its scene's spheres relist at the
+    // next step, as a sphere's own path does.  The bound restarts when the
+    // step that just ran relisted (the lists were made at these positions).
     const d3 oy0 = cross(old.oz, old.ox), oy1 = cross(W.oz, W.ox);
     const double d = len(W.pos - old.pos) + W.reach * ((len(W.ox - old.ox) + len(oy1 - oy0)) + len(W.oz - old.oz));
-    W.path = (flags[step % 3] ? 0.0 : old.path) + d;
-    if (W.path > relist_half) atomicOr(&flags[(step + 1) % 3], 1);
+    W.path = (rflags[3 * sc + step % 3] ? 0.0 : old.path) + d;
+    if (W.path > S.relist_half) atomicOr(&rflags[3 * sc + (step + 1) % 3], 1);
     walls[w] = W;
 }
 
-// Fresh lists after a rebuild: the wall displacement bounds restart.
+// Fresh lists after a rebuild: no scene relists at the next step (its ring
+// slot r) ...
+__global__ void k_ring_clear(int *rflags, int scap, int r)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < scap) rflags[3 * k + r] = 0;
+}
+
+// ... and the wall displacement bounds restart.
 __global__ void k_wall_path_reset(WallDev *walls, int nw)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -333,25 +378,44 @@ __device__ inline unsigned long long ord64(double d)
     return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
 }
 
-// Bounds of the local DOs (LeapConsultant.cpp:119-164): min/max position,
-// max |v|, max range; and the rank -> current-index map.  Stage 1 writes one
-// partial per block, stage 2 (k_bounds_final) reduces them.
-__global__ void k_bounds(Geo g, int n, unsigned long long *part, unsigned *inv, double *xs, double *ys, double *zs)
+// Bounds of each scene's local DOs (LeapConsultant.cpp:119-164): max / min
+// position, max |v|, max range, as order-preserving integers reduced with
+// atomicMax into red[scene][8]; a block whose spheres all belong to one scene
+// (the usual case) reduces in LDS first.  Also the rank -> current-slot map
+// of the next sort and the coordinates for the quantile sorts.
+__global__ void k_bounds(Geo g, int n, const SceneDev *scenes, const ModelDev *models, unsigned long long *red,
+                         unsigned *inv, double *xs, double *ys, double *zs, unsigned *sid, unsigned retired)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ unsigned long long sm[8][256];
+    __shared__ int sc_first, sc_last;
     unsigned long long v[8];
+    int sc = -1;
     if (i < n) {
         const double4 p = g.pos[i], u = g.vel[i];
+        sc = models[g.mdl[i]].scene;
         v[0] = ord64(p.x); v[1] = ord64(p.y); v[2] = ord64(p.z);
         v[3] = ~ord64(p.x); v[4] = ~ord64(p.y); v[5] = ~ord64(p.z);
         v[6] = ord64(len(mk(u.x, u.y, u.z)));
         v[7] = ord64(p.w);
-        inv[g.oid[i]] = i;
+        inv[scenes[sc].rank_off + g.oid[i]] = i;
         xs[i] = p.x; ys[i] = p.y; zs[i] = p.z;
+        sid[i] = scenes[sc].active < 0 ? retired : (unsigned)sc;
+        if (scenes[sc].active < 0)   // retired: no part in any grid
+            for (int k = 0; k < 8; ++k) v[k] = 0;
     } else {
         for (int k = 0; k < 8; ++k) v[k] = 0;
     }
+    const int last_i = min(n, (int)((blockIdx.x + 1) * blockDim.x)) - 1;
+    if (threadIdx.x == 0) sc_first = sc;
+    if (i == last_i) sc_last = sc;
+    __syncthreads();
+    if (sc_first != sc_last) {
+        // the block straddles scenes (block-uniform branch): every sphere adds its own values
+        if (i < n)
+            for (int k = 0; k < 8; ++k) atomicMax(&red[(size_t)sc * 8 + k], v[k]);
+        return;
+    }
     for (int k = 0; k < 8; ++k) sm[k][threadIdx.x] = v[k];
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -359,23 +423,20 @@ __global__ void k_bounds(Geo g, int n, unsigned long long *part, unsigned *inv, 
             for (int k = 0; k < 8; ++k) sm[k][threadIdx.x] = max(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x < 8) part[(size_t)blockIdx.x * 8 + threadIdx.x] = sm[threadIdx.x][0];
+    if (threadIdx.x < 8 && sc_first >= 0) atomicMax(&red[(size_t)sc_first * 8 + threadIdx.x], sm[threadIdx.x][0]);
 }
 
-__global__ void k_bounds_final(const unsigned long long *part, int nblocks, unsigned long long *red)
+// Values at given positions of the sorted coordinate arrays (the 2%-trimmed
+// span quantiles of every scene): idx[m][2] -> out[m][6].
+__global__ void k_quantiles(const double *xs, const double *ys, const double *zs, const long long *idx, int m,
+                            double *out)
 {
-    __shared__ unsigned long long sm[8][256];
-    unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
-        for (int k = 0; k < 8; ++k) v[k] = max(v[k], part[(size_t)b * 8 + k]);
-    for (int k = 0; k < 8; ++k) sm[k][threadIdx.x] = v[k];
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int k = 0; k < 8; ++k) sm[k][threadIdx.x] = max(sm[k][threadIdx.x], sm[k][threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x < 8) red[threadIdx.x] = sm[threadIdx.x][0];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const long long lo = idx[2 * t], hi = idx[2 * t + 1];
+    out[6 * t + 0] = xs[lo]; out[6 * t + 1] = xs[hi];
+    out[6 * t + 2] = ys[lo]; out[6 * t + 3] = ys[hi];
+    out[6 * t + 4] = zs[lo]; out[6 * t + 5] = zs[hi];
 }
 
 __device__ inline void cell_of(const GridDev &G, d3 p, unsigned &cx, unsigned &cy, unsigned &cz)
@@ -402,7 +463,7 @@ __device__ inline long long lin_of(const GridDev &G, long long cx, long long cy,
 // in id order as the reference's per-cell vectors do (LeapConsultant.h:170-188).
 __device__ inline unsigned long long spread3(unsigned v)
 {
-    unsigned long long x = v & 0x1fffffull;
+    unsigned long long x = v & 0x1ffffull;   // 17 bits per axis (checked on the host)
     x = (x | (x << 32)) & 0x1f00000000ffffull;
     x = (x | (x << 16)) & 0x1f0000ff0000ffull;
     x = (x | (x << 8)) & 0x100f00f00f00f00full;
@@ -410,41 +471,53 @@ __device__ inline unsigned long long spread3(unsigned v)
     x = (x | (x << 2)) & 0x1249249249249249ull;
     return x;
 }
-__global__ void k_cellkey(Geo g, int n, GridDev G, const unsigned *inv, unsigned long long *keys, unsigned *vals)
+__global__ void k_cellkey(Geo g, int n_ranks, const SceneDev *scenes, const ModelDev *models, const unsigned *inv,
+                          unsigned long long retired_key, unsigned long long *keys, unsigned *vals)
 {
+    // key = scene slot (bits 51..63) | Morton code of the cell (17 bits per axis);
+    // ranks run scene by scene in DO-id order, retired scenes last
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= n_ranks) return;
     const unsigned i = inv[t];
+    const int sc = models[g.mdl[i]].scene;
+    const SceneDev &S = scenes[sc];
+    vals[t] = i;
+    if (S.active < 0) {
+        keys[t] = retired_key;
+        return;
+    }
     const double4 p = g.pos[i];
     unsigned cx, cy, cz;
-    cell_of(G, mk(p.x, p.y, p.z), cx, cy, cz);
-    keys[t] = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
-    vals[t] = i;
+    cell_of(S.G, mk(p.x, p.y, p.z), cx, cy, cz);
+    keys[t] = ((unsigned long long)sc << 51) | spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
 }
 
 // Gather state into cell order; per-sphere safe distance
 // SafetyFactor*(2*range + dt_up*|v| + 0.5*dt_up^2*|g|) (LeapConsultant.h:117-119).
-__global__ void k_reorder(Geo src, Geo dst, int n, const unsigned *perm, GridDev G, double *safe, int *lin,
-                          const double *imp_src, double *imp_dst)
+__global__ void k_reorder(Geo src, Geo dst, int n, int ld, const unsigned *perm, const SceneDev *scenes,
+                          const ModelDev *models, double *safe, int *lin, const double *imp_src, double *imp_dst)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const unsigned p = perm[s];
     const double4 x = src.pos[p];
     double4 v = src.vel[p];
+    const int m = src.mdl[p];
+    const SceneDev &S = scenes[models[m].scene];
+    const GridDev &G = S.G;
     dst.pos[s] = x;
     safe[s] = G.sf * (2.0 * x.w + G.safe_v * len(mk(v.x, v.y, v.z)) + G.safe_g);
     v.w = 0.0;
     dst.vel[s] = v;
     dst.avl[s] = src.avl[p];
-    for (int c = 0; c < 6; ++c) dst.ori[(size_t)c * n + s] = src.ori[(size_t)c * n + p];
+    for (int c = 0; c < 6; ++c) dst.ori[(size_t)c * ld + s] = src.ori[(size_t)c * ld + p];
     dst.oid[s] = src.oid[p];
-    dst.mdl[s] = src.mdl[p];
+    dst.mdl[s] = m;
     if (imp_src)   // last-step impulses follow their sphere
-        for (int c = 0; c < 6; ++c) imp_dst[(size_t)c * n + s] = imp_src[(size_t)c * n + p];
+        for (int c = 0; c < 6; ++c) imp_dst[(size_t)c * ld + s] = imp_src[(size_t)c * ld + p];
     unsigned cx, cy, cz;
     cell_of(G, mk(x.x, x.y, x.z), cx, cy, cz);
-    lin[s] = (int)lin_of(G, cx, cy, cz);
+    lin[s] = (int)(S.cell_off + lin_of(G, cx, cy, cz));
 }
 
 __global__ void k_fill_int(int *p, long long n, int v)
@@ -461,12 +534,12 @@ __global__ void k_cellstart(const int *lin, int n, int *cstart, int *cend)
     if (s == n - 1 || lin[s] != lin[s + 1]) cend[lin[s]] = s + 1;
 }
 
-__device__ inline bool accept_sphere(const OuterArgs &a, int s, d3 xs, double safe_s, int gs, int j)
+__device__ inline bool accept_sphere(const OuterArgs &a, const SceneDev &S, d3 xs, double safe_s, int gs, int j)
 {
     if (!a.single_group) {
         const int gj = a.t.models[a.g.mdl[j]].group;
-        if (a.t.laws[gs * a.t.n_groups + gj].law == LAW_NONE) return false;
-    } else if (a.t.laws[gs * a.t.n_groups + gs].law == LAW_NONE) {
+        if (a.t.laws[S.law_off + gs * S.n_groups + gj].law == LAW_NONE) return false;
+    } else if (a.t.laws[S.law_off + gs * S.n_groups + gs].law == LAW_NONE) {
         return false;
     }
     const double4 pj = a.g.pos[j];
@@ -480,11 +553,12 @@ __device__ inline bool accept_sphere(const OuterArgs &a, int s, d3 xs, double sa
     return !(len(d) > lim);
 }
 
-__device__ inline bool accept_wall(const OuterArgs &a, d3 xs, double safe_s, int gs, int w)
+__device__ inline bool accept_wall(const OuterArgs &a, const SceneDev &S, d3 xs, double safe_s, int gs, int w)
 {
     const WallDev &W = a.walls[w];
-    if (a.t.laws[gs * a.t.n_groups + W.group].law == LAW_NONE) return false;
-    const double sw = a.G.sf * (2.0 * W.range + a.G.safe_v * len(W.vel) + a.G.safe_g);
+    if (a.t.laws[S.law_off + gs * S.n_groups + W.group].law == LAW_NONE) return false;
+    const GridDev &G = S.G;
+    const double sw = G.sf * (2.0 * W.range + G.safe_v * len(W.vel) + G.safe_g);
     return !(domap_wall_distance(xs, W) > safe_s + sw);
 }
 
@@ -499,22 +573,26 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
     const double4 p = a.g.pos[s];
     const d3 xs = mk(p.x, p.y, p.z);
     const double safe_s = a.safe[s];
-    const int gs = a.t.models[a.g.mdl[s]].group;
+    const ModelDev &ms = a.t.models[a.g.mdl[s]];
+    const int gs = ms.group;
+    const SceneDev &S = a.scenes[ms.scene];
+    const GridDev &G = S.G;
+    const int *cstart = a.cstart + S.cell_off, *cend = a.cend + S.cell_off;
     unsigned cx, cy, cz;
-    cell_of(a.G, xs, cx, cy, cz);
+    cell_of(G, xs, cx, cy, cz);
     // 14 groups: own cell (slot -1) and the 13 cells that hold s's cell in their half shell
     unsigned long long gkey[14];
     int gslot[14];
     int ng = 0;
-    gkey[ng] = cmp_key(a.G, cx, cy, cz);
+    gkey[ng] = cmp_key(G, cx, cy, cz);
     gslot[ng++] = -1;
     for (int k = 0; k < 13; ++k) {
         const long long ox = (long long)cx - NB13[k][0], oy = (long long)cy - NB13[k][1],
                         oz = (long long)cz - NB13[k][2];
-        if (ox < 0 || oy < 0 || oz < 0 || ox >= a.G.nx - 2 || oy >= a.G.ny - 2 || oz >= a.G.nz - 2) continue;
-        const long long l = lin_of(a.G, ox, oy, oz);
-        if (a.cend[l] <= a.cstart[l]) continue;
-        gkey[ng] = cmp_key(a.G, (unsigned)ox, (unsigned)oy, (unsigned)oz);
+        if (ox < 0 || oy < 0 || oz < 0 || ox >= G.nx - 2 || oy >= G.ny - 2 || oz >= G.nz - 2) continue;
+        const long long l = lin_of(G, ox, oy, oz);
+        if (cend[l] <= cstart[l]) continue;
+        gkey[ng] = cmp_key(G, (unsigned)ox, (unsigned)oy, (unsigned)oz);
         gslot[ng++] = k;
     }
     for (int u = 1; u < ng; ++u) {   // insertion sort by cell key
@@ -531,16 +609,16 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
     }
     unsigned cnt = 0;
     auto emit = [&](unsigned j) {
-        if (cnt < (unsigned)a.ko) a.nbr[(size_t)cnt * a.n + s] = j;
+        if (cnt < (unsigned)a.ko) a.nbr[(size_t)cnt * a.ld + s] = j;
         ++cnt;
     };
     auto scan_cell = [&](long long x, long long y, long long z, bool self) {
-        if (x < -1 || y < -1 || z < -1 || x > a.G.nx - 3 || y > a.G.ny - 3 || z > a.G.nz - 3) return;
-        const long long l = lin_of(a.G, x, y, z);
-        const int b = a.cstart[l], e = a.cend[l];
+        if (x < -1 || y < -1 || z < -1 || x > G.nx - 3 || y > G.ny - 3 || z > G.nz - 3) return;
+        const long long l = lin_of(G, x, y, z);
+        const int b = cstart[l], e = cend[l];
         for (int j = b; j < e; ++j) {
             if (self && j == s) continue;
-            if (accept_sphere(a, s, xs, safe_s, gs, j)) emit((unsigned)j);
+            if (accept_sphere(a, S, xs, safe_s, gs, j)) emit((unsigned)j);
         }
     };
     for (int q = 0; q < ng; ++q) {
@@ -548,8 +626,8 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
             for (int k = 0; k < 13; ++k)
                 scan_cell((long long)cx + NB13[k][0], (long long)cy + NB13[k][1], (long long)cz + NB13[k][2], false);
             scan_cell(cx, cy, cz, true);
-            for (int w = 0; w < a.nw; ++w)
-                if (accept_wall(a, xs, safe_s, gs, w)) emit((unsigned)(a.n + w));
+            for (int w = S.w_begin; w < S.w_end; ++w)
+                if (accept_wall(a, S, xs, safe_s, gs, w)) emit((unsigned)(a.ld + w));
         } else {
             const int k = gslot[q];
             scan_cell((long long)cx - NB13[k][0], (long long)cy - NB13[k][1], (long long)cz - NB13[k][2], false);
@@ -560,13 +638,14 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
 }
 
 // Consultant::GetIactNum: sphere pairs are listed from both sides, walls once.
-__global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int n, int ko, unsigned long long *out)
+__global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int b, int e, int ld, int ko,
+                             unsigned long long *out)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+    const int s = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= e) return;
     unsigned sph = 0, wal = 0;
     for (unsigned t = 0; t < cnt[s] && t < (unsigned)ko; ++t) {
-        if (nbr[(size_t)t * n + s] < (unsigned)n) ++sph;
+        if (nbr[(size_t)t * ld + s] < (unsigned)ld) ++sph;
         else ++wal;
     }
     atomicAdd(&out[0], (unsigned long long)sph);
@@ -586,32 +665,33 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
     const unsigned hdr = a.eold.hdr[p];
     const int h_old = hdr >> 7, c_old = hdr & 127;
     for (int k = 0; k < c_old; ++k) {
-        const unsigned j = a.eold.ent[eidx(h_old, k, a.ki, a.n, p)] & PARTNER_MASK;
-        old_oid[k * blockDim.x + threadIdx.x] = j < (unsigned)a.n ? a.gold.oid[j] : 0x80000000u + (j - a.n);
+        const unsigned j = a.eold.ent[eidx(h_old, k, a.ki, a.ld, p)] & PARTNER_MASK;
+        old_oid[k * blockDim.x + threadIdx.x] = j < (unsigned)a.ld ? a.gold.oid[j] : 0x80000000u + (j - a.ld);
     }
+    const double skin = a.scenes[a.models[a.g.mdl[s]].scene].skin;
     const double4 ps = a.g.pos[s];
     const d3 xs = mk(ps.x, ps.y, ps.z);
     const unsigned my_oid = a.g.oid[s];
     int q = 0;
     for (unsigned t = 0; t < a.cnt[s]; ++t) {
-        const unsigned j = a.nbr[(size_t)t * a.n + s];
+        const unsigned j = a.nbr[(size_t)t * a.ld + s];
         Contact c;
         unsigned key;
         bool slave = true;
-        if (j < (unsigned)a.n) {
+        if (j < (unsigned)a.ld) {
             const double4 pj = a.g.pos[j];
             const d3 dx = mk(pj.x, pj.y, pj.z) - xs;
-            const double lim = ps.w + pj.w + a.skin;
+            const double lim = ps.w + pj.w + skin;
             if (dot(dx, dx) > lim * lim * (1.0 + 1e-9)) continue;   // surely beyond the skin
             key = a.g.oid[j];
             slave = my_oid < key;
             if (slave) cd_ss(xs, ps.w, mk(pj.x, pj.y, pj.z), pj.w, c);
             else cd_ss(mk(pj.x, pj.y, pj.z), pj.w, xs, ps.w, c);
         } else {
-            key = 0x80000000u + (j - a.n);
-            cd_wall(xs, ps.w, a.walls[j - a.n], c);
+            key = 0x80000000u + (j - a.ld);
+            cd_wall(xs, ps.w, a.walls[j - a.ld], c);
         }
-        if (c.depth < -a.skin) continue;
+        if (c.depth < -skin) continue;
         if (q >= a.ki) {
             atomicOr(&a.flags[3], ERR_INNER_CAP);
             break;
@@ -623,44 +703,44 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
         if (c.active) {
             for (int k = 0; k < c_old; ++k)
                 if (old_oid[k * blockDim.x + threadIdx.x] == key) {
-                    f = (a.eold.ent[eidx(h_old, k, a.ki, a.n, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
-                    if (f & F_CONTACT) load_hist(a.eold.hist, h_old, k, a.ki, a.n, p, hs);
+                    f = (a.eold.ent[eidx(h_old, k, a.ki, a.ld, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
+                    if (f & F_CONTACT) load_hist(a.eold.hist, h_old, k, a.ki, a.ld, p, hs);
                     break;
                 }
         }
         if (slave) f |= F_SLAVE;
-        a.ell.ent[eidx(0, q, a.ki, a.n, s)] = j | f;
-        if (f & F_CONTACT) store_hist(a.ell.hist, 0, q, a.ki, a.n, s, hs);
+        a.ell.ent[eidx(0, q, a.ki, a.ld, s)] = j | f;
+        if (f & F_CONTACT) store_hist(a.ell.hist, 0, q, a.ki, a.ld, s, hs);
         ++q;
     }
     a.ell.hdr[s] = (unsigned char)q;
     atomicMax(&a.flags[4], q);
 }
 
-// Fresh lists at scene creation (all statuses ImpactStatus()).
-__global__ void k_hdr_zero(unsigned char *hdr, int n)
+// Fresh lists for newly placed spheres (all statuses ImpactStatus()).
+__global__ void k_hdr_zero(unsigned char *hdr, int b, int e)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) hdr[i] = 0;
+    const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < e) hdr[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
 // Export
 // ---------------------------------------------------------------------------
 
-__global__ void k_scatter_state(Geo g, int n, const double *rec_imp, double *out)
+__global__ void k_scatter_state(Geo g, int b, int e, int ld, const double *rec_imp, double *out)
 {
-    // out: [n][21] in sphere-rank order: pos vel ox oz av impact ang_impact
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    // out: [n][21] in the scene's sphere-rank order: pos vel ox oz av impact ang_impact
+    const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
     const double4 p = g.pos[i], v = g.vel[i], w = g.avl[i];
     double *d = out + (size_t)g.oid[i] * 21;
     d[0] = p.x; d[1] = p.y; d[2] = p.z;
     d[3] = v.x; d[4] = v.y; d[5] = v.z;
-    d[6] = g.ori[i]; d[7] = g.ori[n + i]; d[8] = g.ori[2 * n + i];
-    d[9] = g.ori[3 * n + i]; d[10] = g.ori[4 * n + i]; d[11] = g.ori[5 * n + i];
+    d[6] = g.ori[i]; d[7] = g.ori[ld + i]; d[8] = g.ori[2 * (size_t)ld + i];
+    d[9] = g.ori[3 * (size_t)ld + i]; d[10] = g.ori[4 * (size_t)ld + i]; d[11] = g.ori[5 * (size_t)ld + i];
     d[12] = w.x; d[13] = w.y; d[14] = w.z;
-    for (int c = 0; c < 6; ++c) d[15 + c] = rec_imp ? rec_imp[(size_t)c * n + i] : 0.0;
+    for (int c = 0; c < 6; ++c) d[15 + c] = rec_imp ? rec_imp[(size_t)c * ld + i] : 0.0;
 }
 
 // Active contacts at the current positions with their ImpactStatus.  The
@@ -670,21 +750,23 @@ __global__ void k_scatter_state(Geo g, int n, const double *rec_imp, double *out
 // neighbour list is an ordered subsequence of the candidate list, so a merge
 // walk finds each listed pair's entry (flags, history, records); a candidate
 // outside the list has been solved as inactive every step: Clean status.
-__global__ void k_export_active(Geo g, Ell ell, int n, int ki, const unsigned *o_cnt, const unsigned *o_nbr,
-                                const WallDev *walls, const unsigned *wall_id, const unsigned *sphere_id,
-                                const double *rec, ActiveOut *out, unsigned long long cap,
+__global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *o_cnt,
+                                const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,
+                                const unsigned *sphere_id, const double *rec, ActiveOut *out, unsigned long long cap,
                                 unsigned long long *counter)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    // spheres b..e-1 (one scene); sphere_id maps the scene's ranks to DO ids
+    const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
+    const int n = ld;
     const unsigned hdr = ell.hdr[i];
     const int h = hdr >> 7, cnt = hdr & 127;
     const double4 p = g.pos[i];
     const d3 xi = mk(p.x, p.y, p.z);
-    const unsigned e = o_cnt[i];
+    const unsigned ec = o_cnt[i];
     int q = 0;
     unsigned listed = cnt > 0 ? (ell.ent[eidx(h, 0, ki, n, i)] & PARTNER_MASK) : 0xffffffffu;
-    for (unsigned t = 0; t < e; ++t) {
+    for (unsigned t = 0; t < ec; ++t) {
         const unsigned j = o_nbr[(size_t)t * n + i];
         int k = -1;
         unsigned ent;
@@ -728,18 +810,19 @@ __global__ void k_export_active(Geo g, Ell ell, int n, int ki, const unsigned *o
 // Impulse on each wall from the last recorded step: sum of ImpactToMaster
 // and AngularImpactToMaster over its contacts (the DOStatus impact a wall's
 // ClearImpact stores), one block per wall, fixed-order reduction.
-__global__ void k_wall_impulse(Ell ell, int n, int ki, const double *rec, double *out)
+__global__ void k_wall_impulse(Ell ell, int b, int e, int ld, int ki, int w0, const double *rec, double *out)
 {
-    const int w = blockIdx.x;
+    // one block per wall w0 + blockIdx.x, over the scene's spheres b..e-1
+    const int w = w0 + blockIdx.x;
     __shared__ double sm[6][256];
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
         const unsigned hdr = ell.hdr[i];
         const int h = hdr >> 7, cnt = hdr & 127;
         for (int k = 0; k < cnt; ++k) {
-            const unsigned j = ell.ent[eidx(h, k, ki, n, i)] & PARTNER_MASK;
-            if (j != (unsigned)(n + w)) continue;
-            for (int c = 0; c < 6; ++c) acc[c] += rec[((size_t)k * 13 + 6 + c) * n + i];
+            const unsigned j = ell.ent[eidx(h, k, ki, ld, i)] & PARTNER_MASK;
+            if (j != (unsigned)(ld + w)) continue;
+            for (int c = 0; c < 6; ++c) acc[c] += rec[((size_t)k * 13 + 6 + c) * ld + i];
         }
     }
     for (int c = 0; c < 6; ++c) sm[c][threadIdx.x] = acc[c];
@@ -750,25 +833,25 @@ __global__ void k_wall_impulse(Ell ell, int n, int ki, const double *rec, double
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int c = 0; c < 6; ++c) out[w * 6 + c] = sm[c][0];
+        for (int c = 0; c < 6; ++c) out[blockIdx.x * 6 + c] = sm[c][0];
 }
 
 // Assigned sphere statuses of SimMediator::Run(map) (SimMediator.cpp:881-889):
 // after Response, the listed DOs take the given status, which carries zero
 // stored impulses.  buf holds 16 doubles per sphere rank: pos, vel, Ox, Oz,
 // w, and a nonzero last value for the listed ones.
-__global__ void k_assign_spheres(Geo g, int n, const double *buf, double *rec_imp)
+__global__ void k_assign_spheres(Geo g, int b, int e, int ld, const double *buf, double *rec_imp)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double *b = buf + (size_t)g.oid[i] * 16;
-    if (b[15] == 0.0) return;
-    g.pos[i] = make_double4(b[0], b[1], b[2], g.pos[i].w);
-    g.vel[i] = make_double4(b[3], b[4], b[5], 0.0);   // the host forces a relist
-    for (int c = 0; c < 6; ++c) g.ori[(size_t)c * n + i] = b[6 + c];
-    g.avl[i] = make_double4(b[12], b[13], b[14], g.avl[i].w);
+    const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
+    const double *d = buf + (size_t)g.oid[i] * 16;
+    if (d[15] == 0.0) return;
+    g.pos[i] = make_double4(d[0], d[1], d[2], g.pos[i].w);
+    g.vel[i] = make_double4(d[3], d[4], d[5], 0.0);   // the host forces a relist
+    for (int c = 0; c < 6; ++c) g.ori[(size_t)c * ld + i] = d[6 + c];
+    g.avl[i] = make_double4(d[12], d[13], d[14], g.avl[i].w);
     if (rec_imp)
-        for (int c = 0; c < 6; ++c) rec_imp[(size_t)c * n + i] = 0.0;
+        for (int c = 0; c < 6; ++c) rec_imp[(size_t)c * ld + i] = 0.0;
 }
 
 // DOWorld::CalculateSystemEnergy (DOWorld.cpp:481-632) over the spheres (the
@@ -817,15 +900,16 @@ __device__ inline void en_block_reduce(double (*sh)[EN_BLOCK], double *acc, int 
     if (t < K) out[t] = sh[t][0];
 }
 
-__global__ __launch_bounds__(EN_BLOCK) void k_energy_part(Geo g, int n, const double2 *mm, double gx, double gy,
-                                                          double gz, const double *avg, int pass, double *part)
+__global__ __launch_bounds__(EN_BLOCK) void k_energy_part(Geo g, int b, int e, const double2 *mm, double gx,
+                                                          double gy, double gz, const double *avg, int pass,
+                                                          double *part)
 {
     __shared__ double sh[EN_K0][EN_BLOCK];
     double acc[EN_K0];
     for (int k = 0; k < EN_K0; ++k) acc[k] = 0.0;
     acc[9] = acc[11] = -INFINITY;
     acc[10] = acc[12] = INFINITY;
-    for (int i = blockIdx.x * EN_BLOCK + threadIdx.x; i < n; i += gridDim.x * EN_BLOCK) {
+    for (int i = b + blockIdx.x * EN_BLOCK + threadIdx.x; i < e; i += gridDim.x * EN_BLOCK) {
         d3 mv, Lw;
         double t[5];
         en_terms(g, i, mm, gx, gy, gz, mv, Lw, t);

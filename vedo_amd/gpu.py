@@ -50,6 +50,11 @@ class VgSceneDesc(C.Structure):
                 ("inner_skin", C.c_double)]
 
 
+class VgBatchDesc(C.Structure):
+    _fields_ = [("max_spheres", C.c_longlong), ("max_walls", C.c_int), ("max_scenes", C.c_int),
+                ("neighbour_capacity", C.c_int), ("up_iact", C.c_uint)]
+
+
 STATUS_DTYPE = np.dtype([("contact", "u1"), ("last_contact", "u1"), ("bond", "u1"), ("last_bond", "u1"),
                          ("pad", "u1", 4), ("kn", "f8"), ("init_vel", "f8"), ("shear", "f8", 3),
                          ("point", "f8", 3), ("dir", "f8", 3), ("to_master", "f8", 3),
@@ -59,7 +64,9 @@ assert STATUS_DTYPE.itemsize == C.sizeof(VgStatus)
 EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_set_history", "vg_step",
            "vg_sync", "vg_step_assigned", "vg_set_wall_rotation", "vg_get_state", "vg_time",
            "vg_num_interactions", "vg_set_skin", "vg_get_skin", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
-           "vg_system_energy")
+           "vg_system_energy", "vg_batch_create", "vg_admit", "vg_retire", "vg_scene_status",
+           "vg_scene_get_state", "vg_scene_get_active", "vg_scene_energy", "vg_scene_set_wall_rotation",
+           "vg_batch_info")
 
 _lib = None
 
@@ -111,9 +118,19 @@ def lib():
         L.vg_get_skin.restype = C.c_double
         L.vg_get_skin.argtypes = [vp]
         L.vg_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_int),
-                               C.POINTER(C.c_int)]
+                               C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
         L.vg_copy_bench.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.vg_system_energy.argtypes = [vp, _D]
+        L.vg_batch_create.argtypes = [C.POINTER(VgBatchDesc), C.POINTER(vp)]
+        L.vg_admit.argtypes = [vp, C.POINTER(VgSceneDesc), C.c_longlong, C.POINTER(C.c_int)]
+        L.vg_retire.argtypes = [vp, C.c_int]
+        L.vg_scene_status.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong), _D]
+        L.vg_scene_get_state.argtypes = [vp, C.c_int, _D, _D, _D, _D, _D]
+        L.vg_scene_get_active.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), _U, _U, C.POINTER(VgStatus)]
+        L.vg_scene_energy.argtypes = [vp, C.c_int, _D]
+        L.vg_scene_set_wall_rotation.argtypes = [vp, C.c_int, C.c_int, _U, _D, _D, C.c_double, C.c_double]
+        L.vg_batch_info.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_ulonglong)]
         _lib = L
     return _lib
 
@@ -131,6 +148,42 @@ def _dp(a):
     return a.ctypes.data_as(_D)
 
 
+def scene_desc(wa, inner_skin: float = 0.0):
+    """vg_scene_desc of a WorldArrays; returns (desc, buffers to keep alive)."""
+    M = len(wa.model_shape)
+    G = wa.iact_law.shape[0]
+    models = (VgModel * M)()
+    for k in range(M):
+        m = models[k]
+        m.shape, m.behavior, m.scope, m.group = (int(wa.model_shape[k]), int(wa.model_behavior[k]),
+                                                 int(wa.model_scope[k]), int(wa.model_group[k]))
+        m.density, m.density_factor = float(wa.model_density[k]), float(wa.model_density_factor[k])
+        for d in range(3):
+            m.fext[d] = float(wa.model_fext[k, d])
+            m.attr[d] = float(wa.model_attr[k, d])
+    iact = (VgIact * (G * G))()
+    for i in range(G):
+        for j in range(G):
+            it = iact[i * G + j]
+            it.law = int(wa.iact_law[i, j])
+            for p in range(8):
+                it.p[p] = float(wa.iact_param[i, j, p])
+    arrs = [np.ascontiguousarray(x, np.float64) for x in (wa.pos, wa.vel, wa.ox, wa.oz, wa.av)]
+    model_of = np.ascontiguousarray(wa.model_of, np.int32)
+    d = VgSceneDesc()
+    d.n_do, d.n_models, d.n_groups = wa.n_do, M, G
+    d.models, d.iact = models, iact
+    d.model_of = model_of.ctypes.data_as(C.POINTER(C.c_int))
+    d.pos, d.vel, d.ox, d.oz, d.av = (_dp(a) for a in arrs)
+    d.dt, d.t_start, d.t_stop, d.t_current = wa.dt, wa.t_start, wa.t_stop, wa.t_current
+    for c in range(3):
+        d.g[c] = float(wa.g[c])
+    d.safety_factor = wa.safety_factor
+    d.up_iact, d.record, d.consultant = wa.up_iact, wa.record, 0
+    d.inner_skin = inner_skin
+    return d, (models, iact, arrs, model_of)
+
+
 def _up(a):
     return a.ctypes.data_as(_U)
 
@@ -142,37 +195,7 @@ class GpuSimulation:
         L = lib()
         self.L = L
         self.n = wa.n_do
-        M = len(wa.model_shape)
-        G = wa.iact_law.shape[0]
-        self._models = (VgModel * M)()
-        for k in range(M):
-            m = self._models[k]
-            m.shape, m.behavior, m.scope, m.group = (int(wa.model_shape[k]), int(wa.model_behavior[k]),
-                                                     int(wa.model_scope[k]), int(wa.model_group[k]))
-            m.density, m.density_factor = float(wa.model_density[k]), float(wa.model_density_factor[k])
-            for d in range(3):
-                m.fext[d] = float(wa.model_fext[k, d])
-                m.attr[d] = float(wa.model_attr[k, d])
-        self._iact = (VgIact * (G * G))()
-        for i in range(G):
-            for j in range(G):
-                it = self._iact[i * G + j]
-                it.law = int(wa.iact_law[i, j])
-                for p in range(8):
-                    it.p[p] = float(wa.iact_param[i, j, p])
-        arrs = [np.ascontiguousarray(x, np.float64) for x in (wa.pos, wa.vel, wa.ox, wa.oz, wa.av)]
-        model_of = np.ascontiguousarray(wa.model_of, np.int32)
-        d = VgSceneDesc()
-        d.n_do, d.n_models, d.n_groups = self.n, M, G
-        d.models, d.iact = self._models, self._iact
-        d.model_of = model_of.ctypes.data_as(C.POINTER(C.c_int))
-        d.pos, d.vel, d.ox, d.oz, d.av = (_dp(a) for a in arrs)
-        d.dt, d.t_start, d.t_stop, d.t_current = wa.dt, wa.t_start, wa.t_stop, wa.t_current
-        for c in range(3):
-            d.g[c] = float(wa.g[c])
-        d.safety_factor = wa.safety_factor
-        d.up_iact, d.record, d.consultant = wa.up_iact, wa.record, 0
-        d.inner_skin = inner_skin
+        d, self._keep = scene_desc(wa, inner_skin)
         h = C.c_void_p()
         rc = L.vg_create(C.byref(d), C.byref(h))
         if rc:
@@ -260,6 +283,113 @@ class GpuSimulation:
         return ms.value, k.value
 
     def stats(self):
-        rb, rl, mf, cap = C.c_longlong(), C.c_longlong(), C.c_int(), C.c_int()
-        self._check(self.L.vg_stats(self.h, C.byref(rb), C.byref(rl), C.byref(mf), C.byref(cap)))
-        return dict(rebuilds=rb.value, relists=rl.value, max_fill=mf.value, capacity=cap.value)
+        rb, rl, mf, cap, cs = C.c_longlong(), C.c_longlong(), C.c_int(), C.c_int(), C.c_longlong()
+        self._check(self.L.vg_stats(self.h, C.byref(rb), C.byref(rl), C.byref(mf), C.byref(cap), C.byref(cs)))
+        return dict(rebuilds=rb.value, relists=rl.value, max_fill=mf.value, capacity=cap.value,
+                    contact_steps=cs.value)
+
+
+SCENE_FREE, SCENE_QUEUED, SCENE_RUNNING, SCENE_FINISHED, SCENE_RETIRING, SCENE_DONE = range(6)
+
+
+class GpuBatch:
+    """Many independent scenes on the current GPU, stepped together (one
+    kernel launch per step for all of them).  Scenes are admitted with a step
+    count, placed on the device at the next rebuild boundary, frozen when
+    they reach their step count and moved to the host at the boundary after;
+    each scene's result equals running it alone."""
+
+    def __init__(self, max_spheres: int, max_walls: int = 4096, max_scenes: int = 1024, up_iact: int = 20,
+                 neighbour_capacity: int = 0):
+        L = lib()
+        self.L = L
+        bd = VgBatchDesc()
+        bd.max_spheres, bd.max_walls, bd.max_scenes = int(max_spheres), int(max_walls), int(max_scenes)
+        bd.neighbour_capacity, bd.up_iact = int(neighbour_capacity), int(up_iact)
+        h = C.c_void_p()
+        rc = L.vg_batch_create(C.byref(bd), C.byref(h))
+        if rc:
+            raise GpuError(f"vg_batch_create failed ({rc}): {L.vg_create_error().decode()}")
+        self.h = h
+        self.n_do = {}
+
+    def _check(self, rc):
+        if rc:
+            raise GpuError(f"libvedo_hip error {rc}: {self.L.vg_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.vg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def admit(self, wa, steps: int = -1, inner_skin: float = 0.0) -> int:
+        d, keep = scene_desc(wa, inner_skin)
+        sid = C.c_int()
+        self._check(self.L.vg_admit(self.h, C.byref(d), int(steps), C.byref(sid)))
+        del keep
+        self.n_do[sid.value] = wa.n_do
+        return sid.value
+
+    def retire(self, scene: int):
+        self._check(self.L.vg_retire(self.h, int(scene)))
+
+    def step(self, k: int = 1, sync: bool = True):
+        self._check(self.L.vg_step(self.h, int(k)))
+        if sync:
+            self.sync()
+
+    def sync(self):
+        self._check(self.L.vg_sync(self.h))
+
+    def status(self, scene: int):
+        st, n, t = C.c_int(), C.c_longlong(), C.c_double()
+        self._check(self.L.vg_scene_status(self.h, int(scene), C.byref(st), C.byref(n), C.byref(t)))
+        return st.value, n.value, t.value
+
+    def state(self, scene: int):
+        n = self.n_do[scene]
+        out = {k: np.zeros((n, 3)) for k in ("pos", "vel", "ox", "oz", "av")}
+        self._check(self.L.vg_scene_get_state(self.h, int(scene), *(_dp(out[k]) for k in
+                                                                    ("pos", "vel", "ox", "oz", "av"))))
+        return out
+
+    def active(self, scene: int):
+        n = C.c_longlong(0)
+        self._check(self.L.vg_scene_get_active(self.h, int(scene), C.byref(n), None, None, None))
+        cap = n.value
+        lo = np.zeros(cap, np.uint32)
+        hi = np.zeros(cap, np.uint32)
+        st = np.zeros(cap, STATUS_DTYPE)
+        n = C.c_longlong(cap)
+        self._check(self.L.vg_scene_get_active(self.h, int(scene), C.byref(n), _up(lo), _up(hi),
+                                               st.ctypes.data_as(C.POINTER(VgStatus))))
+        return lo[:n.value], hi[:n.value], st[:n.value]
+
+    def energy(self, scene: int):
+        out = np.zeros(16)
+        self._check(self.L.vg_scene_energy(self.h, int(scene), _dp(out)))
+        return energy_dict(out)
+
+    def set_wall_rotation(self, scene: int, ids, axis_point, axis_dir, omega: float, t_ref: float = 0.0):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        c = np.ascontiguousarray(axis_point, np.float64)
+        a = np.ascontiguousarray(axis_dir, np.float64)
+        self._check(self.L.vg_scene_set_wall_rotation(self.h, int(scene), len(ids), _up(ids), _dp(c), _dp(a),
+                                                      omega, t_ref))
+
+    def set_skin(self, skin: float):
+        self._check(self.L.vg_set_skin(self.h, float(skin)))
+
+    def info(self):
+        sp, on, q, st = C.c_longlong(), C.c_int(), C.c_int(), C.c_ulonglong()
+        self._check(self.L.vg_batch_info(self.h, C.byref(sp), C.byref(on), C.byref(q), C.byref(st)))
+        return dict(spheres=sp.value, on_device=on.value, queued=q.value, steps=st.value)
+
+    def stats(self):
+        rb, rl, mf, cap, cs = C.c_longlong(), C.c_longlong(), C.c_int(), C.c_int(), C.c_longlong()
+        self._check(self.L.vg_stats(self.h, C.byref(rb), C.byref(rl), C.byref(mf), C.byref(cap), C.byref(cs)))
+        return dict(rebuilds=rb.value, relists=rl.value, max_fill=mf.value, capacity=cap.value,
+                    contact_steps=cs.value)

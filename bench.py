@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -107,6 +108,113 @@ def settle(sim, n, total_mass, max_steps, chunk=100, v_rms_stop=2e-3):
     return done, quenches
 
 
+def batched_boxes(count, steps, up_iact, seed=4, loop=True):
+    """The batched configuration: `count` box-settling scenes (ISwH), sphere
+    counts log-uniform in [1e3, 2e5], mu ~ U[0.2, 0.7], e ~ U[0.3, 0.9],
+    E ~ logU[1e6, 1e8], r = 1 mm, box sized to its lattice, dt = 0.1 of the
+    critical time step.  All scenes run `steps` steps in one batch (admission
+    to results on the host, timed), and, when `loop`, the plain per-scene GPU
+    loop: each scene alone, one after the other, in its own handle."""
+    from vedo_amd import scenes
+    from vedo_amd.gpu import SCENE_DONE, SCENE_FAILED, GpuBatch, GpuError, GpuSimulation
+    rng = scenes.SplitMix64(seed)
+    u = rng.uniform(4 * count, 0.0, 1.0).reshape(count, 4)
+    was = []
+    for k in range(count):
+        n = int(round(math.exp(math.log(1e3) + u[k, 0] * (math.log(2e5) - math.log(1e3)))))
+        was.append(scenes.box_arrays(n, seed=seed * 1000 + k, law="ISwH", mu=0.2 + 0.5 * u[k, 1],
+                                     e=0.3 + 0.6 * u[k, 2], E=10.0 ** (6.0 + 2.0 * u[k, 3]), steps=steps,
+                                     up_iact=up_iact))
+    n_tot = sum(int((wa.model_shape[wa.model_of] == 1).sum()) for wa in was)
+    ps = float(n_tot) * steps
+    log(f"[bench] batched: {count} box scenes, {n_tot} spheres, {steps} steps each")
+    t0 = time.perf_counter()
+    b = GpuBatch(n_tot, max_walls=6 * count, max_scenes=count, up_iact=up_iact)
+    ids = [b.admit(wa, steps) for wa in was]
+    t_adm = time.perf_counter() - t0
+    # neighbour-list skin tuned on the batch's first steps (part of the run)
+    from vedo_amd.autotune import tune_skin
+    skin, tune = tune_skin(b, 1e-3, up_iact, factors=(0.25, 0.5, 1.0, 2.0))
+    log(f"[bench] batched: skin {skin * 1e3:.2f} mm ({({f'{k * 1e3:.2f}': round(v * 1e3, 2) for k, v in tune.items()})} "
+        f"ms/step)")
+    periods = 4 * up_iact // up_iact
+    while True:
+        b.step(up_iact)
+        periods += 1
+        states = [b.status(s)[0] for s in ids]
+        if periods % 5 == 0:
+            log(f"[bench] batched: {periods * up_iact} steps, {sum(x == SCENE_DONE for x in states)} done, "
+                f"{sum(x == SCENE_FAILED for x in states)} failed")
+        if all(x in (SCENE_DONE, SCENE_FAILED) for x in states):
+            break
+    t_run = time.perf_counter() - t0
+    ok = [s for s, x in zip(ids, states) if x == SCENE_DONE]
+    res = [b.state(s) for s in ok]
+    t_batch = time.perf_counter() - t0
+    log(f"[bench] batched: admission {t_adm:.2f} s, stepping {t_run - t_adm:.2f} s, results {t_batch - t_run:.2f} s")
+    stats = b.stats()
+    b.close()
+    del res
+    failed = [k for k, x in enumerate(states) if x == SCENE_FAILED]
+    n_ok = sum(int((was[k].model_shape[was[k].model_of] == 1).sum()) for k in range(count) if k not in failed)
+    ps = float(n_ok) * steps
+    out = {"scenes": count, "spheres": n_tot, "steps_per_scene": steps, "up_iact": up_iact,
+           "failed_scenes": len(failed), "batched_seconds": t_batch, "batched_particle_steps_per_s": ps / t_batch,
+           "batch_steps": periods * up_iact, "relists": stats["relists"], "skin_m": skin}
+    log(f"[bench] batched: {t_batch:.2f} s, {ps / t_batch:.3e} particle-steps/s ({len(failed)} scenes blew up: "
+        f"soft ISwH contacts)")
+    if loop:
+        t0 = time.perf_counter()
+        for k, wa in enumerate(was):
+            if k in failed:    # the same scene blows up alone: skipped in both
+                continue
+            sim = GpuSimulation(wa)
+            try:
+                sim.step(steps)
+                sim.state()
+            except GpuError:
+                log(f"[bench] per-scene loop: scene {k} failed alone")
+            sim.close()
+            if k % 64 == 63:
+                log(f"[bench] per-scene loop: {k + 1} scenes, {time.perf_counter() - t0:.1f} s")
+        t_loop = time.perf_counter() - t0
+        out.update({"loop_seconds": t_loop, "loop_particle_steps_per_s": ps / t_loop,
+                    "speedup_vs_loop": t_loop / t_batch})
+        log(f"[bench] per-scene loop: {t_loop:.2f} s, {ps / t_loop:.3e} particle-steps/s, "
+            f"batched/loop {t_loop / t_batch:.2f}x")
+    return out
+
+
+def serving_run(count, up_iact, seed=5, loop=True):
+    """The serving configuration: `count` requests of a Poisson stream (20/s)
+    of drum / hopper / box scenes, 5e3-5e5 spheres, 2e3-2e4 steps, ISwH or
+    ISwBSDBF, served by continuous batching; p50/p99 time-to-result and the
+    aggregate throughput, and the same stream served one scene at a time."""
+    from vedo_amd import serving
+    reqs = serving.poisson_requests(count, rate=20.0, seed=seed)
+    for r in reqs:
+        serving.build_scene(r, up_iact=up_iact)
+    need = sum(r.n for r in reqs)
+    cap = min(need, 4_000_000)
+    log(f"[bench] serving: {count} requests, {need} spheres in all, capacity {cap} spheres "
+        f"({cap * serving.slot_bytes() / 2 ** 30:.1f} GiB)")
+    srv = serving.Server(capacity=cap, up_iact=up_iact, max_walls=80 * count, max_scenes=max(count, 16),
+                         keep_results=False)
+    st = srv.run(reqs)
+    srv.close()
+    out = {"requests": count, "rate_per_s": 20.0, "spheres": need, "capacity_spheres": cap, "failed": st.failed,
+           "seconds": st.wall, "particle_steps_per_s": st.throughput, "p50_s": st.pct(50), "p99_s": st.pct(99)}
+    log(f"[bench] serving: {st.wall:.2f} s, {st.throughput:.3e} particle-steps/s, "
+        f"time-to-result p50 {st.pct(50):.2f} s p99 {st.pct(99):.2f} s")
+    if loop:
+        st1 = serving.serve_one_by_one(reqs)
+        out.update({"loop_seconds": st1.wall, "loop_particle_steps_per_s": st1.throughput,
+                    "loop_p50_s": st1.pct(50), "loop_p99_s": st1.pct(99), "loop_failed": st1.failed})
+        log(f"[bench] serving one scene at a time: {st1.wall:.2f} s, {st1.throughput:.3e} particle-steps/s, "
+            f"p50 {st1.pct(50):.2f} s p99 {st1.pct(99):.2f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +227,11 @@ def main():
     ap.add_argument("--skin", type=float, default=None, help="neighbour-list skin in m (skips the tuning)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=256, help="batched configuration: scenes (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=1000, help="batched configuration: steps per scene")
+    ap.add_argument("--serve-requests", type=int, default=24, help="serving configuration: requests (0: skip)")
+    ap.add_argument("--no-loop", action="store_true", help="skip the one-scene-at-a-time comparisons")
+    ap.add_argument("--batch-up-iact", type=int, default=20, help="UpIact of the batched and served scenes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU holds the whole domain); "
@@ -200,6 +313,11 @@ def main():
     else:
         out["cpu_baseline"] = None
     sim.close()
+    del sim
+    if args.batch_scenes > 0:
+        out["batched"] = batched_boxes(args.batch_scenes, args.batch_steps, args.batch_up_iact, loop=not args.no_loop)
+    if args.serve_requests > 0:
+        out["serving"] = serving_run(args.serve_requests, args.batch_up_iact, loop=not args.no_loop)
     print(json.dumps(out), flush=True)
 
 

@@ -54,6 +54,7 @@ struct SceneHost {
     std::vector<char> kin0_set;
     // result kept on the host once a finished scene leaves the device
     std::vector<double> result;         // [n_do][21]
+    bool failed = false;                // blew up: left the batch without a result
 };
 
 struct vg_scene {
@@ -98,6 +99,8 @@ struct vg_scene {
     double *assigned = nullptr;
     double *assigned_sph = nullptr;     // [ld][16] assigned sphere statuses by rank (allocated on first use)
     double *en_part = nullptr, *en_out = nullptr;
+    double *fetch = nullptr;            // state download scratch (grows)
+    size_t fetch_cap = 0;
     int *flags = nullptr;
     unsigned long long *relists = nullptr;
     unsigned long long *contacts = nullptr;   // per-wavefront contact-pair counts, summed by vg_stats
@@ -106,6 +109,7 @@ struct vg_scene {
     unsigned long long *keys = nullptr, *skeys = nullptr;
     double *xs[2] = {nullptr, nullptr}, *ys[2] = {nullptr, nullptr}, *zs[2] = {nullptr, nullptr};
     long long *qidx = nullptr, *segs = nullptr;
+    int *ovf = nullptr;                 // [scap] candidate-capacity overflow per scene
     unsigned *sid[2] = {nullptr, nullptr};   // scene slot of each sphere (quantile sorts)
     double *qval = nullptr;
     double *safe = nullptr;
@@ -193,6 +197,20 @@ static double model_range(const vg_model &m)
 }
 
 static const double PI = 4.0 * std::atan(1.0);
+
+#include <chrono>
+// VG_DEBUG phase timing of the rebuild (host clock)
+struct PhaseClock {
+    bool on = getenv("VG_DEBUG") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(const char *what)
+    {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        fprintf(stderr, "[vg] %-28s %8.3f ms\n", what, std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Device tables
@@ -462,6 +480,9 @@ static int place_queued(vg_scene *h)
         SceneHost &S = h->sc[k];
         if (S.state != VG_SCENE_QUEUED) continue;
         const int b = h->n, n = S.n, ld = h->ld;
+        // scenes leaving at this boundary still hold their slots until the
+        // sort: a scene that does not fit now is placed at the next boundary
+        if (b + n > ld) continue;
         std::vector<unsigned> oid(n);
         for (int s = 0; s < n; ++s) oid[s] = (unsigned)s;
         HIPCHK(hipMemcpy(G.pos + b, S.pos0.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
@@ -516,6 +537,7 @@ static int place_queued(vg_scene *h)
 static int rebuild(vg_scene *h)
 {
     hipStream_t st = h->stream;
+    PhaseClock pc;
     // finished scenes: keep the result, retire
     bool retiring = false;
     for (int k = 0; k < h->scap; ++k) {
@@ -529,7 +551,9 @@ static int rebuild(vg_scene *h)
             retiring = true;
         }
     }
+    pc.mark("finished / retiring");
     if (place_queued(h)) return VG_EHIP;
+    pc.mark("place queued");
     const int n_total = h->n;
     // rank offsets: live scenes by slot, then the retiring ones
     int n_live = 0, max_slot = -1;
@@ -552,6 +576,7 @@ static int rebuild(vg_scene *h)
             }
     }
     if (upload_scenes(h)) return VG_EHIP;
+    pc.mark("rank offsets + upload");
     Geo &G0 = h->geo[h->cur];
     Geo &G1 = h->geo[h->cur ^ 1];
     const int S_live = (int)live.size();
@@ -582,6 +607,7 @@ static int rebuild(vg_scene *h)
             qi[2 * q + 1] = base + ihi;
         }
         HIPCHK(hipMemcpy(h->qidx, qi.data(), sizeof(long long) * qi.size(), hipMemcpyHostToDevice));
+        pc.mark("bounds launch + qidx");
         const bool by_scene = S_live > 1 || retiring;
         int sbits = 1;
         while ((1 << sbits) <= max_slot + 1) ++sbits;
@@ -590,6 +616,7 @@ static int rebuild(vg_scene *h)
                                         0, 64, st);
         (void)rocprim::radix_sort_pairs(nullptr, need2, h->sid[1], h->sid[0], h->xs[1], h->xs[0], (size_t)n_total,
                                         0, sbits, st);
+        pc.mark("sort size queries");
         if (ensure_tmp(h, std::max(need, need2))) return VG_EHIP;
         double *coord[3][2] = {{h->xs[0], h->xs[1]}, {h->ys[0], h->ys[1]}, {h->zs[0], h->zs[1]}};
         for (int c = 0; c < 3; ++c) {
@@ -613,6 +640,7 @@ static int rebuild(vg_scene *h)
         HIPCHK(hipMemcpyAsync(red.data(), h->red, sizeof(unsigned long long) * red.size(), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(q6.data(), h->qval, sizeof(double) * q6.size(), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        pc.mark("sorts + readback");
         for (int q = 0; q < S_live; ++q) {
             const int k = live[q];
             const SceneHost &S = h->sc[k];
@@ -650,6 +678,36 @@ static int rebuild(vg_scene *h)
             cell_off[k] = cells;
             cells += c;
         }
+        if (h->batch) {
+            // a scene whose fastest sphere crosses more than its largest radius
+            // in one step has blown up (contact detection cannot follow it): it
+            // leaves the batch (failed) and the rebuild starts over without it;
+            // the other scenes are not affected
+            bool failed = false;
+            for (int q = 0; q < S_live; ++q) {
+                const int k = live[q];
+                const unsigned long long *rr = &red[(size_t)8 * k];
+                const double vmax = from_ord(rr[6]), rmax = from_ord(rr[7]);
+                if (!(vmax * h->sc[k].dt <= rmax)) {
+                    h->sc[k].state = VG_SCENE_RETIRING;
+                    h->sc[k].failed = true;
+                    h->sdev[k].active = -1;
+                    failed = true;
+                    if (getenv("VG_DEBUG"))
+                        fprintf(stderr, "[vg] scene %d failed: vmax %.3e m/s, rmax %.3e m\n", k, vmax, rmax);
+                }
+            }
+            if (failed) return rebuild(h);
+        }
+        if (getenv("VG_DEBUG")) {
+            for (int q = 0; q < S_live; ++q) {
+                const int k = live[q];
+                const unsigned long long *rr = &red[(size_t)8 * k];
+                const GridDev &gd = grids[k];
+                fprintf(stderr, "[vg] rebuild %lld scene %d n %d vmax %.3e rmax %.3e zr %.3e cells %dx%dx%d\n",
+                        h->rebuilds, k, h->sc[k].n, from_ord(rr[6]), from_ord(rr[7]), gd.zr, gd.nx, gd.ny, gd.nz);
+            }
+        }
         if (cells > (1ll << 31)) return fail(h, VG_EUNSUPPORTED, "cell grid too large (sparse domain)");
         for (int q = 0; q < S_live; ++q) {
             const int k = live[q];
@@ -684,6 +742,7 @@ static int rebuild(vg_scene *h)
         HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, h->keys, h->skeys, h->svals, h->perm, (size_t)n_total, 0,
                                          end_bit, st));
     }
+    pc.mark("grids + cell sort launch");
     const int n = n_live;
     if (n > 0) {
         hipLaunchKernelGGL(k_reorder, dim3(nblk(n, 256)), dim3(256), 0, st, G0, G1, n, h->ld, (const unsigned *)h->perm,
@@ -696,6 +755,8 @@ static int rebuild(vg_scene *h)
         oa.g = G1; oa.n = n; oa.ld = h->ld; oa.scenes = h->scenes; oa.cstart = h->cstart; oa.cend = h->cend;
         oa.safe = h->safe; oa.walls = h->walls; oa.t = tables(h); oa.count = h->ocount; oa.nbr = h->onbr;
         oa.ko = h->ko; oa.single_group = h->single_group; oa.flags = h->flags;
+        oa.ovf = h->batch ? h->ovf : nullptr;
+        if (h->batch) HIPCHK(hipMemsetAsync(h->ovf, 0, sizeof(int) * h->scap, st));
         for (;;) {
             HIPCHK(hipMemsetAsync(h->flags + 5, 0, sizeof(int), st));
             hipLaunchKernelGGL(k_outer, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
@@ -703,13 +764,35 @@ static int rebuild(vg_scene *h)
             HIPCHK(hipMemcpyAsync(&maxc, h->flags + 5, sizeof(int), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             if (maxc <= h->ko) break;
+            if (h->batch) {
+                // a batch keeps its candidate capacity (growing it reallocates
+                // [KO][LD] for every scene): a scene with a sphere beyond it is
+                // failed like a blow-up, and the rebuild starts over without it
+                std::vector<int> ovf(h->scap);
+                HIPCHK(hipMemcpy(ovf.data(), h->ovf, sizeof(int) * h->scap, hipMemcpyDeviceToHost));
+                int marked = 0;
+                for (int k = 0; k < h->scap; ++k)
+                    if (ovf[k] && h->sc[k].state == VG_SCENE_RUNNING) {
+                        h->sc[k].state = VG_SCENE_RETIRING;
+                        h->sc[k].failed = true;
+                        ++marked;
+                        if (getenv("VG_DEBUG")) fprintf(stderr, "[vg] scene %d failed: candidates > %d\n", k, h->ko);
+                    }
+                if (marked) return rebuild(h);
+            }
+            // grow geometrically: every growth reallocates [KO][LD] and reruns the pass
+            const int ko_new = std::max(maxc + maxc / 2 + 8, 2 * h->ko);
+            if (getenv("VG_DEBUG")) fprintf(stderr, "[vg] candidate capacity %d -> %d\n", h->ko, ko_new);
+            if (maxc > 8192)
+                return fail(h, VG_ECAPACITY, "a sphere has more than 8192 candidates (a scene is blowing up)");
             HIPCHK(hipFree(h->onbr));
             h->onbr = nullptr;
-            h->ko = maxc + maxc / 4 + 8;
+            h->ko = ko_new;
             if (dalloc(h, &h->onbr, (size_t)h->ko * h->ld)) return VG_ENOMEM;
             oa.nbr = h->onbr;
             oa.ko = h->ko;
         }
+        pc.mark("candidates (synced)");
         // neighbour lists with history carry
         InnerArgs ia;
         ia.g = G1; ia.gold = G0; ia.ell = h->ell[h->ecur ^ 1]; ia.eold = h->ell[h->ecur];
@@ -742,7 +825,7 @@ static int rebuild(vg_scene *h)
             if (S.nw)
                 HIPCHK(hipMemcpy(h->walls + S.w_begin, h->walls_h.data() + S.w_begin, sizeof(WallDev) * S.nw,
                                  hipMemcpyHostToDevice));
-            S.state = S.result.empty() ? VG_SCENE_FREE : VG_SCENE_DONE;
+            S.state = S.failed ? VG_SCENE_FAILED : (S.result.empty() ? VG_SCENE_FREE : VG_SCENE_DONE);
             S.seg_begin = S.seg_end = 0;
             h->sdev[k] = SceneDev();
         }
@@ -754,6 +837,7 @@ static int rebuild(vg_scene *h)
     h->ecur ^= 1;
     h->ent_valid = false;
     h->rebuilds++;
+    pc.mark("inner lists launch, done");
     return 0;
 }
 
@@ -796,7 +880,7 @@ static void free_scene(vg_scene *h)
     F(h->wkin); F(h->assigned); F(h->assigned_sph); F(h->en_part); F(h->en_out); F(h->flags); F(h->relists);
     F(h->contacts); F(h->red); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->qidx); F(h->segs);
     F(h->qval); F(h->safe); F(h->lin); F(h->cstart); F(h->cend); F(h->ocount); F(h->onbr); F(h->tmp); F(h->rec);
-    F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->sid[0]); F(h->sid[1]);
+    F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->sid[0]); F(h->sid[1]); F(h->fetch); F(h->ovf);
     for (auto e : h->kev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -835,7 +919,7 @@ static int alloc_batch(vg_scene *h, int ld, int wcap, int scap, int ki, bool rec
     rc |= dalloc(h, &h->keys, ld) | dalloc(h, &h->skeys, ld);
     for (int b = 0; b < 2; ++b) rc |= dalloc(h, &h->xs[b], ld) | dalloc(h, &h->ys[b], ld) | dalloc(h, &h->zs[b], ld);
     rc |= dalloc(h, &h->qidx, 2 * (size_t)scap) | dalloc(h, &h->segs, 2 * (size_t)scap) | dalloc(h, &h->qval, 6 * (size_t)scap);
-    rc |= dalloc(h, &h->sid[0], ld) | dalloc(h, &h->sid[1], ld);
+    rc |= dalloc(h, &h->sid[0], ld) | dalloc(h, &h->sid[1], ld) | dalloc(h, &h->ovf, scap);
     rc |= dalloc(h, &h->safe, ld) | dalloc(h, &h->lin, ld);
     rc |= dalloc(h, &h->ocount, (size_t)ld + 1);
     rc |= dalloc(h, &h->rec_imp, (size_t)6 * ld) | dalloc(h, &h->rec_imp2, (size_t)6 * ld);
@@ -911,6 +995,9 @@ extern "C" int vg_batch_create(const vg_batch_desc *bd, vg_scene **out)
     vg_scene *h = new vg_scene();
     h->batch = true;
     h->up_iact = bd->up_iact ? bd->up_iact : 1;
+    // large layouts: start the candidate capacity high, since growing
+    // reallocates [KO][LD] and stalls the device for seconds at 1e7 slots
+    h->ko = 512;
     const int ki = bd->neighbour_capacity > 0 ? std::max(8, std::min(64, bd->neighbour_capacity)) : 32;
     int rc = alloc_batch(h, (int)bd->max_spheres, std::max(bd->max_walls, 1), bd->max_scenes, ki, false);
     if (rc) {
@@ -1233,8 +1320,13 @@ static int fetch_scene_state(vg_scene *h, int slot, std::vector<double> &hb)
     const SceneHost &S = h->sc[slot];
     HIPCHK(hipStreamSynchronize(h->stream));
     const int nd = S.n_do, n = S.n;
-    double *dout = nullptr;
-    if (dalloc(h, &dout, (size_t)21 * n)) return VG_ENOMEM;
+    if ((size_t)21 * n > h->fetch_cap) {
+        if (h->fetch) HIPCHK(hipFree(h->fetch));
+        h->fetch = nullptr;
+        h->fetch_cap = (size_t)21 * n;
+        if (dalloc(h, &h->fetch, h->fetch_cap)) return VG_ENOMEM;
+    }
+    double *dout = h->fetch;
     std::vector<double> sb((size_t)21 * n, 0.0);
     hipLaunchKernelGGL(k_scatter_state, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->geo[h->cur], S.seg_begin,
                        S.seg_end, h->ld, h->rec_valid ? (const double *)h->rec_imp : (const double *)nullptr, dout);
@@ -1256,7 +1348,6 @@ static int fetch_scene_state(vg_scene *h, int slot, std::vector<double> &hb)
         HIPCHK(hipMemcpyAsync(walls.data(), h->walls + S.w_begin, sizeof(WallDev) * S.nw, hipMemcpyDeviceToHost,
                               h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipFree(dout));
     if (dw) HIPCHK(hipFree(dw));
     hb.assign((size_t)21 * nd, 0.0);
     for (int s = 0; s < n; ++s)
@@ -1613,7 +1704,7 @@ extern "C" int vg_retire(vg_scene *h, int scene)
 {
     if (int rc = check_slot(h, scene)) return rc;
     SceneHost &S = h->sc[scene];
-    if (S.state == VG_SCENE_DONE) {
+    if (S.state == VG_SCENE_DONE || S.state == VG_SCENE_FAILED) {
         S = SceneHost();   // forget the kept result
         return 0;
     }

@@ -92,6 +92,7 @@ struct OuterArgs {
     unsigned *nbr;           // ELL [KO][LD]
     int ko;
     int *flags;              // [5]: largest candidate count seen
+    int *ovf;                // per scene: a sphere has more candidates than ko (batches; may be null)
 };
 
 struct InnerArgs {

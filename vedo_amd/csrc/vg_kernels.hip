@@ -634,7 +634,10 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
         }
     }
     a.count[s] = cnt;
-    if (cnt > (unsigned)a.ko) atomicMax(&a.flags[5], (int)cnt);
+    if (cnt > (unsigned)a.ko) {
+        atomicMax(&a.flags[5], (int)cnt);
+        if (a.ovf) atomicOr(&a.ovf[ms.scene], 1);
+    }
 }
 
 // Consultant::GetIactNum: sphere pairs are listed from both sides, walls once.

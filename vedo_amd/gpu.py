@@ -289,7 +289,7 @@ class GpuSimulation:
                     contact_steps=cs.value)
 
 
-SCENE_FREE, SCENE_QUEUED, SCENE_RUNNING, SCENE_FINISHED, SCENE_RETIRING, SCENE_DONE = range(6)
+SCENE_FREE, SCENE_QUEUED, SCENE_RUNNING, SCENE_FINISHED, SCENE_RETIRING, SCENE_DONE, SCENE_FAILED = range(7)
 
 
 class GpuBatch:

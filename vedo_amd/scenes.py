@@ -12,7 +12,7 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from .world import DOModel, DOStatus, DOWorld, IactModel, SystemParameter
+from .world import DOModel, DOStatus, DOWorld, IactModel, SystemParameter, WorldArrays
 
 MASK64 = (1 << 64) - 1
 
@@ -176,7 +176,8 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
                L: Optional[float] = None, omega: float = 2.0, kn: float = 1e4, kt_ratio: float = 0.8,
                e: float = 0.6, mu: float = 0.4, dt: float = 5e-6, steps: int = 10000,
                sides: int = 64, thick: float = 2e-3, gap: float = 1e-6, arrays: bool = False,
-               up_iact: int = 20, record: int = 10 ** 9):
+               up_iact: int = 20, record: int = 10 ** 9, law: str = "ISwBSDBF", E: float = 5e7,
+               nu: float = 0.25):
     """Polydisperse spheres in a rotating drum of quasi-plates.
 
     Radii are log-normal (median, sigma) clipped to [rmin, rmax] and quantised
@@ -215,12 +216,17 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
     g = 9.81
     cos30 = math.sqrt(3.0) / 2.0
 
+    # ISwH gives a contact that starts at rest its full Hertz stiffness
+    # (ISwH.cpp:90-103, kn = k_H when |vn| = 0): its bed starts a hair apart
+    # instead, so contacts form with a nonzero approach speed
+    compress = law != "ISwH"
+
     def rows_z(r, a, M):
         lim = rin - r
         mg = density * 4.0 / 3.0 * math.pi * r ** 3 * g
         z = [-lim]
         for j in range(M - 1):
-            d = 2.0 * r - (M - 1 - j) * mg / (2.0 * cos30 * kn)
+            d = 2.0 * r - (M - 1 - j) * mg / (2.0 * cos30 * kn) if compress else a
             z.append(z[-1] + math.sqrt(d * d - 0.25 * a * a))
         return lim, z
 
@@ -304,8 +310,19 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
     for y in (0.0, Ld):
         wall_ids.append(n + len(wall_ids))
         statuses.append(DOStatus("EndCap", (0.0, y, 0.0), (0, 0, 0), (1, 0, 0), (0, 1, 0), (0.0, omega, 0.0)))
-    iact = [IactModel("Granule", "Granule", "ISwBSDBF", spring_dashpot(kn, kt_ratio, e, mu, True)),
-            IactModel("Wall", "Granule", "ISwBSDBF", spring_dashpot(kn, kt_ratio, e, mu, False))]
+    if law == "ISwH":
+        # one IactModel per group pair: Hertz-Mindlin mechanisms of the median
+        # sphere; dt from the lightest pair's first-contact stiffness at 1 m/s
+        m_med = density * 4.0 / 3.0 * math.pi * median ** 3
+        pair = hertz_mindlin(E, nu, mu, e, median, median, m_med, m_med)
+        iact = [IactModel("Granule", "Granule", "ISwH", pair),
+                IactModel("Wall", "Granule", "ISwH", hertz_mindlin(E, nu, mu, e, median, None, m_med, None))]
+        m_min = density * 4.0 / 3.0 * math.pi * float(cls.min()) ** 3
+        k = ish_first_contact_kn(pair["HertzNormalStiffness"], pair["EquivalentMass"], pair["DampingRatio"], 1.0)
+        dt = critical_dt(0.5 * m_min, k)
+    else:
+        iact = [IactModel("Granule", "Granule", law, spring_dashpot(kn, kt_ratio, e, mu, True)),
+                IactModel("Wall", "Granule", law, spring_dashpot(kn, kt_ratio, e, mu, False))]
     sp = SystemParameter(0.0, steps * dt, dt, 0.0, (0.0, 0.0, -9.81))
     world = DOWorld(sp, models, iact, statuses)
     if arrays:
@@ -339,3 +356,160 @@ def rigid_rotation_status(pos0, vel0, ox0, oz0, axis_point, axis_dir, omega: flo
     w = omega * a
     v = np.cross(np.broadcast_to(w, p.shape), p - c)
     return p, v, rot(np.asarray(ox0)), rot(np.asarray(oz0)), np.broadcast_to(w, p.shape).copy()
+
+
+# ---------------------------------------------------------------------------
+# Array-built scenes for batched and served runs (no per-sphere Python objects)
+# ---------------------------------------------------------------------------
+
+def _spliced(world: DOWorld, sphere_model: int, pos, vel, av, up_iact: int, record: int) -> WorldArrays:
+    """WorldArrays of `world` (walls only) with n spheres of model index
+    `sphere_model` placed first (DO ids 0..n-1), walls after them."""
+    wa = world.to_arrays(up_iact, record)
+    n = len(pos)
+    one = np.ones((n, 1))
+    wa.model_of = np.concatenate([np.full(n, sphere_model, np.int32), wa.model_of])
+    wa.pos = np.concatenate([np.asarray(pos, np.float64), wa.pos])
+    wa.vel = np.concatenate([np.asarray(vel, np.float64), wa.vel])
+    wa.ox = np.concatenate([one * np.array([1.0, 0.0, 0.0]), wa.ox])
+    wa.oz = np.concatenate([one * np.array([0.0, 0.0, 1.0]), wa.oz])
+    wa.av = np.concatenate([np.asarray(av, np.float64), wa.av])
+    return wa
+
+
+def ish_first_contact_kn(k_h: float, m_eff: float, beta: float, vn: float) -> float:
+    """ISwH's normal stiffness at first contact for an impact speed vn
+    (ISwH.cpp:90-103, the beta < 0.99 branch): the stiffness the critical
+    time step of an ISwH scene is based on."""
+    return 0.957832746294131 / (1.0 - beta * beta) * m_eff ** 0.2 * k_h ** 0.8 * vn ** 0.4
+
+
+def critical_dt(m_eff: float, k: float, fraction: float = 0.1) -> float:
+    """fraction * ISwModels::CriticalTimeStep = fraction * 2 pi sqrt(m/k)
+    (ISwModels.cpp:8-11)."""
+    return fraction * 2.0 * math.pi * math.sqrt(m_eff / k)
+
+
+def box_arrays(n: int, seed: int = 1, radius: float = 1e-3, density: float = 2500.0, law: str = "ISwH",
+               E: float = 1e7, nu: float = 0.3, mu: float = 0.5, e: float = 0.5, kn: float = 1e4,
+               dt: Optional[float] = None, steps: int = 5000, spacing: float = 2.5, jitter: float = 0.1,
+               vmax: float = 0.1, thick: float = 2e-3, offset: float = 5.0, gravity: float = -9.81,
+               up_iact: int = 20, record: int = 10 ** 9) -> WorldArrays:
+    """The box settling scene of the batched configuration: n spheres on a
+    seeded jittered cubic lattice (spacing*r, as box_scene) in a closed box
+    sized to the lattice (side = lattice extent + offset*r), one law for
+    sphere-sphere and sphere-wall.  dt None: 0.1 of the critical time step of
+    the sphere-sphere contact, for ISwH at the impact speed of a fall through
+    the lattice height."""
+    rng = SplitMix64(seed)
+    s = spacing * radius
+    nx = max(1, int(math.ceil(n ** (1.0 / 3.0))))
+    nz = max(1, int(math.ceil(n / (nx * nx))))
+    idx = np.arange(n)
+    ix, iy, iz = idx % nx, (idx // nx) % nx, idx // (nx * nx)
+    base = offset * radius + np.stack([ix, iy, iz], 1) * s
+    pos = base + rng.uniform(3 * n, -jitter * radius, jitter * radius).reshape(n, 3)
+    vel = rng.uniform(3 * n, -vmax, vmax).reshape(n, 3)
+    side = float(pos.max()) + offset * radius
+    sphere = DOModel("Sphere", "Granule", "mobile", "local", density, 1.0, (0, 0, 0), "Sphere", (radius,))
+    mass = sphere.analysis()[1]
+    wm, ws = box_walls(side, thick)
+    if law == "ISwH":
+        pair = hertz_mindlin(E, nu, mu, e, radius, radius, mass, mass)
+        wall = hertz_mindlin(E, nu, mu, e, radius, None, mass, None)
+        if dt is None:
+            v_ref = math.sqrt(2.0 * abs(gravity) * nz * s) + vmax
+            k = ish_first_contact_kn(pair["HertzNormalStiffness"], pair["EquivalentMass"],
+                                     pair["DampingRatio"], v_ref)
+            dt = critical_dt(pair["EquivalentMass"], k)
+    else:
+        pair = spring_dashpot(kn, 0.8, e, mu, True)
+        wall = spring_dashpot(kn, 0.8, e, mu, False)
+        if dt is None:
+            dt = critical_dt(0.5 * mass, kn)
+    iact = [IactModel("Granule", "Granule", law, pair), IactModel("Wall", "Granule", law, wall)]
+    world = DOWorld(SystemParameter(0.0, steps * dt, dt, 0.0, (0.0, 0.0, gravity)), [sphere] + wm, iact, ws)
+    return _spliced(world, 0, pos, vel, np.zeros((n, 3)), up_iact, record)
+
+
+def hopper_arrays(n: int, seed: int = 3, radius: float = 1e-3, density: float = 2500.0, law: str = "ISwH",
+                  E: float = 5e7, nu: float = 0.25, mu: float = 0.5, e: float = 0.5, kn: float = 1e4,
+                  outlet: float = 0.03, dt: Optional[float] = None, steps: int = 5000, thick: float = 2e-3,
+                  up_iact: int = 20, record: int = 10 ** 9) -> WorldArrays:
+    """Spheres discharging from a square hopper: four quasi-plates inclined at
+    45 degrees down to a square outlet (side `outlet`, at most half the
+    hopper's), four vertical plates above them holding the initial column
+    (a jittered cubic lattice at 2.2 r), and a closed bin below the outlet
+    that catches the discharge.  The hopper's half-width A is sized so the
+    column is about twice as tall as it is wide."""
+    rng = SplitMix64(seed)
+    s = 2.2 * radius
+    A = max(0.012, (n * s ** 3 / 16.0) ** (1.0 / 3.0)) + 2.0 * radius + thick
+    a = min(0.5 * outlet, 0.5 * A)
+    H = A - a                          # 45-degree funnel
+    per = max(1, int((2.0 * (A - thick) - 2.0 * radius) / s))
+    layers = int(math.ceil(n / (per * per)))
+    Hc = layers * s + 4.0 * radius     # column walls above the funnel
+    idx = np.arange(n)
+    ix, iy, iz = idx % per, (idx // per) % per, idx // (per * per)
+    x0 = -(per - 1) * 0.5 * s
+    pos = np.stack([x0 + ix * s, x0 + iy * s, H + 2.0 * radius + iz * s], 1)
+    pos += rng.uniform(3 * n, -0.05 * radius, 0.05 * radius).reshape(n, 3)
+    vel = np.zeros((n, 3))
+    sphere = DOModel("Sphere", "Granule", "mobile", "local", density, 1.0, (0, 0, 0), "Sphere", (radius,))
+    mass = sphere.analysis()[1]
+    Ls = math.hypot(A - a, H)
+    d, hh = (A - a) / Ls, H / Ls
+    funnel = DOModel("Funnel", "Wall", "fixed", "global", 0.0, 1.0, (0, 0, 0), "QuasiPlate",
+                     (Ls + 2.0 * thick, thick, 2.0 * A + 2.0 * thick))
+    column = DOModel("Column", "Wall", "fixed", "global", 0.0, 1.0, (0, 0, 0), "QuasiPlate",
+                     (Hc, thick, 2.0 * A + 2.0 * thick))
+    # bin: as wide as the hopper, deep enough for every sphere at a loose packing
+    Hb = max(4.0 * radius, 1.6 * n * (2.0 * radius) ** 3 / (2.0 * A) ** 2) + 0.05
+    binwall = DOModel("Bin", "Wall", "fixed", "global", 0.0, 1.0, (0, 0, 0), "QuasiPlate",
+                      (2.0 * A + 2.0 * thick, thick, Hb + 2.0 * thick))
+    floor = DOModel("Floor", "Wall", "fixed", "global", 0.0, 1.0, (0, 0, 0), "QuasiPlate",
+                    (2.0 * A + 2.0 * thick, thick, 2.0 * A + 2.0 * thick))
+    c = 0.5 * (a + A)
+    zf = 0.5 * H
+    st = [
+        # inclined plates: ox up the slope, oz (normal) into the hopper
+        _plate("Funnel", (c, 0.0, zf), (d, 0.0, hh), (-hh, 0.0, d)),
+        _plate("Funnel", (-c, 0.0, zf), (-d, 0.0, hh), (hh, 0.0, d)),
+        _plate("Funnel", (0.0, c, zf), (0.0, d, hh), (0.0, -hh, d)),
+        _plate("Funnel", (0.0, -c, zf), (0.0, -d, hh), (0.0, hh, d)),
+        # column: vertical plates at |x| = A, |y| = A above the funnel
+        _plate("Column", (A, 0.0, H + 0.5 * Hc), (0.0, 0.0, 1.0), (-1.0, 0.0, 0.0)),
+        _plate("Column", (-A, 0.0, H + 0.5 * Hc), (0.0, 0.0, 1.0), (1.0, 0.0, 0.0)),
+        _plate("Column", (0.0, A, H + 0.5 * Hc), (0.0, 0.0, 1.0), (0.0, -1.0, 0.0)),
+        _plate("Column", (0.0, -A, H + 0.5 * Hc), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0)),
+        # bin below the outlet
+        _plate("Floor", (0.0, 0.0, -Hb), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0)),
+        _plate("Bin", (A, 0.0, -0.5 * Hb), (0.0, 1.0, 0.0), (-1.0, 0.0, 0.0)),
+        _plate("Bin", (-A, 0.0, -0.5 * Hb), (0.0, 1.0, 0.0), (1.0, 0.0, 0.0)),
+        _plate("Bin", (0.0, A, -0.5 * Hb), (1.0, 0.0, 0.0), (0.0, -1.0, 0.0)),
+        _plate("Bin", (0.0, -A, -0.5 * Hb), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0)),
+    ]
+    if law == "ISwH":
+        pair = hertz_mindlin(E, nu, mu, e, radius, radius, mass, mass)
+        wall = hertz_mindlin(E, nu, mu, e, radius, None, mass, None)
+        if dt is None:
+            v_ref = math.sqrt(2.0 * 9.81 * (H + Hc + Hb))
+            k = ish_first_contact_kn(pair["HertzNormalStiffness"], pair["EquivalentMass"],
+                                     pair["DampingRatio"], v_ref)
+            dt = critical_dt(pair["EquivalentMass"], k)
+    else:
+        pair = spring_dashpot(kn, 0.8, e, mu, True)
+        wall = spring_dashpot(kn, 0.8, e, mu, False)
+        if dt is None:
+            dt = critical_dt(0.5 * mass, kn)
+    iact = [IactModel("Granule", "Granule", law, pair), IactModel("Wall", "Granule", law, wall)]
+    world = DOWorld(SystemParameter(0.0, steps * dt, dt, 0.0, (0.0, 0.0, -9.81)),
+                    [sphere, funnel, column, binwall, floor], iact, st)
+    return _spliced(world, 0, pos, vel, np.zeros((n, 3)), up_iact, record)
+
+
+def drum_radius_for(n: int, median: float = 1e-3) -> float:
+    """Drum radius whose lower half holds n spheres of the median radius at
+    a dense packing in a drum as long as its radius."""
+    return max(0.02, (2.0 * n * (2.0 * median) ** 3 / (0.7 * math.pi)) ** (1.0 / 3.0))

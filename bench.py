@@ -40,6 +40,20 @@ BYTES_PER_SPHERE = 280
 BYTES_PER_CONTACT = 72
 
 
+def measured_traffic():
+    """HBM bytes per step launch of the drum workload from the committed
+    counter summary (two rocprofv3 --pmc passes calibrated with the copy
+    kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "k_step_drum1m_hbm_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1]), "profiles/k_step_drum1m_hbm_counters.md"
+    except OSError:
+        pass
+    return None, None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -290,6 +304,7 @@ def main():
     k_avg_s = k_ms / 1e3 / max(k_launches, 1)
     achieved = bytes_step / k_avg_s / 1e9
     value = n * args.steps / wall
+    traffic, traffic_src = measured_traffic()
     log(f"[bench] timed {args.steps} steps in {wall:.3f} s; step kernel avg {k_avg_s * 1e6:.1f} us over "
         f"{k_launches} launches; {c_act:.0f} contact pairs per step; {stats}")
     out = {
@@ -301,7 +316,7 @@ def main():
                    "prep_steps": done, "prep_quenches": quenches, "active_contacts": c_act, "neighbour_skin_m": skin,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "copy_kernel_gbps": copy_gbps, "frac_of_copy": achieved / copy_gbps,
                      "kernel": "k_step", "kernel_avg_us": k_avg_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_step},

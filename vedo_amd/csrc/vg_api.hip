@@ -83,6 +83,8 @@ struct vg_scene {
     bool record = true;                 // per-entry ImpactStatus records (one-scene handles)
     // device
     Geo geo[2];
+    unsigned *oid_spare = nullptr;      // rank / model arrays the next rebuild permutes into
+    int *mdl_spare = nullptr;
     int cur = 0;
     Ell ell[2];
     int ecur = 0;
@@ -744,15 +746,20 @@ static int rebuild(vg_scene *h)
     }
     pc.mark("grids + cell sort launch");
     const int n = n_live;
+    // the permuted rank / model arrays go to the spares, which both buffers
+    // then share; the old ones stay readable through G0 for the history carry
+    Geo G1w = G1;
+    G1w.oid = h->oid_spare;
+    G1w.mdl = h->mdl_spare;
     if (n > 0) {
-        hipLaunchKernelGGL(k_reorder, dim3(nblk(n, 256)), dim3(256), 0, st, G0, G1, n, h->ld, (const unsigned *)h->perm,
+        hipLaunchKernelGGL(k_reorder, dim3(nblk(n, 256)), dim3(256), 0, st, G0, G1w, n, h->ld, (const unsigned *)h->perm,
                            (const SceneDev *)h->scenes, (const ModelDev *)h->models, h->safe, h->lin,
                            h->rec_valid ? (const double *)h->rec_imp : (const double *)nullptr, h->rec_imp2);
         if (h->rec_valid) std::swap(h->rec_imp, h->rec_imp2);
         hipLaunchKernelGGL(k_cellstart, dim3(nblk(n, 256)), dim3(256), 0, st, h->lin, n, h->cstart, h->cend);
         // candidate list in one pass; grow the capacity and redo if a sphere overflowed
         OuterArgs oa;
-        oa.g = G1; oa.n = n; oa.ld = h->ld; oa.scenes = h->scenes; oa.cstart = h->cstart; oa.cend = h->cend;
+        oa.g = G1w; oa.n = n; oa.ld = h->ld; oa.scenes = h->scenes; oa.cstart = h->cstart; oa.cend = h->cend;
         oa.safe = h->safe; oa.walls = h->walls; oa.t = tables(h); oa.count = h->ocount; oa.nbr = h->onbr;
         oa.ko = h->ko; oa.single_group = h->single_group; oa.flags = h->flags;
         oa.ovf = h->batch ? h->ovf : nullptr;
@@ -795,7 +802,7 @@ static int rebuild(vg_scene *h)
         pc.mark("candidates (synced)");
         // neighbour lists with history carry
         InnerArgs ia;
-        ia.g = G1; ia.gold = G0; ia.ell = h->ell[h->ecur ^ 1]; ia.eold = h->ell[h->ecur];
+        ia.g = G1w; ia.gold = G0; ia.ell = h->ell[h->ecur ^ 1]; ia.eold = h->ell[h->ecur];
         ia.n = n; ia.ld = h->ld; ia.ki = h->ki; ia.perm = h->perm; ia.cnt = h->ocount; ia.nbr = h->onbr;
         ia.ko = h->ko; ia.scenes = h->scenes; ia.models = h->models; ia.walls = h->walls; ia.flags = h->flags;
         hipLaunchKernelGGL(k_inner_build, dim3(nblk(n, 128)), dim3(128), (size_t)h->ki * 128 * sizeof(unsigned), st,
@@ -832,6 +839,12 @@ static int rebuild(vg_scene *h)
         if (upload_scenes(h)) return VG_EHIP;
         update_batch_flags(h);
     }
+    if (n > 0) {
+        std::swap(h->geo[0].oid, h->oid_spare);
+        std::swap(h->geo[0].mdl, h->mdl_spare);
+        h->geo[1].oid = h->geo[0].oid;
+        h->geo[1].mdl = h->geo[0].mdl;
+    }
     h->n = n;
     h->cur ^= 1;
     h->ecur ^= 1;
@@ -853,16 +866,18 @@ extern "C" const char *vg_last_error(const vg_scene *h) { return h ? h->err.c_st
 
 static int alloc_geo(vg_scene *h, Geo &g, int n)
 {
-    if (dalloc(h, &g.pos, n) || dalloc(h, &g.vel, n) || dalloc(h, &g.avl, n) || dalloc(h, &g.ori, 6 * (size_t)n) ||
-        dalloc(h, &g.oid, n) || dalloc(h, &g.mdl, n))
+    if (dalloc(h, &g.pos, n) || dalloc(h, &g.vel, n) || dalloc(h, &g.avl, n) || dalloc(h, &g.ori, 6 * (size_t)n))
         return VG_ENOMEM;
     return 0;
 }
 
 static int alloc_ell(vg_scene *h, Ell &e, int n, int ki)
 {
-    if (dalloc(h, &e.hdr, n) || dalloc(h, &e.ent, (size_t)2 * ki * n) || dalloc(h, &e.hist, (size_t)2 * ki * n))
+    if (dalloc(h, &e.hdr, n) || dalloc(h, &e.ent, (size_t)2 * ki * n) || dalloc(h, &e.hs, (size_t)6 * ki * n) ||
+        dalloc(h, &e.ha, (size_t)2 * ki * n))
         return VG_ENOMEM;
+    // the spring-dashpot kernels never write the fourth history word: it stays zero
+    HIPCHK(hipMemset(e.ha, 0, sizeof(double) * 2 * ki * (size_t)n));
     return 0;
 }
 
@@ -871,9 +886,10 @@ static void free_scene(vg_scene *h)
     auto F = [](void *p) {
         if (p) (void)hipFree(p);
     };
+    F(h->geo[0].oid); F(h->geo[0].mdl); F(h->oid_spare); F(h->mdl_spare);
     for (int b = 0; b < 2; ++b) {
-        F(h->geo[b].pos); F(h->geo[b].vel); F(h->geo[b].avl); F(h->geo[b].ori); F(h->geo[b].oid); F(h->geo[b].mdl);
-        F(h->ell[b].hdr); F(h->ell[b].ent); F(h->ell[b].hist);
+        F(h->geo[b].pos); F(h->geo[b].vel); F(h->geo[b].avl); F(h->geo[b].ori);
+        F(h->ell[b].hdr); F(h->ell[b].ent); F(h->ell[b].hs); F(h->ell[b].ha);
         F(h->xs[b]); F(h->ys[b]); F(h->zs[b]);
     }
     F(h->models); F(h->mm); F(h->laws); F(h->scenes); F(h->stime); F(h->rflags); F(h->walls); F(h->walls_ref);
@@ -910,6 +926,12 @@ static int alloc_batch(vg_scene *h, int ld, int wcap, int scap, int ki, bool rec
     (void)hipEventCreate(&h->ev1);
     int rc = 0;
     rc |= alloc_geo(h, h->geo[0], ld) | alloc_geo(h, h->geo[1], ld);
+    // rank and model of a slot change only at rebuilds: both state buffers
+    // share one copy, the rebuild's permutation writes the spare
+    rc |= dalloc(h, &h->geo[0].oid, ld) | dalloc(h, &h->geo[0].mdl, ld);
+    rc |= dalloc(h, &h->oid_spare, ld) | dalloc(h, &h->mdl_spare, ld);
+    h->geo[1].oid = h->geo[0].oid;
+    h->geo[1].mdl = h->geo[0].mdl;
     rc |= alloc_ell(h, h->ell[0], ld, ki) | alloc_ell(h, h->ell[1], ld, ki);
     rc |= dalloc(h, &h->scenes, scap) | dalloc(h, &h->stime, 2 * (size_t)scap) | dalloc(h, &h->rflags, 3 * (size_t)scap);
     rc |= dalloc(h, &h->walls, wcap) | dalloc(h, &h->walls_ref, wcap) | dalloc(h, &h->wkin, wcap);

@@ -28,15 +28,55 @@ __device__ inline size_t eidx(int h, int k, int ki, int n, int i)
 {
     return ((size_t)(h * ki + k)) * (size_t)n + (size_t)i;
 }
-__device__ inline void load_hist(const double4 *hist, int h, int k, int ki, int n, int i, Hist &hs)
+// Streaming accesses: data only the owning sphere touches (its orientation,
+// list entries, histories, next-step state) bypass the caches' retention so
+// that L2 keeps the neighbour positions and velocities many threads gather.
+#ifndef VG_NT
+#define VG_NT 0
+#endif
+template <class T> __device__ inline T nt_load(const T *p)
 {
-    const double4 v = hist[eidx(h, k, ki, n, i)];
-    hs.s = mk(v.x, v.y, v.z);
-    hs.aux = v.w;
+    if (VG_NT >= 1) return __builtin_nontemporal_load(p);
+    return *p;
 }
-__device__ inline void store_hist(double4 *hist, int h, int k, int ki, int n, int i, const Hist &hs)
+template <class T> __device__ inline void nt_store(T v, T *p)
 {
-    hist[eidx(h, k, ki, n, i)] = make_double4(hs.s.x, hs.s.y, hs.s.z, hs.aux);
+    if (VG_NT >= 1) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+// next-step state: plain stores keep it in the memory-side cache for the
+// next step's reads unless VG_NT >= 2
+typedef double v4d __attribute__((ext_vector_type(4)));
+__device__ inline void nt_store4(double4 v, double4 *p)
+{
+    if (VG_NT >= 2) __builtin_nontemporal_store(*reinterpret_cast<const v4d *>(&v), reinterpret_cast<v4d *>(p));
+    else *p = v;
+}
+template <class T> __device__ inline void st_store(T v, T *p)
+{
+    if (VG_NT >= 2) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// History of entry (h, k) of sphere i: the shear force always, the fourth
+// word only where a Hertz law may have set it (AUX).
+template <bool AUX = true>
+__device__ inline void load_hist(const Ell &e, int h, int k, int ki, int n, int i, Hist &hs)
+{
+    const size_t x = eidx(h, k, ki, n, i);
+    const double *p = e.hs + 3 * x;
+    hs.s = mk(nt_load(p), nt_load(p + 1), nt_load(p + 2));
+    hs.aux = AUX ? nt_load(e.ha + x) : 0.0;
+}
+template <bool AUX = true>
+__device__ inline void store_hist(const Ell &e, int h, int k, int ki, int n, int i, const Hist &hs)
+{
+    const size_t x = eidx(h, k, ki, n, i);
+    double *p = e.hs + 3 * x;
+    nt_store(hs.s.x, p);
+    nt_store(hs.s.y, p + 1);
+    nt_store(hs.s.z, p + 2);
+    if (AUX) nt_store(hs.aux, e.ha + x);
 }
 
 // ---------------------------------------------------------------------------
@@ -71,7 +111,7 @@ __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_o
         Hist hs;
         if (carried) {
             f = a.ell.ent[eidx(h_old, p, a.ki, a.ld, i)] & ~PARTNER_MASK;
-            if (f & F_CONTACT) load_hist(a.ell.hist, h_old, p, a.ki, a.ld, i, hs);
+            if (f & F_CONTACT) load_hist(a.ell, h_old, p, a.ki, a.ld, i, hs);
             ++p;
             old_partner = p < c_old ? (a.ell.ent[eidx(h_old, p, a.ki, a.ld, i)] & PARTNER_MASK) : 0xffffffffu;
         }
@@ -89,7 +129,7 @@ __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_o
                 hs.aux = 0.0;
             }
             a.ell.ent[eidx(h_new, q, a.ki, a.ld, i)] = j | f;
-            if (f & F_CONTACT) store_hist(a.ell.hist, h_new, q, a.ki, a.ld, i, hs);
+            if (f & F_CONTACT) store_hist(a.ell, h_new, q, a.ki, a.ld, i, hs);
             ++q;
         }
     }
@@ -131,8 +171,6 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
         a.out.vel[i] = a.in.vel[i];
         a.out.avl[i] = a.in.avl[i];
         for (int c = 0; c < 6; ++c) a.out.ori[(size_t)c * a.ld + i] = a.in.ori[(size_t)c * a.ld + i];
-        a.out.oid[i] = a.in.oid[i];
-        a.out.mdl[i] = mdl_i;
         return;
     }
     const bool do_relist = a.rflags[3 * sc + a.step % 3] != 0;
@@ -160,27 +198,42 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
     // physics as many times as its busiest lane has near contacts, not as
     // many times as its longest list has entries.
     unsigned long long todo = 0;
-    for (int k = 0; k < cnt; ++k) {
-        const size_t ei = eidx(h, k, a.ki, a.ld, i);
-        const unsigned ent = a.ell.ent[ei];
-        const unsigned j = ent & PARTNER_MASK;
-        bool full = true;
-        if (a.no_bond && j < (unsigned)a.ld) {
-            const double4 pj = a.in.pos[j];
-            const d3 dx = ld3(pj) - xi;
-            const double rs = ri + pj.w;
-            if (dot(dx, dx) > rs * rs * (1.0 + 1e-9)) {
-                full = false;
-                // Backup, then Clean() of an inactive pair (ImpactStatus.cpp:259-274)
-                const unsigned f = ent & ~PARTNER_MASK & ~F_STATUS & ~F_NOBACKUP;
-                if (f != (ent & ~PARTNER_MASK)) a.ell.ent[ei] = j | f;
-                if (REC) {
-                    double *rp = a.rec + ((size_t)k * 13) * a.ld + i;
-                    for (int q = 0; q < 13; ++q) rp[(size_t)q * a.ld] = 0.0;
+    // entries in groups of PF: the entry loads, then the partner-position
+    // gathers of a group are independent and overlap in flight
+    constexpr int PF = 8;
+    for (int k0 = 0; k0 < cnt; k0 += PF) {
+        unsigned ent[PF];
+#pragma unroll
+        for (int u = 0; u < PF; ++u)
+            ent[u] = k0 + u < cnt ? nt_load(a.ell.ent + eidx(h, k0 + u, a.ki, a.ld, i)) : PARTNER_MASK;
+        double4 pj[PF];
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const unsigned j = ent[u] & PARTNER_MASK;
+            pj[u] = (a.no_bond && j < (unsigned)a.ld) ? a.in.pos[j] : make_double4(0.0, 0.0, 0.0, -1.0);
+        }
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int k = k0 + u;
+            if (k >= cnt) break;
+            const unsigned j = ent[u] & PARTNER_MASK;
+            bool full = true;
+            if (pj[u].w >= 0.0) {   // a sphere partner, unbonded laws
+                const d3 dx = ld3(pj[u]) - xi;
+                const double rs = ri + pj[u].w;
+                if (dot(dx, dx) > rs * rs * (1.0 + 1e-9)) {
+                    full = false;
+                    // Backup, then Clean() of an inactive pair (ImpactStatus.cpp:259-274)
+                    const unsigned f = ent[u] & ~PARTNER_MASK & ~F_STATUS & ~F_NOBACKUP;
+                    if (f != (ent[u] & ~PARTNER_MASK)) nt_store(j | f, a.ell.ent + eidx(h, k, a.ki, a.ld, i));
+                    if (REC) {
+                        double *rp = a.rec + ((size_t)k * 13) * a.ld + i;
+                        for (int q = 0; q < 13; ++q) rp[(size_t)q * a.ld] = 0.0;
+                    }
                 }
             }
+            if (full) todo |= 1ull << k;
         }
-        if (full) todo |= 1ull << k;
     }
     d3 I = mk(0, 0, 0), L = mk(0, 0, 0);
     unsigned n_contact = 0;   // pairs left in contact, counted on their slave side
@@ -188,7 +241,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
         const int k = __builtin_ctzll(todo);
         todo &= todo - 1;
         const size_t ei = eidx(h, k, a.ki, a.ld, i);
-        const unsigned ent = a.ell.ent[ei];
+        const unsigned ent = nt_load(a.ell.ent + ei);
         const unsigned j = ent & PARTNER_MASK;
         Hist hs;
         hs.f = ent & ~PARTNER_MASK;
@@ -201,7 +254,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             hs.f = f;
         }
         hs.f &= ~F_NOBACKUP;
-        if (had_contact) load_hist(a.ell.hist, h, k, a.ki, a.ld, i, hs);
+        if (had_contact) load_hist<!SD>(a.ell, h, k, a.ki, a.ld, i, hs);
         else { hs.s = mk(0, 0, 0); hs.aux = 0.0; }
         PairOut o;
         o.added = false;
@@ -240,9 +293,9 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
                 if (!o.linear_only) L = L + o.ang_a;
             }
         }
-        if (hs.f != (ent & ~PARTNER_MASK)) a.ell.ent[ei] = j | hs.f;
+        if (hs.f != (ent & ~PARTNER_MASK)) nt_store(j | hs.f, a.ell.ent + ei);
         if (hs.f & F_CONTACT) {
-            store_hist(a.ell.hist, h, k, a.ki, a.ld, i, hs);
+            store_hist<!SD>(a.ell, h, k, a.ki, a.ld, i, hs);
             n_contact += (hs.f & F_SLAVE) ? 1u : 0u;
         }
         if (REC) {
@@ -263,19 +316,20 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
     const d3 dp = smul(S.dt, vi + smul(0.5, dv));
     const d3 dw = smul(S.dt, wi + smul(0.5, dav));
     const d3 xn = xi + dp;
-    d3 oxn = mk(a.in.ori[i], a.in.ori[a.ld + i], a.in.ori[2 * a.ld + i]);
-    d3 ozn = mk(a.in.ori[3 * a.ld + i], a.in.ori[4 * a.ld + i], a.in.ori[5 * a.ld + i]);
+    const double *oi = a.in.ori + i;
+    const size_t LD = (size_t)a.ld;
+    d3 oxn = mk(nt_load(oi), nt_load(oi + LD), nt_load(oi + 2 * LD));
+    d3 ozn = mk(nt_load(oi + 3 * LD), nt_load(oi + 4 * LD), nt_load(oi + 5 * LD));
     rotate_pair(oxn, ozn, dw);
     const d3 vn = vi + dv, wn = wi + dav;
     const double path = v4.w + len(dp);
     if (path > S.relist_half) atomicOr(&a.rflags[3 * sc + (a.step + 1) % 3], 1);
-    a.out.pos[i] = make_double4(xn.x, xn.y, xn.z, ri);
-    a.out.vel[i] = make_double4(vn.x, vn.y, vn.z, path);
-    a.out.avl[i] = make_double4(wn.x, wn.y, wn.z, mi);
-    a.out.ori[i] = oxn.x; a.out.ori[a.ld + i] = oxn.y; a.out.ori[2 * a.ld + i] = oxn.z;
-    a.out.ori[3 * a.ld + i] = ozn.x; a.out.ori[4 * a.ld + i] = ozn.y; a.out.ori[5 * a.ld + i] = ozn.z;
-    a.out.oid[i] = a.in.oid[i];
-    a.out.mdl[i] = mdl_i;
+    nt_store4(make_double4(xn.x, xn.y, xn.z, ri), a.out.pos + i);
+    nt_store4(make_double4(vn.x, vn.y, vn.z, path), a.out.vel + i);
+    nt_store4(make_double4(wn.x, wn.y, wn.z, mi), a.out.avl + i);
+    double *oo = a.out.ori + i;
+    st_store(oxn.x, oo); st_store(oxn.y, oo + LD); st_store(oxn.z, oo + 2 * LD);
+    st_store(ozn.x, oo + 3 * LD); st_store(ozn.y, oo + 4 * LD); st_store(ozn.z, oo + 5 * LD);
     if (REC) {
         a.rec_imp[i] = I.x; a.rec_imp[a.ld + i] = I.y; a.rec_imp[2 * a.ld + i] = I.z;
         a.rec_imp[3 * a.ld + i] = L.x; a.rec_imp[4 * a.ld + i] = L.y; a.rec_imp[5 * a.ld + i] = L.z;
@@ -707,13 +761,13 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
             for (int k = 0; k < c_old; ++k)
                 if (old_oid[k * blockDim.x + threadIdx.x] == key) {
                     f = (a.eold.ent[eidx(h_old, k, a.ki, a.ld, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
-                    if (f & F_CONTACT) load_hist(a.eold.hist, h_old, k, a.ki, a.ld, p, hs);
+                    if (f & F_CONTACT) load_hist(a.eold, h_old, k, a.ki, a.ld, p, hs);
                     break;
                 }
         }
         if (slave) f |= F_SLAVE;
         a.ell.ent[eidx(0, q, a.ki, a.ld, s)] = j | f;
-        if (f & F_CONTACT) store_hist(a.ell.hist, 0, q, a.ki, a.ld, s, hs);
+        if (f & F_CONTACT) store_hist(a.ell, 0, q, a.ki, a.ld, s, hs);
         ++q;
     }
     a.ell.hdr[s] = (unsigned char)q;
@@ -804,7 +858,7 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
         Hist hs;
         hs.s = mk(0, 0, 0);
         hs.aux = 0.0;
-        if (k >= 0 && (ent & F_CONTACT)) load_hist(ell.hist, h, k, ki, n, i, hs);
+        if (k >= 0 && (ent & F_CONTACT)) load_hist(ell, h, k, ki, n, i, hs);
         A.hist[0] = hs.s.x; A.hist[1] = hs.s.y; A.hist[2] = hs.s.z; A.hist[3] = hs.aux;
         for (int r = 0; r < 13; ++r) A.rec[r] = (rec && k >= 0) ? rec[((size_t)k * 13 + r) * n + i] : 0.0;
     }

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvedo_hip.so")
+# VG_LIBRARY: another build of the same library (A/B timing of build variants)
+LIB_PATH = os.environ.get("VG_LIBRARY") or os.path.join(HERE, "libvedo_hip.so")
 
 
 class VgModel(C.Structure):

@@ -196,7 +196,8 @@ int vg_system_energy(vg_scene *h, double *out);
 #define VG_SCENE_FINISHED 3   /* reached its step count; state held until the next boundary */
 #define VG_SCENE_RETIRING 4   /* leaves the device at the next boundary */
 #define VG_SCENE_DONE 5       /* left the device; its final state is kept on the host */
-#define VG_SCENE_FAILED 6     /* blew up (a sphere crossing more than its radius per step) and left the batch */
+#define VG_SCENE_FAILED 6     /* blew up (a sphere crossing more than the scene's largest radius in one
+                                 step) or outgrew the candidate capacity, and left the batch */
 
 typedef struct {
     long long max_spheres;     /* sphere capacity of the device layout */

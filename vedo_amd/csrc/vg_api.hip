@@ -468,6 +468,30 @@ static int admit(vg_scene *h, const vg_scene_desc *d, long long steps, int *slot
     return 0;
 }
 
+// Comparator order of the 14 cell groups of a sphere's Leap traversal: its
+// own cell (13) and the cells at -NB13[k] (k < 13), whose half shells hold
+// it.  The comparator compares cell coordinates lexicographically in the
+// axis priority `perm`, so for cells at fixed offsets from one cell it
+// compares the offsets; the order, packed 4 bits per group, is the same for
+// every cell of the grid.
+static unsigned long long group_order(const int perm[3])
+{
+    static const int nb[13][3] = VG_NB13_INIT;
+    int off[14][3], idx[14];
+    for (int k = 0; k < 13; ++k)
+        for (int c = 0; c < 3; ++c) off[k][c] = -nb[k][c];
+    off[13][0] = off[13][1] = off[13][2] = 0;
+    for (int k = 0; k < 14; ++k) idx[k] = k;
+    std::sort(idx, idx + 14, [&](int u, int v) {
+        for (int c = 0; c < 3; ++c)
+            if (off[u][perm[c]] != off[v][perm[c]]) return off[u][perm[c]] < off[v][perm[c]];
+        return false;
+    });
+    unsigned long long order = 0;
+    for (int q = 0; q < 14; ++q) order |= (unsigned long long)idx[q] << (4 * q);
+    return order;
+}
+
 // ---------------------------------------------------------------------------
 // Rebuild boundary: place queued scenes, drop finished ones, rebuild
 // ---------------------------------------------------------------------------
@@ -676,6 +700,7 @@ static int rebuild(vg_scene *h)
             gd.sf = S.sf;
             gd.nx = (int)ncx + 3; gd.ny = (int)ncy + 3; gd.nz = (int)ncz + 3;
             gd.perm[0] = perm[0]; gd.perm[1] = perm[1]; gd.perm[2] = perm[2];
+            gd.order = group_order(perm);
             const long long c = (long long)gd.nx * gd.ny * gd.nz;
             cell_off[k] = cells;
             cells += c;

@@ -55,7 +55,7 @@ struct WallDev {
     double a0, a1, a2;   // plate: w, h, l ; cylinder: r, h
     double range;        // DOModel::Analysis range (global scope)
     double reach;        // distance from the centre to the farthest point of the wall
-    double path;         // bound on any wall point's displacement since the last relist
+    double path;         // bound on the displacement of the wall's contact rectangle / axis segment since the last relist
     int shape, group;
     int behavior, kin;   // kin: 0 none, 1 host-assigned, 2 device rotation
     int scene, pad;      // scene slot of the batch

@@ -31,7 +31,19 @@ struct GridDev {
     double sf;
     int nx, ny, nz;          // dense dims (reference ncel + 3)
     int perm[3];             // comparator axis priority
+    // The 14 cell groups of a sphere's traversal in comparator order, 4 bits
+    // each from bit 0: 13 is the sphere's own cell, k < 13 the cell at
+    // -NB13[k] (whose half shell holds the own cell).  The offsets alone
+    // decide their comparator order, so it is the same for every cell.
+    unsigned long long order;
 };
+
+// The 13-cell half shell of the Leap traversal (LeapConsultant.h:207-444).
+#define VG_NB13_INIT                                                                                             \
+    {                                                                                                           \
+        {-1, -1, -1}, {-1, -1, 0}, {-1, -1, 1}, {0, -1, -1}, {0, -1, 0}, {0, -1, 1}, {1, -1, -1}, {1, -1, 0},  \
+            {1, -1, 1}, {-1, 0, -1}, {-1, 0, 0}, {0, 0, -1}, {1, 0, -1}                                         \
+    }
 
 // One scene of a batch: its SystemParameter scalars, IactModel table, walls,
 // grid of the last rebuild and state.  A scene's spheres occupy one

@@ -20,9 +20,7 @@
 
 namespace vg {
 
-__constant__ int NB13[13][3] = {{-1, -1, -1}, {-1, -1, 0}, {-1, -1, 1}, {0, -1, -1}, {0, -1, 0},
-                                {0, -1, 1},   {1, -1, -1}, {1, -1, 0},  {1, -1, 1},  {-1, 0, -1},
-                                {-1, 0, 0},   {0, 0, -1},  {1, 0, -1}};
+__constant__ int NB13[13][3] = VG_NB13_INIT;
 
 __device__ inline size_t eidx(int h, int k, int ki, int n, int i)
 {
@@ -393,15 +391,36 @@ __global__ void k_walls(WallDev *walls, int nw, const SceneDev *scenes, double *
         W.ox = rotate_around(W.ox, dw);
         W.oz = rotate_around(W.oz, dw);
     }
-    // Relist guard for moving walls.  A wall point q = a Ox + b Oy + c Oz
-    // relative to the centre, |a|, |b|, |c| <= reach, moves by at most
-    // |dpos| + reach (|dOx| + |dOy| + |dOz|), so a sphere-wall depth changes
-    // by at most the sphere's path plus this accumulated bound: a wall whose
-    // bound exceeds half the skin makes its scene's spheres relist at the
-    // next step, as a sphere's own path does.  The bound restarts when the
-    // step that just ran relisted (the lists were made at these positions).
-    const d3 oy0 = cross(old.oz, old.ox), oy1 = cross(W.oz, W.ox);
-    const double d = len(W.pos - old.pos) + W.reach * ((len(W.ox - old.ox) + len(oy1 - oy0)) + len(W.oz - old.oz));
+    // Relist guard for moving walls.  A sphere-wall depth is the sphere's
+    // distance to the plate's mid-plane rectangle (the cylinder's axis
+    // segment) subtracted from constants (cd_sp, cd_sc), so it changes by at
+    // most the largest displacement of a point of that rectangle (segment).
+    // Under a rigid motion a point's displacement is affine in its position
+    // on the rectangle, so the largest is at a corner (an end); the bound
+    // allows for the rounding of the corners.  Accumulated since the last
+    // relist, together with a sphere's own path it bounds every depth change:
+    // a wall whose bound exceeds half the skin makes its scene's spheres
+    // relist at the next step, as a sphere's own path does.  The bound
+    // restarts when the step that just ran relisted (the lists were made at
+    // these positions).
+    double d = 0.0;
+    if (W.shape == SH_PLATE) {
+        const d3 oy0 = cross(old.oz, old.ox), oy1 = cross(W.oz, W.ox);
+        const double hw = 0.5 * W.a0, hl = 0.5 * W.a2;
+        for (int c = 0; c < 4; ++c) {
+            const double sx = (c & 1) ? hw : -hw, sy = (c & 2) ? hl : -hl;
+            const d3 c0 = (old.pos + muls(old.ox, sx)) + muls(oy0, sy);
+            const d3 c1 = (W.pos + muls(W.ox, sx)) + muls(oy1, sy);
+            d = fmax(d, len(c1 - c0));
+        }
+    } else {
+        const double hh = 0.5 * W.a1;
+        for (int c = 0; c < 2; ++c) {
+            const double sz = c ? hh : -hh;
+            d = fmax(d, len((W.pos + muls(W.oz, sz)) - (old.pos + muls(old.oz, sz))));
+        }
+    }
+    d = d * (1.0 + 1e-9) + 1e-15 * (len(W.pos) + W.reach);
     W.path = (rflags[3 * sc + step % 3] ? 0.0 : old.path) + d;
     if (W.path > S.relist_half) atomicOr(&rflags[3 * sc + (step + 1) % 3], 1);
     walls[w] = W;
@@ -499,12 +518,6 @@ __device__ inline void cell_of(const GridDev &G, d3 p, unsigned &cx, unsigned &c
     cy = (unsigned)((p.y - G.ymin) / G.zr);
     cz = (unsigned)((p.z - G.zmin) / G.zr);
 }
-__device__ inline unsigned long long cmp_key(const GridDev &G, unsigned cx, unsigned cy, unsigned cz)
-{
-    const unsigned c[3] = {cx, cy, cz};
-    return ((unsigned long long)c[G.perm[0]] << 42) | ((unsigned long long)c[G.perm[1]] << 21) |
-           (unsigned long long)c[G.perm[2]];
-}
 __device__ inline long long lin_of(const GridDev &G, long long cx, long long cy, long long cz)
 {
     return ((cz + 1) * G.ny + (cy + 1)) * G.nx + (cx + 1);
@@ -588,25 +601,6 @@ __global__ void k_cellstart(const int *lin, int n, int *cstart, int *cend)
     if (s == n - 1 || lin[s] != lin[s + 1]) cend[lin[s]] = s + 1;
 }
 
-__device__ inline bool accept_sphere(const OuterArgs &a, const SceneDev &S, d3 xs, double safe_s, int gs, int j)
-{
-    if (!a.single_group) {
-        const int gj = a.t.models[a.g.mdl[j]].group;
-        if (a.t.laws[S.law_off + gs * S.n_groups + gj].law == LAW_NONE) return false;
-    } else if (a.t.laws[S.law_off + gs * S.n_groups + gs].law == LAW_NONE) {
-        return false;
-    }
-    const double4 pj = a.g.pos[j];
-    const d3 d = xs - mk(pj.x, pj.y, pj.z);
-    const double lim = safe_s + a.safe[j];
-    // the squared distance decides every pair away from the threshold; near it
-    // the reference's own test `len(d) > safe_i + safe_j` decides
-    const double d2 = dot(d, d), l2 = lim * lim;
-    if (d2 > l2 * (1.0 + 1e-9)) return false;
-    if (d2 < l2 * (1.0 - 1e-9)) return true;
-    return !(len(d) > lim);
-}
-
 __device__ inline bool accept_wall(const OuterArgs &a, const SceneDev &S, d3 xs, double safe_s, int gs, int w)
 {
     const WallDev &W = a.walls[w];
@@ -620,77 +614,108 @@ __device__ inline bool accept_wall(const OuterArgs &a, const SceneDev &S, d3 xs,
 // groups "current cell = c" of the Leap traversal are visited in comparator
 // order; within the own cell the 13 half-shell cells, the self cell and the
 // global DOs follow in reference order (LeapConsultant.h:207-444).
+// The scene's grid and law table are read once into registers and the
+// arrays are declared unaliased, so each candidate test costs one gather of
+// the partner's position and safe distance, issued one candidate ahead.
 __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.n) return;
-    const double4 p = a.g.pos[s];
+    const double4 *__restrict__ pos = a.g.pos;
+    const double *__restrict__ safe = a.safe;
+    const int *__restrict__ mdl = a.g.mdl;
+    const ModelDev *__restrict__ models = a.t.models;
+    const LawDev *__restrict__ laws = a.t.laws;
+    unsigned *__restrict__ nbr = a.nbr;
+    const size_t ld = (size_t)a.ld;
+    const unsigned ko = (unsigned)a.ko;
+    const bool single_group = a.single_group != 0;
+
+    const double4 p = pos[s];
     const d3 xs = mk(p.x, p.y, p.z);
-    const double safe_s = a.safe[s];
-    const ModelDev &ms = a.t.models[a.g.mdl[s]];
-    const int gs = ms.group;
-    const SceneDev &S = a.scenes[ms.scene];
-    const GridDev &G = S.G;
-    const int *cstart = a.cstart + S.cell_off, *cend = a.cend + S.cell_off;
+    const double safe_s = safe[s];
+    const ModelDev &ms = models[mdl[s]];
+    const int gs = ms.group, scene = ms.scene;
+    const SceneDev &Sg = a.scenes[scene];
+    const GridDev G = Sg.G;
+    const int law_off = Sg.law_off, n_groups = Sg.n_groups, w_begin = Sg.w_begin, w_end = Sg.w_end;
+    const int *__restrict__ cstart = a.cstart + Sg.cell_off;
+    const int *__restrict__ cend = a.cend + Sg.cell_off;
+    // partner groups with a sphere-sphere law (DOWorld::GetIactModel != 0)
+    const bool mask_ok = n_groups <= 64;
+    unsigned long long lawmask = 0;
+    if (mask_ok)
+        for (int g = 0; g < n_groups; ++g)
+            if (laws[law_off + gs * n_groups + g].law != LAW_NONE) lawmask |= 1ull << g;
+    const bool any_sphere_law = !single_group || !mask_ok || ((lawmask >> gs) & 1ull);
+
     unsigned cx, cy, cz;
     cell_of(G, xs, cx, cy, cz);
-    // 14 groups: own cell (slot -1) and the 13 cells that hold s's cell in their half shell
-    unsigned long long gkey[14];
-    int gslot[14];
-    int ng = 0;
-    gkey[ng] = cmp_key(G, cx, cy, cz);
-    gslot[ng++] = -1;
-    for (int k = 0; k < 13; ++k) {
-        const long long ox = (long long)cx - NB13[k][0], oy = (long long)cy - NB13[k][1],
-                        oz = (long long)cz - NB13[k][2];
-        if (ox < 0 || oy < 0 || oz < 0 || ox >= G.nx - 2 || oy >= G.ny - 2 || oz >= G.nz - 2) continue;
-        const long long l = lin_of(G, ox, oy, oz);
-        if (cend[l] <= cstart[l]) continue;
-        gkey[ng] = cmp_key(G, (unsigned)ox, (unsigned)oy, (unsigned)oz);
-        gslot[ng++] = k;
-    }
-    for (int u = 1; u < ng; ++u) {   // insertion sort by cell key
-        const unsigned long long kk = gkey[u];
-        const int ks = gslot[u];
-        int v = u - 1;
-        while (v >= 0 && gkey[v] > kk) {
-            gkey[v + 1] = gkey[v];
-            gslot[v + 1] = gslot[v];
-            --v;
-        }
-        gkey[v + 1] = kk;
-        gslot[v + 1] = ks;
-    }
     unsigned cnt = 0;
     auto emit = [&](unsigned j) {
-        if (cnt < (unsigned)a.ko) a.nbr[(size_t)cnt * a.ld + s] = j;
+        if (cnt < ko) nbr[(size_t)cnt * ld + s] = j;
         ++cnt;
     };
+    // DOMap::CalDistance of the pair against safe_i + safe_j: the squared
+    // distance decides every pair away from the threshold; near it the
+    // reference's own test `len(d) > safe_i + safe_j` decides
+    auto near = [&](double4 pj, double sj) -> bool {
+        const d3 d = xs - mk(pj.x, pj.y, pj.z);
+        const double lim = safe_s + sj;
+        const double d2 = dot(d, d), l2 = lim * lim;
+        if (d2 > l2 * (1.0 + 1e-9)) return false;
+        if (d2 < l2 * (1.0 - 1e-9)) return true;
+        return !(len(d) > lim);
+    };
+    auto has_law = [&](int j) -> bool {
+        if (single_group) return true;   // any_sphere_law holds
+        const int gj = models[mdl[j]].group;
+        return mask_ok ? ((lawmask >> gj) & 1ull) != 0 : laws[law_off + gs * n_groups + gj].law != LAW_NONE;
+    };
+    // a cell's spheres in groups of 4: the gathers of a group are in flight together
     auto scan_cell = [&](long long x, long long y, long long z, bool self) {
+        if (!any_sphere_law) return;
         if (x < -1 || y < -1 || z < -1 || x > G.nx - 3 || y > G.ny - 3 || z > G.nz - 3) return;
         const long long l = lin_of(G, x, y, z);
         const int b = cstart[l], e = cend[l];
-        for (int j = b; j < e; ++j) {
-            if (self && j == s) continue;
-            if (accept_sphere(a, S, xs, safe_s, gs, j)) emit((unsigned)j);
+        for (int j0 = b; j0 < e; j0 += 4) {
+            double4 pj[4];
+            double sj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int jj = j0 + u < e ? j0 + u : e - 1;
+                pj[u] = pos[jj];
+                sj[u] = safe[jj];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + u;
+                if (j >= e) break;
+                if (self && j == s) continue;
+                if (has_law(j) && near(pj[u], sj[u])) emit((unsigned)j);
+            }
         }
     };
-    for (int q = 0; q < ng; ++q) {
-        if (gslot[q] < 0) {
+    for (int q = 0; q < 14; ++q) {
+        const int slot = (int)((G.order >> (4 * q)) & 15ull);
+        if (slot == 13) {
             for (int k = 0; k < 13; ++k)
                 scan_cell((long long)cx + NB13[k][0], (long long)cy + NB13[k][1], (long long)cz + NB13[k][2], false);
             scan_cell(cx, cy, cz, true);
-            for (int w = S.w_begin; w < S.w_end; ++w)
-                if (accept_wall(a, S, xs, safe_s, gs, w)) emit((unsigned)(a.ld + w));
+            for (int w = w_begin; w < w_end; ++w)
+                if (accept_wall(a, Sg, xs, safe_s, gs, w)) emit((unsigned)(a.ld + w));
         } else {
-            const int k = gslot[q];
-            scan_cell((long long)cx - NB13[k][0], (long long)cy - NB13[k][1], (long long)cz - NB13[k][2], false);
+            // the cell whose half shell holds s's cell, if it exists (LeapConsultant.h:207-444)
+            const long long ox = (long long)cx - NB13[slot][0], oy = (long long)cy - NB13[slot][1],
+                            oz = (long long)cz - NB13[slot][2];
+            if (ox < 0 || oy < 0 || oz < 0 || ox >= G.nx - 2 || oy >= G.ny - 2 || oz >= G.nz - 2) continue;
+            scan_cell(ox, oy, oz, false);
         }
     }
     a.count[s] = cnt;
-    if (cnt > (unsigned)a.ko) {
+    if (cnt > ko) {
         atomicMax(&a.flags[5], (int)cnt);
-        if (a.ovf) atomicOr(&a.ovf[ms.scene], 1);
+        if (a.ovf) atomicOr(&a.ovf[scene], 1);
     }
 }
 
@@ -718,57 +743,82 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     extern __shared__ unsigned old_oid[];   // [ki][blockDim]
     if (s >= a.n) return;
+    const double4 *__restrict__ pos = a.g.pos;
+    const unsigned *__restrict__ oid = a.g.oid;
+    const unsigned *__restrict__ nbr = a.nbr;
+    const unsigned *__restrict__ ent_old = a.eold.ent;
+    unsigned *__restrict__ ent_new = a.ell.ent;
+    const size_t ld = (size_t)a.ld;
     const unsigned p = a.perm[s];
     const unsigned hdr = a.eold.hdr[p];
     const int h_old = hdr >> 7, c_old = hdr & 127;
     for (int k = 0; k < c_old; ++k) {
-        const unsigned j = a.eold.ent[eidx(h_old, k, a.ki, a.ld, p)] & PARTNER_MASK;
+        const unsigned j = ent_old[eidx(h_old, k, a.ki, a.ld, p)] & PARTNER_MASK;
         old_oid[k * blockDim.x + threadIdx.x] = j < (unsigned)a.ld ? a.gold.oid[j] : 0x80000000u + (j - a.ld);
     }
     const double skin = a.scenes[a.models[a.g.mdl[s]].scene].skin;
-    const double4 ps = a.g.pos[s];
+    const double4 ps = pos[s];
     const d3 xs = mk(ps.x, ps.y, ps.z);
-    const unsigned my_oid = a.g.oid[s];
+    const unsigned my_oid = oid[s];
+    const unsigned cnt = a.cnt[s];
     int q = 0;
-    for (unsigned t = 0; t < a.cnt[s]; ++t) {
-        const unsigned j = a.nbr[(size_t)t * a.ld + s];
-        Contact c;
-        unsigned key;
-        bool slave = true;
-        if (j < (unsigned)a.ld) {
-            const double4 pj = a.g.pos[j];
-            const d3 dx = mk(pj.x, pj.y, pj.z) - xs;
-            const double lim = ps.w + pj.w + skin;
-            if (dot(dx, dx) > lim * lim * (1.0 + 1e-9)) continue;   // surely beyond the skin
-            key = a.g.oid[j];
-            slave = my_oid < key;
-            if (slave) cd_ss(xs, ps.w, mk(pj.x, pj.y, pj.z), pj.w, c);
-            else cd_ss(mk(pj.x, pj.y, pj.z), pj.w, xs, ps.w, c);
-        } else {
-            key = 0x80000000u + (j - a.ld);
-            cd_wall(xs, ps.w, a.walls[j - a.ld], c);
+    // candidates in groups of 4: the list entries, then the partner gathers
+    // of a group are in flight together
+    for (unsigned t0 = 0; t0 < cnt; t0 += 4) {
+        unsigned jv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) jv[u] = nbr[(size_t)(t0 + u < cnt ? t0 + u : cnt - 1) * ld + s];
+        double4 pj[4];
+        unsigned kj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool sph = jv[u] < (unsigned)a.ld;
+            pj[u] = sph ? pos[jv[u]] : make_double4(0.0, 0.0, 0.0, 0.0);
+            kj[u] = sph ? oid[jv[u]] : 0x80000000u + (jv[u] - a.ld);
         }
-        if (c.depth < -skin) continue;
-        if (q >= a.ki) {
-            atomicOr(&a.flags[3], ERR_INNER_CAP);
-            break;
+        bool full = false;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (t0 + u >= cnt || full) break;
+            const unsigned j = jv[u];
+            Contact c;
+            const unsigned key = kj[u];
+            bool slave = true;
+            if (j < (unsigned)a.ld) {
+                const d3 xj = mk(pj[u].x, pj[u].y, pj[u].z);
+                const d3 dx = xj - xs;
+                const double lim = ps.w + pj[u].w + skin;
+                if (dot(dx, dx) > lim * lim * (1.0 + 1e-9)) continue;   // surely beyond the skin
+                slave = my_oid < key;
+                if (slave) cd_ss(xs, ps.w, xj, pj[u].w, c);
+                else cd_ss(xj, pj[u].w, xs, ps.w, c);
+            } else {
+                cd_wall(xs, ps.w, a.walls[j - a.ld], c);
+            }
+            if (c.depth < -skin) continue;
+            if (q >= a.ki) {
+                atomicOr(&a.flags[3], ERR_INNER_CAP);
+                full = true;
+                break;
+            }
+            unsigned f = F_DEFAULT;
+            Hist hs;
+            hs.s = mk(0, 0, 0);
+            hs.aux = 0.0;
+            if (c.active) {
+                for (int k = 0; k < c_old; ++k)
+                    if (old_oid[k * blockDim.x + threadIdx.x] == key) {
+                        f = (ent_old[eidx(h_old, k, a.ki, a.ld, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
+                        if (f & F_CONTACT) load_hist(a.eold, h_old, k, a.ki, a.ld, p, hs);
+                        break;
+                    }
+            }
+            if (slave) f |= F_SLAVE;
+            ent_new[eidx(0, q, a.ki, a.ld, s)] = j | f;
+            if (f & F_CONTACT) store_hist(a.ell, 0, q, a.ki, a.ld, s, hs);
+            ++q;
         }
-        unsigned f = F_DEFAULT;
-        Hist hs;
-        hs.s = mk(0, 0, 0);
-        hs.aux = 0.0;
-        if (c.active) {
-            for (int k = 0; k < c_old; ++k)
-                if (old_oid[k * blockDim.x + threadIdx.x] == key) {
-                    f = (a.eold.ent[eidx(h_old, k, a.ki, a.ld, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
-                    if (f & F_CONTACT) load_hist(a.eold, h_old, k, a.ki, a.ld, p, hs);
-                    break;
-                }
-        }
-        if (slave) f |= F_SLAVE;
-        a.ell.ent[eidx(0, q, a.ki, a.ld, s)] = j | f;
-        if (f & F_CONTACT) store_hist(a.ell, 0, q, a.ki, a.ld, s, hs);
-        ++q;
+        if (full) break;
     }
     a.ell.hdr[s] = (unsigned char)q;
     atomicMax(&a.flags[4], q);

@@ -5,8 +5,10 @@ sigma_ln 0.2, clipped to [0.5, 2] mm, 64 radius classes, rho 2500) in a drum
 of 64 quasi-plates + 2 end plates (R = 0.15 m) rotating at 2 rad/s about a
 horizontal axis; ISwBSDBF linear spring-dashpot kn = 1e4 N/m, ks = 0.8 kn,
 e = 0.6, mu = 0.4; dt = 5e-6 s; LeapConsultant candidate lists rebuilt every
-UpIact steps; FP64 throughout.  A "step" is one SimMediator::Run over all
-spheres.  The run is the configuration's own: the spheres start as a packed
+UpIact steps.  The headline is FP64, the reference's arithmetic; the same
+workload then runs with the FP32/FP64-mixed pair solves (vg_set_precision),
+reported under "mixed" with the error of a mixed step against FP64 on the
+run's end state.  A "step" is one SimMediator::Run over all spheres.  The run is the configuration's own: the spheres start as a packed
 bed (see scenes.drum_scene) and settle into the drum, which turns at 2 rad/s
 from t = 0.  The neighbour-list skin is tuned on the first steps of the run,
 --warmup steps follow, then --steps steps (10,000 by default, the
@@ -229,42 +231,16 @@ def serving_run(count, up_iact, seed=5, loop=True):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--prep", type=int, default=0, help="untimed settling steps in the resting drum")
-    ap.add_argument("--up-iact", type=int, default=100)
-    ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--skin", type=float, default=None, help="neighbour-list skin in m (skips the tuning)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batch-scenes", type=int, default=256, help="batched configuration: scenes (0: skip)")
-    ap.add_argument("--batch-steps", type=int, default=1000, help="batched configuration: steps per scene")
-    ap.add_argument("--serve-requests", type=int, default=24, help="serving configuration: requests (0: skip)")
-    ap.add_argument("--no-loop", action="store_true", help="skip the one-scene-at-a-time comparisons")
-    ap.add_argument("--batch-up-iact", type=int, default=20, help="UpIact of the batched and served scenes")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (one GPU holds the whole domain); "
-              "run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
-    from vedo_amd import scenes
-    from vedo_amd.gpu import GpuSimulation, copy_bandwidth
-
-    copy_gbps = copy_bandwidth(1 << 30, 10)
-    log(f"[bench] streaming copy kernel: {copy_gbps:.0f} GB/s")
-
-    t = time.perf_counter()
-    wa, wall_ids, c, a = scenes.drum_scene(args.n, arrays=True, up_iact=args.up_iact)
-    log(f"[bench] scene: {args.n} spheres + {len(wall_ids)} walls, drum length {wa.pos[:, 1].max():.3f} m "
-        f"({time.perf_counter() - t:.1f} s)")
+def drum_run(args, wa, wall_ids, c, a, mixed=False, skin=None, tune=True):
+    """The drum workload on a fresh handle: optional resting prep, the drum
+    turning at 2 rad/s, skin tuning (unless given), --warmup steps, then
+    args.steps timed steps.  Returns (handle, measurements)."""
+    from vedo_amd.gpu import PRECISION_MIXED, GpuSimulation
     t = time.perf_counter()
     sim = GpuSimulation(wa)
-    log(f"[bench] created on GPU ({time.perf_counter() - t:.1f} s)")
+    if mixed:
+        sim.set_precision(PRECISION_MIXED)
+    log(f"[bench] created on GPU ({time.perf_counter() - t:.1f} s){' (mixed precision)' if mixed else ''}")
     done, quenches = 0, 0
     if args.prep > 0:
         t = time.perf_counter()
@@ -275,15 +251,14 @@ def main():
         log(f"[bench] prep {done} steps, {quenches} quenches, {time.perf_counter() - t:.1f} s")
     # the drum turns at 2 rad/s (from t = 0 unless a resting prep ran)
     sim.set_wall_rotation(wall_ids, c, a, 2.0, sim.time)
-    if args.skin is not None:
-        sim.set_skin(args.skin)
-    skin = sim.skin
-    tune = None
-    if not args.no_tune and args.skin is None:
+    if skin is not None:
+        sim.set_skin(skin)
+    elif tune:
         from vedo_amd.autotune import tune_skin
-        skin, tune = tune_skin(sim, float(wa.model_attr[wa.model_shape == 1, 0].min()), args.up_iact)
-        log(f"[bench] skin autotune: {({f'{k * 1e3:.3f}mm': round(v * 1e6, 1) for k, v in tune.items()})} us/step "
-            f"-> {skin * 1e3:.3f} mm")
+        skin, table = tune_skin(sim, float(wa.model_attr[wa.model_shape == 1, 0].min()), args.up_iact)
+        log(f"[bench] skin autotune: {({f'{k * 1e3:.3f}mm': round(v * 1e6, 1) for k, v in table.items()})} "
+            f"us/step -> {skin * 1e3:.3f} mm")
+    skin = sim.skin
     sim.step(args.warmup)
     s0 = sim.stats()
     # timed region: K steps bracketed by device synchronisation
@@ -302,33 +277,103 @@ def main():
     c_act = stats["contact_steps"] / args.steps
     bytes_step = BYTES_PER_SPHERE * n + BYTES_PER_CONTACT * c_act
     k_avg_s = k_ms / 1e3 / max(k_launches, 1)
-    achieved = bytes_step / k_avg_s / 1e9
-    value = n * args.steps / wall
-    traffic, traffic_src = measured_traffic()
     log(f"[bench] timed {args.steps} steps in {wall:.3f} s; step kernel avg {k_avg_s * 1e6:.1f} us over "
         f"{k_launches} launches; {c_act:.0f} contact pairs per step; {stats}")
+    return sim, {"value": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
+                 "kernel_avg_us": k_avg_s * 1e6, "achieved_gbs": bytes_step / k_avg_s / 1e9,
+                 "bytes_per_step": bytes_step, "active_contacts": c_act, "skin": skin,
+                 "rebuilds": stats["rebuilds"], "relists": stats["relists"], "prep_steps": done,
+                 "prep_quenches": quenches}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--prep", type=int, default=0, help="untimed settling steps in the resting drum")
+    ap.add_argument("--up-iact", type=int, default=100)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--skin", type=float, default=None, help="neighbour-list skin in m (skips the tuning)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=256, help="batched configuration: scenes (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=1000, help="batched configuration: steps per scene")
+    ap.add_argument("--serve-requests", type=int, default=24, help="serving configuration: requests (0: skip)")
+    ap.add_argument("--no-loop", action="store_true", help="skip the one-scene-at-a-time comparisons")
+    ap.add_argument("--batch-up-iact", type=int, default=20, help="UpIact of the batched and served scenes")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="arithmetic of the headline run (fp64: the reference's)")
+    ap.add_argument("--mixed-steps", type=int, default=None,
+                    help="steps of the FP32/FP64-mixed run of the same workload (default --steps; 0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one GPU holds the whole domain); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from vedo_amd import scenes
+    from vedo_amd.gpu import copy_bandwidth
+
+    copy_gbps = copy_bandwidth(1 << 30, 10)
+    log(f"[bench] streaming copy kernel: {copy_gbps:.0f} GB/s")
+
+    t = time.perf_counter()
+    wa, wall_ids, c, a = scenes.drum_scene(args.n, arrays=True, up_iact=args.up_iact)
+    log(f"[bench] scene: {args.n} spheres + {len(wall_ids)} walls, drum length {wa.pos[:, 1].max():.3f} m "
+        f"({time.perf_counter() - t:.1f} s)")
+    headline_mixed = args.precision == "mixed"
+    sim, run = drum_run(args, wa, wall_ids, c, a, mixed=headline_mixed, skin=args.skin, tune=not args.no_tune)
+    n = args.n
+    traffic, traffic_src = measured_traffic()
     out = {
-        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "metric": METRIC, "value": run["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": run["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32" if headline_mixed else "f64",
+        "data": "synthetic",
         "config": {"workload": "rotating-drum polydisperse spheres, ISwBSDBF, LeapConsultant", "spheres": n,
                    "walls": len(wall_ids), "radius_classes": 64, "dt": wa.dt, "up_iact": args.up_iact,
-                   "prep_steps": done, "prep_quenches": quenches, "active_contacts": c_act, "neighbour_skin_m": skin,
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "copy_kernel_gbps": copy_gbps, "frac_of_copy": achieved / copy_gbps,
-                     "kernel": "k_step", "kernel_avg_us": k_avg_s * 1e6,
-                     "algorithmic_bytes_per_launch": bytes_step},
-        "rebuilds": stats["rebuilds"], "relists": stats["relists"],
+                   "prep_steps": run["prep_steps"], "prep_quenches": run["prep_quenches"],
+                   "active_contacts": run["active_contacts"], "neighbour_skin_m": run["skin"],
+                   "precision": args.precision, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": run["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": run["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "copy_kernel_gbps": copy_gbps, "frac_of_copy": run["achieved_gbs"] / copy_gbps,
+                     "kernel": "k_step", "kernel_avg_us": run["kernel_avg_us"],
+                     "algorithmic_bytes_per_launch": run["bytes_per_step"]},
+        "rebuilds": run["rebuilds"], "relists": run["relists"],
     }
     if not args.no_cpu:
         st = sim.state()
         out["cpu_baseline"] = cpu_baseline(st, wa, n, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
-    sim.close()
-    del sim
+    mixed_steps = args.steps if args.mixed_steps is None else args.mixed_steps
+    if mixed_steps > 0 and not headline_mixed:
+        # the same workload with the FP32/FP64-mixed pair solves, and the
+        # error of one mixed step against FP64 from the end state of the run
+        import dataclasses
+        from vedo_amd.precision import compare_precisions
+        st = sim.state()
+        wa_end = dataclasses.replace(wa, pos=st["pos"], vel=st["vel"], ox=st["ox"], oz=st["oz"], av=st["av"],
+                                     t_current=sim.time)
+        sim.close()
+        del sim
+        margs = argparse.Namespace(**vars(args))
+        margs.steps = mixed_steps
+        msim, mrun = drum_run(margs, wa, wall_ids, c, a, mixed=True, skin=run["skin"], tune=False)
+        msim.close()
+        del msim
+        err = compare_precisions(wa_end, steps=1)
+        log(f"[bench] mixed vs fp64 on the end state: {err}")
+        out["mixed"] = {"value": mrun["value"], "unit": "particle-steps/s", "steps": mixed_steps,
+                        "ms_per_step": mrun["ms_per_step"], "kernel_avg_us": mrun["kernel_avg_us"],
+                        "achieved_gbs": mrun["achieved_gbs"], "frac": mrun["achieved_gbs"] / HBM_PEAK_GBS,
+                        "speedup_vs_fp64": mrun["value"] / run["value"], "error_vs_fp64": err}
+    else:
+        sim.close()
+        del sim
     if args.batch_scenes > 0:
         out["batched"] = batched_boxes(args.batch_scenes, args.batch_steps, args.batch_up_iact, loop=not args.no_loop)
     if args.serve_requests > 0:

@@ -154,6 +154,19 @@ int vg_num_interactions(vg_scene *h, long long *n);
  * reference walks its whole candidate list every step. */
 int vg_set_skin(vg_scene *h, double skin);
 double vg_get_skin(const vg_scene *h);
+/* Arithmetic of the pair solve.  VG_PRECISION_FP64 (the default) is the
+ * reference's: every operation in FP64, in its order, so results are the
+ * reference's bits.  VG_PRECISION_MIXED keeps in FP64 the contact depth (so
+ * the active set and every ImpactStatus flag are the FP64 ones), the stored
+ * histories, the impulse sums and the integration, and forms the contact
+ * normal, lever arms, relative velocity, forces and torques of each pair in
+ * FP32 (per-pair force error of a few FP32 rounding units).  Takes effect
+ * at the next step.  No reference counterpart: the reference is FP64 only. */
+#define VG_PRECISION_FP64 0
+#define VG_PRECISION_MIXED 1
+int vg_set_precision(vg_scene *h, int precision);
+int vg_get_precision(const vg_scene *h);
+
 /* Active contacts (ContactInfo.bActive of the last detect) with their
  * ImpactStatus, keyed (lo, hi) -- the IactRecordTab payload that
  * RebuildIactRecordTab saves (NBSConsultant.cpp:376-395).  Call with

@@ -76,6 +76,7 @@ struct vg_scene {
     // batch-wide properties of the placed scenes
     bool sd_only = true;
     int single_group = 1, no_bond = 1;
+    int precision = VG_PRECISION_FP64;   // vg_set_precision
     unsigned up_iact = 1;
     unsigned long long round = 0, step_count = 0;
     long long rebuilds = 0;
@@ -1167,12 +1168,20 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
     const bool kt = 2 * (h->kev_used + 1) <= (int)h->kev.size();
     if (kt) HIPCHK(hipEventRecord(h->kev[2 * h->kev_used], h->stream));
     const dim3 grid(nblk(std::max(h->n, h->scap), 256)), blk(256);
-    if (h->sd_only) {
-        if (rec) hipLaunchKernelGGL((k_step<true, true>), grid, blk, 0, h->stream, a);
-        else hipLaunchKernelGGL((k_step<false, true>), grid, blk, 0, h->stream, a);
+    if (h->precision == VG_PRECISION_MIXED) {
+        if (h->sd_only) {
+            if (rec) hipLaunchKernelGGL((k_step<true, true, true>), grid, blk, 0, h->stream, a);
+            else hipLaunchKernelGGL((k_step<false, true, true>), grid, blk, 0, h->stream, a);
+        } else {
+            if (rec) hipLaunchKernelGGL((k_step<true, false, true>), grid, blk, 0, h->stream, a);
+            else hipLaunchKernelGGL((k_step<false, false, true>), grid, blk, 0, h->stream, a);
+        }
+    } else if (h->sd_only) {
+        if (rec) hipLaunchKernelGGL((k_step<true, true, false>), grid, blk, 0, h->stream, a);
+        else hipLaunchKernelGGL((k_step<false, true, false>), grid, blk, 0, h->stream, a);
     } else {
-        if (rec) hipLaunchKernelGGL((k_step<true, false>), grid, blk, 0, h->stream, a);
-        else hipLaunchKernelGGL((k_step<false, false>), grid, blk, 0, h->stream, a);
+        if (rec) hipLaunchKernelGGL((k_step<true, false, false>), grid, blk, 0, h->stream, a);
+        else hipLaunchKernelGGL((k_step<false, false, false>), grid, blk, 0, h->stream, a);
     }
     if (kt) {
         HIPCHK(hipEventRecord(h->kev[2 * h->kev_used + 1], h->stream));
@@ -1479,6 +1488,15 @@ extern "C" int vg_set_skin(vg_scene *h, double skin)
     HIPCHK(hipMemcpy(h->rflags, r.data(), sizeof(int) * r.size(), hipMemcpyHostToDevice));
     return 0;
 }
+
+extern "C" int vg_set_precision(vg_scene *h, int precision)
+{
+    if (!h || (precision != VG_PRECISION_FP64 && precision != VG_PRECISION_MIXED)) return VG_EINVAL;
+    h->precision = precision;
+    return 0;
+}
+
+extern "C" int vg_get_precision(const vg_scene *h) { return h ? h->precision : -1; }
 
 extern "C" double vg_get_skin(const vg_scene *h)
 {

@@ -127,7 +127,7 @@ struct ActiveOut {
     double rec[13];
 };
 
-template <bool REC, bool SD> __global__ void k_step(StepArgs a);
+template <bool REC, bool SD, bool MIX> __global__ void k_step(StepArgs a);
 __global__ void k_walls(WallDev *walls, int nw, const SceneDev *scenes, double *stime, const double *assigned,
                         const WallDev *ref, const WallKin *kin, int *rflags, int step);
 __global__ void k_wall_path_reset(WallDev *walls, int nw);

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include "vg_kernels.h"
+#include "vg_mixed.h"
 
 namespace vg {
 
@@ -149,7 +150,7 @@ __device__ inline int xcd_block(int b, int nb)
     return x < full ? x * per + q : full * per + (x - full) * (per - 1) + q;
 }
 
-template <bool REC, bool SD>
+template <bool REC, bool SD, bool MIX>
 __global__ __launch_bounds__(256) void k_step(StepArgs a)
 {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -267,15 +268,27 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             const int gj = a.single_group ? gi : a.t.models[a.in.mdl[j]].group;
             const LawDev &law = a.t.laws[S.law_off + gi * S.n_groups + gj];
             if (hs.f & F_SLAVE) {
-                cd_ss(xi, ri, ld3(pj), pj.w, c);
-                solve<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(pj), ld3(vj), ld3(wj), c, S.dt, hs, o, r);
+                if (MIX) {
+                    ContactM cm;
+                    cd_ss_m(xi, ri, ld3(pj), pj.w, cm);
+                    solve_mixed<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(vj), ld3(wj), cm, S.dt, hs, o, r);
+                } else {
+                    cd_ss(xi, ri, ld3(pj), pj.w, c);
+                    solve<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(pj), ld3(vj), ld3(wj), c, S.dt, hs, o, r);
+                }
                 if (o.added) {
                     I = I + o.imp;
                     if (!o.linear_only) L = L + o.ang_a;
                 }
             } else {
-                cd_ss(ld3(pj), pj.w, xi, ri, c);
-                solve<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), xi, vi, wi, c, S.dt, hs, o, r);
+                if (MIX) {
+                    ContactM cm;
+                    cd_ss_m(ld3(pj), pj.w, xi, ri, cm);
+                    solve_mixed<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), vi, wi, cm, S.dt, hs, o, r);
+                } else {
+                    cd_ss(ld3(pj), pj.w, xi, ri, c);
+                    solve<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), xi, vi, wi, c, S.dt, hs, o, r);
+                }
                 if (o.added) {
                     I = I + (o.linear_only ? -o.imp : smul(-1.0, o.imp));
                     if (!o.linear_only) L = L + o.ang_b;
@@ -285,7 +298,13 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             const WallDev &w = a.walls[j - a.ld];
             const LawDev &law = a.t.laws[S.law_off + gi * S.n_groups + w.group];
             cd_wall(xi, ri, w, c);
-            solve<REC, SD>(law, mi, 0.0, xi, vi, wi, w.pos, w.vel, w.av, c, S.dt, hs, o, r);
+            if (MIX) {
+                ContactM cm;
+                to_mixed(c, xi, w.pos, cm);
+                solve_mixed<REC, SD>(law, mi, 0.0, xi, vi, wi, w.vel, w.av, cm, S.dt, hs, o, r);
+            } else {
+                solve<REC, SD>(law, mi, 0.0, xi, vi, wi, w.pos, w.vel, w.av, c, S.dt, hs, o, r);
+            }
             if (o.added) {
                 I = I + o.imp;
                 if (!o.linear_only) L = L + o.ang_a;
@@ -340,10 +359,14 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
     if (__lane_id() == (unsigned)(__ffsll((long long)live) - 1)) a.contacts[i >> 6] += tot;
 }
 
-template __global__ void k_step<false, false>(StepArgs);
-template __global__ void k_step<true, false>(StepArgs);
-template __global__ void k_step<false, true>(StepArgs);
-template __global__ void k_step<true, true>(StepArgs);
+template __global__ void k_step<false, false, false>(StepArgs);
+template __global__ void k_step<true, false, false>(StepArgs);
+template __global__ void k_step<false, true, false>(StepArgs);
+template __global__ void k_step<true, true, false>(StepArgs);
+template __global__ void k_step<false, false, true>(StepArgs);
+template __global__ void k_step<true, false, true>(StepArgs);
+template __global__ void k_step<false, true, true>(StepArgs);
+template __global__ void k_step<true, true, true>(StepArgs);
 
 // Wall responses after the spheres' (DOFixedQuasiPlate.h:16-19,
 // DOOrbitalQuasiPlate.cpp:18-34) and kinematic prescriptions

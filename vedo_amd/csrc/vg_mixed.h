@@ -1,0 +1,257 @@
+// vg_mixed.h -- the FP32/FP64 mixed-precision pair solve.
+//
+// Same physics and branch structure as solve() in vg_device.h (the five
+// ImpactSolvers of the reference), with this split of precisions:
+//   FP64: every decision that sets an ImpactStatus flag -- the overlap
+//         depth of the contact detection (so the active set is the FP64
+//         one, bit for bit), the normal-bond test -- the stored history
+//         (shear force, Kn / initial velocity) and its update, the
+//         impulse sums and the integration (in the step kernel);
+//   FP32: contact normal and lever arms, relative velocity, the normal and
+//         damping forces, the tangential increment, the friction cap and
+//         the impulse and torque of the pair.
+// The per-pair force error against the FP64 path is a few FP32 rounding
+// units of the pair's force, far below 1e-5 relative.
+#pragma once
+
+#include "vg_device.h"
+
+namespace vg {
+
+struct f3 {
+    float x, y, z;
+};
+
+__device__ inline f3 mkf(float x, float y, float z) { return f3{x, y, z}; }
+__device__ inline f3 tof(d3 a) { return mkf((float)a.x, (float)a.y, (float)a.z); }
+__device__ inline d3 tod(f3 a) { return mk((double)a.x, (double)a.y, (double)a.z); }
+__device__ inline f3 operator+(f3 a, f3 b) { return mkf(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline f3 operator-(f3 a, f3 b) { return mkf(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline f3 operator-(f3 a) { return mkf(-a.x, -a.y, -a.z); }
+__device__ inline f3 operator*(float s, f3 v) { return mkf(s * v.x, s * v.y, s * v.z); }
+__device__ inline float dotf(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ inline f3 crossf(f3 a, f3 b)
+{
+    return mkf(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ inline float lenf(f3 a) { return sqrtf(dotf(a, a)); }
+// unit vector; zero for the zero vector (Vector3d::direction)
+__device__ inline f3 dirf(f3 a)
+{
+    const float l = lenf(a);
+    return l != 0.0f ? (1.0f / l) * a : mkf(0.0f, 0.0f, 0.0f);
+}
+
+// Contact geometry of a pair for the mixed solve: the FP64 depth and
+// activity, the FP32 normal (slave -> master) and lever arms from the slave
+// and the master centres to the contact point.
+struct ContactM {
+    double depth;
+    bool active;
+    f3 n, ra, rb;
+    d3 n64;   // the normal in FP64, for the split of the centre-velocity difference
+};
+
+// CDSphere_Sphere (CDSphere2Sphere.cpp:14-60) for slave a, master b: the
+// depth rA + rB - |xB - xA| in FP64 exactly as cd_ss forms it; the contact
+// point P = xA + (rA - depth/2) n enters only through P - xA and P - xB.
+__device__ inline void cd_ss_m(d3 xa, double ra, d3 xb, double rb, ContactM &c)
+{
+    const d3 im = xb - xa;
+    const double l = len(im);
+    c.depth = ra + rb - l;
+    c.active = c.depth > 0;
+    const f3 imf = tof(im);
+    const double il = l != 0.0 ? 1.0 / l : 0.0;
+    c.n64 = mk(im.x * il, im.y * il, im.z * il);
+    c.n = tof(c.n64);
+    c.ra = (float)(ra - 0.5 * c.depth) * c.n;
+    c.rb = c.ra - imf;
+}
+
+// A contact detected in FP64 (sphere-wall) in the mixed form.
+__device__ inline void to_mixed(const Contact &c, d3 xa, d3 xb, ContactM &m)
+{
+    m.depth = c.depth;
+    m.active = c.active;
+    m.n64 = c.n;
+    m.n = tof(c.n);
+    m.ra = tof(c.p - xa);
+    m.rb = tof(c.p - xb);
+}
+
+// Relative velocity at the contact point, v = vA - vB + wA x rA - wB x rB,
+// split into its normal part vn = (v.n) n and tangential part vs = v - vn.
+// The centre-velocity difference is split in FP64: its tangential part can
+// be far smaller than its normal part, and an FP32 rounding of the whole
+// difference would swamp it.  The rotational part is formed in FP32.
+__device__ inline void rel_vel(d3 va, d3 vb, f3 wa, f3 wb, const ContactM &c, f3 &vn, f3 &vs)
+{
+    const d3 dv = va - vb;
+    const double dn = dot(dv, c.n64);
+    const f3 dvt = tof(dv - smul(dn, c.n64));
+    const f3 rot = crossf(wa, c.ra) - crossf(wb, c.rb);
+    const float rn = dotf(rot, c.n);
+    vn = ((float)dn + rn) * c.n;
+    vs = dvt + (rot - rn * c.n);
+}
+
+// ShearForceRotation (ISwModels.cpp:110-123) of the FP64 history onto the
+// FP32 direction of the tangential velocity, in FP64.
+__device__ inline d3 shear_rotation_m(d3 hs, f3 vs)
+{
+    const float mag = sqrtf((float)dot(hs, hs));
+    const f3 u = dirf(vs);
+    const float sgn = (dot(hs, tod(vs)) <= 0.0) ? -mag : mag;
+    return tod(sgn * u);
+}
+
+// Torques: a sphere's lever arm to the contact point lies along the contact
+// normal, as do the normal spring and dashpot forces, so their torque is zero
+// and the sphere's torque is that of the tangential force alone; forming it
+// from the tangential force avoids the FP32 cancellation of ra x fn.  A
+// wall's lever arm (master with zero sudo-mass) takes the whole force.
+// xa: the slave's centre, for the recorded contact point only.
+template <bool REC, bool SD>
+__device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa, d3 va, d3 wa, d3 vb, d3 wb,
+                                   const ContactM &c, double dt, Hist &h, PairOut &o, RecOut &r)
+{
+    o.added = false;
+    o.linear_only = false;
+    const int law = L.law;
+    const double depth = c.depth;
+    const float df = (float)depth, dtf = (float)dt;
+    const f3 n = c.n;
+    const f3 waf = tof(wa), wbf = tof(wb);
+    if (!SD && law == LAW_HERTZ) {   // ISwHertz (ISwHertz.cpp:20-94)
+        o.linear_only = true;
+        if (!c.active) {
+            st_clean(h);
+            return;
+        }
+        f3 fn = mkf(0.0f, 0.0f, 0.0f);
+        if (depth > 0.0) {
+            f3 vn, vs;
+            rel_vel(va, vb, waf, wbf, c, vn, vs);
+            if (!(h.f & F_CONTACT)) {
+                h.f |= F_CONTACT;
+                h.aux = (double)lenf(vn);
+            }
+            fn = -((float)L.p[0] * (df * sqrtf(df))) * n - (float)L.p[1] * vn;
+        } else {
+            st_clean_contact(h);
+        }
+        const f3 imp = dtf * fn;
+        o.imp = tod(imp);
+        o.ang_a = o.ang_b = mk(0, 0, 0);
+        o.added = true;
+        if (REC) {
+            r.point = xa + tod(c.ra); r.dir = tod(n); r.overlap = depth;
+            r.to_master = -o.imp; r.ang_to_master = mk(0, 0, 0);
+        }
+        return;
+    }
+    if (!SD && law == LAW_H) {   // ISwH (ISwH.cpp:30-156)
+        const double knh = L.p[0], meff = L.p[1], betan = L.p[4];
+        if (!c.active) {
+            st_clean(h);
+            return;
+        }
+        f3 ra = mkf(0.0f, 0.0f, 0.0f), rb = ra, f = ra, fst = ra;
+        if (depth > 0.0) {
+            ra = c.ra;
+            rb = c.rb;
+            f3 vn, vs;
+            rel_vel(va, vb, waf, wbf, c, vn, vs);
+            double kn;
+            if (!(h.f & F_CONTACT)) {
+                // first contact: Kn from the approach speed, kept in the history (FP64)
+                const double vl = (double)lenf(vn);
+                if (vl == 0.0) kn = knh;
+                else if (betan < 0.99)
+                    kn = 0.957832746294131 / (1.0 - betan * betan) * pow(meff, 0.2) * pow(knh, 0.8) * pow(vl, 0.4);
+                else
+                    kn = 479.155951122633 * pow(meff, 0.2) * pow(knh, 0.8) * pow(vl, 0.4);
+                h.aux = kn;
+                h.f |= F_CONTACT;
+            } else {
+                kn = h.aux;
+            }
+            const float knf = (float)kn;
+            const float ncd = 2.0f * sqrtf((float)meff * knf);
+            const float ks = (float)L.p[2] * cbrtf((float)L.p[3] * fabsf(knf * df));
+            const f3 fn = -(knf * df) * n - (ncd * (float)betan) * vn;
+            const float ff = (float)L.p[5] * knf * df;
+            d3 fs = shear_rotation_m(h.s, vs) - tod((ks * dtf) * vs);
+            const f3 fsf = tof(fs);
+            if (lenf(fsf) >= ff) fs = tod(ff * dirf(fsf));
+            h.s = fs;
+            fst = tof(fs);
+            f = fn + fst;
+        } else {
+            st_clean_contact(h);
+        }
+        o.imp = tod(dtf * f);
+        o.ang_a = tod(dtf * crossf(ra, fst));
+        o.ang_b = tod(-dtf * crossf(rb, mb == 0 ? f : fst));
+        o.added = true;
+        if (REC) {
+            r.point = xa + tod(c.ra); r.dir = tod(n); r.overlap = depth;
+            r.to_master = smul(-1.0, o.imp); r.ang_to_master = o.ang_b;
+        }
+        return;
+    }
+    // spring-dashpot family (ISwBSDBF.cpp:26-139, ISwBSDB2F.cpp:26-149, ISwBtSDBF.cpp:26-137)
+    const double kn = L.p[0], bn = L.p[4], bs = L.p[5];
+    const float knf = (float)kn, ksf = (float)L.p[1], fc = (float)L.p[6];
+    float me;   // ISwModels::CriticalDamping (ISwModels.cpp:88-108)
+    if (mb == 0) me = (float)ma;
+    else if (ma == 0) me = (float)mb;
+    else me = (float)(0.5 * (ma + mb));
+    const float cn = (float)L.p[2] * (2.0f * sqrtf(me * knf));
+    const float cs = (float)L.p[3] * (2.0f * sqrtf(me * ksf));
+    const bool nbond = (depth <= 0.0) && (kn != 0.0) && (bn != 0.0) && (-depth * kn < bn);
+    if (!c.active && !nbond) {
+        st_clean(h);
+        return;
+    }
+    f3 ra = mkf(0.0f, 0.0f, 0.0f), rb = ra, fn = ra, fsf = ra;
+    if (depth > 0.0) {
+        ra = c.ra;
+        rb = c.rb;
+        f3 vn, vs;
+        rel_vel(va, vb, waf, wbf, c, vn, vs);
+        fn = -(knf * df) * n - cn * vn;
+        const float fmax = fc * knf * df;
+        if (law == BTSDBF) {
+            fsf = (-ksf * dtf) * vs - cs * vs;
+            if (bs == 0.0 ? lenf(fsf) > fmax : lenf(fsf) > (float)bs) fsf = fmax * dirf(fsf);
+            else fsf = fsf - cs * vs;
+        } else {
+            d3 fs = shear_rotation_m(h.s, vs) - tod((ksf * dtf) * vs);
+            const f3 ft = tof(fs);
+            if (bs == 0.0 ? lenf(ft) > fmax : lenf(ft) > (float)bs) fs = tod(fmax * dirf(ft));
+            else fs = fs - tod(cs * vs);
+            h.s = fs;
+            fsf = tof(fs);
+        }
+        h.f |= F_CONTACT;
+    } else if (law != BSDB2F || (h.f & F_LASTBOND)) {
+        // tension of a normal bond: relative velocity of the centres only
+        const f3 vn = (float)dot(va - vb, c.n64) * n;
+        if (law == BTSDBF || dotf(vn, n) < 0.0f) fn = -(knf * df) * n - cn * vn;
+        else fn = -cn * vn;
+        st_clean_contact(h);
+    }
+    const f3 f = fn + fsf;
+    o.imp = tod(dtf * f);
+    o.ang_a = tod(dtf * crossf(ra, fsf));
+    o.ang_b = tod(-dtf * crossf(rb, mb == 0 ? f : fsf));
+    o.added = true;
+    if (REC) {
+        r.point = xa + tod(c.ra); r.dir = tod(n); r.overlap = depth;
+        r.to_master = smul(-1.0, o.imp); r.ang_to_master = o.ang_b;
+    }
+}
+
+}  // namespace vg

@@ -64,7 +64,7 @@ assert STATUS_DTYPE.itemsize == C.sizeof(VgStatus)
 
 EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_set_history", "vg_step",
            "vg_sync", "vg_step_assigned", "vg_set_wall_rotation", "vg_get_state", "vg_time",
-           "vg_num_interactions", "vg_set_skin", "vg_get_skin", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
+           "vg_num_interactions", "vg_set_skin", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
            "vg_system_energy", "vg_batch_create", "vg_admit", "vg_retire", "vg_scene_status",
            "vg_scene_get_state", "vg_scene_get_active", "vg_scene_energy", "vg_scene_set_wall_rotation",
            "vg_batch_info")
@@ -73,6 +73,8 @@ _lib = None
 
 # Layout of vg_system_energy's output (DOWorld::CalculateSystemEnergy,
 # DOWorld.cpp:481-632, the values it stores in SystemParameter).
+PRECISION_FP64, PRECISION_MIXED = 0, 1   # vg_set_precision
+
 ENERGY_FIELDS = ("potential", "translation", "rotation", "momentum_avg", "angular_momentum_avg", "momentum_norm",
                  "angular_momentum_norm", "velocity_max", "velocity_min", "angular_velocity_max",
                  "angular_velocity_min", "mobile")
@@ -118,6 +120,8 @@ def lib():
         L.vg_set_skin.argtypes = [vp, C.c_double]
         L.vg_get_skin.restype = C.c_double
         L.vg_get_skin.argtypes = [vp]
+        L.vg_set_precision.argtypes = [vp, C.c_int]
+        L.vg_get_precision.argtypes = [vp]
         L.vg_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_int),
                                C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
         L.vg_copy_bench.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
@@ -275,6 +279,14 @@ class GpuSimulation:
     def skin(self) -> float:
         return self.L.vg_get_skin(self.h)
 
+    def set_precision(self, mode: int):
+        """PRECISION_FP64 (the reference's arithmetic) or PRECISION_MIXED."""
+        self._check(self.L.vg_set_precision(self.h, int(mode)))
+
+    @property
+    def precision(self) -> int:
+        return self.L.vg_get_precision(self.h)
+
     def kernel_timing(self, max_launches: int):
         self._check(self.L.vg_kernel_timing(self.h, int(max_launches)))
 
@@ -383,6 +395,9 @@ class GpuBatch:
 
     def set_skin(self, skin: float):
         self._check(self.L.vg_set_skin(self.h, float(skin)))
+
+    def set_precision(self, mode: int):
+        self._check(self.L.vg_set_precision(self.h, int(mode)))
 
     def info(self):
         sp, on, q, st = C.c_longlong(), C.c_int(), C.c_int(), C.c_ulonglong()

@@ -31,7 +31,8 @@ def load_golden(name):
     return z, wa
 
 
-GOLDEN_CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_nbs", "drum_kin", "box_assign"]
+GOLDEN_CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_nbs", "drum_kin", "box_assign",
+                "box_piston"]
 
 
 def step_golden(sim, z, step):

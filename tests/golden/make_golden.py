@@ -110,7 +110,33 @@ def make_cases():
         st = np.concatenate([p, v, ox, oz, av], 1)
         rows[step - 1][sel] = st[sel]
     cases.append(("box_assign", wa, 0, nsteps, [0, 2, 3, 4, 7, 8, 9, 21, 22, 40], (ids, rows)))
+    # a constrained piston (DOConstrainedQuasiPlate): the top plate of a small
+    # box rests on the top layer of spheres and presses down under its weight
+    # and an external force, moving along its Oz
+    w = scenes.box_scene(64, seed=15, law="ISwBSDBF", dt=2e-5, steps=400, kn=2e3, e=0.6, mu=0.4, side=0.0125,
+                         spacing=2.02, jitter=0.05, vmax=0.05, offset=3.0, omega_max=5.0)
+    cases.append(("box_piston", piston_world(w).to_arrays(up_iact=7, record=1000), 0, 150,
+                  [0, 1, 2, 6, 7, 8, 50, 150], None))
     return cases
+
+
+def piston_world(w, density=100.0, force=-0.02, overlap=0.02):
+    """The box's top plate as a constrained piston touching the top spheres."""
+    from vedo_amd.world import DOModel, DOStatus
+    sph = [st for st in w.statuses if st.name == "Sphere"]
+    r = [m for m in w.models if m.name == "Sphere"][0].attributes[0]
+    plate = [m for m in w.models if m.name == "Plate"][0]
+    thick = plate.attributes[1]
+    ztop = max(st.position[2] for st in sph)
+    piston = DOModel("Piston", plate.group, "constrained", "global", density, 1.0, (0.0, 0.0, force),
+                     "QuasiPlate", plate.attributes)
+    tops = [k for k, st in enumerate(w.statuses) if st.name == "Plate" and tuple(st.orientation_z) == (0, 0, 1)]
+    k = max(tops, key=lambda k: w.statuses[k].position[2])
+    old = w.statuses[k]
+    w.statuses[k] = DOStatus("Piston", (old.position[0], old.position[1], ztop + r + 0.5 * thick - overlap * r),
+                             (0.0, 0.0, 0.0), old.orientation_x, old.orientation_z, (0.0, 0.0, 0.0))
+    w.models.append(piston)
+    return w
 
 
 def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE):

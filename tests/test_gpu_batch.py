@@ -94,3 +94,23 @@ def test_batch_rejects_mismatched_up_iact_and_capacity():
     with pytest.raises(GpuError, match="capacity"):
         b.admit(_box(64, 32, "ISwBSDBF"), 10)
     b.close()
+
+
+def test_batch_with_constrained_piston_equals_alone():
+    # a constrained piston scene (its wall moves under its contacts' impulses,
+    # summed in pair order) among scenes with fixed walls
+    from conftest import load_golden
+    from vedo_amd.gpu import SCENE_DONE, GpuBatch
+    _, piston = load_golden("box_piston")
+    cases = [(_box(64, 21, "ISwBSDBF"), 35), (piston, 42), (_box(27, 22, "ISwH", dt=1e-5), 28)]
+    b = GpuBatch(max_spheres=1000, max_walls=64, max_scenes=4, up_iact=UP)
+    ids = [b.admit(wa, steps) for wa, steps in cases]
+    for _ in range(8):
+        b.step(UP)
+    for (wa, steps), sid in zip(cases, ids):
+        state, done, _ = b.status(sid)
+        assert state == SCENE_DONE and done == steps, (sid, state, done)
+        got, ref = b.state(sid), _alone(wa, steps)
+        ok = _same(got, ref, fields=("pos", "vel"))
+        print(f"\n  scene {sid}: {wa.n_do} DOs (walls included), bitwise equal alone and batched: {ok}", flush=True)
+        assert ok

@@ -43,6 +43,7 @@ struct SceneHost {
     int w_begin = 0, law_off = 0, n_groups = 0, model_off = 0, n_models = 0;
     double dt = 0, t = 0, t_stop = 0, g[3] = {0, 0, 0}, sf = 1.1, skin = 0, bond_reach = 0, rmin = 0;
     bool sd_only = true, single_group = true, no_bond = true;
+    bool has_constrained = false;       // a wall moves under its contacts (DOConstrained*)
     long long steps_done = 0, steps_target = -1;
     int seg_begin = 0, seg_end = 0;     // slots of the spheres in the current layout
     // initial state until the scene is placed
@@ -132,6 +133,19 @@ struct vg_scene {
     long long snap_count = 0;
     double *snap_wimp = nullptr;
     bool snap_valid = false;
+    // Constrained walls: each one's candidate pairs in IactPairTab order,
+    // laid down at every rebuild; a step writes each pair's master impulse
+    // with the step's stamp, and k_cw_sum adds a wall's pairs in that order.
+    bool any_constrained = false;       // a placed scene has a constrained wall
+    long long cw_cap = 0, cw_n = 0;     // pair slots allocated / in use
+    double *cw_val = nullptr;           // [cw_cap][3]
+    unsigned *cw_stamp = nullptr;       // [cw_cap]
+    unsigned long long *cw_key[2] = {nullptr, nullptr};
+    unsigned *cw_vidx[2] = {nullptr, nullptr};
+    int *cw_w = nullptr, *cw_p = nullptr;    // [2][ld] a sphere's constrained walls, their pair slots
+    unsigned *cw_cnt = nullptr;         // [ld + 1] per-sphere pair counts, then offsets
+    int *cw_beg = nullptr, *cw_end = nullptr;   // [wcap] pair range of each wall slot
+    double *cw_imp = nullptr;           // [wcap][3] each constrained wall's impulse of the step
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
@@ -251,6 +265,10 @@ static void update_batch_flags(vg_scene *h)
         h->single_group = h->single_group && S.single_group;
         h->no_bond = h->no_bond && S.no_bond;
     }
+    h->any_constrained = false;
+    for (const SceneHost &S : h->sc)
+        if (S.state == VG_SCENE_RUNNING || S.state == VG_SCENE_FINISHED || S.state == VG_SCENE_QUEUED)
+            h->any_constrained = h->any_constrained || S.has_constrained;
 }
 
 // ---------------------------------------------------------------------------
@@ -297,8 +315,13 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
             S.do_to_sphere[i] = (int)S.sphere_ids.size();
             S.sphere_ids.push_back((unsigned)i);
         } else if (M.shape == VG_QUASIPLATE || M.shape == VG_QUASICYLINDER) {
-            if (M.scope != VG_GLOBAL || (M.behavior != VG_FIXED && M.behavior != VG_ORBITAL)) {
-                err = "walls must be global and fixed or orbital";
+            if (M.scope != VG_GLOBAL ||
+                (M.behavior != VG_FIXED && M.behavior != VG_ORBITAL && M.behavior != VG_CONSTRAINED)) {
+                err = "walls must be global and fixed, orbital or constrained";
+                return VG_EUNSUPPORTED;
+            }
+            if (M.behavior == VG_CONSTRAINED && !(M.density * M.density_factor > 0.0)) {
+                err = "a constrained wall needs a positive mass (density and density factor)";
                 return VG_EUNSUPPORTED;
             }
             S.do_to_wall[i] = (int)S.wall_ids.size();   // wall order; the slot offset is added at admission
@@ -379,6 +402,16 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
                                            : std::sqrt(M.attr[0] * M.attr[0] + 0.25 * M.attr[1] * M.attr[1]);
         W.path = 0.0;
         W.shape = M.shape; W.group = M.group; W.behavior = M.behavior; W.kin = 0;
+        // DOModel::Analysis (DOModel.cpp:56-84): the constrained walls' sudo-mass;
+        // fixed and orbital walls carry none (DiscreteObject of DOFixed*/DOOrbital*)
+        W.constrained = M.behavior == VG_CONSTRAINED;
+        if (W.constrained) {
+            const double vol = M.shape == VG_QUASIPLATE ? M.attr[0] * M.attr[1] * M.attr[2]
+                                                         : PI * M.attr[0] * M.attr[0] * M.attr[1];
+            W.msudo = M.density_factor * (M.density * vol);
+        }
+        W.fext[0] = M.fext[0]; W.fext[1] = M.fext[1]; W.fext[2] = M.fext[2];
+        S.has_constrained = S.has_constrained || W.constrained;
     }
     (void)h;
     return 0;
@@ -561,6 +594,65 @@ static int place_queued(vg_scene *h)
 // LeapConsultant.h:57-514) for every placed scene, followed by
 // SimMediator::Initiate's history restore.  Finished scenes leave the device
 // here (their final state is kept on the host).
+// Constrained walls: the pairs of each constrained wall in IactPairTab order
+// (see k_cw_emit), the pair slot of each (sphere, wall), and the pair range
+// of each wall slot, for the lists the rebuild has just made.
+static int build_cw_pairs(vg_scene *h, const Geo &G, int n)
+{
+    hipStream_t st = h->stream;
+    const int ld = h->ld;
+    if (h->wcap > (1 << 13)) return fail(h, VG_EUNSUPPORTED, "constrained walls need at most 8192 wall slots");
+    if (!h->cw_w) {
+        if (dalloc(h, &h->cw_w, (size_t)2 * ld) || dalloc(h, &h->cw_p, (size_t)2 * ld) ||
+            dalloc(h, &h->cw_cnt, (size_t)2 * (ld + 1)))
+            return VG_ENOMEM;
+    }
+    unsigned *cnt = h->cw_cnt, *off = h->cw_cnt + (ld + 1);
+    hipLaunchKernelGGL(k_cw_count, dim3(nblk(n + 1, 256)), dim3(256), 0, st, n, ld, h->ko,
+                       (const unsigned *)h->ocount, (const unsigned *)h->onbr, (const WallDev *)h->walls, cnt,
+                       h->flags);
+    size_t need = 0;
+    (void)rocprim::exclusive_scan(nullptr, need, cnt, off, 0u, (size_t)n + 1, rocprim::plus<unsigned>(), st);
+    if (ensure_tmp(h, need)) return VG_EHIP;
+    HIPCHK(rocprim::exclusive_scan(h->tmp, need, cnt, off, 0u, (size_t)n + 1, rocprim::plus<unsigned>(), st));
+    unsigned np = 0;
+    HIPCHK(hipMemcpyAsync(&np, off + n, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if ((long long)np > h->cw_cap) {
+        auto F = [](void *p) {
+            if (p) (void)hipFree(p);
+        };
+        F(h->cw_val); F(h->cw_stamp); F(h->cw_key[0]); F(h->cw_key[1]); F(h->cw_vidx[0]); F(h->cw_vidx[1]);
+        h->cw_val = nullptr; h->cw_stamp = nullptr;
+        h->cw_key[0] = h->cw_key[1] = nullptr; h->cw_vidx[0] = h->cw_vidx[1] = nullptr;
+        h->cw_cap = (long long)np + np / 2 + 1024;
+        const size_t c = (size_t)h->cw_cap;
+        if (dalloc(h, &h->cw_val, 3 * c) || dalloc(h, &h->cw_stamp, c) || dalloc(h, &h->cw_key[0], c) ||
+            dalloc(h, &h->cw_key[1], c) || dalloc(h, &h->cw_vidx[0], c) || dalloc(h, &h->cw_vidx[1], c))
+            return VG_ENOMEM;
+    }
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(2 * ld, 256)), dim3(256), 0, st, h->cw_w, 2ll * ld, -1);
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(h->wcap, 256)), dim3(256), 0, st, h->cw_beg, (long long)h->wcap, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(h->wcap, 256)), dim3(256), 0, st, h->cw_end, (long long)h->wcap, 0);
+    if (np > 0) {
+        hipLaunchKernelGGL(k_cw_emit, dim3(nblk(n, 256)), dim3(256), 0, st, n, ld, h->ko, (const unsigned *)h->ocount,
+                           (const unsigned *)h->onbr, (const WallDev *)h->walls, G, (const ModelDev *)h->models,
+                           (const SceneDev *)h->scenes, (const unsigned *)off, h->cw_key[0], h->cw_vidx[0]);
+        need = 0;
+        (void)rocprim::radix_sort_pairs(nullptr, need, h->cw_key[0], h->cw_key[1], h->cw_vidx[0], h->cw_vidx[1],
+                                        (size_t)np, 0, 64, st);
+        if (ensure_tmp(h, need)) return VG_EHIP;
+        HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, h->cw_key[0], h->cw_key[1], h->cw_vidx[0], h->cw_vidx[1],
+                                         (size_t)np, 0, 64, st));
+        hipLaunchKernelGGL(k_cw_finalize, dim3(nblk(np, 256)), dim3(256), 0, st, (long long)np,
+                           (const unsigned long long *)h->cw_key[1], (const unsigned *)h->cw_vidx[1], ld, h->cw_w,
+                           h->cw_p, h->cw_stamp, h->cw_beg, h->cw_end);
+    }
+    h->cw_n = np;
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 static int rebuild(vg_scene *h)
 {
     hipStream_t st = h->stream;
@@ -833,6 +925,10 @@ static int rebuild(vg_scene *h)
         ia.ko = h->ko; ia.scenes = h->scenes; ia.models = h->models; ia.walls = h->walls; ia.flags = h->flags;
         hipLaunchKernelGGL(k_inner_build, dim3(nblk(n, 128)), dim3(128), (size_t)h->ki * 128 * sizeof(unsigned), st,
                            ia);
+        if (h->any_constrained) {
+            const int rc = build_cw_pairs(h, G1w, n);
+            if (rc) return rc;
+        }
     }
     h->n_iact = -1;   // counted on request (vg_num_interactions)
     // the fresh lists need no relist: clear the ring slot the next step reads
@@ -923,6 +1019,8 @@ static void free_scene(vg_scene *h)
     F(h->contacts); F(h->red); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->qidx); F(h->segs);
     F(h->qval); F(h->safe); F(h->lin); F(h->cstart); F(h->cend); F(h->ocount); F(h->onbr); F(h->tmp); F(h->rec);
     F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->sid[0]); F(h->sid[1]); F(h->fetch); F(h->ovf);
+    F(h->cw_val); F(h->cw_stamp); F(h->cw_key[0]); F(h->cw_key[1]); F(h->cw_vidx[0]); F(h->cw_vidx[1]);
+    F(h->cw_w); F(h->cw_p); F(h->cw_cnt); F(h->cw_beg); F(h->cw_end); F(h->cw_imp);
     for (auto e : h->kev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -962,6 +1060,8 @@ static int alloc_batch(vg_scene *h, int ld, int wcap, int scap, int ki, bool rec
     rc |= dalloc(h, &h->scenes, scap) | dalloc(h, &h->stime, 2 * (size_t)scap) | dalloc(h, &h->rflags, 3 * (size_t)scap);
     rc |= dalloc(h, &h->walls, wcap) | dalloc(h, &h->walls_ref, wcap) | dalloc(h, &h->wkin, wcap);
     rc |= dalloc(h, &h->assigned, (size_t)15 * wcap);
+    rc |= dalloc(h, &h->cw_beg, std::max(wcap, 1)) | dalloc(h, &h->cw_end, std::max(wcap, 1)) |
+          dalloc(h, &h->cw_imp, (size_t)3 * std::max(wcap, 1));
     rc |= dalloc(h, &h->flags, 8) | dalloc(h, &h->relists, 1) | dalloc(h, &h->red, (size_t)8 * scap);
     rc |= dalloc(h, &h->inv, ld) | dalloc(h, &h->perm, ld) | dalloc(h, &h->svals, ld);
     rc |= dalloc(h, &h->keys, ld) | dalloc(h, &h->skeys, ld);
@@ -1086,6 +1186,8 @@ static StepArgs step_args(vg_scene *h)
     a.rec_imp = h->rec_imp; a.rec = h->rec;
     a.single_group = h->single_group;
     a.no_bond = h->no_bond;
+    a.cw_w = h->cw_w; a.cw_p = h->cw_p; a.cw_val = h->cw_val; a.cw_stamp = h->cw_stamp;
+    a.stamp = (unsigned)(h->step_count & 0x7fffffffull);
     return a;
 }
 
@@ -1188,11 +1290,15 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
         h->kev_used++;
     }
     h->last_launches++;
+    if (h->nw_used && h->any_constrained)
+        hipLaunchKernelGGL(k_cw_sum, dim3(nblk(h->nw_used, 64)), dim3(64), 0, h->stream, (const WallDev *)h->walls,
+                           h->nw_used, (const SceneDev *)h->scenes, (const int *)h->cw_beg, (const int *)h->cw_end,
+                           (const double *)h->cw_val, (const unsigned *)h->cw_stamp, a.stamp, h->cw_imp);
     if (h->nw_used)
         hipLaunchKernelGGL(k_walls, dim3(nblk(h->nw_used, 64)), dim3(64), 0, h->stream, h->walls, h->nw_used,
                            (const SceneDev *)h->scenes, h->stime, (const double *)h->assigned,
-                           (const WallDev *)h->walls_ref, (const WallKin *)h->wkin, h->rflags,
-                           (int)(h->step_count % 6));
+                           (const WallDev *)h->walls_ref, (const WallKin *)h->wkin, (const double *)h->cw_imp,
+                           h->rflags, (int)(h->step_count % 6));
     h->cur ^= 1;
     h->step_count++;
     h->round++;
@@ -1257,6 +1363,8 @@ extern "C" int vg_sync(vg_scene *h)
     HIPCHK(hipMemcpy(flags, h->flags, sizeof(flags), hipMemcpyDeviceToHost));
     if (flags[3] & ERR_INNER_CAP)
         return fail(h, VG_ECAPACITY, "neighbour list capacity exceeded (set VG_NEIGHBOUR_CAPACITY)");
+    if (flags[3] & ERR_CW)
+        return fail(h, VG_EUNSUPPORTED, "a sphere is a candidate of more than two constrained walls");
     if (h->timing) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->last_ms = ms;

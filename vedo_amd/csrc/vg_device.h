@@ -13,8 +13,8 @@
 // spheres is one contiguous 256-byte run:
 //   hdr  u8  [LD]          count (7 bits) | current half (bit 7)
 //   ent  u32 [2][KI][LD]   partner (26 bits: sphere slot, or LD + wall slot) | status flags
-//   hist double4 [2][KI][LD] tangential force history xyz + Kn / initial velocity
-//        (one 32-byte sector per entry: divergent contacts touch one sector each)
+//   hs   f64 [2][KI][LD][3] tangential force history xyz (24 bytes per entry)
+//   ha   f64 [2][KI][LD]    Kn (ISwH) / initial velocity (ISwHertz)
 // Every sphere keeps its own copy of each pair's ImpactStatus; the two copies
 // are updated by the same instruction sequence from the same inputs, so they
 // stay bit-identical and no thread ever reads another thread's history.
@@ -56,9 +56,12 @@ struct WallDev {
     double range;        // DOModel::Analysis range (global scope)
     double reach;        // distance from the centre to the farthest point of the wall
     double path;         // bound on the displacement of the wall's contact rectangle / axis segment since the last relist
+    double msudo;        // sudo-mass: DensityFactor * mass for a constrained wall, 0 for fixed / orbital
+    double fext[3];      // DOModel external force (AddConstrainedImpact)
     int shape, group;
     int behavior, kin;   // kin: 0 none, 1 host-assigned, 2 device rotation
-    int scene, pad;      // scene slot of the batch
+    int scene;           // scene slot of the batch
+    int constrained;     // DOConstrained* response (moves along Oz under its impulse)
 };
 
 struct Contact {

@@ -79,12 +79,18 @@ struct StepArgs {
     int step;
     int single_group;        // every scene's spheres share one group: no partner group lookup
     int no_bond;             // no bonded law: surely separated pairs skip the contact geometry
+    // constrained walls: a sphere's constrained walls and their pair slots [2][LD],
+    // each pair's master impulse and the step it was solved in
+    const int *cw_w, *cw_p;
+    double *cw_val;
+    unsigned *cw_stamp;
+    unsigned stamp;
     // record path
     double *rec_imp;         // [6][LD] last impulse / angular impulse per sphere
     double *rec;             // [KI][13][LD] point, dir, to_master, ang_to_master, overlap
 };
 
-enum { ERR_INNER_CAP = 1, ERR_OUTER_CAP = 2 };
+enum { ERR_INNER_CAP = 1, ERR_OUTER_CAP = 2, ERR_CW = 4 };
 
 // Kinematic prescription of a wall: rigid rotation about axis (c, a).
 struct WallKin {
@@ -129,7 +135,16 @@ struct ActiveOut {
 
 template <bool REC, bool SD, bool MIX> __global__ void k_step(StepArgs a);
 __global__ void k_walls(WallDev *walls, int nw, const SceneDev *scenes, double *stime, const double *assigned,
-                        const WallDev *ref, const WallKin *kin, int *rflags, int step);
+                        const WallDev *ref, const WallKin *kin, const double *cw_imp, int *rflags, int step);
+__global__ void k_cw_count(int n, int ld, int ko, const unsigned *cnt, const unsigned *nbr, const WallDev *walls,
+                           unsigned *out, int *flags);
+__global__ void k_cw_emit(int n, int ld, int ko, const unsigned *cnt, const unsigned *nbr, const WallDev *walls,
+                          Geo g, const ModelDev *models, const SceneDev *scenes, const unsigned *off,
+                          unsigned long long *key, unsigned *val);
+__global__ void k_cw_finalize(long long np, const unsigned long long *key, const unsigned *val, int ld, int *cw_w,
+                              int *cw_p, unsigned *stamp, int *beg, int *end);
+__global__ void k_cw_sum(const WallDev *walls, int nw, const SceneDev *scenes, const int *beg, const int *end,
+                         const double *val, const unsigned *stamp, unsigned cur, double *imp);
 __global__ void k_wall_path_reset(WallDev *walls, int nw);
 __global__ void k_ring_clear(int *rflags, int scap, int r);
 __global__ void k_bounds(Geo g, int n, const SceneDev *scenes, const ModelDev *models, unsigned long long *red,

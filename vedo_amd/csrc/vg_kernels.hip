@@ -301,13 +301,28 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             if (MIX) {
                 ContactM cm;
                 to_mixed(c, xi, w.pos, cm);
-                solve_mixed<REC, SD>(law, mi, 0.0, xi, vi, wi, w.vel, w.av, cm, S.dt, hs, o, r);
+                solve_mixed<REC, SD>(law, mi, w.msudo, xi, vi, wi, w.vel, w.av, cm, S.dt, hs, o, r);
             } else {
-                solve<REC, SD>(law, mi, 0.0, xi, vi, wi, w.pos, w.vel, w.av, c, S.dt, hs, o, r);
+                solve<REC, SD>(law, mi, w.msudo, xi, vi, wi, w.pos, w.vel, w.av, c, S.dt, hs, o, r);
             }
             if (o.added) {
                 I = I + o.imp;
                 if (!o.linear_only) L = L + o.ang_a;
+                if (w.constrained) {
+                    // the wall's share (DiscreteObject::AddImpact on the master), added in pair order by k_cw_sum
+                    const int ws = (int)(j - a.ld);
+                    int p = -1;
+                    if (a.cw_w[i] == ws) p = a.cw_p[i];
+                    else if (a.cw_w[a.ld + i] == ws) p = a.cw_p[a.ld + i];
+                    if (p >= 0) {
+                        const d3 m = o.linear_only ? -o.imp : smul(-1.0, o.imp);
+                        double *v = a.cw_val + 3 * (size_t)p;
+                        v[0] = m.x; v[1] = m.y; v[2] = m.z;
+                        a.cw_stamp[p] = a.stamp;
+                    } else {
+                        atomicOr(&a.flags[3], ERR_CW);
+                    }
+                }
             }
         }
         if (hs.f != (ent & ~PARTNER_MASK)) nt_store(j | hs.f, a.ell.ent + ei);
@@ -373,7 +388,7 @@ template __global__ void k_step<true, true, true>(StepArgs);
 // (SimMediator.cpp:1121-1163).  kin 1: host-assigned status in `assigned`;
 // kin 2: rigid rotation about (c, a) at omega from the reference status.
 __global__ void k_walls(WallDev *walls, int nw, const SceneDev *scenes, double *stime, const double *assigned,
-                        const WallDev *ref, const WallKin *kin, int *rflags, int step)
+                        const WallDev *ref, const WallKin *kin, const double *cw_imp, int *rflags, int step)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nw) return;
@@ -413,6 +428,18 @@ __global__ void k_walls(WallDev *walls, int nw, const SceneDev *scenes, double *
         W.pos = W.pos + dp;
         W.ox = rotate_around(W.ox, dw);
         W.oz = rotate_around(W.oz, dw);
+    } else if (W.constrained) {
+        // DOConstrainedQuasiPlate/Cylinder::Response (DOConstrainedQuasiPlate.cpp:18-35):
+        // the impulse's component along Oz accelerates the wall; it turns at its own rate
+        const d3 I = mk(cw_imp[3 * (size_t)w], cw_imp[3 * (size_t)w + 1], cw_imp[3 * (size_t)w + 2]);
+        const d3 V = W.vel, Oz = W.oz;
+        const d3 dv = smul(1.0 / W.msudo, smul(dot(I, Oz), dir(Oz)));
+        const d3 dp = smul(dt, V + smul(0.5, dv));
+        const d3 dw = muls(W.av, dt);
+        W.pos = W.pos + dp;
+        W.ox = rotate_around(W.ox, dw);
+        W.oz = rotate_around(Oz, dw);
+        W.vel = V + dv;
     }
     // Relist guard for moving walls.  A sphere-wall depth is the sphere's
     // distance to the plate's mid-plane rectangle (the cylinder's axis
@@ -845,6 +872,108 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
     }
     a.ell.hdr[s] = (unsigned char)q;
     atomicMax(&a.flags[4], q);
+}
+
+// ---------------------------------------------------------------------------
+// Constrained walls
+// ---------------------------------------------------------------------------
+// A constrained wall's impulse is the sum of its pairs' master impulses in
+// IactPairTab order (IactContainer::CalculateImpact, IactContainer.cpp:33-37,
+// adding with DiscreteObject::AddImpact), then the field and external
+// impulses (DOContainer.cpp:20-50).  Its pairs in that order are the spheres
+// that list it as a candidate, ordered by the comparator key of their cell at
+// the rebuild and, within a cell, by their in-cell (DO-id) order, which is
+// the memory order: k_cw_count / k_cw_emit and a stable sort lay them down.
+
+// Comparator key of a sphere's cell, 17 bits per axis in priority order.
+__device__ inline unsigned long long cmp_key17(const GridDev &G, d3 x)
+{
+    unsigned c[3];
+    cell_of(G, x, c[0], c[1], c[2]);
+    return ((unsigned long long)c[G.perm[0]] << 34) | ((unsigned long long)c[G.perm[1]] << 17) |
+           (unsigned long long)c[G.perm[2]];
+}
+
+__global__ void k_cw_count(int n, int ld, int ko, const unsigned *cnt, const unsigned *nbr, const WallDev *walls,
+                           unsigned *out, int *flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) {
+        out[n] = 0;
+        return;
+    }
+    const unsigned e = cnt[i] < (unsigned)ko ? cnt[i] : (unsigned)ko;
+    unsigned c = 0;
+    for (unsigned t = 0; t < e; ++t) {
+        const unsigned j = nbr[(size_t)t * ld + i];
+        if (j >= (unsigned)ld && walls[j - ld].constrained) ++c;
+    }
+    if (c > 2) {   // a sphere near more than two constrained walls
+        atomicOr(&flags[3], ERR_CW);
+        c = 2;
+    }
+    out[i] = c;
+}
+
+__global__ void k_cw_emit(int n, int ld, int ko, const unsigned *cnt, const unsigned *nbr, const WallDev *walls,
+                          Geo g, const ModelDev *models, const SceneDev *scenes, const unsigned *off,
+                          unsigned long long *key, unsigned *val)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned b = off[i], lim = off[i + 1];
+    if (b == lim) return;
+    const double4 p = g.pos[i];
+    const unsigned long long ck = cmp_key17(scenes[models[g.mdl[i]].scene].G, mk(p.x, p.y, p.z));
+    const unsigned e = cnt[i] < (unsigned)ko ? cnt[i] : (unsigned)ko;
+    unsigned q = 0;
+    for (unsigned t = 0; t < e && b + q < lim; ++t) {
+        const unsigned j = nbr[(size_t)t * ld + i];
+        if (j >= (unsigned)ld && walls[j - ld].constrained) {
+            key[b + q] = ((unsigned long long)(j - ld) << 51) | ck;
+            val[b + q] = ((unsigned)i << 1) | q;
+            ++q;
+        }
+    }
+}
+
+__global__ void k_cw_finalize(long long np, const unsigned long long *key, const unsigned *val, int ld, int *cw_w,
+                              int *cw_p, unsigned *stamp, int *beg, int *end)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const unsigned v = val[p];
+    const int i = (int)(v >> 1), q = (int)(v & 1u);
+    const int w = (int)(key[p] >> 51);
+    cw_w[(size_t)q * ld + i] = w;
+    cw_p[(size_t)q * ld + i] = (int)p;
+    stamp[p] = 0xffffffffu;
+    if (p == 0 || (int)(key[p - 1] >> 51) != w) beg[w] = (int)p;
+    if (p == np - 1 || (int)(key[p + 1] >> 51) != w) end[w] = (int)p + 1;
+}
+
+// Each constrained wall's impulse of the step: its pairs' master impulses
+// (those solved in this step carry its stamp) in pair order, then the field
+// impulse g dt m* and the external impulse dt F (DOContainer.cpp:20-50).
+__global__ void k_cw_sum(const WallDev *walls, int nw, const SceneDev *scenes, const int *beg, const int *end,
+                         const double *val, const unsigned *stamp, unsigned cur, double *imp)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const WallDev &W = walls[w];
+    if (W.scene < 0 || !W.constrained) return;
+    const SceneDev &S = scenes[W.scene];
+    if (S.active <= 0) return;
+    d3 I = mk(0, 0, 0);
+    for (int p = beg[w]; p < end[w]; ++p)
+        if (stamp[p] == cur) I = I + mk(val[3 * (size_t)p], val[3 * (size_t)p + 1], val[3 * (size_t)p + 2]);
+    const d3 gdt = muls(mk(S.gx, S.gy, S.gz), S.dt);
+    I = I + muls(gdt, W.msudo);
+    I = I + smul(S.dt, mk(W.fext[0], W.fext[1], W.fext[2]));
+    imp[3 * (size_t)w] = I.x;
+    imp[3 * (size_t)w + 1] = I.y;
+    imp[3 * (size_t)w + 2] = I.z;
 }
 
 // Fresh lists for newly placed spheres (all statuses ImpactStatus()).

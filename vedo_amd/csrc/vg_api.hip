@@ -1690,10 +1690,58 @@ extern "C" int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *
 
 extern "C" int vg_set_history(vg_scene *h, int n, const unsigned *lo, const unsigned *hi, const vg_status *st)
 {
-    (void)lo; (void)hi; (void)st;
-    if (!h) return VG_EINVAL;
+    if (!h || n < 0 || (n > 0 && (!lo || !hi || !st))) return VG_EINVAL;
+    const SceneHost *S0 = scene0(h);
+    if (!S0) return fail(h, VG_EINVAL, "vg_set_history applies to a one-scene handle");
+    if (h->step_count != 0)
+        return fail(h, VG_EINVAL, "histories are restored before the first step (SimMediator::Initiate)");
     if (n == 0) return 0;
-    return fail(h, VG_EUNSUPPORTED, "history restore is not implemented yet");
+    // records by key lo << 32 | hi (IactRecordTab is keyed by the DO-id pair)
+    std::vector<std::pair<unsigned long long, int>> order((size_t)n);
+    for (int k = 0; k < n; ++k) {
+        if (lo[k] >= (unsigned)S0->n_do || hi[k] >= (unsigned)S0->n_do || lo[k] == hi[k])
+            return fail(h, VG_EINVAL, "a record names an unknown DO or pairs a DO with itself");
+        const unsigned a = std::min(lo[k], hi[k]), b = std::max(lo[k], hi[k]);
+        order[(size_t)k] = {((unsigned long long)a << 32) | b, k};
+    }
+    std::stable_sort(order.begin(), order.end());
+    std::vector<unsigned long long> keys((size_t)n);
+    std::vector<RestoreRec> recs((size_t)n);
+    for (int q = 0; q < n; ++q) {
+        if (q > 0 && order[(size_t)q].first == order[(size_t)q - 1].first)
+            return fail(h, VG_EINVAL, "two records for one pair");
+        const vg_status &s = st[order[(size_t)q].second];
+        RestoreRec &r = recs[(size_t)q];
+        r = RestoreRec();
+        r.flags = (s.contact ? F_CONTACT : 0u) | (s.last_contact ? F_LASTCONTACT : 0u) | (s.bond ? F_BOND : 0u) |
+                  (s.last_bond ? F_LASTBOND : 0u);
+        r.kn = s.kn;
+        r.init_vel = s.init_vel;
+        for (int c = 0; c < 3; ++c) r.shear[c] = s.shear[c];
+        keys[(size_t)q] = order[(size_t)q].first;
+    }
+    unsigned long long *d_keys = nullptr;
+    RestoreRec *d_recs = nullptr;
+    unsigned *d_sid = nullptr, *d_wid = nullptr;
+    int rc = 0;
+    if (dalloc(h, &d_keys, (size_t)n) || dalloc(h, &d_recs, (size_t)n) || id_maps(h, 0, &d_sid, &d_wid)) rc = VG_ENOMEM;
+    if (!rc && (hipMemcpy(d_keys, keys.data(), sizeof(unsigned long long) * n, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d_recs, recs.data(), sizeof(RestoreRec) * n, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(h, VG_EHIP, "record upload failed");
+    if (!rc && S0->n > 0) {
+        hipLaunchKernelGGL(k_restore_history, dim3(nblk(S0->n, 128)), dim3(128), 0, h->stream, h->geo[h->cur],
+                           h->ell[h->ecur], S0->seg_begin, S0->seg_end, h->ld, h->ki, (const unsigned *)d_sid,
+                           (const unsigned *)d_wid, (const unsigned long long *)d_keys, (const RestoreRec *)d_recs,
+                           (long long)n, (const ModelDev *)h->models, (const SceneDev *)h->scenes,
+                           (const LawDev *)h->laws, (const WallDev *)h->walls);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess)
+            rc = fail(h, VG_EHIP, "history restore kernel failed");
+    }
+    for (void *p : {(void *)d_keys, (void *)d_recs, (void *)d_sid, (void *)d_wid})
+        if (p) (void)hipFree(p);
+    h->snap_valid = false;
+    h->ent_valid = false;
+    return rc;
 }
 
 // ---------------------------------------------------------------------------

@@ -162,6 +162,11 @@ __global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int b, in
                              unsigned long long *out);
 __global__ void k_inner_build(InnerArgs a);
 __global__ void k_hdr_zero(unsigned char *hdr, int b, int e);
+struct RestoreRec;
+__global__ void k_restore_history(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *sphere_id,
+                                  const unsigned *wall_id, const unsigned long long *keys, const RestoreRec *recs,
+                                  long long nrec, const ModelDev *models, const SceneDev *scenes, const LawDev *laws,
+                                  const WallDev *walls);
 __global__ void k_scatter_state(Geo g, int b, int e, int ld, const double *rec_imp, double *out);
 __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *o_cnt,
                                 const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,

@@ -976,6 +976,61 @@ __global__ void k_cw_sum(const WallDev *walls, int nw, const SceneDev *scenes, c
     imp[3 * (size_t)w + 2] = I.z;
 }
 
+// IactRecordTab restore (SimMediator::Initiate, SimMediator.cpp:690-707, via
+// Consultant::RetrieveImpactStatus): every listed pair whose (lo, hi) DO ids
+// have a record takes the record's status, as a rebuild's history carry does
+// (flags, shear force, Kn / initial velocity; the pending Backup is skipped).
+// Records are sorted by key lo << 32 | hi.
+struct RestoreRec {
+    unsigned flags, pad;   // F_CONTACT | F_LASTCONTACT | F_BOND | F_LASTBOND
+    double kn, init_vel, shear[3];
+};
+
+__global__ void k_restore_history(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *sphere_id,
+                                  const unsigned *wall_id, const unsigned long long *keys, const RestoreRec *recs,
+                                  long long nrec, const ModelDev *models, const SceneDev *scenes, const LawDev *laws,
+                                  const WallDev *walls)
+{
+    const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
+    const unsigned hdr = ell.hdr[i];
+    const int h = hdr >> 7, cnt = hdr & 127;
+    const ModelDev &mi = models[g.mdl[i]];
+    const SceneDev &S = scenes[mi.scene];
+    const unsigned id = sphere_id[g.oid[i]];
+    for (int k = 0; k < cnt; ++k) {
+        const size_t x = eidx(h, k, ki, ld, i);
+        const unsigned ent = ell.ent[x];
+        const unsigned j = ent & PARTNER_MASK;
+        unsigned pid;
+        int gj;
+        if (j < (unsigned)ld) {
+            pid = sphere_id[g.oid[j]];
+            gj = models[g.mdl[j]].group;
+        } else {
+            pid = wall_id[j - ld];
+            gj = walls[j - ld].group;
+        }
+        const unsigned lo = id < pid ? id : pid, hi = id < pid ? pid : id;
+        const unsigned long long key = ((unsigned long long)lo << 32) | hi;
+        long long l = 0, r = nrec;   // first record with key >= key
+        while (l < r) {
+            const long long m = (l + r) >> 1;
+            if (keys[m] < key) l = m + 1;
+            else r = m;
+        }
+        if (l >= nrec || keys[l] != key) continue;
+        const RestoreRec &R = recs[l];
+        const unsigned f = (R.flags & F_STATUS) | (ent & F_SLAVE) | F_NOBACKUP;
+        ell.ent[x] = j | f;
+        const int law = laws[S.law_off + mi.group * S.n_groups + gj].law;
+        Hist hs;
+        hs.s = (f & F_CONTACT) ? mk(R.shear[0], R.shear[1], R.shear[2]) : mk(0, 0, 0);
+        hs.aux = law == LAW_H ? R.kn : (law == LAW_HERTZ ? R.init_vel : 0.0);
+        store_hist(ell, h, k, ki, ld, i, hs);
+    }
+}
+
 // Fresh lists for newly placed spheres (all statuses ImpactStatus()).
 __global__ void k_hdr_zero(unsigned char *hdr, int b, int e)
 {

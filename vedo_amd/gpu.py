@@ -272,6 +272,14 @@ class GpuSimulation:
                                          st.ctypes.data_as(C.POINTER(VgStatus))))
         return lo[:n.value], hi[:n.value], st[:n.value]
 
+    def set_history(self, lo, hi, st):
+        """Restore IactRecordTab statuses before the first step (vg_set_history)."""
+        lo = np.ascontiguousarray(lo, np.uint32)
+        hi = np.ascontiguousarray(hi, np.uint32)
+        st = np.ascontiguousarray(st, STATUS_DTYPE)
+        self._check(self.L.vg_set_history(self.h, len(lo), _up(lo), _up(hi),
+                                          st.ctypes.data_as(C.POINTER(VgStatus))))
+
     def set_skin(self, skin: float):
         self._check(self.L.vg_set_skin(self.h, float(skin)))
 

@@ -43,7 +43,8 @@ class VoScene(C.Structure):
                 ("av", C.POINTER(C.c_double)),
                 ("dt", C.c_double), ("t_start", C.c_double), ("t_stop", C.c_double),
                 ("t_current", C.c_double), ("g", C.c_double * 3), ("safety_factor", C.c_double),
-                ("up_iact", C.c_uint), ("record", C.c_uint), ("consultant", C.c_int)]
+                ("up_iact", C.c_uint), ("record", C.c_uint), ("consultant", C.c_int),
+                ("zoi_lo", C.c_double * 3), ("zoi_hi", C.c_double * 3)]
 
 
 class VoContact(C.Structure):
@@ -81,8 +82,10 @@ def lib():
         L.vo_time.restype = C.c_double
         L.vo_time.argtypes = [C.c_void_p]
         L.vo_system_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
-        for f in ("vo_num_interactions", "vo_num_candidates", "vo_num_active"):
+        for f in ("vo_num_interactions", "vo_num_candidates", "vo_num_active", "vo_num_do"):
             getattr(L, f).argtypes = [C.c_void_p]
+        L.vo_get_ids.argtypes = [C.c_void_p, U]
+        L.vo_cleanup_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
         L.vo_get_interactions.argtypes = [C.c_void_p, U, U]
         L.vo_get_candidates.argtypes = [C.c_void_p, U, U]
         L.vo_get_active.argtypes = [C.c_void_p, U, U, C.POINTER(VoStatus)]
@@ -141,6 +144,9 @@ class OracleSim:
             sc.g[d] = float(wa.g[d])
         sc.safety_factor = wa.safety_factor
         sc.up_iact, sc.record, sc.consultant = wa.up_iact, wa.record, consultant
+        zlo, zhi = getattr(wa, "zoi_lo", np.zeros(3)), getattr(wa, "zoi_hi", np.zeros(3))
+        for d in range(3):
+            sc.zoi_lo[d], sc.zoi_hi[d] = float(zlo[d]), float(zhi[d])
         err = C.create_string_buffer(256)
         self.h = L.vo_create(C.byref(sc), err, 256)
         if not self.h:
@@ -168,7 +174,25 @@ class OracleSim:
             raise RuntimeError("oracle step failed")
         return r
 
+    @property
+    def n_do(self) -> int:
+        """Current number of DOs (the zone of interest erases DOs)."""
+        return self.L.vo_num_do(self.h)
+
+    def cleanup_stats(self):
+        """Counts of Consultant::CleanUp so far: erased, moved, from_dead, replaced."""
+        out = (C.c_longlong * 4)()
+        self.L.vo_cleanup_stats(self.h, out)
+        return dict(zip(("erased", "moved", "from_dead", "replaced"), list(out)))
+
+    def ids(self):
+        """The scene's original DO id of each current DO."""
+        out = np.zeros(self.n_do, np.uint32)
+        self.L.vo_get_ids(self.h, _up(out))
+        return out
+
     def state(self):
+        self.n = self.n_do
         out = {k: np.zeros((self.n, 3)) for k in ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")}
         self.L.vo_get_state(self.h, *(_dp(out[k]) for k in ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")))
         return out

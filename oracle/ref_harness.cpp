@@ -85,6 +85,7 @@ int main(int argc, char **argv)
     double sf = 1.1;
     unsigned up = 1, rec = 1;
     int cons = 0;
+    Vector3df zoi_lo, zoi_hi;   // equal: no zone of interest
     std::vector<std::string> groups;
     std::list<DOModel *> models;
     std::vector<std::string> model_names;
@@ -94,6 +95,7 @@ int main(int argc, char **argv)
         if (tok == "time") { t0 = rd(in); t1 = rd(in); dt = rd(in); tc = rd(in); }
         else if (tok == "gravity") g = rv(in);
         else if (tok == "safety") sf = rd(in);
+        else if (tok == "zoi") { zoi_lo = rv(in); zoi_hi = rv(in); }
         else if (tok == "cadence") in >> up >> rec >> cons;
         else if (tok == "groups") {
             int n; in >> n; groups.resize(n);
@@ -136,7 +138,7 @@ int main(int argc, char **argv)
         }
     }
     aries::Constants::Instance()->SetSafetyFactor(sf);
-    Boundary zoi, pbc;
+    Boundary zoi(zoi_lo, zoi_hi), pbc;
     SystemParameter *sp = new SystemParameter(t0, t1, dt, tc, g, zoi, pbc);
     unsigned ndo = (unsigned)statuses.size();
     sp->SetDONumber(ndo);

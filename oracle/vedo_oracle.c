@@ -433,6 +433,7 @@ typedef struct {
 
 typedef struct {
     unsigned lo, hi;
+    int order;   /* position among the keys ModifyPair moves (ties: the later wins) */
     vo_status st;
 } record;
 
@@ -459,6 +460,13 @@ struct vo_sim {
     /* IactRecordTab */
     record *rec;
     int n_rec;
+    /* zone of interest (Boundary, Boundary.cpp:79-129) and the scene's DO id of each current DO */
+    int zoi_on[3], zoi_active;
+    double zoi_lo[3], zoi_hi[3];
+    unsigned *orig_id;
+    /* CleanUp counts: DOs erased, records moved, moved records of an erased DO,
+       kept records replaced by a moved one */
+    long long cu_erased, cu_moved, cu_from_dead, cu_replaced;
 };
 
 static double model_range(const vo_model *m)
@@ -892,6 +900,109 @@ static int rebuild_records(vo_sim *s)
     return 0;
 }
 
+/* Boundary::InBoundary (Boundary.cpp:161-191) */
+static int in_zoi(const vo_sim *s, v3 p)
+{
+    const double c[3] = {p.x, p.y, p.z};
+    for (int d = 0; d < 3; ++d)
+        if (s->zoi_on[d] && (c[d] < s->zoi_lo[d] || c[d] > s->zoi_hi[d])) return 0;
+    return 1;
+}
+
+static int cmp_rec_pos(const void *pa, const void *pb)
+{
+    /* by key, then by position in the source table (the later assignment wins) */
+    const record *a = pa, *b = pb;
+    int c = cmp_rec(pa, pb);
+    if (c) return c;
+    return (a->order > b->order) - (a->order < b->order);
+}
+
+/* Consultant::CleanUp (Consultant.cpp:264-319): mobile DOs outside the zone
+ * of interest are erased; the remaining DO ids close up, and the
+ * IactRecordTab keys move with IactRecordTab::ModifyPair
+ * (IactRecordTab.cpp:41-69).  ModifyPair moves every key by the mapping
+ * id -> id + jump[id], jump[id] being minus the number of erased DOs up to
+ * and including id (unsigned arithmetic): records of erased DOs are not
+ * dropped but move to their predecessor's new id; moved keys are
+ * collected in a std::map (the later of two equal keys wins, in key order)
+ * and written over the table's entries with the same keys. */
+static int cleanup(vo_sim *s)
+{
+    if (!s->zoi_active) return 0;
+    const int n = s->n_do;
+    unsigned *jump = malloc(sizeof(unsigned) * (n ? n : 1));
+    char *dead = calloc(n ? n : 1, 1);
+    if (!jump || !dead) { free(jump); free(dead); return -1; }
+    unsigned ul_dead = 0;
+    int nd = 0;
+    for (int i = 0; i < n; ++i) {
+        if (s->models[s->model_of[i]].behavior == VO_MOBILE && !in_zoi(s, s->pos[i])) {
+            dead[i] = 1;
+            ++nd;
+            --ul_dead;
+        }
+        jump[i] = ul_dead;
+    }
+    if (nd == 0) { free(jump); free(dead); return 0; }
+    /* ModifyPair */
+    record *moved = malloc(sizeof(record) * (s->n_rec ? s->n_rec : 1));
+    record *kept = malloc(sizeof(record) * (s->n_rec ? s->n_rec : 1));
+    if (!moved || !kept) { free(jump); free(dead); free(moved); free(kept); return -1; }
+    int nm = 0, nk = 0;
+    for (int k = 0; k < s->n_rec; ++k) {   /* the table is in key order */
+        const record *r = &s->rec[k];
+        const unsigned lo = r->lo + jump[r->lo], hi = r->hi + jump[r->hi];
+        if (lo != r->lo || hi != r->hi) {
+            s->cu_moved++;
+            s->cu_from_dead += dead[r->lo] || dead[r->hi];
+            moved[nm] = *r;
+            moved[nm].lo = lo; moved[nm].hi = hi;
+            moved[nm].order = nm;
+            ++nm;
+        } else {
+            kept[nk++] = *r;
+        }
+    }
+    qsort(moved, nm, sizeof(record), cmp_rec_pos);
+    int nu = 0;   /* the last of each run of equal keys */
+    for (int k = 0; k < nm; ++k) {
+        if (k + 1 < nm && moved[k + 1].lo == moved[k].lo && moved[k + 1].hi == moved[k].hi) continue;
+        moved[nu++] = moved[k];
+    }
+    /* merge: a moved record replaces a kept one with the same key */
+    int nr = 0, a = 0, b = 0;
+    while (a < nk || b < nu) {
+        if (b >= nu || (a < nk && cmp_rec(&kept[a], &moved[b]) < 0)) s->rec[nr++] = kept[a++];
+        else {
+            if (a < nk && cmp_rec(&kept[a], &moved[b]) == 0) {
+                ++a;
+                s->cu_replaced++;
+            }
+            s->rec[nr++] = moved[b++];
+        }
+    }
+    s->n_rec = nr;
+    free(moved); free(kept);
+    /* DOContainer::Erase + DOWorld::EraseDOStatus: the DOs close up */
+    int w = 0;
+    for (int i = 0; i < n; ++i) {
+        if (dead[i]) continue;
+        s->pos[w] = s->pos[i]; s->vel[w] = s->vel[i]; s->ox[w] = s->ox[i]; s->oz[w] = s->oz[i];
+        s->av[w] = s->av[i]; s->imp[w] = s->imp[i]; s->aimp[w] = s->aimp[i];
+        s->last_imp[w] = s->last_imp[i]; s->last_aimp[w] = s->last_aimp[i];
+        s->do_sudo[w] = s->do_sudo[i]; s->model_of[w] = s->model_of[i]; s->orig_id[w] = s->orig_id[i];
+        ++w;
+    }
+    s->cu_erased += n - w;
+    s->n_do = w;
+    /* the interactions and the candidate list are rebuilt by the Reset that follows */
+    s->n_ia = 0;
+    s->n_cand = 0;
+    free(jump); free(dead);
+    return 0;
+}
+
 static int consultant_reset(vo_sim *s)
 {
     return s->consultant == 1 ? reset_nbs(s) : reset_leap(s);
@@ -912,7 +1023,8 @@ vo_sim *vo_create(const vo_scene *sc, char *err, int errlen)
     v3 **vs[] = {&s->pos, &s->vel, &s->ox, &s->oz, &s->av, &s->imp, &s->aimp, &s->last_imp, &s->last_aimp};
     for (unsigned k = 0; k < sizeof(vs) / sizeof(vs[0]); ++k) *vs[k] = calloc(n ? n : 1, sizeof(v3));
     s->do_sudo = malloc(sizeof(double) * (n ? n : 1));
-    if (!s->models || !s->iact || !s->model_of || !s->range || !s->mass || !s->sudo || !s->do_sudo ||
+    s->orig_id = malloc(sizeof(unsigned) * (n ? n : 1));
+    if (!s->models || !s->iact || !s->model_of || !s->range || !s->mass || !s->sudo || !s->do_sudo || !s->orig_id ||
         !s->pos || !s->vel || !s->ox || !s->oz || !s->av || !s->imp || !s->aimp || !s->last_imp ||
         !s->last_aimp) {
         snprintf(err, errlen, "out of memory");
@@ -939,8 +1051,29 @@ vo_sim *vo_create(const vo_scene *sc, char *err, int errlen)
     s->g = ld(sc->g); s->sf = sc->safety_factor;
     s->up_iact = sc->up_iact ? sc->up_iact : 1; s->record_every = sc->record ? sc->record : 0xffffffffu;
     s->consultant = sc->consultant;
+    for (int i = 0; i < n; ++i) s->orig_id[i] = (unsigned)i;
+    /* Boundary::Correct: lower/upper sorted per axis, an axis bounded where they differ */
+    for (int d = 0; d < 3; ++d) {
+        s->zoi_lo[d] = fmin(sc->zoi_lo[d], sc->zoi_hi[d]);
+        s->zoi_hi[d] = fmax(sc->zoi_lo[d], sc->zoi_hi[d]);
+        s->zoi_on[d] = s->zoi_lo[d] != s->zoi_hi[d];
+    }
+    s->zoi_active = s->zoi_on[0] || s->zoi_on[1] || s->zoi_on[2];
+    if (s->zoi_active && s->consultant != 0) {
+        snprintf(err, errlen, "zone of interest: only the LeapConsultant path is restated");
+        vo_destroy(s);
+        return NULL;
+    }
     if (consultant_reset(s) || initiate(s)) {
         snprintf(err, errlen, "out of memory in consultant reset");
+        vo_destroy(s);
+        return NULL;
+    }
+    /* SimMediator::SimMediator (SimMediator.cpp:122-134): CleanUp after the
+       first Initiate; a scene that lost DOs is reset and initiated again */
+    const int before = s->n_do;
+    if (cleanup(s) || (s->n_do != before && (consultant_reset(s) || initiate(s)))) {
+        snprintf(err, errlen, "out of memory in the zone-of-interest clean-up");
         vo_destroy(s);
         return NULL;
     }
@@ -952,7 +1085,7 @@ void vo_destroy(vo_sim *s)
     if (!s) return;
     free(s->models); free(s->iact); free(s->model_of); free(s->range); free(s->mass); free(s->sudo);
     free(s->pos); free(s->vel); free(s->ox); free(s->oz); free(s->av); free(s->imp); free(s->aimp);
-    free(s->last_imp); free(s->last_aimp); free(s->do_sudo);
+    free(s->last_imp); free(s->last_aimp); free(s->do_sudo); free(s->orig_id);
     free(s->cand_lo); free(s->cand_hi); free(s->ia); free(s->rec);
     free(s);
 }
@@ -1056,6 +1189,7 @@ static int step_impl(vo_sim *s, int nas, const unsigned *ids, const double *apos
     if (reset) {
         if (rebuild_records(s)) return -1;
         rebuild = 1;
+        if (cleanup(s)) return -1;   /* LeapConsultant::NextStep (LeapConsultant.cpp:58-68) */
         if (consultant_reset(s)) return -1;
     }
     if ((s->round % s->record_every) == 0 && !rebuild) {
@@ -1098,6 +1232,18 @@ void vo_get_state(const vo_sim *s, double *pos, double *vel, double *ox, double 
 }
 
 double vo_time(const vo_sim *s) { return s->t_current; }
+
+int vo_num_do(const vo_sim *s) { return s->n_do; }
+
+void vo_cleanup_stats(const vo_sim *s, long long *out)
+{
+    out[0] = s->cu_erased; out[1] = s->cu_moved; out[2] = s->cu_from_dead; out[3] = s->cu_replaced;
+}
+
+void vo_get_ids(const vo_sim *s, unsigned *ids)
+{
+    for (int i = 0; i < s->n_do; ++i) ids[i] = s->orig_id[i];
+}
 
 /* DOModel::Analysis vMassMomentInertia (DOModel.cpp:22-29, 56-89) */
 static v3 model_mmi(const vo_sim *s, int m)

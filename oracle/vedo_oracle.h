@@ -70,6 +70,8 @@ typedef struct {
     double safety_factor;          /* Constants.h:184 default 1.1 */
     unsigned up_iact, record;
     int consultant;                /* 0 Leap (LeapConsultant), 1 NBS (NBSConsultant) */
+    double zoi_lo[3], zoi_hi[3];   /* zone of interest (SystemParameter ZoneOfInterest); an axis
+                                      is bounded where lo != hi, none: inactive */
 } vo_scene;
 
 typedef struct vo_sim vo_sim;
@@ -107,6 +109,13 @@ void vo_get_candidates(const vo_sim *s, unsigned *lo, unsigned *hi);
  * histories keyed by (lo,hi) -- the IactRecordTab payload. Returns count. */
 int vo_num_active(const vo_sim *s);
 void vo_get_active(const vo_sim *s, unsigned *lo, unsigned *hi, vo_status *st);
+/* Current number of DOs (the zone of interest erases mobile DOs that leave
+ * it, Consultant::CleanUp) and the scene's original DO id of each. */
+int vo_num_do(const vo_sim *s);
+void vo_get_ids(const vo_sim *s, unsigned *ids);
+/* CleanUp counts so far: DOs erased, IactRecordTab keys moved, moved keys of
+ * an erased DO, kept records replaced by a moved one (test instrumentation). */
+void vo_cleanup_stats(const vo_sim *s, long long *out);
 
 /* DOWorld::CalculateSystemEnergy (DOWorld.cpp:481-632) of the current state,
  * sequentially over the mobile DOs in DO-id order.  out[16]: potential,

@@ -28,11 +28,13 @@ def load_golden(name):
         model_of=z["model_of"], pos=z["pos"], vel=z["vel"], ox=z["ox"], oz=z["oz"], av=z["av"],
         dt=float(sc[0]), t_start=float(sc[1]), t_stop=float(sc[2]), t_current=float(sc[3]), g=z["g"],
         safety_factor=float(sc[4]), up_iact=int(z["cadence"][0]), record=int(z["cadence"][1]))
+    if "zoi" in z.files:
+        wa.zoi_lo, wa.zoi_hi = z["zoi"][:3].copy(), z["zoi"][3:].copy()
     return z, wa
 
 
 GOLDEN_CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_nbs", "drum_kin", "box_assign",
-                "box_piston"]
+                "box_piston", "box_zoi"]
 
 
 def step_golden(sim, z, step):

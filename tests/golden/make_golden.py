@@ -117,7 +117,36 @@ def make_cases():
                          spacing=2.02, jitter=0.05, vmax=0.05, offset=3.0, omega_max=5.0)
     cases.append(("box_piston", piston_world(w).to_arrays(up_iact=7, record=1000), 0, 150,
                   [0, 1, 2, 6, 7, 8, 50, 150], None))
+    # a zone of interest: the box has no floor, the spheres move down and
+    # the lowest leave the zone while touching the layer above; they are
+    # erased at the next rebuilds (Consultant::CleanUp) and the records
+    # move with IactRecordTab::ModifyPair
+    # (pre-compressed lattice moving down as one: the lowest layer leaves the
+    # zone at the first rebuild still pressed against the layer above, so
+    # records of erased spheres move onto other pairs)
+    w = scenes.box_scene(64, seed=16, law="ISwBSDBF", dt=2e-5, steps=400, kn=2e3, e=0.6, mu=0.4,
+                         spacing=1.98, jitter=0.0, vmax=0.0, offset=3.0)
+    cases.append(("box_zoi", zoi_world(w, vz=-0.4, below=0.01).to_arrays(up_iact=3, record=1000), 0, 60,
+                  [0, 2, 3, 4, 6, 7, 30, 60], None))
     return cases
+
+
+def zoi_world(w, vz=-0.4, below=0.35):
+    """No floor plate; every sphere moves down at vz; the zone of interest
+    ends `below` radii under the lowest sphere centres."""
+    from vedo_amd.world import DOStatus
+    r = [m for m in w.models if m.name == "Sphere"][0].attributes[0]
+    floor = [k for k, st in enumerate(w.statuses) if st.name == "Plate" and tuple(st.orientation_z) == (0, 0, 1)]
+    k = min(floor, key=lambda k: w.statuses[k].position[2])
+    del w.statuses[k]
+    zmin = min(st.position[2] for st in w.statuses if st.name == "Sphere")
+    w.statuses = [DOStatus(st.name, st.position, (st.velocity[0], st.velocity[1], st.velocity[2] + vz),
+                           st.orientation_x, st.orientation_z, st.angular_velocity)
+                  if st.name == "Sphere" else st for st in w.statuses]
+    sp = w.system_parameter
+    sp.zoi_lower = (0.0, 0.0, zmin - below * r)
+    sp.zoi_upper = (0.0, 0.0, 1.0)
+    return w
 
 
 def piston_world(w, density=100.0, force=-0.02, overlap=0.02):
@@ -161,7 +190,8 @@ def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE):
         iact_law=wa.iact_law, iact_param=wa.iact_param, model_of=wa.model_of, pos=wa.pos, vel=wa.vel,
         ox=wa.ox, oz=wa.oz, av=wa.av,
         scalars=np.array([wa.dt, wa.t_start, wa.t_stop, wa.t_current, wa.safety_factor]),
-        g=wa.g, cadence=np.array([wa.up_iact, wa.record], np.int64))
+        g=wa.g, cadence=np.array([wa.up_iact, wa.record], np.int64),
+        zoi=np.concatenate([np.asarray(wa.zoi_lo, np.float64), np.asarray(wa.zoi_hi, np.float64)]))
     if assigned is not None:
         arrays["assigned_ids"] = np.asarray(assigned[0], np.uint32)
         arrays["assigned"] = assigned[1]

@@ -68,6 +68,11 @@ class SystemParameter:
     time_current: float
     field_acceleration: Tuple[float, float, float] = (0.0, 0.0, -9.81)
     safety_factor: float = 1.1
+    # ZoneOfInterest (SystemParameter.h:275, Boundary.cpp:79-129): an axis is
+    # bounded where its lower and upper values differ; mobile DOs outside are
+    # erased at candidate-list rebuilds (Consultant::CleanUp)
+    zoi_lower: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    zoi_upper: Tuple[float, float, float] = (0.0, 0.0, 0.0)
 
 
 @dataclass
@@ -163,6 +168,8 @@ class WorldArrays:
     record: int
     group_names: List[str] = field(default_factory=list)
     model_names: List[str] = field(default_factory=list)
+    zoi_lo: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    zoi_hi: np.ndarray = field(default_factory=lambda: np.zeros(3))
 
     @property
     def n_do(self) -> int:
@@ -242,7 +249,8 @@ class DOWorld:
             dt=sp.time_interval, t_start=sp.time_start, t_stop=sp.time_stop,
             t_current=sp.time_current, g=np.array(sp.field_acceleration, np.float64),
             safety_factor=sp.safety_factor, up_iact=int(up_iact), record=int(record),
-            group_names=groups, model_names=[m.name for m in ms])
+            group_names=groups, model_names=[m.name for m in ms],
+            zoi_lo=np.array(sp.zoi_lower, np.float64), zoi_hi=np.array(sp.zoi_upper, np.float64))
 
 
 def write_vscene(wa: WorldArrays, path: str, consultant: int = 0):
@@ -254,6 +262,8 @@ def write_vscene(wa: WorldArrays, path: str, consultant: int = 0):
     lines.append("time %s %s %s %s" % (h(wa.t_start), h(wa.t_stop), h(wa.dt), h(wa.t_current)))
     lines.append("gravity %s %s %s" % tuple(h(float(x)) for x in wa.g))
     lines.append("safety %s" % h(wa.safety_factor))
+    if np.any(np.asarray(wa.zoi_lo) != np.asarray(wa.zoi_hi)):
+        lines.append("zoi %s %s %s %s %s %s" % tuple(h(float(x)) for x in list(wa.zoi_lo) + list(wa.zoi_hi)))
     lines.append("cadence %d %d %d" % (wa.up_iact, wa.record, consultant))
     lines.append("groups %d %s" % (len(wa.group_names), " ".join(wa.group_names)))
     lines.append("models %d" % len(wa.model_names))

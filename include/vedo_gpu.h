@@ -92,6 +92,12 @@ typedef struct {
     unsigned record;           /* record period (anne's Record) */
     int consultant;            /* 0: LeapConsultant rules (the only one on the GPU) */
     double inner_skin;         /* <= 0: automatic */
+    /* SystemParameter ZoneOfInterest (SystemParameter.h:275): an axis is
+     * bounded where lower != upper (Boundary::Correct, Boundary.cpp:79-129);
+     * with a bounded axis, mobile DOs outside are erased at construction and
+     * at every candidate-list rebuild (Consultant::CleanUp, Consultant.cpp:
+     * 264-319) and the remaining DO ids close up. */
+    double zoi_lo[3], zoi_hi[3];
 } vg_scene_desc;
 
 /* ImpactStatus payload of one contact (ImpactStatus.h:208-222). */
@@ -144,6 +150,11 @@ int vg_set_wall_rotation(vg_scene *h, int n, const unsigned *ids, const double *
 int vg_get_state(vg_scene *h, double *pos, double *vel, double *ox, double *oz, double *av,
                  double *impact, double *ang_impact);
 double vg_time(const vg_scene *h);
+
+/* Current number of DOs of a scene (slot 0 of a one-scene handle): the zone
+ * of interest erases DOs, and vg_get_state / vg_scene_get_state return the
+ * remaining ones in their (closed-up) DO-id order. */
+int vg_scene_num_do(vg_scene *h, int scene, int *n_do);
 
 /* Number of interactions in the candidate list (Consultant::GetIactNum). */
 int vg_num_interactions(vg_scene *h, long long *n);

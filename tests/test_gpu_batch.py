@@ -114,3 +114,29 @@ def test_batch_with_constrained_piston_equals_alone():
         ok = _same(got, ref, fields=("pos", "vel"))
         print(f"\n  scene {sid}: {wa.n_do} DOs (walls included), bitwise equal alone and batched: {ok}", flush=True)
         assert ok
+
+
+def test_batch_with_zone_of_interest_equals_alone():
+    # a scene losing spheres to its zone of interest (DO ids close up,
+    # records re-keyed) among scenes that keep theirs
+    from conftest import load_golden
+    from vedo_amd.gpu import SCENE_DONE, GpuBatch
+    _, zoi = load_golden("box_zoi")
+    cases = [(_with_up(_box(64, 31, "ISwBSDBF"), 3), 24), (zoi, 30), (_with_up(_box(27, 32, "ISwBSDBF"), 3), 21)]
+    b = GpuBatch(max_spheres=1000, max_walls=64, max_scenes=4, up_iact=3)
+    ids = [b.admit(wa, steps) for wa, steps in cases]
+    for _ in range(14):
+        b.step(3)
+    for (wa, steps), sid in zip(cases, ids):
+        state, done, _ = b.status(sid)
+        assert state == SCENE_DONE and done == steps, (sid, state, done)
+        got, ref = b.state(sid), _alone(wa, steps)
+        assert len(got["pos"]) == len(ref["pos"])
+        ok = _same(got, ref, fields=("pos", "vel"))
+        print(f"\n  scene {sid}: {len(got['pos'])} DOs left, bitwise equal alone and batched: {ok}", flush=True)
+        assert ok
+
+
+def _with_up(wa, up):
+    import dataclasses
+    return dataclasses.replace(wa, up_iact=up)

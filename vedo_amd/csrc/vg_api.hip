@@ -44,6 +44,8 @@ struct SceneHost {
     double dt = 0, t = 0, t_stop = 0, g[3] = {0, 0, 0}, sf = 1.1, skin = 0, bond_reach = 0, rmin = 0;
     bool sd_only = true, single_group = true, no_bond = true;
     bool has_constrained = false;       // a wall moves under its contacts (DOConstrained*)
+    int zon = 0;                        // zone of interest: bounded axes (bits), bounds
+    double zlo[3] = {0, 0, 0}, zhi[3] = {0, 0, 0};
     long long steps_done = 0, steps_target = -1;
     int seg_begin = 0, seg_end = 0;     // slots of the spheres in the current layout
     // initial state until the scene is placed
@@ -146,6 +148,11 @@ struct vg_scene {
     unsigned *cw_cnt = nullptr;         // [ld + 1] per-sphere pair counts, then offsets
     int *cw_beg = nullptr, *cw_end = nullptr;   // [wcap] pair range of each wall slot
     double *cw_imp = nullptr;           // [wcap][3] each constrained wall's impulse of the step
+    // zone of interest: per-sphere erase flags, per-scene counts, ranks
+    bool any_zoi = false;
+    unsigned char *zoi_dead = nullptr;   // [ld]
+    int *zoi_cnt = nullptr;             // [scap]
+    unsigned *zoi_rank = nullptr, *zoi_pre = nullptr;   // [ld + 1] erase flags by rank, their prefix count
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
@@ -266,9 +273,12 @@ static void update_batch_flags(vg_scene *h)
         h->no_bond = h->no_bond && S.no_bond;
     }
     h->any_constrained = false;
+    h->any_zoi = false;
     for (const SceneHost &S : h->sc)
-        if (S.state == VG_SCENE_RUNNING || S.state == VG_SCENE_FINISHED || S.state == VG_SCENE_QUEUED)
+        if (S.state == VG_SCENE_RUNNING || S.state == VG_SCENE_FINISHED || S.state == VG_SCENE_QUEUED) {
             h->any_constrained = h->any_constrained || S.has_constrained;
+            h->any_zoi = h->any_zoi || S.zon != 0;
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -360,6 +370,13 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
     S.dt = d->dt; S.t = d->t_current; S.t_stop = d->t_stop;
     for (int c = 0; c < 3; ++c) S.g[c] = d->g[c];
     S.sf = d->safety_factor;
+    // Boundary::Correct (Boundary.cpp:79-129): bounds sorted per axis, an axis bounded where they differ
+    S.zon = 0;
+    for (int c = 0; c < 3; ++c) {
+        S.zlo[c] = std::min(d->zoi_lo[c], d->zoi_hi[c]);
+        S.zhi[c] = std::max(d->zoi_lo[c], d->zoi_hi[c]);
+        if (S.zlo[c] != S.zhi[c]) S.zon |= 1 << c;
+    }
     double rmin = 1e300;
     for (unsigned id : S.sphere_ids) rmin = std::min(rmin, d->models[d->model_of[id]].attr[0]);
     S.rmin = rmin;
@@ -576,6 +593,8 @@ static int place_queued(vg_scene *h)
         D.skin = S.skin; D.relist_half = 0.5 * S.skin * (1.0 - 1e-9);
         D.law_off = S.law_off; D.n_groups = S.n_groups;
         D.w_begin = S.w_begin; D.w_end = S.w_begin + S.nw;
+        D.zon = S.zon;
+        for (int c = 0; c < 3; ++c) { D.zlo[c] = S.zlo[c]; D.zhi[c] = S.zhi[c]; }
         D.active = 1;
         S.seg_begin = b;
         S.seg_end = b + n;
@@ -653,6 +672,180 @@ static int build_cw_pairs(vg_scene *h, const Geo &G, int n)
     return 0;
 }
 
+static int export_active(vg_scene *h, int slot, ActiveOut *out, size_t cap, unsigned long long *dcnt,
+                         unsigned *d_sid, unsigned *d_wid);
+static int id_maps(vg_scene *h, int slot, unsigned **d_sid, unsigned **d_wid);
+
+// A scene that loses spheres to its zone of interest at a rebuild: its
+// records re-keyed by ModifyPair, restored over the rebuilt lists.
+struct ZoiPlan {
+    int slot = -1;
+    unsigned long long *keys = nullptr;   // sorted new keys (of equal keys the last applies)
+    RestoreRec *recs = nullptr;
+    long long nrec = 0;
+};
+
+static void zoi_free(std::vector<ZoiPlan> &plans)
+{
+    for (ZoiPlan &p : plans) {
+        if (p.keys) (void)hipFree(p.keys);
+        if (p.recs) (void)hipFree(p.recs);
+        p.keys = nullptr;
+        p.recs = nullptr;
+    }
+}
+
+// Consultant::CleanUp (Consultant.cpp:264-319) at a rebuild boundary, after
+// RebuildIactRecordTab and before Reset (LeapConsultant.cpp:58-68), and at a
+// scene's placement (SimMediator.cpp:122-134): spheres of stepping scenes
+// outside their zone of interest are erased.  Their scene's records (the
+// active pairs of the lists about to be replaced) move with ModifyPair; the
+// DO ids and ranks close up; the erased spheres take ranks in the dropped
+// region of the rebuild's sort.  The host maps of DO ids follow.
+static int zoi_prepare(vg_scene *h, std::vector<ZoiPlan> &plans)
+{
+    hipStream_t st = h->stream;
+    const int n = h->n, ld = h->ld;
+    if (upload_scenes(h)) return VG_EHIP;   // scenes placed at this boundary carry their zone
+    if (!h->zoi_dead) {
+        if (dalloc(h, &h->zoi_dead, (size_t)ld) || dalloc(h, &h->zoi_cnt, (size_t)h->scap) ||
+            dalloc(h, &h->zoi_rank, (size_t)ld + 1) || dalloc(h, &h->zoi_pre, (size_t)ld + 1))
+            return VG_ENOMEM;
+    }
+    HIPCHK(hipMemsetAsync(h->zoi_cnt, 0, sizeof(int) * h->scap, st));
+    hipLaunchKernelGGL(k_zoi_mark, dim3(nblk(n, 256)), dim3(256), 0, st, h->geo[h->cur], n,
+                       (const SceneDev *)h->scenes, (const ModelDev *)h->models, h->zoi_dead, h->zoi_cnt);
+    std::vector<int> cnt(h->scap);
+    HIPCHK(hipMemcpyAsync(cnt.data(), h->zoi_cnt, sizeof(int) * h->scap, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int> dead_slots;
+    for (int k = 0; k < h->scap; ++k)
+        if (cnt[k] > 0 && h->sc[k].state == VG_SCENE_RUNNING) dead_slots.push_back(k);
+    if (dead_slots.empty()) return 0;
+    size_t need = 0;
+    for (int k : dead_slots) {
+        SceneHost &S = h->sc[k];
+        const int b = S.seg_begin, e = S.seg_end, n_old = S.n;
+        ZoiPlan plan;
+        plan.slot = k;
+        // the scene's records, keyed by its DO ids before the erase (no lists yet at its placement)
+        ActiveOut *recs = nullptr;
+        long long m = 0;
+        if (S.steps_done > 0) {
+            unsigned long long *dcnt = nullptr;
+            unsigned *d_sid = nullptr, *d_wid = nullptr;
+            if (dalloc(h, &dcnt, 1) || id_maps(h, k, &d_sid, &d_wid)) return VG_ENOMEM;
+            if (export_active(h, k, nullptr, 0, dcnt, d_sid, d_wid)) return VG_EHIP;
+            unsigned long long c = 0;
+            HIPCHK(hipMemcpyAsync(&c, dcnt, sizeof(c), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            m = (long long)c;
+            if (m > 0) {
+                if (dalloc(h, &recs, (size_t)m)) return VG_ENOMEM;
+                if (export_active(h, k, recs, (size_t)m, dcnt, d_sid, d_wid)) return VG_EHIP;
+            }
+            HIPCHK(hipStreamSynchronize(st));
+            (void)hipFree(dcnt); (void)hipFree(d_sid); (void)hipFree(d_wid);
+        }
+        // erase flags by rank and their prefix count
+        hipLaunchKernelGGL(k_zoi_rank, dim3(nblk(e - b, 256)), dim3(256), 0, st, h->geo[h->cur], b, e,
+                           (const unsigned char *)h->zoi_dead, h->zoi_rank);
+        HIPCHK(hipMemsetAsync(h->zoi_rank + e, 0, sizeof(unsigned), st));
+        need = 0;
+        (void)rocprim::exclusive_scan(nullptr, need, h->zoi_rank + b, h->zoi_pre + b, 0u, (size_t)n_old + 1,
+                                      rocprim::plus<unsigned>(), st);
+        if (ensure_tmp(h, need)) return VG_EHIP;
+        HIPCHK(rocprim::exclusive_scan(h->tmp, need, h->zoi_rank + b, h->zoi_pre + b, 0u, (size_t)n_old + 1,
+                                       rocprim::plus<unsigned>(), st));
+        std::vector<unsigned> dead_by_rank((size_t)n_old);
+        HIPCHK(hipMemcpyAsync(dead_by_rank.data(), h->zoi_rank + b, sizeof(unsigned) * n_old, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        // DO ids close up (DOWorld::EraseDOStatus); ModifyPair's map of every DO id:
+        // id -> id + jump, jump = minus the erased DOs up to and including id
+        std::vector<char> dead_do((size_t)S.n_do, 0);
+        for (int r = 0; r < n_old; ++r)
+            if (dead_by_rank[(size_t)r]) dead_do[S.sphere_ids[(size_t)r]] = 1;
+        std::vector<unsigned> jump((size_t)S.n_do);
+        unsigned ul_dead = 0;
+        for (int id = 0; id < S.n_do; ++id) {
+            if (dead_do[(size_t)id]) --ul_dead;
+            jump[(size_t)id] = (unsigned)id + ul_dead;
+        }
+        if (m > 0) {
+            unsigned *d_jump = nullptr, *prio_src = nullptr, *prio = nullptr, *idx[2] = {nullptr, nullptr};
+            unsigned long long *okey = nullptr, *nkey = nullptr, *k2[2] = {nullptr, nullptr};
+            RestoreRec *rr = nullptr;
+            const size_t M = (size_t)m;
+            if (dalloc(h, &d_jump, (size_t)S.n_do) || dalloc(h, &prio_src, M) || dalloc(h, &prio, M) ||
+                dalloc(h, &idx[0], M) || dalloc(h, &idx[1], M) || dalloc(h, &okey, M) || dalloc(h, &nkey, M) ||
+                dalloc(h, &k2[0], M) || dalloc(h, &k2[1], M) || dalloc(h, &rr, M) || dalloc(h, &plan.keys, M) ||
+                dalloc(h, &plan.recs, M))
+                return VG_ENOMEM;
+            HIPCHK(hipMemcpyAsync(d_jump, jump.data(), sizeof(unsigned) * S.n_do, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_zoi_records, dim3(nblk(m, 256)), dim3(256), 0, st, (const ActiveOut *)recs, m,
+                               (const unsigned *)d_jump, okey, nkey, prio_src, rr);
+            // the table's (std::map) order: records by old key
+            need = 0;
+            (void)rocprim::radix_sort_pairs(nullptr, need, okey, k2[0], prio_src, idx[0], M, 0, 64, st);
+            if (ensure_tmp(h, need)) return VG_EHIP;
+            HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, okey, k2[0], prio_src, idx[0], M, 0, 64, st));
+            hipLaunchKernelGGL(k_zoi_prio, dim3(nblk(m, 256)), dim3(256), 0, st, (const unsigned long long *)k2[0],
+                               (const unsigned *)idx[0], (const unsigned long long *)nkey, m, prio);
+            // order by (new key, priority): by priority, then stably by new key
+            hipLaunchKernelGGL(k_iota, dim3(nblk(m, 256)), dim3(256), 0, st, prio_src, m);
+            need = 0;
+            (void)rocprim::radix_sort_pairs(nullptr, need, prio, (unsigned *)k2[1], prio_src, idx[1], M, 0, 32, st);
+            if (ensure_tmp(h, need)) return VG_EHIP;
+            HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, prio, (unsigned *)k2[1], prio_src, idx[1], M, 0, 32, st));
+            hipLaunchKernelGGL(k_gather_u64, dim3(nblk(m, 256)), dim3(256), 0, st, (const unsigned long long *)nkey,
+                               (const unsigned *)idx[1], m, k2[0]);
+            need = 0;
+            (void)rocprim::radix_sort_pairs(nullptr, need, k2[0], plan.keys, idx[1], idx[0], M, 0, 64, st);
+            if (ensure_tmp(h, need)) return VG_EHIP;
+            HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, k2[0], plan.keys, idx[1], idx[0], M, 0, 64, st));
+            hipLaunchKernelGGL(k_gather_restore, dim3(nblk(m, 256)), dim3(256), 0, st, (const RestoreRec *)rr,
+                               (const unsigned *)idx[0], m, plan.recs);
+            plan.nrec = m;
+            HIPCHK(hipStreamSynchronize(st));
+            for (void *q : {(void *)d_jump, (void *)prio_src, (void *)prio, (void *)idx[0], (void *)idx[1],
+                            (void *)okey, (void *)nkey, (void *)k2[0], (void *)k2[1], (void *)rr, (void *)recs})
+                if (q) (void)hipFree(q);
+        } else if (recs) {
+            (void)hipFree(recs);
+        }
+        // the host maps of the closed-up DO ids
+        std::vector<unsigned> sid_new;
+        sid_new.reserve((size_t)(n_old - cnt[k]));
+        for (int r = 0; r < n_old; ++r)
+            if (!dead_by_rank[(size_t)r]) sid_new.push_back(jump[S.sphere_ids[(size_t)r]]);
+        for (unsigned &w : S.wall_ids) w = jump[w];
+        S.n_do -= cnt[k];
+        S.n = (int)sid_new.size();
+        S.sphere_ids.swap(sid_new);
+        S.do_to_sphere.assign((size_t)S.n_do, -1);
+        S.do_to_wall.assign((size_t)S.n_do, -1);
+        for (int r = 0; r < S.n; ++r) S.do_to_sphere[S.sphere_ids[(size_t)r]] = r;
+        for (int w = 0; w < S.nw; ++w) S.do_to_wall[S.wall_ids[(size_t)w]] = S.w_begin + w;
+        plans.push_back(plan);
+    }
+    // ranks close up; erased spheres go to the dropped region after the live and retiring scenes
+    long long dead_base = 0;
+    for (int k = 0; k < h->scap; ++k) {
+        const SceneHost &S = h->sc[k];
+        if (S.state == VG_SCENE_RUNNING || S.state == VG_SCENE_FINISHED || S.state == VG_SCENE_RETIRING)
+            dead_base += S.n;
+    }
+    for (int k : dead_slots) {
+        const SceneHost &S = h->sc[k];
+        hipLaunchKernelGGL(k_zoi_newoid, dim3(nblk(S.seg_end - S.seg_begin, 256)), dim3(256), 0, st,
+                           h->geo[h->cur], S.seg_begin, S.seg_end, (const unsigned char *)h->zoi_dead,
+                           (const unsigned *)h->zoi_pre, (unsigned)dead_base);
+        dead_base += cnt[k];
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 static int rebuild(vg_scene *h)
 {
     hipStream_t st = h->stream;
@@ -673,6 +866,20 @@ static int rebuild(vg_scene *h)
     pc.mark("finished / retiring");
     if (place_queued(h)) return VG_EHIP;
     pc.mark("place queued");
+    std::vector<ZoiPlan> zplans;
+    struct ZoiGuard {
+        std::vector<ZoiPlan> &p;
+        ~ZoiGuard() { zoi_free(p); }
+    } zguard{zplans};
+    if (h->any_zoi && h->n > 0) {
+        const int rc = zoi_prepare(h, zplans);
+        if (rc) return rc;
+        if (!zplans.empty()) {
+            // the pre-rebuild snapshot and per-entry records name erased DOs and old ids
+            h->snap_valid = false;
+            h->ent_valid = false;
+        }
+    }
     const int n_total = h->n;
     // rank offsets: live scenes by slot, then the retiring ones
     int n_live = 0, max_slot = -1;
@@ -929,6 +1136,22 @@ static int rebuild(vg_scene *h)
             const int rc = build_cw_pairs(h, G1w, n);
             if (rc) return rc;
         }
+        // zone of interest: the re-keyed records over the carried lists (SimMediator::Initiate)
+        for (const ZoiPlan &P : zplans) {
+            if (P.nrec == 0) continue;
+            const SceneHost &S = h->sc[P.slot];
+            unsigned *d_sid = nullptr, *d_wid = nullptr;
+            if (id_maps(h, P.slot, &d_sid, &d_wid)) return VG_ENOMEM;
+            const int b = h->sdev[P.slot].rank_off;
+            hipLaunchKernelGGL(k_restore_history, dim3(nblk(S.n, 128)), dim3(128), 0, st, G1w, h->ell[h->ecur ^ 1],
+                               b, b + S.n, h->ld, h->ki, (const unsigned *)d_sid, (const unsigned *)d_wid,
+                               (const unsigned long long *)P.keys, (const RestoreRec *)P.recs, P.nrec,
+                               (const ModelDev *)h->models, (const SceneDev *)h->scenes, (const LawDev *)h->laws,
+                               (const WallDev *)h->walls);
+            HIPCHK(hipStreamSynchronize(st));
+            (void)hipFree(d_sid);
+            (void)hipFree(d_wid);
+        }
     }
     h->n_iact = -1;   // counted on request (vg_num_interactions)
     // the fresh lists need no relist: clear the ring slot the next step reads
@@ -1021,6 +1244,7 @@ static void free_scene(vg_scene *h)
     F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->sid[0]); F(h->sid[1]); F(h->fetch); F(h->ovf);
     F(h->cw_val); F(h->cw_stamp); F(h->cw_key[0]); F(h->cw_key[1]); F(h->cw_vidx[0]); F(h->cw_vidx[1]);
     F(h->cw_w); F(h->cw_p); F(h->cw_cnt); F(h->cw_beg); F(h->cw_end); F(h->cw_imp);
+    F(h->zoi_dead); F(h->zoi_cnt); F(h->zoi_rank); F(h->zoi_pre);
     for (auto e : h->kev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1678,6 +1902,15 @@ static int get_active(vg_scene *h, int slot, long long *n, unsigned *lo, unsigne
         status_of(ho[k], st[k]);
     }
     *n = (long long)ho.size();
+    return 0;
+}
+
+extern "C" int vg_scene_num_do(vg_scene *h, int scene, int *n_do)
+{
+    if (!h || !n_do || scene < 0 || scene >= h->scap) return VG_EINVAL;
+    const SceneHost &S = h->sc[scene];
+    if (S.state == VG_SCENE_FREE) return fail(h, VG_EINVAL, "no scene in this slot");
+    *n_do = S.n_do;
     return 0;
 }
 

@@ -58,8 +58,13 @@ struct SceneDev {
     int rank_off;            // offset of the scene's ranks in the sort of the next rebuild
     int sid_off;             // offset of the scene's sphere rank -> DO id map
     int active;              // 1 stepping, 0 frozen (state held), -1 retired (dropped at the rebuild)
-    int pad;
+    int zon;                 // zone of interest: bit d set where axis d is bounded (0: none)
+    double zlo[3], zhi[3];   // its bounds (SystemParameter ZoneOfInterest, Boundary.cpp:79-191)
 };
+
+// A sphere erased by the zone of interest at a rebuild carries, in place of
+// its rank, OID_DEAD | its rank in the dropped region of the rebuild's sort.
+constexpr unsigned OID_DEAD = 0xC0000000u;
 
 struct StepArgs {
     int n, ld, nw, ki;       // live sphere slots, leading dimension, wall slots, list capacity
@@ -163,6 +168,17 @@ __global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int b, in
 __global__ void k_inner_build(InnerArgs a);
 __global__ void k_hdr_zero(unsigned char *hdr, int b, int e);
 struct RestoreRec;
+__global__ void k_zoi_mark(Geo g, int n, const SceneDev *scenes, const ModelDev *models, unsigned char *dead,
+                           int *count);
+__global__ void k_zoi_rank(Geo g, int b, int e, const unsigned char *dead, unsigned *dead_by_rank);
+__global__ void k_zoi_newoid(Geo g, int b, int e, const unsigned char *dead, const unsigned *pre, unsigned dead_base);
+__global__ void k_zoi_records(const ActiveOut *in, long long n, const unsigned *jump, unsigned long long *okey,
+                              unsigned long long *nkey, unsigned *prio_src, RestoreRec *recs);
+__global__ void k_iota(unsigned *p, long long n);
+__global__ void k_gather_u64(const unsigned long long *src, const unsigned *idx, long long n, unsigned long long *dst);
+__global__ void k_gather_restore(const RestoreRec *src, const unsigned *idx, long long n, RestoreRec *dst);
+__global__ void k_zoi_prio(const unsigned long long *okey_sorted, const unsigned *idx_sorted,
+                           const unsigned long long *nkey, long long n, unsigned *prio);
 __global__ void k_restore_history(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *sphere_id,
                                   const unsigned *wall_id, const unsigned long long *keys, const RestoreRec *recs,
                                   long long nrec, const ModelDev *models, const SceneDev *scenes, const LawDev *laws,

@@ -521,10 +521,12 @@ __global__ void k_bounds(Geo g, int n, const SceneDev *scenes, const ModelDev *m
         v[3] = ~ord64(p.x); v[4] = ~ord64(p.y); v[5] = ~ord64(p.z);
         v[6] = ord64(len(mk(u.x, u.y, u.z)));
         v[7] = ord64(p.w);
-        inv[scenes[sc].rank_off + g.oid[i]] = i;
+        const unsigned o = g.oid[i];
+        const bool dead = (o & OID_DEAD) == OID_DEAD;   // erased by the zone of interest
+        inv[dead ? (o & ~OID_DEAD) : scenes[sc].rank_off + o] = i;
         xs[i] = p.x; ys[i] = p.y; zs[i] = p.z;
-        sid[i] = scenes[sc].active < 0 ? retired : (unsigned)sc;
-        if (scenes[sc].active < 0)   // retired: no part in any grid
+        sid[i] = (scenes[sc].active < 0 || dead) ? retired : (unsigned)sc;
+        if (scenes[sc].active < 0 || dead)   // retired or erased: no part in any grid
             for (int k = 0; k < 8; ++k) v[k] = 0;
     } else {
         for (int k = 0; k < 8; ++k) v[k] = 0;
@@ -599,7 +601,7 @@ __global__ void k_cellkey(Geo g, int n_ranks, const SceneDev *scenes, const Mode
     const int sc = models[g.mdl[i]].scene;
     const SceneDev &S = scenes[sc];
     vals[t] = i;
-    if (S.active < 0) {
+    if (S.active < 0 || (g.oid[i] & OID_DEAD) == OID_DEAD) {
         keys[t] = retired_key;
         return;
     }
@@ -980,7 +982,7 @@ __global__ void k_cw_sum(const WallDev *walls, int nw, const SceneDev *scenes, c
 // Consultant::RetrieveImpactStatus): every listed pair whose (lo, hi) DO ids
 // have a record takes the record's status, as a rebuild's history carry does
 // (flags, shear force, Kn / initial velocity; the pending Backup is skipped).
-// Records are sorted by key lo << 32 | hi.
+// Records are sorted by key lo << 32 | hi; of equal keys the last applies.
 struct RestoreRec {
     unsigned flags, pad;   // F_CONTACT | F_LASTCONTACT | F_BOND | F_LASTBOND
     double kn, init_vel, shear[3];
@@ -1013,14 +1015,14 @@ __global__ void k_restore_history(Geo g, Ell ell, int b, int e, int ld, int ki, 
         }
         const unsigned lo = id < pid ? id : pid, hi = id < pid ? pid : id;
         const unsigned long long key = ((unsigned long long)lo << 32) | hi;
-        long long l = 0, r = nrec;   // first record with key >= key
+        long long l = 0, r = nrec;   // first record with key > key; the last of equal keys wins
         while (l < r) {
             const long long m = (l + r) >> 1;
-            if (keys[m] < key) l = m + 1;
+            if (keys[m] <= key) l = m + 1;
             else r = m;
         }
-        if (l >= nrec || keys[l] != key) continue;
-        const RestoreRec &R = recs[l];
+        if (l == 0 || keys[l - 1] != key) continue;
+        const RestoreRec &R = recs[l - 1];
         const unsigned f = (R.flags & F_STATUS) | (ent & F_SLAVE) | F_NOBACKUP;
         ell.ent[x] = j | f;
         const int law = laws[S.law_off + mi.group * S.n_groups + gj].law;
@@ -1029,6 +1031,99 @@ __global__ void k_restore_history(Geo g, Ell ell, int b, int e, int ld, int ki, 
         hs.aux = law == LAW_H ? R.kn : (law == LAW_HERTZ ? R.init_vel : 0.0);
         store_hist(ell, h, k, ki, ld, i, hs);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Zone of interest (Consultant::CleanUp, Consultant.cpp:264-319)
+// ---------------------------------------------------------------------------
+
+// Spheres of stepping scenes outside their zone of interest (Boundary::
+// InBoundary, Boundary.cpp:161-191): flagged, counted per scene.
+__global__ void k_zoi_mark(Geo g, int n, const SceneDev *scenes, const ModelDev *models, unsigned char *dead,
+                           int *count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int sc = models[g.mdl[i]].scene;
+    const SceneDev &S = scenes[sc];
+    unsigned char d = 0;
+    if (S.zon && S.active > 0) {
+        const double4 p = g.pos[i];
+        const double c[3] = {p.x, p.y, p.z};
+        for (int k = 0; k < 3; ++k)
+            if (((S.zon >> k) & 1) && (c[k] < S.zlo[k] || c[k] > S.zhi[k])) d = 1;
+    }
+    dead[i] = d;
+    if (d) atomicAdd(&count[sc], 1);
+}
+
+// dead flags in rank order for the scene occupying slots b..e-1
+__global__ void k_zoi_rank(Geo g, int b, int e, const unsigned char *dead, unsigned *dead_by_rank)
+{
+    const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < e) dead_by_rank[b + g.oid[i]] = dead[i];
+}
+
+// Ranks close up over the erased spheres (their DO ids close up: DOWorld::
+// EraseDOStatus); an erased sphere takes OID_DEAD | its rank in the dropped
+// region.  pre: exclusive prefix count of erased spheres in rank order.
+__global__ void k_zoi_newoid(Geo g, int b, int e, const unsigned char *dead, const unsigned *pre, unsigned dead_base)
+{
+    const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
+    const unsigned o = g.oid[i];
+    const unsigned before = pre[b + o] - pre[b];
+    g.oid[i] = dead[i] ? (OID_DEAD | (dead_base + before)) : o - before;
+}
+
+// IactRecordTab::ModifyPair (IactRecordTab.cpp:41-69) of the records: each key
+// moves by DO id -> jump[id] (the new id of an erased DO is its predecessor's);
+// moved keys are written over the table in key order, so of equal new keys
+// the moved one wins, and of moved ones the later in old-key order.
+__global__ void k_zoi_records(const ActiveOut *in, long long n, const unsigned *jump, unsigned long long *okey,
+                              unsigned long long *nkey, unsigned *prio_src, RestoreRec *recs)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const ActiveOut &A = in[k];
+    okey[k] = ((unsigned long long)A.lo << 32) | A.hi;
+    nkey[k] = ((unsigned long long)jump[A.lo] << 32) | jump[A.hi];
+    prio_src[k] = (unsigned)k;
+    RestoreRec &R = recs[k];
+    R.flags = A.flags & F_STATUS;
+    R.pad = 0;
+    R.kn = A.hist[3];
+    R.init_vel = A.hist[3];
+    R.shear[0] = A.hist[0]; R.shear[1] = A.hist[1]; R.shear[2] = A.hist[2];
+}
+
+__global__ void k_iota(unsigned *p, long long n)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = (unsigned)k;
+}
+
+__global__ void k_gather_u64(const unsigned long long *src, const unsigned *idx, long long n, unsigned long long *dst)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[idx[k]];
+}
+
+__global__ void k_gather_restore(const RestoreRec *src, const unsigned *idx, long long n, RestoreRec *dst)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[idx[k]];
+}
+
+// priority of record idx_sorted[t] (t: its rank in old-key order): 0 if its
+// key does not move, else 1 + t
+__global__ void k_zoi_prio(const unsigned long long *okey_sorted, const unsigned *idx_sorted,
+                           const unsigned long long *nkey, long long n, unsigned *prio)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const unsigned k = idx_sorted[t];
+    prio[k] = nkey[k] == okey_sorted[t] ? 0u : (unsigned)(t + 1);
 }
 
 // Fresh lists for newly placed spheres (all statuses ImpactStatus()).

@@ -48,7 +48,7 @@ class VgSceneDesc(C.Structure):
                 ("dt", C.c_double), ("t_start", C.c_double), ("t_stop", C.c_double),
                 ("t_current", C.c_double), ("g", C.c_double * 3), ("safety_factor", C.c_double),
                 ("up_iact", C.c_uint), ("record", C.c_uint), ("consultant", C.c_int),
-                ("inner_skin", C.c_double)]
+                ("inner_skin", C.c_double), ("zoi_lo", C.c_double * 3), ("zoi_hi", C.c_double * 3)]
 
 
 class VgBatchDesc(C.Structure):
@@ -64,7 +64,7 @@ assert STATUS_DTYPE.itemsize == C.sizeof(VgStatus)
 
 EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_set_history", "vg_step",
            "vg_sync", "vg_step_assigned", "vg_set_wall_rotation", "vg_get_state", "vg_time",
-           "vg_num_interactions", "vg_set_skin", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
+           "vg_num_interactions", "vg_scene_num_do", "vg_set_skin", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
            "vg_system_energy", "vg_batch_create", "vg_admit", "vg_retire", "vg_scene_status",
            "vg_scene_get_state", "vg_scene_get_active", "vg_scene_energy", "vg_scene_set_wall_rotation",
            "vg_batch_info")
@@ -114,6 +114,7 @@ def lib():
         L.vg_time.restype = C.c_double
         L.vg_time.argtypes = [vp]
         L.vg_num_interactions.argtypes = [vp, C.POINTER(C.c_longlong)]
+        L.vg_scene_num_do.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         L.vg_get_active.argtypes = [vp, C.POINTER(C.c_longlong), _U, _U, C.POINTER(VgStatus)]
         L.vg_step_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.vg_kernel_timing.argtypes = [vp, C.c_int]
@@ -186,6 +187,9 @@ def scene_desc(wa, inner_skin: float = 0.0):
     d.safety_factor = wa.safety_factor
     d.up_iact, d.record, d.consultant = wa.up_iact, wa.record, 0
     d.inner_skin = inner_skin
+    zlo, zhi = getattr(wa, "zoi_lo", np.zeros(3)), getattr(wa, "zoi_hi", np.zeros(3))
+    for k in range(3):
+        d.zoi_lo[k], d.zoi_hi[k] = float(zlo[k]), float(zhi[k])
     return d, (models, iact, arrs, model_of)
 
 
@@ -244,7 +248,15 @@ class GpuSimulation:
         a = np.ascontiguousarray(axis_dir, np.float64)
         self._check(self.L.vg_set_wall_rotation(self.h, len(ids), _up(ids), _dp(c), _dp(a), omega, t_ref))
 
+    @property
+    def n_do(self) -> int:
+        """Current number of DOs (the zone of interest erases DOs)."""
+        n = C.c_int()
+        self._check(self.L.vg_scene_num_do(self.h, 0, C.byref(n)))
+        return n.value
+
     def state(self):
+        self.n = self.n_do
         out = {k: np.zeros((self.n, 3)) for k in ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")}
         self._check(self.L.vg_get_state(self.h, *(_dp(out[k]) for k in
                                                    ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact"))))
@@ -371,7 +383,9 @@ class GpuBatch:
         return st.value, n.value, t.value
 
     def state(self, scene: int):
-        n = self.n_do[scene]
+        m = C.c_int()
+        self._check(self.L.vg_scene_num_do(self.h, int(scene), C.byref(m)))
+        n = self.n_do[scene] = m.value
         out = {k: np.zeros((n, 3)) for k in ("pos", "vel", "ox", "oz", "av")}
         self._check(self.L.vg_scene_get_state(self.h, int(scene), *(_dp(out[k]) for k in
                                                                     ("pos", "vel", "ox", "oz", "av"))))

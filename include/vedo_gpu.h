@@ -156,6 +156,12 @@ double vg_time(const vg_scene *h);
  * remaining ones in their (closed-up) DO-id order. */
 int vg_scene_num_do(vg_scene *h, int scene, int *n_do);
 
+/* DOModel index (into the description's models) of each remaining DO of a
+ * scene, in the DO-id order vg_scene_num_do counts: the DOName each DOStatus
+ * carries (DOStatus::GetDOName, src/vedo/framework/DOStatus.h:78), so that a
+ * state taken after the zone of interest erased DOs can seed a new scene. */
+int vg_scene_models(vg_scene *h, int scene, int *model_of);
+
 /* Number of interactions in the candidate list (Consultant::GetIactNum). */
 int vg_num_interactions(vg_scene *h, long long *n);
 

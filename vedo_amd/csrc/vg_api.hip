@@ -40,6 +40,7 @@ struct SceneHost {
     std::vector<unsigned> wall_ids;     // wall order -> DO id
     std::vector<int> do_to_sphere;      // DO id -> rank or -1
     std::vector<int> do_to_wall;        // DO id -> wall slot (batch-wide) or -1
+    std::vector<int> do_model;          // DO id -> DOModel index (the description's)
     int w_begin = 0, law_off = 0, n_groups = 0, model_off = 0, n_models = 0;
     double dt = 0, t = 0, t_stop = 0, g[3] = {0, 0, 0}, sf = 1.1, skin = 0, bond_reach = 0, rmin = 0;
     bool sd_only = true, single_group = true, no_bond = true;
@@ -302,6 +303,7 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
     S.n_models = d->n_models;
     S.do_to_sphere.assign(d->n_do, -1);
     S.do_to_wall.assign(d->n_do, -1);
+    S.do_model.assign(d->model_of, d->model_of + d->n_do);
     for (int i = 0; i < d->n_do; ++i) {
         const int m = d->model_of[i];
         if (m < 0 || m >= d->n_models) {
@@ -819,6 +821,11 @@ static int zoi_prepare(vg_scene *h, std::vector<ZoiPlan> &plans)
         for (int r = 0; r < n_old; ++r)
             if (!dead_by_rank[(size_t)r]) sid_new.push_back(jump[S.sphere_ids[(size_t)r]]);
         for (unsigned &w : S.wall_ids) w = jump[w];
+        std::vector<int> model_new;
+        model_new.reserve((size_t)(S.n_do - cnt[k]));
+        for (int id = 0; id < S.n_do; ++id)
+            if (!dead_do[(size_t)id]) model_new.push_back(S.do_model[(size_t)id]);
+        S.do_model.swap(model_new);
         S.n_do -= cnt[k];
         S.n = (int)sid_new.size();
         S.sphere_ids.swap(sid_new);
@@ -1911,6 +1918,15 @@ extern "C" int vg_scene_num_do(vg_scene *h, int scene, int *n_do)
     const SceneHost &S = h->sc[scene];
     if (S.state == VG_SCENE_FREE) return fail(h, VG_EINVAL, "no scene in this slot");
     *n_do = S.n_do;
+    return 0;
+}
+
+extern "C" int vg_scene_models(vg_scene *h, int scene, int *model_of)
+{
+    if (!h || !model_of || scene < 0 || scene >= h->scap) return VG_EINVAL;
+    const SceneHost &S = h->sc[scene];
+    if (S.state == VG_SCENE_FREE) return fail(h, VG_EINVAL, "no scene in this slot");
+    std::copy(S.do_model.begin(), S.do_model.end(), model_of);
     return 0;
 }
 

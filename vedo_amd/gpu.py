@@ -64,7 +64,7 @@ assert STATUS_DTYPE.itemsize == C.sizeof(VgStatus)
 
 EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_set_history", "vg_step",
            "vg_sync", "vg_step_assigned", "vg_set_wall_rotation", "vg_get_state", "vg_time",
-           "vg_num_interactions", "vg_scene_num_do", "vg_set_skin", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
+           "vg_num_interactions", "vg_scene_num_do", "vg_scene_models", "vg_set_skin", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
            "vg_system_energy", "vg_batch_create", "vg_admit", "vg_retire", "vg_scene_status",
            "vg_scene_get_state", "vg_scene_get_active", "vg_scene_energy", "vg_scene_set_wall_rotation",
            "vg_batch_info")
@@ -115,6 +115,7 @@ def lib():
         L.vg_time.argtypes = [vp]
         L.vg_num_interactions.argtypes = [vp, C.POINTER(C.c_longlong)]
         L.vg_scene_num_do.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+        L.vg_scene_models.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         L.vg_get_active.argtypes = [vp, C.POINTER(C.c_longlong), _U, _U, C.POINTER(VgStatus)]
         L.vg_step_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.vg_kernel_timing.argtypes = [vp, C.c_int]
@@ -255,6 +256,12 @@ class GpuSimulation:
         self._check(self.L.vg_scene_num_do(self.h, 0, C.byref(n)))
         return n.value
 
+    def models(self):
+        """DOModel index of each remaining DO (the DOName of its DOStatus)."""
+        out = np.zeros(self.n_do, np.int32)
+        self._check(self.L.vg_scene_models(self.h, 0, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
     def state(self):
         self.n = self.n_do
         out = {k: np.zeros((self.n, 3)) for k in ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")}
@@ -381,6 +388,14 @@ class GpuBatch:
         st, n, t = C.c_int(), C.c_longlong(), C.c_double()
         self._check(self.L.vg_scene_status(self.h, int(scene), C.byref(st), C.byref(n), C.byref(t)))
         return st.value, n.value, t.value
+
+    def models(self, scene: int):
+        """DOModel index of each remaining DO of a scene (the DOName of its DOStatus)."""
+        m = C.c_int()
+        self._check(self.L.vg_scene_num_do(self.h, int(scene), C.byref(m)))
+        out = np.zeros(m.value, np.int32)
+        self._check(self.L.vg_scene_models(self.h, int(scene), out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
 
     def state(self, scene: int):
         m = C.c_int()

@@ -509,6 +509,84 @@ def hopper_arrays(n: int, seed: int = 3, radius: float = 1e-3, density: float = 
     return _spliced(world, 0, pos, vel, np.zeros((n, 3)), up_iact, record)
 
 
+def hopper_cfg3(n: int = 4_000_000, seed: int = 6, r_small: float = 0.8e-3, r_large: float = 1.6e-3,
+                frac_small: float = 0.7, density: float = 2500.0, E: float = 5e7, nu: float = 0.25,
+                mu: float = 0.5, e: float = 0.5, outlet: float = 0.03, dt: float = 2e-6, steps: int = 20000,
+                thick: float = 2e-3, zoi_below: float = 2.0, up_iact: int = 20,
+                record: int = 10 ** 9) -> WorldArrays:
+    """Hopper discharge with a zone of interest: bimodal spheres (r_small /
+    r_large, frac_small of them small, by a seeded draw per site), two
+    DOModels in two DOGroups with ISwH between every pair of groups and
+    against the walls (E, nu, mu, e mapped as in hertz_mindlin); four
+    quasi-plates inclined at 45 degrees down to a square outlet of side
+    `outlet` at z = 0, four vertical plates above.  The spheres fill the
+    funnel and the column on a cubic lattice of 2.2 r_large (jitter
+    0.05 r_large), so the lowest fall through the outlet from the start; the
+    zone of interest ends zoi_below r_large under the outlet, and spheres
+    below it are erased at rebuilds (Consultant::CleanUp).  The hopper's
+    half-width A is sized so the filled column is about twice as tall as
+    wide."""
+    rng = SplitMix64(seed)
+    s = 2.2 * r_large
+    a = 0.5 * outlet
+    A = max(0.03, (n * s ** 3 / 16.0) ** (1.0 / 3.0)) + 2.0 * r_large + thick
+    H = A - a                                   # 45-degree funnel
+    m_in = (r_large + 0.5 * thick) * math.sqrt(2.0)   # a centre's distance from an inclined mid-plane
+    per = int((2.0 * (A - 0.5 * thick - r_large)) / s)
+    x0 = -(per - 1) * 0.5 * s
+    grid = x0 + s * np.arange(per)
+    X, Y = np.meshgrid(grid, grid, indexing="ij")
+    X, Y = X.ravel(), Y.ravel()
+    sites = []
+    count = 0
+    z = r_large + 0.5 * s
+    while count < n:
+        w = (a + z - m_in) if z < H else (A - 0.5 * thick - r_large)
+        keep = (np.abs(X) <= w) & (np.abs(Y) <= w)
+        k = int(keep.sum())
+        if k:
+            sites.append(np.stack([X[keep], Y[keep], np.full(k, z)], 1))
+            count += k
+        z += s
+    pos = np.concatenate(sites)[:n]
+    pos = pos + rng.uniform(3 * n, -0.05 * r_large, 0.05 * r_large).reshape(n, 3)
+    small = rng.uniform(n, 0.0, 1.0) < frac_small
+    Hc = float(pos[:, 2].max()) - H + 4.0 * r_large
+    m_s = DOModel("Small", "Small", "mobile", "local", density, 1.0, (0, 0, 0), "Sphere", (r_small,))
+    m_l = DOModel("Large", "Large", "mobile", "local", density, 1.0, (0, 0, 0), "Sphere", (r_large,))
+    mass_s, mass_l = m_s.analysis()[1], m_l.analysis()[1]
+    Ls = math.hypot(A - a, H)
+    d, hh = (A - a) / Ls, H / Ls
+    funnel = DOModel("Funnel", "Wall", "fixed", "global", 0.0, 1.0, (0, 0, 0), "QuasiPlate",
+                     (Ls + 2.0 * thick, thick, 2.0 * A + 2.0 * thick))
+    column = DOModel("Column", "Wall", "fixed", "global", 0.0, 1.0, (0, 0, 0), "QuasiPlate",
+                     (Hc, thick, 2.0 * A + 2.0 * thick))
+    c = 0.5 * (a + A)
+    zf = 0.5 * H
+    st = [
+        _plate("Funnel", (c, 0.0, zf), (d, 0.0, hh), (-hh, 0.0, d)),
+        _plate("Funnel", (-c, 0.0, zf), (-d, 0.0, hh), (hh, 0.0, d)),
+        _plate("Funnel", (0.0, c, zf), (0.0, d, hh), (0.0, -hh, d)),
+        _plate("Funnel", (0.0, -c, zf), (0.0, -d, hh), (0.0, hh, d)),
+        _plate("Column", (A, 0.0, H + 0.5 * Hc), (0.0, 0.0, 1.0), (-1.0, 0.0, 0.0)),
+        _plate("Column", (-A, 0.0, H + 0.5 * Hc), (0.0, 0.0, 1.0), (1.0, 0.0, 0.0)),
+        _plate("Column", (0.0, A, H + 0.5 * Hc), (0.0, 0.0, 1.0), (0.0, -1.0, 0.0)),
+        _plate("Column", (0.0, -A, H + 0.5 * Hc), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0)),
+    ]
+    iact = [IactModel("Small", "Small", "ISwH", hertz_mindlin(E, nu, mu, e, r_small, r_small, mass_s, mass_s)),
+            IactModel("Small", "Large", "ISwH", hertz_mindlin(E, nu, mu, e, r_small, r_large, mass_s, mass_l)),
+            IactModel("Large", "Large", "ISwH", hertz_mindlin(E, nu, mu, e, r_large, r_large, mass_l, mass_l)),
+            IactModel("Wall", "Small", "ISwH", hertz_mindlin(E, nu, mu, e, r_small, None, mass_s, None)),
+            IactModel("Wall", "Large", "ISwH", hertz_mindlin(E, nu, mu, e, r_large, None, mass_l, None))]
+    sp = SystemParameter(0.0, steps * dt, dt, 0.0, (0.0, 0.0, -9.81))
+    sp.zoi_lower = (0.0, 0.0, -zoi_below * r_large)
+    sp.zoi_upper = (0.0, 0.0, H + Hc + 1.0)
+    world = DOWorld(sp, [m_s, m_l, funnel, column], iact, st)
+    wa = _spliced(world, 0, pos, np.zeros((n, 3)), np.zeros((n, 3)), up_iact, record)
+    wa.model_of[:n] = np.where(small, 0, 1).astype(np.int32)
+    return wa
+
+
 def drum_radius_for(n: int, median: float = 1e-3) -> float:
     """Drum radius whose lower half holds n spheres of the median radius at
     a dense packing in a drum as long as its radius."""

@@ -15,10 +15,12 @@ properties that hold at any size:
 
 The zone's lower bound is raised into the funnel, 60 um under the centre of
 the sixth lattice layer (layers 2.2 r_large apart, centres jittered by
-+-0.05 r_large), so that erasing happens within the steps a test can afford:
-in 1200 steps of 2e-6 s a sphere falls about 28 um, and the configuration's
-own bound, 2 r under the outlet, is first crossed after many thousands of
-steps."""
++-0.05 r_large), so that erasing starts at once (a sphere falls 28 um in the first 1200
+steps of 2e-6 s); the configuration's own bound, 2 r under the outlet, is
+first crossed after many thousands of steps.  The lattice starts without
+contacts: the run goes on until the lowest layers press on the funnel
+plates, so that the restart carries contact histories (ISwH's shear force
+and Kn)."""
 import dataclasses
 import time
 
@@ -31,6 +33,8 @@ N = 4_000_000
 UP = 20
 R_LARGE = 1.6e-3
 Z_LO = R_LARGE + 5.5 * 2.2 * R_LARGE - 0.06e-3
+MIN_CONTACTS = 1000
+MAX_STEPS = 20000
 
 
 def _bits(a, b):
@@ -57,11 +61,17 @@ def test_hopper_4m_zone_determinism_restart():
     assert a.n_do == n0 + int((~sph).sum())
     b = GpuSimulation(wa)
     counts = [n0]
-    for chunk in range(6):
+    done = 0
+    # the lattice starts without contacts: the lowest layers reach the funnel
+    # plates after some thousands of steps
+    while done < MAX_STEPS and a.count_active() < MIN_CONTACTS:
         a.step(10 * UP)
         b.step(10 * UP)
+        done += 10 * UP
         counts.append(_spheres(a, wa))
-        print(f"  step {(chunk + 1) * 10 * UP}: {counts[-1]} spheres", flush=True)
+        if done % 2000 == 0:
+            print(f"  step {done}: {counts[-1]} spheres, {a.count_active()} contacts", flush=True)
+    print(f"  step {done}: {counts[-1]} spheres, {a.count_active()} contacts", flush=True)
     assert all(x >= y for x, y in zip(counts, counts[1:])), counts
     assert counts[-1] < counts[0], "the falling bed crosses the zone's bound"
     sa, sb = a.state(), b.state()
@@ -73,7 +83,7 @@ def test_hopper_4m_zone_determinism_restart():
     # restart at this step (a multiple of UpIact: a rebuild step)
     lo, hi, rec = a.active()
     print(f"  {len(lo)} active records at the restart", flush=True)
-    assert len(lo) > 0
+    assert len(lo) >= MIN_CONTACTS
     w2 = dataclasses.replace(wa, pos=sa["pos"], vel=sa["vel"], ox=sa["ox"], oz=sa["oz"], av=sa["av"],
                              model_of=a.models(), t_current=a.time)
     c = GpuSimulation(w2)

@@ -3,7 +3,10 @@
 // Per step one fused kernel (k_step) does, for every sphere: contact detection
 // against its neighbour list, the impact solver of every pair (both sides of a
 // sphere pair evaluate the pair identically), the field/external impulse and
-// DOSphere::Response.  It reads state buffer A and writes buffer B, so no
+// DOSphere::Response.  A workgroup stages its 256 spheres in LDS, settles the
+// surely separated pairs per sphere, queues the rest and solves the queue one
+// entry per thread, so that lanes stay busy whatever the spread of contact
+// counts; each sphere then adds its entries' impulses in list order.  It reads state buffer A and writes buffer B, so no
 // thread sees a half-updated neighbour.  Per-sphere impulse sums follow the
 // neighbour-list order, which the rebuild lays down in the reference's
 // IactPairTab order (LeapConsultant.h:207-444), so the FP64 sums round exactly
@@ -150,52 +153,213 @@ __device__ inline int xcd_block(int b, int nb)
     return x < full ? x * per + q : full * per + (x - full) * (per - 1) + q;
 }
 
+// One entry (owner sphere i, list half h, slot k) of the pair solves: the
+// backup of the previous step's status, the contact detection and the law,
+// the entry's new status and history, the constrained-wall share and the
+// record.  Returns the owner's impulse and angular impulse from the pair
+// (zero where the solver adds nothing) and whether the pair stays in contact
+// on its slave side.
+template <bool REC, bool SD, bool MIX>
+__device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi, double ri, d3 vi, d3 wi, double mi,
+                                   int gi, const SceneDev &S, const double4 *s_pos, const double4 *s_vel,
+                                   const double4 *s_avl, unsigned b0, unsigned nb, d3 &dI, d3 &dL)
+{
+    dI = mk(0, 0, 0);
+    dL = mk(0, 0, 0);
+    const size_t ei = eidx(h, k, a.ki, a.ld, i);
+    const unsigned ent = nt_load(a.ell.ent + ei);
+    const unsigned j = ent & PARTNER_MASK;
+    Hist hs;
+    hs.f = ent & ~PARTNER_MASK;
+    const bool had_contact = (hs.f & F_CONTACT) != 0;
+    // IactContainer::BackupImpactStatus of the previous step (IactContainer.cpp:128-131)
+    if (!(hs.f & F_NOBACKUP)) {
+        unsigned f = hs.f & ~(F_LASTCONTACT | F_LASTBOND);
+        if (hs.f & F_CONTACT) f |= F_LASTCONTACT;
+        if (hs.f & F_BOND) f |= F_LASTBOND;
+        hs.f = f;
+    }
+    hs.f &= ~F_NOBACKUP;
+    if (had_contact) load_hist<!SD>(a.ell, h, k, a.ki, a.ld, i, hs);
+    else { hs.s = mk(0, 0, 0); hs.aux = 0.0; }
+    PairOut o;
+    o.added = false;
+    RecOut r;
+    if (REC) {
+        r.point = r.dir = r.to_master = r.ang_to_master = mk(0, 0, 0);
+        r.overlap = 0.0;
+    }
+    Contact c;
+    if (j < (unsigned)a.ld) {
+        const bool loc = j - b0 < nb;
+        const double4 pj = loc ? s_pos[j - b0] : a.in.pos[j];
+        const double4 vj = loc ? s_vel[j - b0] : a.in.vel[j];
+        const double4 wj = loc ? s_avl[j - b0] : a.in.avl[j];
+        const int gj = a.single_group ? gi : a.t.models[a.in.mdl[j]].group;
+        const LawDev &law = a.t.laws[S.law_off + gi * S.n_groups + gj];
+        if (hs.f & F_SLAVE) {
+            if (MIX) {
+                ContactM cm;
+                cd_ss_m(xi, ri, ld3(pj), pj.w, cm);
+                solve_mixed<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(vj), ld3(wj), cm, S.dt, hs, o, r);
+            } else {
+                cd_ss(xi, ri, ld3(pj), pj.w, c);
+                solve<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(pj), ld3(vj), ld3(wj), c, S.dt, hs, o, r);
+            }
+            if (o.added) {
+                dI = o.imp;
+                if (!o.linear_only) dL = o.ang_a;
+            }
+        } else {
+            if (MIX) {
+                ContactM cm;
+                cd_ss_m(ld3(pj), pj.w, xi, ri, cm);
+                solve_mixed<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), vi, wi, cm, S.dt, hs, o, r);
+            } else {
+                cd_ss(ld3(pj), pj.w, xi, ri, c);
+                solve<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), xi, vi, wi, c, S.dt, hs, o, r);
+            }
+            if (o.added) {
+                dI = o.linear_only ? -o.imp : smul(-1.0, o.imp);
+                if (!o.linear_only) dL = o.ang_b;
+            }
+        }
+    } else {
+        const WallDev &w = a.walls[j - a.ld];
+        const LawDev &law = a.t.laws[S.law_off + gi * S.n_groups + w.group];
+        cd_wall(xi, ri, w, c);
+        if (MIX) {
+            ContactM cm;
+            to_mixed(c, xi, w.pos, cm);
+            solve_mixed<REC, SD>(law, mi, w.msudo, xi, vi, wi, w.vel, w.av, cm, S.dt, hs, o, r);
+        } else {
+            solve<REC, SD>(law, mi, w.msudo, xi, vi, wi, w.pos, w.vel, w.av, c, S.dt, hs, o, r);
+        }
+        if (o.added) {
+            dI = o.imp;
+            if (!o.linear_only) dL = o.ang_a;
+            if (w.constrained) {
+                // the wall's share (DiscreteObject::AddImpact on the master), added in pair order by k_cw_sum
+                const int ws = (int)(j - a.ld);
+                int p = -1;
+                if (a.cw_w[i] == ws) p = a.cw_p[i];
+                else if (a.cw_w[a.ld + i] == ws) p = a.cw_p[a.ld + i];
+                if (p >= 0) {
+                    const d3 m = o.linear_only ? -o.imp : smul(-1.0, o.imp);
+                    double *v = a.cw_val + 3 * (size_t)p;
+                    v[0] = m.x; v[1] = m.y; v[2] = m.z;
+                    a.cw_stamp[p] = a.stamp;
+                } else {
+                    atomicOr(&a.flags[3], ERR_CW);
+                }
+            }
+        }
+    }
+    if (hs.f != (ent & ~PARTNER_MASK)) nt_store(j | hs.f, a.ell.ent + ei);
+    bool slave_contact = false;
+    if (hs.f & F_CONTACT) {
+        store_hist<!SD>(a.ell, h, k, a.ki, a.ld, i, hs);
+        slave_contact = (hs.f & F_SLAVE) != 0;
+    }
+    if (REC) {
+        double *rp = a.rec + ((size_t)k * 13) * a.ld + i;
+        const double vals[13] = {r.point.x, r.point.y, r.point.z, r.dir.x, r.dir.y, r.dir.z,
+                                 r.to_master.x, r.to_master.y, r.to_master.z, r.ang_to_master.x,
+                                 r.ang_to_master.y, r.ang_to_master.z, r.overlap};
+        for (int q = 0; q < 13; ++q) rp[(size_t)q * a.ld] = vals[q];
+    }
+    return slave_contact;
+}
+
+// Exclusive prefix sum of v over the block's 256 threads; *total gets the sum.
+__device__ inline int block_scan256(int v, int *s_wsum, int &total)
+{
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_wsum[w] = x;
+    __syncthreads();
+    int base = 0;
+    for (int q = 0; q < w; ++q) base += s_wsum[q];
+    total = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+    __syncthreads();
+    return base + x - v;
+}
+
 template <bool REC, bool SD, bool MIX>
 __global__ __launch_bounds__(256) void k_step(StepArgs a)
 {
-    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    constexpr int NT = 256;   // threads per block: the launch's block size
+    constexpr int QPT = 16;   // queued entries per thread and round of the queue
+    // the block's spheres (partners inside the block are read from here) and
+    // its queue of pair solves, one per thread per pass, with their results
+    __shared__ double4 s_pos[NT], s_vel[NT], s_avl[NT];
+    __shared__ int s_mdl[NT];
+    __shared__ unsigned short s_item[NT * QPT];
+    __shared__ double s_out[6][NT];
+    __shared__ int s_wsum[4];
+    __shared__ unsigned s_ncontact;
+
+    const int t = threadIdx.x;
+    const int b0 = xcd_block(blockIdx.x, gridDim.x) * NT;
+    const int i = b0 + t;
     if (i < a.n_scenes) {
         // scene i's relist ring: clear the slot of the step after next, count this step's relist
         a.rflags[3 * i + (a.step + 2) % 3] = 0;
         if (a.rflags[3 * i + a.step % 3]) atomicAdd(a.relists, 1ull);
     }
-    if (i >= a.n) return;
-    const int mdl_i = a.in.mdl[i];
+    if (b0 >= a.n) return;   // the whole block is past the spheres (uniform)
+    const unsigned nb = (unsigned)min(NT, a.n - b0);
+    const bool in = i < a.n;
+    const int mdl_i = in ? a.in.mdl[i] : 0;
+    double4 p4 = make_double4(0.0, 0.0, 0.0, 0.0), v4 = p4, w4 = p4;
+    if (in) {
+        p4 = a.in.pos[i];
+        v4 = a.in.vel[i];
+        w4 = a.in.avl[i];
+    }
+    s_pos[t] = p4;
+    s_vel[t] = v4;
+    s_avl[t] = w4;
+    s_mdl[t] = mdl_i;
+    if (t == 0) s_ncontact = 0;
     const ModelDev &md = a.t.models[mdl_i];
     const int sc = md.scene;
     const SceneDev &S = a.scenes[sc];
-    if (S.active <= 0) {
+    const bool live = in && S.active > 0;
+    if (in && !live) {
         // a frozen scene (finished, awaiting retirement) is carried over unchanged
-        a.out.pos[i] = a.in.pos[i];
-        a.out.vel[i] = a.in.vel[i];
-        a.out.avl[i] = a.in.avl[i];
+        a.out.pos[i] = p4;
+        a.out.vel[i] = v4;
+        a.out.avl[i] = w4;
         for (int c = 0; c < 6; ++c) a.out.ori[(size_t)c * a.ld + i] = a.in.ori[(size_t)c * a.ld + i];
-        return;
     }
-    const bool do_relist = a.rflags[3 * sc + a.step % 3] != 0;
+    const d3 xi = ld3(p4);
+    const double ri = p4.w;
 
-    const double4 p4 = a.in.pos[i];
-    double4 v4 = a.in.vel[i];
-    const double4 w4 = a.in.avl[i];
-    const d3 xi = ld3(p4), vi = ld3(v4), wi = ld3(w4);
-    const double ri = p4.w, mi = w4.w;
-    const int gi = md.group;
-
-    unsigned hdr = a.ell.hdr[i];
-    int h = hdr >> 7, cnt = hdr & 127;
-    if (do_relist) {
-        const int hn = h ^ 1;
-        cnt = relist(a, i, xi, ri, h, cnt, hn, S.skin);
-        h = hn;
-        a.ell.hdr[i] = (unsigned char)((h << 7) | cnt);
-        v4.w = 0.0;
+    int h = 0, cnt = 0;
+    bool relisted = false;
+    if (live) {
+        const bool do_relist = a.rflags[3 * sc + a.step % 3] != 0;
+        const unsigned hdr = a.ell.hdr[i];
+        h = hdr >> 7;
+        cnt = hdr & 127;
+        if (do_relist) {
+            const int hn = h ^ 1;
+            cnt = relist(a, i, xi, ri, h, cnt, hn, S.skin);
+            h = hn;
+            a.ell.hdr[i] = (unsigned char)((h << 7) | cnt);
+            relisted = true;
+        }
     }
+    __syncthreads();
 
     // Pass 1: surely separated sphere pairs are settled here (Clean status,
-    // no impulse); every other entry is marked for the full solve.  Pass 2
-    // runs the solves in list order, so a wavefront executes the contact
-    // physics as many times as its busiest lane has near contacts, not as
-    // many times as its longest list has entries.
+    // no impulse); every other entry is queued for the full solve.
     unsigned long long todo = 0;
     // entries in groups of PF: the entry loads, then the partner-position
     // gathers of a group are independent and overlap in flight
@@ -209,7 +373,8 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             const unsigned j = ent[u] & PARTNER_MASK;
-            pj[u] = (a.no_bond && j < (unsigned)a.ld) ? a.in.pos[j] : make_double4(0.0, 0.0, 0.0, -1.0);
+            pj[u] = (a.no_bond && j < (unsigned)a.ld) ? (j - (unsigned)b0 < nb ? s_pos[j - b0] : a.in.pos[j])
+                                                       : make_double4(0.0, 0.0, 0.0, -1.0);
         }
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
@@ -234,144 +399,100 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
             if (full) todo |= 1ull << k;
         }
     }
+
+    // Pass 2: the block's queued entries are solved one per thread per pass
+    // (each thread takes at most QPT of its own entries per round, in list
+    // order); after each pass every owner adds its entries' impulses in list
+    // order, so the sums round exactly as a sequential walk of its list.
     d3 I = mk(0, 0, 0), L = mk(0, 0, 0);
-    unsigned n_contact = 0;   // pairs left in contact, counted on their slave side
-    while (todo) {
-        const int k = __builtin_ctzll(todo);
-        todo &= todo - 1;
-        const size_t ei = eidx(h, k, a.ki, a.ld, i);
-        const unsigned ent = nt_load(a.ell.ent + ei);
-        const unsigned j = ent & PARTNER_MASK;
-        Hist hs;
-        hs.f = ent & ~PARTNER_MASK;
-        const bool had_contact = (hs.f & F_CONTACT) != 0;
-        // IactContainer::BackupImpactStatus of the previous step (IactContainer.cpp:128-131)
-        if (!(hs.f & F_NOBACKUP)) {
-            unsigned f = hs.f & ~(F_LASTCONTACT | F_LASTBOND);
-            if (hs.f & F_CONTACT) f |= F_LASTCONTACT;
-            if (hs.f & F_BOND) f |= F_LASTBOND;
-            hs.f = f;
+    unsigned n_contact = 0;
+    unsigned long long rem = todo;
+    for (;;) {
+        unsigned long long take = rem;
+        for (int q = __popcll(take); q > QPT; --q) take &= ~(1ull << (63 - __builtin_clzll(take)));
+        const int c = __popcll(take);
+        int total = 0;
+        const int off = block_scan256(c, s_wsum, total);
+        {
+            unsigned long long tk = take;
+            for (int q = 0; q < c; ++q) {
+                const int k = __builtin_ctzll(tk);
+                tk &= tk - 1;
+                s_item[off + q] = (unsigned short)((t << 7) | (h << 6) | k);
+            }
         }
-        hs.f &= ~F_NOBACKUP;
-        if (had_contact) load_hist<!SD>(a.ell, h, k, a.ki, a.ld, i, hs);
-        else { hs.s = mk(0, 0, 0); hs.aux = 0.0; }
-        PairOut o;
-        o.added = false;
-        RecOut r;
+        rem &= ~take;
+        __syncthreads();
+        for (int r0 = 0; r0 < total; r0 += NT) {
+            const int q = r0 + t;
+            if (q < total) {
+                const unsigned it = s_item[q];
+                const int lt = (int)(it >> 7), hq = (int)((it >> 6) & 1), k = (int)(it & 63);
+                const int io = b0 + lt;
+                const double4 po = s_pos[lt], vo = s_vel[lt], wo = s_avl[lt];
+                const ModelDev &mo = a.t.models[s_mdl[lt]];
+                const SceneDev &So = a.scenes[mo.scene];
+                d3 dI, dL;
+                const bool sc_ = solve_entry<REC, SD, MIX>(a, io, hq, k, ld3(po), po.w, ld3(vo), ld3(wo), wo.w,
+                                                           mo.group, So, s_pos, s_vel, s_avl, (unsigned)b0, nb,
+                                                           dI, dL);
+                n_contact += sc_ ? 1u : 0u;
+                s_out[0][t] = dI.x; s_out[1][t] = dI.y; s_out[2][t] = dI.z;
+                s_out[3][t] = dL.x; s_out[4][t] = dL.y; s_out[5][t] = dL.z;
+            }
+            __syncthreads();
+            const int lo = max(off, r0), hi = min(off + c, r0 + NT);
+            for (int q2 = lo; q2 < hi; ++q2) {
+                const int u = q2 - r0;
+                I = I + mk(s_out[0][u], s_out[1][u], s_out[2][u]);
+                L = L + mk(s_out[3][u], s_out[4][u], s_out[5][u]);
+            }
+            __syncthreads();
+        }
+        if (!__syncthreads_or(rem != 0)) break;
+    }
+    if (n_contact) atomicAdd(&s_ncontact, n_contact);
+
+    if (live) {
+        // the sphere's own state again from LDS (not held in registers through the solves)
+        const double4 p4b = s_pos[t], v4b = s_vel[t], w4b = s_avl[t];
+        const d3 xi = ld3(p4b), vi = ld3(v4b), wi = ld3(w4b);
+        const double ri = p4b.w, mi = w4b.w;
+        const ModelDev &md = a.t.models[s_mdl[t]];
+        const SceneDev &S = a.scenes[md.scene];
+        const int sc = md.scene;
+        // DOContainer::AddFieldImpact + AddConstrainedImpact (DOContainer.cpp:20-50)
+        const d3 gdt = muls(mk(S.gx, S.gy, S.gz), S.dt);
+        I = I + muls(gdt, mi);
+        I = I + smul(S.dt, mk(md.fext[0], md.fext[1], md.fext[2]));
+        // DOSphere::Response (DOSphere.cpp:21-48)
+        const d3 dv = smul(1.0 / mi, I);
+        const d3 dav = smul(2.5 / mi / ri / ri, L);
+        const d3 dp = smul(S.dt, vi + smul(0.5, dv));
+        const d3 dw = smul(S.dt, wi + smul(0.5, dav));
+        const d3 xn = xi + dp;
+        const double *oi = a.in.ori + i;
+        const size_t LD = (size_t)a.ld;
+        d3 oxn = mk(nt_load(oi), nt_load(oi + LD), nt_load(oi + 2 * LD));
+        d3 ozn = mk(nt_load(oi + 3 * LD), nt_load(oi + 4 * LD), nt_load(oi + 5 * LD));
+        rotate_pair(oxn, ozn, dw);
+        const d3 vn = vi + dv, wn = wi + dav;
+        const double path = (relisted ? 0.0 : v4b.w) + len(dp);
+        if (path > S.relist_half) atomicOr(&a.rflags[3 * sc + (a.step + 1) % 3], 1);
+        nt_store4(make_double4(xn.x, xn.y, xn.z, ri), a.out.pos + i);
+        nt_store4(make_double4(vn.x, vn.y, vn.z, path), a.out.vel + i);
+        nt_store4(make_double4(wn.x, wn.y, wn.z, mi), a.out.avl + i);
+        double *oo = a.out.ori + i;
+        st_store(oxn.x, oo); st_store(oxn.y, oo + LD); st_store(oxn.z, oo + 2 * LD);
+        st_store(ozn.x, oo + 3 * LD); st_store(ozn.y, oo + 4 * LD); st_store(ozn.z, oo + 5 * LD);
         if (REC) {
-            r.point = r.dir = r.to_master = r.ang_to_master = mk(0, 0, 0);
-            r.overlap = 0.0;
-        }
-        Contact c;
-        if (j < (unsigned)a.ld) {
-            const double4 pj = a.in.pos[j], vj = a.in.vel[j], wj = a.in.avl[j];
-            const int gj = a.single_group ? gi : a.t.models[a.in.mdl[j]].group;
-            const LawDev &law = a.t.laws[S.law_off + gi * S.n_groups + gj];
-            if (hs.f & F_SLAVE) {
-                if (MIX) {
-                    ContactM cm;
-                    cd_ss_m(xi, ri, ld3(pj), pj.w, cm);
-                    solve_mixed<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(vj), ld3(wj), cm, S.dt, hs, o, r);
-                } else {
-                    cd_ss(xi, ri, ld3(pj), pj.w, c);
-                    solve<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(pj), ld3(vj), ld3(wj), c, S.dt, hs, o, r);
-                }
-                if (o.added) {
-                    I = I + o.imp;
-                    if (!o.linear_only) L = L + o.ang_a;
-                }
-            } else {
-                if (MIX) {
-                    ContactM cm;
-                    cd_ss_m(ld3(pj), pj.w, xi, ri, cm);
-                    solve_mixed<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), vi, wi, cm, S.dt, hs, o, r);
-                } else {
-                    cd_ss(ld3(pj), pj.w, xi, ri, c);
-                    solve<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), xi, vi, wi, c, S.dt, hs, o, r);
-                }
-                if (o.added) {
-                    I = I + (o.linear_only ? -o.imp : smul(-1.0, o.imp));
-                    if (!o.linear_only) L = L + o.ang_b;
-                }
-            }
-        } else {
-            const WallDev &w = a.walls[j - a.ld];
-            const LawDev &law = a.t.laws[S.law_off + gi * S.n_groups + w.group];
-            cd_wall(xi, ri, w, c);
-            if (MIX) {
-                ContactM cm;
-                to_mixed(c, xi, w.pos, cm);
-                solve_mixed<REC, SD>(law, mi, w.msudo, xi, vi, wi, w.vel, w.av, cm, S.dt, hs, o, r);
-            } else {
-                solve<REC, SD>(law, mi, w.msudo, xi, vi, wi, w.pos, w.vel, w.av, c, S.dt, hs, o, r);
-            }
-            if (o.added) {
-                I = I + o.imp;
-                if (!o.linear_only) L = L + o.ang_a;
-                if (w.constrained) {
-                    // the wall's share (DiscreteObject::AddImpact on the master), added in pair order by k_cw_sum
-                    const int ws = (int)(j - a.ld);
-                    int p = -1;
-                    if (a.cw_w[i] == ws) p = a.cw_p[i];
-                    else if (a.cw_w[a.ld + i] == ws) p = a.cw_p[a.ld + i];
-                    if (p >= 0) {
-                        const d3 m = o.linear_only ? -o.imp : smul(-1.0, o.imp);
-                        double *v = a.cw_val + 3 * (size_t)p;
-                        v[0] = m.x; v[1] = m.y; v[2] = m.z;
-                        a.cw_stamp[p] = a.stamp;
-                    } else {
-                        atomicOr(&a.flags[3], ERR_CW);
-                    }
-                }
-            }
-        }
-        if (hs.f != (ent & ~PARTNER_MASK)) nt_store(j | hs.f, a.ell.ent + ei);
-        if (hs.f & F_CONTACT) {
-            store_hist<!SD>(a.ell, h, k, a.ki, a.ld, i, hs);
-            n_contact += (hs.f & F_SLAVE) ? 1u : 0u;
-        }
-        if (REC) {
-            double *rp = a.rec + ((size_t)k * 13) * a.ld + i;
-            const double vals[13] = {r.point.x, r.point.y, r.point.z, r.dir.x, r.dir.y, r.dir.z,
-                                     r.to_master.x, r.to_master.y, r.to_master.z, r.ang_to_master.x,
-                                     r.ang_to_master.y, r.ang_to_master.z, r.overlap};
-            for (int q = 0; q < 13; ++q) rp[(size_t)q * a.ld] = vals[q];
+            a.rec_imp[i] = I.x; a.rec_imp[a.ld + i] = I.y; a.rec_imp[2 * a.ld + i] = I.z;
+            a.rec_imp[3 * a.ld + i] = L.x; a.rec_imp[4 * a.ld + i] = L.y; a.rec_imp[5 * a.ld + i] = L.z;
         }
     }
-    // DOContainer::AddFieldImpact + AddConstrainedImpact (DOContainer.cpp:20-50)
-    const d3 gdt = muls(mk(S.gx, S.gy, S.gz), S.dt);
-    I = I + muls(gdt, mi);
-    I = I + smul(S.dt, mk(md.fext[0], md.fext[1], md.fext[2]));
-    // DOSphere::Response (DOSphere.cpp:21-48)
-    const d3 dv = smul(1.0 / mi, I);
-    const d3 dav = smul(2.5 / mi / ri / ri, L);
-    const d3 dp = smul(S.dt, vi + smul(0.5, dv));
-    const d3 dw = smul(S.dt, wi + smul(0.5, dav));
-    const d3 xn = xi + dp;
-    const double *oi = a.in.ori + i;
-    const size_t LD = (size_t)a.ld;
-    d3 oxn = mk(nt_load(oi), nt_load(oi + LD), nt_load(oi + 2 * LD));
-    d3 ozn = mk(nt_load(oi + 3 * LD), nt_load(oi + 4 * LD), nt_load(oi + 5 * LD));
-    rotate_pair(oxn, ozn, dw);
-    const d3 vn = vi + dv, wn = wi + dav;
-    const double path = v4.w + len(dp);
-    if (path > S.relist_half) atomicOr(&a.rflags[3 * sc + (a.step + 1) % 3], 1);
-    nt_store4(make_double4(xn.x, xn.y, xn.z, ri), a.out.pos + i);
-    nt_store4(make_double4(vn.x, vn.y, vn.z, path), a.out.vel + i);
-    nt_store4(make_double4(wn.x, wn.y, wn.z, mi), a.out.avl + i);
-    double *oo = a.out.ori + i;
-    st_store(oxn.x, oo); st_store(oxn.y, oo + LD); st_store(oxn.z, oo + 2 * LD);
-    st_store(ozn.x, oo + 3 * LD); st_store(ozn.y, oo + 4 * LD); st_store(ozn.z, oo + 5 * LD);
-    if (REC) {
-        a.rec_imp[i] = I.x; a.rec_imp[a.ld + i] = I.y; a.rec_imp[2 * a.ld + i] = I.z;
-        a.rec_imp[3 * a.ld + i] = L.x; a.rec_imp[4 * a.ld + i] = L.y; a.rec_imp[5 * a.ld + i] = L.z;
-    }
-    // Wavefront sum of the contact counts (each < 128) bit by bit over the
-    // live lanes; the first live lane adds it to the wavefront's own slot.
-    unsigned long long tot = 0;
-    for (int b = 0; b < 7; ++b) tot += (unsigned long long)__popcll(__ballot((n_contact >> b) & 1u)) << b;
-    const unsigned long long live = __ballot(1);
-    if (__lane_id() == (unsigned)(__ffsll((long long)live) - 1)) a.contacts[i >> 6] += tot;
+    // the block's count of pairs left in contact (slave sides) goes to its first wavefront's slot
+    __syncthreads();
+    if (t == 0 && s_ncontact) a.contacts[b0 >> 6] += s_ncontact;
 }
 
 template __global__ void k_step<false, false, false>(StepArgs);

@@ -189,69 +189,68 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
         r.point = r.dir = r.to_master = r.ang_to_master = mk(0, 0, 0);
         r.overlap = 0.0;
     }
+    // The pair's slave (A) and master (B) inputs: the owner is A on its slave
+    // side and against walls, B on its master side.  One contact detection
+    // and one law evaluation serve every case, so a wavefront runs the solver
+    // once per pass whatever mix of sides its lanes hold.
     Contact c;
+    ContactM cm;
+    d3 xa, va, wa, xb, vb, wb;
+    double ma, mb;
+    const LawDev *law;
+    bool owner_a = true;
+    const WallDev *w = nullptr;
     if (j < (unsigned)a.ld) {
         const bool loc = j - b0 < nb;
         const double4 pj = loc ? s_pos[j - b0] : a.in.pos[j];
         const double4 vj = loc ? s_vel[j - b0] : a.in.vel[j];
         const double4 wj = loc ? s_avl[j - b0] : a.in.avl[j];
         const int gj = a.single_group ? gi : a.t.models[a.in.mdl[j]].group;
-        const LawDev &law = a.t.laws[S.law_off + gi * S.n_groups + gj];
-        if (hs.f & F_SLAVE) {
-            if (MIX) {
-                ContactM cm;
-                cd_ss_m(xi, ri, ld3(pj), pj.w, cm);
-                solve_mixed<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(vj), ld3(wj), cm, S.dt, hs, o, r);
-            } else {
-                cd_ss(xi, ri, ld3(pj), pj.w, c);
-                solve<REC, SD>(law, mi, wj.w, xi, vi, wi, ld3(pj), ld3(vj), ld3(wj), c, S.dt, hs, o, r);
-            }
-            if (o.added) {
-                dI = o.imp;
-                if (!o.linear_only) dL = o.ang_a;
-            }
-        } else {
-            if (MIX) {
-                ContactM cm;
-                cd_ss_m(ld3(pj), pj.w, xi, ri, cm);
-                solve_mixed<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), vi, wi, cm, S.dt, hs, o, r);
-            } else {
-                cd_ss(ld3(pj), pj.w, xi, ri, c);
-                solve<REC, SD>(law, wj.w, mi, ld3(pj), ld3(vj), ld3(wj), xi, vi, wi, c, S.dt, hs, o, r);
-            }
-            if (o.added) {
-                dI = o.linear_only ? -o.imp : smul(-1.0, o.imp);
-                if (!o.linear_only) dL = o.ang_b;
-            }
-        }
+        law = &a.t.laws[S.law_off + gi * S.n_groups + gj];
+        owner_a = (hs.f & F_SLAVE) != 0;
+        const d3 xj = ld3(pj), vjj = ld3(vj), wjj = ld3(wj);
+        xa = owner_a ? xi : xj;
+        va = owner_a ? vi : vjj;
+        wa = owner_a ? wi : wjj;
+        ma = owner_a ? mi : wj.w;
+        xb = owner_a ? xj : xi;
+        vb = owner_a ? vjj : vi;
+        wb = owner_a ? wjj : wi;
+        mb = owner_a ? wj.w : mi;
+        const double ra = owner_a ? ri : pj.w, rb = owner_a ? pj.w : ri;
+        if (MIX) cd_ss_m(xa, ra, xb, rb, cm);
+        else cd_ss(xa, ra, xb, rb, c);
     } else {
-        const WallDev &w = a.walls[j - a.ld];
-        const LawDev &law = a.t.laws[S.law_off + gi * S.n_groups + w.group];
-        cd_wall(xi, ri, w, c);
-        if (MIX) {
-            ContactM cm;
-            to_mixed(c, xi, w.pos, cm);
-            solve_mixed<REC, SD>(law, mi, w.msudo, xi, vi, wi, w.vel, w.av, cm, S.dt, hs, o, r);
-        } else {
-            solve<REC, SD>(law, mi, w.msudo, xi, vi, wi, w.pos, w.vel, w.av, c, S.dt, hs, o, r);
-        }
-        if (o.added) {
+        w = &a.walls[j - a.ld];
+        law = &a.t.laws[S.law_off + gi * S.n_groups + w->group];
+        cd_wall(xi, ri, *w, c);
+        xa = xi; va = vi; wa = wi; ma = mi;
+        xb = w->pos; vb = w->vel; wb = w->av; mb = w->msudo;
+        if (MIX) to_mixed(c, xi, w->pos, cm);
+    }
+    if (MIX) solve_mixed<REC, SD>(*law, ma, mb, xa, va, wa, vb, wb, cm, S.dt, hs, o, r);
+    else solve<REC, SD>(*law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r);
+    if (o.added) {
+        if (owner_a) {
             dI = o.imp;
             if (!o.linear_only) dL = o.ang_a;
-            if (w.constrained) {
-                // the wall's share (DiscreteObject::AddImpact on the master), added in pair order by k_cw_sum
-                const int ws = (int)(j - a.ld);
-                int p = -1;
-                if (a.cw_w[i] == ws) p = a.cw_p[i];
-                else if (a.cw_w[a.ld + i] == ws) p = a.cw_p[a.ld + i];
-                if (p >= 0) {
-                    const d3 m = o.linear_only ? -o.imp : smul(-1.0, o.imp);
-                    double *v = a.cw_val + 3 * (size_t)p;
-                    v[0] = m.x; v[1] = m.y; v[2] = m.z;
-                    a.cw_stamp[p] = a.stamp;
-                } else {
-                    atomicOr(&a.flags[3], ERR_CW);
-                }
+        } else {
+            dI = o.linear_only ? -o.imp : smul(-1.0, o.imp);
+            if (!o.linear_only) dL = o.ang_b;
+        }
+        if (w && w->constrained) {
+            // the wall's share (DiscreteObject::AddImpact on the master), added in pair order by k_cw_sum
+            const int ws = (int)(j - a.ld);
+            int p = -1;
+            if (a.cw_w[i] == ws) p = a.cw_p[i];
+            else if (a.cw_w[a.ld + i] == ws) p = a.cw_p[a.ld + i];
+            if (p >= 0) {
+                const d3 m = o.linear_only ? -o.imp : smul(-1.0, o.imp);
+                double *v = a.cw_val + 3 * (size_t)p;
+                v[0] = m.x; v[1] = m.y; v[2] = m.z;
+                a.cw_stamp[p] = a.stamp;
+            } else {
+                atomicOr(&a.flags[3], ERR_CW);
             }
         }
     }

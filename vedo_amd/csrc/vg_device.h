@@ -13,8 +13,8 @@
 // spheres is one contiguous 256-byte run:
 //   hdr  u8  [LD]          count (7 bits) | current half (bit 7)
 //   ent  u32 [2][KI][LD]   partner (26 bits: sphere slot, or LD + wall slot) | status flags
-//   hs   f64 [2][KI][LD][3] tangential force history xyz (24 bytes per entry)
-//   ha   f64 [2][KI][LD]    Kn (ISwH) / initial velocity (ISwHertz)
+//   hd   double4 [2][KI][LD] tangential force history xyz, Kn (ISwH) /
+//                            initial velocity (ISwHertz); 32 bytes per entry
 // Every sphere keeps its own copy of each pair's ImpactStatus; the two copies
 // are updated by the same instruction sequence from the same inputs, so they
 // stay bit-identical and no thread ever reads another thread's history.

@@ -15,8 +15,8 @@ struct Geo {   // one buffer set of sphere state (see vg_device.h)
 struct Ell {
     unsigned char *hdr;
     unsigned *ent;    // [2][KI][LD]
-    double *hs;       // [2][KI][LD][3]: shear force xyz (24 bytes per entry)
-    double *ha;       // [2][KI][LD]: Kn (ISwH) / initial velocity (ISwHertz); zero for the spring-dashpot laws
+    double4 *hd;      // [2][KI][LD]: shear force xyz, then Kn (ISwH) / initial velocity (ISwHertz),
+                      // zero for the spring-dashpot laws; 32 bytes per entry, two 16-byte accesses
 };
 
 struct Tables {

@@ -61,24 +61,22 @@ template <class T> __device__ inline void st_store(T v, T *p)
 }
 
 // History of entry (h, k) of sphere i: the shear force always, the fourth
-// word only where a Hertz law may have set it (AUX).
+// word only where a Hertz law may have set it (AUX; without it the word is
+// read as zero and written as zero).
 template <bool AUX = true>
 __device__ inline void load_hist(const Ell &e, int h, int k, int ki, int n, int i, Hist &hs)
 {
-    const size_t x = eidx(h, k, ki, n, i);
-    const double *p = e.hs + 3 * x;
-    hs.s = mk(nt_load(p), nt_load(p + 1), nt_load(p + 2));
-    hs.aux = AUX ? nt_load(e.ha + x) : 0.0;
+    const double2 *p = reinterpret_cast<const double2 *>(e.hd + eidx(h, k, ki, n, i));
+    const double2 u = p[0], v = p[1];
+    hs.s = mk(u.x, u.y, v.x);
+    hs.aux = AUX ? v.y : 0.0;
 }
 template <bool AUX = true>
 __device__ inline void store_hist(const Ell &e, int h, int k, int ki, int n, int i, const Hist &hs)
 {
-    const size_t x = eidx(h, k, ki, n, i);
-    double *p = e.hs + 3 * x;
-    nt_store(hs.s.x, p);
-    nt_store(hs.s.y, p + 1);
-    nt_store(hs.s.z, p + 2);
-    if (AUX) nt_store(hs.aux, e.ha + x);
+    double2 *p = reinterpret_cast<double2 *>(e.hd + eidx(h, k, ki, n, i));
+    p[0] = make_double2(hs.s.x, hs.s.y);
+    p[1] = make_double2(hs.s.z, AUX ? hs.aux : 0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -180,8 +178,9 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
         hs.f = f;
     }
     hs.f &= ~F_NOBACKUP;
-    if (had_contact) load_hist<!SD>(a.ell, h, k, a.ki, a.ld, i, hs);
-    else { hs.s = mk(0, 0, 0); hs.aux = 0.0; }
+    // read with the entry word (most queued entries are contacts); meaningful only for contacts
+    load_hist<!SD>(a.ell, h, k, a.ki, a.ld, i, hs);
+    if (!had_contact) { hs.s = mk(0, 0, 0); hs.aux = 0.0; }
     PairOut o;
     o.added = false;
     RecOut r;

@@ -4,14 +4,15 @@
 // ImpactSolvers of the reference), with this split of precisions:
 //   FP64: every decision that sets an ImpactStatus flag -- the overlap
 //         depth of the contact detection (so the active set is the FP64
-//         one, bit for bit), the normal-bond test -- the stored history
-//         (shear force, Kn / initial velocity) and its update, the
-//         impulse sums and the integration (in the step kernel);
-//   FP32: contact normal and lever arms, relative velocity, the normal and
+//         one, bit for bit), the normal-bond test -- the relative velocity
+//         at the contact point, the stored history (shear force, Kn /
+//         initial velocity) and its update, the impulse sums and the
+//         integration (in the step kernel);
+//   FP32: contact normal and lever arms for the torques, the normal and
 //         damping forces, the tangential increment, the friction cap and
 //         the impulse and torque of the pair.
 // The per-pair force error against the FP64 path is a few FP32 rounding
-// units of the pair's force, far below 1e-5 relative.
+// units of the pair's force terms.
 #pragma once
 
 #include "vg_device.h"
@@ -49,7 +50,8 @@ struct ContactM {
     double depth;
     bool active;
     f3 n, ra, rb;
-    d3 n64;   // the normal in FP64, for the split of the centre-velocity difference
+    d3 n64;         // the normal in FP64
+    d3 ra64, rb64;  // the lever arms in FP64, for the relative velocity
 };
 
 // CDSphere_Sphere (CDSphere2Sphere.cpp:14-60) for slave a, master b: the
@@ -67,6 +69,8 @@ __device__ inline void cd_ss_m(d3 xa, double ra, d3 xb, double rb, ContactM &c)
     c.n = tof(c.n64);
     c.ra = (float)(ra - 0.5 * c.depth) * c.n;
     c.rb = c.ra - imf;
+    c.ra64 = smul(ra - 0.5 * c.depth, c.n64);
+    c.rb64 = c.ra64 - im;
 }
 
 // A contact detected in FP64 (sphere-wall) in the mixed form.
@@ -76,24 +80,25 @@ __device__ inline void to_mixed(const Contact &c, d3 xa, d3 xb, ContactM &m)
     m.active = c.active;
     m.n64 = c.n;
     m.n = tof(c.n);
-    m.ra = tof(c.p - xa);
-    m.rb = tof(c.p - xb);
+    m.ra64 = c.p - xa;
+    m.rb64 = c.p - xb;
+    m.ra = tof(m.ra64);
+    m.rb = tof(m.rb64);
 }
 
 // Relative velocity at the contact point, v = vA - vB + wA x rA - wB x rB,
-// split into its normal part vn = (v.n) n and tangential part vs = v - vn.
-// The centre-velocity difference is split in FP64: its tangential part can
-// be far smaller than its normal part, and an FP32 rounding of the whole
-// difference would swamp it.  The rotational part is formed in FP32.
-__device__ inline void rel_vel(d3 va, d3 vb, f3 wa, f3 wb, const ContactM &c, f3 &vn, f3 &vs)
+// split into its normal part vn = (v.n) n and tangential part vs = v - vn,
+// all in FP64 and rounded to FP32 at the end.  The terms cancel: in a
+// rolling contact the two spheres' rotational parts nearly annul each other,
+// and in a resting one the centre velocities do, so FP32 terms would leave
+// the small tangential velocity (whose direction sets a sliding contact's
+// friction) with a relative error far above FP32's.
+__device__ inline void rel_vel(d3 va, d3 vb, d3 wa, d3 wb, const ContactM &c, f3 &vn, f3 &vs)
 {
-    const d3 dv = va - vb;
-    const double dn = dot(dv, c.n64);
-    const f3 dvt = tof(dv - smul(dn, c.n64));
-    const f3 rot = crossf(wa, c.ra) - crossf(wb, c.rb);
-    const float rn = dotf(rot, c.n);
-    vn = ((float)dn + rn) * c.n;
-    vs = dvt + (rot - rn * c.n);
+    const d3 v = ((va - vb) + cross(wa, c.ra64)) - cross(wb, c.rb64);
+    const double dn = dot(v, c.n64);
+    vn = (float)dn * c.n;
+    vs = tof(v - smul(dn, c.n64));
 }
 
 // ShearForceRotation (ISwModels.cpp:110-123) of the FP64 history onto the
@@ -122,7 +127,6 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
     const double depth = c.depth;
     const float df = (float)depth, dtf = (float)dt;
     const f3 n = c.n;
-    const f3 waf = tof(wa), wbf = tof(wb);
     if (!SD && law == LAW_HERTZ) {   // ISwHertz (ISwHertz.cpp:20-94)
         o.linear_only = true;
         if (!c.active) {
@@ -132,7 +136,7 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
         f3 fn = mkf(0.0f, 0.0f, 0.0f);
         if (depth > 0.0) {
             f3 vn, vs;
-            rel_vel(va, vb, waf, wbf, c, vn, vs);
+            rel_vel(va, vb, wa, wb, c, vn, vs);
             if (!(h.f & F_CONTACT)) {
                 h.f |= F_CONTACT;
                 h.aux = (double)lenf(vn);
@@ -162,7 +166,7 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
             ra = c.ra;
             rb = c.rb;
             f3 vn, vs;
-            rel_vel(va, vb, waf, wbf, c, vn, vs);
+            rel_vel(va, vb, wa, wb, c, vn, vs);
             double kn;
             if (!(h.f & F_CONTACT)) {
                 // first contact: Kn from the approach speed, kept in the history (FP64)
@@ -220,7 +224,7 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
         ra = c.ra;
         rb = c.rb;
         f3 vn, vs;
-        rel_vel(va, vb, waf, wbf, c, vn, vs);
+        rel_vel(va, vb, wa, wb, c, vn, vs);
         fn = -(knf * df) * n - cn * vn;
         const float fmax = fc * knf * df;
         if (law == BTSDBF) {

@@ -44,6 +44,7 @@ struct SceneHost {
     int w_begin = 0, law_off = 0, n_groups = 0, model_off = 0, n_models = 0;
     double dt = 0, t = 0, t_stop = 0, g[3] = {0, 0, 0}, sf = 1.1, skin = 0, bond_reach = 0, rmin = 0;
     bool sd_only = true, single_group = true, no_bond = true;
+    bool h_only = true;                 // every law present is ISwH
     bool has_constrained = false;       // a wall moves under its contacts (DOConstrained*)
     int zon = 0;                        // zone of interest: bounded axes (bits), bounds
     double zlo[3] = {0, 0, 0}, zhi[3] = {0, 0, 0};
@@ -78,7 +79,7 @@ struct vg_scene {
     std::vector<WallDev> walls_h;       // host copy of wall kinematic modes (kin)
     std::vector<char> wall_used;
     // batch-wide properties of the placed scenes
-    bool sd_only = true;
+    bool sd_only = true, h_only = true;
     int single_group = 1, no_bond = 1;
     int precision = VG_PRECISION_FP64;   // vg_set_precision
     unsigned up_iact = 1;
@@ -265,11 +266,13 @@ template <class T> static int grow_table(vg_scene *h, T **d, int *cap, const std
 static void update_batch_flags(vg_scene *h)
 {
     h->sd_only = true;
+    h->h_only = true;
     h->single_group = 1;
     h->no_bond = 1;
     for (const SceneHost &S : h->sc) {
         if (S.state != VG_SCENE_RUNNING && S.state != VG_SCENE_FINISHED) continue;
         h->sd_only = h->sd_only && S.sd_only;
+        h->h_only = h->h_only && S.h_only;
         h->single_group = h->single_group && S.single_group;
         h->no_bond = h->no_bond && S.no_bond;
     }
@@ -359,6 +362,7 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
         if ((L == VG_BSDBF || L == VG_BSDB2F) && d->iact[k].p[4] != 0.0 && d->iact[k].p[0] != 0.0)
             bond_reach = std::max(bond_reach, std::fabs(d->iact[k].p[4] / d->iact[k].p[0]));
         if (L == VG_H || L == VG_HERTZ) S.sd_only = false;
+        if (L != 0 && L != VG_H) S.h_only = false;
         if ((L == VG_BSDBF || L == VG_BSDB2F || L == VG_BTSDBF) && d->iact[k].p[4] != 0.0) S.no_bond = false;
     }
     {
@@ -493,6 +497,14 @@ static int admit(vg_scene *h, const vg_scene_desc *d, long long steps, int *slot
         L.law = d->iact[k].law;
         L.pad = 0;
         for (int p = 0; p < 8; ++p) L.p[p] = d->iact[k].p[p];
+        if (L.law == VG_H) {
+            // ISwH's first-contact stiffness kn = K |vn|^0.4 (ISwH.cpp:90-103): the law's
+            // factor K, evaluated here in the reference's left-to-right order with the
+            // host's pow, so that the device evaluates one pow per first contact
+            const double knh = L.p[0], meff = L.p[1], betan = L.p[4];
+            L.p[6] = betan < 0.99 ? 0.957832746294131 / (1.0 - betan * betan) * pow(meff, 0.2) * pow(knh, 0.8)
+                                  : 479.155951122633 * pow(meff, 0.2) * pow(knh, 0.8);
+        }
         h->laws_h.push_back(L);
     }
     for (int &m : S.mdl0) m += S.model_off;
@@ -1500,21 +1512,18 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
     const bool kt = 2 * (h->kev_used + 1) <= (int)h->kev.size();
     if (kt) HIPCHK(hipEventRecord(h->kev[2 * h->kev_used], h->stream));
     const dim3 grid(nblk(std::max(h->n, h->scap), 256)), blk(256);
-    if (h->precision == VG_PRECISION_MIXED) {
-        if (h->sd_only) {
-            if (rec) hipLaunchKernelGGL((k_step<true, true, true>), grid, blk, 0, h->stream, a);
-            else hipLaunchKernelGGL((k_step<false, true, true>), grid, blk, 0, h->stream, a);
-        } else {
-            if (rec) hipLaunchKernelGGL((k_step<true, false, true>), grid, blk, 0, h->stream, a);
-            else hipLaunchKernelGGL((k_step<false, false, true>), grid, blk, 0, h->stream, a);
-        }
-    } else if (h->sd_only) {
-        if (rec) hipLaunchKernelGGL((k_step<true, true, false>), grid, blk, 0, h->stream, a);
-        else hipLaunchKernelGGL((k_step<false, true, false>), grid, blk, 0, h->stream, a);
-    } else {
-        if (rec) hipLaunchKernelGGL((k_step<true, false, false>), grid, blk, 0, h->stream, a);
-        else hipLaunchKernelGGL((k_step<false, false, false>), grid, blk, 0, h->stream, a);
-    }
+    // kernel specialisation: law set of the placed scenes, record path, precision
+    const int ls = h->sd_only ? LS_SD : (h->h_only ? LS_H : LS_ALL);
+    const bool mix = h->precision == VG_PRECISION_MIXED;
+    using KFn = void (*)(StepArgs);
+    static const KFn kfn[2][3][2] = {
+        {{k_step<false, LS_SD, false>, k_step<false, LS_SD, true>},
+         {k_step<false, LS_H, false>, k_step<false, LS_H, true>},
+         {k_step<false, LS_ALL, false>, k_step<false, LS_ALL, true>}},
+        {{k_step<true, LS_SD, false>, k_step<true, LS_SD, true>},
+         {k_step<true, LS_H, false>, k_step<true, LS_H, true>},
+         {k_step<true, LS_ALL, false>, k_step<true, LS_ALL, true>}}};
+    hipLaunchKernelGGL(kfn[rec ? 1 : 0][ls][mix ? 1 : 0], grid, blk, 0, h->stream, a);
     if (kt) {
         HIPCHK(hipEventRecord(h->kev[2 * h->kev_used + 1], h->stream));
         h->kev_used++;

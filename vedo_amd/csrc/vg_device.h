@@ -195,11 +195,14 @@ __device__ inline d3 shear_rotation(d3 f, d3 d)
     return smul(len(f), dir(d));
 }
 
-// One ImpactSolver::NextStep for slave A and master B.  SD: only the
-// spring-dashpot family is present in the scene (smaller kernel).
+// Law sets a step kernel is specialised for: only the spring-dashpot family,
+// only ISwH, or all five (smaller kernels where fewer laws are present).
+enum { LS_SD = 0, LS_H = 1, LS_ALL = 2 };
+
+// One ImpactSolver::NextStep for slave A and master B, for the laws of set LS.
 // ISwBSDBF/ISwBSDB2F/ISwBtSDBF (ISwBSDBF.cpp:26-139, ISwBSDB2F.cpp:26-149,
 // ISwBtSDBF.cpp:26-137), ISwH (ISwH.cpp:30-156), ISwHertz (ISwHertz.cpp:20-94).
-template <bool REC, bool SD>
+template <bool REC, int LS>
 __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va, d3 wa, d3 xb, d3 vb,
                              d3 wb, const Contact &c, double dt, Hist &h, PairOut &o, RecOut &r)
 {
@@ -208,7 +211,7 @@ __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va
     const int law = L.law;
     const double depth = c.depth;
     const d3 n = c.n;
-    if (!SD && law == LAW_HERTZ) {
+    if (LS == LS_ALL && law == LAW_HERTZ) {
         o.linear_only = true;
         if (!c.active) {
             st_clean(h);
@@ -236,7 +239,7 @@ __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va
         }
         return;
     }
-    if (!SD && law == LAW_H) {
+    if (LS != LS_SD && law == LAW_H) {
         const double knh = L.p[0], meff = L.p[1], ksc1 = L.p[2], ksc2 = L.p[3], betan = L.p[4], fk = L.p[5];
         if (!c.active) {
             st_clean(h);
@@ -251,12 +254,9 @@ __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va
             d3 vs = rv - vn;
             double kn;
             if (!(h.f & F_CONTACT)) {
+                // p[6]: the law's factor of |vn|^0.4, set on the host
                 if (len(vn) == 0.0) kn = knh;
-                else if (betan < 0.99)
-                    kn = 0.957832746294131 / (1.0 - betan * betan) * pow(meff, 0.2) * pow(knh, 0.8) *
-                         pow(len(vn), 0.4);
-                else
-                    kn = 479.155951122633 * pow(meff, 0.2) * pow(knh, 0.8) * pow(len(vn), 0.4);
+                else kn = L.p[6] * pow(len(vn), 0.4);
                 h.aux = kn;
                 h.f |= F_CONTACT;
             } else {
@@ -283,6 +283,7 @@ __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va
         }
         return;
     }
+    if constexpr (LS == LS_H) return;
     // spring-dashpot family
     const double kn = L.p[0], ks = L.p[1], bn = L.p[4], bs = L.p[5], fc = L.p[6];
     double me;   // ISwModels::CriticalDamping (ISwModels.cpp:88-108)

@@ -138,7 +138,7 @@ struct ActiveOut {
     double rec[13];
 };
 
-template <bool REC, bool SD, bool MIX> __global__ void k_step(StepArgs a);
+template <bool REC, int LS, bool MIX> __global__ void k_step(StepArgs a);
 __global__ void k_walls(WallDev *walls, int nw, const SceneDev *scenes, double *stime, const double *assigned,
                         const WallDev *ref, const WallKin *kin, const double *cw_imp, int *rflags, int step);
 __global__ void k_cw_count(int n, int ld, int ko, const unsigned *cnt, const unsigned *nbr, const WallDev *walls,

@@ -157,7 +157,7 @@ __device__ inline int xcd_block(int b, int nb)
 // record.  Returns the owner's impulse and angular impulse from the pair
 // (zero where the solver adds nothing) and whether the pair stays in contact
 // on its slave side.
-template <bool REC, bool SD, bool MIX>
+template <bool REC, int LS, bool MIX>
 __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi, double ri, d3 vi, d3 wi, double mi,
                                    int gi, const SceneDev &S, const double4 *s_pos, const double4 *s_vel,
                                    const double4 *s_avl, unsigned b0, unsigned nb, d3 &dI, d3 &dL)
@@ -179,7 +179,7 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
     }
     hs.f &= ~F_NOBACKUP;
     // read with the entry word (most queued entries are contacts); meaningful only for contacts
-    load_hist<!SD>(a.ell, h, k, a.ki, a.ld, i, hs);
+    load_hist<LS != LS_SD>(a.ell, h, k, a.ki, a.ld, i, hs);
     if (!had_contact) { hs.s = mk(0, 0, 0); hs.aux = 0.0; }
     PairOut o;
     o.added = false;
@@ -227,8 +227,8 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
         xb = w->pos; vb = w->vel; wb = w->av; mb = w->msudo;
         if (MIX) to_mixed(c, xi, w->pos, cm);
     }
-    if (MIX) solve_mixed<REC, SD>(*law, ma, mb, xa, va, wa, vb, wb, cm, S.dt, hs, o, r);
-    else solve<REC, SD>(*law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r);
+    if (MIX) solve_mixed<REC, LS>(*law, ma, mb, xa, va, wa, vb, wb, cm, S.dt, hs, o, r);
+    else solve<REC, LS>(*law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r);
     if (o.added) {
         if (owner_a) {
             dI = o.imp;
@@ -256,7 +256,7 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
     if (hs.f != (ent & ~PARTNER_MASK)) nt_store(j | hs.f, a.ell.ent + ei);
     bool slave_contact = false;
     if (hs.f & F_CONTACT) {
-        store_hist<!SD>(a.ell, h, k, a.ki, a.ld, i, hs);
+        store_hist<LS != LS_SD>(a.ell, h, k, a.ki, a.ld, i, hs);
         slave_contact = (hs.f & F_SLAVE) != 0;
     }
     if (REC) {
@@ -288,8 +288,10 @@ __device__ inline int block_scan256(int v, int *s_wsum, int &total)
     return base + x - v;
 }
 
-template <bool REC, bool SD, bool MIX>
-__global__ __launch_bounds__(256) void k_step(StepArgs a)
+// 3 workgroups per CU (12 wavefronts), as the LDS allows: the register
+// budget is capped at 168 VGPRs, the larger law sets spill a few registers
+template <bool REC, int LS, bool MIX>
+__global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
 {
     constexpr int NT = 256;   // threads per block: the launch's block size
     constexpr int QPT = 16;   // queued entries per thread and round of the queue
@@ -431,7 +433,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
                 const ModelDev &mo = a.t.models[s_mdl[lt]];
                 const SceneDev &So = a.scenes[mo.scene];
                 d3 dI, dL;
-                const bool sc_ = solve_entry<REC, SD, MIX>(a, io, hq, k, ld3(po), po.w, ld3(vo), ld3(wo), wo.w,
+                const bool sc_ = solve_entry<REC, LS, MIX>(a, io, hq, k, ld3(po), po.w, ld3(vo), ld3(wo), wo.w,
                                                            mo.group, So, s_pos, s_vel, s_avl, (unsigned)b0, nb,
                                                            dI, dL);
                 n_contact += sc_ ? 1u : 0u;
@@ -493,14 +495,18 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
     if (t == 0 && s_ncontact) a.contacts[b0 >> 6] += s_ncontact;
 }
 
-template __global__ void k_step<false, false, false>(StepArgs);
-template __global__ void k_step<true, false, false>(StepArgs);
-template __global__ void k_step<false, true, false>(StepArgs);
-template __global__ void k_step<true, true, false>(StepArgs);
-template __global__ void k_step<false, false, true>(StepArgs);
-template __global__ void k_step<true, false, true>(StepArgs);
-template __global__ void k_step<false, true, true>(StepArgs);
-template __global__ void k_step<true, true, true>(StepArgs);
+template __global__ void k_step<false, LS_SD, false>(StepArgs);
+template __global__ void k_step<true, LS_SD, false>(StepArgs);
+template __global__ void k_step<false, LS_H, false>(StepArgs);
+template __global__ void k_step<true, LS_H, false>(StepArgs);
+template __global__ void k_step<false, LS_ALL, false>(StepArgs);
+template __global__ void k_step<true, LS_ALL, false>(StepArgs);
+template __global__ void k_step<false, LS_SD, true>(StepArgs);
+template __global__ void k_step<true, LS_SD, true>(StepArgs);
+template __global__ void k_step<false, LS_H, true>(StepArgs);
+template __global__ void k_step<true, LS_H, true>(StepArgs);
+template __global__ void k_step<false, LS_ALL, true>(StepArgs);
+template __global__ void k_step<true, LS_ALL, true>(StepArgs);
 
 // Wall responses after the spheres' (DOFixedQuasiPlate.h:16-19,
 // DOOrbitalQuasiPlate.cpp:18-34) and kinematic prescriptions

@@ -117,7 +117,7 @@ __device__ inline d3 shear_rotation_m(d3 hs, f3 vs)
 // from the tangential force avoids the FP32 cancellation of ra x fn.  A
 // wall's lever arm (master with zero sudo-mass) takes the whole force.
 // xa: the slave's centre, for the recorded contact point only.
-template <bool REC, bool SD>
+template <bool REC, int LS>
 __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa, d3 va, d3 wa, d3 vb, d3 wb,
                                    const ContactM &c, double dt, Hist &h, PairOut &o, RecOut &r)
 {
@@ -127,7 +127,7 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
     const double depth = c.depth;
     const float df = (float)depth, dtf = (float)dt;
     const f3 n = c.n;
-    if (!SD && law == LAW_HERTZ) {   // ISwHertz (ISwHertz.cpp:20-94)
+    if (LS == LS_ALL && law == LAW_HERTZ) {   // ISwHertz (ISwHertz.cpp:20-94)
         o.linear_only = true;
         if (!c.active) {
             st_clean(h);
@@ -155,7 +155,7 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
         }
         return;
     }
-    if (!SD && law == LAW_H) {   // ISwH (ISwH.cpp:30-156)
+    if (LS != LS_SD && law == LAW_H) {   // ISwH (ISwH.cpp:30-156)
         const double knh = L.p[0], meff = L.p[1], betan = L.p[4];
         if (!c.active) {
             st_clean(h);
@@ -172,10 +172,7 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
                 // first contact: Kn from the approach speed, kept in the history (FP64)
                 const double vl = (double)lenf(vn);
                 if (vl == 0.0) kn = knh;
-                else if (betan < 0.99)
-                    kn = 0.957832746294131 / (1.0 - betan * betan) * pow(meff, 0.2) * pow(knh, 0.8) * pow(vl, 0.4);
-                else
-                    kn = 479.155951122633 * pow(meff, 0.2) * pow(knh, 0.8) * pow(vl, 0.4);
+                else kn = L.p[6] * pow(vl, 0.4);   // p[6]: the law's factor, set on the host
                 h.aux = kn;
                 h.f |= F_CONTACT;
             } else {
@@ -205,6 +202,7 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
         }
         return;
     }
+    if constexpr (LS == LS_H) return;
     // spring-dashpot family (ISwBSDBF.cpp:26-139, ISwBSDB2F.cpp:26-149, ISwBtSDBF.cpp:26-137)
     const double kn = L.p[0], bn = L.p[4], bs = L.p[5];
     const float knf = (float)kn, ksf = (float)L.p[1], fc = (float)L.p[6];

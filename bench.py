@@ -153,6 +153,10 @@ def batched_boxes(count, steps, up_iact, seed=4, loop=True):
     skin, tune = tune_skin(b, 1e-3, up_iact, factors=(0.25, 0.5, 1.0, 2.0))
     log(f"[bench] batched: skin {skin * 1e3:.2f} mm ({({f'{k * 1e3:.2f}': round(v * 1e3, 2) for k, v in tune.items()})} "
         f"ms/step)")
+    # from the next rebuild on, each scene's skin follows its speed (the tuned
+    # skin as the floor): the spheres fall faster as the run goes on
+    skin_cap = max(4.0 * skin, 1e-3)
+    b.set_skin_auto(skin, skin_cap)
     periods = 4 * up_iact // up_iact
     while True:
         b.step(up_iact)
@@ -185,6 +189,7 @@ def batched_boxes(count, steps, up_iact, seed=4, loop=True):
             if k in failed:    # the same scene blows up alone: skipped in both
                 continue
             sim = GpuSimulation(wa)
+            sim.set_skin_auto(skin, skin_cap)   # the batch's skin policy
             try:
                 sim.step(steps)
                 sim.state()

@@ -171,6 +171,12 @@ int vg_num_interactions(vg_scene *h, long long *n);
  * reference walks its whole candidate list every step. */
 int vg_set_skin(vg_scene *h, double skin);
 double vg_get_skin(const vg_scene *h);
+/* Skin per scene from its speed: at every rebuild each scene's skin becomes
+ * 2.2 times the path its fastest sphere can cover in a rebuild period (speed
+ * and gravity), clamped to [floor, cap], so that relists between rebuilds
+ * stay rare while slow scenes keep short lists.  Results do not depend on it.
+ * vg_set_skin switches back to one fixed skin.  No reference counterpart. */
+int vg_set_skin_auto(vg_scene *h, double floor, double cap);
 /* Arithmetic of the pair solve.  VG_PRECISION_FP64 (the default) is the
  * reference's: every operation in FP64, in its order, so results are the
  * reference's bits.  VG_PRECISION_MIXED keeps in FP64 the contact depth (so

@@ -46,6 +46,9 @@ class FakeBatch:
     def set_wall_rotation(self, *a):
         pass
 
+    def set_skin_auto(self, floor, cap):
+        assert 0 < floor <= cap
+
     def step(self, k):
         for _ in range(k):
             for s in self.sc.values():

@@ -82,6 +82,8 @@ struct vg_scene {
     bool sd_only = true, h_only = true;
     int single_group = 1, no_bond = 1;
     int precision = VG_PRECISION_FP64;   // vg_set_precision
+    bool skin_auto = false;             // vg_set_skin_auto: skins follow each scene's speed
+    double skin_floor = 0.0, skin_cap = 0.0;
     unsigned up_iact = 1;
     unsigned long long round = 0, step_count = 0;
     long long rebuilds = 0;
@@ -869,6 +871,11 @@ static int rebuild(vg_scene *h)
 {
     hipStream_t st = h->stream;
     PhaseClock pc;
+    if (pc.on) {
+        // debugging: the steps queued since the last rebuild, timed apart from the rebuild
+        HIPCHK(hipStreamSynchronize(st));
+        pc.mark("steps before (synced)");
+    }
     // finished scenes: keep the result, retire
     bool retiring = false;
     for (int k = 0; k < h->scap; ++k) {
@@ -1008,6 +1015,16 @@ static int rebuild(vg_scene *h)
             const double dtu = h->up_iact * S.dt;
             const double glen = olen(S.g);
             const double zr = S.sf * (2.0 * rmax + dtu * vmax + 0.5 * dtu * dtu * glen);
+            if (h->skin_auto) {
+                // twice the fastest sphere's path over the coming rebuild period (with a
+                // margin), so that a relist between rebuilds stays the exception
+                double sk = 2.2 * (dtu * vmax + 0.5 * dtu * dtu * glen);
+                sk = std::min(std::max(sk, h->skin_floor), h->skin_cap);
+                sk = std::max(sk, S.bond_reach * 1.01);
+                h->sc[k].skin = sk;
+                h->sdev[k].skin = sk;
+                h->sdev[k].relist_half = 0.5 * sk * (1.0 - 1e-9);
+            }
             unsigned ncx = (unsigned)std::ceil((xmax - xmin) / zr); ncx = std::max(ncx, 1u);
             unsigned ncy = (unsigned)std::ceil((ymax - ymin) / zr); ncy = std::max(ncy, 1u);
             unsigned ncz = (unsigned)std::ceil((zmax - zmin) / zr); ncz = std::max(ncz, 1u);
@@ -1816,9 +1833,19 @@ extern "C" int vg_num_interactions(vg_scene *h, long long *n)
     return 0;
 }
 
+extern "C" int vg_set_skin_auto(vg_scene *h, double floor, double cap)
+{
+    if (!h || !(floor >= 0.0) || !(cap >= floor)) return VG_EINVAL;
+    h->skin_auto = true;
+    h->skin_floor = floor;
+    h->skin_cap = cap;
+    return 0;
+}
+
 extern "C" int vg_set_skin(vg_scene *h, double skin)
 {
     if (!h || !(skin >= 0.0)) return VG_EINVAL;
+    h->skin_auto = false;
     HIPCHK(hipStreamSynchronize(h->stream));
     // every scene takes the skin (at least its bond reach) and relists at the next step
     for (int k = 0; k < h->scap; ++k) {

@@ -64,7 +64,7 @@ assert STATUS_DTYPE.itemsize == C.sizeof(VgStatus)
 
 EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_set_history", "vg_step",
            "vg_sync", "vg_step_assigned", "vg_set_wall_rotation", "vg_get_state", "vg_time",
-           "vg_num_interactions", "vg_scene_num_do", "vg_scene_models", "vg_set_skin", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
+           "vg_num_interactions", "vg_scene_num_do", "vg_scene_models", "vg_set_skin", "vg_set_skin_auto", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
            "vg_system_energy", "vg_batch_create", "vg_admit", "vg_retire", "vg_scene_status",
            "vg_scene_get_state", "vg_scene_get_active", "vg_scene_energy", "vg_scene_set_wall_rotation",
            "vg_batch_info")
@@ -120,6 +120,7 @@ def lib():
         L.vg_step_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.vg_kernel_timing.argtypes = [vp, C.c_int]
         L.vg_set_skin.argtypes = [vp, C.c_double]
+        L.vg_set_skin_auto.argtypes = [vp, C.c_double, C.c_double]
         L.vg_get_skin.restype = C.c_double
         L.vg_get_skin.argtypes = [vp]
         L.vg_set_precision.argtypes = [vp, C.c_int]
@@ -302,6 +303,10 @@ class GpuSimulation:
     def set_skin(self, skin: float):
         self._check(self.L.vg_set_skin(self.h, float(skin)))
 
+    def set_skin_auto(self, floor: float, cap: float):
+        """Skin per scene from its speed at every rebuild (vg_set_skin_auto)."""
+        self._check(self.L.vg_set_skin_auto(self.h, float(floor), float(cap)))
+
     @property
     def skin(self) -> float:
         return self.L.vg_get_skin(self.h)
@@ -432,6 +437,10 @@ class GpuBatch:
 
     def set_skin(self, skin: float):
         self._check(self.L.vg_set_skin(self.h, float(skin)))
+
+    def set_skin_auto(self, floor: float, cap: float):
+        """Skin per scene from its speed at every rebuild (vg_set_skin_auto)."""
+        self._check(self.L.vg_set_skin_auto(self.h, float(floor), float(cap)))
 
     def set_precision(self, mode: int):
         self._check(self.L.vg_set_precision(self.h, int(mode)))

@@ -32,6 +32,9 @@ SCENE_FAILED = 6   # the scene blew up and left the batch (VG_SCENE_FAILED)
 # Device bytes one sphere slot of a batch takes (two state buffers, two
 # neighbour lists of `ki` entries with history, candidate list, rebuild
 # scratch); the footprint of a scene is n * slot_bytes.
+SKIN_FLOOR, SKIN_CAP = 1e-4, 1e-3   # neighbour-list skin bounds of the served scenes (m)
+
+
 def slot_bytes(ki: int = 32, ko: int = 160) -> int:
     state = 2 * (3 * 32 + 48 + 8)
     lists = 2 * (1 + 2 * ki * 4 + 2 * ki * 32)
@@ -130,6 +133,9 @@ class Server:
             batch = GpuBatch(capacity, max_walls=max_walls, max_scenes=max_scenes, up_iact=up_iact,
                              neighbour_capacity=neighbour_capacity)
         self.batch = batch
+        # each scene's neighbour-list skin follows its speed (0.1 to 1 mm for
+        # these millimetre spheres): few relists between rebuilds
+        self.batch.set_skin_auto(SKIN_FLOOR, SKIN_CAP)
         self.capacity = capacity
         self.up_iact = up_iact
         self.keep_results = keep_results
@@ -202,6 +208,7 @@ def serve_one_by_one(requests: List[Request], clock: Callable[[], float] = time.
         if wait > 0:
             time.sleep(wait)
         sim = GpuSimulation(r.wa)
+        sim.set_skin_auto(SKIN_FLOOR, SKIN_CAP)   # the server's skin policy
         try:
             if r.walls is not None:
                 sim.set_wall_rotation(r.walls, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 2.0, r.wa.t_current)

@@ -255,6 +255,32 @@ int vg_scene_status(vg_scene *h, int scene, int *state, long long *steps_done, d
 int vg_scene_get_state(vg_scene *h, int scene, double *pos, double *vel, double *ox, double *oz, double *av);
 int vg_scene_get_active(vg_scene *h, int scene, long long *n, unsigned *lo, unsigned *hi, vg_status *st);
 int vg_scene_energy(vg_scene *h, int scene, double *out);
+
+/* Knight analyses (the knight tool, src/vedo/knight) of a scene's current
+ * state on the device (slot 0 for a one-scene handle).  box = xmin, xmax,
+ * ymin, ymax, zmin, zmax; a sphere belongs to it when its centre lies within
+ * the box widened by its radius (strict comparisons).
+ *
+ * CalcCoordinateNum::computeAvgCoordinateNum
+ * (src/vedo/knight/CalcCoordinateNum.cpp:8-77): the mean number of spheres
+ * whose centre distance is at most the sum of the radii, over the box's
+ * spheres (NaN for none).  Touching pairs are read from the candidate list:
+ * every pair of sphere groups needs an IactModel (VG_EUNSUPPORTED otherwise). */
+int vg_scene_coordination_number(vg_scene *h, int scene, const double box[6], double *avg);
+/* CalcDensity::computeDensity (src/vedo/knight/CalcDensity.cpp:12-72): the
+ * percentage of n_points random points of the box (njr::RandomGenerator,
+ * src/njr/RandomGenerator.cpp, seeded with `seed`; the reference's first
+ * generator in a process uses 999 and RandomHitTestNum is 20000) that lie in
+ * a sphere of the box. */
+int vg_scene_packing_density(vg_scene *h, int scene, const double box[6], int n_points, int seed, double *percent);
+/* CalculateGranularTemperature (src/vedo/knight/Knight.cpp:1162-1225) of the
+ * spheres of DOModel `model` (the description's index) inside a GSRectangle
+ * (src/vedo/common/GSRectangle.cpp:15-30: centre, axes Ox and Oz, width along
+ * Ox, length along Oz x Ox, height along Oz).  out[5] = count, |mean v|,
+ * T_v, |mean w|, T_w.  The sums run in a fixed order of their own, not the
+ * reference's DO-id order: results agree to rounding (1e-12 relative). */
+int vg_scene_granular_temperature(vg_scene *h, int scene, int model, const double center[3], const double ox[3],
+                                  const double oz[3], const double size[3], double out[5]);
 /* vg_set_wall_rotation for one scene of a batch (also before placement). */
 int vg_scene_set_wall_rotation(vg_scene *h, int scene, int n, const unsigned *ids, const double *axis_point,
                                const double *axis_dir, double omega, double t_ref);

@@ -82,6 +82,11 @@ def lib():
         L.vo_time.restype = C.c_double
         L.vo_time.argtypes = [C.c_void_p]
         L.vo_system_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.vo_coordination_number.restype = C.c_double
+        L.vo_coordination_number.argtypes = [C.c_void_p, D]
+        L.vo_packing_density.restype = C.c_double
+        L.vo_packing_density.argtypes = [C.c_void_p, D, C.c_int, C.c_int]
+        L.vo_granular_temperature.argtypes = [C.c_void_p, C.c_int, D, D, D, D, D]
         for f in ("vo_num_interactions", "vo_num_candidates", "vo_num_active", "vo_num_do"):
             getattr(L, f).argtypes = [C.c_void_p]
         L.vo_get_ids.argtypes = [C.c_void_p, U]
@@ -207,6 +212,24 @@ class OracleSim:
         out = np.zeros(16)
         self.L.vo_system_energy(self.h, _dp(out))
         return energy_dict(out)
+
+    def coordination_number(self, box):
+        """CalcCoordinateNum::computeAvgCoordinateNum over box (xmin xmax ymin ymax zmin zmax)."""
+        return self.L.vo_coordination_number(self.h, _dp(np.ascontiguousarray(box, np.float64)))
+
+    def packing_density(self, box, n_points: int = 20000, seed: int = 999):
+        """CalcDensity::computeDensity over box (percent), njr generator seeded with `seed`."""
+        return self.L.vo_packing_density(self.h, _dp(np.ascontiguousarray(box, np.float64)), int(n_points),
+                                         int(seed))
+
+    def granular_temperature(self, model: int, center, size, ox=(1.0, 0.0, 0.0), oz=(0.0, 0.0, 1.0)):
+        """CalculateGranularTemperature of one model over a GSRectangle:
+        (count, |mean v|, T_v, |mean w|, T_w)."""
+        out = np.zeros(5)
+        arr = [np.ascontiguousarray(x, np.float64) for x in (center, ox, oz, size)]
+        self.L.vo_granular_temperature(self.h, int(model), _dp(arr[0]), _dp(arr[1]), _dp(arr[2]), _dp(arr[3]),
+                                       _dp(out))
+        return out
 
     def interactions(self):
         n = self.L.vo_num_interactions(self.h)

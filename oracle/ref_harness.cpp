@@ -10,7 +10,12 @@
 // SimMediator::Run(const std::map<_uint_t,DOStatus>&) (SimMediator.cpp:1121).
 //
 // usage: vedo_ref_harness scene.txt out.bin nsteps dump_step[,dump_step...]
-//                         [--assigned statuses.bin]
+//                         [--assigned statuses.bin] [--metrics requests.txt results.txt]
+// requests.txt, one per line, evaluated on the final state (DOWorld synced):
+//   coord xmin xmax ymin ymax zmin zmax     CalcCoordinateNum::computeAvgCoordinateNum
+//   density xmin xmax ymin ymax zmin zmax   CalcDensity::computeDensity (njr generator
+//                                           reseeded with 999 before each request)
+// results.txt: one line of %.17g values per request.
 // statuses.bin: int count, count DO ids, then per step count rows of 15
 // doubles (pos, vel, Ox, Oz, w); a row whose first value is NaN is not
 // assigned at that step.
@@ -23,6 +28,9 @@
 #include <vedo/framework/DOWorld.h>
 #include <vedo/framework/IactRecordTab.h>
 #include <vedo/framework/SimMediator.h>
+#include <vedo/knight/CalcCoordinateNum.h>
+#include <vedo/knight/CalcDensity.h>
+#include <njr/RandomGenerator.h>
 
 #include <cstdio>
 #include <cstdlib>
@@ -160,6 +168,12 @@ int main(int argc, char **argv)
     std::ifstream assigned;
     int nassigned = 0;
     std::vector<unsigned> assigned_ids;
+    std::string metrics_in, metrics_out;
+    for (int k = 5; k < argc; ++k)
+        if (std::string(argv[k]) == "--metrics" && k + 2 < argc) {
+            metrics_in = argv[k + 1];
+            metrics_out = argv[k + 2];
+        }
     for (int k = 5; k < argc; ++k)
         if (std::string(argv[k]) == "--assigned" && k + 1 < argc) {
             assigned.open(argv[k + 1], std::ios::binary);
@@ -236,5 +250,27 @@ int main(int argc, char **argv)
         if (di < dumps.size() && dumps[di] == k) { dump(k); di++; }
     }
     out.close();
+    if (!metrics_in.empty()) {
+        // the knight tool reads a DOWorld: sync it with the container's statuses
+        world->UpdateDOStatus(sm->GetDOContainer()->GetDOStatus());
+        std::ifstream req(metrics_in);
+        std::ofstream res(metrics_out);
+        res.precision(17);
+        std::string kind;
+        while (req >> kind) {
+            if (kind == "coord" || kind == "density") {
+                double b[6];
+                for (double &x : b) x = rd(req);
+                double v;
+                if (kind == "coord") {
+                    v = CalcCoordinateNum().computeAvgCoordinateNum(world, b[0], b[1], b[2], b[3], b[4], b[5]);
+                } else {
+                    njr::RandomGenerator reseed(999);
+                    v = CalcDensity().computeDensity(world, b[0], b[1], b[2], b[3], b[4], b[5]);
+                }
+                res << v << "\n";
+            }
+        }
+    }
     return 0;
 }

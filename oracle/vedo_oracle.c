@@ -1321,6 +1321,147 @@ void vo_system_energy(const vo_sim *s, double *out)
     for (int k = 0; k < 16; ++k) out[k] = o[k];
 }
 
+/* ---- Knight analyses of a DOWorld state (src/vedo/knight) ---- */
+
+/* The spheres whose centre lies within the box widened by their radius, all
+ * comparisons strict (CalcCoordinateNum.cpp:24-49, CalcDensity.cpp:20-43). */
+static int knight_in_box(const vo_sim *s, int i, const double *b)
+{
+    const vo_model *m = &s->models[s->model_of[i]];
+    if (m->shape != VO_SPHERE) return 0;
+    const double r = m->attr[0];
+    const v3 p = s->pos[i];
+    return p.x < b[1] + r && p.x > b[0] - r && p.y < b[3] + r && p.y > b[2] - r && p.z < b[5] + r &&
+           p.z > b[4] - r;
+}
+
+/* CalcCoordinateNum::computeAvgCoordinateNum (CalcCoordinateNum.cpp:8-77):
+ * for every sphere of the box, the other spheres of the whole world whose
+ * centre distance |x_i - x_j| is at most r_i + r_j; total over the box's
+ * spheres divided by their number (NaN for none). */
+double vo_coordination_number(const vo_sim *s, const double *box)
+{
+    unsigned total = 0, n = 0;
+    for (int i = 0; i < s->n_do; ++i) {
+        if (!knight_in_box(s, i, box)) continue;
+        ++n;
+        const double ri = s->models[s->model_of[i]].attr[0];
+        unsigned count = 0;
+        for (int j = 0; j < s->n_do; ++j) {
+            if (j == i) continue;
+            const vo_model *mj = &s->models[s->model_of[j]];
+            if (mj->shape != VO_SPHERE) continue;
+            if (len(sub(s->pos[i], s->pos[j])) <= ri + mj->attr[0]) count++;
+        }
+        total += count;
+    }
+    return total / (double)n;
+}
+
+/* njr::RandomGenerator (RandomGenerator.cpp:70-126): a subtractive
+ * lagged-Fibonacci generator modulo 1e9 over 55 words; sprand(seed), then
+ * 165 draws discarded.  GeneratingDouble(lo, hi) = lprand()/1e9*(hi-lo)+lo. */
+typedef struct { int a, b, arr[55]; } njr_rng;
+static int njr_next(njr_rng *g)
+{
+    if ((g->a--) == 0) g->a = 54;
+    if ((g->b--) == 0) g->b = 54;
+    int t = g->arr[g->a] - g->arr[g->b];
+    if (t < 0) t += 1000000000;
+    g->arr[g->a] = t;
+    return t;
+}
+static void njr_seed(njr_rng *g, int seed)
+{
+    int last = seed, next = 1;
+    g->arr[0] = seed;
+    for (int i = 1; i < 55; i++) {
+        const int ii = (21 * i) % 55;
+        g->arr[ii] = next;
+        next = last - next;
+        if (next < 0) next += 1000000000;
+        last = g->arr[ii];
+    }
+    g->a = 0;
+    g->b = 24;
+    for (int i = 0; i < 165; i++) njr_next(g);
+}
+static double njr_double(njr_rng *g, double lo, double hi)
+{
+    if (lo > hi) { const double t = lo; lo = hi; hi = t; }
+    else if (lo == hi) return hi;
+    return (double)njr_next(g) / 1000000000 * (hi - lo) + lo;
+}
+
+/* CalcDensity::computeDensity (CalcDensity.cpp:12-72): n_points random
+ * points of the box (x, y, z drawn in that order from a generator seeded
+ * with `seed`; the reference's first generator of a process uses 999); the
+ * percentage of points inside a sphere of the box, formed in float as the
+ * reference does (hit / (float)n * 100.0). */
+double vo_packing_density(const vo_sim *s, const double *box, int n_points, int seed)
+{
+    njr_rng g;
+    njr_seed(&g, seed);
+    unsigned hit = 0;
+    for (int k = 0; k < n_points; ++k) {
+        const double x = njr_double(&g, box[0], box[1]);
+        const double y = njr_double(&g, box[2], box[3]);
+        const double z = njr_double(&g, box[4], box[5]);
+        const v3 p = mk(x, y, z);
+        for (int j = 0; j < s->n_do; ++j) {
+            if (!knight_in_box(s, j, box)) continue;
+            if (len(sub(p, s->pos[j])) <= s->models[s->model_of[j]].attr[0]) {
+                hit++;
+                break;
+            }
+        }
+    }
+    const float f = (float)hit / (float)n_points;
+    return (double)f * 100.0;
+}
+
+/* CalculateGranularTemperature (Knight.cpp:1162-1225) over a GSRectangle
+ * (GSRectangle.cpp:15-30: centre c, axes Ox, Oz, width/length/height along
+ * Ox, Oz x Ox, Oz): the DOs of model `model` inside it, in DO-id order; their
+ * mean velocity and angular velocity, and the mean squared deviations / 3.
+ * out[5]: count, |mean v|, T_v, |mean w|, T_w. */
+void vo_granular_temperature(const vo_sim *s, int model, const double *c, const double *ox, const double *oz,
+                             const double *size, double *out)
+{
+    const v3 C = ld(c), Ox = ld(ox), Oz = ld(oz), Oy = cross(Oz, Ox);
+    v3 av = mk(0, 0, 0), aw = mk(0, 0, 0);
+    unsigned n = 0;
+    char *in = calloc(s->n_do ? s->n_do : 1, 1);
+    for (int i = 0; i < s->n_do; ++i) {
+        if (s->model_of[i] != model) continue;
+        const v3 lp = sub(s->pos[i], C);
+        const double lx = dot(lp, Ox), ly = dot(lp, Oy), lz = dot(lp, Oz);
+        if (fabs(lx) <= 0.5 * size[0] && fabs(ly) <= 0.5 * size[1] && fabs(lz) <= 0.5 * size[2]) {
+            av = add(av, s->vel[i]);
+            aw = add(aw, s->av[i]);
+            in[i] = 1;
+            n++;
+        }
+    }
+    if (n != 0) {
+        av = smul(1.0 / (double)n, av);
+        aw = smul(1.0 / (double)n, aw);
+    }
+    double tv = 0.0, tw = 0.0;
+    for (int i = 0; i < s->n_do; ++i) {
+        if (!in[i]) continue;
+        const v3 dv = sub(s->vel[i], av), dw = sub(s->av[i], aw);
+        tv += dot(dv, dv);
+        tw += dot(dw, dw);
+    }
+    if (n != 0) {
+        tv = tv / (double)n / 3.0;
+        tw = tw / (double)n / 3.0;
+    }
+    free(in);
+    out[0] = (double)n; out[1] = len(av); out[2] = tv; out[3] = len(aw); out[4] = tw;
+}
+
 int vo_num_interactions(const vo_sim *s) { return s->n_ia; }
 void vo_get_interactions(const vo_sim *s, unsigned *slave, unsigned *master)
 {

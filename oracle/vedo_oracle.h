@@ -124,6 +124,17 @@ void vo_cleanup_stats(const vo_sim *s, long long *out);
  * w min, number of mobile DOs. */
 void vo_system_energy(const vo_sim *s, double *out);
 
+/* Knight analyses of the current state: CalcCoordinateNum::
+ * computeAvgCoordinateNum (CalcCoordinateNum.cpp:8-77) and CalcDensity::
+ * computeDensity (CalcDensity.cpp:12-72; njr::RandomGenerator seeded with
+ * `seed`) over box = xmin xmax ymin ymax zmin zmax; CalculateGranularTemperature
+ * (Knight.cpp:1162-1225) of one model over a GSRectangle (centre, Ox, Oz,
+ * width, length, height), out[5] = count, |mean v|, T_v, |mean w|, T_w. */
+double vo_coordination_number(const vo_sim *s, const double *box);
+double vo_packing_density(const vo_sim *s, const double *box, int n_points, int seed);
+void vo_granular_temperature(const vo_sim *s, int model, const double *c, const double *ox, const double *oz,
+                             const double *size, double *out);
+
 /* Unit-level entry points used by closed-form tests. */
 typedef struct { double depth; double n[3]; double p[3]; int active; } vo_contact;
 void vo_cd_sphere_sphere(const double *xa, double ra, const double *xb, double rb, vo_contact *c);

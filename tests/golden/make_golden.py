@@ -168,11 +168,34 @@ def piston_world(w, density=100.0, force=-0.02, overlap=0.02):
     return w
 
 
-def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE):
+# knight analyses on the final state of these cases (CalcCoordinateNum,
+# CalcDensity): regions from the spheres' final bounding box
+METRIC_CASES = ("box_bsdbf", "box_h", "drum_kin", "box_piston")
+
+
+def metric_regions(state, sphere):
+    """(kind, box) requests: the spheres' bounding box, its lower half and its
+    middle third, as xmin xmax ymin ymax zmin zmax."""
+    p = state[sphere, 0]
+    lo, hi = p.min(0), p.max(0)
+    whole = np.array([lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]])
+    lower = whole.copy()
+    lower[5] = 0.5 * (lo[2] + hi[2])
+    mid = np.array([lo[0] + (hi[0] - lo[0]) / 3, hi[0] - (hi[0] - lo[0]) / 3, lo[1] + (hi[1] - lo[1]) / 3,
+                    hi[1] - (hi[1] - lo[1]) / 3, lo[2] + (hi[2] - lo[2]) / 3, hi[2] - (hi[2] - lo[2]) / 3])
+    return [("coord", whole), ("coord", lower), ("coord", mid), ("density", whole), ("density", lower)]
+
+
+def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE, metrics=None):
     with tempfile.TemporaryDirectory() as td:
         scene = os.path.join(td, "scene.txt")
         write_vscene(wa, scene, consultant)
         args = [HARNESS, scene, os.path.join(td, "out.bin"), str(nsteps), ",".join(map(str, dumps))]
+        if metrics is not None:
+            with open(os.path.join(td, "metrics.txt"), "w") as f:
+                for kind, box in metrics:
+                    f.write(kind + " " + " ".join(repr(float(x)) for x in box) + "\n")
+            args += ["--metrics", os.path.join(td, "metrics.txt"), os.path.join(td, "metrics_out.txt")]
         if assigned is not None:
             ids, st = assigned
             with open(os.path.join(td, "assigned.bin"), "wb") as f:
@@ -182,6 +205,11 @@ def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE):
             args += ["--assigned", os.path.join(td, "assigned.bin")]
         subprocess.run(args, cwd=td, check=True, stdout=subprocess.DEVNULL)
         d = parse_dumps(os.path.join(td, "out.bin"))
+        values = None
+        if metrics is not None:
+            with open(os.path.join(td, "metrics_out.txt")) as f:
+                values = np.array([float(x) for x in f.read().split()])
+            assert len(values) == len(metrics)
     arrays = dict(
         consultant=np.int32(consultant), nsteps=np.int32(nsteps), dumps=np.array(dumps, np.int32),
         model_shape=wa.model_shape, model_behavior=wa.model_behavior, model_scope=wa.model_scope,
@@ -201,6 +229,11 @@ def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE):
         arrays[f"d{k}_state"] = rec["state"]
         arrays[f"d{k}_iact"] = rec["iact"]
         arrays[f"d{k}_status"] = rec["status"]
+    if metrics is not None:
+        # knight analyses of the final state (after nsteps)
+        arrays["metric_kind"] = np.array([0 if k == "coord" else 1 for k, _ in metrics], np.int32)
+        arrays["metric_box"] = np.array([b for _, b in metrics], np.float64)
+        arrays["metric_value"] = values
     np.savez_compressed(os.path.join(outdir, f"{name}.npz"), **arrays)
     return d
 
@@ -214,6 +247,11 @@ def main():
         if only and name not in only:
             continue
         d = run_case(name, wa, cons, nsteps, dumps, assigned)
+        if name in METRIC_CASES:
+            assert dumps[-1] == nsteps
+            sphere = wa.model_shape[wa.model_of] == 1
+            req = metric_regions(d[-1]["state"], sphere)
+            d = run_case(name, wa, cons, nsteps, dumps, assigned, metrics=req)
         nact = [int(r["iact"]["active"].sum()) for r in d]
         print(f"{name}: {wa.n_do} DOs, {len(d)} dumps, interactions {[len(r['iact']) for r in d]}, active {nact}")
 

@@ -41,6 +41,8 @@ struct SceneHost {
     std::vector<int> do_to_sphere;      // DO id -> rank or -1
     std::vector<int> do_to_wall;        // DO id -> wall slot (batch-wide) or -1
     std::vector<int> do_model;          // DO id -> DOModel index (the description's)
+    std::vector<char> model_is_sphere;  // per DOModel of the description
+    bool all_sphere_laws = true;        // an IactModel for every pair of sphere groups
     int w_begin = 0, law_off = 0, n_groups = 0, model_off = 0, n_models = 0;
     double dt = 0, t = 0, t_stop = 0, g[3] = {0, 0, 0}, sf = 1.1, skin = 0, bond_reach = 0, rmin = 0;
     bool sd_only = true, single_group = true, no_bond = true;
@@ -366,6 +368,19 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
         if (L == VG_H || L == VG_HERTZ) S.sd_only = false;
         if (L != 0 && L != VG_H) S.h_only = false;
         if ((L == VG_BSDBF || L == VG_BSDB2F || L == VG_BTSDBF) && d->iact[k].p[4] != 0.0) S.no_bond = false;
+    }
+    {
+        // which groups hold spheres, and does every pair of them interact
+        std::vector<char> sg((size_t)d->n_groups, 0);
+        S.model_is_sphere.assign((size_t)d->n_models, 0);
+        for (int m = 0; m < d->n_models; ++m)
+            if (d->models[m].shape == VG_SPHERE) {
+                S.model_is_sphere[(size_t)m] = 1;
+                sg[(size_t)d->models[m].group] = 1;
+            }
+        for (int a = 0; a < d->n_groups; ++a)
+            for (int b = 0; b < d->n_groups; ++b)
+                if (sg[(size_t)a] && sg[(size_t)b] && d->iact[a * d->n_groups + b].law == 0) S.all_sphere_laws = false;
     }
     {
         int g0 = -1;
@@ -2055,6 +2070,201 @@ extern "C" int vg_system_energy(vg_scene *h, double *out)
     if (!h || !out) return VG_EINVAL;
     if (!scene0(h)) return fail(h, VG_EINVAL, "vg_system_energy needs a one-scene handle (vg_scene_energy)");
     return system_energy(h, 0, out);
+}
+
+// ---------------------------------------------------------------------------
+// Knight analyses (src/vedo/knight) of a scene's current state
+// ---------------------------------------------------------------------------
+
+static int check_slot(vg_scene *h, int scene);
+
+static int knight_scene(vg_scene *h, int scene, const SceneHost **out)
+{
+    if (int rc = check_slot(h, scene)) return rc;
+    const SceneHost &S = h->sc[scene];
+    if (S.state != VG_SCENE_RUNNING && S.state != VG_SCENE_FINISHED)
+        return fail(h, VG_EINVAL, "scene is not on the device");
+    *out = &S;
+    return 0;
+}
+
+static KnightBox knight_box(const double *b)
+{
+    KnightBox B;
+    B.lo[0] = b[0]; B.hi[0] = b[1];
+    B.lo[1] = b[2]; B.hi[1] = b[3];
+    B.lo[2] = b[4]; B.hi[2] = b[5];
+    return B;
+}
+
+extern "C" int vg_scene_coordination_number(vg_scene *h, int scene, const double *box, double *avg)
+{
+    const SceneHost *S = nullptr;
+    if (!box || !avg) return VG_EINVAL;
+    if (int rc = knight_scene(h, scene, &S)) return rc;
+    if (!S->all_sphere_laws)
+        return fail(h, VG_EUNSUPPORTED, "coordination number needs an IactModel for every pair of sphere groups");
+    unsigned long long *acc = nullptr, c[2] = {0, 0};
+    if (dalloc(h, &acc, 2)) return VG_ENOMEM;
+    int rc = 0;
+    if (hipMemsetAsync(acc, 0, sizeof(c), h->stream) != hipSuccess) rc = fail(h, VG_EHIP, "memset failed");
+    if (!rc && S->n > 0)
+        hipLaunchKernelGGL(k_coordination, dim3(nblk(S->n, 256)), dim3(256), 0, h->stream, h->geo[h->cur], S->seg_begin,
+                           S->seg_end, h->ld, (const unsigned *)h->ocount, (const unsigned *)h->onbr, knight_box(box),
+                           acc);
+    if (!rc && (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(c, acc, sizeof(c), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess))
+        rc = fail(h, VG_EHIP, "coordination kernel failed");
+    (void)hipFree(acc);
+    if (rc) return rc;
+    *avg = (double)(unsigned)c[0] / (double)c[1];   // total / number of the box's spheres (NaN for none)
+    return 0;
+}
+
+// njr::RandomGenerator (RandomGenerator.cpp:70-126), the knight tool's
+// generator: a subtractive lagged-Fibonacci sequence modulo 1e9 over 55
+// words, seeded by sprand(seed) and 165 discarded draws.
+namespace {
+struct NjrRng {
+    int a = 0, b = 24, arr[55];
+    explicit NjrRng(int seed)
+    {
+        int last = seed, next = 1;
+        arr[0] = seed;
+        for (int i = 1; i < 55; i++) {
+            const int ii = (21 * i) % 55;
+            arr[ii] = next;
+            next = last - next;
+            if (next < 0) next += 1000000000;
+            last = arr[ii];
+        }
+        a = 0;
+        b = 24;
+        for (int i = 0; i < 165; i++) draw();
+    }
+    int draw()
+    {
+        if ((a--) == 0) a = 54;
+        if ((b--) == 0) b = 54;
+        int t = arr[a] - arr[b];
+        if (t < 0) t += 1000000000;
+        arr[a] = t;
+        return t;
+    }
+    double uniform(double lo, double hi)   // GeneratingDouble
+    {
+        if (lo > hi) std::swap(lo, hi);
+        else if (lo == hi) return hi;
+        return (double)draw() / 1000000000 * (hi - lo) + lo;
+    }
+};
+}  // namespace
+
+extern "C" int vg_scene_packing_density(vg_scene *h, int scene, const double *box, int n_points, int seed,
+                                        double *percent)
+{
+    const SceneHost *S = nullptr;
+    if (!box || !percent || n_points <= 0) return VG_EINVAL;
+    if (int rc = knight_scene(h, scene, &S)) return rc;
+    std::vector<double> pts((size_t)3 * n_points);
+    NjrRng rng(seed);
+    for (int k = 0; k < n_points; ++k) {
+        pts[3 * (size_t)k] = rng.uniform(box[0], box[1]);
+        pts[3 * (size_t)k + 1] = rng.uniform(box[2], box[3]);
+        pts[3 * (size_t)k + 2] = rng.uniform(box[4], box[5]);
+    }
+    double *dp = nullptr;
+    unsigned long long *hits = nullptr, hh = 0;
+    if (dalloc(h, &dp, pts.size()) || dalloc(h, &hits, 1)) {
+        if (dp) (void)hipFree(dp);
+        return VG_ENOMEM;
+    }
+    int rc = 0;
+    const SceneDev &D = h->sdev[scene];
+    if (hipMemcpyAsync(dp, pts.data(), sizeof(double) * pts.size(), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemsetAsync(hits, 0, sizeof(hh), h->stream) != hipSuccess)
+        rc = fail(h, VG_EHIP, "upload failed");
+    if (!rc && S->n > 0)
+        hipLaunchKernelGGL(k_density_hits, dim3(nblk(n_points, 256)), dim3(256), 0, h->stream, h->geo[h->cur], D.G,
+                           (long long)D.cell_off, (const int *)h->cstart, (const int *)h->cend, (const double *)dp,
+                           n_points, knight_box(box), hits);
+    if (!rc && (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(&hh, hits, sizeof(hh), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess))
+        rc = fail(h, VG_EHIP, "density kernel failed");
+    (void)hipFree(dp);
+    (void)hipFree(hits);
+    if (rc) return rc;
+    // hit / (float)RandomHitTestNum * 100.0, formed in float as the reference does
+    const float f = (float)(unsigned)hh / (float)n_points;
+    *percent = (double)f * 100.0;
+    return 0;
+}
+
+extern "C" int vg_scene_granular_temperature(vg_scene *h, int scene, int model, const double *center,
+                                             const double *ox, const double *oz, const double *size, double *out)
+{
+    const SceneHost *S = nullptr;
+    if (!center || !ox || !oz || !size || !out) return VG_EINVAL;
+    if (int rc = knight_scene(h, scene, &S)) return rc;
+    if (model < 0 || model >= S->n_models) return fail(h, VG_EINVAL, "unknown DOModel");
+    if (!S->model_is_sphere[(size_t)model])
+        return fail(h, VG_EUNSUPPORTED, "granular temperature of sphere models only");
+    GtRegion q;
+    q.model = S->model_off + model;
+    q.c = mk(center[0], center[1], center[2]);
+    q.ox = mk(ox[0], ox[1], ox[2]);
+    q.oz = mk(oz[0], oz[1], oz[2]);
+    q.oy = cross(q.oz, q.ox);
+    q.w = size[0]; q.l = size[1]; q.ht = size[2];
+    const int nb = (int)std::max(1u, std::min(512u, nblk(S->n, 256)));
+    double *part = nullptr, *davg = nullptr;
+    if (dalloc(h, &part, (size_t)7 * nb) || dalloc(h, &davg, 6)) {
+        if (part) (void)hipFree(part);
+        return VG_ENOMEM;
+    }
+    std::vector<double> hp((size_t)7 * nb);
+    double sums[7] = {0, 0, 0, 0, 0, 0, 0}, avg[6] = {0, 0, 0, 0, 0, 0}, dev[2] = {0, 0};
+    int rc = 0;
+    for (int pass = 0; pass < 2 && !rc; ++pass) {
+        if (pass == 1 && hipMemcpyAsync(davg, avg, sizeof(avg), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+            rc = fail(h, VG_EHIP, "upload failed");
+            break;
+        }
+        hipLaunchKernelGGL(k_gt_part, dim3(nb), dim3(256), 0, h->stream, h->geo[h->cur], S->seg_begin, S->seg_end, q,
+                           pass, (const double *)davg, part);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            rc = fail(h, VG_EHIP, "granular temperature kernel failed");
+            break;
+        }
+        for (int blk = 0; blk < nb; ++blk)   // block sums in block order
+            for (int k = 0; k < 7; ++k) {
+                if (pass == 0) sums[k] += hp[(size_t)7 * blk + k];
+                else if (k < 2) dev[k] += hp[(size_t)7 * blk + k];
+            }
+        if (pass == 0 && sums[6] != 0.0) {
+            const double inv = 1.0 / sums[6];   // 1.0 / Counter * sum (Knight.cpp:1186-1187)
+            for (int k = 0; k < 6; ++k) avg[k] = inv * sums[k];
+        }
+    }
+    (void)hipFree(part);
+    (void)hipFree(davg);
+    if (rc) return rc;
+    const double n = sums[6];
+    double tv = dev[0], tw = dev[1];
+    if (n != 0.0) {
+        tv = tv / n / 3.0;
+        tw = tw / n / 3.0;
+    }
+    out[0] = n;
+    out[1] = std::sqrt((avg[0] * avg[0] + avg[1] * avg[1]) + avg[2] * avg[2]);
+    out[2] = tv;
+    out[3] = std::sqrt((avg[3] * avg[3] + avg[4] * avg[4]) + avg[5] * avg[5]);
+    out[4] = tw;
+    return 0;
 }
 
 extern "C" int vg_step_timing(vg_scene *h, double *ms, int *launches)

@@ -190,6 +190,21 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
                                 unsigned long long *counter);
 __global__ void k_wall_impulse(Ell ell, int b, int e, int ld, int ki, int w0, const double *rec, double *out);
 __global__ void k_assign_spheres(Geo g, int b, int e, int ld, const double *buf, double *rec_imp);
+// Knight analyses (src/vedo/knight) of a scene's current state.
+struct KnightBox {
+    double lo[3], hi[3];   // xmin, ymin, zmin / xmax, ymax, zmax
+};
+__global__ void k_coordination(Geo g, int b, int e, int ld, const unsigned *o_cnt, const unsigned *o_nbr, KnightBox box,
+                               unsigned long long *acc);
+__global__ void k_density_hits(Geo g, GridDev G, long long cell_off, const int *cstart, const int *cend,
+                               const double *pts, int np, KnightBox box, unsigned long long *hits);
+struct GtRegion {
+    int model;                 // batch-wide DOModel index
+    d3 c, ox, oy, oz;          // GSRectangle centre and axes (oy = oz x ox)
+    double w, l, ht;           // width, length, height
+};
+__global__ void k_gt_part(Geo g, int b, int e, GtRegion q, int pass, const double *avg, double *part);
+
 __global__ void k_energy_part(Geo g, int b, int e, const double2 *mm, double gx, double gy, double gz,
                               const double *avg, int pass, double *part);
 __global__ void k_energy_final(const double *part, int nb, int n, int pass, double *avg, double *out);

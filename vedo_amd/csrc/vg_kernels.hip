@@ -1393,6 +1393,114 @@ __global__ void k_assign_spheres(Geo g, int b, int e, int ld, const double *buf,
 // forms it; the sums are a fixed-shape tree, so they differ from the
 // reference's sequential sums only in rounding order.
 //
+// ---------------------------------------------------------------------------
+// Knight analyses (src/vedo/knight) of a scene's current state
+// ---------------------------------------------------------------------------
+
+// A sphere counts for a box when its centre lies within the box widened by
+// its radius, every comparison strict (CalcCoordinateNum.cpp:24-49,
+// CalcDensity.cpp:20-43).
+__device__ inline bool knight_in_box(const KnightBox &B, double4 p)
+{
+    const double r = p.w;
+    return p.x < B.hi[0] + r && p.x > B.lo[0] - r && p.y < B.hi[1] + r && p.y > B.lo[1] - r && p.z < B.hi[2] + r &&
+           p.z > B.lo[2] - r;
+}
+
+// CalcCoordinateNum::computeAvgCoordinateNum (CalcCoordinateNum.cpp:8-77):
+// per sphere of the box, the spheres whose centre distance is at most the sum
+// of the radii.  Touching spheres are candidates of each other (the candidate
+// list covers every pair that can touch before the next rebuild), so each
+// sphere walks its candidate list.  acc[0] += touching partners, acc[1] += 1.
+__global__ void k_coordination(Geo g, int b, int e, int ld, const unsigned *o_cnt, const unsigned *o_nbr, KnightBox box,
+                               unsigned long long *acc)
+{
+    const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= e) return;
+    const double4 p = g.pos[i];
+    if (!knight_in_box(box, p)) return;
+    const d3 xi = ld3(p);
+    unsigned count = 0;
+    const unsigned c = o_cnt[i];
+    for (unsigned t = 0; t < c; ++t) {
+        const unsigned j = o_nbr[(size_t)t * ld + i];
+        if (j >= (unsigned)ld) continue;   // a wall
+        const double4 q = g.pos[j];
+        if (len(xi - ld3(q)) <= p.w + q.w) ++count;
+    }
+    atomicAdd(acc, (unsigned long long)count);
+    atomicAdd(acc + 1, 1ull);
+}
+
+// CalcDensity::computeDensity (CalcDensity.cpp:12-72), the hit test of each
+// random point: is it inside a sphere of the box?  The spheres come from the
+// 27 cells around the point in the grid of the last rebuild: a sphere that
+// holds the point was, at the rebuild, within its radius plus the path the
+// cell size allows for (the safe distance's own assumption) of the point.
+__global__ void k_density_hits(Geo g, GridDev G, long long cell_off, const int *cstart, const int *cend,
+                               const double *pts, int np, KnightBox box, unsigned long long *hits)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const d3 p = mk(pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]);
+    const long long cx = (long long)floor((p.x - G.xmin) / G.zr);
+    const long long cy = (long long)floor((p.y - G.ymin) / G.zr);
+    const long long cz = (long long)floor((p.z - G.zmin) / G.zr);
+    // sphere cells run 0 .. n-3 per axis (the grid keeps a layer of padding)
+    const long long x0 = max(cx - 1, 0ll), x1 = min(cx + 1, (long long)G.nx - 3);
+    const long long y0 = max(cy - 1, 0ll), y1 = min(cy + 1, (long long)G.ny - 3);
+    const long long z0 = max(cz - 1, 0ll), z1 = min(cz + 1, (long long)G.nz - 3);
+    for (long long z = z0; z <= z1; ++z)
+        for (long long y = y0; y <= y1; ++y)
+            for (long long x = x0; x <= x1; ++x) {
+                const long long l = cell_off + lin_of(G, x, y, z);
+                const int s1 = cend[l];
+                for (int s = cstart[l]; s < s1; ++s) {
+                    const double4 sp = g.pos[s];
+                    if (knight_in_box(box, sp) && len(p - ld3(sp)) <= sp.w) {
+                        atomicAdd(hits, 1ull);
+                        return;
+                    }
+                }
+            }
+}
+
+// CalculateGranularTemperature (Knight.cpp:1162-1225) over a GSRectangle
+// (GSRectangle.cpp:15-30): pass 0 writes per block the sums of v, w and the
+// count of the selected spheres; pass 1 the sums of |v - avg v|^2 and
+// |w - avg w|^2 (avg: 6 values).  Blocks reduce in a fixed order.
+__global__ __launch_bounds__(256) void k_gt_part(Geo g, int b, int e, GtRegion q, int pass, const double *avg,
+                                                 double *part)
+{
+    __shared__ double sm[7][256];
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = b + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
+        if (g.mdl[i] != q.model) continue;
+        const double4 p = g.pos[i];
+        const d3 lp = ld3(p) - q.c;
+        const double lx = dot(lp, q.ox), ly = dot(lp, q.oy), lz = dot(lp, q.oz);
+        if (!(fabs(lx) <= 0.5 * q.w && fabs(ly) <= 0.5 * q.l && fabs(lz) <= 0.5 * q.ht)) continue;
+        const double4 u = g.vel[i], w = g.avl[i];
+        if (pass == 0) {
+            v[0] += u.x; v[1] += u.y; v[2] += u.z;
+            v[3] += w.x; v[4] += w.y; v[5] += w.z;
+            v[6] += 1.0;
+        } else {
+            const d3 dv = ld3(u) - mk(avg[0], avg[1], avg[2]), dw = ld3(w) - mk(avg[3], avg[4], avg[5]);
+            v[0] += dot(dv, dv);
+            v[1] += dot(dw, dw);
+        }
+    }
+    for (int k = 0; k < 7; ++k) sm[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+        if ((int)threadIdx.x < s2)
+            for (int k = 0; k < 7; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) part[(size_t)blockIdx.x * 7 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
 // k_energy_part, pass 0, writes per block: [0] sum m (g.x), [1] sum 0.5 m v v,
 // [2] sum 0.5 (I wx wx + I wy wy + I wz wz), [3..5] sum m v, [6..8] sum
 // mmi x w (the reference's "angular momentum"), [9] max |v|, [10] min |v|,

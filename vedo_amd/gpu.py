@@ -67,7 +67,8 @@ EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_se
            "vg_num_interactions", "vg_scene_num_do", "vg_scene_models", "vg_set_skin", "vg_set_skin_auto", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
            "vg_system_energy", "vg_batch_create", "vg_admit", "vg_retire", "vg_scene_status",
            "vg_scene_get_state", "vg_scene_get_active", "vg_scene_energy", "vg_scene_set_wall_rotation",
-           "vg_batch_info")
+           "vg_batch_info", "vg_scene_coordination_number", "vg_scene_packing_density",
+           "vg_scene_granular_temperature")
 
 _lib = None
 
@@ -120,6 +121,9 @@ def lib():
         L.vg_step_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.vg_kernel_timing.argtypes = [vp, C.c_int]
         L.vg_set_skin.argtypes = [vp, C.c_double]
+        L.vg_scene_coordination_number.argtypes = [vp, C.c_int, _D, _D]
+        L.vg_scene_packing_density.argtypes = [vp, C.c_int, _D, C.c_int, C.c_int, _D]
+        L.vg_scene_granular_temperature.argtypes = [vp, C.c_int, C.c_int, _D, _D, _D, _D, _D]
         L.vg_set_skin_auto.argtypes = [vp, C.c_double, C.c_double]
         L.vg_get_skin.restype = C.c_double
         L.vg_get_skin.argtypes = [vp]
@@ -199,7 +203,36 @@ def _up(a):
     return a.ctypes.data_as(_U)
 
 
-class GpuSimulation:
+class _Knight:
+    """Knight analyses (src/vedo/knight) of a scene's current state, on the device."""
+
+    def coordination_number(self, box, scene: int = 0) -> float:
+        """CalcCoordinateNum::computeAvgCoordinateNum over box (xmin xmax ymin ymax zmin zmax)."""
+        b = np.ascontiguousarray(box, np.float64)
+        out = C.c_double()
+        self._check(self.L.vg_scene_coordination_number(self.h, int(scene), _dp(b), C.byref(out)))
+        return out.value
+
+    def packing_density(self, box, n_points: int = 20000, seed: int = 999, scene: int = 0) -> float:
+        """CalcDensity::computeDensity over box, in percent."""
+        b = np.ascontiguousarray(box, np.float64)
+        out = C.c_double()
+        self._check(self.L.vg_scene_packing_density(self.h, int(scene), _dp(b), int(n_points), int(seed),
+                                                    C.byref(out)))
+        return out.value
+
+    def granular_temperature(self, model: int, center, size, ox=(1.0, 0.0, 0.0), oz=(0.0, 0.0, 1.0),
+                             scene: int = 0):
+        """CalculateGranularTemperature of one sphere model over a GSRectangle:
+        (count, |mean v|, T_v, |mean w|, T_w)."""
+        arr = [np.ascontiguousarray(x, np.float64) for x in (center, ox, oz, size)]
+        out = np.zeros(5)
+        self._check(self.L.vg_scene_granular_temperature(self.h, int(scene), int(model), _dp(arr[0]), _dp(arr[1]),
+                                                         _dp(arr[2]), _dp(arr[3]), _dp(out)))
+        return out
+
+
+class GpuSimulation(_Knight):
     """One scene on the current GPU (a WorldArrays from vedo_amd.world)."""
 
     def __init__(self, wa, inner_skin: float = 0.0):
@@ -337,7 +370,7 @@ class GpuSimulation:
 SCENE_FREE, SCENE_QUEUED, SCENE_RUNNING, SCENE_FINISHED, SCENE_RETIRING, SCENE_DONE, SCENE_FAILED = range(7)
 
 
-class GpuBatch:
+class GpuBatch(_Knight):
     """Many independent scenes on the current GPU, stepped together (one
     kernel launch per step for all of them).  Scenes are admitted with a step
     count, placed on the device at the next rebuild boundary, frozen when

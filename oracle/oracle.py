@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libvedo_oracle.so")
 class VoModel(C.Structure):
     _fields_ = [("shape", C.c_int), ("behavior", C.c_int), ("scope", C.c_int), ("group", C.c_int),
                 ("density", C.c_double), ("density_factor", C.c_double),
-                ("fext", C.c_double * 3), ("attr", C.c_double * 3)]
+                ("fext", C.c_double * 3), ("attr", C.c_double * 3), ("hole", C.c_double * 3)]
 
 
 class VoIact(C.Structure):
@@ -127,9 +127,11 @@ class OracleSim:
             m.shape, m.behavior, m.scope, m.group = (int(wa.model_shape[k]), int(wa.model_behavior[k]),
                                                      int(wa.model_scope[k]), int(wa.model_group[k]))
             m.density, m.density_factor = float(wa.model_density[k]), float(wa.model_density_factor[k])
+            hole = getattr(wa, "model_hole", None)
             for d in range(3):
                 m.fext[d] = float(wa.model_fext[k, d])
                 m.attr[d] = float(wa.model_attr[k, d])
+                m.hole[d] = float(hole[k, d]) if hole is not None else 0.0
         self._iact = (VoIact * (G * G))()
         for i in range(G):
             for j in range(G):

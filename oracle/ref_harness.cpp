@@ -99,12 +99,21 @@ int main(int argc, char **argv)
     std::vector<std::string> model_names;
     std::list<IactModel *> iacts;
     std::vector<DOStatus *> statuses;
+    std::map<int, std::vector<double>> holes;   // model -> hole radius, x / y offset
     while (in >> tok) {
         if (tok == "time") { t0 = rd(in); t1 = rd(in); dt = rd(in); tc = rd(in); }
         else if (tok == "gravity") g = rv(in);
         else if (tok == "safety") sf = rd(in);
         else if (tok == "zoi") { zoi_lo = rv(in); zoi_hi = rv(in); }
         else if (tok == "cadence") in >> up >> rec >> cons;
+        else if (tok == "holes") {
+            int n; in >> n;
+            for (int k = 0; k < n; ++k) {
+                int m; in >> m;
+                double hr = rd(in), hx = rd(in), hy = rd(in);
+                holes[m] = {hr, hx, hy};
+            }
+        }
         else if (tok == "groups") {
             int n; in >> n; groups.resize(n);
             for (int k = 0; k < n; ++k) in >> groups[k];
@@ -121,6 +130,14 @@ int main(int argc, char **argv)
                 if (shape == Sphere) at.sphere.radius = a0;
                 else if (shape == QuasiCylinder) { at.quasicylinder.radius = a0; at.quasicylinder.height = a1; }
                 else if (shape == QuasiPlate) { at.quasiplate.width = a0; at.quasiplate.height = a1; at.quasiplate.length = a2; }
+                else if (shape == QuasiPlateWithCircularHole) {
+                    at.quasiplatewithcircularhole.width = a0;
+                    at.quasiplatewithcircularhole.height = a1;
+                    at.quasiplatewithcircularhole.length = a2;
+                    at.quasiplatewithcircularhole.holeradius = holes[k][0];
+                    at.quasiplatewithcircularhole.holexoffset = holes[k][1];
+                    at.quasiplatewithcircularhole.holeyoffset = holes[k][2];
+                }
                 models.push_back(new DOModel(name, groups[grp], (DOBehaviorType)beh, (DOScopeType)scope, dens, df,
                                              fext, (DOShapeType)shape, at, njrdxf::Color(0)));
                 model_names.push_back(name);

@@ -123,6 +123,45 @@ static void cd_sp(v3 xa, double ra, v3 xb, v3 ox, v3 oz, double w, double h, dou
     c->active = c->depth > 0;
 }
 
+/* CDSphere_QuasiPlateWithCircularHole::CalDistance + Detect
+ * (CDSphere2QuasiPlateWithCircularHole.cpp:16-134).  As the plate, with the
+ * closest point pushed to the hole's rim when the sphere's centre projects
+ * into the hole; the hole's centre is xb + (hx, hy, 0) in global axes
+ * (:46) while the rim point is placed along the plate's own axes. */
+static void cd_sph(v3 xa, double ra, v3 xb, v3 ox, v3 oz, double w, double h, double l, double hr, double hx,
+                   double hy, cinfo *c)
+{
+    v3 oy = cross(oz, ox);
+    v3 cap = sub(xa, project_on(sub(xa, xb), oz));
+    double hh = 0.5 * h, hw = 0.5 * w, hl = 0.5 * l;
+    v3 ch = add(xb, mk(hx, hy, 0.0));
+    v3 cha = sub(xa, ch);
+    double dx = dot(sub(cap, xb), ox);
+    if (dx < -hw) dx = -hw;
+    else if (dx > hw) dx = hw;
+    double dy = dot(sub(cap, xb), oy);
+    if (dy < -hl) dy = -hl;
+    else if (dy > hl) dy = hl;
+    double dhx = dot(cha, ox), dhy = dot(cha, oy);
+    double dh = sqrt(dhx * dhx + dhy * dhy);
+    if (dh < hr) {
+        if (dh == 0.0) {
+            dx = hx;
+            dy = hy;
+        } else {
+            dx = hx + dhx / dh * hr;
+            dy = hy + dhy / dh * hr;
+        }
+    }
+    v3 caps = add(add(muls(ox, dx), muls(oy, dy)), xb);
+    v3 im = sub(caps, xa);
+    if (dh == 0.0) c->depth = ra + hh - sqrt(len(im) * len(im) + hr * hr);
+    else c->depth = ra + hh - len(im);
+    c->n = dir(im);
+    c->p = add(xa, smul(ra - 0.5 * c->depth, c->n));
+    c->active = c->depth > 0;
+}
+
 /* CDSphere_QuasiCylinder::CalDistance + Detect (CDSphere2QuasiCylinder.cpp:16-80) */
 static void cd_sc(v3 xa, double ra, v3 xb, v3 ax, double rc, double hc, cinfo *c)
 {
@@ -477,6 +516,7 @@ static double model_range(const vo_model *m)
         if (m->scope == VO_LOCAL) return fmax(m->attr[0], 0.5 * m->attr[1]);
         return m->attr[0];
     case VO_QUASIPLATE:
+    case VO_QUASIPLATE_HOLE:   /* DOModel.cpp:90-106 */
         if (m->scope == VO_LOCAL) {
             double a = m->attr[0] > m->attr[1] ? m->attr[0] : m->attr[1];
             a = a > m->attr[2] ? a : m->attr[2];
@@ -503,6 +543,12 @@ static void model_mass(const vo_model *m, double *mass, double *sudo)
     case VO_QUASIPLATE:
         w = m->attr[0]; h = m->attr[1]; l = m->attr[2];
         vol = w * h * l;
+        break;
+    case VO_QUASIPLATE_HOLE:   /* DOModel.cpp:108-116 */
+        w = m->attr[0]; h = m->attr[1]; l = m->attr[2];
+        r = m->hole[0];
+        if (fabs(m->hole[1]) <= (0.5 * w - r) && fabs(m->hole[2]) <= (0.5 * l - r)) vol = w * h * l - r * r * PI * h;
+        else vol = w * h * l;
         break;
     }
     *mass = m->density * vol;
@@ -538,7 +584,8 @@ static double domap_distance(const vo_sim *s, unsigned a, unsigned b)
     if (ma->shape == VO_SPHERE && mb->shape == VO_SPHERE) {
         return len(sub(s->pos[a], s->pos[b]));
     }
-    if (ma->shape == VO_SPHERE && mb->shape == VO_QUASIPLATE) {
+    /* the plate with a hole: the plate's distance, the hole ignored (DOMap.cpp:336-374) */
+    if (ma->shape == VO_SPHERE && (mb->shape == VO_QUASIPLATE || mb->shape == VO_QUASIPLATE_HOLE)) {
         v3 ca = s->pos[a], cb = s->pos[b], oz = s->oz[b], ox = s->ox[b];
         v3 oy = cross(oz, ox);
         v3 cap = sub(ca, project_on(sub(ca, cb), oz));
@@ -843,6 +890,9 @@ static void detect(vo_sim *s, iact *it)
         cd_sp(s->pos[a], ma->attr[0], s->pos[b], s->ox[b], s->oz[b], mb->attr[0], mb->attr[1], mb->attr[2], &it->ci);
     else if (mb->shape == VO_QUASICYLINDER)
         cd_sc(s->pos[a], ma->attr[0], s->pos[b], s->oz[b], mb->attr[0], mb->attr[1], &it->ci);
+    else if (mb->shape == VO_QUASIPLATE_HOLE)
+        cd_sph(s->pos[a], ma->attr[0], s->pos[b], s->ox[b], s->oz[b], mb->attr[0], mb->attr[1], mb->attr[2],
+               mb->hole[0], mb->hole[1], mb->hole[2], &it->ci);
 }
 
 /* SimMediator::Initiate (SimMediator.cpp:640-712) + Assembler::CreateInteraction
@@ -867,9 +917,9 @@ static int initiate(vo_sim *s)
         else if (m1->density == 0.0) { sl = p2; ms = p1; }
         else { sl = p1; ms = p2; }
         const vo_model *msl = &s->models[s->model_of[sl]], *mms = &s->models[s->model_of[ms]];
-        /* CDcf: only the three sphere-slave detectors are on this path */
-        if (msl->shape != VO_SPHERE ||
-            (mms->shape != VO_SPHERE && mms->shape != VO_QUASIPLATE && mms->shape != VO_QUASICYLINDER))
+        /* CDcf: only the four sphere-slave detectors are on this path */
+        if (msl->shape != VO_SPHERE || (mms->shape != VO_SPHERE && mms->shape != VO_QUASIPLATE &&
+                                        mms->shape != VO_QUASICYLINDER && mms->shape != VO_QUASIPLATE_HOLE))
             continue;
         iact *it = &s->ia[s->n_ia++];
         it->slave = sl; it->master = ms; it->lo = p1; it->hi = p2;

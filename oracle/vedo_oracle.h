@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-enum { VO_SPHERE = 1, VO_QUASICYLINDER = 6, VO_QUASIPLATE = 7 };
+enum { VO_SPHERE = 1, VO_QUASICYLINDER = 6, VO_QUASIPLATE = 7, VO_QUASIPLATE_HOLE = 8 };
 enum { VO_CONSTRAINED = 1, VO_FIXED = 2, VO_MOBILE = 3, VO_ORBITAL = 4 };
 enum { VO_GLOBAL = 1, VO_LOCAL = 2 };
 
@@ -44,7 +44,8 @@ typedef struct {
     int shape, behavior, scope, group;
     double density, density_factor;
     double fext[3];
-    double attr[3];           /* sphere: r | cylinder: r, h | plate: w, h, l */
+    double attr[3];           /* sphere: r | cylinder: r, h | plate (also with a hole): w, h, l */
+    double hole[3];           /* QuasiPlateWithCircularHole: hole radius, x and y offsets */
 } vo_model;
 
 typedef struct { int law; double p[VO_NPARAM]; } vo_iact;

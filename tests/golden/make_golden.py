@@ -128,6 +128,10 @@ def make_cases():
                          spacing=1.98, jitter=0.0, vmax=0.0, offset=3.0)
     cases.append(("box_zoi", zoi_world(w, vz=-0.4, below=0.01).to_arrays(up_iact=3, record=1000), 0, 60,
                   [0, 2, 3, 4, 6, 7, 30, 60], None))
+    # a floor with a circular hole (CDSphere_QuasiPlateWithCircularHole)
+    w = scenes.box_scene(64, seed=17, law="ISwBSDBF", dt=2e-5, steps=400, kn=2e3, e=0.6, mu=0.4, **common)
+    cases.append(("box_hole", hole_world(w).to_arrays(up_iact=7, record=1000), 0, 200,
+                  [0, 1, 2, 6, 7, 8, 50, 100, 200], None))
     return cases
 
 
@@ -186,6 +190,43 @@ def metric_regions(state, sphere):
     return [("coord", whole), ("coord", lower), ("coord", mid), ("density", whole), ("density", lower)]
 
 
+def hole_world(w, hole=0.8, off=(0.1, -0.05), vz=-0.3):
+    """The box's floor as a fixed plate with a circular hole
+    (DOFixedQuasiPlateWithCircularHole), raised to just under the lowest
+    spheres and turned a quarter about z (Ox = +y), so that the hole centre's
+    global-axes offset (CDSphere2QuasiPlateWithCircularHole.cpp:46) and the
+    rim point's plate-axes offset differ; the spheres move down onto the
+    plate, its rim and through the hole."""
+    from vedo_amd.world import DOModel, DOStatus
+    sph = [k for k, st in enumerate(w.statuses) if st.name == "Sphere"]
+    r = [m for m in w.models if m.name == "Sphere"][0].attributes[0]
+    plate = [m for m in w.models if m.name == "Plate"][0]
+    wdt, thick, lng = plate.attributes
+    zlow = min(w.statuses[k].position[2] for k in sph)
+    floors = [k for k, st in enumerate(w.statuses) if st.name == "Plate" and tuple(st.orientation_z) == (0, 0, 1)]
+    k = min(floors, key=lambda k: w.statuses[k].position[2])
+    old = w.statuses[k]
+    # a hole smaller than a sphere, a little off the plate's centre (`off`
+    # radii), the plate moved so that the hole lies under a sphere of the
+    # lowest layer: that sphere rests on the rim.  The rim point is placed
+    # along the turned plate's axes, the hole's centre along global ones: the
+    # small offset makes them differ
+    low = [q for q in sph if w.statuses[q].position[2] < zlow + 0.5 * r]
+    q0 = low[len(low) // 2]
+    hx, hy = off[0] * r, off[1] * r
+    px = w.statuses[q0].position[0] - hx
+    py = w.statuses[q0].position[1] - hy
+    w.models.append(DOModel("HolePlate", plate.group, "fixed", "global", plate.density, 1.0, (0.0, 0.0, 0.0),
+                            "QuasiPlateWithCircularHole", (lng, thick, wdt, hole * r, hx, hy)))
+    w.statuses[k] = DOStatus("HolePlate", (px, py, zlow - r - 0.5 * thick + 0.02 * r),
+                             (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), (0.0, 0.0, 0.0))
+    for q in sph:
+        st = w.statuses[q]
+        w.statuses[q] = DOStatus(st.name, st.position, (st.velocity[0], st.velocity[1], st.velocity[2] + vz),
+                                 st.orientation_x, st.orientation_z, st.angular_velocity)
+    return w
+
+
 def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE, metrics=None):
     with tempfile.TemporaryDirectory() as td:
         scene = os.path.join(td, "scene.txt")
@@ -219,7 +260,8 @@ def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE, metrics
         ox=wa.ox, oz=wa.oz, av=wa.av,
         scalars=np.array([wa.dt, wa.t_start, wa.t_stop, wa.t_current, wa.safety_factor]),
         g=wa.g, cadence=np.array([wa.up_iact, wa.record], np.int64),
-        zoi=np.concatenate([np.asarray(wa.zoi_lo, np.float64), np.asarray(wa.zoi_hi, np.float64)]))
+        zoi=np.concatenate([np.asarray(wa.zoi_lo, np.float64), np.asarray(wa.zoi_hi, np.float64)]),
+        model_hole=np.asarray(wa.model_hole, np.float64))
     if assigned is not None:
         arrays["assigned_ids"] = np.asarray(assigned[0], np.uint32)
         arrays["assigned"] = assigned[1]

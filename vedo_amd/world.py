@@ -32,11 +32,12 @@ from typing import Dict, List, Sequence, Tuple
 import numpy as np
 
 # DOShapeType / DOBehaviorType / DOScopeType under -D_STD_CPP_11 (DOModel.h:26-28)
-SPHERE, QUASICYLINDER, QUASIPLATE = 1, 6, 7
+SPHERE, QUASICYLINDER, QUASIPLATE, QUASIPLATE_HOLE = 1, 6, 7, 8
 CONSTRAINED, FIXED, MOBILE, ORBITAL = 1, 2, 3, 4
 GLOBAL, LOCAL = 1, 2
 
-SHAPES = {"Sphere": SPHERE, "QuasiCylinder": QUASICYLINDER, "QuasiPlate": QUASIPLATE}
+SHAPES = {"Sphere": SPHERE, "QuasiCylinder": QUASICYLINDER, "QuasiPlate": QUASIPLATE,
+          "QuasiPlateWithCircularHole": QUASIPLATE_HOLE}
 BEHAVIORS = {"constrained": CONSTRAINED, "fixed": FIXED, "mobile": MOBILE, "orbital": ORBITAL}
 SCOPES = {"global": GLOBAL, "local": LOCAL}
 
@@ -86,12 +87,13 @@ class DOModel:
     external_force: Tuple[float, float, float]
     shape: str
     attributes: Tuple[float, ...]   # Sphere: (r,) QuasiCylinder: (r, h) QuasiPlate: (w, h, l)
+    #                                 QuasiPlateWithCircularHole: (w, h, l, hole radius, hole x / y offset)
 
     def analysis(self):
         """DOModel::Analysis (DOModel.cpp:17-149): range, mass, sudo-mass."""
         pi = 4.0 * math.atan(1.0)
         shape = SHAPES[self.shape]
-        a = tuple(self.attributes) + (0.0,) * (3 - len(self.attributes))
+        a = tuple(self.attributes) + (0.0,) * (6 - len(self.attributes))
         if shape == SPHERE:
             r = a[0]
             rng = r
@@ -101,9 +103,16 @@ class DOModel:
             rng = max(r, 0.5 * h) if self.scope == "local" else r
             vol = pi * r * r * h
         elif shape == QUASIPLATE:
-            w, h, l = a
+            w, h, l = a[:3]
             rng = 0.5 * max(max(w, h), l) if self.scope == "local" else 0.5 * h
             vol = w * h * l
+        elif shape == QUASIPLATE_HOLE:   # DOModel.cpp:90-120
+            w, h, l, hr, hxo, hyo = a
+            rng = 0.5 * max(max(w, h), l) if self.scope == "local" else 0.5 * h
+            if abs(hxo) <= 0.5 * w - hr and abs(hyo) <= 0.5 * l - hr:
+                vol = w * h * l - hr * hr * pi * h
+            else:
+                vol = w * h * l
         else:
             raise VedoError(f"shape {self.shape} is not on the GPU path")
         mass = self.density * vol
@@ -170,6 +179,11 @@ class WorldArrays:
     model_names: List[str] = field(default_factory=list)
     zoi_lo: np.ndarray = field(default_factory=lambda: np.zeros(3))
     zoi_hi: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    model_hole: np.ndarray = None   # f64 [M,3]: hole radius, x / y offset (QuasiPlateWithCircularHole)
+
+    def __post_init__(self):
+        if self.model_hole is None:
+            self.model_hole = np.zeros((len(self.model_shape), 3))
 
     @property
     def n_do(self) -> int:
@@ -226,8 +240,12 @@ class DOWorld:
         ms = self.models
 
         def attr3(m):
-            a = list(m.attributes) + [0.0] * (3 - len(m.attributes))
+            a = list(m.attributes) + [0.0] * (6 - len(m.attributes))
             return a[:3]
+
+        def hole3(m):
+            a = list(m.attributes) + [0.0] * (6 - len(m.attributes))
+            return a[3:6] if SHAPES[m.shape] == QUASIPLATE_HOLE else [0.0, 0.0, 0.0]
 
         sp = self.system_parameter
         model_of = np.array([midx[s.name] for s in self.statuses], np.int32)
@@ -250,7 +268,8 @@ class DOWorld:
             t_current=sp.time_current, g=np.array(sp.field_acceleration, np.float64),
             safety_factor=sp.safety_factor, up_iact=int(up_iact), record=int(record),
             group_names=groups, model_names=[m.name for m in ms],
-            zoi_lo=np.array(sp.zoi_lower, np.float64), zoi_hi=np.array(sp.zoi_upper, np.float64))
+            zoi_lo=np.array(sp.zoi_lower, np.float64), zoi_hi=np.array(sp.zoi_upper, np.float64),
+            model_hole=np.array([hole3(m) for m in ms], np.float64).reshape(-1, 3))
 
 
 def write_vscene(wa: WorldArrays, path: str, consultant: int = 0):
@@ -266,6 +285,12 @@ def write_vscene(wa: WorldArrays, path: str, consultant: int = 0):
         lines.append("zoi %s %s %s %s %s %s" % tuple(h(float(x)) for x in list(wa.zoi_lo) + list(wa.zoi_hi)))
     lines.append("cadence %d %d %d" % (wa.up_iact, wa.record, consultant))
     lines.append("groups %d %s" % (len(wa.group_names), " ".join(wa.group_names)))
+    # hole radius and offsets of QuasiPlateWithCircularHole models, before the models
+    holes = [k for k in range(len(wa.model_names)) if wa.model_shape[k] == QUASIPLATE_HOLE]
+    if holes:
+        lines.append("holes %d" % len(holes))
+        for k in holes:
+            lines.append("%d %s" % (k, " ".join(h(float(x)) for x in wa.model_hole[k])))
     lines.append("models %d" % len(wa.model_names))
     for k, name in enumerate(wa.model_names):
         lines.append("%s %d %d %d %d %s %s %s %s %s %s %s %s" % (

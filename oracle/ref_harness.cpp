@@ -11,6 +11,9 @@
 //
 // usage: vedo_ref_harness scene.txt out.bin nsteps dump_step[,dump_step...]
 //                         [--assigned statuses.bin] [--metrics requests.txt results.txt]
+//        vedo_ref_harness restart.ido out.bin nsteps dumps --cadence up_iact record consultant
+// A scene argument ending in .ido is a restart file read with
+// DOWorld::ReadIDO (world and IactRecordTab).
 // requests.txt, one per line, evaluated on the final state (DOWorld synced):
 //   coord xmin xmax ymin ymax zmin zmax     CalcCoordinateNum::computeAvgCoordinateNum
 //   density xmin xmax ymin ymax zmin zmax   CalcDensity::computeDensity (njr generator
@@ -83,11 +86,15 @@ int main(int argc, char **argv)
         std::cerr << "usage: vedo_ref_harness scene.txt out.bin nsteps dumps [--assigned f]\n";
         return 2;
     }
-    std::ifstream in(argv[1]);
+    const std::string scene = argv[1];
+    const bool from_ido = scene.size() > 4 && scene.compare(scene.size() - 4, 4, ".ido") == 0;
+    std::ifstream in(from_ido ? "" : argv[1]);
     std::string tok;
-    in >> tok;   // VSCENE
-    int ver;
-    in >> ver;
+    if (!from_ido) {
+        in >> tok;   // VSCENE
+        int ver;
+        in >> ver;
+    }
     double t0 = 0, t1 = 0, dt = 0, tc = 0;
     Vector3df g;
     double sf = 1.1;
@@ -162,13 +169,28 @@ int main(int argc, char **argv)
             }
         }
     }
-    aries::Constants::Instance()->SetSafetyFactor(sf);
-    Boundary zoi(zoi_lo, zoi_hi), pbc;
-    SystemParameter *sp = new SystemParameter(t0, t1, dt, tc, g, zoi, pbc);
-    unsigned ndo = (unsigned)statuses.size();
-    sp->SetDONumber(ndo);
-    DOWorld *world = new DOWorld(sp, models, iacts, statuses);
+    DOWorld *world;
     IactRecordTab *irt = new IactRecordTab();
+    if (from_ido) {
+        // a restart: DOWorld::ReadIDO fills the world, the IactRecordTab and
+        // the safety factor (DOWorld_RWIDO.cpp:9-55); the cadence comes from
+        // --cadence
+        for (int k = 5; k + 3 < argc; ++k)
+            if (std::string(argv[k]) == "--cadence") {
+                up = (unsigned)std::atol(argv[k + 1]);
+                rec = (unsigned)std::atol(argv[k + 2]);
+                cons = std::atoi(argv[k + 3]);
+            }
+        world = new DOWorld();
+        if (!world->ReadIDO(scene, irt)) return 3;
+    } else {
+        aries::Constants::Instance()->SetSafetyFactor(sf);
+        Boundary zoi(zoi_lo, zoi_hi), pbc;
+        SystemParameter *sp = new SystemParameter(t0, t1, dt, tc, g, zoi, pbc);
+        unsigned ndo = (unsigned)statuses.size();
+        sp->SetDONumber(ndo);
+        world = new DOWorld(sp, models, iacts, statuses);
+    }
     Assembler *as = CreateNewAssembler();
     Consultant *c;
     if (cons == 1) c = new NBSConsultant(world, irt, "ref_run.ido", rec, up);

@@ -280,11 +280,55 @@ def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE, metrics
     return d
 
 
+IDO_DUMPS = [1, 2, 6, 7, 8, 30, 60]
+
+
+def make_ido_restart(outdir=HERE):
+    """A restart from the reference's own checkpoint: a lattice pressed a little
+    together runs `record` steps and RecordIDO writes ref_run_000001.ido
+    (Consultant.cpp:213-229); a second harness process reads that file with
+    DOWorld::ReadIDO and runs 60 more steps, writing its own first .ido after
+    `record` of them.  Saved: both files' bytes, the cadence and the
+    restart's dumps."""
+    import shutil
+    w = scenes.box_scene(64, seed=18, law="ISwBSDB2F", dt=2e-5, steps=400, omega_max=5.0, spacing=1.99,
+                         jitter=0.01, vmax=0.02, offset=0.0, kn=2e3, e=0.6, mu=0.4)
+    up, rec, nsteps = 7, 10, 60
+    wa = w.to_arrays(up_iact=up, record=rec)
+    with tempfile.TemporaryDirectory() as td:
+        scene = os.path.join(td, "scene.txt")
+        write_vscene(wa, scene, 0)
+        subprocess.run([HARNESS, scene, os.path.join(td, "a.bin"), str(rec), str(rec)], cwd=td, check=True,
+                       stdout=subprocess.DEVNULL)
+        rd = os.path.join(td, "restart")
+        os.mkdir(rd)
+        shutil.copy(os.path.join(td, "ref_run_000001.ido"), os.path.join(rd, "start.ido"))
+        subprocess.run([HARNESS, "start.ido", "out.bin", str(nsteps), ",".join(map(str, IDO_DUMPS)), "--cadence",
+                        str(up), str(rec), "0"], cwd=rd, check=True, stdout=subprocess.DEVNULL)
+        with open(os.path.join(rd, "start.ido"), "rb") as f:
+            ido0 = f.read()
+        with open(os.path.join(rd, "ref_run_000001.ido"), "rb") as f:
+            ido1 = f.read()
+        d = parse_dumps(os.path.join(rd, "out.bin"))
+    arrays = dict(ido0=np.frombuffer(ido0, np.uint8), ido1=np.frombuffer(ido1, np.uint8),
+                  cadence=np.array([up, rec], np.int64), nsteps=np.int32(nsteps), dumps=np.array(IDO_DUMPS, np.int32))
+    for k, r in enumerate(d):
+        arrays[f"d{k}_step"] = np.int32(r["step"])
+        arrays[f"d{k}_t"] = np.float64(r["t"])
+        arrays[f"d{k}_state"] = r["state"]
+        arrays[f"d{k}_iact"] = r["iact"]
+        arrays[f"d{k}_status"] = r["status"]
+    np.savez_compressed(os.path.join(outdir, "ido_restart.npz"), **arrays)
+    print(f"ido_restart: {len(ido0)} + {len(ido1)} bytes of .ido, active {[int(r['iact']['active'].sum()) for r in d]}")
+
+
 def main():
     """Writes every case, or only those named on the command line."""
     if not os.path.exists(HARNESS):
         raise SystemExit("build the reference harness first: make -C oracle ref")
     only = set(sys.argv[1:])
+    if not only or "ido_restart" in only:
+        make_ido_restart()
     for name, wa, cons, nsteps, dumps, assigned in make_cases():
         if only and name not in only:
             continue

@@ -12,6 +12,7 @@
 // usage: vedo_ref_harness scene.txt out.bin nsteps dump_step[,dump_step...]
 //                         [--assigned statuses.bin] [--metrics requests.txt results.txt]
 //        vedo_ref_harness restart.ido out.bin nsteps dumps --cadence up_iact record consultant
+// --xml file.xml writes DOWorld::WriteXML(file, IactRecordTab) after the last step.
 // A scene argument ending in .ido is a restart file read with
 // DOWorld::ReadIDO (world and IactRecordTab).
 // requests.txt, one per line, evaluated on the final state (DOWorld synced):
@@ -311,5 +312,12 @@ int main(int argc, char **argv)
             }
         }
     }
+    for (int k = 5; k + 1 < argc; ++k)
+        if (std::string(argv[k]) == "--xml") {
+            // DOWorld::WriteXML of the final state and the IactRecordTab as
+            // the consultant last filled it
+            world->UpdateDOStatus(sm->GetDOContainer()->GetDOStatus());
+            world->WriteXML(argv[k + 1], irt);
+        }
     return 0;
 }

@@ -305,12 +305,21 @@ def make_ido_restart(outdir=HERE):
         shutil.copy(os.path.join(td, "ref_run_000001.ido"), os.path.join(rd, "start.ido"))
         subprocess.run([HARNESS, "start.ido", "out.bin", str(nsteps), ",".join(map(str, IDO_DUMPS)), "--cadence",
                         str(up), str(rec), "0"], cwd=rd, check=True, stdout=subprocess.DEVNULL)
+        # DOWorld::WriteXML after `record` restart steps (state and records)
+        xd = os.path.join(td, "xml")
+        os.mkdir(xd)
+        shutil.copy(os.path.join(td, "ref_run_000001.ido"), os.path.join(xd, "start.ido"))
+        subprocess.run([HARNESS, "start.ido", "x.bin", str(rec), str(rec), "--cadence", str(up), str(rec), "0",
+                        "--xml", "world.xml"], cwd=xd, check=True, stdout=subprocess.DEVNULL)
+        with open(os.path.join(xd, "world.xml"), "rb") as f:
+            xml = f.read()
         with open(os.path.join(rd, "start.ido"), "rb") as f:
             ido0 = f.read()
         with open(os.path.join(rd, "ref_run_000001.ido"), "rb") as f:
             ido1 = f.read()
         d = parse_dumps(os.path.join(rd, "out.bin"))
     arrays = dict(ido0=np.frombuffer(ido0, np.uint8), ido1=np.frombuffer(ido1, np.uint8),
+                  xml=np.frombuffer(xml, np.uint8),
                   cadence=np.array([up, rec], np.int64), nsteps=np.int32(nsteps), dumps=np.array(IDO_DUMPS, np.int32))
     for k, r in enumerate(d):
         arrays[f"d{k}_step"] = np.int32(r["step"])

@@ -1,0 +1,156 @@
+"""DOWorld XML without Boost (vedo_amd.xmlio) against a file the compiled
+reference wrote: DOWorld::WriteXML after a restart of 10 steps from the
+reference's own .ido (fixture ido_restart, key "xml").  The writer
+reproduces it byte for byte from the oracle's bit-exact state and records;
+the reader reads it back so that writing again gives the same bytes (with
+the orientations as stored: the reader normalises them, as
+Vector3d::normalized does in DOWorld_ReadXML.cpp:558-559), and its records
+match the reference's .ido of the same step to the printed precision.  The
+reference's reader needs Boost, which this image lacks, so the reader's
+defaults are checked against the behaviour DOWorld_ReadXML.cpp states."""
+import os
+import re
+import textwrap
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+
+def _z():
+    return np.load(os.path.join(GOLDEN, "ido_restart.npz"))
+
+
+def _write(tmp_path, name, data: bytes):
+    p = tmp_path / name
+    p.write_bytes(data)
+    return str(p)
+
+
+def test_xml_writer_matches_reference(tmp_path):
+    from oracle.oracle import OracleSim
+    from vedo_amd.ido import ido_history, ido_world, read_ido
+    from vedo_amd.xmlio import write_xml
+    z = _z()
+    d0 = read_ido(_write(tmp_path, "start.ido", z["ido0"].tobytes()))
+    up, rec = (int(x) for x in z["cadence"])
+    wa = ido_world(d0, up, rec)
+    o = OracleSim(wa)
+    o.set_history(*ido_history(d0))
+    o.step(rec)
+    out = tmp_path / "mine.xml"
+    write_xml(str(out), wa, o.state(), o.active(), time=o.time)
+    assert out.read_bytes() == z["xml"].tobytes()
+
+
+def _raw_orientations(text):
+    """OrientationX / OrientationZ as printed, per DOStatus."""
+    def grab(tag):
+        pat = tag + r'\s+x="([^"]+)" y="([^"]+)" z="([^"]+)"'
+        return np.array([[float(v) for v in m] for m in re.findall(pat, text)])
+    return grab("OrientationX"), grab("OrientationZ")
+
+
+def test_xml_reader_round_trip(tmp_path):
+    from vedo_amd.ido import read_ido
+    from vedo_amd.xmlio import read_xml, write_xml
+    z = _z()
+    text = z["xml"].tobytes()
+    up, rec = (int(x) for x in z["cadence"])
+    wa, recs, (imp, aimp) = read_xml(_write(tmp_path, "ref.xml", text), up, rec)
+    ox_raw, oz_raw = _raw_orientations(text.decode())
+    # the in-place normalisation: v / sqrt((x*x + y*y) + z*z)
+    for got, raw in ((wa.ox, ox_raw), (wa.oz, oz_raw)):
+        n = np.sqrt((raw[:, 0] * raw[:, 0] + raw[:, 1] * raw[:, 1]) + raw[:, 2] * raw[:, 2])
+        assert np.array_equal(got, raw / n[:, None])
+    st = dict(pos=wa.pos, vel=wa.vel, ox=ox_raw, oz=oz_raw, av=wa.av, impact=imp, ang_impact=aimp)
+    out = tmp_path / "again.xml"
+    write_xml(str(out), wa, st, recs, max_id=0)
+    assert out.read_bytes() == text
+    # the writer's safety-factor constant is not the name the reader reads
+    assert wa.safety_factor == 1.1
+    # records against the .ido the reference wrote at the same step
+    d1 = read_ido(_write(tmp_path, "ref1.ido", z["ido1"].tobytes()))
+    lo, hi, s = recs
+    r = d1.records
+    assert np.array_equal(lo, r["first"]) and np.array_equal(hi, r["second"])
+    assert np.array_equal(s["contact"], r["contact"]) and np.array_equal(s["last_contact"], r["contact"])
+    assert np.array_equal(s["bond"], r["bond"]) and np.array_equal(s["last_bond"], r["bond"])
+    for f in ("kn", "init_vel", "overlap", "shear", "point", "to_master", "ang_to_master"):
+        assert np.allclose(s[f], r[f], rtol=6e-7, atol=0.0), f
+    assert np.allclose(s["dir"], r["dir"], rtol=0.0, atol=5.1e-7)
+    v1 = np.stack([x.vectors for x in d1.statuses])
+    assert np.allclose(wa.pos, v1[:, 0], rtol=6e-7, atol=0.0)
+
+
+def test_xml_reader_defaults_and_errors(tmp_path):
+    from vedo_amd.world import VedoError
+    from vedo_amd.xmlio import read_xml
+    doc = textwrap.dedent("""\
+        <?xml version="1.0" encoding="UTF-8"?>
+        <DOWorld>
+          <SimParameter>
+            <SimConstant><Constant Name="SafetyFactor" Value="0.5"/></SimConstant>
+            <TimeControl Start="0" Stop="1" Interval="1e-4" Current="0"/>
+            <FieldAcceleration x="0" y="0" z="-9.81"/>
+            <ZOI ZMin="-1" ZMax="2"/>
+          </SimParameter>
+          <DOModelTab>
+            <DOModel DOName="Ball" DOGroup="G" Behavior="mobile" Density="2500">
+              <Shape><Sphere Radius="0.001"/></Shape>
+            </DOModel>
+            <DOModel DOName="Floor" DOGroup="W" Behavior="fixed" Density="0" DensityFactor="2" Scope="undefined">
+              <ExternalForce x="1" y="2" z="3"/>
+              <Shape><QuasiPlate Width="0.1" Length="0.3" Height="0.002"/></Shape>
+            </DOModel>
+          </DOModelTab>
+          <IactModelTab>
+            <IactModel MasterGroup="G" SlaveGroup="W" EquationType="ISwHertz">
+              <Mechanism Name="NormalStiffness" Value="1e6"/>
+              <Mechanism Name="NormalDampingCoefficient" Value="0.1"/>
+            </IactModel>
+          </IactModelTab>
+          <DOStatusTab>
+            <DOStatus DOName="Ball"><Position x="0" y="0" z="0.01"/><Velocity x="0" y="0" z="0"/>
+              <OrientationX x="2" y="0" z="0"/><OrientationZ x="0" y="0" z="0"/>
+              <AngularVelocity x="0" y="0" z="0"/></DOStatus>
+            <DOStatus DOName="Floor"><Position x="0" y="0" z="0"/><Velocity x="0" y="0" z="0"/>
+              <OrientationX x="1" y="0" z="0"/><OrientationZ x="0" y="0" z="1"/>
+              <AngularVelocity x="0" y="0" z="0"/><Impact x="4" y="5" z="6"/></DOStatus>
+          </DOStatusTab>
+          <IactStatusTab>
+            <IactStatus MasterDOStatusSN="0" SlaveDOStatusSN="1" Contact="1" Bond="0" NormalStiffness="5"
+                        InitialVelocity="0" Overlap="0">
+              <ShearForce x="0" y="0" z="0"/><ImpactPoint x="0" y="0" z="0"/><ImpactDirection x="0" y="0" z="0"/>
+              <ImpactToMaster x="0" y="0" z="0"/><AngularImpactToMaster x="0" y="0" z="0"/></IactStatus>
+            <IactStatus MasterDOStatusSN="0" SlaveDOStatusSN="1" Contact="0" Bond="1" NormalStiffness="7"
+                        InitialVelocity="0" Overlap="0">
+              <ShearForce x="0" y="0" z="0"/><ImpactPoint x="0" y="0" z="0"/><ImpactDirection x="0" y="0" z="0"/>
+              <ImpactToMaster x="0" y="0" z="0"/><AngularImpactToMaster x="0" y="0" z="0"/></IactStatus>
+          </IactStatusTab>
+        </DOWorld>
+        """)
+    p = tmp_path / "w.xml"
+    p.write_text(doc)
+    wa, recs, (imp, aimp) = read_xml(str(p), 5, 100)
+    assert wa.safety_factor == 1.0                       # max(1, 0.5)
+    assert list(wa.model_scope) == [2, 1]                # default Scope: local for mobile, global otherwise
+    assert list(wa.model_density_factor) == [1.0, 2.0]   # DensityFactor defaults to 1
+    assert np.array_equal(wa.model_fext, [[0, 0, 0], [1, 2, 3]])
+    assert np.array_equal(wa.model_attr[1], [0.1, 0.002, 0.3])   # (width, height, length)
+    assert np.array_equal(wa.zoi_lo, [0, 0, -1]) and np.array_equal(wa.zoi_hi, [0, 0, 2])
+    assert np.array_equal(wa.ox[0], [1, 0, 0]) and np.array_equal(wa.oz[0], [0, 0, 0])   # zero stays zero
+    assert np.array_equal(imp, [[0, 0, 0], [4, 5, 6]]) and not aimp.any()
+    assert wa.iact_law[0, 1] == wa.iact_law[1, 0] == 5
+    lo, hi, st = recs                                    # one record per key, the first kept
+    assert len(lo) == 1 and st["kn"][0] == 5.0 and st["contact"][0] == 1 and st["last_contact"][0] == 1
+    p.write_text(doc.replace("<Sphere Radius=\"0.001\"/>", "<Ellipsoid XLength=\"1\" YLength=\"1\" ZLength=\"1\"/>"))
+    with pytest.raises(VedoError, match="not on the GPU path"):
+        read_xml(str(p), 5, 100)
+    p.write_text(doc.replace('Radius="0.001"', 'Radius="0"'))
+    with pytest.raises(VedoError, match="Radius"):
+        read_xml(str(p), 5, 100)
+    p.write_text(doc.replace('Behavior="mobile"', 'Behavior="floating"'))
+    with pytest.raises(VedoError, match="Behavior"):
+        read_xml(str(p), 5, 100)

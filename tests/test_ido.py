@@ -114,6 +114,8 @@ def restart_and_compare(make_sim, tmp_path):
             assert np.array_equal(np.asarray(st[f]).view(np.uint64), np.asarray(rs[f]).view(np.uint64)), f
         np.testing.assert_array_equal(st["contact"], rs["flags"][:, 0])
         np.testing.assert_array_equal(st["last_contact"], rs["flags"][:, 1])
+        np.testing.assert_array_equal(st["bond"], rs["flags"][:, 2])
+        np.testing.assert_array_equal(st["last_bond"], rs["flags"][:, 3])
     return sim
 
 

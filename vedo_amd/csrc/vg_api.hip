@@ -704,7 +704,7 @@ static int build_cw_pairs(vg_scene *h, const Geo &G, int n)
 }
 
 static int export_active(vg_scene *h, int slot, ActiveOut *out, size_t cap, unsigned long long *dcnt,
-                         unsigned *d_sid, unsigned *d_wid);
+                         unsigned *d_sid, unsigned *d_wid, bool backup);
 static int id_maps(vg_scene *h, int slot, unsigned **d_sid, unsigned **d_wid);
 
 // A scene that loses spheres to its zone of interest at a rebuild: its
@@ -766,14 +766,14 @@ static int zoi_prepare(vg_scene *h, std::vector<ZoiPlan> &plans)
             unsigned long long *dcnt = nullptr;
             unsigned *d_sid = nullptr, *d_wid = nullptr;
             if (dalloc(h, &dcnt, 1) || id_maps(h, k, &d_sid, &d_wid)) return VG_ENOMEM;
-            if (export_active(h, k, nullptr, 0, dcnt, d_sid, d_wid)) return VG_EHIP;
+            if (export_active(h, k, nullptr, 0, dcnt, d_sid, d_wid, false)) return VG_EHIP;
             unsigned long long c = 0;
             HIPCHK(hipMemcpyAsync(&c, dcnt, sizeof(c), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             m = (long long)c;
             if (m > 0) {
                 if (dalloc(h, &recs, (size_t)m)) return VG_ENOMEM;
-                if (export_active(h, k, recs, (size_t)m, dcnt, d_sid, d_wid)) return VG_EHIP;
+                if (export_active(h, k, recs, (size_t)m, dcnt, d_sid, d_wid, false)) return VG_EHIP;
             }
             HIPCHK(hipStreamSynchronize(st));
             (void)hipFree(dcnt); (void)hipFree(d_sid); (void)hipFree(d_wid);
@@ -1471,7 +1471,7 @@ static const SceneHost *scene0(const vg_scene *h)
 }
 
 static int export_active(vg_scene *h, int slot, ActiveOut *out, size_t cap, unsigned long long *dcnt,
-                         unsigned *d_sid, unsigned *d_wid)
+                         unsigned *d_sid, unsigned *d_wid, bool backup)
 {
     const SceneHost &S = h->sc[slot];
     HIPCHK(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), h->stream));
@@ -1479,7 +1479,7 @@ static int export_active(vg_scene *h, int slot, ActiveOut *out, size_t cap, unsi
                        h->ell[h->ecur], S.seg_begin, S.seg_end, h->ld, h->ki, (const unsigned *)h->ocount,
                        (const unsigned *)h->onbr, (const WallDev *)h->walls, (const unsigned *)d_wid,
                        (const unsigned *)d_sid, h->ent_valid ? (const double *)h->rec : (const double *)nullptr, out,
-                       (unsigned long long)cap, dcnt);
+                       (unsigned long long)cap, dcnt, backup ? 1 : 0);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1505,7 +1505,7 @@ static int snapshot_records(vg_scene *h)
     unsigned long long *dcnt = nullptr;
     unsigned *d_sid = nullptr, *d_wid = nullptr;
     if (dalloc(h, &dcnt, 1) || id_maps(h, 0, &d_sid, &d_wid)) return VG_ENOMEM;
-    if (export_active(h, 0, nullptr, 0, dcnt, d_sid, d_wid)) return VG_EHIP;
+    if (export_active(h, 0, nullptr, 0, dcnt, d_sid, d_wid, false)) return VG_EHIP;
     unsigned long long cnt = 0;
     HIPCHK(hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -1515,7 +1515,7 @@ static int snapshot_records(vg_scene *h)
         h->snap_cap = cnt + cnt / 4 + 64;
         if (dalloc(h, &h->snap, h->snap_cap)) return VG_ENOMEM;
     }
-    if (export_active(h, 0, h->snap, h->snap_cap, dcnt, d_sid, d_wid)) return VG_EHIP;
+    if (export_active(h, 0, h->snap, h->snap_cap, dcnt, d_sid, d_wid, false)) return VG_EHIP;
     const SceneHost &S = h->sc[0];
     if (S.nw && h->ent_valid)
         hipLaunchKernelGGL(k_wall_impulse, dim3(S.nw), dim3(256), 0, h->stream, h->ell[h->ecur], S.seg_begin,
@@ -1927,14 +1927,14 @@ static int get_active(vg_scene *h, int slot, long long *n, unsigned *lo, unsigne
         unsigned long long *dcnt = nullptr;
         unsigned *d_sid = nullptr, *d_wid = nullptr;
         if (dalloc(h, &dcnt, 1) || id_maps(h, slot, &d_sid, &d_wid)) return VG_ENOMEM;
-        if (export_active(h, slot, nullptr, 0, dcnt, d_sid, d_wid)) return VG_EHIP;
+        if (export_active(h, slot, nullptr, 0, dcnt, d_sid, d_wid, true)) return VG_EHIP;
         HIPCHK(hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         if (st) {
             const size_t cap = (size_t)std::max(*n, 0ll);
             ActiveOut *dout = nullptr;
             if (dalloc(h, &dout, cap)) return VG_ENOMEM;
-            if (export_active(h, slot, dout, cap, dcnt, d_sid, d_wid)) return VG_EHIP;
+            if (export_active(h, slot, dout, cap, dcnt, d_sid, d_wid, true)) return VG_EHIP;
             ho.resize(std::min<size_t>(cnt, cap));
             if (!ho.empty())
                 HIPCHK(hipMemcpyAsync(ho.data(), dout, sizeof(ActiveOut) * ho.size(), hipMemcpyDeviceToHost, h->stream));

@@ -163,6 +163,20 @@ __device__ inline double domap_wall_distance(d3 ca, const WallDev &w)
     return len(im);
 }
 
+// Entry flags as IactContainer::BackupImpactStatus leaves them after the step
+// (IactContainer.cpp:128-131, ImpactStatus::Backup): an entry stores the
+// Backup of its last solve as pending (applied when it is next solved), so
+// readers of a stepped entry apply it; a restored entry (F_NOBACKUP) has no
+// pending Backup.
+__host__ __device__ inline unsigned backed_up(unsigned f)
+{
+    if (f & F_NOBACKUP) return f;
+    unsigned g = f & ~(F_LASTCONTACT | F_LASTBOND);
+    if (f & F_CONTACT) g |= F_LASTCONTACT;
+    if (f & F_BOND) g |= F_LASTBOND;
+    return g;
+}
+
 // ---- ImpactStatus transitions (ImpactStatus.cpp:248-274) ----
 __device__ inline void st_clean(Hist &h)
 {

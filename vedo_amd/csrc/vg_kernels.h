@@ -187,7 +187,7 @@ __global__ void k_scatter_state(Geo g, int b, int e, int ld, const double *rec_i
 __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *o_cnt,
                                 const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,
                                 const unsigned *sphere_id, const double *rec, ActiveOut *out, unsigned long long cap,
-                                unsigned long long *counter);
+                                unsigned long long *counter, int backup);
 __global__ void k_wall_impulse(Ell ell, int b, int e, int ld, int ki, int w0, const double *rec, double *out);
 __global__ void k_assign_spheres(Geo g, int b, int e, int ld, const double *buf, double *rec_imp);
 // Knight analyses (src/vedo/knight) of a scene's current state.

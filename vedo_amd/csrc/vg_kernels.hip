@@ -1287,7 +1287,7 @@ __global__ void k_scatter_state(Geo g, int b, int e, int ld, const double *rec_i
 __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *o_cnt,
                                 const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,
                                 const unsigned *sphere_id, const double *rec, ActiveOut *out, unsigned long long cap,
-                                unsigned long long *counter)
+                                unsigned long long *counter, int backup)
 {
     // spheres b..e-1 (one scene); sphere_id maps the scene's ranks to DO ids
     const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1331,7 +1331,11 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
         const unsigned me = sphere_id[g.oid[i]];
         A.lo = me < other ? me : other;
         A.hi = me < other ? other : me;
-        A.flags = ent & ~PARTNER_MASK;
+        // after a plain step the reference has run BackupImpactStatus (the
+        // entry holds it as pending); records taken for a rebuild or a record
+        // are read before it, as RebuildIactRecordTab reads them
+        // (LeapConsultant.cpp:62-83)
+        A.flags = (backup ? backed_up(ent & ~PARTNER_MASK) : (ent & ~PARTNER_MASK)) & ~F_NOBACKUP;
         Hist hs;
         hs.s = mk(0, 0, 0);
         hs.aux = 0.0;

@@ -113,6 +113,8 @@ struct vg_scene {
     double *en_part = nullptr, *en_out = nullptr;
     double *fetch = nullptr;            // state download scratch (grows)
     size_t fetch_cap = 0;
+    double *pin = nullptr;              // pinned host staging of state downloads (grows)
+    size_t pin_cap = 0;
     int *flags = nullptr;
     unsigned long long *relists = nullptr;
     unsigned long long *contacts = nullptr;   // per-wavefront contact-pair counts, summed by vg_stats
@@ -1295,6 +1297,7 @@ static void free_scene(vg_scene *h)
     F(h->cw_val); F(h->cw_stamp); F(h->cw_key[0]); F(h->cw_key[1]); F(h->cw_vidx[0]); F(h->cw_vidx[1]);
     F(h->cw_w); F(h->cw_p); F(h->cw_cnt); F(h->cw_beg); F(h->cw_end); F(h->cw_imp);
     F(h->zoi_dead); F(h->zoi_cnt); F(h->zoi_rank); F(h->zoi_pre);
+    if (h->pin) (void)hipHostFree(h->pin);
     for (auto e : h->kev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1762,7 +1765,16 @@ static int fetch_scene_state(vg_scene *h, int slot, std::vector<double> &hb)
         if (dalloc(h, &h->fetch, h->fetch_cap)) return VG_ENOMEM;
     }
     double *dout = h->fetch;
-    std::vector<double> sb((size_t)21 * n, 0.0);
+    // pinned staging: a pageable download of a large scene runs several
+    // times slower and holds the stream meanwhile
+    if ((size_t)21 * n > h->pin_cap) {
+        if (h->pin) HIPCHK(hipHostFree(h->pin));
+        h->pin = nullptr;
+        h->pin_cap = 0;
+        HIPCHK(hipHostMalloc((void **)&h->pin, sizeof(double) * 21 * (size_t)std::max(n, 1), hipHostMallocDefault));
+        h->pin_cap = (size_t)21 * std::max(n, 1);
+    }
+    double *sb = h->pin;
     hipLaunchKernelGGL(k_scatter_state, dim3(nblk(n, 256)), dim3(256), 0, h->stream, h->geo[h->cur], S.seg_begin,
                        S.seg_end, h->ld, h->rec_valid ? (const double *)h->rec_imp : (const double *)nullptr, dout);
     std::vector<double> wimp((size_t)6 * std::max(S.nw, 1), 0.0);
@@ -1776,7 +1788,7 @@ static int fetch_scene_state(vg_scene *h, int slot, std::vector<double> &hb)
         hipLaunchKernelGGL(k_wall_impulse, dim3(S.nw), dim3(256), 0, h->stream, h->ell[h->ecur], S.seg_begin,
                            S.seg_end, h->ld, h->ki, S.w_begin, (const double *)h->rec, dw);
     }
-    HIPCHK(hipMemcpyAsync(sb.data(), dout, sizeof(double) * sb.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(sb, dout, sizeof(double) * 21 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
     if (dw) HIPCHK(hipMemcpyAsync(wimp.data(), dw, sizeof(double) * 6 * S.nw, hipMemcpyDeviceToHost, h->stream));
     std::vector<WallDev> walls(std::max(S.nw, 1));
     if (S.nw)

@@ -122,3 +122,23 @@ def restart_and_compare(make_sim, tmp_path):
 def test_oracle_restart_from_reference_ido(tmp_path):
     from oracle.oracle import OracleSim
     restart_and_compare(OracleSim, tmp_path)
+
+
+def test_record_ido_from_the_scene_matches_reference(tmp_path):
+    """The .ido the oracle records after 10 steps of the fixture's scene
+    (rebuilt here with the seeded generator that made it), with models and
+    IactModels taken from the scene description, equals byte for byte the
+    first file the reference recorded (Consultant::RecordIDO)."""
+    from oracle.oracle import OracleSim
+    from vedo_amd import scenes
+    from vedo_amd.ido import record_ido
+    z = _z()
+    up, rec = (int(x) for x in z["cadence"])
+    w = scenes.box_scene(64, seed=18, law="ISwBSDB2F", dt=2e-5, steps=400, omega_max=5.0, spacing=1.99,
+                         jitter=0.01, vmax=0.02, offset=0.0, kn=2e3, e=0.6, mu=0.4)
+    wa = w.to_arrays(up_iact=up, record=rec)
+    o = OracleSim(wa)
+    o.step(rec)
+    out = tmp_path / "mine.ido"
+    record_ido(str(out), wa, o.state(), o.time, *o.active())
+    assert out.read_bytes() == z["ido0"].tobytes()

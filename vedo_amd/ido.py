@@ -252,7 +252,8 @@ def record_ido(path: str, wa: WorldArrays, state, time: float, lo, hi, st, templ
     (Consultant.cpp:213-229): the models and IactModels of the description
     (or of `template`, an .ido the scene was read from), the DOStatus list of
     `state` (a GpuSimulation.state() dict) and the records (lo, hi, statuses
-    as vg_get_active returns them, in key order)."""
+    as vg_get_active returns them, in key order).  Without `template` the
+    models and IactModels come from `wa` itself."""
     names = wa.model_names
     if template is not None:
         models, iacts = template.models, template.iacts
@@ -260,7 +261,23 @@ def record_ido(path: str, wa: WorldArrays, state, time: float, lo, hi, st, templ
                     zoi_lo=template.zoi_lo, zoi_hi=template.zoi_hi, pbc_switch=template.pbc_switch,
                     pbc_lo=template.pbc_lo, pbc_hi=template.pbc_hi, t_stop=template.t_stop)
     else:
-        raise ValueError("record_ido needs the .ido the scene was read from (models, IactModels, boundaries)")
+        # from the description alone: colour 0, no material options, the
+        # shape union holding the model's sizes (zero elsewhere), one
+        # IactModel per group pair with a law (master the lower group)
+        from .xmlio import _iact_models
+        models = []
+        for k, name in enumerate(names):
+            a = list(wa.model_attr[k]) + list(wa.model_hole[k]) + [0.0] * 4
+            models.append(IdoModel(name, wa.group_names[int(wa.model_group[k])], int(wa.model_behavior[k]),
+                                   int(wa.model_scope[k]), float(wa.model_density[k]),
+                                   float(wa.model_density_factor[k]), tuple(float(x) for x in wa.model_fext[k]), 0,
+                                   int(wa.model_shape[k]), struct.pack("<10d", *a)))
+        iacts = [IdoIact(m, s, eq, mech) for m, s, eq, mech in _iact_models(wa)]
+        lo3 = np.minimum(wa.zoi_lo, wa.zoi_hi)
+        hi3 = np.maximum(wa.zoi_lo, wa.zoi_hi)
+        head = dict(release=RELEASE, max_id=0, zoi_switch=bytes(int(x) for x in (lo3 != hi3)),
+                    zoi_lo=tuple(float(x) for x in lo3), zoi_hi=tuple(float(x) for x in hi3), pbc_switch=bytes(3),
+                    pbc_lo=(0.0, 0.0, 0.0), pbc_hi=(0.0, 0.0, 0.0), t_stop=wa.t_stop)
     n = len(state["pos"])
     vectors = np.stack([state[k] for k in ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")], 1)
     sts = [IdoStatus(names[int(wa.model_of[i])], 0, vectors[i]) for i in range(n)]

@@ -35,8 +35,8 @@ def load_golden(name):
     return z, wa
 
 
-GOLDEN_CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_nbs", "drum_kin", "box_assign",
-                "box_piston", "box_zoi", "box_hole"]
+GOLDEN_CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_bond", "box_bond_bt", "box_cyl",
+                "box_orbital", "box_nbs", "drum_kin", "box_assign", "box_piston", "box_zoi", "box_hole"]
 
 
 def step_golden(sim, z, step):

@@ -142,3 +142,35 @@ def test_record_ido_from_the_scene_matches_reference(tmp_path):
     out = tmp_path / "mine.ido"
     record_ido(str(out), wa, o.state(), o.time, *o.active())
     assert out.read_bytes() == z["ido0"].tobytes()
+
+
+def test_record_ido_after_zone_of_interest_erasures(tmp_path):
+    """After Consultant::CleanUp erased spheres (box_zoi), the remaining DOs
+    are renumbered: the recorded .ido names each by its own model (the walls
+    stay walls), reads back with the same shapes, and a record without the
+    per-DO models is refused instead of naming walls after the sphere model."""
+    from conftest import load_golden
+    from oracle.oracle import OracleSim
+    from vedo_amd.ido import ido_world, read_ido, record_ido
+    from vedo_amd.world import VedoError
+    z, wa = load_golden("box_zoi")
+    # the fixture's scene: the sphere model and the box's plate model
+    assert list(wa.model_shape) == [1, 7]
+    wa.model_names, wa.group_names = ["Sphere", "Plate"], ["Granule", "Wall"]
+    o = OracleSim(wa)
+    o.step(4)
+    ids = o.ids()
+    assert len(ids) < wa.n_do, "the zone of interest erased nothing"
+    model_of = wa.model_of[ids]
+    with pytest.raises(VedoError):
+        record_ido(str(tmp_path / "bad.ido"), wa, o.state(), o.time, *o.active())
+    out = tmp_path / "zoi.ido"
+    record_ido(str(out), wa, o.state(), o.time, *o.active(), model_of=model_of)
+    d = read_ido(str(out))
+    names = wa.model_names
+    assert [s.name for s in d.statuses] == [names[int(m)] for m in model_of]
+    back = ido_world(d, int(wa.up_iact), int(wa.record))
+    assert back.n_do == len(ids)
+    assert np.array_equal(back.model_shape[back.model_of], wa.model_shape[model_of])
+    assert int((back.model_shape[back.model_of] != 1).sum()) == int((wa.model_shape[wa.model_of] != 1).sum())
+    np.testing.assert_array_equal(back.pos, o.state()["pos"])

@@ -60,6 +60,8 @@ def test_oracle_bit_exact_vs_reference(name):
             assert np.array_equal(np.asarray(stt[f]).view(np.uint64), np.asarray(rs[f]).view(np.uint64)), f
         np.testing.assert_array_equal(stt["contact"], rs["flags"][:, 0])
         np.testing.assert_array_equal(stt["last_contact"], rs["flags"][:, 1])
+        np.testing.assert_array_equal(stt["bond"], rs["flags"][:, 2])
+        np.testing.assert_array_equal(stt["last_bond"], rs["flags"][:, 3])
         checked += 1
     assert checked == len(load_golden(name)[0]["dumps"])
 
@@ -70,3 +72,32 @@ def test_golden_cases_have_contacts():
         z, _ = load_golden(name)
         total = sum(int(z[f"d{k}_iact"]["active"].sum()) for k in range(len(z["dumps"])))
         assert total > 0, name
+
+
+def _final_pos(sim, n):
+    sim.step(n)
+    return sim.state()["pos"]
+
+
+@pytest.mark.parametrize("name", ["box_b2f", "box_bond", "box_bond_bt"])
+def test_bond_fixtures_reach_the_bond_branches(name):
+    """The bonded fixtures pin the bond code: without the normal bond
+    (NormalBondStrength = 0) or without the shear bond (ShearBondStrength = 0)
+    the same scene ends elsewhere, and the three spring-dashpot laws end in
+    three different places (ISwBSDB2F's LastBond branch, ISwBtSDBF's
+    history-free shear)."""
+    z, wa = load_golden(name)
+    last = len(z["dumps"]) - 1
+    ref = z[f"d{last}_state"][:, 0]
+    n = int(z["dumps"][-1])
+    for p, label in ((4, "normal bond"), (5, "shear bond")):
+        assert np.all(wa.iact_param[wa.iact_law > 0][:, p] > 0), label
+        w2 = load_golden(name)[1]
+        w2.iact_param = w2.iact_param.copy()
+        w2.iact_param[..., p] = 0.0
+        other = _final_pos(OracleSim(w2), n)
+        assert np.abs(other - ref).max() > 1e-6, f"{label} changes nothing"
+    others = {"box_b2f", "box_bond", "box_bond_bt"} - {name}
+    for o in others:
+        zo, _ = load_golden(o)
+        assert np.abs(zo[f"d{last}_state"][:, 0] - ref).max() > 1e-6, o

@@ -247,14 +247,29 @@ def ido_history(d: Ido):
     return r["first"].copy(), r["second"].copy(), st
 
 
-def record_ido(path: str, wa: WorldArrays, state, time: float, lo, hi, st, template: Ido = None):
+def record_ido(path: str, wa: WorldArrays, state, time: float, lo, hi, st, template: Ido = None,
+               model_of=None):
     """Write a scene's state and IactRecordTab as RecordIDO does
     (Consultant.cpp:213-229): the models and IactModels of the description
     (or of `template`, an .ido the scene was read from), the DOStatus list of
     `state` (a GpuSimulation.state() dict) and the records (lo, hi, statuses
     as vg_get_active returns them, in key order).  Without `template` the
-    models and IactModels come from `wa` itself."""
+    models and IactModels come from `wa` itself.
+
+    `model_of` is the model index of each DO of `state` (GpuSimulation.models()).
+    It may be left out only while the scene still holds every DO of `wa`: once
+    a zone of interest has erased DOs (Consultant::CleanUp), the remaining ones
+    are renumbered and `wa.model_of` no longer names them."""
     names = wa.model_names
+    n = len(state["pos"])
+    if model_of is None:
+        if n != wa.n_do:
+            raise VedoError(f"the state holds {n} DOs, the description {wa.n_do}: pass model_of "
+                            "(the model of each remaining DO)")
+        model_of = wa.model_of
+    model_of = np.asarray(model_of)
+    if len(model_of) != n:
+        raise VedoError(f"model_of names {len(model_of)} DOs, the state holds {n}")
     if template is not None:
         models, iacts = template.models, template.iacts
         head = dict(release=template.release, max_id=template.max_id, zoi_switch=template.zoi_switch,
@@ -278,9 +293,8 @@ def record_ido(path: str, wa: WorldArrays, state, time: float, lo, hi, st, templ
         head = dict(release=RELEASE, max_id=0, zoi_switch=bytes(int(x) for x in (lo3 != hi3)),
                     zoi_lo=tuple(float(x) for x in lo3), zoi_hi=tuple(float(x) for x in hi3), pbc_switch=bytes(3),
                     pbc_lo=(0.0, 0.0, 0.0), pbc_hi=(0.0, 0.0, 0.0), t_stop=wa.t_stop)
-    n = len(state["pos"])
     vectors = np.stack([state[k] for k in ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")], 1)
-    sts = [IdoStatus(names[int(wa.model_of[i])], 0, vectors[i]) for i in range(n)]
+    sts = [IdoStatus(names[int(model_of[i])], 0, vectors[i]) for i in range(n)]
     order = np.lexsort((np.asarray(hi), np.asarray(lo)))
     recs = np.zeros(len(order), RECORD_DTYPE)
     recs["first"], recs["second"] = np.asarray(lo)[order], np.asarray(hi)[order]

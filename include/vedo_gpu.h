@@ -273,6 +273,18 @@ int vg_scene_coordination_number(vg_scene *h, int scene, const double box[6], do
  * generator in a process uses 999 and RandomHitTestNum is 20000) that lie in
  * a sphere of the box. */
 int vg_scene_packing_density(vg_scene *h, int scene, const double box[6], int n_points, int seed, double *percent);
+/* vg_scene_packing_density without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * around HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, same result bit for bit; the sample points and
+ * the hit counter live in a fixed per-GPU scratch array of the library
+ * (VG_DENSITY_NOALLOC_MAX_POINTS points, 1.5 MiB, held until the process
+ * ends), so n_points above that maximum is rejected (VG_EINVAL).  Every call
+ * on one GPU shares that GPU's scratch: calls that overlap on one GPU (two
+ * threads, two handles) race and must be serialised by the caller. */
+#define VG_DENSITY_NOALLOC_MAX_POINTS 65536
+int vg_scene_packing_density_noalloc(vg_scene *h, int scene, const double box[6], int n_points, int seed,
+                                     double *percent);
 /* CalculateGranularTemperature (src/vedo/knight/Knight.cpp:1162-1225) of the
  * spheres of DOModel `model` (the description's index) inside a GSRectangle
  * (src/vedo/common/GSRectangle.cpp:15-30: centre, axes Ox and Oz, width along

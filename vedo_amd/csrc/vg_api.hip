@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include <rocprim/rocprim.hpp>
 #include <string>
@@ -1312,6 +1313,37 @@ extern "C" int vg_destroy(vg_scene *h)
     return 0;
 }
 
+// Scratch of vg_scene_packing_density_noalloc: the sample points (x, y, z
+// each) followed by the 64-bit hit counter.  Every GPU the library runs on
+// holds its own copy until the process ends.  In host code `scratch` is only
+// the symbol's host-side stand-in: kernels and copies use the device address
+// that scratch_setup looks up for each device.
+constexpr int SCRATCH_POINTS = VG_DENSITY_NOALLOC_MAX_POINTS;
+static __device__ __align__(256) double scratch[3 * SCRATCH_POINTS + 32];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[SCRATCH_MAX_DEVICES];
+
+// Once per device, with that device current (handle set-up).
+static int scratch_setup(vg_scene *h)
+{
+    if (h->device < 0 || h->device >= SCRATCH_MAX_DEVICES) return fail(h, VG_EUNSUPPORTED, "device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[h->device]) return 0;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+        return fail(h, VG_EHIP, "hipGetSymbolAddress of the scratch array failed");
+    g_scratch_dev[h->device] = static_cast<double *>(p);
+    return 0;
+}
+
+static double *scratch_of(int device)
+{
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+
 // Device buffers for ld sphere slots, wcap wall slots and scap scene slots.
 static int alloc_batch(vg_scene *h, int ld, int wcap, int scap, int ki, bool record)
 {
@@ -1321,6 +1353,7 @@ static int alloc_batch(vg_scene *h, int ld, int wcap, int scap, int ki, bool rec
     h->walls_h.assign(std::max(wcap, 1), WallDev());
     h->wall_used.assign(std::max(wcap, 1), 0);
     if (hipGetDevice(&h->device) != hipSuccess) return fail(h, VG_EHIP, "no HIP device");
+    if (int rc = scratch_setup(h)) return rc;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(h, VG_EHIP, "hipStreamCreate failed");
     (void)hipEventCreate(&h->ev0);
@@ -2208,6 +2241,56 @@ extern "C" int vg_scene_packing_density(vg_scene *h, int scene, const double *bo
     (void)hipFree(dp);
     (void)hipFree(hits);
     if (rc) return rc;
+    // hit / (float)RandomHitTestNum * 100.0, formed in float as the reference does
+    const float f = (float)(unsigned)hh / (float)n_points;
+    *percent = (double)f * 100.0;
+    return 0;
+}
+
+extern "C" int vg_scene_packing_density_noalloc(vg_scene *h, int scene, const double *box, int n_points, int seed,
+                                                double *percent)
+{
+    const SceneHost *S = nullptr;
+    if (!box || !percent || n_points <= 0) return VG_EINVAL;
+    if (int rc = knight_scene(h, scene, &S)) return rc;
+    if (n_points > SCRATCH_POINTS) {
+        char msg[128];
+        std::snprintf(msg, sizeof(msg), "vg_scene_packing_density_noalloc takes at most %d points (%d asked)",
+                      SCRATCH_POINTS, n_points);
+        return fail(h, VG_EINVAL, msg);
+    }
+    double *pts = scratch_of(h->device);
+    if (!pts) return fail(h, VG_EHIP, "scratch array not set up on this device");
+    unsigned long long *hits = reinterpret_cast<unsigned long long *>(pts + 3 * (size_t)SCRATCH_POINTS);
+    // the counter is cleared by a kernel (no memset node if the caller captures the stream)
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, h->stream, reinterpret_cast<int *>(hits), 2ll, 0);
+    // the points, drawn on the host in the reference's order, go up through a
+    // stack buffer chunk by chunk (each copy completes before the buffer is refilled)
+    constexpr int CHUNK = 2048;
+    double buf[3 * CHUNK];
+    NjrRng rng(seed);
+    for (int k0 = 0; k0 < n_points; k0 += CHUNK) {
+        const int m = std::min(CHUNK, n_points - k0);
+        for (int k = 0; k < m; ++k) {
+            buf[3 * k] = rng.uniform(box[0], box[1]);
+            buf[3 * k + 1] = rng.uniform(box[2], box[3]);
+            buf[3 * k + 2] = rng.uniform(box[4], box[5]);
+        }
+        if (hipMemcpyAsync(pts + 3 * (size_t)k0, buf, sizeof(double) * 3 * m, hipMemcpyHostToDevice, h->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess)
+            return fail(h, VG_EHIP, "upload failed");
+    }
+    const SceneDev &D = h->sdev[scene];
+    if (S->n > 0)
+        hipLaunchKernelGGL(k_density_hits, dim3(nblk(n_points, 256)), dim3(256), 0, h->stream, h->geo[h->cur], D.G,
+                           (long long)D.cell_off, (const int *)h->cstart, (const int *)h->cend, (const double *)pts,
+                           n_points, knight_box(box), hits);
+    unsigned long long hh = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&hh, hits, sizeof(hh), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return fail(h, VG_EHIP, "density kernel failed");
     // hit / (float)RandomHitTestNum * 100.0, formed in float as the reference does
     const float f = (float)(unsigned)hh / (float)n_points;
     *percent = (double)f * 100.0;

@@ -68,7 +68,7 @@ EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_se
            "vg_system_energy", "vg_batch_create", "vg_admit", "vg_retire", "vg_scene_status",
            "vg_scene_get_state", "vg_scene_get_active", "vg_scene_energy", "vg_scene_set_wall_rotation",
            "vg_batch_info", "vg_scene_coordination_number", "vg_scene_packing_density",
-           "vg_scene_granular_temperature")
+           "vg_scene_packing_density_noalloc", "vg_scene_granular_temperature")
 
 _lib = None
 
@@ -123,6 +123,7 @@ def lib():
         L.vg_set_skin.argtypes = [vp, C.c_double]
         L.vg_scene_coordination_number.argtypes = [vp, C.c_int, _D, _D]
         L.vg_scene_packing_density.argtypes = [vp, C.c_int, _D, C.c_int, C.c_int, _D]
+        L.vg_scene_packing_density_noalloc.argtypes = [vp, C.c_int, _D, C.c_int, C.c_int, _D]
         L.vg_scene_granular_temperature.argtypes = [vp, C.c_int, C.c_int, _D, _D, _D, _D, _D]
         L.vg_set_skin_auto.argtypes = [vp, C.c_double, C.c_double]
         L.vg_get_skin.restype = C.c_double

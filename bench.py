@@ -1,19 +1,23 @@
 """Benchmark of the DEM step on one MI355X.
 
-Workload (N=1 GPU): a settled bed of polydisperse spheres (log-normal radii, median 1 mm,
+Workload (N=1 GPU): a bed of polydisperse spheres (log-normal radii, median 1 mm,
 sigma_ln 0.2, clipped to [0.5, 2] mm, 64 radius classes, rho 2500) in a drum
 of 64 quasi-plates + 2 end plates (R = 0.15 m) rotating at 2 rad/s about a
 horizontal axis; ISwBSDBF linear spring-dashpot kn = 1e4 N/m, ks = 0.8 kn,
 e = 0.6, mu = 0.4; dt = 5e-6 s; LeapConsultant candidate lists rebuilt every
-UpIact steps.  The headline is FP64, the reference's arithmetic; the same
-workload then runs with the FP32/FP64-mixed pair solves (vg_set_precision),
-reported under "mixed" with the error of a mixed step against FP64 on the
-run's end state.  A "step" is one SimMediator::Run over all spheres.  The run is the configuration's own: the spheres start as a packed
-bed (see scenes.drum_scene) and settle into the drum, which turns at 2 rad/s
-from t = 0.  The neighbour-list skin is tuned on the first steps of the run,
---warmup steps follow, then --steps steps (10,000 by default, the
-configuration's step count) are timed.  --prep > 0 first relaxes the bed in
-the resting drum instead (see settle()).
+UpIact steps.  A "step" is one SimMediator::Run over all spheres.  The run is
+the configuration's own: the spheres start as a packed bed (see
+scenes.drum_scene) and settle into the drum, which turns at 2 rad/s from
+t = 0.  The neighbour-list skin is tuned on the first steps of the run,
+--warmup steps follow, the run goes on to the next rebuild boundary, and then
+--steps steps (10,000 by default, the configuration's step count) are timed.
+UpIact is the largest period not above --up-iact that divides --steps, so the
+window is whole rebuild periods with their rebuilds.  The headline is FP64,
+the reference's arithmetic; a second handle brought to the same state steps
+the same window with the FP32/FP64-mixed pair solves (vg_set_precision),
+reported under "mixed" with the error of one mixed step against one FP64 step
+from the window's end state (contact histories included).  --prep > 0 first
+relaxes the bed in the resting drum instead (see settle()).
 
 Prints one JSON line (see the metric/roofline/cpu_baseline fields).
 """
@@ -81,16 +85,31 @@ def cpu_baseline(state, wa, n_gpu, seconds):
     for f in ("pos", "vel", "ox", "oz", "av"):
         setattr(sub, f, np.ascontiguousarray(state[f][ids]))
     sim = OracleSim(sub)
-    t0 = time.perf_counter()
-    steps = 0
-    while time.perf_counter() - t0 < seconds:
-        sim.step(1)
-        steps += 1
-    el = time.perf_counter() - t0
-    return {"value": len(keep) * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (FP64 C restatement, 1 thread) on the {len(keep)} spheres of the same bed in "
-                      f"the axial slab {y_lo:.4f} <= y <= {y_hi:.4f} m (middle of the drum) with its {len(walls)} "
-                      f"walls held fixed, {steps} steps in {el:.1f} s"}
+    # one core: the process is pinned to the first core it may use while the oracle runs
+    nproc, model = cpu_info()
+    old_aff = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else None
+    core = min(old_aff) if old_aff else None
+    if core is not None:
+        os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.perf_counter()
+        steps = 0
+        while time.perf_counter() - t0 < seconds:
+            sim.step(1)
+            steps += 1
+        el = time.perf_counter() - t0
+    finally:
+        if old_aff:
+            os.sched_setaffinity(0, old_aff)
+    out = {"value": len(keep) * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+           "nproc": nproc, "cpu_model": model, "pinned_core": core,
+           "sample": f"oracle (FP64 C restatement, 1 thread pinned to core {core}) on the {len(keep)} spheres of "
+                     f"the same bed in the axial slab {y_lo:.4f} <= y <= {y_hi:.4f} m (middle of the drum) with its "
+                     f"{len(walls)} walls held fixed, {steps} steps in {el:.1f} s"}
+    cal = cpu_calibration()
+    if cal:
+        out["calibration"] = cal
+    return out
 
 
 def settle(sim, n, total_mass, max_steps, chunk=100, v_rms_stop=2e-3):
@@ -216,15 +235,21 @@ def serving_run(count, up_iact, seed=5, loop=True):
     for r in reqs:
         serving.build_scene(r, up_iact=up_iact)
     need = sum(r.n for r in reqs)
-    cap = min(need, 4_000_000)
+    # a device layout for half the stream's spheres (at least its largest
+    # scene): admission by footprint binds, later requests wait for slots
+    cap = max(max(r.n for r in reqs), need // 2)
     log(f"[bench] serving: {count} requests, {need} spheres in all, capacity {cap} spheres "
         f"({cap * serving.slot_bytes() / 2 ** 30:.1f} GiB)")
     srv = serving.Server(capacity=cap, up_iact=up_iact, max_walls=80 * count, max_scenes=max(count, 16),
                          keep_results=False)
     st = srv.run(reqs)
     srv.close()
+    waits = np.array([r.t_admit - r.arrival for r in reqs])
+    period = max(w for w in waits) if len(waits) else 0.0
     out = {"requests": count, "rate_per_s": 20.0, "spheres": need, "capacity_spheres": cap, "failed": st.failed,
-           "seconds": st.wall, "particle_steps_per_s": st.throughput, "p50_s": st.pct(50), "p99_s": st.pct(99)}
+           "seconds": st.wall, "particle_steps_per_s": st.throughput, "p50_s": st.pct(50), "p99_s": st.pct(99),
+           "admission_wait_p50_s": float(np.percentile(waits, 50)), "admission_wait_max_s": float(period),
+           "requests_waiting_over_1s": int((waits > 1.0).sum())}
     log(f"[bench] serving: {st.wall:.2f} s, {st.throughput:.3e} particle-steps/s, "
         f"time-to-result p50 {st.pct(50):.2f} s p99 {st.pct(99):.2f} s")
     if loop:
@@ -236,17 +261,30 @@ def serving_run(count, up_iact, seed=5, loop=True):
     return out
 
 
-def drum_run(args, wa, wall_ids, c, a, mixed=False, skin=None, tune=True):
-    """The drum workload on a fresh handle: optional resting prep, the drum
-    turning at 2 rad/s, skin tuning (unless given), --warmup steps, then
-    args.steps timed steps.  Returns (handle, measurements)."""
-    from vedo_amd.gpu import PRECISION_MIXED, GpuSimulation
+def window_up_iact(steps, up_max):
+    """The largest rebuild period (UpIact) not above `up_max` that divides
+    `steps`: the timed window is then whole rebuild periods, each with its
+    candidate-list rebuild."""
+    for u in range(min(up_max, max(steps, 1)), 0, -1):
+        if steps % u == 0:
+            return u
+    return 1
+
+
+def drum_prepare(args, wa, wall_ids, c, a, skin=None, steps=None):
+    """A fresh FP64 handle of the drum workload advanced to the start of the
+    timed window: the optional resting prep, the drum turning at 2 rad/s from
+    then on, the neighbour-list skin tuned on the live run, --warmup steps,
+    then on to the next rebuild boundary.  With `skin` and `steps` given (a
+    second handle for the other precision) the handle takes that skin and
+    runs the same number of steps: results do not depend on the skin, so it
+    arrives at the first handle's state bit for bit.  Returns
+    (handle, skin, steps run)."""
+    from vedo_amd.gpu import GpuSimulation
     t = time.perf_counter()
     sim = GpuSimulation(wa)
-    if mixed:
-        sim.set_precision(PRECISION_MIXED)
-    log(f"[bench] created on GPU ({time.perf_counter() - t:.1f} s){' (mixed precision)' if mixed else ''}")
-    done, quenches = 0, 0
+    log(f"[bench] created on GPU ({time.perf_counter() - t:.1f} s)")
+    done = 0
     if args.prep > 0:
         t = time.perf_counter()
         sim.set_wall_rotation(wall_ids, c, a, 0.0, 0.0)
@@ -254,41 +292,80 @@ def drum_run(args, wa, wall_ids, c, a, mixed=False, skin=None, tune=True):
         total_mass = float(np.sum(wa.model_density[wa.model_of[:args.n]] * 4.0 / 3.0 * np.pi * r_sph ** 3))
         done, quenches = settle(sim, args.n, total_mass, args.prep)
         log(f"[bench] prep {done} steps, {quenches} quenches, {time.perf_counter() - t:.1f} s")
-    # the drum turns at 2 rad/s (from t = 0 unless a resting prep ran)
     sim.set_wall_rotation(wall_ids, c, a, 2.0, sim.time)
+    up = int(wa.up_iact)
     if skin is not None:
         sim.set_skin(skin)
-    elif tune:
+        sim.step(steps - done)
+        return sim, skin, steps
+    if args.skin is not None:
+        sim.set_skin(args.skin)
+    elif not args.no_tune:
         from vedo_amd.autotune import tune_skin
-        skin, table = tune_skin(sim, float(wa.model_attr[wa.model_shape == 1, 0].min()), args.up_iact)
+        r0 = done
+        skin, table = tune_skin(sim, float(wa.model_attr[wa.model_shape == 1, 0].min()), up)
+        done += 4 * (up + 2)
+        assert r0 + 4 * (up + 2) == done
         log(f"[bench] skin autotune: {({f'{k * 1e3:.3f}mm': round(v * 1e6, 1) for k, v in table.items()})} "
             f"us/step -> {skin * 1e3:.3f} mm")
-    skin = sim.skin
     sim.step(args.warmup)
+    done += args.warmup
+    # on to a rebuild boundary: the window's steps are whole rebuild periods
+    align = (-done) % up
+    sim.step(align)
+    done += align
+    return sim, sim.skin, done
+
+
+def time_window(sim, n, steps):
+    """Times `steps` steps (whole rebuild periods) bracketed by device
+    synchronisation; the step kernel's launches and the rebuilds are also
+    timed with HIP events on the library's stream."""
     s0 = sim.stats()
-    # timed region: K steps bracketed by device synchronisation
-    sim.kernel_timing(args.steps)
+    sim.kernel_timing(steps)
     sim.sync()
     t0 = time.perf_counter()
-    sim.step(args.steps, sync=False)
+    sim.step(steps, sync=False)
     sim.sync()
     wall = time.perf_counter() - t0
     k_ms, k_launches = sim.timing()
+    r_ms, r_count = sim.rebuild_timing()
     s1 = sim.stats()
     stats = {k: s1[k] - s0[k] for k in ("rebuilds", "relists", "contact_steps")}
-    stats["max_fill"], stats["capacity"] = s1["max_fill"], s1["capacity"]
-    n = args.n
     # average number of contact pairs each timed step solved (on-device count)
-    c_act = stats["contact_steps"] / args.steps
+    c_act = stats["contact_steps"] / steps
     bytes_step = BYTES_PER_SPHERE * n + BYTES_PER_CONTACT * c_act
     k_avg_s = k_ms / 1e3 / max(k_launches, 1)
-    log(f"[bench] timed {args.steps} steps in {wall:.3f} s; step kernel avg {k_avg_s * 1e6:.1f} us over "
-        f"{k_launches} launches; {c_act:.0f} contact pairs per step; {stats}")
-    return sim, {"value": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
-                 "kernel_avg_us": k_avg_s * 1e6, "achieved_gbs": bytes_step / k_avg_s / 1e9,
-                 "bytes_per_step": bytes_step, "active_contacts": c_act, "skin": skin,
-                 "rebuilds": stats["rebuilds"], "relists": stats["relists"], "prep_steps": done,
-                 "prep_quenches": quenches}
+    log(f"[bench] timed {steps} steps in {wall:.3f} s: step kernel avg {k_avg_s * 1e6:.1f} us over {k_launches} "
+        f"launches, {r_count} rebuilds {r_ms:.2f} ms in all, {c_act:.0f} contact pairs per step, {stats}")
+    return {"value": n * steps / wall, "ms_per_step": wall * 1e3 / steps, "wall_s": wall,
+            "kernel_avg_us": k_avg_s * 1e6, "kernel_launches": k_launches, "achieved_gbs": bytes_step / k_avg_s / 1e9,
+            "bytes_per_step": bytes_step, "active_contacts": c_act, "rebuilds": r_count,
+            "rebuild_ms_each": r_ms / max(r_count, 1), "rebuild_share": r_ms / 1e3 / wall,
+            "step_kernel_share": k_ms / 1e3 / wall, "relists": stats["relists"]}
+
+
+def cpu_info():
+    """(nproc, CPU model) of this host."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count(), model
+
+
+def cpu_calibration():
+    """The oracle against the compiled reference on one core of the build
+    host (oracle/cpu_calibration.json, written by oracle/calibrate_cpu.py)."""
+    try:
+        with open(os.path.join(ROOT, "oracle", "cpu_calibration.json")) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -298,7 +375,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--prep", type=int, default=0, help="untimed settling steps in the resting drum")
-    ap.add_argument("--up-iact", type=int, default=100)
+    ap.add_argument("--up-iact", type=int, default=100,
+                    help="largest candidate-list rebuild period; the run uses the largest one that divides --steps")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--skin", type=float, default=None, help="neighbour-list skin in m (skips the tuning)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -309,9 +387,8 @@ def main():
     ap.add_argument("--no-loop", action="store_true", help="skip the one-scene-at-a-time comparisons")
     ap.add_argument("--batch-up-iact", type=int, default=20, help="UpIact of the batched and served scenes")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
-                    help="arithmetic of the headline run (fp64: the reference's)")
-    ap.add_argument("--mixed-steps", type=int, default=None,
-                    help="steps of the FP32/FP64-mixed run of the same workload (default --steps; 0: skip)")
+                    help="arithmetic of the headline (fp64: the reference's); the other one is reported beside it")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32/FP64-mixed leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU holds the whole domain); "
@@ -319,18 +396,33 @@ def main():
         sys.exit(2)
 
     from vedo_amd import scenes
-    from vedo_amd.gpu import copy_bandwidth
+    from vedo_amd.gpu import PRECISION_MIXED, copy_bandwidth
 
     copy_gbps = copy_bandwidth(1 << 30, 10)
     log(f"[bench] streaming copy kernel: {copy_gbps:.0f} GB/s")
 
+    up = window_up_iact(args.steps, args.up_iact)
     t = time.perf_counter()
-    wa, wall_ids, c, a = scenes.drum_scene(args.n, arrays=True, up_iact=args.up_iact)
-    log(f"[bench] scene: {args.n} spheres + {len(wall_ids)} walls, drum length {wa.pos[:, 1].max():.3f} m "
-        f"({time.perf_counter() - t:.1f} s)")
-    headline_mixed = args.precision == "mixed"
-    sim, run = drum_run(args, wa, wall_ids, c, a, mixed=headline_mixed, skin=args.skin, tune=not args.no_tune)
+    wa, wall_ids, c, a = scenes.drum_scene(args.n, arrays=True, up_iact=up)
+    log(f"[bench] scene: {args.n} spheres + {len(wall_ids)} walls, drum length {wa.pos[:, 1].max():.3f} m, "
+        f"UpIact {up} ({time.perf_counter() - t:.1f} s)")
     n = args.n
+    # two handles at the same state at the start of the same window: one
+    # steps in FP64, the other in the FP32/FP64-mixed precision
+    sim, skin, done = drum_prepare(args, wa, wall_ids, c, a)
+    other = None
+    if not args.no_mixed:
+        other, _, _ = drum_prepare(args, wa, wall_ids, c, a, skin=skin, steps=done)
+    log(f"[bench] window starts after {done} steps (a rebuild boundary), skin {skin * 1e3:.3f} mm")
+    run64 = time_window(sim, n, args.steps)
+    runmix = None
+    if other is not None:
+        other.set_precision(PRECISION_MIXED)
+        runmix = time_window(other, n, args.steps)
+        other.close()
+        del other
+    headline_mixed = args.precision == "mixed" and runmix is not None
+    run = runmix if headline_mixed else run64
     traffic, traffic_src = measured_traffic()
     out = {
         "metric": METRIC, "value": run["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -338,44 +430,43 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32" if headline_mixed else "f64",
         "data": "synthetic",
         "config": {"workload": "rotating-drum polydisperse spheres, ISwBSDBF, LeapConsultant", "spheres": n,
-                   "walls": len(wall_ids), "radius_classes": 64, "dt": wa.dt, "up_iact": args.up_iact,
-                   "prep_steps": run["prep_steps"], "prep_quenches": run["prep_quenches"],
-                   "active_contacts": run["active_contacts"], "neighbour_skin_m": run["skin"],
-                   "precision": args.precision, "parallelism": "single GPU"},
+                   "walls": len(wall_ids), "radius_classes": 64, "dt": wa.dt, "up_iact": up,
+                   "window": f"steps {done + 1}-{done + args.steps} of the run: {args.steps // up} whole rebuild "
+                             f"periods, their {run['rebuilds']} rebuilds included",
+                   "prep_steps": done, "active_contacts": run["active_contacts"], "neighbour_skin_m": skin,
+                   "precision": "mixed" if headline_mixed else "fp64", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": run["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": run["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "copy_kernel_gbps": copy_gbps, "frac_of_copy": run["achieved_gbs"] / copy_gbps,
                      "kernel": "k_step", "kernel_avg_us": run["kernel_avg_us"],
                      "algorithmic_bytes_per_launch": run["bytes_per_step"]},
-        "rebuilds": run["rebuilds"], "relists": run["relists"],
+        "rebuild": {"count": run["rebuilds"], "ms_each": run["rebuild_ms_each"], "share_of_window": run["rebuild_share"],
+                    "step_kernel_share_of_window": run["step_kernel_share"]},
+        "relists": run["relists"],
     }
+    st = sim.state()
     if not args.no_cpu:
-        st = sim.state()
         out["cpu_baseline"] = cpu_baseline(st, wa, n, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
-    mixed_steps = args.steps if args.mixed_steps is None else args.mixed_steps
-    if mixed_steps > 0 and not headline_mixed:
-        # the same workload with the FP32/FP64-mixed pair solves, and the
-        # error of one mixed step against FP64 from the end state of the run
+    if runmix is not None:
+        # error of one mixed step against one FP64 step from the window's end
+        # state (positions, velocities, orientations and the contacts' histories)
         import dataclasses
         from vedo_amd.precision import compare_precisions
-        st = sim.state()
+        hist = sim.active()
         wa_end = dataclasses.replace(wa, pos=st["pos"], vel=st["vel"], ox=st["ox"], oz=st["oz"], av=st["av"],
                                      t_current=sim.time)
         sim.close()
         del sim
-        margs = argparse.Namespace(**vars(args))
-        margs.steps = mixed_steps
-        msim, mrun = drum_run(margs, wa, wall_ids, c, a, mixed=True, skin=run["skin"], tune=False)
-        msim.close()
-        del msim
-        err = compare_precisions(wa_end, steps=1)
+        err = compare_precisions(wa_end, steps=1, history=hist)
         log(f"[bench] mixed vs fp64 on the end state: {err}")
-        out["mixed"] = {"value": mrun["value"], "unit": "particle-steps/s", "steps": mixed_steps,
-                        "ms_per_step": mrun["ms_per_step"], "kernel_avg_us": mrun["kernel_avg_us"],
-                        "achieved_gbs": mrun["achieved_gbs"], "frac": mrun["achieved_gbs"] / HBM_PEAK_GBS,
-                        "speedup_vs_fp64": mrun["value"] / run["value"], "error_vs_fp64": err}
+        side, name = (run64, "fp64") if headline_mixed else (runmix, "mixed")
+        out[name] = {"value": side["value"], "unit": "particle-steps/s", "steps": args.steps,
+                     "ms_per_step": side["ms_per_step"], "kernel_avg_us": side["kernel_avg_us"],
+                     "achieved_gbs": side["achieved_gbs"], "frac": side["achieved_gbs"] / HBM_PEAK_GBS,
+                     "rebuild_ms_each": side["rebuild_ms_each"], "window": "the headline's (same start state)",
+                     "speedup_mixed_vs_fp64": runmix["value"] / run64["value"], "error_vs_fp64": err}
     else:
         sim.close()
         del sim

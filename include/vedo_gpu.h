@@ -202,6 +202,11 @@ int vg_get_active(vg_scene *h, long long *n, unsigned *lo, unsigned *hi, vg_stat
  * of the last vg_step call (all its kernels) and its step count. */
 int vg_kernel_timing(vg_scene *h, int max_launches);
 int vg_step_timing(vg_scene *h, double *ms, int *launches);
+/* No counterpart (measurement): after vg_kernel_timing(h, k), the candidate-
+ * list rebuilds of the following steps (every UpIact-th step's
+ * LeapConsultant::Reset + history carry, host work included) are bracketed by
+ * device events too; their summed duration and count. */
+int vg_rebuild_timing(vg_scene *h, double *ms, int *rebuilds);
 /* Counters since vg_create: rebuilds, relists, max neighbour-list fill, the
  * list capacity, and contact_steps, the sum over steps of the number of
  * sphere-sphere and sphere-wall pairs left in contact by the step's solve.

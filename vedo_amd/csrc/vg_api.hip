@@ -170,6 +170,11 @@ struct vg_scene {
     // per-launch events around the step kernel (vg_kernel_timing)
     std::vector<hipEvent_t> kev;
     int kev_used = 0;
+    std::vector<hipEvent_t> rev;   // candidate-list rebuilds bracketed while kernel timing is on
+    int rev_used = 0;
+    // the rebuild due after the last step of a vg_step call waits for the
+    // handle's next use, so that the step's records stay readable in place
+    bool rebuild_pending = false;
 };
 
 #define HIPCHK(x)                                                                              \
@@ -1300,6 +1305,7 @@ static void free_scene(vg_scene *h)
     F(h->zoi_dead); F(h->zoi_cnt); F(h->zoi_rank); F(h->zoi_pre);
     if (h->pin) (void)hipHostFree(h->pin);
     for (auto e : h->kev) (void)hipEventDestroy(e);
+    for (auto e : h->rev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1565,8 +1571,35 @@ static int snapshot_records(vg_scene *h)
     return 0;
 }
 
+// LeapConsultant::NextStep's rebuild (candidate list, history carry, zone of
+// interest), bracketed by timing events while kernel timing is on.
+static int do_rebuild(vg_scene *h)
+{
+    const bool rt = 2 * (h->rev_used + 1) <= (int)h->rev.size();
+    if (rt) HIPCHK(hipEventRecord(h->rev[2 * h->rev_used], h->stream));
+    const int rc = rebuild(h);
+    if (rt && !rc) {
+        HIPCHK(hipEventRecord(h->rev[2 * h->rev_used + 1], h->stream));
+        h->rev_used++;
+    }
+    return rc;
+}
+
+// A deferred rebuild runs before anything that needs the rebuilt lists: the
+// next step (the same device work in the same order as a rebuild at the end of
+// the previous step), or a query of the candidate list.  With keep_records the
+// step's records are kept for the exports first, as an immediate rebuild does.
+static int flush_rebuild(vg_scene *h, bool keep_records)
+{
+    if (!h->rebuild_pending) return 0;
+    h->rebuild_pending = false;
+    if (keep_records && h->ent_valid && snapshot_records(h)) return VG_EHIP;
+    return do_rebuild(h);
+}
+
 static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>> *assign_spheres)
 {
+    if (int rc = flush_rebuild(h, false)) return rc;
     if (h->n == 0) {
         // nothing placed yet: the first boundary places the queued scenes
         bool queued = false;
@@ -1642,8 +1675,15 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
     h->snap_valid = false;
     HIPCHK(hipGetLastError());
     if (h->round % h->up_iact == 0) {
+        // the last step of a call: its records are read in place until the
+        // handle is next used (a zone-of-interest clean-up erases DOs at the
+        // rebuild, which the exports must see: that rebuild runs now)
+        if (rec && !h->batch && !h->any_zoi) {
+            h->rebuild_pending = true;
+            return 0;
+        }
         if (rec && !h->batch && snapshot_records(h)) return VG_EHIP;
-        return rebuild(h);
+        return do_rebuild(h);
     }
     return 0;
 }
@@ -1878,6 +1918,7 @@ extern "C" int vg_num_interactions(vg_scene *h, long long *n)
     if (!h || !n) return VG_EINVAL;
     const SceneHost *S = scene0(h);
     if (!S) return fail(h, VG_EINVAL, "vg_num_interactions needs a one-scene handle");
+    if (int rc = flush_rebuild(h, true)) return rc;
     if (h->n_iact < 0) {
         unsigned long long *d = nullptr, c[2] = {0, 0};
         if (dalloc(h, &d, 2)) return VG_ENOMEM;
@@ -1972,14 +2013,18 @@ static int get_active(vg_scene *h, int slot, long long *n, unsigned *lo, unsigne
         unsigned long long *dcnt = nullptr;
         unsigned *d_sid = nullptr, *d_wid = nullptr;
         if (dalloc(h, &dcnt, 1) || id_maps(h, slot, &d_sid, &d_wid)) return VG_ENOMEM;
-        if (export_active(h, slot, nullptr, 0, dcnt, d_sid, d_wid, true)) return VG_EHIP;
+        // after a plain step the entries hold the step's Backup as pending; at
+        // a rebuild step (its rebuild deferred) the records are read before it,
+        // as RebuildIactRecordTab reads them
+        const bool backup = !h->rebuild_pending;
+        if (export_active(h, slot, nullptr, 0, dcnt, d_sid, d_wid, backup)) return VG_EHIP;
         HIPCHK(hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         if (st) {
             const size_t cap = (size_t)std::max(*n, 0ll);
             ActiveOut *dout = nullptr;
             if (dalloc(h, &dout, cap)) return VG_ENOMEM;
-            if (export_active(h, slot, dout, cap, dcnt, d_sid, d_wid, true)) return VG_EHIP;
+            if (export_active(h, slot, dout, cap, dcnt, d_sid, d_wid, backup)) return VG_EHIP;
             ho.resize(std::min<size_t>(cnt, cap));
             if (!ho.empty())
                 HIPCHK(hipMemcpyAsync(ho.data(), dout, sizeof(ActiveOut) * ho.size(), hipMemcpyDeviceToHost, h->stream));
@@ -2149,6 +2194,7 @@ extern "C" int vg_scene_coordination_number(vg_scene *h, int scene, const double
     if (int rc = knight_scene(h, scene, &S)) return rc;
     if (!S->all_sphere_laws)
         return fail(h, VG_EUNSUPPORTED, "coordination number needs an IactModel for every pair of sphere groups");
+    if (int rc = flush_rebuild(h, true)) return rc;   // the touching pairs of the rebuilt candidate list
     unsigned long long *acc = nullptr, c[2] = {0, 0};
     if (dalloc(h, &acc, 2)) return VG_ENOMEM;
     int rc = 0;
@@ -2387,13 +2433,38 @@ extern "C" int vg_kernel_timing(vg_scene *h, int max_launches)
     if (!h || max_launches < 0) return VG_EINVAL;
     HIPCHK(hipStreamSynchronize(h->stream));
     for (auto e : h->kev) HIPCHK(hipEventDestroy(e));
+    for (auto e : h->rev) HIPCHK(hipEventDestroy(e));
     h->kev.clear();
+    h->rev.clear();
     h->kev_used = 0;
+    h->rev_used = 0;
     for (int k = 0; k < 2 * max_launches; ++k) {
         hipEvent_t e;
         HIPCHK(hipEventCreate(&e));
         h->kev.push_back(e);
     }
+    // a rebuild comes at most every step
+    const int max_rebuilds = std::min(max_launches, 4096);
+    for (int k = 0; k < 2 * max_rebuilds; ++k) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        h->rev.push_back(e);
+    }
+    return 0;
+}
+
+extern "C" int vg_rebuild_timing(vg_scene *h, double *ms, int *rebuilds)
+{
+    if (!h) return VG_EINVAL;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    double tot = 0.0;
+    for (int k = 0; k < h->rev_used; ++k) {
+        float m = 0.0f;
+        HIPCHK(hipEventElapsedTime(&m, h->rev[2 * k], h->rev[2 * k + 1]));
+        tot += m;
+    }
+    if (ms) *ms = tot;
+    if (rebuilds) *rebuilds = h->rev_used;
     return 0;
 }
 

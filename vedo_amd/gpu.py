@@ -64,7 +64,7 @@ assert STATUS_DTYPE.itemsize == C.sizeof(VgStatus)
 
 EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_set_history", "vg_step",
            "vg_sync", "vg_step_assigned", "vg_set_wall_rotation", "vg_get_state", "vg_time",
-           "vg_num_interactions", "vg_scene_num_do", "vg_scene_models", "vg_set_skin", "vg_set_skin_auto", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_stats", "vg_copy_bench",
+           "vg_num_interactions", "vg_scene_num_do", "vg_scene_models", "vg_set_skin", "vg_set_skin_auto", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_rebuild_timing", "vg_stats", "vg_copy_bench",
            "vg_system_energy", "vg_batch_create", "vg_admit", "vg_retire", "vg_scene_status",
            "vg_scene_get_state", "vg_scene_get_active", "vg_scene_energy", "vg_scene_set_wall_rotation",
            "vg_batch_info", "vg_scene_coordination_number", "vg_scene_packing_density",
@@ -120,6 +120,7 @@ def lib():
         L.vg_get_active.argtypes = [vp, C.POINTER(C.c_longlong), _U, _U, C.POINTER(VgStatus)]
         L.vg_step_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.vg_kernel_timing.argtypes = [vp, C.c_int]
+        L.vg_rebuild_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.vg_set_skin.argtypes = [vp, C.c_double]
         L.vg_scene_coordination_number.argtypes = [vp, C.c_int, _D, _D]
         L.vg_scene_packing_density.argtypes = [vp, C.c_int, _D, C.c_int, C.c_int, _D]
@@ -359,6 +360,12 @@ class GpuSimulation(_Knight):
     def timing(self):
         ms, k = C.c_double(), C.c_int()
         self._check(self.L.vg_step_timing(self.h, C.byref(ms), C.byref(k)))
+        return ms.value, k.value
+
+    def rebuild_timing(self):
+        """(summed ms, count) of the rebuilds since kernel_timing()."""
+        ms, k = C.c_double(), C.c_int()
+        self._check(self.L.vg_rebuild_timing(self.h, C.byref(ms), C.byref(k)))
         return ms.value, k.value
 
     def stats(self):

@@ -33,13 +33,20 @@ def _rel_err(x, ref, tau=TAU, scale=0.0):
     return e
 
 
-def compare_precisions(wa, steps: int = 1, sphere_mask=None):
-    """Step `wa` `steps` times with each precision and compare; returns a dict."""
+def compare_precisions(wa, steps: int = 1, sphere_mask=None, history=None):
+    """Step `wa` `steps` times with each precision and compare; returns a dict.
+    `history` = (lo, hi, statuses) of the state's IactRecordTab (as
+    GpuSimulation.active() returns it) restores the contacts' histories on both
+    handles, so that the comparison starts from the full state, shear springs
+    included."""
     from .gpu import PRECISION_MIXED, GpuSimulation
 
     out = {}
     sims = [GpuSimulation(wa), GpuSimulation(wa)]
     try:
+        if history is not None:
+            for s in sims:
+                s.set_history(*history)
         sims[1].set_precision(PRECISION_MIXED)
         for s in sims:
             s.step(steps)

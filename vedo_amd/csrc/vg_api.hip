@@ -127,6 +127,10 @@ struct vg_scene {
     int *ovf = nullptr;                 // [scap] candidate-capacity overflow per scene
     unsigned *sid[2] = {nullptr, nullptr};   // scene slot of each sphere (quantile sorts)
     double *qval = nullptr;
+    QselTarget *qsel_tg = nullptr;   // quantile selection: 6 targets per scene slot
+    unsigned *qsel_hist = nullptr;   // 256 counters per target
+    int *qmap = nullptr;             // scene slot -> live-scene index, -1
+
     double *safe = nullptr;
     int *lin = nullptr;
     int *cstart = nullptr, *cend = nullptr;
@@ -967,49 +971,36 @@ static int rebuild(vg_scene *h)
     long long cells = 0;
     if (S_live > 0) {
         // 2%-trimmed spans pick the comparator (LeapConsultant.cpp:139-191):
-        // each coordinate sorted over all spheres, then stably by scene slot,
-        // which leaves every live scene's coordinates sorted in its rank range
-        std::vector<long long> qi(2 * S_live);
+        // the elements the reference reads from each sorted coordinate,
+        // found by radix selection over every live scene at once
+        std::vector<QselTarget> tg((size_t)6 * S_live);
+        std::vector<int> qmap(h->scap, -1);
         const double discard = 0.02, keep = 1.0 - discard;
         for (int q = 0; q < S_live; ++q) {
             const SceneHost &S = h->sc[live[q]];
             const double n = (double)S.n;
-            const long long base = h->sdev[live[q]].rank_off;
+            qmap[live[q]] = q;
             // (size_t)(discard*n) and (size_t)(keep*n - 1) as the reference indexes, clamped to the scene
-            const long long ilo = std::min((long long)(discard * n), (long long)S.n - 1);
+            const long long ilo = std::max(0ll, std::min((long long)(discard * n), (long long)S.n - 1));
             const long long ihi = std::max(0ll, std::min((long long)std::floor(keep * n - 1.0), (long long)S.n - 1));
-            qi[2 * q] = base + ilo;
-            qi[2 * q + 1] = base + ihi;
-        }
-        HIPCHK(hipMemcpy(h->qidx, qi.data(), sizeof(long long) * qi.size(), hipMemcpyHostToDevice));
-        pc.mark("bounds launch + qidx");
-        const bool by_scene = S_live > 1 || retiring;
-        int sbits = 1;
-        while ((1 << sbits) <= max_slot + 1) ++sbits;
-        size_t need = 0, need2 = 0;
-        (void)rocprim::radix_sort_pairs(nullptr, need, h->xs[0], h->xs[1], h->sid[0], h->sid[1], (size_t)n_total,
-                                        0, 64, st);
-        (void)rocprim::radix_sort_pairs(nullptr, need2, h->sid[1], h->sid[0], h->xs[1], h->xs[0], (size_t)n_total,
-                                        0, sbits, st);
-        pc.mark("sort size queries");
-        if (ensure_tmp(h, std::max(need, need2))) return VG_EHIP;
-        double *coord[3][2] = {{h->xs[0], h->xs[1]}, {h->ys[0], h->ys[1]}, {h->zs[0], h->zs[1]}};
-        for (int c = 0; c < 3; ++c) {
-            if (by_scene) {
-                // sid[0] holds the slots in layout order; keep it for the next coordinate
-                HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, coord[c][0], coord[c][1], h->sid[0],
-                                                 (unsigned *)h->keys, (size_t)n_total, 0, 64, st));
-                HIPCHK(rocprim::radix_sort_pairs(h->tmp, need2, (unsigned *)h->keys, (unsigned *)h->skeys,
-                                                 coord[c][1], coord[c][0], (size_t)n_total, 0, sbits, st));
-            } else {
-                HIPCHK(rocprim::radix_sort_keys(h->tmp, need, coord[c][0], coord[c][1], (size_t)n_total, 0, 64, st));
-                HIPCHK(hipMemcpyAsync(coord[c][0], coord[c][1], sizeof(double) * n_total, hipMemcpyDeviceToDevice,
-                                      st));
+            for (int c = 0; c < 3; ++c) {
+                tg[(size_t)6 * q + 2 * c] = QselTarget{0ull, (unsigned long long)ilo};
+                tg[(size_t)6 * q + 2 * c + 1] = QselTarget{0ull, (unsigned long long)ihi};
             }
         }
-        hipLaunchKernelGGL(k_quantiles, dim3(nblk(S_live, 64)), dim3(64), 0, st, (const double *)h->xs[0],
-                           (const double *)h->ys[0], (const double *)h->zs[0], (const long long *)h->qidx, S_live,
-                           h->qval);
+        HIPCHK(hipMemcpyAsync(h->qsel_tg, tg.data(), sizeof(QselTarget) * tg.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(h->qmap, qmap.data(), sizeof(int) * qmap.size(), hipMemcpyHostToDevice, st));
+        const long long nh = (long long)tg.size() * 256;
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(nh, 256)), dim3(256), 0, st, reinterpret_cast<int *>(h->qsel_hist), nh,
+                           0);
+        for (int pass = 0; pass < 8; ++pass) {
+            hipLaunchKernelGGL(k_qsel_hist, dim3(nblk(n_total, 256)), dim3(256), 0, st, G0, n_total,
+                               (const unsigned *)h->sid[0], (const int *)h->qmap, h->scap,
+                               (const QselTarget *)h->qsel_tg, pass, h->qsel_hist);
+            hipLaunchKernelGGL(k_qsel_pick, dim3(nblk((long long)tg.size(), 4)), dim3(256), 0, st, h->qsel_tg,
+                               (int)tg.size(), pass, h->qsel_hist, h->qval);
+        }
+        pc.mark("quantile selection launch");
         std::vector<unsigned long long> red((size_t)8 * h->scap);
         std::vector<double> q6((size_t)6 * S_live);
         HIPCHK(hipMemcpyAsync(red.data(), h->red, sizeof(unsigned long long) * red.size(), hipMemcpyDeviceToHost, st));
@@ -1298,7 +1289,7 @@ static void free_scene(vg_scene *h)
     F(h->models); F(h->mm); F(h->laws); F(h->scenes); F(h->stime); F(h->rflags); F(h->walls); F(h->walls_ref);
     F(h->wkin); F(h->assigned); F(h->assigned_sph); F(h->en_part); F(h->en_out); F(h->flags); F(h->relists);
     F(h->contacts); F(h->red); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->qidx); F(h->segs);
-    F(h->qval); F(h->safe); F(h->lin); F(h->cstart); F(h->cend); F(h->ocount); F(h->onbr); F(h->tmp); F(h->rec);
+    F(h->qval); F(h->qsel_tg); F(h->qsel_hist); F(h->qmap); F(h->safe); F(h->lin); F(h->cstart); F(h->cend); F(h->ocount); F(h->onbr); F(h->tmp); F(h->rec);
     F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->sid[0]); F(h->sid[1]); F(h->fetch); F(h->ovf);
     F(h->cw_val); F(h->cw_stamp); F(h->cw_key[0]); F(h->cw_key[1]); F(h->cw_vidx[0]); F(h->cw_vidx[1]);
     F(h->cw_w); F(h->cw_p); F(h->cw_cnt); F(h->cw_beg); F(h->cw_end); F(h->cw_imp);
@@ -1383,6 +1374,8 @@ static int alloc_batch(vg_scene *h, int ld, int wcap, int scap, int ki, bool rec
     rc |= dalloc(h, &h->keys, ld) | dalloc(h, &h->skeys, ld);
     for (int b = 0; b < 2; ++b) rc |= dalloc(h, &h->xs[b], ld) | dalloc(h, &h->ys[b], ld) | dalloc(h, &h->zs[b], ld);
     rc |= dalloc(h, &h->qidx, 2 * (size_t)scap) | dalloc(h, &h->segs, 2 * (size_t)scap) | dalloc(h, &h->qval, 6 * (size_t)scap);
+    rc |= dalloc(h, &h->qsel_tg, 6 * (size_t)scap) | dalloc(h, &h->qsel_hist, (size_t)6 * 256 * scap) |
+          dalloc(h, &h->qmap, (size_t)scap);
     rc |= dalloc(h, &h->sid[0], ld) | dalloc(h, &h->sid[1], ld) | dalloc(h, &h->ovf, scap);
     rc |= dalloc(h, &h->safe, ld) | dalloc(h, &h->lin, ld);
     rc |= dalloc(h, &h->ocount, (size_t)ld + 1);
@@ -1521,7 +1514,7 @@ static int export_active(vg_scene *h, int slot, ActiveOut *out, size_t cap, unsi
                        h->ell[h->ecur], S.seg_begin, S.seg_end, h->ld, h->ki, (const unsigned *)h->ocount,
                        (const unsigned *)h->onbr, (const WallDev *)h->walls, (const unsigned *)d_wid,
                        (const unsigned *)d_sid, h->ent_valid ? (const double *)h->rec : (const double *)nullptr, out,
-                       (unsigned long long)cap, dcnt, backup ? 1 : 0);
+                       (unsigned long long)cap, dcnt, backup ? 1 : 0, h->no_bond);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1561,7 +1554,7 @@ static int snapshot_records(vg_scene *h)
     const SceneHost &S = h->sc[0];
     if (S.nw && h->ent_valid)
         hipLaunchKernelGGL(k_wall_impulse, dim3(S.nw), dim3(256), 0, h->stream, h->ell[h->ecur], S.seg_begin,
-                           S.seg_end, h->ld, h->ki, S.w_begin, (const double *)h->rec, h->snap_wimp);
+                           S.seg_end, h->ld, h->ki, S.w_begin, (const double *)h->rec, h->no_bond, h->snap_wimp);
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipFree(dcnt));
     HIPCHK(hipFree(d_sid));
@@ -1859,7 +1852,7 @@ static int fetch_scene_state(vg_scene *h, int slot, std::vector<double> &hb)
     } else if (one && S.nw && h->ent_valid) {
         if (dalloc(h, &dw, (size_t)6 * S.nw)) return VG_ENOMEM;
         hipLaunchKernelGGL(k_wall_impulse, dim3(S.nw), dim3(256), 0, h->stream, h->ell[h->ecur], S.seg_begin,
-                           S.seg_end, h->ld, h->ki, S.w_begin, (const double *)h->rec, dw);
+                           S.seg_end, h->ld, h->ki, S.w_begin, (const double *)h->rec, h->no_bond, dw);
     }
     HIPCHK(hipMemcpyAsync(sb, dout, sizeof(double) * 21 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
     if (dw) HIPCHK(hipMemcpyAsync(wimp.data(), dw, sizeof(double) * 6 * S.nw, hipMemcpyDeviceToHost, h->stream));

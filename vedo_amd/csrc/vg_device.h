@@ -25,6 +25,11 @@
 namespace vg {
 
 constexpr unsigned PARTNER_MASK = (1u << 26) - 1;
+// Candidate-list entries carry two marks of the rebuild that listed them:
+// the pair is within the scene's skin of contact (depth >= -skin) and in
+// contact (depth > 0) at the rebuild's positions.
+constexpr unsigned NBR_NEAR = 1u << 31;
+constexpr unsigned NBR_ACTIVE = 1u << 30;
 constexpr unsigned F_CONTACT = 1u << 26;
 constexpr unsigned F_LASTCONTACT = 1u << 27;
 constexpr unsigned F_BOND = 1u << 28;

@@ -154,6 +154,13 @@ __global__ void k_wall_path_reset(WallDev *walls, int nw);
 __global__ void k_ring_clear(int *rflags, int scap, int r);
 __global__ void k_bounds(Geo g, int n, const SceneDev *scenes, const ModelDev *models, unsigned long long *red,
                          unsigned *inv, double *xs, double *ys, double *zs, unsigned *sid, unsigned retired);
+struct QselTarget {
+    unsigned long long prefix;   // key bits settled so far
+    unsigned long long rank;     // rank still to find among the keys with that prefix
+};
+__global__ void k_qsel_hist(Geo g, int n, const unsigned *sid, const int *qmap, int n_slots, const QselTarget *tg,
+                            int pass, unsigned *hist);
+__global__ void k_qsel_pick(QselTarget *tg, int m, int pass, unsigned *hist, double *out);
 __global__ void k_quantiles(const double *xs, const double *ys, const double *zs, const long long *idx, int m,
                             double *out);
 __global__ void k_cellkey(Geo g, int n_ranks, const SceneDev *scenes, const ModelDev *models, const unsigned *inv,
@@ -187,8 +194,8 @@ __global__ void k_scatter_state(Geo g, int b, int e, int ld, const double *rec_i
 __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *o_cnt,
                                 const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,
                                 const unsigned *sphere_id, const double *rec, ActiveOut *out, unsigned long long cap,
-                                unsigned long long *counter, int backup);
-__global__ void k_wall_impulse(Ell ell, int b, int e, int ld, int ki, int w0, const double *rec, double *out);
+                                unsigned long long *counter, int backup, int no_bond);
+__global__ void k_wall_impulse(Ell ell, int b, int e, int ld, int ki, int w0, const double *rec, int no_bond, double *out);
 __global__ void k_assign_spheres(Geo g, int b, int e, int ld, const double *buf, double *rec_imp);
 // Knight analyses (src/vedo/knight) of a scene's current state.
 struct KnightBox {

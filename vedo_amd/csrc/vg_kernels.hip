@@ -93,7 +93,7 @@ __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_o
     int p = 0, q = 0;
     unsigned old_partner = c_old > 0 ? (a.ell.ent[eidx(h_old, 0, a.ki, a.ld, i)] & PARTNER_MASK) : 0xffffffffu;
     for (unsigned t = 0; t < e; ++t) {
-        const unsigned j = a.o_nbr[(size_t)t * a.ld + i];
+        const unsigned j = a.o_nbr[(size_t)t * a.ld + i] & PARTNER_MASK;
         Contact c;
         bool near = true;
         if (j < (unsigned)a.ld) {
@@ -259,7 +259,9 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
         store_hist<LS != LS_SD>(a.ell, h, k, a.ki, a.ld, i, hs);
         slave_contact = (hs.f & F_SLAVE) != 0;
     }
-    if (REC) {
+    // Without bonds a status without Contact is Clean (all record fields
+    // zero): only contacts store their record, readers take the rest as zero
+    if (REC && ((hs.f & F_CONTACT) || !a.no_bond)) {
         double *rp = a.rec + ((size_t)k * 13) * a.ld + i;
         const double vals[13] = {r.point.x, r.point.y, r.point.z, r.dir.x, r.dir.y, r.dir.z,
                                  r.to_master.x, r.to_master.y, r.to_master.z, r.ang_to_master.x,
@@ -388,12 +390,10 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
                 if (dot(dx, dx) > rs * rs * (1.0 + 1e-9)) {
                     full = false;
                     // Backup, then Clean() of an inactive pair (ImpactStatus.cpp:259-274)
+                    // (its record is all zero: readers take a record without
+                    // Contact, in a scene without bonds, as zero)
                     const unsigned f = ent[u] & ~PARTNER_MASK & ~F_STATUS & ~F_NOBACKUP;
                     if (f != (ent[u] & ~PARTNER_MASK)) nt_store(j | f, a.ell.ent + eidx(h, k, a.ki, a.ld, i));
-                    if (REC) {
-                        double *rp = a.rec + ((size_t)k * 13) * a.ld + i;
-                        for (int q = 0; q < 13; ++q) rp[(size_t)q * a.ld] = 0.0;
-                    }
                 }
             }
             if (full) todo |= 1ull << k;
@@ -689,6 +689,134 @@ __global__ void k_quantiles(const double *xs, const double *ys, const double *zs
     out[6 * t + 4] = zs[lo]; out[6 * t + 5] = zs[hi];
 }
 
+// Exact order statistics of each scene's coordinates by radix selection.
+// LeapConsultant::Reset sorts every coordinate to read its 2% and 98%
+// elements (LeapConsultant.cpp:139-191); the same elements are found here
+// without a sort.  Target t = 6 q + 2 c + side (q: the live scene's index,
+// c: x / y / z, side 0 / 1: the lower / upper element) keeps the rank still
+// to find and the bits of the element's order-preserving key settled so far.
+// Pass p (0..7) counts, per target, the byte at bits 56 - 8p .. 63 - 8p of
+// the keys that share the target's settled bits; k_qsel_pick then settles
+// that byte.  Only spheres of live scenes count (sid names their slot;
+// erased and retiring spheres carry another value).
+__device__ inline double unord64(unsigned long long u)
+{
+    const unsigned long long b = (u >> 63) ? (u & 0x7fffffffffffffffull) : ~u;
+    return __longlong_as_double((long long)b);
+}
+
+__global__ void k_qsel_hist(Geo g, int n, const unsigned *sid, const int *qmap, int n_slots, const QselTarget *tg,
+                            int pass, unsigned *hist)
+{
+    __shared__ unsigned sh[6][256];
+    __shared__ int q_lo, q_hi;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = __lane_id();
+    int q = -1;
+    if (i < n) {
+        const unsigned s = sid[i];
+        q = s < (unsigned)n_slots ? qmap[s] : -1;
+    }
+    for (int k = threadIdx.x; k < 6 * 256; k += blockDim.x) sh[k >> 8][k & 255] = 0u;
+    if (threadIdx.x == 0) {
+        q_lo = 0x7fffffff;
+        q_hi = -1;
+    }
+    // the block's range of live-scene indices (wavefront reductions, then LDS)
+    int lo = q >= 0 ? q : 0x7fffffff, hi = q;
+    for (int d = 32; d > 0; d >>= 1) {
+        lo = min(lo, __shfl_xor(lo, d, 64));
+        hi = max(hi, __shfl_xor(hi, d, 64));
+    }
+    __syncthreads();
+    if (lane == 0) {
+        atomicMin(&q_lo, lo);
+        atomicMax(&q_hi, hi);
+    }
+    __syncthreads();
+    // one scene in the block (the usual case): counts gather in LDS first
+    const bool one = q_lo == q_hi;
+    const int shift = 56 - 8 * pass;
+    double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (q >= 0) p = g.pos[i];
+    const double cc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const unsigned long long key = ord64(cc[c]);
+        const unsigned d = (unsigned)(key >> shift) & 255u;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            bool match = q >= 0;
+            if (match && pass > 0) {
+                const unsigned long long pre = tg[6 * q + 2 * c + side].prefix;
+                match = (key >> (shift + 8)) == (pre >> (shift + 8));
+            }
+            if (one) {
+                // one LDS atomic per distinct byte of the wavefront's matching lanes
+                unsigned long long act = __ballot(match);
+                while (act) {
+                    const int first = __builtin_ctzll(act);
+                    const unsigned d0 = __shfl(d, first, 64);
+                    const unsigned long long grp = act & __ballot(d == d0);
+                    if (lane == first) atomicAdd(&sh[2 * c + side][d0], (unsigned)__popcll(grp));
+                    act &= ~grp;
+                }
+            } else if (match) {
+                atomicAdd(&hist[((size_t)6 * q + 2 * c + side) * 256 + d], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    if (one && q_hi >= 0)
+        for (int k = threadIdx.x; k < 6 * 256; k += blockDim.x) {
+            const unsigned v = sh[k >> 8][k & 255];
+            if (v) atomicAdd(&hist[((size_t)6 * q_hi + (k >> 8)) * 256 + (k & 255)], v);
+        }
+}
+
+// One wavefront per target: the byte whose counts cover the target's rank,
+// then the counters are cleared for the next pass.
+__global__ void k_qsel_pick(QselTarget *tg, int m, int pass, unsigned *hist, double *out)
+{
+    const int t = blockIdx.x * (blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    const int lane = __lane_id();
+    if (t >= m) return;   // uniform across the wavefront
+    uint4 *hh = reinterpret_cast<uint4 *>(hist + (size_t)t * 256);
+    const uint4 b = hh[lane];   // bytes 4 lane .. 4 lane + 3
+    const unsigned long long r = tg[t].rank;
+    const unsigned long long own = (unsigned long long)b.x + b.y + b.z + b.w;
+    unsigned long long incl = own;
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned long long y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    const unsigned long long over = __ballot(incl > r);
+    const int L = over ? __builtin_ctzll(over) : 63;
+    unsigned long long cum = __shfl(incl - own, L, 64);
+    const unsigned c0 = __shfl(b.x, L, 64), c1 = __shfl(b.y, L, 64), c2 = __shfl(b.z, L, 64);
+    int d = 4 * L;
+    if (cum + c0 <= r) {
+        cum += c0;
+        ++d;
+        if (cum + c1 <= r) {
+            cum += c1;
+            ++d;
+            if (cum + c2 <= r) {
+                cum += c2;
+                ++d;
+            }
+        }
+    }
+    hh[lane] = make_uint4(0u, 0u, 0u, 0u);
+    if (lane == 0) {
+        const int shift = 56 - 8 * pass;
+        const unsigned long long pre = tg[t].prefix | ((unsigned long long)d << shift);
+        tg[t].prefix = pre;
+        tg[t].rank = r - cum;
+        if (pass == 7) out[t] = unord64(pre);
+    }
+}
+
 __device__ inline void cell_of(const GridDev &G, d3 p, unsigned &cx, unsigned &cy, unsigned &cz)
 {
     cx = (unsigned)((p.x - G.xmin) / G.zr);
@@ -790,7 +918,9 @@ __device__ inline bool accept_wall(const OuterArgs &a, const SceneDev &S, d3 xs,
 // Candidate partners of sphere s in the reference's IactPairTab order: the
 // groups "current cell = c" of the Leap traversal are visited in comparator
 // order; within the own cell the 13 half-shell cells, the self cell and the
-// global DOs follow in reference order (LeapConsultant.h:207-444).
+// global DOs follow in reference order (LeapConsultant.h:207-444).  Each
+// accepted pair also carries the marks of its contact depth for the
+// neighbour list.
 // The scene's grid and law table are read once into registers and the
 // arrays are declared unaliased, so each candidate test costs one gather of
 // the partner's position and safe distance, issued one candidate ahead.
@@ -828,10 +958,24 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
 
     unsigned cx, cy, cz;
     cell_of(G, xs, cx, cy, cz);
+    const double skin = Sg.skin;
     unsigned cnt = 0;
     auto emit = [&](unsigned j) {
         if (cnt < ko) nbr[(size_t)cnt * ld + s] = j;
         ++cnt;
+    };
+    // the contact depth of an accepted pair (CDSphere_Sphere's, the same bits
+    // from either side) marks it for the neighbour list: within the skin of
+    // contact, in contact
+    auto marks = [&](double depth) -> unsigned {
+        return (depth >= -skin ? NBR_NEAR : 0u) | (depth > 0.0 ? NBR_ACTIVE : 0u);
+    };
+    auto sphere_marks = [&](double4 pj) -> unsigned {
+        const d3 xj = mk(pj.x, pj.y, pj.z);
+        const d3 dx = xj - xs;
+        const double lim = p.w + pj.w + skin;
+        if (dot(dx, dx) > lim * lim * (1.0 + 1e-9)) return 0u;   // surely beyond the skin
+        return marks(p.w + pj.w - len(dx));
     };
     // DOMap::CalDistance of the pair against safe_i + safe_j: the squared
     // distance decides every pair away from the threshold; near it the
@@ -869,7 +1013,7 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
                 const int j = j0 + u;
                 if (j >= e) break;
                 if (self && j == s) continue;
-                if (has_law(j) && near(pj[u], sj[u])) emit((unsigned)j);
+                if (has_law(j) && near(pj[u], sj[u])) emit((unsigned)j | sphere_marks(pj[u]));
             }
         }
     };
@@ -880,7 +1024,11 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
                 scan_cell((long long)cx + NB13[k][0], (long long)cy + NB13[k][1], (long long)cz + NB13[k][2], false);
             scan_cell(cx, cy, cz, true);
             for (int w = w_begin; w < w_end; ++w)
-                if (accept_wall(a, Sg, xs, safe_s, gs, w)) emit((unsigned)(a.ld + w));
+                if (accept_wall(a, Sg, xs, safe_s, gs, w)) {
+                    Contact c;
+                    cd_wall(xs, p.w, a.walls[w], c);
+                    emit((unsigned)(a.ld + w) | marks(c.depth));
+                }
         } else {
             // the cell whose half shell holds s's cell, if it exists (LeapConsultant.h:207-444)
             const long long ox = (long long)cx - NB13[slot][0], oy = (long long)cy - NB13[slot][1],
@@ -904,7 +1052,7 @@ __global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int b, in
     if (s >= e) return;
     unsigned sph = 0, wal = 0;
     for (unsigned t = 0; t < cnt[s] && t < (unsigned)ko; ++t) {
-        if (nbr[(size_t)t * ld + s] < (unsigned)ld) ++sph;
+        if ((nbr[(size_t)t * ld + s] & PARTNER_MASK) < (unsigned)ld) ++sph;
         else ++wal;
     }
     atomicAdd(&out[0], (unsigned long long)sph);
@@ -920,7 +1068,6 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     extern __shared__ unsigned old_oid[];   // [ki][blockDim]
     if (s >= a.n) return;
-    const double4 *__restrict__ pos = a.g.pos;
     const unsigned *__restrict__ oid = a.g.oid;
     const unsigned *__restrict__ nbr = a.nbr;
     const unsigned *__restrict__ ent_old = a.eold.ent;
@@ -933,46 +1080,28 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
         const unsigned j = ent_old[eidx(h_old, k, a.ki, a.ld, p)] & PARTNER_MASK;
         old_oid[k * blockDim.x + threadIdx.x] = j < (unsigned)a.ld ? a.gold.oid[j] : 0x80000000u + (j - a.ld);
     }
-    const double skin = a.scenes[a.models[a.g.mdl[s]].scene].skin;
-    const double4 ps = pos[s];
-    const d3 xs = mk(ps.x, ps.y, ps.z);
     const unsigned my_oid = oid[s];
     const unsigned cnt = a.cnt[s];
     int q = 0;
-    // candidates in groups of 4: the list entries, then the partner gathers
-    // of a group are in flight together
+    // the candidates the rebuild marked as within the skin, in list order;
+    // their ranks are gathered 4 at a time
     for (unsigned t0 = 0; t0 < cnt; t0 += 4) {
         unsigned jv[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) jv[u] = nbr[(size_t)(t0 + u < cnt ? t0 + u : cnt - 1) * ld + s];
-        double4 pj[4];
+        for (int u = 0; u < 4; ++u) jv[u] = t0 + u < cnt ? nbr[(size_t)(t0 + u) * ld + s] : 0u;
         unsigned kj[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const bool sph = jv[u] < (unsigned)a.ld;
-            pj[u] = sph ? pos[jv[u]] : make_double4(0.0, 0.0, 0.0, 0.0);
-            kj[u] = sph ? oid[jv[u]] : 0x80000000u + (jv[u] - a.ld);
+            const unsigned j = jv[u] & PARTNER_MASK;
+            kj[u] = !(jv[u] & NBR_NEAR) ? 0u : (j < (unsigned)a.ld ? oid[j] : 0x80000000u + (j - a.ld));
         }
         bool full = false;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            if (t0 + u >= cnt || full) break;
-            const unsigned j = jv[u];
-            Contact c;
+            if (!(jv[u] & NBR_NEAR)) continue;
+            const unsigned j = jv[u] & PARTNER_MASK;
             const unsigned key = kj[u];
-            bool slave = true;
-            if (j < (unsigned)a.ld) {
-                const d3 xj = mk(pj[u].x, pj[u].y, pj[u].z);
-                const d3 dx = xj - xs;
-                const double lim = ps.w + pj[u].w + skin;
-                if (dot(dx, dx) > lim * lim * (1.0 + 1e-9)) continue;   // surely beyond the skin
-                slave = my_oid < key;
-                if (slave) cd_ss(xs, ps.w, xj, pj[u].w, c);
-                else cd_ss(xj, pj[u].w, xs, ps.w, c);
-            } else {
-                cd_wall(xs, ps.w, a.walls[j - a.ld], c);
-            }
-            if (c.depth < -skin) continue;
+            const bool slave = j >= (unsigned)a.ld || my_oid < key;
             if (q >= a.ki) {
                 atomicOr(&a.flags[3], ERR_INNER_CAP);
                 full = true;
@@ -982,7 +1111,7 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
             Hist hs;
             hs.s = mk(0, 0, 0);
             hs.aux = 0.0;
-            if (c.active) {
+            if (jv[u] & NBR_ACTIVE) {
                 for (int k = 0; k < c_old; ++k)
                     if (old_oid[k * blockDim.x + threadIdx.x] == key) {
                         f = (ent_old[eidx(h_old, k, a.ki, a.ld, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
@@ -1033,7 +1162,7 @@ __global__ void k_cw_count(int n, int ld, int ko, const unsigned *cnt, const uns
     const unsigned e = cnt[i] < (unsigned)ko ? cnt[i] : (unsigned)ko;
     unsigned c = 0;
     for (unsigned t = 0; t < e; ++t) {
-        const unsigned j = nbr[(size_t)t * ld + i];
+        const unsigned j = nbr[(size_t)t * ld + i] & PARTNER_MASK;
         if (j >= (unsigned)ld && walls[j - ld].constrained) ++c;
     }
     if (c > 2) {   // a sphere near more than two constrained walls
@@ -1056,7 +1185,7 @@ __global__ void k_cw_emit(int n, int ld, int ko, const unsigned *cnt, const unsi
     const unsigned e = cnt[i] < (unsigned)ko ? cnt[i] : (unsigned)ko;
     unsigned q = 0;
     for (unsigned t = 0; t < e && b + q < lim; ++t) {
-        const unsigned j = nbr[(size_t)t * ld + i];
+        const unsigned j = nbr[(size_t)t * ld + i] & PARTNER_MASK;
         if (j >= (unsigned)ld && walls[j - ld].constrained) {
             key[b + q] = ((unsigned long long)(j - ld) << 51) | ck;
             val[b + q] = ((unsigned)i << 1) | q;
@@ -1287,7 +1416,7 @@ __global__ void k_scatter_state(Geo g, int b, int e, int ld, const double *rec_i
 __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *o_cnt,
                                 const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,
                                 const unsigned *sphere_id, const double *rec, ActiveOut *out, unsigned long long cap,
-                                unsigned long long *counter, int backup)
+                                unsigned long long *counter, int backup, int no_bond)
 {
     // spheres b..e-1 (one scene); sphere_id maps the scene's ranks to DO ids
     const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1301,7 +1430,7 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
     int q = 0;
     unsigned listed = cnt > 0 ? (ell.ent[eidx(h, 0, ki, n, i)] & PARTNER_MASK) : 0xffffffffu;
     for (unsigned t = 0; t < ec; ++t) {
-        const unsigned j = o_nbr[(size_t)t * n + i];
+        const unsigned j = o_nbr[(size_t)t * n + i] & PARTNER_MASK;
         int k = -1;
         unsigned ent;
         if (listed == j) {
@@ -1341,14 +1470,16 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
         hs.aux = 0.0;
         if (k >= 0 && (ent & F_CONTACT)) load_hist(ell, h, k, ki, n, i, hs);
         A.hist[0] = hs.s.x; A.hist[1] = hs.s.y; A.hist[2] = hs.s.z; A.hist[3] = hs.aux;
-        for (int r = 0; r < 13; ++r) A.rec[r] = (rec && k >= 0) ? rec[((size_t)k * 13 + r) * n + i] : 0.0;
+        const bool stored = rec && k >= 0 && ((ent & F_CONTACT) || !no_bond);
+        for (int r = 0; r < 13; ++r) A.rec[r] = stored ? rec[((size_t)k * 13 + r) * n + i] : 0.0;
     }
 }
 
 // Impulse on each wall from the last recorded step: sum of ImpactToMaster
 // and AngularImpactToMaster over its contacts (the DOStatus impact a wall's
 // ClearImpact stores), one block per wall, fixed-order reduction.
-__global__ void k_wall_impulse(Ell ell, int b, int e, int ld, int ki, int w0, const double *rec, double *out)
+__global__ void k_wall_impulse(Ell ell, int b, int e, int ld, int ki, int w0, const double *rec, int no_bond,
+                               double *out)
 {
     // one block per wall w0 + blockIdx.x, over the scene's spheres b..e-1
     const int w = w0 + blockIdx.x;
@@ -1358,8 +1489,9 @@ __global__ void k_wall_impulse(Ell ell, int b, int e, int ld, int ki, int w0, co
         const unsigned hdr = ell.hdr[i];
         const int h = hdr >> 7, cnt = hdr & 127;
         for (int k = 0; k < cnt; ++k) {
-            const unsigned j = ell.ent[eidx(h, k, ki, ld, i)] & PARTNER_MASK;
-            if (j != (unsigned)(ld + w)) continue;
+            const unsigned ent = ell.ent[eidx(h, k, ki, ld, i)];
+            if ((ent & PARTNER_MASK) != (unsigned)(ld + w)) continue;
+            if (!(ent & F_CONTACT) && no_bond) continue;   // a Clean status: no record stored, zero
             for (int c = 0; c < 6; ++c) acc[c] += rec[((size_t)k * 13 + 6 + c) * ld + i];
         }
     }
@@ -1427,7 +1559,7 @@ __global__ void k_coordination(Geo g, int b, int e, int ld, const unsigned *o_cn
     unsigned count = 0;
     const unsigned c = o_cnt[i];
     for (unsigned t = 0; t < c; ++t) {
-        const unsigned j = o_nbr[(size_t)t * ld + i];
+        const unsigned j = o_nbr[(size_t)t * ld + i] & PARTNER_MASK;
         if (j >= (unsigned)ld) continue;   // a wall
         const double4 q = g.pos[j];
         if (len(xi - ld3(q)) <= p.w + q.w) ++count;

@@ -147,19 +147,12 @@ def batched_boxes(count, steps, up_iact, seed=4, loop=True):
     """The batched configuration: `count` box-settling scenes (ISwH), sphere
     counts log-uniform in [1e3, 2e5], mu ~ U[0.2, 0.7], e ~ U[0.3, 0.9],
     E ~ logU[1e6, 1e8], r = 1 mm, box sized to its lattice, dt = 0.1 of the
-    critical time step.  All scenes run `steps` steps in one batch (admission
+    critical time step of the contacts' tangential-rotational mode.  All scenes run `steps` steps in one batch (admission
     to results on the host, timed), and, when `loop`, the plain per-scene GPU
     loop: each scene alone, one after the other, in its own handle."""
     from vedo_amd import scenes
     from vedo_amd.gpu import SCENE_DONE, SCENE_FAILED, GpuBatch, GpuError, GpuSimulation
-    rng = scenes.SplitMix64(seed)
-    u = rng.uniform(4 * count, 0.0, 1.0).reshape(count, 4)
-    was = []
-    for k in range(count):
-        n = int(round(math.exp(math.log(1e3) + u[k, 0] * (math.log(2e5) - math.log(1e3)))))
-        was.append(scenes.box_arrays(n, seed=seed * 1000 + k, law="ISwH", mu=0.2 + 0.5 * u[k, 1],
-                                     e=0.3 + 0.6 * u[k, 2], E=10.0 ** (6.0 + 2.0 * u[k, 3]), steps=steps,
-                                     up_iact=up_iact))
+    was = [scenes.batched_box(k, steps, up_iact, count=count, seed=seed) for k in range(count)]
     n_tot = sum(int((wa.model_shape[wa.model_of] == 1).sum()) for wa in was)
     ps = float(n_tot) * steps
     log(f"[bench] batched: {count} box scenes, {n_tot} spheres, {steps} steps each")
@@ -197,8 +190,10 @@ def batched_boxes(count, steps, up_iact, seed=4, loop=True):
     failed = [k for k, x in enumerate(states) if x == SCENE_FAILED]
     n_ok = sum(int((was[k].model_shape[was[k].model_of] == 1).sum()) for k in range(count) if k not in failed)
     ps = float(n_ok) * steps
+    n_of = [int((wa.model_shape[wa.model_of] == 1).sum()) for wa in was]
     out = {"scenes": count, "spheres": n_tot, "steps_per_scene": steps, "up_iact": up_iact,
-           "failed_scenes": len(failed), "batched_seconds": t_batch, "batched_particle_steps_per_s": ps / t_batch,
+           "failed_scenes": len(failed), "failed": [[k, n_of[k]] for k in failed],
+           "batched_seconds": t_batch, "batched_particle_steps_per_s": ps / t_batch,
            "batch_steps": periods * up_iact, "relists": stats["relists"], "skin_m": skin}
     log(f"[bench] batched: {t_batch:.2f} s, {ps / t_batch:.3e} particle-steps/s ({len(failed)} scenes blew up: "
         f"soft ISwH contacts)")

@@ -319,7 +319,7 @@ def drum_scene(n: int, seed: int = 2, median: float = 1e-3, sigma: float = 0.2, 
                 IactModel("Wall", "Granule", "ISwH", hertz_mindlin(E, nu, mu, e, median, None, m_med, None))]
         m_min = density * 4.0 / 3.0 * math.pi * float(cls.min()) ** 3
         k = ish_first_contact_kn(pair["HertzNormalStiffness"], pair["EquivalentMass"], pair["DampingRatio"], 1.0)
-        dt = critical_dt(0.5 * m_min, k)
+        dt = critical_dt(tangential_mass(0.5 * m_min), k)
     else:
         iact = [IactModel("Granule", "Granule", law, spring_dashpot(kn, kt_ratio, e, mu, True)),
                 IactModel("Wall", "Granule", law, spring_dashpot(kn, kt_ratio, e, mu, False))]
@@ -390,6 +390,17 @@ def critical_dt(m_eff: float, k: float, fraction: float = 0.1) -> float:
     return fraction * 2.0 * math.pi * math.sqrt(m_eff / k)
 
 
+def tangential_mass(m_eff: float) -> float:
+    """Mass of a contact's tangential-rotational mode, the one that bounds the
+    stable time step of a law with a tangential spring: a force at the contact
+    point of a solid sphere (I = 0.4 m r^2) moves that point as a mass
+    m / (1 + m r^2 / I) = 2m/7 would, so a pair of reduced mass M* has
+    2/7 M*.  With the normal mode's M* instead, the compiled reference's
+    ISwH runs of soft, weakly damped box scenes gain energy until spheres
+    cross more than their radius in a step; with this mass they settle."""
+    return 2.0 / 7.0 * m_eff
+
+
 def box_arrays(n: int, seed: int = 1, radius: float = 1e-3, density: float = 2500.0, law: str = "ISwH",
                E: float = 1e7, nu: float = 0.3, mu: float = 0.5, e: float = 0.5, kn: float = 1e4,
                dt: Optional[float] = None, steps: int = 5000, spacing: float = 2.5, jitter: float = 0.1,
@@ -399,8 +410,8 @@ def box_arrays(n: int, seed: int = 1, radius: float = 1e-3, density: float = 250
     seeded jittered cubic lattice (spacing*r, as box_scene) in a closed box
     sized to the lattice (side = lattice extent + offset*r), one law for
     sphere-sphere and sphere-wall.  dt None: 0.1 of the critical time step of
-    the sphere-sphere contact, for ISwH at the impact speed of a fall through
-    the lattice height."""
+    the sphere-sphere contact's tangential-rotational mode (tangential_mass),
+    for ISwH at the impact speed of a fall through the lattice height."""
     rng = SplitMix64(seed)
     s = spacing * radius
     nx = max(1, int(math.ceil(n ** (1.0 / 3.0))))
@@ -421,15 +432,27 @@ def box_arrays(n: int, seed: int = 1, radius: float = 1e-3, density: float = 250
             v_ref = math.sqrt(2.0 * abs(gravity) * nz * s) + vmax
             k = ish_first_contact_kn(pair["HertzNormalStiffness"], pair["EquivalentMass"],
                                      pair["DampingRatio"], v_ref)
-            dt = critical_dt(pair["EquivalentMass"], k)
+            dt = critical_dt(tangential_mass(pair["EquivalentMass"]), k)
     else:
         pair = spring_dashpot(kn, 0.8, e, mu, True)
         wall = spring_dashpot(kn, 0.8, e, mu, False)
         if dt is None:
-            dt = critical_dt(0.5 * mass, kn)
+            dt = critical_dt(tangential_mass(0.5 * mass), kn)
     iact = [IactModel("Granule", "Granule", law, pair), IactModel("Wall", "Granule", law, wall)]
     world = DOWorld(SystemParameter(0.0, steps * dt, dt, 0.0, (0.0, 0.0, gravity)), [sphere] + wm, iact, ws)
     return _spliced(world, 0, pos, vel, np.zeros((n, 3)), up_iact, record)
+
+
+def batched_box(k: int, steps: int, up_iact: int = 20, count: int = 256, seed: int = 4) -> WorldArrays:
+    """Scene k of the batched configuration: `count` box-settling scenes
+    (ISwH), sphere counts log-uniform in [1e3, 2e5], mu ~ U[0.2, 0.7],
+    e ~ U[0.3, 0.9], E ~ logU[1e6, 1e8], r = 1 mm, box sized to its lattice,
+    the box_arrays time step."""
+    rng = SplitMix64(seed)
+    u = rng.uniform(4 * count, 0.0, 1.0).reshape(count, 4)
+    n = int(round(math.exp(math.log(1e3) + u[k, 0] * (math.log(2e5) - math.log(1e3)))))
+    return box_arrays(n, seed=seed * 1000 + k, law="ISwH", mu=0.2 + 0.5 * u[k, 1], e=0.3 + 0.6 * u[k, 2],
+                      E=10.0 ** (6.0 + 2.0 * u[k, 3]), steps=steps, up_iact=up_iact)
 
 
 def hopper_arrays(n: int, seed: int = 3, radius: float = 1e-3, density: float = 2500.0, law: str = "ISwH",
@@ -497,12 +520,12 @@ def hopper_arrays(n: int, seed: int = 3, radius: float = 1e-3, density: float = 
             v_ref = math.sqrt(2.0 * 9.81 * (H + Hc + Hb))
             k = ish_first_contact_kn(pair["HertzNormalStiffness"], pair["EquivalentMass"],
                                      pair["DampingRatio"], v_ref)
-            dt = critical_dt(pair["EquivalentMass"], k)
+            dt = critical_dt(tangential_mass(pair["EquivalentMass"]), k)
     else:
         pair = spring_dashpot(kn, 0.8, e, mu, True)
         wall = spring_dashpot(kn, 0.8, e, mu, False)
         if dt is None:
-            dt = critical_dt(0.5 * mass, kn)
+            dt = critical_dt(tangential_mass(0.5 * mass), kn)
     iact = [IactModel("Granule", "Granule", law, pair), IactModel("Wall", "Granule", law, wall)]
     world = DOWorld(SystemParameter(0.0, steps * dt, dt, 0.0, (0.0, 0.0, -9.81)),
                     [sphere, funnel, column, binwall, floor], iact, st)

@@ -377,7 +377,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=256, help="batched configuration: scenes (0: skip)")
-    ap.add_argument("--batch-steps", type=int, default=1000, help="batched configuration: steps per scene")
+    ap.add_argument("--batch-steps", type=int, default=5000, help="batched configuration: steps per scene")
     ap.add_argument("--serve-requests", type=int, default=24, help="serving configuration: requests (0: skip)")
     ap.add_argument("--no-loop", action="store_true", help="skip the one-scene-at-a-time comparisons")
     ap.add_argument("--batch-up-iact", type=int, default=20, help="UpIact of the batched and served scenes")

@@ -4,15 +4,19 @@
 // ImpactSolvers of the reference), with this split of precisions:
 //   FP64: every decision that sets an ImpactStatus flag -- the overlap
 //         depth of the contact detection (so the active set is the FP64
-//         one, bit for bit), the normal-bond test -- the relative velocity
-//         at the contact point, the stored history (shear force, Kn /
+//         one, bit for bit), the normal-bond test -- the contact normal and
+//         lever arms, the relative velocity at the contact point, the
+//         normal spring and dashpot force (formed as the FP64 path forms
+//         it: in a compressed bed a sphere's net impulse is a small
+//         remainder of its contacts' normal forces, which FP32 rounding of
+//         those forces would swamp), the stored history (shear force, Kn /
 //         initial velocity) and its update, the impulse sums and the
 //         integration (in the step kernel);
-//   FP32: contact normal and lever arms for the torques, the normal and
-//         damping forces, the tangential increment, the friction cap and
-//         the impulse and torque of the pair.
+//   FP32: the tangential spring increment, the friction cap and the
+//         tangential dashpot (the torques are formed from that tangential
+//         force in FP64).
 // The per-pair force error against the FP64 path is a few FP32 rounding
-// units of the pair's force terms.
+// units of the pair's tangential force.
 #pragma once
 
 #include "vg_device.h"
@@ -49,9 +53,8 @@ __device__ inline f3 dirf(f3 a)
 struct ContactM {
     double depth;
     bool active;
-    f3 n, ra, rb;
-    d3 n64;         // the normal in FP64
-    d3 ra64, rb64;  // the lever arms in FP64, for the relative velocity
+    d3 n64;         // the normal
+    d3 ra64, rb64;  // the lever arms from the slave and master centres
 };
 
 // CDSphere_Sphere (CDSphere2Sphere.cpp:14-60) for slave a, master b: the
@@ -63,12 +66,8 @@ __device__ inline void cd_ss_m(d3 xa, double ra, d3 xb, double rb, ContactM &c)
     const double l = len(im);
     c.depth = ra + rb - l;
     c.active = c.depth > 0;
-    const f3 imf = tof(im);
     const double il = l != 0.0 ? 1.0 / l : 0.0;
     c.n64 = mk(im.x * il, im.y * il, im.z * il);
-    c.n = tof(c.n64);
-    c.ra = (float)(ra - 0.5 * c.depth) * c.n;
-    c.rb = c.ra - imf;
     c.ra64 = smul(ra - 0.5 * c.depth, c.n64);
     c.rb64 = c.ra64 - im;
 }
@@ -79,26 +78,23 @@ __device__ inline void to_mixed(const Contact &c, d3 xa, d3 xb, ContactM &m)
     m.depth = c.depth;
     m.active = c.active;
     m.n64 = c.n;
-    m.n = tof(c.n);
     m.ra64 = c.p - xa;
     m.rb64 = c.p - xb;
-    m.ra = tof(m.ra64);
-    m.rb = tof(m.rb64);
 }
 
 // Relative velocity at the contact point, v = vA - vB + wA x rA - wB x rB,
 // split into its normal part vn = (v.n) n and tangential part vs = v - vn,
-// all in FP64 and rounded to FP32 at the end.  The terms cancel: in a
+// in FP64 (vn64, the normal part, stays FP64 for the normal force; vs is
+// rounded to FP32).  The terms cancel: in a
 // rolling contact the two spheres' rotational parts nearly annul each other,
 // and in a resting one the centre velocities do, so FP32 terms would leave
 // the small tangential velocity (whose direction sets a sliding contact's
 // friction) with a relative error far above FP32's.
-__device__ inline void rel_vel(d3 va, d3 vb, d3 wa, d3 wb, const ContactM &c, f3 &vn, f3 &vs)
+__device__ inline void rel_vel(d3 va, d3 vb, d3 wa, d3 wb, const ContactM &c, d3 &vn64, f3 &vs)
 {
     const d3 v = ((va - vb) + cross(wa, c.ra64)) - cross(wb, c.rb64);
-    const double dn = dot(v, c.n64);
-    vn = (float)dn * c.n;
-    vs = tof(v - smul(dn, c.n64));
+    vn64 = smul(dot(v, c.n64), c.n64);
+    vs = tof(v - vn64);
 }
 
 // ShearForceRotation (ISwModels.cpp:110-123) of the FP64 history onto the
@@ -126,31 +122,30 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
     const int law = L.law;
     const double depth = c.depth;
     const float df = (float)depth, dtf = (float)dt;
-    const f3 n = c.n;
     if (LS == LS_ALL && law == LAW_HERTZ) {   // ISwHertz (ISwHertz.cpp:20-94)
         o.linear_only = true;
         if (!c.active) {
             st_clean(h);
             return;
         }
-        f3 fn = mkf(0.0f, 0.0f, 0.0f);
+        d3 fn = mk(0, 0, 0);
         if (depth > 0.0) {
-            f3 vn, vs;
+            d3 vn;
+            f3 vs;
             rel_vel(va, vb, wa, wb, c, vn, vs);
             if (!(h.f & F_CONTACT)) {
                 h.f |= F_CONTACT;
-                h.aux = (double)lenf(vn);
+                h.aux = len(vn);
             }
-            fn = -((float)L.p[0] * (df * sqrtf(df))) * n - (float)L.p[1] * vn;
+            fn = smul(-L.p[0] * pow(depth, 1.5), c.n64) - smul(L.p[1], vn);
         } else {
             st_clean_contact(h);
         }
-        const f3 imp = dtf * fn;
-        o.imp = tod(imp);
+        o.imp = muls(fn, dt);
         o.ang_a = o.ang_b = mk(0, 0, 0);
         o.added = true;
         if (REC) {
-            r.point = xa + tod(c.ra); r.dir = tod(n); r.overlap = depth;
+            r.point = xa + c.ra64; r.dir = c.n64; r.overlap = depth;
             r.to_master = -o.imp; r.ang_to_master = mk(0, 0, 0);
         }
         return;
@@ -161,16 +156,18 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
             st_clean(h);
             return;
         }
-        f3 ra = mkf(0.0f, 0.0f, 0.0f), rb = ra, f = ra, fst = ra;
+        f3 fst = mkf(0.0f, 0.0f, 0.0f);
+        d3 f = mk(0, 0, 0), ra = mk(0, 0, 0), rb = ra;
         if (depth > 0.0) {
-            ra = c.ra;
-            rb = c.rb;
-            f3 vn, vs;
+            ra = c.ra64;
+            rb = c.rb64;
+            d3 vn;
+            f3 vs;
             rel_vel(va, vb, wa, wb, c, vn, vs);
             double kn;
             if (!(h.f & F_CONTACT)) {
                 // first contact: Kn from the approach speed, kept in the history (FP64)
-                const double vl = (double)lenf(vn);
+                const double vl = len(vn);
                 if (vl == 0.0) kn = knh;
                 else kn = L.p[6] * pow(vl, 0.4);   // p[6]: the law's factor, set on the host
                 h.aux = kn;
@@ -178,26 +175,26 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
             } else {
                 kn = h.aux;
             }
+            const double ncd = 2.0 * sqrt(meff * kn);
+            const d3 fn = normal_force(kn, ncd * betan, depth, c.n64, vn);
             const float knf = (float)kn;
-            const float ncd = 2.0f * sqrtf((float)meff * knf);
             const float ks = (float)L.p[2] * cbrtf((float)L.p[3] * fabsf(knf * df));
-            const f3 fn = -(knf * df) * n - (ncd * (float)betan) * vn;
             const float ff = (float)L.p[5] * knf * df;
             d3 fs = shear_rotation_m(h.s, vs) - tod((ks * dtf) * vs);
             const f3 fsf = tof(fs);
             if (lenf(fsf) >= ff) fs = tod(ff * dirf(fsf));
             h.s = fs;
             fst = tof(fs);
-            f = fn + fst;
+            f = fn + tod(fst);
         } else {
             st_clean_contact(h);
         }
-        o.imp = tod(dtf * f);
-        o.ang_a = tod(dtf * crossf(ra, fst));
-        o.ang_b = tod(-dtf * crossf(rb, mb == 0 ? f : fst));
+        o.imp = muls(f, dt);
+        o.ang_a = smul(dt, cross(ra, tod(fst)));
+        o.ang_b = smul(-dt, cross(rb, mb == 0 ? f : tod(fst)));
         o.added = true;
         if (REC) {
-            r.point = xa + tod(c.ra); r.dir = tod(n); r.overlap = depth;
+            r.point = xa + c.ra64; r.dir = c.n64; r.overlap = depth;
             r.to_master = smul(-1.0, o.imp); r.ang_to_master = o.ang_b;
         }
         return;
@@ -210,20 +207,26 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
     if (mb == 0) me = (float)ma;
     else if (ma == 0) me = (float)mb;
     else me = (float)(0.5 * (ma + mb));
-    const float cn = (float)L.p[2] * (2.0f * sqrtf(me * knf));
+    double me64;   // ISwModels::CriticalDamping, as the FP64 path forms it
+    if (mb == 0) me64 = ma;
+    else if (ma == 0) me64 = mb;
+    else me64 = 0.5 * (ma + mb);
+    const double cn = L.p[2] * (2.0 * sqrt(me64 * kn));
     const float cs = (float)L.p[3] * (2.0f * sqrtf(me * ksf));
     const bool nbond = (depth <= 0.0) && (kn != 0.0) && (bn != 0.0) && (-depth * kn < bn);
     if (!c.active && !nbond) {
         st_clean(h);
         return;
     }
-    f3 ra = mkf(0.0f, 0.0f, 0.0f), rb = ra, fn = ra, fsf = ra;
+    f3 fsf = mkf(0.0f, 0.0f, 0.0f);
+    d3 fn = mk(0, 0, 0), ra = fn, rb = fn;
     if (depth > 0.0) {
-        ra = c.ra;
-        rb = c.rb;
-        f3 vn, vs;
+        ra = c.ra64;
+        rb = c.rb64;
+        d3 vn;
+        f3 vs;
         rel_vel(va, vb, wa, wb, c, vn, vs);
-        fn = -(knf * df) * n - cn * vn;
+        fn = normal_force(kn, cn, depth, c.n64, vn);
         const float fmax = fc * knf * df;
         if (law == BTSDBF) {
             fsf = (-ksf * dtf) * vs - cs * vs;
@@ -240,18 +243,18 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
         h.f |= F_CONTACT;
     } else if (law != BSDB2F || (h.f & F_LASTBOND)) {
         // tension of a normal bond: relative velocity of the centres only
-        const f3 vn = (float)dot(va - vb, c.n64) * n;
-        if (law == BTSDBF || dotf(vn, n) < 0.0f) fn = -(knf * df) * n - cn * vn;
-        else fn = -cn * vn;
+        const d3 vn = smul(dot(va - vb, c.n64), c.n64);
+        if (law == BTSDBF) fn = normal_force(kn, cn, depth, c.n64, vn);
+        else fn = normal_force_bond(kn, cn, depth, c.n64, vn);
         st_clean_contact(h);
     }
-    const f3 f = fn + fsf;
-    o.imp = tod(dtf * f);
-    o.ang_a = tod(dtf * crossf(ra, fsf));
-    o.ang_b = tod(-dtf * crossf(rb, mb == 0 ? f : fsf));
+    const d3 f = fn + tod(fsf);
+    o.imp = muls(f, dt);
+    o.ang_a = smul(dt, cross(ra, tod(fsf)));
+    o.ang_b = smul(-dt, cross(rb, mb == 0 ? f : tod(fsf)));
     o.added = true;
     if (REC) {
-        r.point = xa + tod(c.ra); r.dir = tod(n); r.overlap = depth;
+        r.point = xa + c.ra64; r.dir = c.n64; r.overlap = depth;
         r.to_master = smul(-1.0, o.imp); r.ang_to_master = o.ang_b;
     }
 }

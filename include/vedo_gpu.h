@@ -207,6 +207,15 @@ int vg_step_timing(vg_scene *h, double *ms, int *launches);
  * LeapConsultant::Reset + history carry, host work included) are bracketed by
  * device events too; their summed duration and count. */
 int vg_rebuild_timing(vg_scene *h, double *ms, int *rebuilds);
+/* No counterpart (diagnostic): checks on the device that every index the
+ * step, wall and rebuild kernels take from the handle's lists is in range
+ * (each sphere's model and scene; its neighbour-list and candidate-list
+ * counts and entries: placed spheres of its own scene or wall slots of its
+ * scene's range; its constrained-wall and pair slots).  counts[0..7]:
+ * violations by kind (0 model, 1 scene, 2 neighbour count, 3 neighbour
+ * entry, 5 candidate count, 6 candidate entry, 7 constrained-wall slot);
+ * counts[8..10]: the first violation's kind + 1, sphere slot and entry. */
+int vg_check_lists(vg_scene *h, long long counts[11]);
 /* Counters since vg_create: rebuilds, relists, max neighbour-list fill, the
  * list capacity, and contact_steps, the sum over steps of the number of
  * sphere-sphere and sphere-wall pairs left in contact by the step's solve.

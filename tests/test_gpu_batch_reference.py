@@ -91,6 +91,8 @@ def test_batched_configuration_scenes_equal_alone():
     ids = [b.admit(wa, steps) for wa in was]
     for _ in range(steps // up + 1):
         b.step(up)
+        chk = b.check_lists()
+        assert not any(chk["violations"].values()), chk
     got = []
     for sid in ids:
         state, done, _ = b.status(sid)

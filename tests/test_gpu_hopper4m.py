@@ -11,7 +11,9 @@ properties that hold at any size:
   atomics);
 * a restart at a rebuild step from the remaining DOStatus (with their
   DONames) and the exported IactRecordTab continues bitwise as the
-  uninterrupted run.
+  uninterrupted run;
+* every index the step, wall and rebuild kernels take from the device
+  lists is in range (vg_check_lists) every 2,000 steps and at the end.
 
 The zone's lower bound is raised into the funnel, 60 um under the centre of
 the sixth lattice layer (layers 2.2 r_large apart, centres jittered by
@@ -70,7 +72,9 @@ def test_hopper_4m_zone_determinism_restart():
         done += 10 * UP
         counts.append(_spheres(a, wa))
         if done % 2000 == 0:
-            print(f"  step {done}: {counts[-1]} spheres, {a.count_active()} contacts", flush=True)
+            chk = a.check_lists()
+            print(f"  step {done}: {counts[-1]} spheres, {a.count_active()} contacts, list check {chk}", flush=True)
+            assert not any(chk["violations"].values()), chk
     print(f"  step {done}: {counts[-1]} spheres, {a.count_active()} contacts", flush=True)
     assert all(x >= y for x, y in zip(counts, counts[1:])), counts
     assert counts[-1] < counts[0], "the falling bed crosses the zone's bound"
@@ -96,5 +100,8 @@ def test_hopper_4m_zone_determinism_restart():
     for f in ("pos", "vel", "av"):
         assert _bits(s1[f], s2[f]) == 0, f"{f}: restart differs"
     print(f"  restart: bitwise equal after {3 * UP + 7} more steps ({a.n_do} DOs)", flush=True)
+    for sim in (a, c):
+        chk = sim.check_lists()
+        assert not any(chk["violations"].values()), chk
     a.close()
     c.close()

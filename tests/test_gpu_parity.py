@@ -74,6 +74,8 @@ def test_gpu_matches_reference(name):
         ref = z[f"d{k}_state"]
         ids = orc.ids() if zoi else np.arange(wa.n_do)
         assert sim.n_do == len(ids) == ref.shape[0], (sim.n_do, len(ids), ref.shape[0])
+        chk = sim.check_lists()
+        assert not any(chk["violations"].values()), chk
         assert np.array_equal(sim.models(), wa.model_of[ids]), "DONames of the remaining DOs"
         sphere = wa.model_shape[wa.model_of[ids]] == 1
         fields = dict(pos=0, vel=1, ox=2, oz=3, av=4, impact=5, ang_impact=6)

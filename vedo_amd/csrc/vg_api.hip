@@ -2461,6 +2461,31 @@ extern "C" int vg_rebuild_timing(vg_scene *h, double *ms, int *rebuilds)
     return 0;
 }
 
+extern "C" int vg_check_lists(vg_scene *h, long long *counts)
+{
+    if (!h || !counts) return VG_EINVAL;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    unsigned long long *d = nullptr, c[11] = {0};
+    if (dalloc(h, &d, 11)) return VG_ENOMEM;
+    int rc = 0;
+    if (hipMemsetAsync(d, 0, sizeof(c), h->stream) != hipSuccess) rc = fail(h, VG_EHIP, "memset failed");
+    if (!rc && h->n > 0)
+        hipLaunchKernelGGL(k_check_lists, dim3(nblk(h->n, 256)), dim3(256), 0, h->stream, h->geo[h->cur],
+                           h->ell[h->ecur], h->n, h->ld, h->ki, h->ko, (const unsigned *)h->ocount,
+                           (const unsigned *)h->onbr, (const ModelDev *)h->models, h->models_cap,
+                           (const SceneDev *)h->scenes, h->scap, (const WallDev *)h->walls, h->wcap,
+                           h->any_constrained ? (const int *)h->cw_w : (const int *)nullptr,
+                           (const int *)h->cw_p, h->cw_cap, d);
+    if (!rc && (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(c, d, sizeof(c), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess))
+        rc = fail(h, VG_EHIP, "list check failed");
+    (void)hipFree(d);
+    if (rc) return rc;
+    for (int k = 0; k < 11; ++k) counts[k] = (long long)c[k];
+    return 0;
+}
+
 extern "C" int vg_stats(vg_scene *h, long long *rebuilds, long long *relists, int *max_fill, int *capacity,
                         long long *contact_steps)
 {

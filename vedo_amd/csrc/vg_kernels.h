@@ -195,6 +195,10 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
                                 const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,
                                 const unsigned *sphere_id, const double *rec, ActiveOut *out, unsigned long long cap,
                                 unsigned long long *counter, int backup, int no_bond);
+__global__ void k_check_lists(Geo g, Ell ell, int n, int ld, int ki, int ko, const unsigned *o_cnt,
+                              const unsigned *o_nbr, const ModelDev *models, int n_models, const SceneDev *scenes,
+                              int scap, const WallDev *walls, int wcap, const int *cw_w, const int *cw_p,
+                              long long cw_cap, unsigned long long *out);
 __global__ void k_wall_impulse(Ell ell, int b, int e, int ld, int ki, int w0, const double *rec, int no_bond, double *out);
 __global__ void k_assign_spheres(Geo g, int b, int e, int ld, const double *buf, double *rec_imp);
 // Knight analyses (src/vedo/knight) of a scene's current state.

@@ -1044,6 +1044,60 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
     }
 }
 
+// Consistency of the device lists: every index the step, wall and rebuild
+// kernels form from them lies where it must (checked before it is used):
+// each sphere's model and scene, its neighbour-list count and entries
+// (a placed sphere of the same scene, or a wall slot of the scene's range),
+// its candidate-list count and entries likewise, its constrained-wall slots
+// and pair slots.  out[kind] counts violations; out[8..10] hold the first
+// one (kind, sphere, entry).
+__global__ void k_check_lists(Geo g, Ell ell, int n, int ld, int ki, int ko, const unsigned *o_cnt,
+                              const unsigned *o_nbr, const ModelDev *models, int n_models, const SceneDev *scenes,
+                              int scap, const WallDev *walls, int wcap, const int *cw_w, const int *cw_p,
+                              long long cw_cap, unsigned long long *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    auto bad = [&](int kind, int k) {
+        if (atomicAdd(&out[kind], 1ull) == 0ull && atomicCAS(&out[8], 0ull, (unsigned long long)kind + 1) == 0ull) {
+            out[9] = (unsigned long long)i;
+            out[10] = (unsigned long long)(long long)k;
+        }
+    };
+    const int m = g.mdl[i];
+    if (m < 0 || m >= n_models) { bad(0, -1); return; }
+    const int sc = models[m].scene;
+    if (sc < 0 || sc >= scap) { bad(1, -1); return; }
+    const SceneDev &S = scenes[sc];
+    if (S.w_begin < 0 || S.w_end > wcap || S.w_begin > S.w_end) { bad(1, -2); return; }
+    auto partner_ok = [&](unsigned j) -> bool {
+        if (j < (unsigned)ld) {
+            if (j >= (unsigned)n) return false;
+            const int mj = g.mdl[j];
+            return mj >= 0 && mj < n_models && models[mj].scene == sc;
+        }
+        const unsigned w = j - (unsigned)ld;
+        return w >= (unsigned)S.w_begin && w < (unsigned)S.w_end && walls[w].scene == sc;
+    };
+    const unsigned hdr = ell.hdr[i];
+    const int h = hdr >> 7, cnt = hdr & 127;
+    if (cnt > ki) bad(2, cnt);
+    for (int k = 0; k < min(cnt, ki); ++k)
+        if (!partner_ok(ell.ent[((size_t)(h * ki + k)) * (size_t)ld + i] & PARTNER_MASK)) bad(3, k);
+    const unsigned oc = o_cnt[i];
+    if (oc > (unsigned)ko) bad(5, (int)oc);
+    for (unsigned t = 0; t < min(oc, (unsigned)ko); ++t)
+        if (!partner_ok(o_nbr[(size_t)t * ld + i] & PARTNER_MASK)) bad(6, (int)t);
+    if (cw_w)
+        for (int q = 0; q < 2; ++q) {
+            const int w = cw_w[(size_t)q * ld + i];
+            if (w == -1) continue;
+            if (w < S.w_begin || w >= S.w_end || !walls[w].constrained) bad(7, q);
+            const int pq = cw_p[(size_t)q * ld + i];
+            if (pq < 0 || (long long)pq >= cw_cap) bad(7, 2 + q);
+        }
+}
+
 // Consultant::GetIactNum: sphere pairs are listed from both sides, walls once.
 __global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int b, int e, int ld, int ko,
                              unsigned long long *out)

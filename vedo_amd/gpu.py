@@ -64,7 +64,7 @@ assert STATUS_DTYPE.itemsize == C.sizeof(VgStatus)
 
 EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_set_history", "vg_step",
            "vg_sync", "vg_step_assigned", "vg_set_wall_rotation", "vg_get_state", "vg_time",
-           "vg_num_interactions", "vg_scene_num_do", "vg_scene_models", "vg_set_skin", "vg_set_skin_auto", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_rebuild_timing", "vg_stats", "vg_copy_bench",
+           "vg_num_interactions", "vg_scene_num_do", "vg_scene_models", "vg_set_skin", "vg_set_skin_auto", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_rebuild_timing", "vg_check_lists", "vg_stats", "vg_copy_bench",
            "vg_system_energy", "vg_batch_create", "vg_admit", "vg_retire", "vg_scene_status",
            "vg_scene_get_state", "vg_scene_get_active", "vg_scene_energy", "vg_scene_set_wall_rotation",
            "vg_batch_info", "vg_scene_coordination_number", "vg_scene_packing_density",
@@ -121,6 +121,7 @@ def lib():
         L.vg_step_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.vg_kernel_timing.argtypes = [vp, C.c_int]
         L.vg_rebuild_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.vg_check_lists.argtypes = [vp, C.POINTER(C.c_longlong)]
         L.vg_set_skin.argtypes = [vp, C.c_double]
         L.vg_scene_coordination_number.argtypes = [vp, C.c_int, _D, _D]
         L.vg_scene_packing_density.argtypes = [vp, C.c_int, _D, C.c_int, C.c_int, _D]
@@ -374,6 +375,19 @@ class GpuSimulation(_Knight):
         return dict(rebuilds=rb.value, relists=rl.value, max_fill=mf.value, capacity=cap.value,
                     contact_steps=cs.value)
 
+    def check_lists(self):
+        """vg_check_lists: violations by kind and the first one."""
+        return _check_lists(self)
+
+
+def _check_lists(obj):
+    out = (C.c_longlong * 11)()
+    obj._check(obj.L.vg_check_lists(obj.h, out))
+    v = list(out)
+    names = ("model", "scene", "neighbour_count", "neighbour_entry", "unused", "candidate_count",
+             "candidate_entry", "constrained_slot")
+    return {"violations": {names[k]: v[k] for k in range(8) if k != 4}, "first": v[8:11]}
+
 
 SCENE_FREE, SCENE_QUEUED, SCENE_RUNNING, SCENE_FINISHED, SCENE_RETIRING, SCENE_DONE, SCENE_FAILED = range(7)
 
@@ -490,6 +504,10 @@ class GpuBatch(_Knight):
         sp, on, q, st = C.c_longlong(), C.c_int(), C.c_int(), C.c_ulonglong()
         self._check(self.L.vg_batch_info(self.h, C.byref(sp), C.byref(on), C.byref(q), C.byref(st)))
         return dict(spheres=sp.value, on_device=on.value, queued=q.value, steps=st.value)
+
+    def check_lists(self):
+        """vg_check_lists: violations by kind and the first one."""
+        return _check_lists(self)
 
     def stats(self):
         rb, rl, mf, cap, cs = C.c_longlong(), C.c_longlong(), C.c_int(), C.c_int(), C.c_longlong()

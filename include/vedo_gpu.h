@@ -2,10 +2,12 @@
  * vedo_gpu.h -- C-ABI of the MI355X DEM step library (libvedo_hip.so).
  *
  * The library replaces the inner work of VEDO's SimMediator::Run()
- * (SimMediator.cpp:714-842) for scenes of spheres against quasi-plates and
- * quasi-cylinders: the Consultant's candidate list and rebuild cadence
- * (LeapConsultant.cpp:108-195, NBSConsultant.cpp:85-316), the contact
+ * (SimMediator.cpp:714-842) for scenes of spheres against quasi-plates,
+ * quasi-plates with a circular hole and quasi-cylinders: the Consultant's
+ * candidate list and rebuild cadence (LeapConsultant.cpp:108-195,
+ * NBSConsultant.cpp:85-316), the contact
  * detectors (CDSphere2Sphere.cpp:14-60, CDSphere2QuasiPlate.cpp:15-97,
+ * CDSphere2QuasiPlateWithCircularHole.cpp:16-134,
  * CDSphere2QuasiCylinder.cpp:16-80), the five impact solvers (ISwBSDBF.cpp,
  * ISwBSDB2F.cpp, ISwBtSDBF.cpp, ISwH.cpp, ISwHertz.cpp), the per-contact
  * ImpactStatus history and its IactRecordTab remap (NBSConsultant.cpp:376-395,
@@ -38,6 +40,7 @@ extern "C" {
 #define VG_SPHERE 1
 #define VG_QUASICYLINDER 6
 #define VG_QUASIPLATE 7
+#define VG_QUASIPLATE_HOLE 8   /* QuasiPlateWithCircularHole */
 #define VG_CONSTRAINED 1
 #define VG_FIXED 2
 #define VG_MOBILE 3
@@ -67,7 +70,10 @@ typedef struct {
     int shape, behavior, scope, group;
     double density, density_factor;
     double fext[3];
-    double attr[3]; /* Sphere: radius | QuasiCylinder: radius, height | QuasiPlate: width, height, length */
+    /* Sphere: radius | QuasiCylinder: radius, height | QuasiPlate: width,
+     * height, length | QuasiPlateWithCircularHole: width, height, length,
+     * hole radius, hole x / y offset (DOModel.cpp:90-120) */
+    double attr[6];
 } vg_model;
 
 /* One entry of the dense group x group IactModel table (IactModel.cpp:16-30,

@@ -4,6 +4,8 @@ The golden fixtures (tests/golden, written by the compiled reference) with
 UpIact 7 run together in one batch (vg_batch_create / vg_admit) and through
 the continuous-batching server with a capacity below their total size, so
 that later scenes wait for slots and are placed at later rebuild boundaries.
+The plate with a circular hole of box_hole makes the whole batch run the
+kernels that read the hole table, the other scenes' walls marked hole-free.
 Each scene's state is compared directly with the reference's dumps:
 bit-exact for the spring-dashpot laws, relative 1e-9 for ISwH / ISwHertz
 (device pow), as in tests/test_gpu_parity.py.  A full-size property test
@@ -18,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 UP = 7
 CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_bond", "box_bond_bt", "box_cyl",
-         "box_orbital", "box_piston"]
+         "box_orbital", "box_piston", "box_hole"]
 EXACT_LAWS = {1, 2, 3}
 
 

@@ -25,7 +25,7 @@ from vedo_amd import metrics, scenes
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["box_bsdbf", "box_cyl", "box_orbital", "box_piston", "drum_kin"])
+@pytest.mark.parametrize("name", ["box_bsdbf", "box_cyl", "box_orbital", "box_piston", "box_hole", "drum_kin"])
 def test_gpu_wall_pressure_equals_reference_records(name):
     from vedo_amd.gpu import GpuSimulation
     z, wa = load_golden(name)

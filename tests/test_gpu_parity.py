@@ -24,12 +24,7 @@ from conftest import GOLDEN_CASES, load_golden, step_golden
 
 pytestmark = pytest.mark.gpu
 
-# box_hole: the plate with a circular hole is restated in the oracle only; its
-# GPU path was taken out after a build that carried it made the step kernel
-# misbehave on an unrelated scene, with no cause shown yet (vg_create rejects
-# the shape)
-GPU_CASES = [c if c != "box_hole" else pytest.param(c, marks=pytest.mark.skip(
-    reason="plate with a circular hole: GPU path taken out, oracle only")) for c in GOLDEN_CASES if c != "box_nbs"]
+GPU_CASES = [c for c in GOLDEN_CASES if c != "box_nbs"]
 EXACT_LAWS = {1, 2, 3}
 
 

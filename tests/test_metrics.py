@@ -17,7 +17,7 @@ from conftest import load_golden, step_golden
 from oracle.oracle import OracleSim
 from vedo_amd import metrics, scenes
 
-WALL_CASES = ["box_bsdbf", "box_cyl", "box_orbital", "box_piston", "drum_kin"]
+WALL_CASES = ["box_bsdbf", "box_cyl", "box_orbital", "box_piston", "box_hole", "drum_kin"]
 
 
 def _fixture_records(z, k):

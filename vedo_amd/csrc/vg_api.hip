@@ -58,6 +58,8 @@ struct SceneHost {
     std::vector<double> ori0;
     std::vector<int> mdl0;
     std::vector<WallDev> walls0;
+    std::vector<double4> holes0;        // per wall: hole radius, x / y offset, 1 for a holed plate
+    bool has_hole = false;              // a QuasiPlateWithCircularHole among the walls
     std::vector<WallKin> kin0;          // rotations prescribed before placement
     std::vector<char> kin0_set;
     // result kept on the host once a finished scene leaves the device
@@ -109,6 +111,7 @@ struct vg_scene {
     int *rflags = nullptr;              // [scap][3] relist rings
     WallDev *walls = nullptr, *walls_ref = nullptr;
     WallKin *wkin = nullptr;
+    double4 *holes = nullptr;           // [wcap] hole table; allocated when a scene with a holed plate is placed
     double *assigned = nullptr;
     double *assigned_sph = nullptr;     // [ld][16] assigned sphere statuses by rank (allocated on first use)
     double *en_part = nullptr, *en_out = nullptr;
@@ -231,7 +234,7 @@ static double model_range(const vg_model &m)
     // DOModel::Analysis (DOModel.cpp:17-149), global walls
     if (m.shape == VG_SPHERE) return m.attr[0];
     if (m.shape == VG_QUASICYLINDER) return m.scope == VG_LOCAL ? std::max(m.attr[0], 0.5 * m.attr[1]) : m.attr[0];
-    if (m.shape == VG_QUASIPLATE) {
+    if (m.shape == VG_QUASIPLATE || m.shape == VG_QUASIPLATE_HOLE) {   // DOModel.cpp:90-106
         if (m.scope == VG_LOCAL) return 0.5 * std::max(std::max(m.attr[0], m.attr[1]), m.attr[2]);
         return 0.5 * m.attr[1];
     }
@@ -345,7 +348,7 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
             }
             S.do_to_sphere[i] = (int)S.sphere_ids.size();
             S.sphere_ids.push_back((unsigned)i);
-        } else if (M.shape == VG_QUASIPLATE || M.shape == VG_QUASICYLINDER) {
+        } else if (M.shape == VG_QUASIPLATE || M.shape == VG_QUASICYLINDER || M.shape == VG_QUASIPLATE_HOLE) {
             if (M.scope != VG_GLOBAL ||
                 (M.behavior != VG_FIXED && M.behavior != VG_ORBITAL && M.behavior != VG_CONSTRAINED)) {
                 err = "walls must be global and fixed, orbital or constrained";
@@ -437,6 +440,7 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
         S.mdl0[s] = d->model_of[id];
     }
     S.walls0.resize(S.nw);
+    S.holes0.assign(S.nw, make_double4(0.0, 0.0, 0.0, 0.0));
     for (int w = 0; w < S.nw; ++w) {
         const unsigned id = S.wall_ids[w];
         const vg_model &M = d->models[d->model_of[id]];
@@ -449,17 +453,28 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
         W.av = mk(d->av[3 * id], d->av[3 * id + 1], d->av[3 * id + 2]);
         W.a0 = M.attr[0]; W.a1 = M.attr[1]; W.a2 = M.attr[2];
         W.range = model_range(M);
-        W.reach = M.shape == VG_QUASIPLATE ? 0.5 * std::sqrt(M.attr[0] * M.attr[0] + M.attr[1] * M.attr[1] +
-                                                             M.attr[2] * M.attr[2])
-                                           : std::sqrt(M.attr[0] * M.attr[0] + 0.25 * M.attr[1] * M.attr[1]);
+        // a holed plate moves, bounds its contacts and enters the candidate
+        // list as its plate (DOMap::CalDistance ignores the hole, DOMap.cpp:
+        // 336-374); only its contact detection reads the hole table
+        const bool holed = M.shape == VG_QUASIPLATE_HOLE;
+        const bool plate = M.shape == VG_QUASIPLATE || holed;
+        if (holed) {
+            S.holes0[w] = make_double4(M.attr[3], M.attr[4], M.attr[5], 1.0);
+            S.has_hole = true;
+        }
+        W.reach = plate ? 0.5 * std::sqrt(M.attr[0] * M.attr[0] + M.attr[1] * M.attr[1] + M.attr[2] * M.attr[2])
+                        : std::sqrt(M.attr[0] * M.attr[0] + 0.25 * M.attr[1] * M.attr[1]);
         W.path = 0.0;
-        W.shape = M.shape; W.group = M.group; W.behavior = M.behavior; W.kin = 0;
+        W.shape = plate ? VG_QUASIPLATE : M.shape; W.group = M.group; W.behavior = M.behavior; W.kin = 0;
         // DOModel::Analysis (DOModel.cpp:56-84): the constrained walls' sudo-mass;
         // fixed and orbital walls carry none (DiscreteObject of DOFixed*/DOOrbital*)
         W.constrained = M.behavior == VG_CONSTRAINED;
         if (W.constrained) {
-            const double vol = M.shape == VG_QUASIPLATE ? M.attr[0] * M.attr[1] * M.attr[2]
-                                                         : PI * M.attr[0] * M.attr[0] * M.attr[1];
+            double vol = plate ? M.attr[0] * M.attr[1] * M.attr[2] : PI * M.attr[0] * M.attr[0] * M.attr[1];
+            // DOModel.cpp:108-116: the hole is cut out when it lies inside the plate
+            if (holed && std::fabs(M.attr[4]) <= (0.5 * M.attr[0] - M.attr[3]) &&
+                std::fabs(M.attr[5]) <= (0.5 * M.attr[2] - M.attr[3]))
+                vol = M.attr[0] * M.attr[1] * M.attr[2] - M.attr[3] * M.attr[3] * PI * M.attr[1];
             W.msudo = M.density_factor * (M.density * vol);
         }
         W.fext[0] = M.fext[0]; W.fext[1] = M.fext[1]; W.fext[2] = M.fext[2];
@@ -624,6 +639,15 @@ static int place_queued(vg_scene *h)
             }
             HIPCHK(hipMemcpy(h->walls + S.w_begin, S.walls0.data(), sizeof(WallDev) * S.nw, hipMemcpyHostToDevice));
             for (int w = 0; w < S.nw; ++w) h->walls_h[S.w_begin + w] = S.walls0[w];
+            // the hole table exists once a scene with a holed plate is placed;
+            // from then on every placed scene writes its slots (zeros: no hole)
+            if (S.has_hole && !h->holes) {
+                if (dalloc(h, &h->holes, (size_t)h->wcap)) return VG_ENOMEM;
+                HIPCHK(hipMemset(h->holes, 0, sizeof(double4) * (size_t)std::max(h->wcap, 1)));
+            }
+            if (h->holes)
+                HIPCHK(hipMemcpy(h->holes + S.w_begin, S.holes0.data(), sizeof(double4) * S.nw,
+                                 hipMemcpyHostToDevice));
             h->nw_used = std::max(h->nw_used, S.w_begin + S.nw);
         }
         const double t2[2] = {S.t, S.t};
@@ -1138,10 +1162,12 @@ static int rebuild(vg_scene *h)
         oa.safe = h->safe; oa.walls = h->walls; oa.t = tables(h); oa.count = h->ocount; oa.nbr = h->onbr;
         oa.ko = h->ko; oa.single_group = h->single_group; oa.flags = h->flags;
         oa.ovf = h->batch ? h->ovf : nullptr;
+        oa.holes = h->holes;
         if (h->batch) HIPCHK(hipMemsetAsync(h->ovf, 0, sizeof(int) * h->scap, st));
         for (;;) {
             HIPCHK(hipMemsetAsync(h->flags + 5, 0, sizeof(int), st));
-            hipLaunchKernelGGL(k_outer, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
+            if (h->holes) hipLaunchKernelGGL(k_outer<true>, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
+            else hipLaunchKernelGGL(k_outer<false>, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
             int maxc = 0;
             HIPCHK(hipMemcpyAsync(&maxc, h->flags + 5, sizeof(int), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -1287,7 +1313,7 @@ static void free_scene(vg_scene *h)
         F(h->xs[b]); F(h->ys[b]); F(h->zs[b]);
     }
     F(h->models); F(h->mm); F(h->laws); F(h->scenes); F(h->stime); F(h->rflags); F(h->walls); F(h->walls_ref);
-    F(h->wkin); F(h->assigned); F(h->assigned_sph); F(h->en_part); F(h->en_out); F(h->flags); F(h->relists);
+    F(h->wkin); F(h->holes); F(h->assigned); F(h->assigned_sph); F(h->en_part); F(h->en_out); F(h->flags); F(h->relists);
     F(h->contacts); F(h->red); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->qidx); F(h->segs);
     F(h->qval); F(h->qsel_tg); F(h->qsel_hist); F(h->qmap); F(h->safe); F(h->lin); F(h->cstart); F(h->cend); F(h->ocount); F(h->onbr); F(h->tmp); F(h->rec);
     F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->sid[0]); F(h->sid[1]); F(h->fetch); F(h->ovf);
@@ -1493,6 +1519,7 @@ static StepArgs step_args(vg_scene *h)
     a.rflags = h->rflags; a.flags = h->flags; a.relists = h->relists; a.contacts = h->contacts;
     a.step = (int)(h->step_count % 3);
     a.rec_imp = h->rec_imp; a.rec = h->rec;
+    a.holes = h->holes;
     a.single_group = h->single_group;
     a.no_bond = h->no_bond;
     a.cw_w = h->cw_w; a.cw_p = h->cw_p; a.cw_val = h->cw_val; a.cw_stamp = h->cw_stamp;
@@ -1510,11 +1537,12 @@ static int export_active(vg_scene *h, int slot, ActiveOut *out, size_t cap, unsi
 {
     const SceneHost &S = h->sc[slot];
     HIPCHK(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), h->stream));
-    hipLaunchKernelGGL(k_export_active, dim3(nblk(S.n, 256)), dim3(256), 0, h->stream, h->geo[h->cur],
-                       h->ell[h->ecur], S.seg_begin, S.seg_end, h->ld, h->ki, (const unsigned *)h->ocount,
-                       (const unsigned *)h->onbr, (const WallDev *)h->walls, (const unsigned *)d_wid,
-                       (const unsigned *)d_sid, h->ent_valid ? (const double *)h->rec : (const double *)nullptr, out,
-                       (unsigned long long)cap, dcnt, backup ? 1 : 0, h->no_bond);
+    hipLaunchKernelGGL(h->holes ? k_export_active<true> : k_export_active<false>, dim3(nblk(S.n, 256)), dim3(256),
+                       0, h->stream, h->geo[h->cur], h->ell[h->ecur], S.seg_begin, S.seg_end, h->ld, h->ki,
+                       (const unsigned *)h->ocount, (const unsigned *)h->onbr, (const WallDev *)h->walls,
+                       (const unsigned *)d_wid, (const unsigned *)d_sid,
+                       h->ent_valid ? (const double *)h->rec : (const double *)nullptr, out, (unsigned long long)cap,
+                       dcnt, backup ? 1 : 0, h->no_bond, (const double4 *)h->holes);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1607,16 +1635,18 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
     if (kt) HIPCHK(hipEventRecord(h->kev[2 * h->kev_used], h->stream));
     const dim3 grid(nblk(std::max(h->n, h->scap), 256)), blk(256);
     // kernel specialisation: law set of the placed scenes, record path, precision
-    const int ls = h->sd_only ? LS_SD : (h->h_only ? LS_H : LS_ALL);
+    const int ls = h->holes ? LS_HOLE : (h->sd_only ? LS_SD : (h->h_only ? LS_H : LS_ALL));
     const bool mix = h->precision == VG_PRECISION_MIXED;
     using KFn = void (*)(StepArgs);
-    static const KFn kfn[2][3][2] = {
+    static const KFn kfn[2][4][2] = {
         {{k_step<false, LS_SD, false>, k_step<false, LS_SD, true>},
          {k_step<false, LS_H, false>, k_step<false, LS_H, true>},
-         {k_step<false, LS_ALL, false>, k_step<false, LS_ALL, true>}},
+         {k_step<false, LS_ALL, false>, k_step<false, LS_ALL, true>},
+         {k_step<false, LS_HOLE, false>, k_step<false, LS_HOLE, true>}},
         {{k_step<true, LS_SD, false>, k_step<true, LS_SD, true>},
          {k_step<true, LS_H, false>, k_step<true, LS_H, true>},
-         {k_step<true, LS_ALL, false>, k_step<true, LS_ALL, true>}}};
+         {k_step<true, LS_ALL, false>, k_step<true, LS_ALL, true>},
+         {k_step<true, LS_HOLE, false>, k_step<true, LS_HOLE, true>}}};
     hipLaunchKernelGGL(kfn[rec ? 1 : 0][ls][mix ? 1 : 0], grid, blk, 0, h->stream, a);
     if (kt) {
         HIPCHK(hipEventRecord(h->kev[2 * h->kev_used + 1], h->stream));

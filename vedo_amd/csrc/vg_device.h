@@ -142,6 +142,65 @@ __device__ inline void cd_wall(d3 xa, double ra, const WallDev &w, Contact &c)
     else cd_sc(xa, ra, w, c);
 }
 
+// CDSphere_QuasiPlateWithCircularHole::CalDistance + Detect
+// (CDSphere2QuasiPlateWithCircularHole.cpp:16-134).  The plate's rectangle
+// as in cd_sp; where the sphere's centre projects into the hole, the closest
+// point moves out to the hole's rim.  hole = (radius, x offset, y offset,
+// 1): the hole's centre is the plate's centre plus (x, y, 0) in global axes
+// (:46), its rim point is placed along the plate's own axes.
+__device__ inline void cd_sph(d3 xa, double ra, const WallDev &w, double4 hole, Contact &c)
+{
+    const double hr = hole.x, hx = hole.y, hy = hole.z;
+    d3 oy = cross(w.oz, w.ox);
+    d3 cap = xa - project_on(xa - w.pos, w.oz);
+    double hh = 0.5 * w.a1, hw = 0.5 * w.a0, hl = 0.5 * w.a2;
+    const d3 cha = xa - (w.pos + mk(hx, hy, 0.0));
+    double dx = dot(cap - w.pos, w.ox);
+    if (dx < -hw) dx = -hw;
+    else if (dx > hw) dx = hw;
+    double dy = dot(cap - w.pos, oy);
+    if (dy < -hl) dy = -hl;
+    else if (dy > hl) dy = hl;
+    const double dhx = dot(cha, w.ox), dhy = dot(cha, oy);
+    const double dh = sqrt(dhx * dhx + dhy * dhy);
+    if (dh < hr) {
+        if (dh == 0.0) {
+            dx = hx;
+            dy = hy;
+        } else {
+            dx = hx + dhx / dh * hr;
+            dy = hy + dhy / dh * hr;
+        }
+    }
+    d3 caps = (muls(w.ox, dx) + muls(oy, dy)) + w.pos;
+    d3 im = caps - xa;
+    if (dh == 0.0) {
+        const double l = len(im);
+        c.depth = ra + hh - sqrt(l * l + hr * hr);
+    } else {
+        c.depth = ra + hh - len(im);
+    }
+    c.n = dir(im);
+    c.p = xa + smul(ra - 0.5 * c.depth, c.n);
+    c.active = c.depth > 0;
+}
+
+// Contact with wall slot ws.  Kernels built for scenes with holed plates
+// (HOLE) look the slot up in the hole table (hole.w = 1 marks a holed plate,
+// which the wall table stores as a plate); the others never read it.
+template <bool HOLE>
+__device__ inline void cd_wall_at(d3 xa, double ra, const WallDev &w, const double4 *holes, int ws, Contact &c)
+{
+    if (HOLE) {
+        const double4 hole = holes[ws];
+        if (hole.w != 0.0) {
+            cd_sph(xa, ra, w, hole, c);
+            return;
+        }
+    }
+    cd_wall(xa, ra, w, c);
+}
+
 // DOMap::CalDistance for sphere-plate / sphere-cylinder (DOMap.cpp:353-412)
 __device__ inline double domap_wall_distance(d3 ca, const WallDev &w)
 {
@@ -215,8 +274,10 @@ __device__ inline d3 shear_rotation(d3 f, d3 d)
 }
 
 // Law sets a step kernel is specialised for: only the spring-dashpot family,
-// only ISwH, or all five (smaller kernels where fewer laws are present).
-enum { LS_SD = 0, LS_H = 1, LS_ALL = 2 };
+// only ISwH, or all five (smaller kernels where fewer laws are present);
+// LS_HOLE is all five laws in scenes that hold a plate with a circular hole
+// (the only set whose kernels evaluate that shape).
+enum { LS_SD = 0, LS_H = 1, LS_ALL = 2, LS_HOLE = 3 };
 
 // One ImpactSolver::NextStep for slave A and master B, for the laws of set LS.
 // ISwBSDBF/ISwBSDB2F/ISwBtSDBF (ISwBSDBF.cpp:26-139, ISwBSDB2F.cpp:26-149,
@@ -230,7 +291,7 @@ __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va
     const int law = L.law;
     const double depth = c.depth;
     const d3 n = c.n;
-    if (LS == LS_ALL && law == LAW_HERTZ) {
+    if (LS >= LS_ALL && law == LAW_HERTZ) {
         o.linear_only = true;
         if (!c.active) {
             st_clean(h);

@@ -93,6 +93,7 @@ struct StepArgs {
     // record path
     double *rec_imp;         // [6][LD] last impulse / angular impulse per sphere
     double *rec;             // [KI][13][LD] point, dir, to_master, ang_to_master, overlap
+    const double4 *holes;    // per wall slot: hole radius, x / y offset, 1 for a holed plate (LS_HOLE kernels)
 };
 
 enum { ERR_INNER_CAP = 1, ERR_OUTER_CAP = 2, ERR_CW = 4 };
@@ -117,6 +118,7 @@ struct OuterArgs {
     int ko;
     int *flags;              // [5]: largest candidate count seen
     int *ovf;                // per scene: a sphere has more candidates than ko (batches; may be null)
+    const double4 *holes;    // per wall slot: hole radius, x / y offset, 1 for a holed plate (HOLE kernels)
 };
 
 struct InnerArgs {
@@ -169,7 +171,7 @@ __global__ void k_reorder(Geo src, Geo dst, int n, int ld, const unsigned *perm,
                           const ModelDev *models, double *safe, int *lin, const double *imp_src, double *imp_dst);
 __global__ void k_fill_int(int *p, long long n, int v);
 __global__ void k_cellstart(const int *lin, int n, int *cstart, int *cend);
-__global__ void k_outer(OuterArgs a);
+template <bool HOLE> __global__ void k_outer(OuterArgs a);
 __global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int b, int e, int ld, int ko,
                              unsigned long long *out);
 __global__ void k_inner_build(InnerArgs a);
@@ -191,10 +193,11 @@ __global__ void k_restore_history(Geo g, Ell ell, int b, int e, int ld, int ki, 
                                   long long nrec, const ModelDev *models, const SceneDev *scenes, const LawDev *laws,
                                   const WallDev *walls);
 __global__ void k_scatter_state(Geo g, int b, int e, int ld, const double *rec_imp, double *out);
+template <bool HOLE>
 __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *o_cnt,
                                 const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,
                                 const unsigned *sphere_id, const double *rec, ActiveOut *out, unsigned long long cap,
-                                unsigned long long *counter, int backup, int no_bond);
+                                unsigned long long *counter, int backup, int no_bond, const double4 *holes);
 __global__ void k_check_lists(Geo g, Ell ell, int n, int ld, int ki, int ko, const unsigned *o_cnt,
                               const unsigned *o_nbr, const ModelDev *models, int n_models, const SceneDev *scenes,
                               int scap, const WallDev *walls, int wcap, const int *cw_w, const int *cw_p,

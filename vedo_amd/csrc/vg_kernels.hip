@@ -86,6 +86,7 @@ __device__ inline void store_hist(const Ell &e, int h, int k, int ki, int n, int
 // Rebuild sphere i's neighbour list from its candidate list: keep partners
 // within `skin` of contact, carry flags and history of partners already
 // listed (both lists are ordered subsequences of the candidate list).
+template <bool HOLE>
 __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_old, int c_old, int h_new,
                              double skin)
 {
@@ -104,7 +105,7 @@ __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_o
             if (dot(dx, dx) > lim * lim * (1.0 + 1e-9)) near = false;
             else cd_ss(xi, ri, ld3(pj), pj.w, c);
         } else {
-            cd_wall(xi, ri, a.walls[j - a.ld], c);
+            cd_wall_at<HOLE>(xi, ri, a.walls[j - a.ld], a.holes, (int)(j - a.ld), c);
         }
         const bool carried = (p < c_old) && (old_partner == j);
         unsigned f = 0;
@@ -222,7 +223,7 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
     } else {
         w = &a.walls[j - a.ld];
         law = &a.t.laws[S.law_off + gi * S.n_groups + w->group];
-        cd_wall(xi, ri, *w, c);
+        cd_wall_at<LS == LS_HOLE>(xi, ri, *w, a.holes, (int)(j - a.ld), c);
         xa = xi; va = vi; wa = wi; ma = mi;
         xb = w->pos; vb = w->vel; wb = w->av; mb = w->msudo;
         if (MIX) to_mixed(c, xi, w->pos, cm);
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
         cnt = hdr & 127;
         if (do_relist) {
             const int hn = h ^ 1;
-            cnt = relist(a, i, xi, ri, h, cnt, hn, S.skin);
+            cnt = relist<LS == LS_HOLE>(a, i, xi, ri, h, cnt, hn, S.skin);
             h = hn;
             a.ell.hdr[i] = (unsigned char)((h << 7) | cnt);
             relisted = true;
@@ -507,6 +508,10 @@ template __global__ void k_step<false, LS_H, true>(StepArgs);
 template __global__ void k_step<true, LS_H, true>(StepArgs);
 template __global__ void k_step<false, LS_ALL, true>(StepArgs);
 template __global__ void k_step<true, LS_ALL, true>(StepArgs);
+template __global__ void k_step<false, LS_HOLE, false>(StepArgs);
+template __global__ void k_step<true, LS_HOLE, false>(StepArgs);
+template __global__ void k_step<false, LS_HOLE, true>(StepArgs);
+template __global__ void k_step<true, LS_HOLE, true>(StepArgs);
 
 // Wall responses after the spheres' (DOFixedQuasiPlate.h:16-19,
 // DOOrbitalQuasiPlate.cpp:18-34) and kinematic prescriptions
@@ -924,6 +929,7 @@ __device__ inline bool accept_wall(const OuterArgs &a, const SceneDev &S, d3 xs,
 // The scene's grid and law table are read once into registers and the
 // arrays are declared unaliased, so each candidate test costs one gather of
 // the partner's position and safe distance, issued one candidate ahead.
+template <bool HOLE>
 __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1026,7 +1032,7 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
             for (int w = w_begin; w < w_end; ++w)
                 if (accept_wall(a, Sg, xs, safe_s, gs, w)) {
                     Contact c;
-                    cd_wall(xs, p.w, a.walls[w], c);
+                    cd_wall_at<HOLE>(xs, p.w, a.walls[w], a.holes, w, c);
                     emit((unsigned)(a.ld + w) | marks(c.depth));
                 }
         } else {
@@ -1043,6 +1049,8 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
         if (a.ovf) atomicOr(&a.ovf[scene], 1);
     }
 }
+template __global__ void k_outer<false>(OuterArgs);
+template __global__ void k_outer<true>(OuterArgs);
 
 // Consistency of the device lists: every index the step, wall and rebuild
 // kernels form from them lies where it must (checked before it is used):
@@ -1467,10 +1475,11 @@ __global__ void k_scatter_state(Geo g, int b, int e, int ld, const double *rec_i
 // neighbour list is an ordered subsequence of the candidate list, so a merge
 // walk finds each listed pair's entry (flags, history, records); a candidate
 // outside the list has been solved as inactive every step: Clean status.
+template <bool HOLE>
 __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *o_cnt,
                                 const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,
                                 const unsigned *sphere_id, const double *rec, ActiveOut *out, unsigned long long cap,
-                                unsigned long long *counter, int backup, int no_bond)
+                                unsigned long long *counter, int backup, int no_bond, const double4 *holes)
 {
     // spheres b..e-1 (one scene); sphere_id maps the scene's ranks to DO ids
     const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1504,7 +1513,7 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
             cd_ss(xi, p.w, mk(pj.x, pj.y, pj.z), pj.w, c);
             other = sphere_id[g.oid[j]];
         } else {
-            cd_wall(xi, p.w, walls[j - n], c);
+            cd_wall_at<HOLE>(xi, p.w, walls[j - n], holes, (int)(j - n), c);
             other = wall_id[j - n];
         }
         if (!c.active) continue;
@@ -1528,6 +1537,15 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
         for (int r = 0; r < 13; ++r) A.rec[r] = stored ? rec[((size_t)k * 13 + r) * n + i] : 0.0;
     }
 }
+
+template __global__ void k_export_active<false>(Geo, Ell, int, int, int, int, const unsigned *, const unsigned *,
+                                               const WallDev *, const unsigned *, const unsigned *, const double *,
+                                               ActiveOut *, unsigned long long, unsigned long long *, int, int,
+                                               const double4 *);
+template __global__ void k_export_active<true>(Geo, Ell, int, int, int, int, const unsigned *, const unsigned *,
+                                              const WallDev *, const unsigned *, const unsigned *, const double *,
+                                              ActiveOut *, unsigned long long, unsigned long long *, int, int,
+                                              const double4 *);
 
 // Impulse on each wall from the last recorded step: sum of ImpactToMaster
 // and AngularImpactToMaster over its contacts (the DOStatus impact a wall's

@@ -122,7 +122,7 @@ __device__ inline void solve_mixed(const LawDev &L, double ma, double mb, d3 xa,
     const int law = L.law;
     const double depth = c.depth;
     const float df = (float)depth, dtf = (float)dt;
-    if (LS == LS_ALL && law == LAW_HERTZ) {   // ISwHertz (ISwHertz.cpp:20-94)
+    if (LS >= LS_ALL && law == LAW_HERTZ) {   // ISwHertz (ISwHertz.cpp:20-94)
         o.linear_only = true;
         if (!c.active) {
             st_clean(h);

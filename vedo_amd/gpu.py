@@ -13,6 +13,8 @@ import os
 
 import numpy as np
 
+from .world import QUASIPLATE_HOLE
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 # VG_LIBRARY: another build of the same library (A/B timing of build variants)
 LIB_PATH = os.environ.get("VG_LIBRARY") or os.path.join(HERE, "libvedo_hip.so")
@@ -21,7 +23,7 @@ LIB_PATH = os.environ.get("VG_LIBRARY") or os.path.join(HERE, "libvedo_hip.so")
 class VgModel(C.Structure):
     _fields_ = [("shape", C.c_int), ("behavior", C.c_int), ("scope", C.c_int), ("group", C.c_int),
                 ("density", C.c_double), ("density_factor", C.c_double),
-                ("fext", C.c_double * 3), ("attr", C.c_double * 3)]
+                ("fext", C.c_double * 3), ("attr", C.c_double * 6)]
 
 
 class VgIact(C.Structure):
@@ -176,6 +178,8 @@ def scene_desc(wa, inner_skin: float = 0.0):
         for d in range(3):
             m.fext[d] = float(wa.model_fext[k, d])
             m.attr[d] = float(wa.model_attr[k, d])
+            # QuasiPlateWithCircularHole: hole radius, x / y offset
+            m.attr[3 + d] = float(wa.model_hole[k, d]) if int(wa.model_shape[k]) == QUASIPLATE_HOLE else 0.0
     iact = (VgIact * (G * G))()
     for i in range(G):
         for j in range(G):

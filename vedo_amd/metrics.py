@@ -6,8 +6,8 @@ reference's own per-contact output, the ImpactStatus impulse to the master
 (ImpactStatus::SetImpactInformation, ISwBSDBF.cpp:133, ISwH.cpp, ISwHertz.cpp):
 for each wall, the sum of |ImpactToMaster| over its active contacts of the
 last step, divided by the step's dt and the wall's area (quasi-plate:
-width x length of its mid-plane rectangle; quasi-cylinder: its lateral area
-2 pi r h).  The records are those vg_get_active exports (IactRecordTab order
+width x length of its mid-plane rectangle, less the hole of a plate with a
+circular hole; quasi-cylinder: its lateral area 2 pi r h).  The records are those vg_get_active exports (IactRecordTab order
 is irrelevant: they are summed in (lo, hi) key order, sequentially, so that
 equal records give equal bits)."""
 from __future__ import annotations
@@ -27,8 +27,14 @@ def wall_areas(wa, ids=None):
     for k, m in enumerate(model):
         shape = int(wa.model_shape[m])
         a = wa.model_attr[m]
-        if shape in (QUASIPLATE, QUASIPLATE_HOLE):
+        if shape == QUASIPLATE:
             out[k] = a[0] * a[2]
+        elif shape == QUASIPLATE_HOLE:
+            # the hole is cut out where it lies inside the plate (as the
+            # volume of DOModel::Analysis, DOModel.cpp:108-116)
+            r, hx, hy = wa.model_hole[m]
+            inside = abs(hx) <= 0.5 * a[0] - r and abs(hy) <= 0.5 * a[2] - r
+            out[k] = a[0] * a[2] - (math.pi * r * r if inside else 0.0)
         elif shape == QUASICYLINDER:
             out[k] = 2.0 * math.pi * a[0] * a[1]
     return out

@@ -44,7 +44,8 @@ class VoScene(C.Structure):
                 ("dt", C.c_double), ("t_start", C.c_double), ("t_stop", C.c_double),
                 ("t_current", C.c_double), ("g", C.c_double * 3), ("safety_factor", C.c_double),
                 ("up_iact", C.c_uint), ("record", C.c_uint), ("consultant", C.c_int),
-                ("zoi_lo", C.c_double * 3), ("zoi_hi", C.c_double * 3)]
+                ("zoi_lo", C.c_double * 3), ("zoi_hi", C.c_double * 3),
+                ("pbc_lo", C.c_double * 3), ("pbc_hi", C.c_double * 3)]
 
 
 class VoContact(C.Structure):
@@ -154,6 +155,9 @@ class OracleSim:
         zlo, zhi = getattr(wa, "zoi_lo", np.zeros(3)), getattr(wa, "zoi_hi", np.zeros(3))
         for d in range(3):
             sc.zoi_lo[d], sc.zoi_hi[d] = float(zlo[d]), float(zhi[d])
+        plo, phi = getattr(wa, "pbc_lo", np.zeros(3)), getattr(wa, "pbc_hi", np.zeros(3))
+        for d in range(3):
+            sc.pbc_lo[d], sc.pbc_hi[d] = float(plo[d]), float(phi[d])
         err = C.create_string_buffer(256)
         self.h = L.vo_create(C.byref(sc), err, 256)
         if not self.h:

@@ -102,6 +102,7 @@ int main(int argc, char **argv)
     unsigned up = 1, rec = 1;
     int cons = 0;
     Vector3df zoi_lo, zoi_hi;   // equal: no zone of interest
+    Vector3df pbc_lo, pbc_hi;   // equal: no periodic boundary
     std::vector<std::string> groups;
     std::list<DOModel *> models;
     std::vector<std::string> model_names;
@@ -113,6 +114,7 @@ int main(int argc, char **argv)
         else if (tok == "gravity") g = rv(in);
         else if (tok == "safety") sf = rd(in);
         else if (tok == "zoi") { zoi_lo = rv(in); zoi_hi = rv(in); }
+        else if (tok == "pbc") { pbc_lo = rv(in); pbc_hi = rv(in); }
         else if (tok == "cadence") in >> up >> rec >> cons;
         else if (tok == "holes") {
             int n; in >> n;
@@ -186,7 +188,7 @@ int main(int argc, char **argv)
         if (!world->ReadIDO(scene, irt)) return 3;
     } else {
         aries::Constants::Instance()->SetSafetyFactor(sf);
-        Boundary zoi(zoi_lo, zoi_hi), pbc;
+        Boundary zoi(zoi_lo, zoi_hi), pbc(pbc_lo, pbc_hi);
         SystemParameter *sp = new SystemParameter(t0, t1, dt, tc, g, zoi, pbc);
         unsigned ndo = (unsigned)statuses.size();
         sp->SetDONumber(ndo);

@@ -84,6 +84,56 @@ void vo_rotate_around(const double *p, const double *w, double *out)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Periodic boundary (SystemParameter PeriodicBoundaryConditions)           */
+/* ------------------------------------------------------------------------ */
+
+/* Boundary after Correct (Boundary.cpp:79-129): lower and upper point sorted
+ * per axis, range = upper - lower, an axis periodic where they differ. */
+typedef struct {
+    int on[3], active;
+    double lo[3], hi[3], len[3];
+} pbc_t;
+
+/* Boundary::DifferenceBoundaryConditions (Boundary.cpp:303-349): the
+ * minimum image of a difference vector, one half range at a time. */
+static double bc_diff1(double d, double l)
+{
+    if (d >= 0.5 * l) d -= l;
+    if (d < -0.5 * l) d += l;
+    return d;
+}
+static v3 bc_diff(const pbc_t *bc, v3 d)
+{
+    if (!bc) return d;
+    if (bc->on[0]) d.x = bc_diff1(d.x, bc->len[0]);
+    if (bc->on[1]) d.y = bc_diff1(d.y, bc->len[1]);
+    if (bc->on[2]) d.z = bc_diff1(d.z, bc->len[2]);
+    return d;
+}
+
+/* Boundary::EnforceBoundaryConditions (Boundary.cpp:225-301): a position
+ * taken back into [lower, upper) by one range, or by fmod when further out. */
+static double bc_wrap1(double x, double lo, double hi, double l)
+{
+    if (x >= hi) {
+        x -= l;
+        if (x >= hi) x = lo + fmod(x - lo, l);
+    }
+    if (x < lo) {
+        x += l;
+        if (x < lo) x = lo + fmod(x - lo, l) + l;
+    }
+    return x;
+}
+static v3 bc_enforce(const pbc_t *bc, v3 p)
+{
+    if (bc->on[0]) p.x = bc_wrap1(p.x, bc->lo[0], bc->hi[0], bc->len[0]);
+    if (bc->on[1]) p.y = bc_wrap1(p.y, bc->lo[1], bc->hi[1], bc->len[1]);
+    if (bc->on[2]) p.z = bc_wrap1(p.z, bc->lo[2], bc->hi[2], bc->len[2]);
+    return p;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Contact detectors                                                        */
 /* ------------------------------------------------------------------------ */
 
@@ -93,10 +143,13 @@ typedef struct {
     int active;
 } cinfo;
 
-/* CDSphere_Sphere::CalDistance + Detect (CDSphere2Sphere.cpp:14-60) */
-static void cd_ss(v3 xa, double ra, v3 xb, double rb, cinfo *c)
+/* CDSphere_Sphere::CalDistance + Detect (CDSphere2Sphere.cpp:14-60); the
+ * centre-to-centre vector is the minimum image under periodic boundaries
+ * (:19-22, bc non-null once SimMediator::Initiate gave the interaction the
+ * boundary, SimMediator.cpp:662-676). */
+static void cd_ss(v3 xa, double ra, v3 xb, double rb, const pbc_t *bc, cinfo *c)
 {
-    v3 im = sub(xb, xa);
+    v3 im = bc_diff(bc, sub(xb, xa));
     c->depth = ra + rb - len(im);
     c->n = dir(im);
     c->p = add(xa, smul(ra - 0.5 * c->depth, c->n));
@@ -185,7 +238,7 @@ static void to_vo(const cinfo *ci, vo_contact *c)
 
 void vo_cd_sphere_sphere(const double *xa, double ra, const double *xb, double rb, vo_contact *c)
 {
-    cinfo ci; cd_ss(ld(xa), ra, ld(xb), rb, &ci); to_vo(&ci, c);
+    cinfo ci; cd_ss(ld(xa), ra, ld(xb), rb, NULL, &ci); to_vo(&ci, c);
 }
 void vo_cd_sphere_plate(const double *xa, double ra, const double *xb, const double *oxb,
                         const double *ozb, double w, double h, double l, vo_contact *c)
@@ -287,7 +340,7 @@ typedef struct { v3 x, v, w; double m; } body;
 /* ISwBSDBF / ISwBSDB2F / ISwBtSDBF ::NextStep
  * (ISwBSDBF.cpp:26-139, ISwBSDB2F.cpp:26-149, ISwBtSDBF.cpp:26-137) */
 static void solve_sd(int law, const double *p, const body *A, const body *B, const cinfo *ci,
-                     double dt, vo_status *st, solve_out *o)
+                     double dt, const pbc_t *bc, vo_status *st, solve_out *o)
 {
     const double kn = p[0], ks = p[1], bn = p[4], bs = p[5], fc = p[6];
     double cn, cs;
@@ -305,8 +358,9 @@ static void solve_sd(int law, const double *p, const body *A, const body *B, con
     st3(st->point, ci->p); st3(st->dir, ci->n); st->overlap = ci->depth;
     v3 ra = mk(0, 0, 0), rb = mk(0, 0, 0), fn = mk(0, 0, 0), fs = mk(0, 0, 0);
     if (depth > 0.0) {
-        ra = sub(ci->p, A->x);
-        rb = sub(ci->p, B->x);
+        /* lever arms as minimum images under periodic boundaries (ISwBSDBF.cpp:69-73) */
+        ra = bc_diff(bc, sub(ci->p, A->x));
+        rb = bc_diff(bc, sub(ci->p, B->x));
         v3 rv = sub(add(sub(A->v, B->v), cross(A->w, ra)), cross(B->w, rb));
         v3 vn = smul(dot(rv, n), n);
         v3 vs = sub(rv, vn);
@@ -345,7 +399,7 @@ static void solve_sd(int law, const double *p, const body *A, const body *B, con
 
 /* ISwH::NextStep (ISwH.cpp:30-156) */
 static void solve_h(const double *p, const body *A, const body *B, const cinfo *ci, double dt,
-                    vo_status *st, solve_out *o)
+                    const pbc_t *bc, vo_status *st, solve_out *o)
 {
     const double knh = p[0], meff = p[1], ksc1 = p[2], ksc2 = p[3], betan = p[4], fk = p[5];
     const double depth = ci->depth;
@@ -359,8 +413,8 @@ static void solve_h(const double *p, const body *A, const body *B, const cinfo *
     st3(st->point, ci->p); st3(st->dir, ci->n); st->overlap = ci->depth;
     v3 ra = mk(0, 0, 0), rb = mk(0, 0, 0), fn = mk(0, 0, 0), fs = mk(0, 0, 0);
     if (depth > 0.0) {
-        ra = sub(ci->p, A->x);
-        rb = sub(ci->p, B->x);
+        ra = bc_diff(bc, sub(ci->p, A->x));   /* ISwH.cpp:69-73 */
+        rb = bc_diff(bc, sub(ci->p, B->x));
         v3 rv = sub(add(sub(A->v, B->v), cross(A->w, ra)), cross(B->w, rb));
         v3 vn = smul(dot(rv, n), n);
         v3 vs = sub(rv, vn);
@@ -399,7 +453,7 @@ static void solve_h(const double *p, const body *A, const body *B, const cinfo *
 
 /* ISwHertz::NextStep (ISwHertz.cpp:20-94) */
 static void solve_hertz(const double *p, const body *A, const body *B, const cinfo *ci, double dt,
-                        vo_status *st, solve_out *o)
+                        const pbc_t *bc, vo_status *st, solve_out *o)
 {
     const double kn = p[0], cn = p[1];
     const double depth = ci->depth;
@@ -414,8 +468,8 @@ static void solve_hertz(const double *p, const body *A, const body *B, const cin
     st3(st->point, ci->p); st3(st->dir, ci->n); st->overlap = ci->depth;
     v3 fn = mk(0, 0, 0);
     if (depth > 0.0) {
-        v3 ra = sub(ci->p, A->x);
-        v3 rb = sub(ci->p, B->x);
+        v3 ra = bc_diff(bc, sub(ci->p, A->x));   /* ISwHertz.cpp:59-63 */
+        v3 rb = bc_diff(bc, sub(ci->p, B->x));
         v3 rv = sub(add(sub(A->v, B->v), cross(A->w, ra)), cross(B->w, rb));
         v3 vn = smul(dot(rv, n), n);
         if (!st->contact) {
@@ -434,12 +488,12 @@ static void solve_hertz(const double *p, const body *A, const body *B, const cin
 }
 
 static void solve(int law, const double *p, const body *A, const body *B, const cinfo *ci,
-                  double dt, vo_status *st, solve_out *o)
+                  double dt, const pbc_t *bc, vo_status *st, solve_out *o)
 {
     switch (law) {
-    case VO_BSDBF: case VO_BSDB2F: case VO_BTSDBF: solve_sd(law, p, A, B, ci, dt, st, o); break;
-    case VO_H: solve_h(p, A, B, ci, dt, st, o); break;
-    case VO_HERTZ: solve_hertz(p, A, B, ci, dt, st, o); break;
+    case VO_BSDBF: case VO_BSDB2F: case VO_BTSDBF: solve_sd(law, p, A, B, ci, dt, bc, st, o); break;
+    case VO_H: solve_h(p, A, B, ci, dt, bc, st, o); break;
+    case VO_HERTZ: solve_hertz(p, A, B, ci, dt, bc, st, o); break;
     default: memset(o, 0, sizeof(*o)); break;
     }
 }
@@ -452,7 +506,7 @@ int vo_solve(int law, const double *p, double ma, double mb, const vo_contact *c
     body A = {ld(xa), ld(va), ld(wa), ma}, B = {ld(xb), ld(vb), ld(wb), mb};
     cinfo ci; ci.depth = c->depth; ci.n = ld(c->n); ci.p = ld(c->p); ci.active = c->active;
     solve_out o;
-    solve(law, p, &A, &B, &ci, dt, st, &o);
+    solve(law, p, &A, &B, &ci, dt, NULL, st, &o);
     st3(imp_a, o.imp_a); st3(ang_a, o.ang_a); st3(ang_b, o.ang_b);
     return o.added;
 }
@@ -502,11 +556,15 @@ struct vo_sim {
     /* zone of interest (Boundary, Boundary.cpp:79-129) and the scene's DO id of each current DO */
     int zoi_on[3], zoi_active;
     double zoi_lo[3], zoi_hi[3];
+    /* periodic boundary; interactions see it only while it is active */
+    pbc_t pbc;
     unsigned *orig_id;
     /* CleanUp counts: DOs erased, records moved, moved records of an erased DO,
        kept records replaced by a moved one */
     long long cu_erased, cu_moved, cu_from_dead, cu_replaced;
 };
+
+static const pbc_t *sim_bc(const vo_sim *s) { return s->pbc.active ? &s->pbc : NULL; }
 
 static double model_range(const vo_model *m)
 {
@@ -576,13 +634,15 @@ static int push_cand(vo_sim *s, unsigned a, unsigned b)
     return 0;
 }
 
-/* DOMap::CalDistance (DOMap.cpp:164-414), no periodic boundary */
-static double domap_distance(const vo_sim *s, unsigned a, unsigned b)
+/* DOMap::CalDistance (DOMap.cpp:142-414); with a boundary (the Leap
+ * consultant passes the periodic one, LeapConsultant.cpp:199-212; NBS none,
+ * NBSConsultant.cpp:330) every distance vector is its minimum image */
+static double domap_distance(const vo_sim *s, unsigned a, unsigned b, const pbc_t *bc)
 {
     const vo_model *ma = &s->models[s->model_of[a]], *mb = &s->models[s->model_of[b]];
     if (ma->shape > mb->shape) { unsigned t = a; a = b; b = t; const vo_model *tm = ma; ma = mb; mb = tm; }
     if (ma->shape == VO_SPHERE && mb->shape == VO_SPHERE) {
-        return len(sub(s->pos[a], s->pos[b]));
+        return len(bc_diff(bc, sub(s->pos[a], s->pos[b])));
     }
     /* the plate with a hole: the plate's distance, the hole ignored (DOMap.cpp:336-374) */
     if (ma->shape == VO_SPHERE && (mb->shape == VO_QUASIPLATE || mb->shape == VO_QUASIPLATE_HOLE)) {
@@ -595,7 +655,7 @@ static double domap_distance(const vo_sim *s, unsigned a, unsigned b)
         double dy = dot(sub(cap, cb), oy);
         if (dy < -hl) dy = -hl; else if (dy > hl) dy = hl;
         v3 caps = add(add(muls(ox, dx), muls(oy, dy)), cb);
-        return len(sub(caps, ca));
+        return len(bc_diff(bc, sub(caps, ca)));
     }
     if (ma->shape == VO_SPHERE && mb->shape == VO_QUASICYLINDER) {
         double hh = 0.5 * mb->attr[1];
@@ -606,7 +666,7 @@ static double domap_distance(const vo_sim *s, unsigned a, unsigned b)
         if (d < hh && d > -hh) im = sub(cap, ca);
         else if (d >= hh) im = sub(add(cb, muls(ax, hh)), ca);
         else im = sub(sub(cb, muls(ax, hh)), ca);
-        return len(im);
+        return len(bc_diff(bc, im));
     }
     return INFINITY;
 }
@@ -614,24 +674,24 @@ static double domap_distance(const vo_sim *s, unsigned a, unsigned b)
 /* BuildIactTab(v1, v2) and BuildIactTab(v) (LeapConsultant.cpp:198-262,
  * NBSConsultant.cpp:318-374) */
 static int build_pairs(vo_sim *s, const unsigned *v1, int n1, const unsigned *v2, int n2,
-                       const double *safe)
+                       const double *safe, const pbc_t *bc)
 {
     for (int i = 0; i < n1; ++i)
         for (int j = 0; j < n2; ++j) {
             unsigned a = v1[i], b = v2[j];
             if (iact_of(s, a, b)->law == VO_LAW_NONE) continue;
-            if (domap_distance(s, a, b) > safe[a] + safe[b]) continue;
+            if (domap_distance(s, a, b, bc) > safe[a] + safe[b]) continue;
             if (push_cand(s, a, b)) return -1;
         }
     return 0;
 }
-static int build_pairs_self(vo_sim *s, const unsigned *v, int n, const double *safe)
+static int build_pairs_self(vo_sim *s, const unsigned *v, int n, const double *safe, const pbc_t *bc)
 {
     for (int i = 0; i < n; ++i)
         for (int j = i + 1; j < n; ++j) {
             unsigned a = v[i], b = v[j];
             if (iact_of(s, a, b)->law == VO_LAW_NONE) continue;
-            if (domap_distance(s, a, b) > safe[a] + safe[b]) continue;
+            if (domap_distance(s, a, b, bc) > safe[a] + safe[b]) continue;
             if (push_cand(s, a, b)) return -1;
         }
     return 0;
@@ -736,8 +796,15 @@ static void safe_lengths(const vo_sim *s, double *safe)
     }
 }
 
-/* LeapConsultant::Reset + subReset (LeapConsultant.cpp:108-195, LeapConsultant.h:57-514),
- * non-periodic branch. */
+/* LeapConsultant::Reset + subReset (LeapConsultant.cpp:108-195, LeapConsultant.h:57-514).
+ * Periodic boundary: an axis takes part where it is periodic and the spheres
+ * span at least half its range (LeapConsultant.h:146-150); a half-shell cell
+ * that does not exist is replaced, for a cell on a border the offset crosses,
+ * by the cell whose index on every periodic offset axis is the far end
+ * (max for -1, 0 for +1) and on the other offset axes the plain neighbour,
+ * unless that index is the cell's other neighbour on some offset axis
+ * (LeapConsultant.h:199-444).  The far-end index is taken on every periodic
+ * offset axis, also where the cell is not on that axis's border. */
 static int reset_leap(vo_sim *s)
 {
     s->n_cand = 0;
@@ -779,22 +846,52 @@ static int reset_leap(vo_sim *s)
         if (ys >= zs) { perm[0] = 1; perm[1] = 0; perm[2] = 2; }
         else { perm[0] = 2; perm[1] = 0; perm[2] = 1; }
     }
-    (void)xmax; (void)ymax; (void)zmax;
     safe_lengths(s, safe);
     double dtu = s->up_iact * s->dt;
     double zr = s->sf * (2.0 * rmax + dtu * vmax + 0.5 * dtu * dtu * len(s->g));
+    const pbc_t *bc = sim_bc(s);
+    /* periodic axes of the traversal and the last zone index per axis */
+    const double span[3] = {xmax - xmin, ymax - ymin, zmax - zmin};
+    int pax[3], has_pbc = 0;
+    unsigned maxid[3];
+    for (int d = 0; d < 3; ++d) {
+        pax[d] = s->pbc.on[d] && span[d] >= 0.5 * s->pbc.len[d];
+        has_pbc = has_pbc || pax[d];
+        unsigned nc = (unsigned)ceil(span[d] / zr);
+        maxid[d] = (nc > 1u ? nc : 1u) - 1u;
+    }
     if (grid_build(s, &g, loc, nloc, xmin, ymin, zmin, zr, perm)) goto out;
     for (int c = 0; c < g.ncell; ++c) {
         unsigned ix, iy, iz;
         unperm(&g, c, &ix, &iy, &iz);
+        const unsigned id[3] = {ix, iy, iz};
         const unsigned *v1 = g.ids + g.cell_start[c];
         int n1 = g.cell_len[c];
         for (int k = 0; k < 13; ++k) {
             int c2 = find_cell(g.ck, g.ncell, g.perm, ix + NB13[k][0], iy + NB13[k][1], iz + NB13[k][2]);
-            if (c2 >= 0 && build_pairs(s, v1, n1, g.ids + g.cell_start[c2], g.cell_len[c2], safe)) goto out;
+            if (c2 < 0 && has_pbc) {
+                int border = 0, dup = 0;
+                unsigned w[3];
+                for (int d = 0; d < 3; ++d) {
+                    const int o = NB13[k][d];
+                    if (o < 0) {
+                        border = border || id[d] == 0u;
+                        w[d] = pax[d] ? maxid[d] : id[d] - 1u;
+                        dup = dup || w[d] == id[d] + 1u;
+                    } else if (o > 0) {
+                        border = border || id[d] == maxid[d];
+                        w[d] = pax[d] ? 0u : id[d] + 1u;
+                        dup = dup || w[d] == id[d] - 1u;
+                    } else {
+                        w[d] = id[d];
+                    }
+                }
+                if (border && !dup) c2 = find_cell(g.ck, g.ncell, g.perm, w[0], w[1], w[2]);
+            }
+            if (c2 >= 0 && build_pairs(s, v1, n1, g.ids + g.cell_start[c2], g.cell_len[c2], safe, bc)) goto out;
         }
-        if (build_pairs_self(s, v1, n1, safe)) goto out;
-        if (build_pairs(s, v1, n1, glob, nglob, safe)) goto out;
+        if (build_pairs_self(s, v1, n1, safe, bc)) goto out;
+        if (build_pairs(s, v1, n1, glob, nglob, safe, bc)) goto out;
     }
     rc = 0;
 out:
@@ -848,12 +945,12 @@ static int reset_nbs(vo_sim *s)
                 int n1 = g.cell_len[c];
                 for (int k = 0; k < 12; ++k) {
                     int c2 = find_cell(g.ck, g.ncell, g.perm, ix + NB13[k][0], iy + NB13[k][1], iz + NB13[k][2]);
-                    if (c2 >= 0 && build_pairs(s, v1, n1, g.ids + g.cell_start[c2], g.cell_len[c2], safe)) goto out;
+                    if (c2 >= 0 && build_pairs(s, v1, n1, g.ids + g.cell_start[c2], g.cell_len[c2], safe, NULL)) goto out;
                 }
-                if (build_pairs_self(s, v1, n1, safe)) goto out;
-                if (build_pairs(s, v1, n1, glob, nglob, safe)) goto out;
+                if (build_pairs_self(s, v1, n1, safe, NULL)) goto out;
+                if (build_pairs(s, v1, n1, glob, nglob, safe, NULL)) goto out;
                 int c2 = find_cell(g.ck, g.ncell, g.perm, ix + 1, iy, iz - 1);
-                if (c2 >= 0 && build_pairs(s, v1, n1, g.ids + g.cell_start[c2], g.cell_len[c2], safe)) goto out;
+                if (c2 >= 0 && build_pairs(s, v1, n1, g.ids + g.cell_start[c2], g.cell_len[c2], safe, NULL)) goto out;
             }
     rc = 0;
 out:
@@ -867,6 +964,14 @@ static int cmp_rec(const void *pa, const void *pb)
     const record *a = pa, *b = pb;
     if (a->lo != b->lo) return a->lo < b->lo ? -1 : 1;
     return (a->hi > b->hi) - (a->hi < b->hi);
+}
+
+static int cmp_rec_first(const void *pa, const void *pb)
+{
+    const int c = cmp_rec(pa, pb);
+    if (c) return c;
+    const record *a = pa, *b = pb;
+    return (a->order > b->order) - (a->order < b->order);
 }
 
 static const record *find_rec(const vo_sim *s, unsigned lo, unsigned hi)
@@ -885,7 +990,7 @@ static void detect(vo_sim *s, iact *it)
     unsigned a = it->slave, b = it->master;
     const vo_model *ma = &s->models[s->model_of[a]], *mb = &s->models[s->model_of[b]];
     if (mb->shape == VO_SPHERE)
-        cd_ss(s->pos[a], ma->attr[0], s->pos[b], mb->attr[0], &it->ci);
+        cd_ss(s->pos[a], ma->attr[0], s->pos[b], mb->attr[0], sim_bc(s), &it->ci);
     else if (mb->shape == VO_QUASIPLATE)
         cd_sp(s->pos[a], ma->attr[0], s->pos[b], s->ox[b], s->oz[b], mb->attr[0], mb->attr[1], mb->attr[2], &it->ci);
     else if (mb->shape == VO_QUASICYLINDER)
@@ -944,9 +1049,16 @@ static int rebuild_records(vo_sim *s)
         if (s->ia[k].ci.active) {
             record *r = &s->rec[s->n_rec++];
             r->lo = s->ia[k].lo; r->hi = s->ia[k].hi; r->st = s->ia[k].st;
+            r->order = s->n_rec;
         }
-    /* std::map keeps the first insertion of a key; keys are unique here */
-    qsort(s->rec, s->n_rec, sizeof(record), cmp_rec);
+    /* IactRecordTab::PushRecord is std::map::insert (IactRecordTab.cpp:36-39):
+       a key listed twice (the periodic traversal can reach a cell pair twice)
+       keeps its first insertion */
+    qsort(s->rec, s->n_rec, sizeof(record), cmp_rec_first);
+    int u = 0;
+    for (int k = 0; k < s->n_rec; ++k)
+        if (u == 0 || s->rec[k].lo != s->rec[u - 1].lo || s->rec[k].hi != s->rec[u - 1].hi) s->rec[u++] = s->rec[k];
+    s->n_rec = u;
     return 0;
 }
 
@@ -1109,6 +1221,13 @@ vo_sim *vo_create(const vo_scene *sc, char *err, int errlen)
         s->zoi_on[d] = s->zoi_lo[d] != s->zoi_hi[d];
     }
     s->zoi_active = s->zoi_on[0] || s->zoi_on[1] || s->zoi_on[2];
+    for (int d = 0; d < 3; ++d) {
+        s->pbc.lo[d] = fmin(sc->pbc_lo[d], sc->pbc_hi[d]);
+        s->pbc.hi[d] = fmax(sc->pbc_lo[d], sc->pbc_hi[d]);
+        s->pbc.len[d] = s->pbc.hi[d] - s->pbc.lo[d];
+        s->pbc.on[d] = s->pbc.lo[d] != s->pbc.hi[d];
+    }
+    s->pbc.active = s->pbc.on[0] || s->pbc.on[1] || s->pbc.on[2];
     if (s->zoi_active && s->consultant != 0) {
         snprintf(err, errlen, "zone of interest: only the LeapConsultant path is restated");
         vo_destroy(s);
@@ -1201,7 +1320,7 @@ static int step_impl(vo_sim *s, int nas, const unsigned *ids, const double *apos
         body A = {s->pos[a], s->vel[a], s->av[a], s->do_sudo[a]};
         body B = {s->pos[b], s->vel[b], s->av[b], s->do_sudo[b]};
         solve_out o;
-        solve(it->law, it->p, &A, &B, &it->ci, dt, &it->st, &o);
+        solve(it->law, it->p, &A, &B, &it->ci, dt, sim_bc(s), &it->st, &o);
         if (o.added) {
             /* DiscreteObject::AddImpact (DiscreteObject.cpp:44-53) */
             s->imp[a] = add(s->imp[a], o.imp_a);
@@ -1230,6 +1349,10 @@ static int step_impl(vo_sim *s, int nas, const unsigned *ids, const double *apos
          * stored last impulses, which an assigned status carries as zero */
         s->last_imp[i] = mk(0, 0, 0); s->last_aimp[i] = mk(0, 0, 0);
     }
+    /* DOContainer::EnforcePeriodicBoundaryConditions (SimMediator.cpp:746, 889;
+       DiscreteObject.cpp:60-65): every DO's position */
+    if (s->pbc.active)
+        for (int i = 0; i < s->n_do; ++i) s->pos[i] = bc_enforce(&s->pbc, s->pos[i]);
     /* 5. CheckContactStatus */
     for (int k = 0; k < s->n_ia; ++k) detect(s, &s->ia[k]);
     /* 6. Consultant::NextStep (LeapConsultant.cpp:49-97 / NBSConsultant.cpp:35-78) */

@@ -73,6 +73,8 @@ typedef struct {
     int consultant;                /* 0 Leap (LeapConsultant), 1 NBS (NBSConsultant) */
     double zoi_lo[3], zoi_hi[3];   /* zone of interest (SystemParameter ZoneOfInterest); an axis
                                       is bounded where lo != hi, none: inactive */
+    double pbc_lo[3], pbc_hi[3];   /* periodic boundary (SystemParameter PeriodicBoundaryConditions);
+                                      an axis is periodic where lo != hi */
 } vo_scene;
 
 typedef struct vo_sim vo_sim;

@@ -30,13 +30,16 @@ def load_golden(name):
         safety_factor=float(sc[4]), up_iact=int(z["cadence"][0]), record=int(z["cadence"][1]))
     if "zoi" in z.files:
         wa.zoi_lo, wa.zoi_hi = z["zoi"][:3].copy(), z["zoi"][3:].copy()
+    if "pbc" in z.files:
+        wa.pbc_lo, wa.pbc_hi = z["pbc"][:3].copy(), z["pbc"][3:].copy()
     if "model_hole" in z.files:
         wa.model_hole = z["model_hole"].copy()
     return z, wa
 
 
 GOLDEN_CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_bond", "box_bond_bt", "box_cyl",
-                "box_orbital", "box_nbs", "drum_kin", "box_assign", "box_piston", "box_zoi", "box_hole"]
+                "box_orbital", "box_nbs", "drum_kin", "box_assign", "box_piston", "box_zoi", "box_hole",
+                "box_pbc_xy", "box_pbc_xyz", "box_pbc_x"]
 
 
 def step_golden(sim, z, step):

@@ -149,6 +149,19 @@ def make_cases():
     w = scenes.box_scene(64, seed=17, law="ISwBSDBF", dt=2e-5, steps=400, kn=2e3, e=0.6, mu=0.4, **common)
     cases.append(("box_hole", hole_world(w).to_arrays(up_iact=7, record=1000), 0, 200,
                   [0, 1, 2, 6, 7, 8, 50, 100, 200], None))
+    # periodic boundaries (PeriodicBoundaryConditions): lattices that fill
+    # the periodic cell, so the outer layers meet across the boundary and
+    # spheres leave through one face and come back through the other.  Two
+    # periodic axes with a floor and a lid (spring-dashpot), three periodic
+    # axes without walls or gravity (ISwH), one periodic axis between walls
+    # (ISwHertz); the Leap traversal's wrapped half-shell cells differ with
+    # the number of periodic axes
+    cases.append(("box_pbc_xy", scenes.periodic_box((6, 6, 4), "xy", seed=31, law="ISwBSDBF", vmax=0.6)
+                  .to_arrays(up_iact=7, record=1000), 0, 300, [0, 1, 2, 6, 7, 8, 50, 100, 200, 300], None))
+    cases.append(("box_pbc_xyz", scenes.periodic_box((5, 5, 5), "xyz", seed=32, law="ISwH", E=1e6, vmax=0.6)
+                  .to_arrays(up_iact=7, record=1000), 0, 300, [0, 1, 2, 6, 7, 8, 50, 100, 200, 300], None))
+    cases.append(("box_pbc_x", scenes.periodic_box((7, 4, 4), "x", seed=33, law="ISwHertz", E=1e6, vmax=0.6)
+                  .to_arrays(up_iact=7, record=1000), 0, 300, [0, 1, 2, 6, 7, 8, 50, 100, 200, 300], None))
     return cases
 
 
@@ -342,6 +355,7 @@ def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE, metrics
         scalars=np.array([wa.dt, wa.t_start, wa.t_stop, wa.t_current, wa.safety_factor]),
         g=wa.g, cadence=np.array([wa.up_iact, wa.record], np.int64),
         zoi=np.concatenate([np.asarray(wa.zoi_lo, np.float64), np.asarray(wa.zoi_hi, np.float64)]),
+        pbc=np.concatenate([np.asarray(wa.pbc_lo, np.float64), np.asarray(wa.pbc_hi, np.float64)]),
         model_hole=np.asarray(wa.model_hole, np.float64))
     if assigned is not None:
         arrays["assigned_ids"] = np.asarray(assigned[0], np.uint32)
@@ -364,16 +378,18 @@ def run_case(name, wa, consultant, nsteps, dumps, assigned, outdir=HERE, metrics
 IDO_DUMPS = [1, 2, 6, 7, 8, 30, 60]
 
 
-def make_ido_restart(outdir=HERE):
+def make_ido_restart(outdir=HERE, name="ido_restart", w=None):
     """A restart from the reference's own checkpoint: a lattice pressed a little
     together runs `record` steps and RecordIDO writes ref_run_000001.ido
     (Consultant.cpp:213-229); a second harness process reads that file with
     DOWorld::ReadIDO and runs 60 more steps, writing its own first .ido after
     `record` of them.  Saved: both files' bytes, the cadence and the
-    restart's dumps."""
+    restart's dumps.  `w` replaces the lattice (ido_restart_pbc: a periodic
+    box, whose files carry the periodic boundary)."""
     import shutil
-    w = scenes.box_scene(64, seed=18, law="ISwBSDB2F", dt=2e-5, steps=400, omega_max=5.0, spacing=1.99,
-                         jitter=0.01, vmax=0.02, offset=0.0, kn=2e3, e=0.6, mu=0.4)
+    if w is None:
+        w = scenes.box_scene(64, seed=18, law="ISwBSDB2F", dt=2e-5, steps=400, omega_max=5.0, spacing=1.99,
+                             jitter=0.01, vmax=0.02, offset=0.0, kn=2e3, e=0.6, mu=0.4)
     up, rec, nsteps = 7, 10, 60
     wa = w.to_arrays(up_iact=up, record=rec)
     with tempfile.TemporaryDirectory() as td:
@@ -408,8 +424,8 @@ def make_ido_restart(outdir=HERE):
         arrays[f"d{k}_state"] = r["state"]
         arrays[f"d{k}_iact"] = r["iact"]
         arrays[f"d{k}_status"] = r["status"]
-    np.savez_compressed(os.path.join(outdir, "ido_restart.npz"), **arrays)
-    print(f"ido_restart: {len(ido0)} + {len(ido1)} bytes of .ido, active {[int(r['iact']['active'].sum()) for r in d]}")
+    np.savez_compressed(os.path.join(outdir, f"{name}.npz"), **arrays)
+    print(f"{name}: {len(ido0)} + {len(ido1)} bytes of .ido, active {[int(r['iact']['active'].sum()) for r in d]}")
 
 
 def main():
@@ -419,6 +435,9 @@ def main():
     only = set(sys.argv[1:])
     if not only or "ido_restart" in only:
         make_ido_restart()
+    if not only or "ido_restart_pbc" in only:
+        make_ido_restart(name="ido_restart_pbc",
+                         w=scenes.periodic_box((6, 6, 4), "xy", seed=34, law="ISwBSDBF", vmax=0.6))
     for name, wa, cons, nsteps, dumps, assigned in make_cases():
         if only and name not in only:
             continue

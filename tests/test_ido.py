@@ -5,7 +5,9 @@ and writer round-trip both byte for byte; a scene read from the first with
 its IactRecordTab continues in the oracle bit for bit as the reference's own
 restart from that file does (DOWorld::ReadIDO, then SimMediator::Run); and
 the .ido the oracle's restarted scene records after `record` steps equals,
-byte for byte, the one the reference's restarted run wrote."""
+byte for byte, the one the reference's restarted run wrote.  The same for
+a periodic box (fixture ido_restart_pbc), whose files carry the periodic
+boundary."""
 import os
 import struct
 
@@ -17,8 +19,21 @@ from conftest import GOLDEN
 FIELDS = ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")
 
 
-def _z():
-    return np.load(os.path.join(GOLDEN, "ido_restart.npz"))
+RESTARTS = ["ido_restart", "ido_restart_pbc"]
+N_DO = {"ido_restart": 70, "ido_restart_pbc": 146}
+
+
+def _z(name="ido_restart"):
+    return np.load(os.path.join(GOLDEN, f"{name}.npz"))
+
+
+def _scene(name):
+    """The scene each restart fixture started from (make_ido_restart)."""
+    from vedo_amd import scenes
+    if name == "ido_restart_pbc":
+        return scenes.periodic_box((6, 6, 4), "xy", seed=34, law="ISwBSDBF", vmax=0.6)
+    return scenes.box_scene(64, seed=18, law="ISwBSDB2F", dt=2e-5, steps=400, omega_max=5.0, spacing=1.99,
+                            jitter=0.01, vmax=0.02, offset=0.0, kn=2e3, e=0.6, mu=0.4)
 
 
 def _file(tmp_path, z, key):
@@ -27,17 +42,21 @@ def _file(tmp_path, z, key):
     return str(p)
 
 
+@pytest.mark.parametrize("name", RESTARTS)
 @pytest.mark.parametrize("key", ["ido0", "ido1"])
-def test_ido_round_trip_is_byte_identical(tmp_path, key):
+def test_ido_round_trip_is_byte_identical(tmp_path, key, name):
     from vedo_amd.ido import read_ido, write_ido
-    z = _z()
+    z = _z(name)
     d = read_ido(_file(tmp_path, z, key))
     out = tmp_path / "out.ido"
     write_ido(str(out), d)
     assert out.read_bytes() == z[key].tobytes()
     cad = z["cadence"]
     assert d.release == "2016-09-23" and d.udd_dos == 0 and d.udd_is == 0
-    assert len(d.statuses) == 70 and len(d.records) > 0
+    assert len(d.statuses) == N_DO[name] and len(d.records) > 0
+    if name == "ido_restart_pbc":
+        # periodic along x and y (SystemParameter::operator>>, SystemParameter.cpp:87-89)
+        assert bytes(d.pbc_switch) == b"\x01\x01\x00" and d.pbc_hi[2] == 0.0 and d.pbc_hi[0] > 0.0
     # records are in the std::map's key order: (first, second) ascending
     keys = d.records["first"].astype(np.int64) << 32 | d.records["second"]
     assert np.all(np.diff(keys) > 0)
@@ -79,11 +98,11 @@ def test_ido_history_mirrors_the_reader():
     assert st["contact"].sum() > 50
 
 
-def restart_and_compare(make_sim, tmp_path):
+def restart_and_compare(make_sim, tmp_path, name="ido_restart"):
     """Run the restart of `ido0` on make_sim(WorldArrays) and compare with the
     reference's dumps and its first recorded .ido."""
     from vedo_amd.ido import ido_history, ido_world, read_ido, record_ido
-    z = _z()
+    z = _z(name)
     d0 = read_ido(_file(tmp_path, z, "ido0"))
     up, rec = (int(x) for x in z["cadence"])
     wa = ido_world(d0, up, rec)
@@ -119,24 +138,23 @@ def restart_and_compare(make_sim, tmp_path):
     return sim
 
 
-def test_oracle_restart_from_reference_ido(tmp_path):
+@pytest.mark.parametrize("name", RESTARTS)
+def test_oracle_restart_from_reference_ido(tmp_path, name):
     from oracle.oracle import OracleSim
-    restart_and_compare(OracleSim, tmp_path)
+    restart_and_compare(OracleSim, tmp_path, name)
 
 
-def test_record_ido_from_the_scene_matches_reference(tmp_path):
+@pytest.mark.parametrize("name", RESTARTS)
+def test_record_ido_from_the_scene_matches_reference(tmp_path, name):
     """The .ido the oracle records after 10 steps of the fixture's scene
     (rebuilt here with the seeded generator that made it), with models and
     IactModels taken from the scene description, equals byte for byte the
     first file the reference recorded (Consultant::RecordIDO)."""
     from oracle.oracle import OracleSim
-    from vedo_amd import scenes
     from vedo_amd.ido import record_ido
-    z = _z()
+    z = _z(name)
     up, rec = (int(x) for x in z["cadence"])
-    w = scenes.box_scene(64, seed=18, law="ISwBSDB2F", dt=2e-5, steps=400, omega_max=5.0, spacing=1.99,
-                         jitter=0.01, vmax=0.02, offset=0.0, kn=2e3, e=0.6, mu=0.4)
-    wa = w.to_arrays(up_iact=up, record=rec)
+    wa = _scene(name).to_arrays(up_iact=up, record=rec)
     o = OracleSim(wa)
     o.step(rec)
     out = tmp_path / "mine.ido"

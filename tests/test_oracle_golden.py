@@ -101,3 +101,33 @@ def test_bond_fixtures_reach_the_bond_branches(name):
     for o in others:
         zo, _ = load_golden(o)
         assert np.abs(zo[f"d{last}_state"][:, 0] - ref).max() > 1e-6, o
+
+
+@pytest.mark.parametrize("name", ["box_pbc_xy", "box_pbc_xyz", "box_pbc_x"])
+def test_pbc_fixtures_reach_the_periodic_branches(name):
+    """The periodic fixtures hold what the periodic code paths change:
+    candidate pairs and active contacts whose centres lie more than half a
+    period apart (minimum image in DOMap::CalDistance, CDSphere_Sphere and
+    the laws' lever arms), spheres that left through one face and came back
+    through the other between two dumps (EnforceBoundaryConditions), and every
+    sphere inside [lower, upper) on the periodic axes at every dump."""
+    z, wa = load_golden(name)
+    L = np.asarray(wa.pbc_hi) - np.asarray(wa.pbc_lo)
+    on = L != 0
+    sph = wa.model_shape[wa.model_of] == 1
+    cross_cand = cross_act = wrapped = 0
+    prev = None
+    for k in range(len(z["dumps"])):
+        st = z[f"d{k}_state"][:, 0]
+        ri = z[f"d{k}_iact"]
+        ss = sph[ri["slave"]] & sph[ri["master"]]
+        far = np.any(np.abs(st[ri["slave"]] - st[ri["master"]])[:, on] > 0.5 * L[on], axis=1) & ss
+        cross_cand += int(far.sum())
+        cross_act += int((far & ri["active"].astype(bool)).sum())
+        p = st[sph][:, on]
+        assert np.all(p >= np.asarray(wa.pbc_lo)[on]) and np.all(p < np.asarray(wa.pbc_hi)[on])
+        if prev is not None:
+            wrapped += int(np.any(np.abs(p - prev) > 0.5 * L[on], axis=1).sum())
+        prev = p
+    print(f"\n  [{name}] cross-boundary candidates {cross_cand}, contacts {cross_act}, wraps {wrapped}")
+    assert cross_cand > 0 and cross_act > 0 and wrapped > 0

@@ -1,6 +1,7 @@
 """DOWorld XML without Boost (vedo_amd.xmlio) against a file the compiled
 reference wrote: DOWorld::WriteXML after a restart of 10 steps from the
-reference's own .ido (fixture ido_restart, key "xml").  The writer
+reference's own .ido (fixtures ido_restart and, with a periodic boundary,
+ido_restart_pbc, key "xml").  The writer
 reproduces it byte for byte from the oracle's bit-exact state and records;
 the reader reads it back so that writing again gives the same bytes (with
 the orientations as stored: the reader normalises them, as
@@ -18,8 +19,11 @@ import pytest
 from conftest import GOLDEN
 
 
-def _z():
-    return np.load(os.path.join(GOLDEN, "ido_restart.npz"))
+RESTARTS = ["ido_restart", "ido_restart_pbc"]
+
+
+def _z(name="ido_restart"):
+    return np.load(os.path.join(GOLDEN, f"{name}.npz"))
 
 
 def _write(tmp_path, name, data: bytes):
@@ -28,11 +32,12 @@ def _write(tmp_path, name, data: bytes):
     return str(p)
 
 
-def test_xml_writer_matches_reference(tmp_path):
+@pytest.mark.parametrize("name", RESTARTS)
+def test_xml_writer_matches_reference(tmp_path, name):
     from oracle.oracle import OracleSim
     from vedo_amd.ido import ido_history, ido_world, read_ido
     from vedo_amd.xmlio import write_xml
-    z = _z()
+    z = _z(name)
     d0 = read_ido(_write(tmp_path, "start.ido", z["ido0"].tobytes()))
     up, rec = (int(x) for x in z["cadence"])
     wa = ido_world(d0, up, rec)
@@ -52,10 +57,11 @@ def _raw_orientations(text):
     return grab("OrientationX"), grab("OrientationZ")
 
 
-def test_xml_reader_round_trip(tmp_path):
+@pytest.mark.parametrize("name", RESTARTS)
+def test_xml_reader_round_trip(tmp_path, name):
     from vedo_amd.ido import read_ido
     from vedo_amd.xmlio import read_xml, write_xml
-    z = _z()
+    z = _z(name)
     text = z["xml"].tobytes()
     up, rec = (int(x) for x in z["cadence"])
     wa, recs, (imp, aimp) = read_xml(_write(tmp_path, "ref.xml", text), up, rec)

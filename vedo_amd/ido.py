@@ -27,7 +27,7 @@ from typing import List, Tuple
 import numpy as np
 
 from .world import (BEHAVIORS, QUASICYLINDER, QUASIPLATE, QUASIPLATE_HOLE, SCOPES, SHAPES, SPHERE, DOModel, DOStatus,
-                    DOWorld, IactModel, SystemParameter, VedoError, WorldArrays)
+                    DOWorld, IactModel, SystemParameter, VedoError, WorldArrays, record_order)
 
 ATTR_BYTES = 80          # sizeof(DOShapeAttributes)
 RELEASE = "2016-09-23"   # aries::information::_ReleaseDate
@@ -216,9 +216,10 @@ def ido_world(d: Ido, up_iact: int, record: int) -> WorldArrays:
     """The DOWorld of an .ido (DOWorld::ReadIDO, DOWorld_RWIDO.cpp:9-55) as the
     WorldArrays a scene is created from; the SystemParameter's safety factor
     becomes the scene's, as SystemParameter::operator<< sets it
-    (SystemParameter.cpp:116-117)."""
-    if any(d.pbc_switch):
-        raise VedoError("periodic boundary conditions are not on the GPU path")
+    (SystemParameter.cpp:116-117).  The zone of interest and the periodic
+    boundary are taken from their lower and upper points, whose differences
+    decide the switches (Boundary::Correct after reading,
+    SystemParameter.cpp:130-139)."""
     shape_name = {v: k for k, v in SHAPES.items()}
     beh_name = {v: k for k, v in BEHAVIORS.items()}
     scope_name = {v: k for k, v in SCOPES.items()}
@@ -228,7 +229,7 @@ def ido_world(d: Ido, up_iact: int, record: int) -> WorldArrays:
     iacts = [IactModel(i.master, i.slave, i.equation, dict(i.mechanisms)) for i in d.iacts]
     sts = [DOStatus(s.name, *(tuple(v) for v in s.vectors[:5])) for s in d.statuses]
     sp = SystemParameter(d.t_start, d.t_stop, d.dt, d.t_current, tuple(d.g), d.safety, tuple(d.zoi_lo),
-                         tuple(d.zoi_hi))
+                         tuple(d.zoi_hi), tuple(d.pbc_lo), tuple(d.pbc_hi))
     return DOWorld(sp, models, iacts, sts).to_arrays(up_iact, record)
 
 
@@ -290,12 +291,15 @@ def record_ido(path: str, wa: WorldArrays, state, time: float, lo, hi, st, templ
         iacts = [IdoIact(m, s, eq, mech) for m, s, eq, mech in _iact_models(wa)]
         lo3 = np.minimum(wa.zoi_lo, wa.zoi_hi)
         hi3 = np.maximum(wa.zoi_lo, wa.zoi_hi)
+        plo = np.minimum(wa.pbc_lo, wa.pbc_hi)
+        phi = np.maximum(wa.pbc_lo, wa.pbc_hi)
         head = dict(release=RELEASE, max_id=0, zoi_switch=bytes(int(x) for x in (lo3 != hi3)),
-                    zoi_lo=tuple(float(x) for x in lo3), zoi_hi=tuple(float(x) for x in hi3), pbc_switch=bytes(3),
-                    pbc_lo=(0.0, 0.0, 0.0), pbc_hi=(0.0, 0.0, 0.0), t_stop=wa.t_stop)
+                    zoi_lo=tuple(float(x) for x in lo3), zoi_hi=tuple(float(x) for x in hi3),
+                    pbc_switch=bytes(int(x) for x in (plo != phi)), pbc_lo=tuple(float(x) for x in plo),
+                    pbc_hi=tuple(float(x) for x in phi), t_stop=wa.t_stop)
     vectors = np.stack([state[k] for k in ("pos", "vel", "ox", "oz", "av", "impact", "ang_impact")], 1)
     sts = [IdoStatus(names[int(model_of[i])], 0, vectors[i]) for i in range(n)]
-    order = np.lexsort((np.asarray(hi), np.asarray(lo)))
+    order = record_order(lo, hi)
     recs = np.zeros(len(order), RECORD_DTYPE)
     recs["first"], recs["second"] = np.asarray(lo)[order], np.asarray(hi)[order]
     for f in ("contact", "bond", "kn", "init_vel", "shear", "point", "dir", "to_master", "ang_to_master", "overlap"):

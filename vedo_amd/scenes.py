@@ -166,6 +166,77 @@ def box_scene(n: int, seed: int = 1, radius: float = 1e-3, density: float = 2500
     return DOWorld(sp, models, iact, statuses)
 
 
+def periodic_box(counts=(6, 6, 4), axes: str = "xy", seed: int = 31, radius: float = 1e-3,
+                 density: float = 2500.0, law: str = "ISwBSDBF", spacing: float = 2.02, jitter: float = 0.05,
+                 vmax: float = 0.3, omega_max: float = 5.0, dt: float = 2e-5, steps: int = 400,
+                 thick: float = 2e-3, gravity: Optional[float] = None, **law_args) -> DOWorld:
+    """Spheres on a jittered cubic lattice that fills a periodic cell.
+
+    ``counts`` lattice sites per axis at ``spacing*r``; along each axis in
+    ``axes`` the cell is periodic (PeriodicBoundaryConditions from 0 to
+    counts*spacing*r, so the first and last layers are one spacing apart
+    across the boundary); along every other axis two fixed quasi-plates bound
+    the lattice one radius beyond its outer layers, their centres inside the
+    periodic range and their sides 1.5 times the cell.  Gravity along -z
+    (default -9.81 unless z is periodic, then 0)."""
+    rng = SplitMix64(seed)
+    s = spacing * radius
+    cnt = [int(c) for c in counts]
+    per = [a in axes for a in "xyz"]
+    n = cnt[0] * cnt[1] * cnt[2]
+    idx = np.arange(n)
+    ijk = np.stack([idx % cnt[0], (idx // cnt[0]) % cnt[1], idx // (cnt[0] * cnt[1])], 1)
+    pos = (ijk + 0.5) * s + rng.uniform(3 * n, -jitter * radius, jitter * radius).reshape(n, 3)
+    vel = rng.uniform(3 * n, -vmax, vmax).reshape(n, 3)
+    av = rng.uniform(3 * n, -omega_max, omega_max).reshape(n, 3) if omega_max else np.zeros((n, 3))
+    ext = [c * s for c in cnt]
+    sphere = DOModel("Sphere", "Granule", "mobile", "local", density, 1.0, (0, 0, 0), "Sphere", (radius,))
+    mass = sphere.analysis()[1]
+    models = [sphere]
+    statuses = [DOStatus("Sphere", tuple(pos[i]), tuple(vel[i]), (1, 0, 0), (0, 0, 1), tuple(av[i]))
+                for i in range(n)]
+    if law == "ISwH":
+        mech = hertz_mindlin(law_args.get("E", 1e6), law_args.get("nu", 0.3), law_args.get("mu", 0.5),
+                             law_args.get("e", 0.5), radius, radius, mass, mass)
+        wall_mech = hertz_mindlin(law_args.get("E", 1e6), law_args.get("nu", 0.3), law_args.get("mu", 0.5),
+                                  law_args.get("e", 0.5), radius, None, mass, None)
+    elif law in ("ISwBSDBF", "ISwBSDB2F", "ISwBtSDBF"):
+        mech = spring_dashpot(law_args.get("kn", 2e3), 0.8, law_args.get("e", 0.6), law_args.get("mu", 0.4), True)
+        wall_mech = spring_dashpot(law_args.get("kn", 2e3), 0.8, law_args.get("e", 0.6),
+                                   law_args.get("mu", 0.4), False)
+    elif law == "ISwHertz":
+        k_h = hertz_mindlin(law_args.get("E", 1e6), law_args.get("nu", 0.3), 0.5, 0.5, radius, radius, mass,
+                            mass)["HertzNormalStiffness"]
+        mech = {"NormalStiffness": k_h, "NormalDampingCoefficient": 1e-3}
+        wall_mech = {"NormalStiffness": 2.0 * k_h, "NormalDampingCoefficient": 1e-3}
+    else:
+        raise ValueError(law)
+    iact = [IactModel("Granule", "Granule", law, mech)]
+    walled = [d for d in range(3) if not per[d]]
+    if walled:
+        # plate d: normal along axis d; Ox along the next axis
+        size = [1.5 * e + 2.0 * thick for e in ext]
+        for d in walled:
+            u, v = (d + 1) % 3, (d + 2) % 3
+            m = DOModel(f"Wall{'xyz'[d]}", "Wall", "fixed", "global", 0.0, 1.0, (0, 0, 0), "QuasiPlate",
+                        (size[u], thick, size[v]))
+            models.append(m)
+            ox = [0.0, 0.0, 0.0]
+            ox[u] = 1.0
+            for side in (0, 1):
+                c = [0.5 * e for e in ext]
+                c[d] = (-radius - 0.5 * thick) if side == 0 else ext[d] + radius + 0.5 * thick
+                oz = [0.0, 0.0, 0.0]
+                oz[d] = 1.0 if side == 0 else -1.0
+                statuses.append(_plate(m.name, c, ox, oz))
+        iact.append(IactModel("Wall", "Granule", law, wall_mech))
+    g = gravity if gravity is not None else (0.0 if per[2] else -9.81)
+    sp = SystemParameter(0.0, steps * dt, dt, 0.0, (0.0, 0.0, g))
+    sp.pbc_lower = (0.0, 0.0, 0.0)
+    sp.pbc_upper = tuple(ext[d] if per[d] else 0.0 for d in range(3))
+    return DOWorld(sp, models, iact, statuses)
+
+
 def lognormal_classes(k: int, median: float, sigma: float, rmin: float, rmax: float) -> np.ndarray:
     """K log-spaced radius classes spanning the clipped range."""
     return np.exp(np.linspace(math.log(rmin), math.log(rmax), k))

@@ -74,6 +74,10 @@ class SystemParameter:
     # erased at candidate-list rebuilds (Consultant::CleanUp)
     zoi_lower: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     zoi_upper: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    # PeriodicBoundaryConditions (SystemParameter.h:276, Boundary.cpp:79-129):
+    # an axis is periodic where its lower and upper values differ
+    pbc_lower: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    pbc_upper: Tuple[float, float, float] = (0.0, 0.0, 0.0)
 
 
 @dataclass
@@ -179,7 +183,9 @@ class WorldArrays:
     model_names: List[str] = field(default_factory=list)
     zoi_lo: np.ndarray = field(default_factory=lambda: np.zeros(3))
     zoi_hi: np.ndarray = field(default_factory=lambda: np.zeros(3))
-    model_hole: np.ndarray = None   # f64 [M,3]: hole radius, x / y offset (QuasiPlateWithCircularHole)
+    model_hole: np.ndarray = None
+    pbc_lo: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    pbc_hi: np.ndarray = field(default_factory=lambda: np.zeros(3))   # f64 [M,3]: hole radius, x / y offset (QuasiPlateWithCircularHole)
 
     def __post_init__(self):
         if self.model_hole is None:
@@ -269,7 +275,23 @@ class DOWorld:
             safety_factor=sp.safety_factor, up_iact=int(up_iact), record=int(record),
             group_names=groups, model_names=[m.name for m in ms],
             zoi_lo=np.array(sp.zoi_lower, np.float64), zoi_hi=np.array(sp.zoi_upper, np.float64),
+            pbc_lo=np.array(sp.pbc_lower, np.float64), pbc_hi=np.array(sp.pbc_upper, np.float64),
             model_hole=np.array([hole3(m) for m in ms], np.float64).reshape(-1, 3))
+
+
+def record_order(lo, hi) -> np.ndarray:
+    """Indices of the records (lo, hi) in IactRecordTab key order, each key
+    once with its first record: PushRecord is std::map::insert
+    (IactRecordTab.cpp:36-39), and a periodic boundary's traversal can list
+    a pair twice (its two interactions carry equal histories)."""
+    lo, hi = np.asarray(lo), np.asarray(hi)
+    order = np.lexsort((hi, lo))   # stable: equal keys keep their input order
+    if len(order) > 1:
+        klo, khi = lo[order], hi[order]
+        first = np.ones(len(order), bool)
+        first[1:] = (klo[1:] != klo[:-1]) | (khi[1:] != khi[:-1])
+        order = order[first]
+    return order
 
 
 def write_vscene(wa: WorldArrays, path: str, consultant: int = 0):
@@ -283,6 +305,8 @@ def write_vscene(wa: WorldArrays, path: str, consultant: int = 0):
     lines.append("safety %s" % h(wa.safety_factor))
     if np.any(np.asarray(wa.zoi_lo) != np.asarray(wa.zoi_hi)):
         lines.append("zoi %s %s %s %s %s %s" % tuple(h(float(x)) for x in list(wa.zoi_lo) + list(wa.zoi_hi)))
+    if np.any(np.asarray(wa.pbc_lo) != np.asarray(wa.pbc_hi)):
+        lines.append("pbc %s %s %s %s %s %s" % tuple(h(float(x)) for x in list(wa.pbc_lo) + list(wa.pbc_hi)))
     lines.append("cadence %d %d %d" % (wa.up_iact, wa.record, consultant))
     lines.append("groups %d %s" % (len(wa.group_names), " ".join(wa.group_names)))
     # hole radius and offsets of QuasiPlateWithCircularHole models, before the models

@@ -8,7 +8,8 @@ sections, %.6e for positions, velocities, angular velocities and impulses,
 IactStatus), booleans as 0/1.  read_xml follows DOWorld::ReadXML
 (DOWorld_ReadXML.cpp:216-695): the defaults it applies (DensityFactor 1,
 Color bylayer, Scope "undefined" -> local for mobile models and global
-otherwise, no ZOI/PBC, zero impulses), the in-place normalisation of both
+otherwise, no ZOI/PBC, zero impulses; ZOI and PBC bounds default to 0
+per missing attribute), the in-place normalisation of both
 orientations (Vector3d::normalized, Vector3d.h:128-141), the SimConstant
 named "SafetyFactor" clamped to at least 1 (the writer names it
 "ContactDetectSafetyFactor", which the reader does not look for: a file the
@@ -17,8 +18,8 @@ with LastContact = Contact and LastBond = Bond (the constructor at
 ImpactStatus.cpp:86-95).
 
 Only what the GPU path runs is accepted: spheres, quasi-plates (also with a
-circular hole) and quasi-cylinders; other shapes, user-defined values and
-periodic boundaries are rejected with VedoError.
+circular hole) and quasi-cylinders; other shapes and user-defined values are
+rejected with VedoError.
 """
 from __future__ import annotations
 
@@ -29,7 +30,7 @@ from typing import List, Optional, Tuple
 import numpy as np
 
 from .world import (BEHAVIORS, LAWS, QUASICYLINDER, QUASIPLATE, QUASIPLATE_HOLE, SCOPES, SHAPES, SPHERE, DOModel,
-                    DOStatus, DOWorld, IactModel, SystemParameter, VedoError, WorldArrays)
+                    DOStatus, DOWorld, IactModel, SystemParameter, VedoError, WorldArrays, record_order)
 
 VERSION = "X6 build 923"   # aries::information::_Version (Constants.h:44)
 DEFAULT_SAFETY = 1.1       # aries::Constants' initial SafetyFactor (Constants.h:184)
@@ -101,6 +102,16 @@ def write_xml(path: str, wa: WorldArrays, state: Optional[dict] = None, records=
             if on[c]:
                 s += f' {a}Min="{_g(lo[c])}" {a}Max="{_g(hi[c])}"'
         L.append(s + "/>")
+    # the periodic boundary the same way (DOWorld_WriteXML.cpp:76-95)
+    lo = np.minimum(wa.pbc_lo, wa.pbc_hi)
+    hi = np.maximum(wa.pbc_lo, wa.pbc_hi)
+    on = lo != hi
+    if on.any():
+        s = "\t\t<PBC"
+        for c, a in enumerate("XYZ"):
+            if on[c]:
+                s += f' {a}Min="{_g(lo[c])}" {a}Max="{_g(hi[c])}"'
+        L.append(s + "/>")
     L += ["\t</SimParameter>", "\t<DOModelTab>"]
     for k, name in enumerate(wa.model_names):
         shape = int(wa.model_shape[k])
@@ -142,7 +153,7 @@ def write_xml(path: str, wa: WorldArrays, state: Optional[dict] = None, records=
     if records is not None and len(records[0]):
         rlo, rhi, st = (np.asarray(x) for x in records)
         L.append("\t<IactStatusTab>")
-        for q in np.lexsort((rhi, rlo)):
+        for q in record_order(rlo, rhi):
             r = st[q]
             L.append(f'\t\t<IactStatus MasterDOStatusSN="{int(rlo[q])}" SlaveDOStatusSN="{int(rhi[q])}" '
                      f'Contact="{int(bool(r["contact"]))}" Bond="{int(bool(r["bond"]))}"')
@@ -229,9 +240,11 @@ def read_xml(path: str, up_iact: int, record: int):
     if z is not None:
         zlo = tuple(float(z.get(f"{a}Min", 0.0)) for a in "XYZ")
         zhi = tuple(float(z.get(f"{a}Max", 0.0)) for a in "XYZ")
+    plo, phi = (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)
     p = sp_el.find("PBC")
-    if p is not None and any(float(p.get(f"{a}Min", 0.0)) != float(p.get(f"{a}Max", 0.0)) for a in "XYZ"):
-        raise VedoError("periodic boundary conditions are not on the GPU path")
+    if p is not None:   # DOWorld_ReadXML.cpp:287-305
+        plo = tuple(float(p.get(f"{a}Min", 0.0)) for a in "XYZ")
+        phi = tuple(float(p.get(f"{a}Max", 0.0)) for a in "XYZ")
     models: List[DOModel] = []
     for m in _req(root, "DOModelTab"):
         beh = m.get("Behavior")
@@ -257,7 +270,7 @@ def read_xml(path: str, up_iact: int, record: int):
         imp.append([_v3(s, t) if s.find(t) is not None else (0.0, 0.0, 0.0) for t in ("Impact", "AngularImpact")])
     imp = np.array(imp, np.float64).reshape(-1, 2, 3)
     impulses = (imp[:, 0].copy(), imp[:, 1].copy())
-    sp = SystemParameter(*times, g, safety, zlo, zhi)
+    sp = SystemParameter(*times, g, safety, zlo, zhi, plo, phi)
     wa = DOWorld(sp, models, iacts, sts).to_arrays(up_iact, record)
     tab = root.find("IactStatusTab")
     if tab is None:

@@ -104,6 +104,14 @@ typedef struct {
      * at every candidate-list rebuild (Consultant::CleanUp, Consultant.cpp:
      * 264-319) and the remaining DO ids close up. */
     double zoi_lo[3], zoi_hi[3];
+    /* SystemParameter PeriodicBoundaryConditions (SystemParameter.h:276): an
+     * axis is periodic where lower != upper.  Positions are taken back into
+     * [lower, upper) after Response (Boundary::EnforceBoundaryConditions,
+     * SimMediator.cpp:746), contact, lever-arm and candidate distances are
+     * minimum images (Boundary::DifferenceBoundaryConditions).  Only with
+     * vg_create (one scene), fixed walls inside the range and no prescribed
+     * statuses or wall rotation. */
+    double pbc_lo[3], pbc_hi[3];
 } vg_scene_desc;
 
 /* ImpactStatus payload of one contact (ImpactStatus.h:208-222). */

@@ -88,13 +88,14 @@ def lib():
         L.vo_packing_density.restype = C.c_double
         L.vo_packing_density.argtypes = [C.c_void_p, D, C.c_int, C.c_int]
         L.vo_granular_temperature.argtypes = [C.c_void_p, C.c_int, D, D, D, D, D]
-        for f in ("vo_num_interactions", "vo_num_candidates", "vo_num_active", "vo_num_do"):
+        for f in ("vo_num_interactions", "vo_num_candidates", "vo_num_active", "vo_num_do", "vo_num_records"):
             getattr(L, f).argtypes = [C.c_void_p]
         L.vo_get_ids.argtypes = [C.c_void_p, U]
         L.vo_cleanup_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
         L.vo_get_interactions.argtypes = [C.c_void_p, U, U]
         L.vo_get_candidates.argtypes = [C.c_void_p, U, U]
         L.vo_get_active.argtypes = [C.c_void_p, U, U, C.POINTER(VoStatus)]
+        L.vo_get_records.argtypes = [C.c_void_p, U, U]
         L.vo_cd_sphere_sphere.argtypes = [D, C.c_double, D, C.c_double, C.POINTER(VoContact)]
         L.vo_cd_sphere_plate.argtypes = [D, C.c_double, D, D, D, C.c_double, C.c_double, C.c_double,
                                          C.POINTER(VoContact)]
@@ -247,6 +248,14 @@ class OracleSim:
         n = self.L.vo_num_candidates(self.h)
         lo, hi = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
         self.L.vo_get_candidates(self.h, _up(lo), _up(hi))
+        return lo, hi
+
+    def records(self):
+        """Keys (lo, hi) of the IactRecordTab of the last rebuild or record step."""
+        n = self.L.vo_num_records(self.h)
+        lo, hi = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        if n:
+            self.L.vo_get_records(self.h, _up(lo), _up(hi))
         return lo, hi
 
     def active(self):

@@ -1646,6 +1646,17 @@ void vo_get_candidates(const vo_sim *s, unsigned *lo, unsigned *hi)
     for (int k = 0; k < s->n_cand; ++k) { lo[k] = s->cand_lo[k]; hi[k] = s->cand_hi[k]; }
 }
 
+/* The IactRecordTab of the last RebuildIactRecordTab (rebuild or record
+ * step; NBSConsultant.cpp:376-395): its keys in key order. */
+int vo_num_records(const vo_sim *s) { return s->n_rec; }
+void vo_get_records(const vo_sim *s, unsigned *lo, unsigned *hi)
+{
+    for (int k = 0; k < s->n_rec; ++k) {
+        lo[k] = s->rec[k].lo;
+        hi[k] = s->rec[k].hi;
+    }
+}
+
 int vo_num_active(const vo_sim *s)
 {
     int n = 0;

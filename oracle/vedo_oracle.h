@@ -111,6 +111,9 @@ void vo_get_candidates(const vo_sim *s, unsigned *lo, unsigned *hi);
 /* Active contacts (ContactInfo.bActive after the last Detect) with their
  * histories keyed by (lo,hi) -- the IactRecordTab payload. Returns count. */
 int vo_num_active(const vo_sim *s);
+/* The IactRecordTab of the last rebuild or record step: keys in key order. */
+int vo_num_records(const vo_sim *s);
+void vo_get_records(const vo_sim *s, unsigned *lo, unsigned *hi);
 void vo_get_active(const vo_sim *s, unsigned *lo, unsigned *hi, vo_status *st);
 /* Current number of DOs (the zone of interest erases mobile DOs that leave
  * it, Consultant::CleanUp) and the scene's original DO id of each. */

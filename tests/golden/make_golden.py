@@ -152,16 +152,21 @@ def make_cases():
     # periodic boundaries (PeriodicBoundaryConditions): lattices that fill
     # the periodic cell, so the outer layers meet across the boundary and
     # spheres leave through one face and come back through the other.  Two
-    # periodic axes with a floor and a lid (spring-dashpot), three periodic
-    # axes without walls or gravity (ISwH), one periodic axis between walls
-    # (ISwHertz); the Leap traversal's wrapped half-shell cells differ with
-    # the number of periodic axes
+    # periodic axes with a floor and a lid (ISwBSDBF), three periodic axes
+    # without walls or gravity (ISwBtSDBF), one periodic axis between walls
+    # (ISwHertz), two with ISwH; the Leap traversal's wrapped half-shell
+    # cells differ with the number of periodic axes.  The pow-based laws run
+    # 150 and 60 steps (their device pow rounds differently, and a dense
+    # ISwH bed amplifies the difference a hundredfold every 50 steps)
     cases.append(("box_pbc_xy", scenes.periodic_box((6, 6, 4), "xy", seed=31, law="ISwBSDBF", vmax=0.6)
                   .to_arrays(up_iact=7, record=1000), 0, 300, [0, 1, 2, 6, 7, 8, 50, 100, 200, 300], None))
-    cases.append(("box_pbc_xyz", scenes.periodic_box((5, 5, 5), "xyz", seed=32, law="ISwH", E=1e6, vmax=0.6)
+    cases.append(("box_pbc_xyz", scenes.periodic_box((5, 5, 5), "xyz", seed=32, law="ISwBtSDBF", vmax=0.6)
                   .to_arrays(up_iact=7, record=1000), 0, 300, [0, 1, 2, 6, 7, 8, 50, 100, 200, 300], None))
-    cases.append(("box_pbc_x", scenes.periodic_box((7, 4, 4), "x", seed=33, law="ISwHertz", E=1e6, vmax=0.6)
-                  .to_arrays(up_iact=7, record=1000), 0, 300, [0, 1, 2, 6, 7, 8, 50, 100, 200, 300], None))
+    cases.append(("box_pbc_x", scenes.periodic_box((7, 4, 4), "x", seed=33, law="ISwHertz", E=1e6, vmax=1.0)
+                  .to_arrays(up_iact=7, record=1000), 0, 150, [0, 1, 2, 6, 7, 8, 50, 100, 150], None))
+    cases.append(("box_pbc_h", scenes.periodic_box((5, 5, 4), "xy", seed=36, law="ISwH", E=1e6, vmax=1.0,
+                                                    drift=(1.5, 1.0, 0.0))
+                  .to_arrays(up_iact=7, record=1000), 0, 60, [0, 1, 2, 6, 7, 8, 14, 15, 28, 29, 49, 50, 60], None))
     return cases
 
 

@@ -1,7 +1,8 @@
 """Restart on the GPU from a checkpoint the compiled reference wrote
 (DOWorld::ReadIDO, DOWorld_RWIDO.cpp:9-55, read by vedo_amd.ido into
 vg_create + vg_set_history), against the reference's own restart from that
-file (tests/golden/ido_restart.npz) at every dump step:
+file (tests/golden/ido_restart.npz, and ido_restart_pbc.npz: a box periodic
+along x and y) at every dump step:
   * active contact set: bit-exact; flags, Kn, initial velocity, shear force
     of the active records: bit-exact (ISwBSDB2F, a spring-dashpot law);
   * positions, velocities, angular velocities: bit-exact; orientations to
@@ -35,10 +36,11 @@ def _bits(a, b):
     return int((np.asarray(a).view(np.uint64) != np.asarray(b).view(np.uint64)).sum())
 
 
-def test_gpu_restart_from_reference_ido(tmp_path):
+@pytest.mark.parametrize("name", ["ido_restart", "ido_restart_pbc"])
+def test_gpu_restart_from_reference_ido(tmp_path, name):
     from vedo_amd.gpu import GpuSimulation
     from vedo_amd.ido import ido_history, ido_world, read_ido, record_ido
-    z = np.load(os.path.join(GOLDEN, "ido_restart.npz"))
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     p0 = tmp_path / "start.ido"
     p0.write_bytes(z["ido0"].tobytes())
     p1 = tmp_path / "ref1.ido"

@@ -81,7 +81,8 @@ def test_precision_switch_is_validated():
 
 
 @pytest.mark.parametrize("name", ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_bond",
-                                  "box_bond_bt", "box_cyl", "box_orbital", "box_piston", "box_hole", "drum_kin"])
+                                  "box_bond_bt", "box_cyl", "box_orbital", "box_piston", "box_hole", "box_pbc_xy",
+                                  "box_pbc_xyz", "box_pbc_x", "box_pbc_h", "drum_kin"])
 def test_mixed_first_step_matches_reference_dump(name):
     """One mixed step from the fixture's start against the reference's own
     impulses of that step (dump 1) and its active set."""

@@ -53,7 +53,8 @@ def test_gpu_matches_reference(name):
     print(f"\n[{name}] laws {sorted(laws)} N={wa.n_do}", flush=True)
     sim = GpuSimulation(wa)
     zoi = bool(np.any(np.asarray(wa.zoi_lo) != np.asarray(wa.zoi_hi)))
-    if zoi:
+    pbc = bool(np.any(np.asarray(wa.pbc_lo) != np.asarray(wa.pbc_hi)))
+    if zoi or pbc:
         # the zone of interest erases DOs; the oracle's map of original ids
         # names the DOs left (the reference's dumps hold them in closed-up order)
         from oracle.oracle import OracleSim
@@ -62,7 +63,7 @@ def test_gpu_matches_reference(name):
     for k, d in enumerate(z["dumps"]):
         while step < d:
             step_golden(sim, z, step)
-            if zoi:
+            if zoi or pbc:
                 orc.step(1)
             step += 1
         st = sim.state()
@@ -91,7 +92,24 @@ def test_gpu_matches_reference(name):
         gset = sorted(zip(lo.tolist(), hi.tolist()))
         print(f"  step {d}: bitdiff {nbits} rel pos {e_pos:.2e} vel {e_vel:.2e} av {e_av:.2e} ox {e_ox:.2e} "
               f"imp {e_imp:.2e} ang {e_ang:.2e} active {len(gset)}/{len(rset)} same={gset == rset}", flush=True)
-        assert gset == rset, f"step {d}: active contact sets differ"
+        if pbc and d > 0 and d % int(wa.up_iact) == 0:
+            # at a rebuild step vg_get_active returns the IactRecordTab the
+            # rebuild starts from (the old interactions' contacts,
+            # RebuildIactRecordTab); the dump lists the new interactions.  The
+            # two differ where a periodic boundary's traversal
+            # (LeapConsultant.h:199-444) lists a pair in contact in one of the
+            # two candidate lists and not in the other: the record tab is
+            # checked against the oracle's, run alongside
+            olo, ohi = orc.records()
+            assert sorted(set(gset)) == sorted(zip(olo.tolist(), ohi.tolist())), f"step {d}: record tabs differ"
+            if gset != rset:
+                print(f"  step {d}: contacts listed by only one of the lists around the rebuild: "
+                      f"{sorted(set(gset) ^ set(rset))}", flush=True)
+        else:
+            assert gset == rset, f"step {d}: active contact sets differ"
+        # the interactions the candidate list holds (a pair the periodic
+        # traversal reaches twice counts twice, as in the reference)
+        assert sim.num_interactions() == len(ri), (d, sim.num_interactions(), len(ri))
         # constrained walls (DOConstrainedQuasiPlate) move under their contacts' impulses
         cw = (wa.model_shape[wa.model_of[ids]] != 1) & (wa.model_behavior[wa.model_of[ids]] == 1)
         if cw.any():
@@ -110,7 +128,9 @@ def test_gpu_matches_reference(name):
             assert nbits["pos"] == 0 and nbits["vel"] == 0 and nbits["av"] == 0, f"step {d}: not bit-exact"
         else:
             assert e_pos <= 1e-9 and e_vel <= 1e-9 and e_av <= 1e-9
-        assert e_ox <= 1e-12
+        # orientations: device cos/sin alone (bit-exact spins) stay within
+        # 1e-12; with the pow laws' spins within 1e-9, so are they
+        assert e_ox <= (1e-12 if exact else 1e-9)
         assert e_imp <= 1e-5 and e_ang <= 1e-5
     sim.close()
 
@@ -174,3 +194,32 @@ def test_gpu_contact_counter_matches_oracle():
     print(f"\n  contact-steps over 60 steps: gpu {got} oracle {expect}", flush=True)
     assert got == expect
     gpu.close()
+
+
+def test_gpu_periodic_scene_limits():
+    """A periodic scene runs in a handle of its own with fixed walls inside
+    the periodic range: a batch refuses it, and so do prescribed statuses
+    and device wall rotation, with an error instead of a wrong result."""
+    from vedo_amd import scenes
+    from vedo_amd.gpu import GpuBatch, GpuError, GpuSimulation
+    wa = scenes.periodic_box((4, 4, 3), "xy", seed=35).to_arrays(up_iact=7, record=1000)
+    b = GpuBatch(max_spheres=1000, max_walls=16, max_scenes=4, up_iact=7)
+    with pytest.raises(GpuError):
+        b.admit(wa, 10)
+    b.close()
+    sim = GpuSimulation(wa)
+    walls = np.nonzero(wa.model_shape[wa.model_of] != 1)[0]
+    with pytest.raises(GpuError):
+        sim.set_wall_rotation(walls, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0)
+    with pytest.raises(GpuError):
+        sim.step_assigned(walls[:1], wa.pos[walls[:1]], wa.vel[walls[:1]], wa.ox[walls[:1]], wa.oz[walls[:1]],
+                          wa.av[walls[:1]])
+    sim.step(3)
+    sim.close()
+    # a wall outside the periodic range would be moved by the boundary
+    moved = scenes.periodic_box((4, 4, 3), "xy", seed=35).to_arrays(up_iact=7, record=1000)
+    moved.pos = moved.pos.copy()
+    moved.pos[walls[0], 0] = -1e-3
+    with pytest.raises(GpuError):
+        GpuSimulation(moved)
+

@@ -103,7 +103,7 @@ def test_bond_fixtures_reach_the_bond_branches(name):
         assert np.abs(zo[f"d{last}_state"][:, 0] - ref).max() > 1e-6, o
 
 
-@pytest.mark.parametrize("name", ["box_pbc_xy", "box_pbc_xyz", "box_pbc_x"])
+@pytest.mark.parametrize("name", ["box_pbc_xy", "box_pbc_xyz", "box_pbc_x", "box_pbc_h"])
 def test_pbc_fixtures_reach_the_periodic_branches(name):
     """The periodic fixtures hold what the periodic code paths change:
     candidate pairs and active contacts whose centres lie more than half a

@@ -60,6 +60,8 @@ struct SceneHost {
     std::vector<WallDev> walls0;
     std::vector<double4> holes0;        // per wall: hole radius, x / y offset, 1 for a holed plate
     bool has_hole = false;              // a QuasiPlateWithCircularHole among the walls
+    bool has_pbc = false;               // periodic boundary (PeriodicBoundaryConditions)
+    PbcDev pbc0{};
     std::vector<WallKin> kin0;          // rotations prescribed before placement
     std::vector<char> kin0_set;
     // result kept on the host once a finished scene leaves the device
@@ -112,6 +114,15 @@ struct vg_scene {
     WallDev *walls = nullptr, *walls_ref = nullptr;
     WallKin *wkin = nullptr;
     double4 *holes = nullptr;           // [wcap] hole table; allocated when a scene with a holed plate is placed
+    PbcDev *pbc = nullptr;              // periodic boundary of the scene; allocated when a periodic scene is placed
+    PbcDev pbc_h{};                     // its host copy (traversal axes updated at every rebuild)
+    // candidate list under a periodic boundary: occupied cells, pairs, list entries
+    unsigned long long *pbc_ckey[2] = {nullptr, nullptr};
+    unsigned *pbc_cfirst[2] = {nullptr, nullptr};
+    unsigned *pbc_ncell = nullptr, *pbc_pcount = nullptr, *pbc_poff = nullptr, *pbc_first = nullptr;
+    uint2 *pbc_pairs = nullptr;
+    unsigned *pbc_ekey[2] = {nullptr, nullptr}, *pbc_eval[2] = {nullptr, nullptr};
+    size_t pbc_cells_cap = 0, pbc_pairs_cap = 0;
     double *assigned = nullptr;
     double *assigned_sph = nullptr;     // [ld][16] assigned sphere statuses by rank (allocated on first use)
     double *en_part = nullptr, *en_out = nullptr;
@@ -415,6 +426,35 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
         S.zhi[c] = std::max(d->zoi_lo[c], d->zoi_hi[c]);
         if (S.zlo[c] != S.zhi[c]) S.zon |= 1 << c;
     }
+    // the periodic boundary likewise (SystemParameter PeriodicBoundaryConditions)
+    S.pbc0 = PbcDev{};
+    for (int c = 0; c < 3; ++c) {
+        S.pbc0.lo[c] = std::min(d->pbc_lo[c], d->pbc_hi[c]);
+        S.pbc0.hi[c] = std::max(d->pbc_lo[c], d->pbc_hi[c]);
+        S.pbc0.len[c] = S.pbc0.hi[c] - S.pbc0.lo[c];
+        S.pbc0.on[c] = S.pbc0.lo[c] != S.pbc0.hi[c];
+    }
+    S.has_pbc = S.pbc0.on[0] || S.pbc0.on[1] || S.pbc0.on[2];
+    if (S.has_pbc) {
+        if (h->batch) {
+            err = "periodic boundaries: only in a handle of one scene (vg_create)";
+            return VG_EUNSUPPORTED;
+        }
+        // EnforcePeriodicBoundaryConditions moves every DO (SimMediator.cpp:746): the
+        // walls of a periodic scene are fixed and inside the range, where it leaves them
+        for (int w = 0; w < S.nw; ++w) {
+            const unsigned id = S.wall_ids[w];
+            const vg_model &M = d->models[d->model_of[id]];
+            bool inside = M.behavior == VG_FIXED;
+            for (int c = 0; c < 3; ++c)
+                if (S.pbc0.on[c])
+                    inside = inside && d->pos[3 * id + c] >= S.pbc0.lo[c] && d->pos[3 * id + c] < S.pbc0.hi[c];
+            if (!inside) {
+                err = "periodic boundaries: walls must be fixed, with centres inside the periodic range";
+                return VG_EUNSUPPORTED;
+            }
+        }
+    }
     double rmin = 1e300;
     for (unsigned id : S.sphere_ids) rmin = std::min(rmin, d->models[d->model_of[id]].attr[0]);
     S.rmin = rmin;
@@ -649,6 +689,12 @@ static int place_queued(vg_scene *h)
                 HIPCHK(hipMemcpy(h->holes + S.w_begin, S.holes0.data(), sizeof(double4) * S.nw,
                                  hipMemcpyHostToDevice));
             h->nw_used = std::max(h->nw_used, S.w_begin + S.nw);
+        }
+        if (S.has_pbc && !h->pbc) {
+            // the traversal axes are set at every rebuild
+            if (dalloc(h, &h->pbc, 1)) return VG_ENOMEM;
+            h->pbc_h = S.pbc0;
+            HIPCHK(hipMemcpy(h->pbc, &h->pbc_h, sizeof(PbcDev), hipMemcpyHostToDevice));
         }
         const double t2[2] = {S.t, S.t};
         HIPCHK(hipMemcpy(h->stime + 2 * k, t2, sizeof(t2), hipMemcpyHostToDevice));
@@ -918,6 +964,89 @@ static int zoi_prepare(vg_scene *h, std::vector<ZoiPlan> &plans)
     return 0;
 }
 
+// Candidate list of a periodic scene (one scene per handle): the occupied
+// cells sorted into comparator order, their pairs counted and laid down in
+// the reference's order (k_pbc_count / k_pbc_emit), then spread to both
+// spheres' lists by a stable sort on the sphere (k_pbc_entries ..
+// k_pbc_lists).  Fills ocount / onbr and flags[5] as k_outer does.
+static int pbc_candidates(vg_scene *h, const Geo &G, int n)
+{
+    hipStream_t st = h->stream;
+    if ((size_t)n + 1 > h->pbc_cells_cap) {
+        for (auto p : {(void *)h->pbc_ckey[0], (void *)h->pbc_ckey[1], (void *)h->pbc_cfirst[0],
+                       (void *)h->pbc_cfirst[1], (void *)h->pbc_pcount, (void *)h->pbc_poff})
+            if (p) HIPCHK(hipFree(p));
+        h->pbc_cells_cap = (size_t)n + 1 + (size_t)n / 4;
+        const size_t c = h->pbc_cells_cap;
+        if (dalloc(h, &h->pbc_ckey[0], c) || dalloc(h, &h->pbc_ckey[1], c) || dalloc(h, &h->pbc_cfirst[0], c) ||
+            dalloc(h, &h->pbc_cfirst[1], c) || dalloc(h, &h->pbc_pcount, c) || dalloc(h, &h->pbc_poff, c))
+            return VG_ENOMEM;
+    }
+    if (!h->pbc_ncell && dalloc(h, &h->pbc_ncell, 1)) return VG_ENOMEM;
+    if (!h->pbc_first && dalloc(h, &h->pbc_first, (size_t)h->ld)) return VG_ENOMEM;
+    HIPCHK(hipMemsetAsync(h->pbc_ncell, 0, sizeof(unsigned), st));
+    hipLaunchKernelGGL(k_pbc_cells, dim3(nblk(n, 256)), dim3(256), 0, st, G, n, (const int *)h->lin,
+                       (const SceneDev *)h->scenes, h->pbc_ckey[0], h->pbc_cfirst[0], h->pbc_ncell);
+    unsigned nc = 0;
+    HIPCHK(hipMemcpyAsync(&nc, h->pbc_ncell, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    size_t need = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, need, h->pbc_ckey[0], h->pbc_ckey[1], h->pbc_cfirst[0],
+                                    h->pbc_cfirst[1], (size_t)nc, 0, 63, st);
+    if (ensure_tmp(h, need)) return VG_EHIP;
+    HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, h->pbc_ckey[0], h->pbc_ckey[1], h->pbc_cfirst[0],
+                                     h->pbc_cfirst[1], (size_t)nc, 0, 63, st));
+    PbcArgs pa;
+    pa.g = G; pa.n = n; pa.ld = h->ld; pa.scene = h->scenes; pa.cstart = h->cstart; pa.cend = h->cend;
+    pa.safe = h->safe; pa.walls = h->walls; pa.holes = h->holes; pa.pbc = h->pbc; pa.t = tables(h);
+    pa.ckey = h->pbc_ckey[1]; pa.cfirst = h->pbc_cfirst[1]; pa.ncell = (int)nc;
+    pa.pcount = h->pbc_pcount; pa.poff = h->pbc_poff; pa.pairs = nullptr;
+    HIPCHK(hipMemsetAsync(h->pbc_pcount + nc, 0, sizeof(unsigned), st));
+    if (nc) hipLaunchKernelGGL(k_pbc_count, dim3(nblk(nc, 64)), dim3(64), 0, st, pa);
+    need = 0;
+    (void)rocprim::exclusive_scan(nullptr, need, h->pbc_pcount, h->pbc_poff, 0u, (size_t)nc + 1,
+                                  rocprim::plus<unsigned>(), st);
+    if (ensure_tmp(h, need)) return VG_EHIP;
+    HIPCHK(rocprim::exclusive_scan(h->tmp, need, h->pbc_pcount, h->pbc_poff, 0u, (size_t)nc + 1,
+                                   rocprim::plus<unsigned>(), st));
+    unsigned np = 0;
+    HIPCHK(hipMemcpyAsync(&np, h->pbc_poff + nc, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (np >= (1u << 30)) return fail(h, VG_ECAPACITY, "periodic candidate list: too many pairs");
+    if (np > h->pbc_pairs_cap) {
+        for (auto p : {(void *)h->pbc_pairs, (void *)h->pbc_ekey[0], (void *)h->pbc_ekey[1], (void *)h->pbc_eval[0],
+                       (void *)h->pbc_eval[1]})
+            if (p) HIPCHK(hipFree(p));
+        h->pbc_pairs_cap = (size_t)np + (size_t)np / 4 + 64;
+        const size_t c = h->pbc_pairs_cap;
+        if (dalloc(h, &h->pbc_pairs, c) || dalloc(h, &h->pbc_ekey[0], 2 * c) || dalloc(h, &h->pbc_ekey[1], 2 * c) ||
+            dalloc(h, &h->pbc_eval[0], 2 * c) || dalloc(h, &h->pbc_eval[1], 2 * c))
+            return VG_ENOMEM;
+    }
+    pa.pairs = h->pbc_pairs;
+    if (nc) hipLaunchKernelGGL(k_pbc_emit, dim3(nblk(nc, 64)), dim3(64), 0, st, pa);
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, st, reinterpret_cast<int *>(h->ocount),
+                       (long long)n, 0);
+    const unsigned ne = 2 * np;
+    if (np) {
+        hipLaunchKernelGGL(k_pbc_entries, dim3(nblk(np, 256)), dim3(256), 0, st, (const uint2 *)h->pbc_pairs, np,
+                           h->ld, h->pbc_ekey[0], h->pbc_eval[0]);
+        need = 0;
+        (void)rocprim::radix_sort_pairs(nullptr, need, h->pbc_ekey[0], h->pbc_ekey[1], h->pbc_eval[0],
+                                        h->pbc_eval[1], (size_t)ne, 0, 32, st);
+        if (ensure_tmp(h, need)) return VG_EHIP;
+        HIPCHK(rocprim::radix_sort_pairs(h->tmp, need, h->pbc_ekey[0], h->pbc_ekey[1], h->pbc_eval[0],
+                                         h->pbc_eval[1], (size_t)ne, 0, 32, st));
+        hipLaunchKernelGGL(k_pbc_segstart, dim3(nblk(ne, 256)), dim3(256), 0, st, (const unsigned *)h->pbc_ekey[1],
+                           ne, h->pbc_first);
+        hipLaunchKernelGGL(k_pbc_lists, dim3(nblk(ne, 256)), dim3(256), 0, st, (const unsigned *)h->pbc_ekey[1],
+                           (const unsigned *)h->pbc_eval[1], ne, (const uint2 *)h->pbc_pairs,
+                           (const unsigned *)h->pbc_first, h->ld, h->ko, h->ocount, h->onbr, h->flags);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 static int rebuild(vg_scene *h)
 {
     hipStream_t st = h->stream;
@@ -1075,6 +1204,16 @@ static int rebuild(vg_scene *h)
             gd.nx = (int)ncx + 3; gd.ny = (int)ncy + 3; gd.nz = (int)ncz + 3;
             gd.perm[0] = perm[0]; gd.perm[1] = perm[1]; gd.perm[2] = perm[2];
             gd.order = group_order(perm);
+            if (h->pbc && S.has_pbc) {
+                // the axes the Leap traversal wraps (LeapConsultant.h:146-153)
+                const double span[3] = {xmax - xmin, ymax - ymin, zmax - zmin};
+                const unsigned ncl[3] = {ncx, ncy, ncz};
+                for (int c = 0; c < 3; ++c) {
+                    h->pbc_h.pax[c] = h->pbc_h.on[c] && span[c] >= 0.5 * h->pbc_h.len[c];
+                    h->pbc_h.maxid[c] = ncl[c] - 1u;
+                }
+                HIPCHK(hipMemcpyAsync(h->pbc, &h->pbc_h, sizeof(PbcDev), hipMemcpyHostToDevice, st));
+            }
             const long long c = (long long)gd.nx * gd.ny * gd.nz;
             cell_off[k] = cells;
             cells += c;
@@ -1166,8 +1305,13 @@ static int rebuild(vg_scene *h)
         if (h->batch) HIPCHK(hipMemsetAsync(h->ovf, 0, sizeof(int) * h->scap, st));
         for (;;) {
             HIPCHK(hipMemsetAsync(h->flags + 5, 0, sizeof(int), st));
-            if (h->holes) hipLaunchKernelGGL(k_outer<true>, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
-            else hipLaunchKernelGGL(k_outer<false>, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
+            if (h->pbc) {
+                if (int rc = pbc_candidates(h, G1w, n)) return rc;
+            } else if (h->holes) {
+                hipLaunchKernelGGL(k_outer<true>, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
+            } else {
+                hipLaunchKernelGGL(k_outer<false>, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
+            }
             int maxc = 0;
             HIPCHK(hipMemcpyAsync(&maxc, h->flags + 5, sizeof(int), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -1313,7 +1457,10 @@ static void free_scene(vg_scene *h)
         F(h->xs[b]); F(h->ys[b]); F(h->zs[b]);
     }
     F(h->models); F(h->mm); F(h->laws); F(h->scenes); F(h->stime); F(h->rflags); F(h->walls); F(h->walls_ref);
-    F(h->wkin); F(h->holes); F(h->assigned); F(h->assigned_sph); F(h->en_part); F(h->en_out); F(h->flags); F(h->relists);
+    F(h->wkin); F(h->holes); F(h->pbc); F(h->assigned);
+    F(h->pbc_ckey[0]); F(h->pbc_ckey[1]); F(h->pbc_cfirst[0]); F(h->pbc_cfirst[1]); F(h->pbc_ncell);
+    F(h->pbc_pcount); F(h->pbc_poff); F(h->pbc_first); F(h->pbc_pairs); F(h->pbc_ekey[0]); F(h->pbc_ekey[1]);
+    F(h->pbc_eval[0]); F(h->pbc_eval[1]); F(h->assigned_sph); F(h->en_part); F(h->en_out); F(h->flags); F(h->relists);
     F(h->contacts); F(h->red); F(h->inv); F(h->perm); F(h->svals); F(h->keys); F(h->skeys); F(h->qidx); F(h->segs);
     F(h->qval); F(h->qsel_tg); F(h->qsel_hist); F(h->qmap); F(h->safe); F(h->lin); F(h->cstart); F(h->cend); F(h->ocount); F(h->onbr); F(h->tmp); F(h->rec);
     F(h->rec_imp); F(h->rec_imp2); F(h->snap); F(h->snap_wimp); F(h->sid[0]); F(h->sid[1]); F(h->fetch); F(h->ovf);
@@ -1520,6 +1667,7 @@ static StepArgs step_args(vg_scene *h)
     a.step = (int)(h->step_count % 3);
     a.rec_imp = h->rec_imp; a.rec = h->rec;
     a.holes = h->holes;
+    a.pbc = h->pbc;
     a.single_group = h->single_group;
     a.no_bond = h->no_bond;
     a.cw_w = h->cw_w; a.cw_p = h->cw_p; a.cw_val = h->cw_val; a.cw_stamp = h->cw_stamp;
@@ -1537,12 +1685,13 @@ static int export_active(vg_scene *h, int slot, ActiveOut *out, size_t cap, unsi
 {
     const SceneHost &S = h->sc[slot];
     HIPCHK(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), h->stream));
-    hipLaunchKernelGGL(h->holes ? k_export_active<true> : k_export_active<false>, dim3(nblk(S.n, 256)), dim3(256),
+    hipLaunchKernelGGL((h->holes || h->pbc) ? k_export_active<true> : k_export_active<false>, dim3(nblk(S.n, 256)),
+                       dim3(256),
                        0, h->stream, h->geo[h->cur], h->ell[h->ecur], S.seg_begin, S.seg_end, h->ld, h->ki,
                        (const unsigned *)h->ocount, (const unsigned *)h->onbr, (const WallDev *)h->walls,
                        (const unsigned *)d_wid, (const unsigned *)d_sid,
                        h->ent_valid ? (const double *)h->rec : (const double *)nullptr, out, (unsigned long long)cap,
-                       dcnt, backup ? 1 : 0, h->no_bond, (const double4 *)h->holes);
+                       dcnt, backup ? 1 : 0, h->no_bond, (const double4 *)h->holes, (const PbcDev *)h->pbc);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1635,18 +1784,18 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
     if (kt) HIPCHK(hipEventRecord(h->kev[2 * h->kev_used], h->stream));
     const dim3 grid(nblk(std::max(h->n, h->scap), 256)), blk(256);
     // kernel specialisation: law set of the placed scenes, record path, precision
-    const int ls = h->holes ? LS_HOLE : (h->sd_only ? LS_SD : (h->h_only ? LS_H : LS_ALL));
+    const int ls = (h->holes || h->pbc) ? LS_EXT : (h->sd_only ? LS_SD : (h->h_only ? LS_H : LS_ALL));
     const bool mix = h->precision == VG_PRECISION_MIXED;
     using KFn = void (*)(StepArgs);
     static const KFn kfn[2][4][2] = {
         {{k_step<false, LS_SD, false>, k_step<false, LS_SD, true>},
          {k_step<false, LS_H, false>, k_step<false, LS_H, true>},
          {k_step<false, LS_ALL, false>, k_step<false, LS_ALL, true>},
-         {k_step<false, LS_HOLE, false>, k_step<false, LS_HOLE, true>}},
+         {k_step<false, LS_EXT, false>, k_step<false, LS_EXT, true>}},
         {{k_step<true, LS_SD, false>, k_step<true, LS_SD, true>},
          {k_step<true, LS_H, false>, k_step<true, LS_H, true>},
          {k_step<true, LS_ALL, false>, k_step<true, LS_ALL, true>},
-         {k_step<true, LS_HOLE, false>, k_step<true, LS_HOLE, true>}}};
+         {k_step<true, LS_EXT, false>, k_step<true, LS_EXT, true>}}};
     hipLaunchKernelGGL(kfn[rec ? 1 : 0][ls][mix ? 1 : 0], grid, blk, 0, h->stream, a);
     if (kt) {
         HIPCHK(hipEventRecord(h->kev[2 * h->kev_used + 1], h->stream));
@@ -1749,6 +1898,7 @@ extern "C" int vg_step_assigned(vg_scene *h, int n, const unsigned *ids, const d
     if (!h || n < 0 || (n > 0 && (!ids || !pos || !vel || !ox || !oz || !av))) return VG_EINVAL;
     const SceneHost *S0 = scene0(h);
     if (!S0) return fail(h, VG_EINVAL, "vg_step_assigned needs a one-scene handle");
+    if (S0->has_pbc) return fail(h, VG_EUNSUPPORTED, "vg_step_assigned: not with periodic boundaries");
     const SceneHost &S = *S0;
     std::vector<double> buf((size_t)15 * h->wcap, 0.0);
     std::vector<char> set(h->wcap, 0);
@@ -1810,6 +1960,7 @@ static int set_rotation(vg_scene *h, int slot, int n, const unsigned *ids, const
                         double omega, double t_ref)
 {
     const SceneHost &S = h->sc[slot];
+    if (S.has_pbc) return fail(h, VG_EUNSUPPORTED, "wall rotation: not with periodic boundaries (walls stay fixed)");
     HIPCHK(hipStreamSynchronize(h->stream));
     std::vector<WallDev> cur(h->wcap), ref(h->wcap);
     std::vector<WallKin> kin(h->wcap);

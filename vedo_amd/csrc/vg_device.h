@@ -81,6 +81,56 @@ struct Hist {
     double aux;   // Kn (ISwH) or initial velocity (ISwHertz)
 };
 
+// SystemParameter PeriodicBoundaryConditions after Boundary::Correct
+// (Boundary.cpp:79-129): an axis is periodic where lower != upper, len =
+// upper - lower.  pax / maxid: the axes the Leap traversal wraps at the last
+// rebuild (periodic and spanned by the spheres for at least half their range,
+// LeapConsultant.h:146-150) and the last zone index per axis.
+struct PbcDev {
+    int on[3];
+    int pax[3];
+    unsigned maxid[3];
+    int pad;
+    double lo[3], hi[3], len[3];
+};
+
+// Boundary::DifferenceBoundaryConditions (Boundary.cpp:303-349)
+__device__ inline double bc_diff1(double d, double l)
+{
+    if (d >= 0.5 * l) d -= l;
+    if (d < -0.5 * l) d += l;
+    return d;
+}
+__device__ inline d3 bc_diff(const PbcDev *b, d3 d)
+{
+    if (!b) return d;
+    if (b->on[0]) d.x = bc_diff1(d.x, b->len[0]);
+    if (b->on[1]) d.y = bc_diff1(d.y, b->len[1]);
+    if (b->on[2]) d.z = bc_diff1(d.z, b->len[2]);
+    return d;
+}
+// Boundary::EnforceBoundaryConditions (Boundary.cpp:225-301)
+__device__ inline double bc_wrap1(double x, double lo, double hi, double l)
+{
+    if (x >= hi) {
+        x -= l;
+        if (x >= hi) x = lo + fmod(x - lo, l);
+    }
+    if (x < lo) {
+        x += l;
+        if (x < lo) x = lo + fmod(x - lo, l) + l;
+    }
+    return x;
+}
+__device__ inline d3 bc_enforce(const PbcDev *b, d3 x)
+{
+    if (!b) return x;
+    if (b->on[0]) x.x = bc_wrap1(x.x, b->lo[0], b->hi[0], b->len[0]);
+    if (b->on[1]) x.y = bc_wrap1(x.y, b->lo[1], b->hi[1], b->len[1]);
+    if (b->on[2]) x.z = bc_wrap1(x.z, b->lo[2], b->hi[2], b->len[2]);
+    return x;
+}
+
 struct PairOut {
     d3 imp, ang_a, ang_b;
     bool added, linear_only;
@@ -95,6 +145,16 @@ struct RecOut {   // ImpactStatus fields only the record path stores
 __device__ inline void cd_ss(d3 xa, double ra, d3 xb, double rb, Contact &c)
 {
     d3 im = xb - xa;
+    c.depth = ra + rb - len(im);
+    c.n = dir(im);
+    c.p = xa + smul(ra - 0.5 * c.depth, c.n);
+    c.active = c.depth > 0;
+}
+// the same under a periodic boundary: the centre-to-centre vector as its
+// minimum image (CDSphere2Sphere.cpp:19-22)
+__device__ inline void cd_ss_bc(d3 xa, double ra, d3 xb, double rb, const PbcDev *b, Contact &c)
+{
+    d3 im = bc_diff(b, xb - xa);
     c.depth = ra + rb - len(im);
     c.n = dir(im);
     c.p = xa + smul(ra - 0.5 * c.depth, c.n);
@@ -185,13 +245,14 @@ __device__ inline void cd_sph(d3 xa, double ra, const WallDev &w, double4 hole, 
     c.active = c.depth > 0;
 }
 
-// Contact with wall slot ws.  Kernels built for scenes with holed plates
-// (HOLE) look the slot up in the hole table (hole.w = 1 marks a holed plate,
-// which the wall table stores as a plate); the others never read it.
+// Contact with wall slot ws.  Kernels built for the extended geometry (HOLE)
+// look the slot up in the hole table where the handle has one (hole.w = 1
+// marks a holed plate, which the wall table stores as a plate); the others
+// never read it.
 template <bool HOLE>
 __device__ inline void cd_wall_at(d3 xa, double ra, const WallDev &w, const double4 *holes, int ws, Contact &c)
 {
-    if (HOLE) {
+    if (HOLE && holes) {
         const double4 hole = holes[ws];
         if (hole.w != 0.0) {
             cd_sph(xa, ra, w, hole, c);
@@ -275,16 +336,19 @@ __device__ inline d3 shear_rotation(d3 f, d3 d)
 
 // Law sets a step kernel is specialised for: only the spring-dashpot family,
 // only ISwH, or all five (smaller kernels where fewer laws are present);
-// LS_HOLE is all five laws in scenes that hold a plate with a circular hole
-// (the only set whose kernels evaluate that shape).
-enum { LS_SD = 0, LS_H = 1, LS_ALL = 2, LS_HOLE = 3 };
+// LS_EXT is all five laws with the extended geometry: plates with a circular
+// hole and periodic boundaries (the only set whose kernels evaluate them).
+enum { LS_SD = 0, LS_H = 1, LS_ALL = 2, LS_EXT = 3 };
 
 // One ImpactSolver::NextStep for slave A and master B, for the laws of set LS.
 // ISwBSDBF/ISwBSDB2F/ISwBtSDBF (ISwBSDBF.cpp:26-139, ISwBSDB2F.cpp:26-149,
 // ISwBtSDBF.cpp:26-137), ISwH (ISwH.cpp:30-156), ISwHertz (ISwHertz.cpp:20-94).
+// Under a periodic boundary (LS_EXT with pb) the lever arms are minimum
+// images (ISwBSDBF.cpp:69-73, ISwH.cpp:69-73, ISwHertz.cpp:59-63).
 template <bool REC, int LS>
 __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va, d3 wa, d3 xb, d3 vb,
-                             d3 wb, const Contact &c, double dt, Hist &h, PairOut &o, RecOut &r)
+                             d3 wb, const Contact &c, double dt, Hist &h, PairOut &o, RecOut &r,
+                             const PbcDev *pb = nullptr)
 {
     o.added = false;
     o.linear_only = false;
@@ -300,6 +364,10 @@ __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va
         d3 fn = mk(0, 0, 0);
         if (depth > 0.0) {
             d3 ra = c.p - xa, rb = c.p - xb;
+            if constexpr (LS == LS_EXT) {
+                ra = bc_diff(pb, ra);
+                rb = bc_diff(pb, rb);
+            }
             d3 rv = ((va - vb) + cross(wa, ra)) - cross(wb, rb);
             d3 vn = smul(dot(rv, n), n);
             if (!(h.f & F_CONTACT)) {
@@ -329,6 +397,10 @@ __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va
         if (depth > 0.0) {
             ra = c.p - xa;
             rb = c.p - xb;
+            if constexpr (LS == LS_EXT) {
+                ra = bc_diff(pb, ra);
+                rb = bc_diff(pb, rb);
+            }
             d3 rv = ((va - vb) + cross(wa, ra)) - cross(wb, rb);
             d3 vn = smul(dot(rv, n), n);
             d3 vs = rv - vn;
@@ -381,6 +453,10 @@ __device__ inline void solve(const LawDev &L, double ma, double mb, d3 xa, d3 va
     if (depth > 0.0) {
         ra = c.p - xa;
         rb = c.p - xb;
+        if constexpr (LS == LS_EXT) {
+            ra = bc_diff(pb, ra);
+            rb = bc_diff(pb, rb);
+        }
         d3 rv = ((va - vb) + cross(wa, ra)) - cross(wb, rb);
         d3 vn = smul(dot(rv, n), n);
         d3 vs = rv - vn;

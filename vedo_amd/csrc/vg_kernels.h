@@ -93,7 +93,8 @@ struct StepArgs {
     // record path
     double *rec_imp;         // [6][LD] last impulse / angular impulse per sphere
     double *rec;             // [KI][13][LD] point, dir, to_master, ang_to_master, overlap
-    const double4 *holes;    // per wall slot: hole radius, x / y offset, 1 for a holed plate (LS_HOLE kernels)
+    const double4 *holes;    // per wall slot: hole radius, x / y offset, 1 for a holed plate (LS_EXT kernels)
+    const PbcDev *pbc;       // periodic boundary of the (single) scene, or null (LS_EXT kernels)
 };
 
 enum { ERR_INNER_CAP = 1, ERR_OUTER_CAP = 2, ERR_CW = 4 };
@@ -119,6 +120,25 @@ struct OuterArgs {
     int *flags;              // [5]: largest candidate count seen
     int *ovf;                // per scene: a sphere has more candidates than ko (batches; may be null)
     const double4 *holes;    // per wall slot: hole radius, x / y offset, 1 for a holed plate (HOLE kernels)
+};
+
+// candidate list under a periodic boundary (a handle of one scene)
+struct PbcArgs {
+    Geo g;
+    int n, ld;
+    const SceneDev *scene;
+    const int *cstart, *cend;
+    const double *safe;
+    const WallDev *walls;
+    const double4 *holes;
+    const PbcDev *pbc;
+    Tables t;
+    const unsigned long long *ckey;   // occupied cells' comparator keys, sorted
+    const unsigned *cfirst;           // first sphere of each sorted cell
+    int ncell;
+    unsigned *pcount;                 // pairs per cell
+    const unsigned *poff;             // first pair of each cell
+    uint2 *pairs;                     // (sphere, partner | marks) in reference order
 };
 
 struct InnerArgs {
@@ -172,6 +192,14 @@ __global__ void k_reorder(Geo src, Geo dst, int n, int ld, const unsigned *perm,
 __global__ void k_fill_int(int *p, long long n, int v);
 __global__ void k_cellstart(const int *lin, int n, int *cstart, int *cend);
 template <bool HOLE> __global__ void k_outer(OuterArgs a);
+__global__ void k_pbc_cells(Geo g, int n, const int *lin, const SceneDev *scene, unsigned long long *ckey,
+                            unsigned *cfirst, unsigned *ncell);
+__global__ void k_pbc_count(PbcArgs a);
+__global__ void k_pbc_emit(PbcArgs a);
+__global__ void k_pbc_entries(const uint2 *pairs, unsigned np, int ld, unsigned *key, unsigned *val);
+__global__ void k_pbc_segstart(const unsigned *key, unsigned ne, unsigned *first);
+__global__ void k_pbc_lists(const unsigned *key, const unsigned *val, unsigned ne, const uint2 *pairs,
+                            const unsigned *first, int ld, int ko, unsigned *count, unsigned *nbr, int *flags);
 __global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int b, int e, int ld, int ko,
                              unsigned long long *out);
 __global__ void k_inner_build(InnerArgs a);
@@ -193,11 +221,12 @@ __global__ void k_restore_history(Geo g, Ell ell, int b, int e, int ld, int ki, 
                                   long long nrec, const ModelDev *models, const SceneDev *scenes, const LawDev *laws,
                                   const WallDev *walls);
 __global__ void k_scatter_state(Geo g, int b, int e, int ld, const double *rec_imp, double *out);
-template <bool HOLE>
+template <bool EXT>
 __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *o_cnt,
                                 const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,
                                 const unsigned *sphere_id, const double *rec, ActiveOut *out, unsigned long long cap,
-                                unsigned long long *counter, int backup, int no_bond, const double4 *holes);
+                                unsigned long long *counter, int backup, int no_bond, const double4 *holes,
+                                const PbcDev *pbc);
 __global__ void k_check_lists(Geo g, Ell ell, int n, int ld, int ki, int ko, const unsigned *o_cnt,
                               const unsigned *o_nbr, const ModelDev *models, int n_models, const SceneDev *scenes,
                               int scap, const WallDev *walls, int wcap, const int *cw_w, const int *cw_p,

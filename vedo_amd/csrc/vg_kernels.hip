@@ -86,7 +86,7 @@ __device__ inline void store_hist(const Ell &e, int h, int k, int ki, int n, int
 // Rebuild sphere i's neighbour list from its candidate list: keep partners
 // within `skin` of contact, carry flags and history of partners already
 // listed (both lists are ordered subsequences of the candidate list).
-template <bool HOLE>
+template <bool EXT>
 __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_old, int c_old, int h_new,
                              double skin)
 {
@@ -99,13 +99,15 @@ __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_o
         bool near = true;
         if (j < (unsigned)a.ld) {
             const double4 pj = a.in.pos[j];
-            const d3 dx = ld3(pj) - xi;
+            d3 dx = ld3(pj) - xi;
+            if constexpr (EXT) dx = bc_diff(a.pbc, dx);
             const double lim = ri + pj.w + skin;
             // far beyond the skin: excluded without the exact geometry
             if (dot(dx, dx) > lim * lim * (1.0 + 1e-9)) near = false;
+            else if constexpr (EXT) cd_ss_bc(xi, ri, ld3(pj), pj.w, a.pbc, c);
             else cd_ss(xi, ri, ld3(pj), pj.w, c);
         } else {
-            cd_wall_at<HOLE>(xi, ri, a.walls[j - a.ld], a.holes, (int)(j - a.ld), c);
+            cd_wall_at<EXT>(xi, ri, a.walls[j - a.ld], a.holes, (int)(j - a.ld), c);
         }
         const bool carried = (p < c_old) && (old_partner == j);
         unsigned f = 0;
@@ -218,17 +220,27 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
         wb = owner_a ? wjj : wi;
         mb = owner_a ? wj.w : mi;
         const double ra = owner_a ? ri : pj.w, rb = owner_a ? pj.w : ri;
-        if (MIX) cd_ss_m(xa, ra, xb, rb, cm);
-        else cd_ss(xa, ra, xb, rb, c);
+        if constexpr (LS == LS_EXT) {
+            if (MIX) cd_ss_m_bc(xa, ra, xb, rb, a.pbc, cm);
+            else cd_ss_bc(xa, ra, xb, rb, a.pbc, c);
+        } else {
+            if (MIX) cd_ss_m(xa, ra, xb, rb, cm);
+            else cd_ss(xa, ra, xb, rb, c);
+        }
     } else {
         w = &a.walls[j - a.ld];
         law = &a.t.laws[S.law_off + gi * S.n_groups + w->group];
-        cd_wall_at<LS == LS_HOLE>(xi, ri, *w, a.holes, (int)(j - a.ld), c);
+        cd_wall_at<LS == LS_EXT>(xi, ri, *w, a.holes, (int)(j - a.ld), c);
         xa = xi; va = vi; wa = wi; ma = mi;
         xb = w->pos; vb = w->vel; wb = w->av; mb = w->msudo;
-        if (MIX) to_mixed(c, xi, w->pos, cm);
+        if constexpr (LS == LS_EXT) {
+            if (MIX) to_mixed(c, xi, w->pos, cm, a.pbc);
+        } else {
+            if (MIX) to_mixed(c, xi, w->pos, cm);
+        }
     }
     if (MIX) solve_mixed<REC, LS>(*law, ma, mb, xa, va, wa, vb, wb, cm, S.dt, hs, o, r);
+    else if constexpr (LS == LS_EXT) solve<REC, LS>(*law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r, a.pbc);
     else solve<REC, LS>(*law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r);
     if (o.added) {
         if (owner_a) {
@@ -353,7 +365,7 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
         cnt = hdr & 127;
         if (do_relist) {
             const int hn = h ^ 1;
-            cnt = relist<LS == LS_HOLE>(a, i, xi, ri, h, cnt, hn, S.skin);
+            cnt = relist<LS == LS_EXT>(a, i, xi, ri, h, cnt, hn, S.skin);
             h = hn;
             a.ell.hdr[i] = (unsigned char)((h << 7) | cnt);
             relisted = true;
@@ -386,7 +398,8 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
             const unsigned j = ent[u] & PARTNER_MASK;
             bool full = true;
             if (pj[u].w >= 0.0) {   // a sphere partner, unbonded laws
-                const d3 dx = ld3(pj[u]) - xi;
+                d3 dx = ld3(pj[u]) - xi;
+                if constexpr (LS == LS_EXT) dx = bc_diff(a.pbc, dx);
                 const double rs = ri + pj[u].w;
                 if (dot(dx, dx) > rs * rs * (1.0 + 1e-9)) {
                     full = false;
@@ -471,7 +484,9 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
         const d3 dav = smul(2.5 / mi / ri / ri, L);
         const d3 dp = smul(S.dt, vi + smul(0.5, dv));
         const d3 dw = smul(S.dt, wi + smul(0.5, dav));
-        const d3 xn = xi + dp;
+        d3 xn = xi + dp;
+        // DOContainer::EnforcePeriodicBoundaryConditions (SimMediator.cpp:746)
+        if constexpr (LS == LS_EXT) xn = bc_enforce(a.pbc, xn);
         const double *oi = a.in.ori + i;
         const size_t LD = (size_t)a.ld;
         d3 oxn = mk(nt_load(oi), nt_load(oi + LD), nt_load(oi + 2 * LD));
@@ -508,10 +523,10 @@ template __global__ void k_step<false, LS_H, true>(StepArgs);
 template __global__ void k_step<true, LS_H, true>(StepArgs);
 template __global__ void k_step<false, LS_ALL, true>(StepArgs);
 template __global__ void k_step<true, LS_ALL, true>(StepArgs);
-template __global__ void k_step<false, LS_HOLE, false>(StepArgs);
-template __global__ void k_step<true, LS_HOLE, false>(StepArgs);
-template __global__ void k_step<false, LS_HOLE, true>(StepArgs);
-template __global__ void k_step<true, LS_HOLE, true>(StepArgs);
+template __global__ void k_step<false, LS_EXT, false>(StepArgs);
+template __global__ void k_step<true, LS_EXT, false>(StepArgs);
+template __global__ void k_step<false, LS_EXT, true>(StepArgs);
+template __global__ void k_step<true, LS_EXT, true>(StepArgs);
 
 // Wall responses after the spheres' (DOFixedQuasiPlate.h:16-19,
 // DOOrbitalQuasiPlate.cpp:18-34) and kinematic prescriptions
@@ -1052,6 +1067,217 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
 template __global__ void k_outer<false>(OuterArgs);
 template __global__ void k_outer<true>(OuterArgs);
 
+// ---------------------------------------------------------------------------
+// Candidate list under a periodic boundary
+// ---------------------------------------------------------------------------
+// The Leap traversal with wrapped half-shell cells (LeapConsultant.h:146-444)
+// visits cell pairs that no fixed per-sphere order describes (a cell on a
+// border can be paired with many cells of the opposite side), so the list is
+// built as the reference builds it: the occupied cells in comparator order,
+// each emitting its pairs in reference order (13 lookups, its own pairs, the
+// global DOs; cell i-major, partner j-minor), then every pair goes to both
+// spheres' lists by a stable sort on the sphere.  A pair the traversal
+// reaches twice is listed twice, as the reference's IactPairTab lists it.
+
+// DOMap::CalDistance's distance vector for sphere-plate / sphere-cylinder
+// (DOMap.cpp:320-414), before the boundary's minimum image and the length.
+__device__ inline d3 domap_wall_vec(d3 ca, const WallDev &w)
+{
+    if (w.shape == SH_PLATE) {
+        d3 oy = cross(w.oz, w.ox);
+        d3 cap = ca - project_on(ca - w.pos, w.oz);
+        double hw = 0.5 * w.a0, hl = 0.5 * w.a2;
+        double dx = dot(cap - w.pos, w.ox);
+        if (dx < -hw) dx = -hw;
+        else if (dx > hw) dx = hw;
+        double dy = dot(cap - w.pos, oy);
+        if (dy < -hl) dy = -hl;
+        else if (dy > hl) dy = hl;
+        d3 caps = (muls(w.ox, dx) + muls(oy, dy)) + w.pos;
+        return caps - ca;
+    }
+    double hh = 0.5 * w.a1;
+    double d = dot(ca - w.pos, w.oz);
+    d3 cap = w.pos + muls(w.oz, d);
+    if (d < hh && d > -hh) return cap - ca;
+    if (d >= hh) return (w.pos + muls(w.oz, hh)) - ca;
+    return (w.pos - muls(w.oz, hh)) - ca;
+}
+
+// cell coordinates of the comparator key (each coordinate + 1, 21 bits)
+__device__ inline void pbc_key_cell(const GridDev &G, unsigned long long key, unsigned c[3])
+{
+    c[G.perm[0]] = (unsigned)(key >> 42) - 1u;
+    c[G.perm[1]] = (unsigned)((key >> 21) & 0x1fffffull) - 1u;
+    c[G.perm[2]] = (unsigned)(key & 0x1fffffull) - 1u;
+}
+
+// occupied cells: the first sphere of each cell run of the sorted layout
+__global__ void k_pbc_cells(Geo g, int n, const int *lin, const SceneDev *scene, unsigned long long *ckey,
+                            unsigned *cfirst, unsigned *ncell)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    if (s > 0 && lin[s] == lin[s - 1]) return;
+    const GridDev &G = scene->G;
+    const double4 p = g.pos[s];
+    unsigned c[3];
+    cell_of(G, mk(p.x, p.y, p.z), c[0], c[1], c[2]);
+    const unsigned q = atomicAdd(ncell, 1u);
+    ckey[q] = ((unsigned long long)(c[G.perm[0]] + 1u) << 42) | ((unsigned long long)(c[G.perm[1]] + 1u) << 21) |
+              (unsigned long long)(c[G.perm[2]] + 1u);
+    cfirst[q] = (unsigned)s;
+}
+
+// Pairs of sorted cell r in reference order; counts them, or writes them
+// to out (EMIT): (sphere i, partner j | marks), j = ld + w for a wall slot.
+template <bool EMIT>
+__device__ unsigned pbc_cell_pairs(const PbcArgs &a, int r, uint2 *out)
+{
+    const SceneDev &S = *a.scene;
+    const GridDev &G = S.G;
+    const PbcDev *pb = a.pbc;
+    const double4 *__restrict__ pos = a.g.pos;
+    const double *__restrict__ safe = a.safe;
+    const int *__restrict__ mdl = a.g.mdl;
+    const ModelDev *__restrict__ models = a.t.models;
+    const LawDev *__restrict__ laws = a.t.laws;
+    const int *cstart = a.cstart + S.cell_off, *cend = a.cend + S.cell_off;
+    unsigned c[3];
+    pbc_key_cell(G, a.ckey[r], c);
+    const int b = (int)a.cfirst[r];
+    const int e = cend[lin_of(G, c[0], c[1], c[2])];
+    const double skin = S.skin;
+    unsigned cnt = 0;
+    auto cell_range = [&](long long x, long long y, long long z, int &tb, int &te) -> bool {
+        if (x < -1 || y < -1 || z < -1 || x > G.nx - 3 || y > G.ny - 3 || z > G.nz - 3) return false;
+        const long long l = lin_of(G, x, y, z);
+        tb = cstart[l];
+        te = cend[l];
+        return te > tb;
+    };
+    auto law_of = [&](int gi, int gj) { return laws[S.law_off + gi * S.n_groups + gj].law; };
+    // BuildIactTab's test for spheres i, j (LeapConsultant.cpp:197-262)
+    auto pair_ss = [&](int i, int j) {
+        if (i == j) return;   // Assembler::CreateInteraction never pairs a DO with itself
+        if (law_of(models[mdl[i]].group, models[mdl[j]].group) == LAW_NONE) return;
+        const double4 pi = pos[i], pj = pos[j];
+        const d3 xi = mk(pi.x, pi.y, pi.z), xj = mk(pj.x, pj.y, pj.z);
+        if (len(bc_diff(pb, xi - xj)) > safe[i] + safe[j]) return;
+        if (EMIT) {
+            const double depth = pi.w + pj.w - len(bc_diff(pb, xj - xi));
+            const unsigned m = (depth >= -skin ? NBR_NEAR : 0u) | (depth > 0.0 ? NBR_ACTIVE : 0u);
+            out[cnt] = make_uint2((unsigned)i, (unsigned)j | m);
+        }
+        ++cnt;
+    };
+    for (int k = 0; k < 13; ++k) {
+        int tb = 0, te = 0;
+        bool found = cell_range((long long)c[0] + NB13[k][0], (long long)c[1] + NB13[k][1],
+                                (long long)c[2] + NB13[k][2], tb, te);
+        if (!found && (pb->pax[0] || pb->pax[1] || pb->pax[2])) {
+            // the wrapped half-shell cell (LeapConsultant.h:199-444)
+            bool border = false, dup = false;
+            unsigned w[3];
+            for (int d = 0; d < 3; ++d) {
+                const int o = NB13[k][d];
+                if (o < 0) {
+                    border = border || c[d] == 0u;
+                    w[d] = pb->pax[d] ? pb->maxid[d] : c[d] - 1u;
+                    dup = dup || w[d] == c[d] + 1u;
+                } else if (o > 0) {
+                    border = border || c[d] == pb->maxid[d];
+                    w[d] = pb->pax[d] ? 0u : c[d] + 1u;
+                    dup = dup || w[d] == c[d] - 1u;
+                } else {
+                    w[d] = c[d];
+                }
+            }
+            if (border && !dup) found = cell_range(w[0], w[1], w[2], tb, te);
+        }
+        if (found)
+            for (int i = b; i < e; ++i)
+                for (int j = tb; j < te; ++j) pair_ss(i, j);
+    }
+    for (int i = b; i < e; ++i)
+        for (int j = i + 1; j < e; ++j) pair_ss(i, j);
+    for (int i = b; i < e; ++i) {
+        const int gi = models[mdl[i]].group;
+        const double4 pi = pos[i];
+        const d3 xi = mk(pi.x, pi.y, pi.z);
+        for (int w = S.w_begin; w < S.w_end; ++w) {
+            const WallDev &W = a.walls[w];
+            if (law_of(gi, W.group) == LAW_NONE) continue;
+            const double sw = G.sf * (2.0 * W.range + G.safe_v * len(W.vel) + G.safe_g);
+            if (len(bc_diff(pb, domap_wall_vec(xi, W))) > safe[i] + sw) continue;
+            if (EMIT) {
+                Contact cw;
+                cd_wall_at<true>(xi, pi.w, W, a.holes, w, cw);
+                const unsigned m = (cw.depth >= -skin ? NBR_NEAR : 0u) | (cw.depth > 0.0 ? NBR_ACTIVE : 0u);
+                out[cnt] = make_uint2((unsigned)i, (unsigned)(a.ld + w) | m);
+            }
+            ++cnt;
+        }
+    }
+    return cnt;
+}
+
+__global__ void k_pbc_count(PbcArgs a)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.ncell) return;
+    a.pcount[r] = pbc_cell_pairs<false>(a, r, nullptr);
+}
+
+__global__ void k_pbc_emit(PbcArgs a)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.ncell) return;
+    pbc_cell_pairs<true>(a, r, a.pairs + a.poff[r]);
+}
+
+// two list entries per sphere pair (one for a wall pair), keyed by the
+// sphere whose list takes them; the value is the pair and its side
+__global__ void k_pbc_entries(const uint2 *pairs, unsigned np, int ld, unsigned *key, unsigned *val)
+{
+    const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const uint2 q = pairs[p];
+    const unsigned j = q.y & PARTNER_MASK;
+    key[2 * p] = q.x;
+    val[2 * p] = 2 * p;
+    key[2 * p + 1] = j < (unsigned)ld ? j : 0xffffffffu;
+    val[2 * p + 1] = 2 * p + 1;
+}
+
+__global__ void k_pbc_segstart(const unsigned *key, unsigned ne, unsigned *first)
+{
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const unsigned s = key[e];
+    if (s == 0xffffffffu) return;
+    if (e == 0 || key[e - 1] != s) first[s] = e;
+}
+
+// each sphere's entries, in pair order, into its candidate list
+__global__ void k_pbc_lists(const unsigned *key, const unsigned *val, unsigned ne, const uint2 *pairs,
+                            const unsigned *first, int ld, int ko, unsigned *count, unsigned *nbr, int *flags)
+{
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const unsigned s = key[e];
+    if (s == 0xffffffffu) return;
+    const unsigned t = e - first[s];
+    const unsigned v = val[e];
+    const uint2 q = pairs[v >> 1];
+    const unsigned ent = (v & 1u) ? (q.x | (q.y & ~PARTNER_MASK)) : q.y;
+    if (t < (unsigned)ko) nbr[(size_t)t * ld + s] = ent;
+    if (e + 1 == ne || key[e + 1] != s) {
+        count[s] = t + 1;
+        if ((int)(t + 1) > ko) atomicMax(&flags[5], (int)(t + 1));
+    }
+}
+
 // Consistency of the device lists: every index the step, wall and rebuild
 // kernels form from them lies where it must (checked before it is used):
 // each sphere's model and scene, its neighbour-list count and entries
@@ -1475,11 +1701,12 @@ __global__ void k_scatter_state(Geo g, int b, int e, int ld, const double *rec_i
 // neighbour list is an ordered subsequence of the candidate list, so a merge
 // walk finds each listed pair's entry (flags, history, records); a candidate
 // outside the list has been solved as inactive every step: Clean status.
-template <bool HOLE>
+template <bool EXT>
 __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, const unsigned *o_cnt,
                                 const unsigned *o_nbr, const WallDev *walls, const unsigned *wall_id,
                                 const unsigned *sphere_id, const double *rec, ActiveOut *out, unsigned long long cap,
-                                unsigned long long *counter, int backup, int no_bond, const double4 *holes)
+                                unsigned long long *counter, int backup, int no_bond, const double4 *holes,
+                                const PbcDev *pbc)
 {
     // spheres b..e-1 (one scene); sphere_id maps the scene's ranks to DO ids
     const int i = b + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1510,10 +1737,11 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
         unsigned other;
         if (j < (unsigned)n) {
             const double4 pj = g.pos[j];
-            cd_ss(xi, p.w, mk(pj.x, pj.y, pj.z), pj.w, c);
+            if constexpr (EXT) cd_ss_bc(xi, p.w, mk(pj.x, pj.y, pj.z), pj.w, pbc, c);
+            else cd_ss(xi, p.w, mk(pj.x, pj.y, pj.z), pj.w, c);
             other = sphere_id[g.oid[j]];
         } else {
-            cd_wall_at<HOLE>(xi, p.w, walls[j - n], holes, (int)(j - n), c);
+            cd_wall_at<EXT>(xi, p.w, walls[j - n], holes, (int)(j - n), c);
             other = wall_id[j - n];
         }
         if (!c.active) continue;
@@ -1541,11 +1769,11 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
 template __global__ void k_export_active<false>(Geo, Ell, int, int, int, int, const unsigned *, const unsigned *,
                                                const WallDev *, const unsigned *, const unsigned *, const double *,
                                                ActiveOut *, unsigned long long, unsigned long long *, int, int,
-                                               const double4 *);
+                                               const double4 *, const PbcDev *);
 template __global__ void k_export_active<true>(Geo, Ell, int, int, int, int, const unsigned *, const unsigned *,
                                               const WallDev *, const unsigned *, const unsigned *, const double *,
                                               ActiveOut *, unsigned long long, unsigned long long *, int, int,
-                                              const double4 *);
+                                              const double4 *, const PbcDev *);
 
 // Impulse on each wall from the last recorded step: sum of ImpactToMaster
 // and AngularImpactToMaster over its contacts (the DOStatus impact a wall's

@@ -72,14 +72,29 @@ __device__ inline void cd_ss_m(d3 xa, double ra, d3 xb, double rb, ContactM &c)
     c.rb64 = c.ra64 - im;
 }
 
-// A contact detected in FP64 (sphere-wall) in the mixed form.
-__device__ inline void to_mixed(const Contact &c, d3 xa, d3 xb, ContactM &m)
+// The same under a periodic boundary: the centre-to-centre vector as its
+// minimum image (CDSphere2Sphere.cpp:19-22), so are both lever arms.
+__device__ inline void cd_ss_m_bc(d3 xa, double ra, d3 xb, double rb, const PbcDev *b, ContactM &c)
+{
+    const d3 im = bc_diff(b, xb - xa);
+    const double l = len(im);
+    c.depth = ra + rb - l;
+    c.active = c.depth > 0;
+    const double il = l != 0.0 ? 1.0 / l : 0.0;
+    c.n64 = mk(im.x * il, im.y * il, im.z * il);
+    c.ra64 = smul(ra - 0.5 * c.depth, c.n64);
+    c.rb64 = c.ra64 - im;
+}
+
+// A contact detected in FP64 (sphere-wall) in the mixed form; under a
+// periodic boundary the lever arms are minimum images (ISwBSDBF.cpp:69-73).
+__device__ inline void to_mixed(const Contact &c, d3 xa, d3 xb, ContactM &m, const PbcDev *b = nullptr)
 {
     m.depth = c.depth;
     m.active = c.active;
     m.n64 = c.n;
-    m.ra64 = c.p - xa;
-    m.rb64 = c.p - xb;
+    m.ra64 = bc_diff(b, c.p - xa);
+    m.rb64 = bc_diff(b, c.p - xb);
 }
 
 // Relative velocity at the contact point, v = vA - vB + wA x rA - wB x rB,

@@ -50,7 +50,8 @@ class VgSceneDesc(C.Structure):
                 ("dt", C.c_double), ("t_start", C.c_double), ("t_stop", C.c_double),
                 ("t_current", C.c_double), ("g", C.c_double * 3), ("safety_factor", C.c_double),
                 ("up_iact", C.c_uint), ("record", C.c_uint), ("consultant", C.c_int),
-                ("inner_skin", C.c_double), ("zoi_lo", C.c_double * 3), ("zoi_hi", C.c_double * 3)]
+                ("inner_skin", C.c_double), ("zoi_lo", C.c_double * 3), ("zoi_hi", C.c_double * 3),
+                ("pbc_lo", C.c_double * 3), ("pbc_hi", C.c_double * 3)]
 
 
 class VgBatchDesc(C.Structure):
@@ -203,6 +204,9 @@ def scene_desc(wa, inner_skin: float = 0.0):
     zlo, zhi = getattr(wa, "zoi_lo", np.zeros(3)), getattr(wa, "zoi_hi", np.zeros(3))
     for k in range(3):
         d.zoi_lo[k], d.zoi_hi[k] = float(zlo[k]), float(zhi[k])
+    plo, phi = getattr(wa, "pbc_lo", np.zeros(3)), getattr(wa, "pbc_hi", np.zeros(3))
+    for k in range(3):
+        d.pbc_lo[k], d.pbc_hi[k] = float(plo[k]), float(phi[k])
     return d, (models, iact, arrs, model_of)
 
 

@@ -169,7 +169,8 @@ def box_scene(n: int, seed: int = 1, radius: float = 1e-3, density: float = 2500
 def periodic_box(counts=(6, 6, 4), axes: str = "xy", seed: int = 31, radius: float = 1e-3,
                  density: float = 2500.0, law: str = "ISwBSDBF", spacing: float = 2.02, jitter: float = 0.05,
                  vmax: float = 0.3, omega_max: float = 5.0, dt: float = 2e-5, steps: int = 400,
-                 thick: float = 2e-3, gravity: Optional[float] = None, **law_args) -> DOWorld:
+                 thick: float = 2e-3, gravity: Optional[float] = None, drift=(0.0, 0.0, 0.0),
+                 **law_args) -> DOWorld:
     """Spheres on a jittered cubic lattice that fills a periodic cell.
 
     ``counts`` lattice sites per axis at ``spacing*r``; along each axis in
@@ -178,7 +179,8 @@ def periodic_box(counts=(6, 6, 4), axes: str = "xy", seed: int = 31, radius: flo
     across the boundary); along every other axis two fixed quasi-plates bound
     the lattice one radius beyond its outer layers, their centres inside the
     periodic range and their sides 1.5 times the cell.  Gravity along -z
-    (default -9.81 unless z is periodic, then 0)."""
+    (default -9.81 unless z is periodic, then 0); ``drift`` is added to every
+    sphere's velocity (the bed streams through the periodic faces)."""
     rng = SplitMix64(seed)
     s = spacing * radius
     cnt = [int(c) for c in counts]
@@ -187,7 +189,7 @@ def periodic_box(counts=(6, 6, 4), axes: str = "xy", seed: int = 31, radius: flo
     idx = np.arange(n)
     ijk = np.stack([idx % cnt[0], (idx // cnt[0]) % cnt[1], idx // (cnt[0] * cnt[1])], 1)
     pos = (ijk + 0.5) * s + rng.uniform(3 * n, -jitter * radius, jitter * radius).reshape(n, 3)
-    vel = rng.uniform(3 * n, -vmax, vmax).reshape(n, 3)
+    vel = rng.uniform(3 * n, -vmax, vmax).reshape(n, 3) + np.asarray(drift, np.float64)
     av = rng.uniform(3 * n, -omega_max, omega_max).reshape(n, 3) if omega_max else np.zeros((n, 3))
     ext = [c * s for c in cnt]
     sphere = DOModel("Sphere", "Granule", "mobile", "local", density, 1.0, (0, 0, 0), "Sphere", (radius,))

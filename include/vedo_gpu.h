@@ -96,7 +96,8 @@ typedef struct {
     double safety_factor;      /* aries::Constants SafetyFactor (Constants.h:184) */
     unsigned up_iact;          /* candidate-list rebuild period (anne's UpIact) */
     unsigned record;           /* record period (anne's Record) */
-    int consultant;            /* 0: LeapConsultant rules (the only one on the GPU) */
+    int consultant;            /* candidate list: 0 LeapConsultant's, 1 NBSConsultant's
+                                  (NBSConsultant.cpp:85-374; one-scene handles, no zone of interest) */
     double inner_skin;         /* <= 0: automatic */
     /* SystemParameter ZoneOfInterest (SystemParameter.h:275): an axis is
      * bounded where lower != upper (Boundary::Correct, Boundary.cpp:79-129);

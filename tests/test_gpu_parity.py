@@ -24,7 +24,7 @@ from conftest import GOLDEN_CASES, load_golden, step_golden
 
 pytestmark = pytest.mark.gpu
 
-GPU_CASES = [c for c in GOLDEN_CASES if c != "box_nbs"]
+GPU_CASES = GOLDEN_CASES
 EXACT_LAWS = {1, 2, 3}
 
 
@@ -51,7 +51,8 @@ def test_gpu_matches_reference(name):
     turning = any(np.any(wa.av[q] != 0) for q in orbital)
     exact = laws <= EXACT_LAWS and not turning
     print(f"\n[{name}] laws {sorted(laws)} N={wa.n_do}", flush=True)
-    sim = GpuSimulation(wa)
+    cons = int(z["consultant"])
+    sim = GpuSimulation(wa, consultant=cons)
     zoi = bool(np.any(np.asarray(wa.zoi_lo) != np.asarray(wa.zoi_hi)))
     pbc = bool(np.any(np.asarray(wa.pbc_lo) != np.asarray(wa.pbc_hi)))
     if zoi or pbc:

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -62,6 +63,8 @@ struct SceneHost {
     bool has_hole = false;              // a QuasiPlateWithCircularHole among the walls
     bool has_pbc = false;               // periodic boundary (PeriodicBoundaryConditions)
     PbcDev pbc0{};
+    bool nbs = false;                   // NBSConsultant's candidate list (else LeapConsultant's)
+    double nbs_range = 0.0;             // the range NBS takes for the zone size: its last local DO's
     std::vector<WallKin> kin0;          // rotations prescribed before placement
     std::vector<char> kin0_set;
     // result kept on the host once a finished scene leaves the device
@@ -115,6 +118,7 @@ struct vg_scene {
     WallKin *wkin = nullptr;
     double4 *holes = nullptr;           // [wcap] hole table; allocated when a scene with a holed plate is placed
     PbcDev *pbc = nullptr;              // periodic boundary of the scene; allocated when a periodic scene is placed
+    bool nbs = false;                   // the scene takes NBSConsultant's candidate list
     PbcDev pbc_h{};                     // its host copy (traversal axes updated at every rebuild)
     // candidate list under a periodic boundary: occupied cells, pairs, list entries
     unsigned long long *pbc_ckey[2] = {nullptr, nullptr};
@@ -321,8 +325,12 @@ static void update_batch_flags(vg_scene *h)
 
 static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, std::string &err)
 {
-    if (d->consultant != 0) {
-        err = "only LeapConsultant candidate-list rules run on the GPU";
+    if (d->consultant != 0 && d->consultant != 1) {
+        err = "consultant: 0 (LeapConsultant) or 1 (NBSConsultant)";
+        return VG_EINVAL;
+    }
+    if (d->consultant == 1 && h->batch) {
+        err = "NBSConsultant's candidate list: only in a handle of one scene (vg_create)";
         return VG_EUNSUPPORTED;
     }
     if (d->n_do <= 0 || d->n_models <= 0 || d->n_groups <= 0 || !d->models || !d->iact || !d->model_of ||
@@ -331,6 +339,7 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
         return VG_EINVAL;
     }
     S = SceneHost();
+    S.nbs = d->consultant == 1;
     S.n_do = d->n_do;
     S.n_groups = d->n_groups;
     S.n_models = d->n_models;
@@ -435,6 +444,18 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
         S.pbc0.on[c] = S.pbc0.lo[c] != S.pbc0.hi[c];
     }
     S.has_pbc = S.pbc0.on[0] || S.pbc0.on[1] || S.pbc0.on[2];
+    if (S.nbs && S.zon) {
+        // NBSConsultant never erases DOs here: the zone of interest is a
+        // LeapConsultant path (its NextStep's CleanUp, LeapConsultant.cpp:58-68)
+        err = "a zone of interest with NBSConsultant's candidate list is not supported";
+        return VG_EUNSUPPORTED;
+    }
+    if (S.nbs && !S.sphere_ids.empty()) {
+        // NBSConsultant::Reset keeps the range of the last local DO it meets
+        // as the zone size's radius (NBSConsultant.cpp:136-141)
+        const vg_model &M = d->models[d->model_of[S.sphere_ids.back()]];
+        S.nbs_range = model_range(M);
+    }
     if (S.has_pbc) {
         if (h->batch) {
             err = "periodic boundaries: only in a handle of one scene (vg_create)";
@@ -690,6 +711,7 @@ static int place_queued(vg_scene *h)
                                  hipMemcpyHostToDevice));
             h->nw_used = std::max(h->nw_used, S.w_begin + S.nw);
         }
+        if (S.nbs) h->nbs = true;
         if (S.has_pbc && !h->pbc) {
             // the traversal axes are set at every rebuild
             if (dalloc(h, &h->pbc, 1)) return VG_ENOMEM;
@@ -969,7 +991,7 @@ static int zoi_prepare(vg_scene *h, std::vector<ZoiPlan> &plans)
 // the reference's order (k_pbc_count / k_pbc_emit), then spread to both
 // spheres' lists by a stable sort on the sphere (k_pbc_entries ..
 // k_pbc_lists).  Fills ocount / onbr and flags[5] as k_outer does.
-static int pbc_candidates(vg_scene *h, const Geo &G, int n)
+static int pbc_candidates(vg_scene *h, const Geo &G, int n, const std::vector<GridDev> &grids)
 {
     hipStream_t st = h->stream;
     if ((size_t)n + 1 > h->pbc_cells_cap) {
@@ -1001,6 +1023,8 @@ static int pbc_candidates(vg_scene *h, const Geo &G, int n)
     pa.safe = h->safe; pa.walls = h->walls; pa.holes = h->holes; pa.pbc = h->pbc; pa.t = tables(h);
     pa.ckey = h->pbc_ckey[1]; pa.cfirst = h->pbc_cfirst[1]; pa.ncell = (int)nc;
     pa.pcount = h->pbc_pcount; pa.poff = h->pbc_poff; pa.pairs = nullptr;
+    pa.nbs = h->nbs ? 1 : 0;
+    pa.nc[0] = (unsigned)(grids[0].nx - 3); pa.nc[1] = (unsigned)(grids[0].ny - 3); pa.nc[2] = (unsigned)(grids[0].nz - 3);
     HIPCHK(hipMemsetAsync(h->pbc_pcount + nc, 0, sizeof(unsigned), st));
     if (nc) hipLaunchKernelGGL(k_pbc_count, dim3(nblk(nc, 64)), dim3(64), 0, st, pa);
     need = 0;
@@ -1164,15 +1188,25 @@ static int rebuild(vg_scene *h)
             const int k = live[q];
             const SceneHost &S = h->sc[k];
             const unsigned long long *rr = &red[(size_t)8 * k];
-            const double xmax = from_ord(rr[0]), ymax = from_ord(rr[1]), zmax = from_ord(rr[2]);
+            double xmax = from_ord(rr[0]), ymax = from_ord(rr[1]), zmax = from_ord(rr[2]);
             const double xmin = from_ord(~rr[3]), ymin = from_ord(~rr[4]), zmin = from_ord(~rr[5]);
-            const double vmax = from_ord(rr[6]), rmax = from_ord(rr[7]);
+            const double vmax = from_ord(rr[6]);
+            double rmax = from_ord(rr[7]);
+            if (S.nbs) {
+                // NBSConsultant::Reset: maxima start from numeric_limits<double>::min() and the
+                // radius is the last local DO's range (NBSConsultant.cpp:106-141)
+                xmax = std::max(xmax, DBL_MIN); ymax = std::max(ymax, DBL_MIN); zmax = std::max(zmax, DBL_MIN);
+                rmax = S.nbs_range;
+            }
             const double *qq = &q6[(size_t)6 * q];
             const double xspan = qq[1] - qq[0], yspan = qq[3] - qq[2], zspan = qq[5] - qq[4];
             GridDev &gd = grids[k];
             // TrirLeX (x,y,z), TrirLeY (y,x,z), TrirLeZ (z,x,y) (NBSConsultant.h:52-118)
             int perm[3];
-            if (xspan >= yspan) {
+            if (S.nbs) {
+                // NBS visits its zones by index, x outermost (NBSConsultant.cpp:197-273)
+                perm[0] = 0; perm[1] = 1; perm[2] = 2;
+            } else if (xspan >= yspan) {
                 if (xspan >= zspan) { perm[0] = 0; perm[1] = 1; perm[2] = 2; }
                 else { perm[0] = 2; perm[1] = 0; perm[2] = 1; }
             } else {
@@ -1305,8 +1339,8 @@ static int rebuild(vg_scene *h)
         if (h->batch) HIPCHK(hipMemsetAsync(h->ovf, 0, sizeof(int) * h->scap, st));
         for (;;) {
             HIPCHK(hipMemsetAsync(h->flags + 5, 0, sizeof(int), st));
-            if (h->pbc) {
-                if (int rc = pbc_candidates(h, G1w, n)) return rc;
+            if (h->pbc || h->nbs) {
+                if (int rc = pbc_candidates(h, G1w, n, grids)) return rc;
             } else if (h->holes) {
                 hipLaunchKernelGGL(k_outer<true>, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
             } else {

@@ -139,6 +139,8 @@ struct PbcArgs {
     unsigned *pcount;                 // pairs per cell
     const unsigned *poff;             // first pair of each cell
     uint2 *pairs;                     // (sphere, partner | marks) in reference order
+    int nbs;                          // NBSConsultant's traversal instead of Leap's
+    unsigned nc[3];                   // NBS: zones per axis (the traversal's ranges)
 };
 
 struct InnerArgs {

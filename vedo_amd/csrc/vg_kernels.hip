@@ -1068,7 +1068,7 @@ template __global__ void k_outer<false>(OuterArgs);
 template __global__ void k_outer<true>(OuterArgs);
 
 // ---------------------------------------------------------------------------
-// Candidate list under a periodic boundary
+// Candidate list by cell pairs: periodic boundaries, NBS
 // ---------------------------------------------------------------------------
 // The Leap traversal with wrapped half-shell cells (LeapConsultant.h:146-444)
 // visits cell pairs that no fixed per-sphere order describes (a cell on a
@@ -1078,6 +1078,12 @@ template __global__ void k_outer<true>(OuterArgs);
 // global DOs; cell i-major, partner j-minor), then every pair goes to both
 // spheres' lists by a stable sort on the sphere.  A pair the traversal
 // reaches twice is listed twice, as the reference's IactPairTab lists it.
+// The NBSConsultant's traversal (NBSConsultant.cpp:85-316) is built the same
+// way: zones in (x, y, z) order within the zone counts (a sphere on the
+// upper bound of an axis sits one zone beyond and is reached only as a
+// neighbour), 12 lookups, its own pairs, the global DOs, then the 13th
+// lookup (+1, 0, -1); its distances take no periodic image
+// (NBSConsultant.cpp:330).
 
 // DOMap::CalDistance's distance vector for sphere-plate / sphere-cylinder
 // (DOMap.cpp:320-414), before the boundary's minimum image and the length.
@@ -1136,7 +1142,8 @@ __device__ unsigned pbc_cell_pairs(const PbcArgs &a, int r, uint2 *out)
 {
     const SceneDev &S = *a.scene;
     const GridDev &G = S.G;
-    const PbcDev *pb = a.pbc;
+    const PbcDev *pb = a.pbc;                     // contact depths (the marks)
+    const PbcDev *pbd = a.nbs ? nullptr : a.pbc;  // candidate distances and wrapped lookups
     const double4 *__restrict__ pos = a.g.pos;
     const double *__restrict__ safe = a.safe;
     const int *__restrict__ mdl = a.g.mdl;
@@ -1145,6 +1152,7 @@ __device__ unsigned pbc_cell_pairs(const PbcArgs &a, int r, uint2 *out)
     const int *cstart = a.cstart + S.cell_off, *cend = a.cend + S.cell_off;
     unsigned c[3];
     pbc_key_cell(G, a.ckey[r], c);
+    if (a.nbs && (c[0] >= a.nc[0] || c[1] >= a.nc[1] || c[2] >= a.nc[2])) return 0u;
     const int b = (int)a.cfirst[r];
     const int e = cend[lin_of(G, c[0], c[1], c[2])];
     const double skin = S.skin;
@@ -1163,7 +1171,7 @@ __device__ unsigned pbc_cell_pairs(const PbcArgs &a, int r, uint2 *out)
         if (law_of(models[mdl[i]].group, models[mdl[j]].group) == LAW_NONE) return;
         const double4 pi = pos[i], pj = pos[j];
         const d3 xi = mk(pi.x, pi.y, pi.z), xj = mk(pj.x, pj.y, pj.z);
-        if (len(bc_diff(pb, xi - xj)) > safe[i] + safe[j]) return;
+        if (len(bc_diff(pbd, xi - xj)) > safe[i] + safe[j]) return;
         if (EMIT) {
             const double depth = pi.w + pj.w - len(bc_diff(pb, xj - xi));
             const unsigned m = (depth >= -skin ? NBR_NEAR : 0u) | (depth > 0.0 ? NBR_ACTIVE : 0u);
@@ -1171,11 +1179,11 @@ __device__ unsigned pbc_cell_pairs(const PbcArgs &a, int r, uint2 *out)
         }
         ++cnt;
     };
-    for (int k = 0; k < 13; ++k) {
+    auto lookup = [&](int k) {
         int tb = 0, te = 0;
         bool found = cell_range((long long)c[0] + NB13[k][0], (long long)c[1] + NB13[k][1],
                                 (long long)c[2] + NB13[k][2], tb, te);
-        if (!found && (pb->pax[0] || pb->pax[1] || pb->pax[2])) {
+        if (!found && pbd && (pbd->pax[0] || pbd->pax[1] || pbd->pax[2])) {
             // the wrapped half-shell cell (LeapConsultant.h:199-444)
             bool border = false, dup = false;
             unsigned w[3];
@@ -1183,11 +1191,11 @@ __device__ unsigned pbc_cell_pairs(const PbcArgs &a, int r, uint2 *out)
                 const int o = NB13[k][d];
                 if (o < 0) {
                     border = border || c[d] == 0u;
-                    w[d] = pb->pax[d] ? pb->maxid[d] : c[d] - 1u;
+                    w[d] = pbd->pax[d] ? pbd->maxid[d] : c[d] - 1u;
                     dup = dup || w[d] == c[d] + 1u;
                 } else if (o > 0) {
-                    border = border || c[d] == pb->maxid[d];
-                    w[d] = pb->pax[d] ? 0u : c[d] + 1u;
+                    border = border || c[d] == pbd->maxid[d];
+                    w[d] = pbd->pax[d] ? 0u : c[d] + 1u;
                     dup = dup || w[d] == c[d] - 1u;
                 } else {
                     w[d] = c[d];
@@ -1198,7 +1206,8 @@ __device__ unsigned pbc_cell_pairs(const PbcArgs &a, int r, uint2 *out)
         if (found)
             for (int i = b; i < e; ++i)
                 for (int j = tb; j < te; ++j) pair_ss(i, j);
-    }
+    };
+    for (int k = 0; k < (a.nbs ? 12 : 13); ++k) lookup(k);
     for (int i = b; i < e; ++i)
         for (int j = i + 1; j < e; ++j) pair_ss(i, j);
     for (int i = b; i < e; ++i) {
@@ -1209,7 +1218,7 @@ __device__ unsigned pbc_cell_pairs(const PbcArgs &a, int r, uint2 *out)
             const WallDev &W = a.walls[w];
             if (law_of(gi, W.group) == LAW_NONE) continue;
             const double sw = G.sf * (2.0 * W.range + G.safe_v * len(W.vel) + G.safe_g);
-            if (len(bc_diff(pb, domap_wall_vec(xi, W))) > safe[i] + sw) continue;
+            if (len(bc_diff(pbd, domap_wall_vec(xi, W))) > safe[i] + sw) continue;
             if (EMIT) {
                 Contact cw;
                 cd_wall_at<true>(xi, pi.w, W, a.holes, w, cw);
@@ -1219,6 +1228,7 @@ __device__ unsigned pbc_cell_pairs(const PbcArgs &a, int r, uint2 *out)
             ++cnt;
         }
     }
+    if (a.nbs) lookup(12);   // NBSConsultant.cpp:268-271
     return cnt;
 }
 

@@ -166,8 +166,9 @@ def _dp(a):
     return a.ctypes.data_as(_D)
 
 
-def scene_desc(wa, inner_skin: float = 0.0):
-    """vg_scene_desc of a WorldArrays; returns (desc, buffers to keep alive)."""
+def scene_desc(wa, inner_skin: float = 0.0, consultant: int = 0):
+    """vg_scene_desc of a WorldArrays; returns (desc, buffers to keep alive).
+    consultant: 0 LeapConsultant's candidate list, 1 NBSConsultant's."""
     M = len(wa.model_shape)
     G = wa.iact_law.shape[0]
     models = (VgModel * M)()
@@ -199,7 +200,7 @@ def scene_desc(wa, inner_skin: float = 0.0):
     for c in range(3):
         d.g[c] = float(wa.g[c])
     d.safety_factor = wa.safety_factor
-    d.up_iact, d.record, d.consultant = wa.up_iact, wa.record, 0
+    d.up_iact, d.record, d.consultant = wa.up_iact, wa.record, int(consultant)
     d.inner_skin = inner_skin
     zlo, zhi = getattr(wa, "zoi_lo", np.zeros(3)), getattr(wa, "zoi_hi", np.zeros(3))
     for k in range(3):
@@ -246,11 +247,11 @@ class _Knight:
 class GpuSimulation(_Knight):
     """One scene on the current GPU (a WorldArrays from vedo_amd.world)."""
 
-    def __init__(self, wa, inner_skin: float = 0.0):
+    def __init__(self, wa, inner_skin: float = 0.0, consultant: int = 0):
         L = lib()
         self.L = L
         self.n = wa.n_do
-        d, self._keep = scene_desc(wa, inner_skin)
+        d, self._keep = scene_desc(wa, inner_skin, consultant)
         h = C.c_void_p()
         rc = L.vg_create(C.byref(d), C.byref(h))
         if rc:

@@ -92,6 +92,11 @@ def make_cases():
     # NBS consultant on the same kind of scene
     w = scenes.box_scene(64, seed=12, law="ISwBSDBF", dt=2e-5, steps=400, kn=2e3, e=0.6, mu=0.4, **common)
     cases.append(("box_nbs", w.to_arrays(up_iact=5, record=1000), 1, 100, [0, 1, 5, 6, 100], None))
+    # NBS's quirks: every coordinate negative (its maxima start from
+    # numeric_limits<double>::min()) and a smaller last sphere (its zone size
+    # takes the last local DO's range)
+    cases.append(("box_nbs_quirks", nbs_quirk_world().to_arrays(up_iact=5, record=1000), 1, 100,
+                  [0, 1, 5, 6, 10, 11, 50, 100], None))
     # polydisperse drum section with kinematic walls (prescribed rigid rotation)
     w, wall_ids, c, a = scenes.drum_scene(300, seed=13, R=0.02, L=0.012, omega=20.0, dt=1e-5, steps=500,
                                           sides=16, rmin=0.6e-3, rmax=1.4e-3, classes=5, kn=2e3)
@@ -179,6 +184,29 @@ def bond_world(law, bn=0.03, bs=2e-3, seed=21):
     for im in w.iact_models:
         im.mechanisms["NormalBondStrength"] = bn
         im.mechanisms["ShearBondStrength"] = bs
+    return w
+
+
+def nbs_quirk_world(seed=24, shift=-0.05, small=0.7):
+    """64 spheres in their box moved to negative coordinates, every fourth
+    sphere (the last one among them) a smaller model of radius small*r."""
+    from vedo_amd.world import DOModel, DOStatus, IactModel
+    w = scenes.box_scene(64, seed=seed, law="ISwBSDBF", dt=2e-5, steps=400, omega_max=5.0, spacing=2.02,
+                         jitter=0.05, vmax=0.3, offset=3.0, kn=2e3, e=0.6, mu=0.4)
+    big = [m for m in w.models if m.name == "Sphere"][0]
+    r = big.attributes[0]
+    w.models.append(DOModel("Small", big.group, "mobile", "local", big.density, 1.0, (0, 0, 0), "Sphere",
+                            (small * r,)))
+    sph = [k for k, st in enumerate(w.statuses) if st.name == "Sphere"]
+    for q, k in enumerate(sph):
+        st = w.statuses[k]
+        name = "Small" if (q % 4 == 3) else st.name
+        w.statuses[k] = DOStatus(name, tuple(np.asarray(st.position) + shift), st.velocity, st.orientation_x,
+                                 st.orientation_z, st.angular_velocity)
+    for k, st in enumerate(w.statuses):
+        if st.name == "Plate":
+            w.statuses[k] = DOStatus(st.name, tuple(np.asarray(st.position) + shift), st.velocity,
+                                     st.orientation_x, st.orientation_z, st.angular_velocity)
     return w
 
 

@@ -172,6 +172,12 @@ def make_cases():
     cases.append(("box_pbc_h", scenes.periodic_box((5, 5, 4), "xy", seed=36, law="ISwH", E=1e6, vmax=1.0,
                                                     drift=(1.5, 1.0, 0.0))
                   .to_arrays(up_iact=7, record=1000), 0, 60, [0, 1, 2, 6, 7, 8, 14, 15, 28, 29, 49, 50, 60], None))
+    # NBSConsultant's candidate list in a periodic cell: its zones never wrap
+    # (pairs across the boundary are not candidates), while contact detection
+    # of the listed pairs and the positions do
+    cases.append(("box_pbc_nbs", scenes.periodic_box((6, 6, 4), "xy", seed=35, law="ISwBSDBF", vmax=0.6,
+                                                      drift=(1.0, 0.5, 0.0))
+                  .to_arrays(up_iact=5, record=1000), 1, 200, [0, 1, 4, 5, 6, 50, 100, 200], None))
     return cases
 
 

@@ -59,7 +59,7 @@ def test_gpu_matches_reference(name):
         # the zone of interest erases DOs; the oracle's map of original ids
         # names the DOs left (the reference's dumps hold them in closed-up order)
         from oracle.oracle import OracleSim
-        orc = OracleSim(wa)
+        orc = OracleSim(wa, consultant=cons)
     step = 0
     for k, d in enumerate(z["dumps"]):
         while step < d:

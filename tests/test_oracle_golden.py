@@ -131,3 +131,30 @@ def test_pbc_fixtures_reach_the_periodic_branches(name):
         prev = p
     print(f"\n  [{name}] cross-boundary candidates {cross_cand}, contacts {cross_act}, wraps {wrapped}")
     assert cross_cand > 0 and cross_act > 0 and wrapped > 0
+
+
+def test_nbs_periodic_fixture_wraps_without_cross_boundary_candidates():
+    """NBSConsultant's zones never wrap (NBSConsultant.cpp has no periodic
+    term): in box_pbc_nbs no candidate pair lies more than half a period
+    apart, while spheres still leave through one face and come back through
+    the other (EnforceBoundaryConditions) and stay inside [lower, upper)."""
+    z, wa = load_golden("box_pbc_nbs")
+    assert int(z["consultant"]) == 1
+    L = np.asarray(wa.pbc_hi) - np.asarray(wa.pbc_lo)
+    on = L != 0
+    sph = wa.model_shape[wa.model_of] == 1
+    wrapped = 0
+    prev = None
+    for k in range(len(z["dumps"])):
+        st = z[f"d{k}_state"][:, 0]
+        ri = z[f"d{k}_iact"]
+        ss = sph[ri["slave"]] & sph[ri["master"]]
+        far = np.any(np.abs(st[ri["slave"]] - st[ri["master"]])[:, on] > 0.5 * L[on], axis=1) & ss
+        p = st[sph][:, on]
+        assert np.all(p >= np.asarray(wa.pbc_lo)[on]) and np.all(p < np.asarray(wa.pbc_hi)[on])
+        if k in (0, 1):
+            assert not far.any()   # candidates as listed at the last rebuild (positions before wraps)
+        if prev is not None:
+            wrapped += int(np.any(np.abs(p - prev) > 0.5 * L[on], axis=1).sum())
+        prev = p
+    assert wrapped > 0

@@ -48,6 +48,7 @@ struct SceneHost {
     int w_begin = 0, law_off = 0, n_groups = 0, model_off = 0, n_models = 0;
     double dt = 0, t = 0, t_stop = 0, g[3] = {0, 0, 0}, sf = 1.1, skin = 0, bond_reach = 0, rmin = 0;
     bool sd_only = true, single_group = true, no_bond = true;
+    int group0 = 0;                     // the spheres' group where they share one
     bool h_only = true;                 // every law present is ISwH
     bool has_constrained = false;       // a wall moves under its contacts (DOConstrained*)
     int zon = 0;                        // zone of interest: bounded axes (bits), bounds
@@ -424,6 +425,7 @@ static int scene_from_desc(vg_scene *h, const vg_scene_desc *d, SceneHost &S, st
             if (g0 < 0) g0 = g;
             else if (g != g0) S.single_group = false;
         }
+        S.group0 = g0 < 0 ? 0 : g0;
     }
     S.dt = d->dt; S.t = d->t_current; S.t_stop = d->t_stop;
     for (int c = 0; c < 3; ++c) S.g[c] = d->g[c];
@@ -1686,6 +1688,13 @@ extern "C" int vg_admit(vg_scene *h, const vg_scene_desc *d, long long steps, in
 // Stepping
 // ---------------------------------------------------------------------------
 
+// one scene slot whose spheres share one group: the step reads its scene row
+// and sphere-sphere law through uniform addresses
+static bool uniform_scene(const vg_scene *h)
+{
+    return h->scap == 1 && h->single_group && !h->sc.empty() && h->sc[0].state == VG_SCENE_RUNNING;
+}
+
 static StepArgs step_args(vg_scene *h)
 {
     StepArgs a;
@@ -1704,6 +1713,13 @@ static StepArgs step_args(vg_scene *h)
     a.pbc = h->pbc;
     a.single_group = h->single_group;
     a.no_bond = h->no_bond;
+    a.group0 = 0;
+    a.law_ss = 0;
+    if (uniform_scene(h)) {
+        const SceneHost &S = h->sc[0];
+        a.group0 = S.group0;
+        a.law_ss = S.law_off + S.group0 * S.n_groups + S.group0;
+    }
     a.cw_w = h->cw_w; a.cw_p = h->cw_p; a.cw_val = h->cw_val; a.cw_stamp = h->cw_stamp;
     a.stamp = (unsigned)(h->step_count & 0x7fffffffull);
     return a;
@@ -1821,16 +1837,24 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
     const int ls = (h->holes || h->pbc) ? LS_EXT : (h->sd_only ? LS_SD : (h->h_only ? LS_H : LS_ALL));
     const bool mix = h->precision == VG_PRECISION_MIXED;
     using KFn = void (*)(StepArgs);
-    static const KFn kfn[2][4][2] = {
-        {{k_step<false, LS_SD, false>, k_step<false, LS_SD, true>},
-         {k_step<false, LS_H, false>, k_step<false, LS_H, true>},
-         {k_step<false, LS_ALL, false>, k_step<false, LS_ALL, true>},
-         {k_step<false, LS_EXT, false>, k_step<false, LS_EXT, true>}},
-        {{k_step<true, LS_SD, false>, k_step<true, LS_SD, true>},
-         {k_step<true, LS_H, false>, k_step<true, LS_H, true>},
-         {k_step<true, LS_ALL, false>, k_step<true, LS_ALL, true>},
-         {k_step<true, LS_EXT, false>, k_step<true, LS_EXT, true>}}};
-    hipLaunchKernelGGL(kfn[rec ? 1 : 0][ls][mix ? 1 : 0], grid, blk, 0, h->stream, a);
+    static const KFn kfn[2][2][4][2] = {
+        {{{k_step<false, LS_SD, false, false>, k_step<false, LS_SD, true, false>},
+          {k_step<false, LS_H, false, false>, k_step<false, LS_H, true, false>},
+          {k_step<false, LS_ALL, false, false>, k_step<false, LS_ALL, true, false>},
+          {k_step<false, LS_EXT, false, false>, k_step<false, LS_EXT, true, false>}},
+         {{k_step<true, LS_SD, false, false>, k_step<true, LS_SD, true, false>},
+          {k_step<true, LS_H, false, false>, k_step<true, LS_H, true, false>},
+          {k_step<true, LS_ALL, false, false>, k_step<true, LS_ALL, true, false>},
+          {k_step<true, LS_EXT, false, false>, k_step<true, LS_EXT, true, false>}}},
+        {{{k_step<false, LS_SD, false, true>, k_step<false, LS_SD, true, true>},
+          {k_step<false, LS_H, false, true>, k_step<false, LS_H, true, true>},
+          {k_step<false, LS_ALL, false, true>, k_step<false, LS_ALL, true, true>},
+          {k_step<false, LS_EXT, false, false>, k_step<false, LS_EXT, true, false>}},
+         {{k_step<true, LS_SD, false, true>, k_step<true, LS_SD, true, true>},
+          {k_step<true, LS_H, false, true>, k_step<true, LS_H, true, true>},
+          {k_step<true, LS_ALL, false, true>, k_step<true, LS_ALL, true, true>},
+          {k_step<true, LS_EXT, false, false>, k_step<true, LS_EXT, true, false>}}}};
+    hipLaunchKernelGGL(kfn[uniform_scene(h) ? 1 : 0][rec ? 1 : 0][ls][mix ? 1 : 0], grid, blk, 0, h->stream, a);
     if (kt) {
         HIPCHK(hipEventRecord(h->kev[2 * h->kev_used + 1], h->stream));
         h->kev_used++;

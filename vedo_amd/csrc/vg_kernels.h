@@ -84,6 +84,7 @@ struct StepArgs {
     int step;
     int single_group;        // every scene's spheres share one group: no partner group lookup
     int no_bond;             // no bonded law: surely separated pairs skip the contact geometry
+    int group0, law_ss;      // one scene whose spheres share one group: that group and its sphere-sphere law
     // constrained walls: a sphere's constrained walls and their pair slots [2][LD],
     // each pair's master impulse and the step it was solved in
     const int *cw_w, *cw_p;
@@ -162,7 +163,7 @@ struct ActiveOut {
     double rec[13];
 };
 
-template <bool REC, int LS, bool MIX> __global__ void k_step(StepArgs a);
+template <bool REC, int LS, bool MIX, bool UNI> __global__ void k_step(StepArgs a);
 __global__ void k_walls(WallDev *walls, int nw, const SceneDev *scenes, double *stime, const double *assigned,
                         const WallDev *ref, const WallKin *kin, const double *cw_imp, int *rflags, int step);
 __global__ void k_cw_count(int n, int ld, int ko, const unsigned *cnt, const unsigned *nbr, const WallDev *walls,

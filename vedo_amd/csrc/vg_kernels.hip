@@ -160,15 +160,19 @@ __device__ inline int xcd_block(int b, int nb)
 // record.  Returns the owner's impulse and angular impulse from the pair
 // (zero where the solver adds nothing) and whether the pair stays in contact
 // on its slave side.
-template <bool REC, int LS, bool MIX>
-__device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi, double ri, d3 vi, d3 wi, double mi,
-                                   int gi, const SceneDev &S, const double4 *s_pos, const double4 *s_vel,
-                                   const double4 *s_avl, unsigned b0, unsigned nb, d3 &dI, d3 &dL)
+// `ent` is the entry's word as pass 1 read it (carried in the queue, so the
+// partner's state is requested together with the history).  UNI: one scene
+// whose spheres share one group; its sphere-sphere law (a.law_ss) and scene
+// row are read through uniform addresses.
+template <bool REC, int LS, bool MIX, bool UNI>
+__device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsigned ent, d3 xi, double ri, d3 vi,
+                                   d3 wi, double mi, int gi, const SceneDev &S, const double4 *s_pos,
+                                   const double4 *s_vel, const double4 *s_avl, unsigned b0, unsigned nb, d3 &dI,
+                                   d3 &dL)
 {
     dI = mk(0, 0, 0);
     dL = mk(0, 0, 0);
     const size_t ei = eidx(h, k, a.ki, a.ld, i);
-    const unsigned ent = nt_load(a.ell.ent + ei);
     const unsigned j = ent & PARTNER_MASK;
     Hist hs;
     hs.f = ent & ~PARTNER_MASK;
@@ -199,7 +203,7 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
     ContactM cm;
     d3 xa, va, wa, xb, vb, wb;
     double ma, mb;
-    const LawDev *law;
+    LawDev law;
     bool owner_a = true;
     const WallDev *w = nullptr;
     if (j < (unsigned)a.ld) {
@@ -207,8 +211,12 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
         const double4 pj = loc ? s_pos[j - b0] : a.in.pos[j];
         const double4 vj = loc ? s_vel[j - b0] : a.in.vel[j];
         const double4 wj = loc ? s_avl[j - b0] : a.in.avl[j];
-        const int gj = a.single_group ? gi : a.t.models[a.in.mdl[j]].group;
-        law = &a.t.laws[S.law_off + gi * S.n_groups + gj];
+        if (UNI) {
+            law = a.t.laws[a.law_ss];
+        } else {
+            const int gj = a.single_group ? gi : a.t.models[a.in.mdl[j]].group;
+            law = a.t.laws[S.law_off + gi * S.n_groups + gj];
+        }
         owner_a = (hs.f & F_SLAVE) != 0;
         const d3 xj = ld3(pj), vjj = ld3(vj), wjj = ld3(wj);
         xa = owner_a ? xi : xj;
@@ -229,7 +237,7 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
         }
     } else {
         w = &a.walls[j - a.ld];
-        law = &a.t.laws[S.law_off + gi * S.n_groups + w->group];
+        law = a.t.laws[S.law_off + gi * S.n_groups + w->group];
         cd_wall_at<LS == LS_EXT>(xi, ri, *w, a.holes, (int)(j - a.ld), c);
         xa = xi; va = vi; wa = wi; ma = mi;
         xb = w->pos; vb = w->vel; wb = w->av; mb = w->msudo;
@@ -239,9 +247,9 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, d3 xi
             if (MIX) to_mixed(c, xi, w->pos, cm);
         }
     }
-    if (MIX) solve_mixed<REC, LS>(*law, ma, mb, xa, va, wa, vb, wb, cm, S.dt, hs, o, r);
-    else if constexpr (LS == LS_EXT) solve<REC, LS>(*law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r, a.pbc);
-    else solve<REC, LS>(*law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r);
+    if (MIX) solve_mixed<REC, LS>(law, ma, mb, xa, va, wa, vb, wb, cm, S.dt, hs, o, r);
+    else if constexpr (LS == LS_EXT) solve<REC, LS>(law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r, a.pbc);
+    else solve<REC, LS>(law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r);
     if (o.added) {
         if (owner_a) {
             dI = o.imp;
@@ -305,19 +313,24 @@ __device__ inline int block_scan256(int v, int *s_wsum, int &total)
 
 // 3 workgroups per CU (12 wavefronts), as the LDS allows: the register
 // budget is capped at 168 VGPRs, the larger law sets spill a few registers
-template <bool REC, int LS, bool MIX>
+template <bool REC, int LS, bool MIX, bool UNI>
 __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
 {
     constexpr int NT = 256;   // threads per block: the launch's block size
-    constexpr int QPT = 16;   // queued entries per thread and round of the queue
+    constexpr int QPT = 8;    // queued entries per thread and round of the queue
+    constexpr int NSTAGE = 12;   // entry words a thread stages in pass 1 for the first round
+    static_assert(NSTAGE >= QPT && NSTAGE * NT * 4 <= 6 * NT * 8, "staging fits the result area");
     // the block's spheres (partners inside the block are read from here) and
-    // its queue of pair solves, one per thread per pass, with their results
+    // its queue of pair solves (item and entry word), one per thread per pass,
+    // with their results; pass 1 stages its queued entry words in the result area
     __shared__ double4 s_pos[NT], s_vel[NT], s_avl[NT];
     __shared__ int s_mdl[NT];
     __shared__ unsigned short s_item[NT * QPT];
+    __shared__ unsigned s_qent[NT * QPT];
     __shared__ double s_out[6][NT];
     __shared__ int s_wsum[4];
     __shared__ unsigned s_ncontact;
+    unsigned *const s_stage = reinterpret_cast<unsigned *>(&s_out[0][0]);   // [NSTAGE][NT], pass 1 only
 
     const int t = threadIdx.x;
     const int b0 = xcd_block(blockIdx.x, gridDim.x) * NT;
@@ -376,6 +389,7 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
     // Pass 1: surely separated sphere pairs are settled here (Clean status,
     // no impulse); every other entry is queued for the full solve.
     unsigned long long todo = 0;
+    int nstage = 0;
     // entries in groups of PF: the entry loads, then the partner-position
     // gathers of a group are independent and overlap in flight
     constexpr int PF = 8;
@@ -410,7 +424,11 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
                     if (f != (ent[u] & ~PARTNER_MASK)) nt_store(j | f, a.ell.ent + eidx(h, k, a.ki, a.ld, i));
                 }
             }
-            if (full) todo |= 1ull << k;
+            if (full) {
+                todo |= 1ull << k;
+                if (nstage < NSTAGE) s_stage[nstage * NT + t] = ent[u];
+                ++nstage;
+            }
         }
     }
 
@@ -421,6 +439,7 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
     d3 I = mk(0, 0, 0), L = mk(0, 0, 0);
     unsigned n_contact = 0;
     unsigned long long rem = todo;
+    bool first_round = true;
     for (;;) {
         unsigned long long take = rem;
         for (int q = __popcll(take); q > QPT; --q) take &= ~(1ull << (63 - __builtin_clzll(take)));
@@ -428,14 +447,18 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
         int total = 0;
         const int off = block_scan256(c, s_wsum, total);
         {
+            // the first round takes the thread's first queued entries, whose
+            // words pass 1 staged in order; later rounds read them again
             unsigned long long tk = take;
             for (int q = 0; q < c; ++q) {
                 const int k = __builtin_ctzll(tk);
                 tk &= tk - 1;
                 s_item[off + q] = (unsigned short)((t << 7) | (h << 6) | k);
+                s_qent[off + q] = first_round ? s_stage[q * NT + t] : a.ell.ent[eidx(h, k, a.ki, a.ld, i)];
             }
         }
         rem &= ~take;
+        first_round = false;
         __syncthreads();
         for (int r0 = 0; r0 < total; r0 += NT) {
             const int q = r0 + t;
@@ -444,12 +467,20 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
                 const int lt = (int)(it >> 7), hq = (int)((it >> 6) & 1), k = (int)(it & 63);
                 const int io = b0 + lt;
                 const double4 po = s_pos[lt], vo = s_vel[lt], wo = s_avl[lt];
-                const ModelDev &mo = a.t.models[s_mdl[lt]];
-                const SceneDev &So = a.scenes[mo.scene];
+                int go;
+                const SceneDev *So;
+                if (UNI) {
+                    go = a.group0;
+                    So = &a.scenes[0];
+                } else {
+                    const ModelDev &mo = a.t.models[s_mdl[lt]];
+                    go = mo.group;
+                    So = &a.scenes[mo.scene];
+                }
                 d3 dI, dL;
-                const bool sc_ = solve_entry<REC, LS, MIX>(a, io, hq, k, ld3(po), po.w, ld3(vo), ld3(wo), wo.w,
-                                                           mo.group, So, s_pos, s_vel, s_avl, (unsigned)b0, nb,
-                                                           dI, dL);
+                const bool sc_ = solve_entry<REC, LS, MIX, UNI>(a, io, hq, k, s_qent[q], ld3(po), po.w, ld3(vo),
+                                                                ld3(wo), wo.w, go, *So, s_pos, s_vel, s_avl,
+                                                                (unsigned)b0, nb, dI, dL);
                 n_contact += sc_ ? 1u : 0u;
                 s_out[0][t] = dI.x; s_out[1][t] = dI.y; s_out[2][t] = dI.z;
                 s_out[3][t] = dL.x; s_out[4][t] = dL.y; s_out[5][t] = dL.z;
@@ -511,22 +542,34 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
     if (t == 0 && s_ncontact) a.contacts[b0 >> 6] += s_ncontact;
 }
 
-template __global__ void k_step<false, LS_SD, false>(StepArgs);
-template __global__ void k_step<true, LS_SD, false>(StepArgs);
-template __global__ void k_step<false, LS_H, false>(StepArgs);
-template __global__ void k_step<true, LS_H, false>(StepArgs);
-template __global__ void k_step<false, LS_ALL, false>(StepArgs);
-template __global__ void k_step<true, LS_ALL, false>(StepArgs);
-template __global__ void k_step<false, LS_SD, true>(StepArgs);
-template __global__ void k_step<true, LS_SD, true>(StepArgs);
-template __global__ void k_step<false, LS_H, true>(StepArgs);
-template __global__ void k_step<true, LS_H, true>(StepArgs);
-template __global__ void k_step<false, LS_ALL, true>(StepArgs);
-template __global__ void k_step<true, LS_ALL, true>(StepArgs);
-template __global__ void k_step<false, LS_EXT, false>(StepArgs);
-template __global__ void k_step<true, LS_EXT, false>(StepArgs);
-template __global__ void k_step<false, LS_EXT, true>(StepArgs);
-template __global__ void k_step<true, LS_EXT, true>(StepArgs);
+template __global__ void k_step<false, LS_SD, false, false>(StepArgs);
+template __global__ void k_step<true, LS_SD, false, false>(StepArgs);
+template __global__ void k_step<false, LS_H, false, false>(StepArgs);
+template __global__ void k_step<true, LS_H, false, false>(StepArgs);
+template __global__ void k_step<false, LS_ALL, false, false>(StepArgs);
+template __global__ void k_step<true, LS_ALL, false, false>(StepArgs);
+template __global__ void k_step<false, LS_SD, true, false>(StepArgs);
+template __global__ void k_step<true, LS_SD, true, false>(StepArgs);
+template __global__ void k_step<false, LS_H, true, false>(StepArgs);
+template __global__ void k_step<true, LS_H, true, false>(StepArgs);
+template __global__ void k_step<false, LS_ALL, true, false>(StepArgs);
+template __global__ void k_step<true, LS_ALL, true, false>(StepArgs);
+template __global__ void k_step<false, LS_EXT, false, false>(StepArgs);
+template __global__ void k_step<true, LS_EXT, false, false>(StepArgs);
+template __global__ void k_step<false, LS_EXT, true, false>(StepArgs);
+template __global__ void k_step<true, LS_EXT, true, false>(StepArgs);
+template __global__ void k_step<false, LS_SD, false, true>(StepArgs);
+template __global__ void k_step<true, LS_SD, false, true>(StepArgs);
+template __global__ void k_step<false, LS_H, false, true>(StepArgs);
+template __global__ void k_step<true, LS_H, false, true>(StepArgs);
+template __global__ void k_step<false, LS_ALL, false, true>(StepArgs);
+template __global__ void k_step<true, LS_ALL, false, true>(StepArgs);
+template __global__ void k_step<false, LS_SD, true, true>(StepArgs);
+template __global__ void k_step<true, LS_SD, true, true>(StepArgs);
+template __global__ void k_step<false, LS_H, true, true>(StepArgs);
+template __global__ void k_step<true, LS_H, true, true>(StepArgs);
+template __global__ void k_step<false, LS_ALL, true, true>(StepArgs);
+template __global__ void k_step<true, LS_ALL, true, true>(StepArgs);
 
 // Wall responses after the spheres' (DOFixedQuasiPlate.h:16-19,
 // DOOrbitalQuasiPlate.cpp:18-34) and kinematic prescriptions

@@ -226,9 +226,11 @@ int vg_rebuild_timing(vg_scene *h, double *ms, int *rebuilds);
  * step, wall and rebuild kernels take from the handle's lists is in range
  * (each sphere's model and scene; its neighbour-list and candidate-list
  * counts and entries: placed spheres of its own scene or wall slots of its
- * scene's range; its constrained-wall and pair slots).  counts[0..7]:
+ * scene's range; the history slots of its contacts: within the list
+ * capacity, none shared; its constrained-wall and pair slots).  counts[0..7]:
  * violations by kind (0 model, 1 scene, 2 neighbour count, 3 neighbour
- * entry, 5 candidate count, 6 candidate entry, 7 constrained-wall slot);
+ * entry, 4 history slot, 5 candidate count, 6 candidate entry, 7
+ * constrained-wall slot);
  * counts[8..10]: the first violation's kind + 1, sphere slot and entry. */
 int vg_check_lists(vg_scene *h, long long counts[11]);
 /* Counters since vg_create: rebuilds, relists, max neighbour-list fill, the

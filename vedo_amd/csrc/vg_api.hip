@@ -1474,10 +1474,11 @@ static int alloc_geo(vg_scene *h, Geo &g, int n)
 
 static int alloc_ell(vg_scene *h, Ell &e, int n, int ki)
 {
-    if (dalloc(h, &e.hdr, n) || dalloc(h, &e.ent, (size_t)2 * ki * n) || dalloc(h, &e.hd, (size_t)2 * ki * n))
+    if (dalloc(h, &e.hdr, n) || dalloc(h, &e.ent, (size_t)2 * ki * n) || dalloc(h, &e.slot, (size_t)2 * ki * n) ||
+        dalloc(h, &e.hd, (size_t)ki * n))
         return VG_ENOMEM;
     // the spring-dashpot kernels write the fourth history word as zero
-    HIPCHK(hipMemset(e.hd, 0, sizeof(double4) * 2 * ki * (size_t)n));
+    HIPCHK(hipMemset(e.hd, 0, sizeof(double4) * ki * (size_t)n));
     return 0;
 }
 
@@ -1489,7 +1490,7 @@ static void free_scene(vg_scene *h)
     F(h->geo[0].oid); F(h->geo[0].mdl); F(h->oid_spare); F(h->mdl_spare);
     for (int b = 0; b < 2; ++b) {
         F(h->geo[b].pos); F(h->geo[b].vel); F(h->geo[b].avl); F(h->geo[b].ori);
-        F(h->ell[b].hdr); F(h->ell[b].ent); F(h->ell[b].hd);
+        F(h->ell[b].hdr); F(h->ell[b].ent); F(h->ell[b].slot); F(h->ell[b].hd);
         F(h->xs[b]); F(h->ys[b]); F(h->zs[b]);
     }
     F(h->models); F(h->mm); F(h->laws); F(h->scenes); F(h->stime); F(h->rflags); F(h->walls); F(h->walls_ref);

@@ -13,8 +13,12 @@
 // spheres is one contiguous 256-byte run:
 //   hdr  u8  [LD]          count (7 bits) | current half (bit 7)
 //   ent  u32 [2][KI][LD]   partner (26 bits: sphere slot, or LD + wall slot) | status flags
-//   hd   double4 [2][KI][LD] tangential force history xyz, Kn (ISwH) /
-//                            initial velocity (ISwHertz); 32 bytes per entry
+//   slot u8  [2][KI][LD]   history slot of an entry whose Contact flag is set
+//   hd   double4 [KI][LD]  per history slot: tangential force history xyz,
+//                          Kn (ISwH) / initial velocity (ISwHertz)
+// A contact keeps its history slot while it lasts; a new contact takes the
+// sphere's lowest free slot, so a sphere's histories fill the first rows and
+// the histories of neighbouring spheres share cache lines.
 // Every sphere keeps its own copy of each pair's ImpactStatus; the two copies
 // are updated by the same instruction sequence from the same inputs, so they
 // stay bit-identical and no thread ever reads another thread's history.

@@ -15,8 +15,9 @@ struct Geo {   // one buffer set of sphere state (see vg_device.h)
 struct Ell {
     unsigned char *hdr;
     unsigned *ent;    // [2][KI][LD]
-    double4 *hd;      // [2][KI][LD]: shear force xyz, then Kn (ISwH) / initial velocity (ISwHertz),
-                      // zero for the spring-dashpot laws; 32 bytes per entry, two 16-byte accesses
+    unsigned char *slot;   // [2][KI][LD]: the entry's history slot while its Contact flag is set
+    double4 *hd;      // [KI][LD] per history slot: shear force xyz, then Kn (ISwH) / initial velocity
+                      // (ISwHertz), zero for the spring-dashpot laws; two 16-byte accesses
 };
 
 struct Tables {

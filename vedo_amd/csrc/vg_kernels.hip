@@ -60,23 +60,31 @@ template <class T> __device__ inline void st_store(T v, T *p)
     else *p = v;
 }
 
-// History of entry (h, k) of sphere i: the shear force always, the fourth
+// History in slot `sl` of sphere i: the shear force always, the fourth
 // word only where a Hertz law may have set it (AUX; without it the word is
 // read as zero and written as zero).
 template <bool AUX = true>
-__device__ inline void load_hist(const Ell &e, int h, int k, int ki, int n, int i, Hist &hs)
+__device__ inline void load_hist(const Ell &e, int sl, int n, int i, Hist &hs)
 {
-    const double2 *p = reinterpret_cast<const double2 *>(e.hd + eidx(h, k, ki, n, i));
+    const double2 *p = reinterpret_cast<const double2 *>(e.hd + (size_t)sl * (size_t)n + (size_t)i);
     const double2 u = p[0], v = p[1];
     hs.s = mk(u.x, u.y, v.x);
     hs.aux = AUX ? v.y : 0.0;
 }
 template <bool AUX = true>
-__device__ inline void store_hist(const Ell &e, int h, int k, int ki, int n, int i, const Hist &hs)
+__device__ inline void store_hist(const Ell &e, int sl, int n, int i, const Hist &hs)
 {
-    double2 *p = reinterpret_cast<double2 *>(e.hd + eidx(h, k, ki, n, i));
+    double2 *p = reinterpret_cast<double2 *>(e.hd + (size_t)sl * (size_t)n + (size_t)i);
     p[0] = make_double2(hs.s.x, hs.s.y);
     p[1] = make_double2(hs.s.z, AUX ? hs.aux : 0.0);
+}
+// The lowest history slot not in `used`, which then holds it.  A sphere's
+// slots in use never outnumber its list entries (at most 64).
+__device__ inline int take_slot(unsigned long long &used)
+{
+    const int z = __builtin_ctzll(~used);
+    used |= 1ull << z;
+    return z;
 }
 
 // ---------------------------------------------------------------------------
@@ -84,7 +92,7 @@ __device__ inline void store_hist(const Ell &e, int h, int k, int ki, int n, int
 // ---------------------------------------------------------------------------
 
 // Rebuild sphere i's neighbour list from its candidate list: keep partners
-// within `skin` of contact, carry flags and history of partners already
+// within `skin` of contact, carry flags and history slots of partners already
 // listed (both lists are ordered subsequences of the candidate list).
 template <bool EXT>
 __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_old, int c_old, int h_new,
@@ -111,10 +119,10 @@ __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_o
         }
         const bool carried = (p < c_old) && (old_partner == j);
         unsigned f = 0;
-        Hist hs;
+        unsigned char sl = 0;
         if (carried) {
             f = a.ell.ent[eidx(h_old, p, a.ki, a.ld, i)] & ~PARTNER_MASK;
-            if (f & F_CONTACT) load_hist(a.ell, h_old, p, a.ki, a.ld, i, hs);
+            if (f & F_CONTACT) sl = a.ell.slot[eidx(h_old, p, a.ki, a.ld, i)];
             ++p;
             old_partner = p < c_old ? (a.ell.ent[eidx(h_old, p, a.ki, a.ld, i)] & PARTNER_MASK) : 0xffffffffu;
         }
@@ -128,11 +136,10 @@ __device__ inline int relist(const StepArgs &a, int i, d3 xi, double ri, int h_o
                 // every step: its status is Clean() (ImpactStatus.cpp:259-274).
                 f = 0;
                 if (j >= (unsigned)a.ld || a.in.oid[i] < a.in.oid[j]) f |= F_SLAVE;
-                hs.s = mk(0, 0, 0);
-                hs.aux = 0.0;
             }
             a.ell.ent[eidx(h_new, q, a.ki, a.ld, i)] = j | f;
-            if (f & F_CONTACT) store_hist(a.ell, h_new, q, a.ki, a.ld, i, hs);
+            // a carried contact keeps its history where it is
+            if (f & F_CONTACT) a.ell.slot[eidx(h_new, q, a.ki, a.ld, i)] = sl;
             ++q;
         }
     }
@@ -161,12 +168,13 @@ __device__ inline int xcd_block(int b, int nb)
 // (zero where the solver adds nothing) and whether the pair stays in contact
 // on its slave side.
 // `ent` is the entry's word as pass 1 read it (carried in the queue, so the
-// partner's state is requested together with the history).  UNI: one scene
+// partner's state is requested together with the history); `sl` its history
+// slot, or the free slot it takes if the solve starts a contact.  UNI: one scene
 // whose spheres share one group; its sphere-sphere law (a.law_ss) and scene
 // row are read through uniform addresses.
 template <bool REC, int LS, bool MIX, bool UNI>
-__device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsigned ent, d3 xi, double ri, d3 vi,
-                                   d3 wi, double mi, int gi, const SceneDev &S, const double4 *s_pos,
+__device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsigned ent, int sl, d3 xi, double ri,
+                                   d3 vi, d3 wi, double mi, int gi, const SceneDev &S, const double4 *s_pos,
                                    const double4 *s_vel, const double4 *s_avl, unsigned b0, unsigned nb, d3 &dI,
                                    d3 &dL)
 {
@@ -185,9 +193,13 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsig
         hs.f = f;
     }
     hs.f &= ~F_NOBACKUP;
-    // read with the entry word (most queued entries are contacts); meaningful only for contacts
-    load_hist<LS != LS_SD>(a.ell, h, k, a.ki, a.ld, i, hs);
-    if (!had_contact) { hs.s = mk(0, 0, 0); hs.aux = 0.0; }
+    // meaningful only for contacts (most queued entries are)
+    if (had_contact) {
+        load_hist<LS != LS_SD>(a.ell, sl, a.ld, i, hs);
+    } else {
+        hs.s = mk(0, 0, 0);
+        hs.aux = 0.0;
+    }
     PairOut o;
     o.added = false;
     RecOut r;
@@ -277,7 +289,8 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsig
     if (hs.f != (ent & ~PARTNER_MASK)) nt_store(j | hs.f, a.ell.ent + ei);
     bool slave_contact = false;
     if (hs.f & F_CONTACT) {
-        store_hist<LS != LS_SD>(a.ell, h, k, a.ki, a.ld, i, hs);
+        store_hist<LS != LS_SD>(a.ell, sl, a.ld, i, hs);
+        if (!had_contact) a.ell.slot[ei] = (unsigned char)sl;
         slave_contact = (hs.f & F_SLAVE) != 0;
     }
     // Without bonds a status without Contact is Clean (all record fields
@@ -318,19 +331,21 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
 {
     constexpr int NT = 256;   // threads per block: the launch's block size
     constexpr int QPT = 8;    // queued entries per thread and round of the queue
-    constexpr int NSTAGE = 12;   // entry words a thread stages in pass 1 for the first round
-    static_assert(NSTAGE >= QPT && NSTAGE * NT * 4 <= 6 * NT * 8, "staging fits the result area");
     // the block's spheres (partners inside the block are read from here) and
-    // its queue of pair solves (item and entry word), one per thread per pass,
-    // with their results; pass 1 stages its queued entry words in the result area
+    // its queue of pair solves (item, entry word, history slot), one per
+    // thread per pass, with their results; pass 1 stages the words and slots
+    // of each thread's first QPT queued entries in the result area
     __shared__ double4 s_pos[NT], s_vel[NT], s_avl[NT];
     __shared__ int s_mdl[NT];
     __shared__ unsigned short s_item[NT * QPT];
     __shared__ unsigned s_qent[NT * QPT];
+    __shared__ unsigned char s_qslot[NT * QPT];
     __shared__ double s_out[6][NT];
     __shared__ int s_wsum[4];
     __shared__ unsigned s_ncontact;
-    unsigned *const s_stage = reinterpret_cast<unsigned *>(&s_out[0][0]);   // [NSTAGE][NT], pass 1 only
+    static_assert(QPT * NT * 5 <= sizeof(s_out), "the staging fits the result area");
+    unsigned *const s_stage = reinterpret_cast<unsigned *>(&s_out[0][0]);                   // [QPT][NT]
+    unsigned char *const s_stage_slot = reinterpret_cast<unsigned char *>(s_stage + QPT * NT);   // [QPT][NT]
 
     const int t = threadIdx.x;
     const int b0 = xcd_block(blockIdx.x, gridDim.x) * NT;
@@ -389,15 +404,19 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
     // Pass 1: surely separated sphere pairs are settled here (Clean status,
     // no impulse); every other entry is queued for the full solve.
     unsigned long long todo = 0;
+    unsigned long long used = 0;   // history slots of the entries in contact
     int nstage = 0;
     // entries in groups of PF: the entry loads, then the partner-position
     // gathers of a group are independent and overlap in flight
     constexpr int PF = 8;
     for (int k0 = 0; k0 < cnt; k0 += PF) {
         unsigned ent[PF];
+        unsigned char esl[PF];
 #pragma unroll
-        for (int u = 0; u < PF; ++u)
+        for (int u = 0; u < PF; ++u) {
             ent[u] = k0 + u < cnt ? nt_load(a.ell.ent + eidx(h, k0 + u, a.ki, a.ld, i)) : PARTNER_MASK;
+            esl[u] = k0 + u < cnt ? a.ell.slot[eidx(h, k0 + u, a.ki, a.ld, i)] : 0;
+        }
         double4 pj[PF];
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
@@ -410,6 +429,7 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
             const int k = k0 + u;
             if (k >= cnt) break;
             const unsigned j = ent[u] & PARTNER_MASK;
+            if (ent[u] & F_CONTACT) used |= 1ull << esl[u];
             bool full = true;
             if (pj[u].w >= 0.0) {   // a sphere partner, unbonded laws
                 d3 dx = ld3(pj[u]) - xi;
@@ -426,7 +446,10 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
             }
             if (full) {
                 todo |= 1ull << k;
-                if (nstage < NSTAGE) s_stage[nstage * NT + t] = ent[u];
+                if (nstage < QPT) {
+                    s_stage[nstage * NT + t] = ent[u];
+                    s_stage_slot[nstage * NT + t] = esl[u];
+                }
                 ++nstage;
             }
         }
@@ -448,13 +471,21 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
         const int off = block_scan256(c, s_wsum, total);
         {
             // the first round takes the thread's first queued entries, whose
-            // words pass 1 staged in order; later rounds read them again
+            // words and slots pass 1 staged in order; later rounds read them
+            // again.  An entry not in contact gets a free slot for the
+            // history it keeps if the solve starts a contact.
             unsigned long long tk = take;
             for (int q = 0; q < c; ++q) {
                 const int k = __builtin_ctzll(tk);
                 tk &= tk - 1;
+                const size_t x = eidx(h, k, a.ki, a.ld, i);
+                const unsigned w = first_round ? s_stage[q * NT + t] : a.ell.ent[x];
+                int sl;
+                if (w & F_CONTACT) sl = first_round ? s_stage_slot[q * NT + t] : a.ell.slot[x];
+                else sl = take_slot(used);
                 s_item[off + q] = (unsigned short)((t << 7) | (h << 6) | k);
-                s_qent[off + q] = first_round ? s_stage[q * NT + t] : a.ell.ent[eidx(h, k, a.ki, a.ld, i)];
+                s_qent[off + q] = w;
+                s_qslot[off + q] = (unsigned char)sl;
             }
         }
         rem &= ~take;
@@ -478,9 +509,9 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
                     So = &a.scenes[mo.scene];
                 }
                 d3 dI, dL;
-                const bool sc_ = solve_entry<REC, LS, MIX, UNI>(a, io, hq, k, s_qent[q], ld3(po), po.w, ld3(vo),
-                                                                ld3(wo), wo.w, go, *So, s_pos, s_vel, s_avl,
-                                                                (unsigned)b0, nb, dI, dL);
+                const bool sc_ = solve_entry<REC, LS, MIX, UNI>(a, io, hq, k, s_qent[q], s_qslot[q], ld3(po), po.w,
+                                                                ld3(vo), ld3(wo), wo.w, go, *So, s_pos, s_vel,
+                                                                s_avl, (unsigned)b0, nb, dI, dL);
                 n_contact += sc_ ? 1u : 0u;
                 s_out[0][t] = dI.x; s_out[1][t] = dI.y; s_out[2][t] = dI.z;
                 s_out[3][t] = dL.x; s_out[4][t] = dL.y; s_out[5][t] = dL.z;
@@ -1369,8 +1400,17 @@ __global__ void k_check_lists(Geo g, Ell ell, int n, int ld, int ki, int ko, con
     const unsigned hdr = ell.hdr[i];
     const int h = hdr >> 7, cnt = hdr & 127;
     if (cnt > ki) bad(2, cnt);
-    for (int k = 0; k < min(cnt, ki); ++k)
-        if (!partner_ok(ell.ent[((size_t)(h * ki + k)) * (size_t)ld + i] & PARTNER_MASK)) bad(3, k);
+    unsigned long long used = 0;
+    for (int k = 0; k < min(cnt, ki); ++k) {
+        const size_t x = ((size_t)(h * ki + k)) * (size_t)ld + i;
+        if (!partner_ok(ell.ent[x] & PARTNER_MASK)) bad(3, k);
+        if (ell.ent[x] & F_CONTACT) {
+            // a contact's history slot: within the capacity, no two contacts share one
+            const unsigned sl = ell.slot[x];
+            if (sl >= (unsigned)ki || sl >= 64u || (used >> sl & 1ull)) bad(4, k);
+            else used |= 1ull << sl;
+        }
+    }
     const unsigned oc = o_cnt[i];
     if (oc > (unsigned)ko) bad(5, (int)oc);
     for (unsigned t = 0; t < min(oc, (unsigned)ko); ++t)
@@ -1417,6 +1457,7 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
     const unsigned p = a.perm[s];
     const unsigned hdr = a.eold.hdr[p];
     const int h_old = hdr >> 7, c_old = hdr & 127;
+    int n_hist = 0;   // the new list's contacts take history slots 0, 1, ... in list order
     for (int k = 0; k < c_old; ++k) {
         const unsigned j = ent_old[eidx(h_old, k, a.ki, a.ld, p)] & PARTNER_MASK;
         old_oid[k * blockDim.x + threadIdx.x] = j < (unsigned)a.ld ? a.gold.oid[j] : 0x80000000u + (j - a.ld);
@@ -1456,13 +1497,17 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
                 for (int k = 0; k < c_old; ++k)
                     if (old_oid[k * blockDim.x + threadIdx.x] == key) {
                         f = (ent_old[eidx(h_old, k, a.ki, a.ld, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
-                        if (f & F_CONTACT) load_hist(a.eold, h_old, k, a.ki, a.ld, p, hs);
+                        if (f & F_CONTACT) load_hist(a.eold, a.eold.slot[eidx(h_old, k, a.ki, a.ld, p)], a.ld, p, hs);
                         break;
                     }
             }
             if (slave) f |= F_SLAVE;
             ent_new[eidx(0, q, a.ki, a.ld, s)] = j | f;
-            if (f & F_CONTACT) store_hist(a.ell, 0, q, a.ki, a.ld, s, hs);
+            if (f & F_CONTACT) {
+                a.ell.slot[eidx(0, q, a.ki, a.ld, s)] = (unsigned char)n_hist;
+                store_hist(a.ell, n_hist, a.ld, s, hs);
+                ++n_hist;
+            }
             ++q;
         }
         if (full) break;
@@ -1595,6 +1640,13 @@ __global__ void k_restore_history(Geo g, Ell ell, int b, int e, int ld, int ki, 
     const ModelDev &mi = models[g.mdl[i]];
     const SceneDev &S = scenes[mi.scene];
     const unsigned id = sphere_id[g.oid[i]];
+    // history slots of the listed contacts; a restored contact keeps its
+    // entry's slot or takes a free one
+    unsigned long long used = 0;
+    for (int k = 0; k < cnt; ++k) {
+        const size_t x = eidx(h, k, ki, ld, i);
+        if (ell.ent[x] & F_CONTACT) used |= 1ull << ell.slot[x];
+    }
     for (int k = 0; k < cnt; ++k) {
         const size_t x = eidx(h, k, ki, ld, i);
         const unsigned ent = ell.ent[x];
@@ -1621,10 +1673,13 @@ __global__ void k_restore_history(Geo g, Ell ell, int b, int e, int ld, int ki, 
         const unsigned f = (R.flags & F_STATUS) | (ent & F_SLAVE) | F_NOBACKUP;
         ell.ent[x] = j | f;
         const int law = laws[S.law_off + mi.group * S.n_groups + gj].law;
+        if (!(f & F_CONTACT)) continue;   // history words are read only for contacts
         Hist hs;
-        hs.s = (f & F_CONTACT) ? mk(R.shear[0], R.shear[1], R.shear[2]) : mk(0, 0, 0);
+        hs.s = mk(R.shear[0], R.shear[1], R.shear[2]);
         hs.aux = law == LAW_H ? R.kn : (law == LAW_HERTZ ? R.init_vel : 0.0);
-        store_hist(ell, h, k, ki, ld, i, hs);
+        const int sl = (ent & F_CONTACT) ? (int)ell.slot[x] : take_slot(used);
+        ell.slot[x] = (unsigned char)sl;
+        store_hist(ell, sl, ld, i, hs);
     }
 }
 
@@ -1812,7 +1867,7 @@ __global__ void k_export_active(Geo g, Ell ell, int b, int e, int ld, int ki, co
         Hist hs;
         hs.s = mk(0, 0, 0);
         hs.aux = 0.0;
-        if (k >= 0 && (ent & F_CONTACT)) load_hist(ell, h, k, ki, n, i, hs);
+        if (k >= 0 && (ent & F_CONTACT)) load_hist(ell, ell.slot[eidx(h, k, ki, n, i)], n, i, hs);
         A.hist[0] = hs.s.x; A.hist[1] = hs.s.y; A.hist[2] = hs.s.z; A.hist[3] = hs.aux;
         const bool stored = rec && k >= 0 && ((ent & F_CONTACT) || !no_bond);
         for (int r = 0; r < 13; ++r) A.rec[r] = stored ? rec[((size_t)k * 13 + r) * n + i] : 0.0;

@@ -393,9 +393,9 @@ def _check_lists(obj):
     out = (C.c_longlong * 11)()
     obj._check(obj.L.vg_check_lists(obj.h, out))
     v = list(out)
-    names = ("model", "scene", "neighbour_count", "neighbour_entry", "unused", "candidate_count",
+    names = ("model", "scene", "neighbour_count", "neighbour_entry", "history_slot", "candidate_count",
              "candidate_entry", "constrained_slot")
-    return {"violations": {names[k]: v[k] for k in range(8) if k != 4}, "first": v[8:11]}
+    return {"violations": {names[k]: v[k] for k in range(8)}, "first": v[8:11]}
 
 
 SCENE_FREE, SCENE_QUEUED, SCENE_RUNNING, SCENE_FINISHED, SCENE_RETIRING, SCENE_DONE, SCENE_FAILED = range(7)

@@ -1716,6 +1716,9 @@ static StepArgs step_args(vg_scene *h)
     a.no_bond = h->no_bond;
     a.group0 = 0;
     a.law_ss = 0;
+    a.any_fext = 0;
+    for (const ModelDev &m : h->models_h)
+        if (m.fext[0] != 0.0 || m.fext[1] != 0.0 || m.fext[2] != 0.0) a.any_fext = 1;
     if (uniform_scene(h)) {
         const SceneHost &S = h->sc[0];
         a.group0 = S.group0;
@@ -1833,7 +1836,7 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
     StepArgs a = step_args(h);
     const bool kt = 2 * (h->kev_used + 1) <= (int)h->kev.size();
     if (kt) HIPCHK(hipEventRecord(h->kev[2 * h->kev_used], h->stream));
-    const dim3 grid(nblk(std::max(h->n, h->scap), 256)), blk(256);
+    const dim3 grid(nblk(std::max(h->n, h->scap), STEP_NT)), blk(STEP_NT);
     // kernel specialisation: law set of the placed scenes, record path, precision
     const int ls = (h->holes || h->pbc) ? LS_EXT : (h->sd_only ? LS_SD : (h->h_only ? LS_H : LS_ALL));
     const bool mix = h->precision == VG_PRECISION_MIXED;

@@ -86,6 +86,7 @@ struct StepArgs {
     int single_group;        // every scene's spheres share one group: no partner group lookup
     int no_bond;             // no bonded law: surely separated pairs skip the contact geometry
     int group0, law_ss;      // one scene whose spheres share one group: that group and its sphere-sphere law
+    int any_fext;            // a sphere model has an external force
     // constrained walls: a sphere's constrained walls and their pair slots [2][LD],
     // each pair's master impulse and the step it was solved in
     const int *cw_w, *cw_p;
@@ -100,6 +101,17 @@ struct StepArgs {
 };
 
 enum { ERR_INNER_CAP = 1, ERR_OUTER_CAP = 2, ERR_CW = 4 };
+
+// Step kernel launch shape: threads per workgroup (spheres per tile) and the
+// workgroups per CU that its registers and LDS allow.
+#ifndef VG_STEP_NT
+#define VG_STEP_NT 256
+#endif
+#ifndef VG_STEP_WG
+#define VG_STEP_WG (768 / VG_STEP_NT)
+#endif
+constexpr int STEP_NT = VG_STEP_NT, STEP_WG_PER_CU = VG_STEP_WG;
+static_assert(STEP_NT % 64 == 0 && STEP_NT <= 256, "the step kernel's block is 1 to 4 wavefronts");
 
 // Kinematic prescription of a wall: rigid rotation about axis (c, a).
 struct WallKin {

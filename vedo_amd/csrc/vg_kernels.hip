@@ -31,10 +31,13 @@ __device__ inline size_t eidx(int h, int k, int ki, int n, int i)
     return ((size_t)(h * ki + k)) * (size_t)n + (size_t)i;
 }
 // Streaming accesses: data only the owning sphere touches (its orientation,
-// list entries, histories, next-step state) bypass the caches' retention so
-// that L2 keeps the neighbour positions and velocities many threads gather.
+// list entries, next-step state) bypass the caches' retention so that L2
+// keeps the neighbour positions and velocities many threads gather (the
+// next step reads the new state from memory in any case: it does not fit
+// L2).  Level 0: plain accesses; 1: streaming loads and list stores; 2: also
+// the next-step state; 3: also the contact histories.
 #ifndef VG_NT
-#define VG_NT 0
+#define VG_NT 2
 #endif
 template <class T> __device__ inline T nt_load(const T *p)
 {
@@ -46,8 +49,6 @@ template <class T> __device__ inline void nt_store(T v, T *p)
     if (VG_NT >= 1) __builtin_nontemporal_store(v, p);
     else *p = v;
 }
-// next-step state: plain stores keep it in the memory-side cache for the
-// next step's reads unless VG_NT >= 2
 typedef double v4d __attribute__((ext_vector_type(4)));
 __device__ inline void nt_store4(double4 v, double4 *p)
 {
@@ -63,11 +64,25 @@ template <class T> __device__ inline void st_store(T v, T *p)
 // History in slot `sl` of sphere i: the shear force always, the fourth
 // word only where a Hertz law may have set it (AUX; without it the word is
 // read as zero and written as zero).
+typedef double v2d __attribute__((ext_vector_type(2)));
+__device__ inline double2 hist_ld(const double2 *p)
+{
+    if (VG_NT >= 3) {
+        const v2d x = __builtin_nontemporal_load(reinterpret_cast<const v2d *>(p));
+        return make_double2(x.x, x.y);
+    }
+    return *p;
+}
+__device__ inline void hist_st(double2 v, double2 *p)
+{
+    if (VG_NT >= 3) __builtin_nontemporal_store(*reinterpret_cast<const v2d *>(&v), reinterpret_cast<v2d *>(p));
+    else *p = v;
+}
 template <bool AUX = true>
 __device__ inline void load_hist(const Ell &e, int sl, int n, int i, Hist &hs)
 {
     const double2 *p = reinterpret_cast<const double2 *>(e.hd + (size_t)sl * (size_t)n + (size_t)i);
-    const double2 u = p[0], v = p[1];
+    const double2 u = hist_ld(p), v = hist_ld(p + 1);
     hs.s = mk(u.x, u.y, v.x);
     hs.aux = AUX ? v.y : 0.0;
 }
@@ -75,8 +90,8 @@ template <bool AUX = true>
 __device__ inline void store_hist(const Ell &e, int sl, int n, int i, const Hist &hs)
 {
     double2 *p = reinterpret_cast<double2 *>(e.hd + (size_t)sl * (size_t)n + (size_t)i);
-    p[0] = make_double2(hs.s.x, hs.s.y);
-    p[1] = make_double2(hs.s.z, AUX ? hs.aux : 0.0);
+    hist_st(make_double2(hs.s.x, hs.s.y), p);
+    hist_st(make_double2(hs.s.z, AUX ? hs.aux : 0.0), p + 1);
 }
 // The lowest history slot not in `used`, which then holds it.  A sphere's
 // slots in use never outnumber its list entries (at most 64).
@@ -305,8 +320,9 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsig
     return slave_contact;
 }
 
-// Exclusive prefix sum of v over the block's 256 threads; *total gets the sum.
-__device__ inline int block_scan256(int v, int *s_wsum, int &total)
+// Exclusive prefix sum of v over the block's NT threads; *total gets the sum.
+template <int NT>
+__device__ inline int block_scan(int v, int *s_wsum, int &total)
 {
     const int lane = __lane_id(), w = threadIdx.x >> 6;
     int x = v;
@@ -319,7 +335,9 @@ __device__ inline int block_scan256(int v, int *s_wsum, int &total)
     __syncthreads();
     int base = 0;
     for (int q = 0; q < w; ++q) base += s_wsum[q];
-    total = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+    total = 0;
+#pragma unroll
+    for (int q = 0; q < NT / 64; ++q) total += s_wsum[q];
     __syncthreads();
     return base + x - v;
 }
@@ -327,9 +345,9 @@ __device__ inline int block_scan256(int v, int *s_wsum, int &total)
 // 3 workgroups per CU (12 wavefronts), as the LDS allows: the register
 // budget is capped at 168 VGPRs, the larger law sets spill a few registers
 template <bool REC, int LS, bool MIX, bool UNI>
-__global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
+__global__ __launch_bounds__(STEP_NT, STEP_WG_PER_CU) void k_step(StepArgs a)
 {
-    constexpr int NT = 256;   // threads per block: the launch's block size
+    constexpr int NT = STEP_NT;   // threads per block: the launch's block size
     constexpr int QPT = 8;    // queued entries per thread and round of the queue
     // the block's spheres (partners inside the block are read from here) and
     // its queue of pair solves (item, entry word, history slot), one per
@@ -341,7 +359,7 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
     __shared__ unsigned s_qent[NT * QPT];
     __shared__ unsigned char s_qslot[NT * QPT];
     __shared__ double s_out[6][NT];
-    __shared__ int s_wsum[4];
+    __shared__ int s_wsum[NT / 64];
     __shared__ unsigned s_ncontact;
     static_assert(QPT * NT * 5 <= sizeof(s_out), "the staging fits the result area");
     unsigned *const s_stage = reinterpret_cast<unsigned *>(&s_out[0][0]);                   // [QPT][NT]
@@ -370,8 +388,8 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
     s_avl[t] = w4;
     s_mdl[t] = mdl_i;
     if (t == 0) s_ncontact = 0;
-    const ModelDev &md = a.t.models[mdl_i];
-    const int sc = md.scene;
+    // UNI: the scene is slot 0 (its row is read through uniform addresses)
+    const int sc = UNI ? 0 : a.t.models[mdl_i].scene;
     const SceneDev &S = a.scenes[sc];
     const bool live = in && S.active > 0;
     if (in && !live) {
@@ -406,51 +424,56 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
     unsigned long long todo = 0;
     unsigned long long used = 0;   // history slots of the entries in contact
     int nstage = 0;
-    // entries in groups of PF: the entry loads, then the partner-position
-    // gathers of a group are independent and overlap in flight
-    constexpr int PF = 8;
-    for (int k0 = 0; k0 < cnt; k0 += PF) {
-        unsigned ent[PF];
-        unsigned char esl[PF];
+    // entries PE at a time (their words and slots in one round of loads),
+    // then in groups of PF: the partner-position gathers of a group are
+    // independent and overlap in flight
+    constexpr int PE = 12, PF = 6;
+    for (int k1 = 0; k1 < cnt; k1 += PE) {
+        unsigned ew[PE];
+        unsigned char es[PE];
 #pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            ent[u] = k0 + u < cnt ? nt_load(a.ell.ent + eidx(h, k0 + u, a.ki, a.ld, i)) : PARTNER_MASK;
-            esl[u] = k0 + u < cnt ? a.ell.slot[eidx(h, k0 + u, a.ki, a.ld, i)] : 0;
-        }
-        double4 pj[PF];
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const unsigned j = ent[u] & PARTNER_MASK;
-            pj[u] = (a.no_bond && j < (unsigned)a.ld) ? (j - (unsigned)b0 < nb ? s_pos[j - b0] : a.in.pos[j])
-                                                       : make_double4(0.0, 0.0, 0.0, -1.0);
+        for (int u = 0; u < PE; ++u) {
+            ew[u] = k1 + u < cnt ? nt_load(a.ell.ent + eidx(h, k1 + u, a.ki, a.ld, i)) : PARTNER_MASK;
+            es[u] = k1 + u < cnt ? a.ell.slot[eidx(h, k1 + u, a.ki, a.ld, i)] : 0;
         }
 #pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const int k = k0 + u;
-            if (k >= cnt) break;
-            const unsigned j = ent[u] & PARTNER_MASK;
-            if (ent[u] & F_CONTACT) used |= 1ull << esl[u];
-            bool full = true;
-            if (pj[u].w >= 0.0) {   // a sphere partner, unbonded laws
-                d3 dx = ld3(pj[u]) - xi;
-                if constexpr (LS == LS_EXT) dx = bc_diff(a.pbc, dx);
-                const double rs = ri + pj[u].w;
-                if (dot(dx, dx) > rs * rs * (1.0 + 1e-9)) {
-                    full = false;
-                    // Backup, then Clean() of an inactive pair (ImpactStatus.cpp:259-274)
-                    // (its record is all zero: readers take a record without
-                    // Contact, in a scene without bonds, as zero)
-                    const unsigned f = ent[u] & ~PARTNER_MASK & ~F_STATUS & ~F_NOBACKUP;
-                    if (f != (ent[u] & ~PARTNER_MASK)) nt_store(j | f, a.ell.ent + eidx(h, k, a.ki, a.ld, i));
-                }
+        for (int g = 0; g < PE; g += PF) {
+            if (k1 + g >= cnt) break;
+            double4 pj[PF];
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const unsigned j = ew[g + u] & PARTNER_MASK;
+                pj[u] = (a.no_bond && j < (unsigned)a.ld) ? (j - (unsigned)b0 < nb ? s_pos[j - b0] : a.in.pos[j])
+                                                           : make_double4(0.0, 0.0, 0.0, -1.0);
             }
-            if (full) {
-                todo |= 1ull << k;
-                if (nstage < QPT) {
-                    s_stage[nstage * NT + t] = ent[u];
-                    s_stage_slot[nstage * NT + t] = esl[u];
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int k = k1 + g + u;
+                if (k >= cnt) break;
+                const unsigned e = ew[g + u], j = e & PARTNER_MASK;
+                if (e & F_CONTACT) used |= 1ull << es[g + u];
+                bool full = true;
+                if (pj[u].w >= 0.0) {   // a sphere partner, unbonded laws
+                    d3 dx = ld3(pj[u]) - xi;
+                    if constexpr (LS == LS_EXT) dx = bc_diff(a.pbc, dx);
+                    const double rs = ri + pj[u].w;
+                    if (dot(dx, dx) > rs * rs * (1.0 + 1e-9)) {
+                        full = false;
+                        // Backup, then Clean() of an inactive pair (ImpactStatus.cpp:259-274)
+                        // (its record is all zero: readers take a record without
+                        // Contact, in a scene without bonds, as zero)
+                        const unsigned f = e & ~PARTNER_MASK & ~F_STATUS & ~F_NOBACKUP;
+                        if (f != (e & ~PARTNER_MASK)) nt_store(j | f, a.ell.ent + eidx(h, k, a.ki, a.ld, i));
+                    }
                 }
-                ++nstage;
+                if (full) {
+                    todo |= 1ull << k;
+                    if (nstage < QPT) {
+                        s_stage[nstage * NT + t] = e;
+                        s_stage_slot[nstage * NT + t] = es[g + u];
+                    }
+                    ++nstage;
+                }
             }
         }
     }
@@ -468,7 +491,7 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
         for (int q = __popcll(take); q > QPT; --q) take &= ~(1ull << (63 - __builtin_clzll(take)));
         const int c = __popcll(take);
         int total = 0;
-        const int off = block_scan256(c, s_wsum, total);
+        const int off = block_scan<NT>(c, s_wsum, total);
         {
             // the first round takes the thread's first queued entries, whose
             // words and slots pass 1 staged in order; later rounds read them
@@ -534,13 +557,17 @@ __global__ __launch_bounds__(256, 3) void k_step(StepArgs a)
         const double4 p4b = s_pos[t], v4b = s_vel[t], w4b = s_avl[t];
         const d3 xi = ld3(p4b), vi = ld3(v4b), wi = ld3(w4b);
         const double ri = p4b.w, mi = w4b.w;
-        const ModelDev &md = a.t.models[s_mdl[t]];
-        const SceneDev &S = a.scenes[md.scene];
-        const int sc = md.scene;
-        // DOContainer::AddFieldImpact + AddConstrainedImpact (DOContainer.cpp:20-50)
+        const int sc = UNI ? 0 : a.t.models[s_mdl[t]].scene;
+        const SceneDev &S = a.scenes[sc];
+        // DOContainer::AddFieldImpact + AddConstrainedImpact (DOContainer.cpp:20-50).
+        // Where no model has an external force the second term adds +0 to a
+        // sum that is never -0 (it starts at +0), which leaves it unchanged.
         const d3 gdt = muls(mk(S.gx, S.gy, S.gz), S.dt);
         I = I + muls(gdt, mi);
-        I = I + smul(S.dt, mk(md.fext[0], md.fext[1], md.fext[2]));
+        if (!UNI || a.any_fext) {
+            const ModelDev &md = a.t.models[s_mdl[t]];
+            I = I + smul(S.dt, mk(md.fext[0], md.fext[1], md.fext[2]));
+        }
         // DOSphere::Response (DOSphere.cpp:21-48)
         const d3 dv = smul(1.0 / mi, I);
         const d3 dav = smul(2.5 / mi / ri / ri, L);

@@ -1302,7 +1302,7 @@ static int rebuild(vg_scene *h)
         hipLaunchKernelGGL(k_fill_int, dim3(nblk(cells, 256)), dim3(256), 0, st, h->cend, cells, 0);
     }
     if (n_total > 0) {
-        // stable sort by (scene slot, Morton cell key): ranks are in DO-id order
+        // stable sort by (scene slot, Hilbert cell key): ranks are in DO-id order
         // within each scene, so each cell's spheres stay in id order
         int slot_bits = 0;
         while ((1 << slot_bits) <= max_slot + 1) ++slot_bits;

@@ -434,7 +434,7 @@ __global__ __launch_bounds__(STEP_NT, STEP_WG_PER_CU) void k_step(StepArgs a)
 #pragma unroll
         for (int u = 0; u < PE; ++u) {
             ew[u] = k1 + u < cnt ? nt_load(a.ell.ent + eidx(h, k1 + u, a.ki, a.ld, i)) : PARTNER_MASK;
-            es[u] = k1 + u < cnt ? a.ell.slot[eidx(h, k1 + u, a.ki, a.ld, i)] : 0;
+            es[u] = k1 + u < cnt ? nt_load(a.ell.slot + eidx(h, k1 + u, a.ki, a.ld, i)) : 0;
         }
 #pragma unroll
         for (int g = 0; g < PE; g += PF) {
@@ -949,11 +949,13 @@ __device__ inline long long lin_of(const GridDev &G, long long cx, long long cy,
     return ((cz + 1) * G.ny + (cy + 1)) * G.nx + (cx + 1);
 }
 
-// Memory order of the spheres: Morton (Z-order) of the cell, so that the
-// cells around a sphere sit close in memory; each sphere's list order is
-// independent of it (k_outer orders by the comparator key).  Keys are
-// produced in DO-id order and the sort is stable, so each cell's spheres stay
-// in id order as the reference's per-cell vectors do (LeapConsultant.h:170-188).
+// Memory order of the spheres: the Hilbert curve through the cells, so that
+// the cells around a sphere sit close in memory (consecutive cells of the
+// curve are always face neighbours, where the Morton order jumps at every
+// power-of-two boundary); each sphere's list order is independent of it
+// (k_outer orders by the comparator key).  Keys are produced in DO-id order
+// and the sort is stable, so each cell's spheres stay in id order as the
+// reference's per-cell vectors do (LeapConsultant.h:170-188).
 __device__ inline unsigned long long spread3(unsigned v)
 {
     unsigned long long x = v & 0x1ffffull;   // 17 bits per axis (checked on the host)
@@ -964,10 +966,41 @@ __device__ inline unsigned long long spread3(unsigned v)
     x = (x | (x << 2)) & 0x1249249249249249ull;
     return x;
 }
+// Hilbert index of cell (x, y, z), 17 bits per axis: J. Skilling's transform
+// of the axes to the "transposed" index ("Programming the Hilbert curve",
+// AIP Conf. Proc. 707, 381 (2004)), whose bits interleave to the index.
+__device__ inline unsigned long long hilbert3(unsigned x, unsigned y, unsigned z)
+{
+    unsigned X[3] = {x, y, z};
+    constexpr unsigned M = 1u << 16;
+    for (unsigned Q = M; Q > 1; Q >>= 1) {   // inverse undo
+        const unsigned P = Q - 1;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const unsigned t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];   // Gray encode
+    X[2] ^= X[1];
+    unsigned t = 0;
+    for (unsigned Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    X[0] ^= t;
+    X[1] ^= t;
+    X[2] ^= t;
+    return (spread3(X[0]) << 2) | (spread3(X[1]) << 1) | spread3(X[2]);
+}
+
 __global__ void k_cellkey(Geo g, int n_ranks, const SceneDev *scenes, const ModelDev *models, const unsigned *inv,
                           unsigned long long retired_key, unsigned long long *keys, unsigned *vals)
 {
-    // key = scene slot (bits 51..63) | Morton code of the cell (17 bits per axis);
+    // key = scene slot (bits 51..63) | Hilbert index of the cell (17 bits per axis);
     // ranks run scene by scene in DO-id order, retired scenes last
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_ranks) return;
@@ -982,7 +1015,7 @@ __global__ void k_cellkey(Geo g, int n_ranks, const SceneDev *scenes, const Mode
     const double4 p = g.pos[i];
     unsigned cx, cy, cz;
     cell_of(S.G, mk(p.x, p.y, p.z), cx, cy, cz);
-    keys[t] = ((unsigned long long)sc << 51) | spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+    keys[t] = ((unsigned long long)sc << 51) | hilbert3(cx, cy, cz);
 }
 
 // Gather state into cell order; per-sphere safe distance

@@ -230,7 +230,11 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsig
     ContactM cm;
     d3 xa, va, wa, xb, vb, wb;
     double ma, mb;
-    LawDev law;
+    // the pair's IactModel: one-scene handles read its fields at once, from a
+    // uniform address for sphere pairs; batches keep it in memory until the
+    // solver reads each field (fewer registers in the larger kernels)
+    [[maybe_unused]] LawDev lawv;
+    const LawDev *lawp = nullptr;
     bool owner_a = true;
     const WallDev *w = nullptr;
     if (j < (unsigned)a.ld) {
@@ -238,11 +242,11 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsig
         const double4 pj = loc ? s_pos[j - b0] : a.in.pos[j];
         const double4 vj = loc ? s_vel[j - b0] : a.in.vel[j];
         const double4 wj = loc ? s_avl[j - b0] : a.in.avl[j];
-        if (UNI) {
-            law = a.t.laws[a.law_ss];
+        if constexpr (UNI) {
+            lawv = a.t.laws[a.law_ss];
         } else {
             const int gj = a.single_group ? gi : a.t.models[a.in.mdl[j]].group;
-            law = a.t.laws[S.law_off + gi * S.n_groups + gj];
+            lawp = &a.t.laws[S.law_off + gi * S.n_groups + gj];
         }
         owner_a = (hs.f & F_SLAVE) != 0;
         const d3 xj = ld3(pj), vjj = ld3(vj), wjj = ld3(wj);
@@ -264,7 +268,8 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsig
         }
     } else {
         w = &a.walls[j - a.ld];
-        law = a.t.laws[S.law_off + gi * S.n_groups + w->group];
+        if constexpr (UNI) lawv = a.t.laws[S.law_off + gi * S.n_groups + w->group];
+        else lawp = &a.t.laws[S.law_off + gi * S.n_groups + w->group];
         cd_wall_at<LS == LS_EXT>(xi, ri, *w, a.holes, (int)(j - a.ld), c);
         xa = xi; va = vi; wa = wi; ma = mi;
         xb = w->pos; vb = w->vel; wb = w->av; mb = w->msudo;
@@ -274,6 +279,7 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsig
             if (MIX) to_mixed(c, xi, w->pos, cm);
         }
     }
+    const LawDev &law = UNI ? lawv : *lawp;
     if (MIX) solve_mixed<REC, LS>(law, ma, mb, xa, va, wa, vb, wb, cm, S.dt, hs, o, r);
     else if constexpr (LS == LS_EXT) solve<REC, LS>(law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r, a.pbc);
     else solve<REC, LS>(law, ma, mb, xa, va, wa, xb, vb, wb, c, S.dt, hs, o, r);

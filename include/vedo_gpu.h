@@ -329,6 +329,13 @@ int vg_scene_set_wall_rotation(vg_scene *h, int scene, int n, const unsigned *id
                                const double *axis_dir, double omega, double t_ref);
 int vg_batch_info(vg_scene *h, long long *spheres, int *scenes_on_device, int *scenes_queued,
                   unsigned long long *steps);
+/* No counterpart (batch handles): step whole rebuild periods as captured HIP
+ * graphs (on = 1, the default) or launch every step from the host (on = 0).
+ * A period is replayed from a graph when it starts at a rebuild, no scene
+ * reaches its step count inside it and no wall is constrained; results are
+ * the same either way.  vg_graph_replays: periods replayed so far. */
+int vg_set_graphs(vg_scene *h, int on);
+int vg_graph_replays(vg_scene *h, long long *replays);
 
 /* Streaming copy of `bytes` bytes, `iters` times, on the current device;
  * returns the achieved read+write bandwidth in GB/s.  Diagnostics: measures

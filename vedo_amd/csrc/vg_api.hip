@@ -73,10 +73,28 @@ struct SceneHost {
     bool failed = false;                // blew up: left the batch without a result
 };
 
+// A captured rebuild period of a batch handle (graph_period).
+struct StepGraph {
+    std::vector<unsigned char> key;   // launch arguments it was captured with
+    hipGraphExec_t exec = nullptr;
+    unsigned long long last_use = 0;
+};
+
+static void graph_free(StepGraph &g)
+{
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    g.key.clear();
+}
+
 struct vg_scene {
     std::string err;
     hipStream_t stream = nullptr;
     int device = 0;
+    bool use_graphs = true;             // batch handles replay captured rebuild periods (vg_set_graphs)
+    std::vector<StepGraph> graphs = std::vector<StepGraph>(8);
+    unsigned long long graph_clock = 0;
+    long long graph_replays = 0;
     bool batch = false;                 // made by vg_batch_create
     // capacities and sizes
     int ld = 0, wcap = 0, scap = 0, ki = 48;
@@ -1506,6 +1524,7 @@ static void free_scene(vg_scene *h)
     F(h->zoi_dead); F(h->zoi_cnt); F(h->zoi_rank); F(h->zoi_pre);
     if (h->pin) (void)hipHostFree(h->pin);
     for (auto e : h->kev) (void)hipEventDestroy(e);
+    for (StepGraph &g : h->graphs) graph_free(g);
     for (auto e : h->rev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1821,6 +1840,16 @@ static int flush_rebuild(vg_scene *h, bool keep_records)
     return do_rebuild(h);
 }
 
+// The step kernel's law set for the placed scenes: spring-dashpot only,
+// ISwH only, all five, or all five with the extended geometry.
+static int law_set(const vg_scene *h)
+{
+    return (h->holes || h->pbc) ? LS_EXT : (h->sd_only ? LS_SD : (h->h_only ? LS_H : LS_ALL));
+}
+
+static int launch_step(vg_scene *h, bool rec, bool kt);
+static int after_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>> *assign_spheres);
+
 static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>> *assign_spheres)
 {
     if (int rc = flush_rebuild(h, false)) return rc;
@@ -1833,12 +1862,24 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
         if (h->n == 0) return 0;
     }
     rec = rec && h->record;
-    StepArgs a = step_args(h);
     const bool kt = 2 * (h->kev_used + 1) <= (int)h->kev.size();
     if (kt) HIPCHK(hipEventRecord(h->kev[2 * h->kev_used], h->stream));
+    if (int rc = launch_step(h, rec, kt)) return rc;
+    h->cur ^= 1;
+    h->step_count++;
+    h->round++;
+    return after_step(h, rec, assign_spheres);
+}
+
+// The kernels of one step on the handle's stream (k_step, the constrained
+// walls' impulse sums, the wall responses), for the handle's current buffers
+// and step count.
+static int launch_step(vg_scene *h, bool rec, bool kt)
+{
+    StepArgs a = step_args(h);
     const dim3 grid(nblk(std::max(h->n, h->scap), STEP_NT)), blk(STEP_NT);
     // kernel specialisation: law set of the placed scenes, record path, precision
-    const int ls = (h->holes || h->pbc) ? LS_EXT : (h->sd_only ? LS_SD : (h->h_only ? LS_H : LS_ALL));
+    const int ls = law_set(h);
     const bool mix = h->precision == VG_PRECISION_MIXED;
     using KFn = void (*)(StepArgs);
     static const KFn kfn[2][2][4][2] = {
@@ -1873,9 +1914,14 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
                            (const SceneDev *)h->scenes, h->stime, (const double *)h->assigned,
                            (const WallDev *)h->walls_ref, (const WallKin *)h->wkin, (const double *)h->cw_imp,
                            h->rflags, (int)(h->step_count % 6));
-    h->cur ^= 1;
-    h->step_count++;
-    h->round++;
+    return 0;
+}
+
+// The host side of a step after its launches: scene times and step counts,
+// scenes that reach their step count, assigned sphere statuses, and the
+// rebuild at the end of a period.
+static int after_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>> *assign_spheres)
+{
     // SystemParameter::NextStep per scene; a scene that reaches its step
     // count is frozen (its state is held until the next boundary)
     bool frozen = false;
@@ -1922,14 +1968,115 @@ static int one_step(vg_scene *h, bool rec, const std::vector<std::pair<int, int>
     return 0;
 }
 
+// A batch handle steps a whole rebuild period as one HIP graph when nothing
+// between its steps needs the host: the period starts at a rebuild, no
+// scene reaches its step count inside it, no wall is constrained (their
+// per-step stamps), and kernel timing is off.  The launches of the period
+// are captured once per set of launch arguments (buffers, counts, the step
+// count modulo 6) and replayed; the host's per-step bookkeeping and the
+// rebuild follow the replay as they follow eager launches.
+static bool graph_period_ok(const vg_scene *h, int remaining)
+{
+    if (!h->batch || !h->use_graphs || h->n == 0 || h->any_constrained || h->rebuild_pending) return false;
+    if (remaining < (int)h->up_iact || h->round % h->up_iact != 0) return false;
+    if (2 * (h->kev_used + 1) <= (int)h->kev.size()) return false;
+    for (const SceneHost &S : h->sc)
+        if (S.state == VG_SCENE_RUNNING && S.steps_target >= 0 && S.steps_target - S.steps_done < (long long)h->up_iact)
+            return false;
+    return true;
+}
+
+static int graph_period(vg_scene *h)
+{
+    const unsigned U = h->up_iact;
+    // the launch arguments of the period's first step determine all of them
+    StepArgs a0 = step_args(h);
+    a0.stamp = 0;
+    // with the kernel specialisation launch_step chooses (law set, precision, one-scene path)
+    const long long extra[6] = {(long long)(h->step_count % 6), h->nw_used, (long long)U, h->precision,
+                                (long long)uniform_scene(h), law_set(h)};
+    std::vector<unsigned char> key(sizeof a0 + sizeof extra);
+    std::memcpy(key.data(), &a0, sizeof a0);
+    std::memcpy(key.data() + sizeof a0, extra, sizeof extra);
+    StepGraph *g = nullptr;
+    for (StepGraph &c : h->graphs)
+        if (c.exec && c.key == key) g = &c;
+    if (!g) {
+        // least recently used slot
+        g = &h->graphs[0];
+        for (StepGraph &c : h->graphs)
+            if (!c.exec || c.last_use < g->last_use) g = &c;
+        graph_free(*g);
+        const int cur = h->cur;
+        const unsigned long long step_count = h->step_count, round = h->round;
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int rc = 0;
+        for (unsigned k = 0; k < U && !rc; ++k) {
+            rc = launch_step(h, false, false);
+            h->cur ^= 1;
+            h->step_count++;
+            h->round++;
+        }
+        h->cur = cur;
+        h->step_count = step_count;
+        h->round = round;
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+        if (rc || e != hipSuccess || !graph) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc ? rc : fail(h, VG_EHIP, "capturing the step launches failed");
+        }
+        const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            g->exec = nullptr;
+            return fail(h, VG_EHIP, "instantiating the step graph failed");
+        }
+        g->key = key;
+    }
+    g->last_use = ++h->graph_clock;
+    HIPCHK(hipGraphLaunch(g->exec, h->stream));
+    h->graph_replays++;
+    h->last_launches += (int)U;
+    for (unsigned k = 0; k < U; ++k) {
+        h->cur ^= 1;
+        h->step_count++;
+        h->round++;
+        if (int rc = after_step(h, false, nullptr)) return rc;
+    }
+    return 0;
+}
+
+extern "C" int vg_set_graphs(vg_scene *h, int on)
+{
+    if (!h) return VG_EINVAL;
+    h->use_graphs = on != 0;
+    if (!h->use_graphs)
+        for (StepGraph &g : h->graphs) graph_free(g);
+    return 0;
+}
+
+extern "C" int vg_graph_replays(vg_scene *h, long long *replays)
+{
+    if (!h || !replays) return VG_EINVAL;
+    *replays = h->graph_replays;
+    return 0;
+}
+
 extern "C" int vg_step(vg_scene *h, int nsteps)
 {
     if (!h || nsteps < 0) return VG_EINVAL;
     h->last_launches = 0;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    for (int k = 0; k < nsteps; ++k) {
+    for (int k = 0; k < nsteps;) {
+        if (graph_period_ok(h, nsteps - k)) {
+            if (int rc = graph_period(h)) return rc;
+            k += (int)h->up_iact;
+            continue;
+        }
         int rc = one_step(h, k == nsteps - 1, nullptr);
         if (rc) return rc;
+        ++k;
     }
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     h->timing = true;

@@ -70,7 +70,7 @@ EXPORTS = ("vg_create", "vg_destroy", "vg_last_error", "vg_create_error", "vg_se
            "vg_num_interactions", "vg_scene_num_do", "vg_scene_models", "vg_set_skin", "vg_set_skin_auto", "vg_get_skin", "vg_set_precision", "vg_get_precision", "vg_get_active", "vg_kernel_timing", "vg_step_timing", "vg_rebuild_timing", "vg_check_lists", "vg_stats", "vg_copy_bench",
            "vg_system_energy", "vg_batch_create", "vg_admit", "vg_retire", "vg_scene_status",
            "vg_scene_get_state", "vg_scene_get_active", "vg_scene_energy", "vg_scene_set_wall_rotation",
-           "vg_batch_info", "vg_scene_coordination_number", "vg_scene_packing_density",
+           "vg_batch_info", "vg_set_graphs", "vg_graph_replays", "vg_scene_coordination_number", "vg_scene_packing_density",
            "vg_scene_packing_density_noalloc", "vg_scene_granular_temperature")
 
 _lib = None
@@ -147,6 +147,8 @@ def lib():
         L.vg_scene_get_active.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), _U, _U, C.POINTER(VgStatus)]
         L.vg_scene_energy.argtypes = [vp, C.c_int, _D]
         L.vg_scene_set_wall_rotation.argtypes = [vp, C.c_int, C.c_int, _U, _D, _D, C.c_double, C.c_double]
+        L.vg_set_graphs.argtypes = [vp, C.c_int]
+        L.vg_graph_replays.argtypes = [vp, C.POINTER(C.c_longlong)]
         L.vg_batch_info.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                     C.POINTER(C.c_ulonglong)]
         _lib = L
@@ -508,6 +510,16 @@ class GpuBatch(_Knight):
 
     def set_precision(self, mode: int):
         self._check(self.L.vg_set_precision(self.h, int(mode)))
+
+    def set_graphs(self, on: bool):
+        """Step whole rebuild periods as captured HIP graphs (default) or
+        launch every step from the host (vg_set_graphs)."""
+        self._check(self.L.vg_set_graphs(self.h, 1 if on else 0))
+
+    def graph_replays(self) -> int:
+        n = C.c_longlong()
+        self._check(self.L.vg_graph_replays(self.h, C.byref(n)))
+        return n.value
 
     def info(self):
         sp, on, q, st = C.c_longlong(), C.c_int(), C.c_int(), C.c_ulonglong()

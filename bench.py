@@ -169,22 +169,28 @@ def batched_boxes(count, steps, up_iact, seed=4, loop=True):
     # skin as the floor): the spheres fall faster as the run goes on
     skin_cap = max(4.0 * skin, 1e-3)
     b.set_skin_auto(skin, skin_cap)
-    periods = 4 * up_iact // up_iact
+    from vedo_amd.gpu import SCENE_RUNNING
+    done_steps = sum(up_iact + 2 for _ in tune)   # the tuning's steps
     while True:
-        b.step(up_iact)
-        periods += 1
-        states = [b.status(s)[0] for s in ids]
-        if periods % 5 == 0:
-            log(f"[bench] batched: {periods * up_iact} steps, {sum(x == SCENE_DONE for x in states)} done, "
-                f"{sum(x == SCENE_FAILED for x in states)} failed")
+        st = [b.status(s) for s in ids]
+        states = [x[0] for x in st]
         if all(x in (SCENE_DONE, SCENE_FAILED) for x in states):
             break
+        # on to the nearest scene's end in one call (whole periods, which the
+        # batch replays as graphs): no host round trip between periods
+        rem = [steps - x[1] for x in st if x[0] == SCENE_RUNNING]
+        chunk = max(up_iact, (min(rem) // up_iact) * up_iact) if rem else up_iact
+        b.step(chunk)
+        done_steps += chunk
+        log(f"[bench] batched: {done_steps} steps, {sum(x == SCENE_DONE for x in states)} done, "
+            f"{sum(x == SCENE_FAILED for x in states)} failed")
     t_run = time.perf_counter() - t0
     ok = [s for s, x in zip(ids, states) if x == SCENE_DONE]
     res = [b.state(s) for s in ok]
     t_batch = time.perf_counter() - t0
     log(f"[bench] batched: admission {t_adm:.2f} s, stepping {t_run - t_adm:.2f} s, results {t_batch - t_run:.2f} s")
     stats = b.stats()
+    graph_replays = b.graph_replays()
     b.close()
     del res
     failed = [k for k, x in enumerate(states) if x == SCENE_FAILED]
@@ -194,7 +200,8 @@ def batched_boxes(count, steps, up_iact, seed=4, loop=True):
     out = {"scenes": count, "spheres": n_tot, "steps_per_scene": steps, "up_iact": up_iact,
            "failed_scenes": len(failed), "failed": [[k, n_of[k]] for k in failed],
            "batched_seconds": t_batch, "batched_particle_steps_per_s": ps / t_batch,
-           "batch_steps": periods * up_iact, "relists": stats["relists"], "skin_m": skin}
+           "batch_steps": done_steps, "relists": stats["relists"], "skin_m": skin,
+           "graph_replays": graph_replays}
     log(f"[bench] batched: {t_batch:.2f} s, {ps / t_batch:.3e} particle-steps/s ({len(failed)} scenes blew up: "
         f"soft ISwH contacts)")
     if loop:

@@ -333,7 +333,9 @@ int vg_batch_info(vg_scene *h, long long *spheres, int *scenes_on_device, int *s
  * graphs (on = 1, the default) or launch every step from the host (on = 0).
  * A period is replayed from a graph when it starts at a rebuild, no scene
  * reaches its step count inside it and no wall is constrained; results are
- * the same either way.  vg_graph_replays: periods replayed so far. */
+ * the same either way.  vg_graph_replays: periods replayed so far.  The
+ * environment variable VG_GRAPHS=0 makes off the default of new batch
+ * handles (for tools that trace each kernel launch). */
 int vg_set_graphs(vg_scene *h, int on);
 int vg_graph_replays(vg_scene *h, long long *replays);
 

@@ -1681,6 +1681,9 @@ extern "C" int vg_batch_create(const vg_batch_desc *bd, vg_scene **out)
     vg_scene *h = new vg_scene();
     h->batch = true;
     h->up_iact = bd->up_iact ? bd->up_iact : 1;
+    // VG_GRAPHS=0: every step launched from the host (for tools that trace
+    // kernel launches and do not follow graph replays)
+    if (const char *g = getenv("VG_GRAPHS")) h->use_graphs = atoi(g) != 0;
     // large layouts: start the candidate capacity high, since growing
     // reallocates [KO][LD] and stalls the device for seconds at 1e7 slots
     h->ko = 512;

@@ -39,9 +39,11 @@ METRIC = "particle-steps/s @1M spheres vs plain GPU; p50/p99 scene latency batch
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per step (fixed here, used by the roofline):
 #   280 B per sphere  = FP64 x,v,w read+write (144) + FP64 Ox,Oz read+write (96)
-#                       + radius/model read (8) + geometry mirror write+read (32)
+#                       + radius, mass and model read (20) + the relist guard's
+#                       path length read+write (16), rounded to 280
 #   72 B per contact  = (lo,hi) key (8) + FP64 tangential history read+write (48)
 #                       + per-contact scalar state read+write (16)
+# (the neighbour-list words of listed partners not in contact are not counted)
 BYTES_PER_SPHERE = 280
 BYTES_PER_CONTACT = 72
 

@@ -13,7 +13,10 @@ properties that hold at any size:
   DONames) and the exported IactRecordTab continues bitwise as the
   uninterrupted run;
 * every index the step, wall and rebuild kernels take from the device
-  lists is in range (vg_check_lists) every 2,000 steps and at the end.
+  lists is in range (vg_check_lists) every 2,000 steps and at the end;
+* one FP32/FP64-mixed step from the end state (histories carried) against
+  one FP64 step: identical active sets and flags, per-contact and
+  per-sphere errors <= 1e-5 (the configuration's mixed-precision leg).
 
 The zone's lower bound is raised into the funnel, 60 um under the centre of
 the sixth lattice layer (layers 2.2 r_large apart, centres jittered by
@@ -103,5 +106,19 @@ def test_hopper_4m_zone_determinism_restart():
     for sim in (a, c):
         chk = sim.check_lists()
         assert not any(chk["violations"].values()), chk
-    a.close()
     c.close()
+    # the configuration's mixed precision: one FP32/FP64-mixed step against one
+    # FP64 step from this state (contact histories carried), at full size, with
+    # the bars of test_gpu_mixed.py
+    from vedo_amd.precision import compare_precisions
+    s3 = a.state()
+    h3 = a.active()
+    w3 = dataclasses.replace(wa, pos=s3["pos"], vel=s3["vel"], ox=s3["ox"], oz=s3["oz"], av=s3["av"],
+                             model_of=a.models(), t_current=a.time)
+    a.close()
+    r = compare_precisions(w3, steps=1, history=h3)
+    print(f"  mixed vs FP64, one step at {N} spheres: {r}", flush=True)
+    assert r["active_pairs"] >= MIN_CONTACTS
+    assert r["active_sets_identical"] and r["status_flags_identical"], r
+    assert r["contact_err_max"] <= 1e-5, r
+    assert r["sphere_impulse_err_max"] <= 1e-5 and r["sphere_ang_impulse_err_max"] <= 1e-5, r

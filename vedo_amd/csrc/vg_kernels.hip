@@ -2219,10 +2219,28 @@ __global__ __launch_bounds__(EN_BLOCK) void k_energy_final(const double *part, i
 
 // Plain streaming copy used to calibrate HBM counters and measure achievable
 // bandwidth.
-__global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n)
+// Streaming copy (the achievable HBM bandwidth): each workgroup copies
+// chunks of 4 x blockDim 16-byte elements, the four loads of a thread in
+// flight together, streaming loads and stores.
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    constexpr int U = 4;
+    const size_t chunk = (size_t)blockDim.x * U;
+    const v2d *s = reinterpret_cast<const v2d *>(src);
+    v2d *d = reinterpret_cast<v2d *>(dst);
+    for (size_t base = (size_t)blockIdx.x * chunk; base < n; base += (size_t)gridDim.x * chunk) {
+        v2d v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t i = base + (size_t)u * blockDim.x + threadIdx.x;
+            if (i < n) v[u] = __builtin_nontemporal_load(s + i);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t i = base + (size_t)u * blockDim.x + threadIdx.x;
+            if (i < n) __builtin_nontemporal_store(v[u], d + i);
+        }
+    }
 }
 
 }  // namespace vg

@@ -139,6 +139,13 @@ def make_cases():
                          spacing=2.02, jitter=0.05, vmax=0.05, offset=3.0, omega_max=5.0)
     cases.append(("box_piston", piston_world(w).to_arrays(up_iact=7, record=1000), 0, 150,
                   [0, 1, 2, 6, 7, 8, 50, 150], None))
+    # three constrained walls meeting at a corner (the piston and the x-min
+    # and y-min side plates, DOConstrainedQuasiPlate): the corner spheres have
+    # all three among their candidates, and those at the top touch them all
+    w = scenes.box_scene(64, seed=18, law="ISwBSDBF", dt=2e-5, steps=400, kn=2e3, e=0.6, mu=0.4, side=0.0125,
+                         spacing=2.02, jitter=0.05, vmax=0.05, offset=3.0, omega_max=5.0)
+    cases.append(("box_cw3", corner_world(piston_world(w)).to_arrays(up_iact=7, record=1000), 0, 150,
+                  [0, 1, 2, 6, 7, 8, 50, 150], None))
     # a zone of interest: the box has no floor, the spheres move down and
     # the lowest leave the zone while touching the layer above; they are
     # erased at the next rebuilds (Consultant::CleanUp) and the records
@@ -302,6 +309,29 @@ def piston_world(w, density=100.0, force=-0.02, overlap=0.02):
     w.statuses[k] = DOStatus("Piston", (old.position[0], old.position[1], ztop + r + 0.5 * thick - overlap * r),
                              (0.0, 0.0, 0.0), old.orientation_x, old.orientation_z, (0.0, 0.0, 0.0))
     w.models.append(piston)
+    return w
+
+
+def corner_world(w, density=100.0):
+    """The x-min and y-min side plates as constrained walls (no external
+    force), each moved to touch the spheres of its side."""
+    from vedo_amd.world import DOModel, DOStatus
+    sph = [st for st in w.statuses if st.name == "Sphere"]
+    r = [m for m in w.models if m.name == "Sphere"][0].attributes[0]
+    plate = [m for m in w.models if m.name == "Plate"][0]
+    thick = plate.attributes[1]
+    for axis, name in ((0, "WallX"), (1, "WallY")):
+        sides = [k for k, st in enumerate(w.statuses)
+                 if st.name == "Plate" and abs(st.orientation_z[axis]) == 1.0]
+        k = min(sides, key=lambda k: w.statuses[k].position[axis])
+        old = w.statuses[k]
+        lo = min(st.position[axis] for st in sph)
+        p = list(old.position)
+        p[axis] = lo - r - 0.5 * thick + 0.02 * r
+        w.statuses[k] = DOStatus(name, tuple(p), (0.0, 0.0, 0.0), old.orientation_x, old.orientation_z,
+                                 (0.0, 0.0, 0.0))
+        w.models.append(DOModel(name, plate.group, "constrained", "global", density, 1.0, (0.0, 0.0, 0.0),
+                                "QuasiPlate", plate.attributes))
     return w
 
 

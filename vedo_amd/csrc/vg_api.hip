@@ -194,7 +194,7 @@ struct vg_scene {
     unsigned *cw_stamp = nullptr;       // [cw_cap]
     unsigned long long *cw_key[2] = {nullptr, nullptr};
     unsigned *cw_vidx[2] = {nullptr, nullptr};
-    int *cw_w = nullptr, *cw_p = nullptr;    // [2][ld] a sphere's constrained walls, their pair slots
+    int *cw_w = nullptr, *cw_p = nullptr;    // [CW_SLOTS][ld] a sphere's constrained walls, their pair slots
     unsigned *cw_cnt = nullptr;         // [ld + 1] per-sphere pair counts, then offsets
     int *cw_beg = nullptr, *cw_end = nullptr;   // [wcap] pair range of each wall slot
     double *cw_imp = nullptr;           // [wcap][3] each constrained wall's impulse of the step
@@ -777,7 +777,7 @@ static int build_cw_pairs(vg_scene *h, const Geo &G, int n)
     const int ld = h->ld;
     if (h->wcap > (1 << 13)) return fail(h, VG_EUNSUPPORTED, "constrained walls need at most 8192 wall slots");
     if (!h->cw_w) {
-        if (dalloc(h, &h->cw_w, (size_t)2 * ld) || dalloc(h, &h->cw_p, (size_t)2 * ld) ||
+        if (dalloc(h, &h->cw_w, (size_t)CW_SLOTS * ld) || dalloc(h, &h->cw_p, (size_t)CW_SLOTS * ld) ||
             dalloc(h, &h->cw_cnt, (size_t)2 * (ld + 1)))
             return VG_ENOMEM;
     }
@@ -805,7 +805,8 @@ static int build_cw_pairs(vg_scene *h, const Geo &G, int n)
             dalloc(h, &h->cw_key[1], c) || dalloc(h, &h->cw_vidx[0], c) || dalloc(h, &h->cw_vidx[1], c))
             return VG_ENOMEM;
     }
-    hipLaunchKernelGGL(k_fill_int, dim3(nblk(2 * ld, 256)), dim3(256), 0, st, h->cw_w, 2ll * ld, -1);
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk((long long)CW_SLOTS * ld, 256)), dim3(256), 0, st, h->cw_w,
+                       (long long)CW_SLOTS * ld, -1);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(h->wcap, 256)), dim3(256), 0, st, h->cw_beg, (long long)h->wcap, 0);
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(h->wcap, 256)), dim3(256), 0, st, h->cw_end, (long long)h->wcap, 0);
     if (np > 0) {
@@ -2095,7 +2096,7 @@ extern "C" int vg_sync(vg_scene *h)
     if (flags[3] & ERR_INNER_CAP)
         return fail(h, VG_ECAPACITY, "neighbour list capacity exceeded (set VG_NEIGHBOUR_CAPACITY)");
     if (flags[3] & ERR_CW)
-        return fail(h, VG_EUNSUPPORTED, "a sphere is a candidate of more than two constrained walls");
+        return fail(h, VG_EUNSUPPORTED, "a sphere is a candidate of more than four constrained walls");
     if (h->timing) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->last_ms = ms;

@@ -87,7 +87,7 @@ struct StepArgs {
     int no_bond;             // no bonded law: surely separated pairs skip the contact geometry
     int group0, law_ss;      // one scene whose spheres share one group: that group and its sphere-sphere law
     int any_fext;            // a sphere model has an external force
-    // constrained walls: a sphere's constrained walls and their pair slots [2][LD],
+    // constrained walls: a sphere's constrained walls and their pair slots [CW_SLOTS][LD],
     // each pair's master impulse and the step it was solved in
     const int *cw_w, *cw_p;
     double *cw_val;
@@ -101,6 +101,9 @@ struct StepArgs {
 };
 
 enum { ERR_INNER_CAP = 1, ERR_OUTER_CAP = 2, ERR_CW = 4 };
+
+// Constrained walls a sphere may have among its candidates (pair slots per sphere).
+constexpr int CW_SLOTS = 4;
 
 // Step kernel launch shape: threads per workgroup (spheres per tile) and the
 // workgroups per CU that its registers and LDS allow.

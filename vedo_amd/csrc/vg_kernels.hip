@@ -295,8 +295,8 @@ __device__ inline bool solve_entry(const StepArgs &a, int i, int h, int k, unsig
             // the wall's share (DiscreteObject::AddImpact on the master), added in pair order by k_cw_sum
             const int ws = (int)(j - a.ld);
             int p = -1;
-            if (a.cw_w[i] == ws) p = a.cw_p[i];
-            else if (a.cw_w[a.ld + i] == ws) p = a.cw_p[a.ld + i];
+            for (int q = 0; q < CW_SLOTS && p < 0; ++q)
+                if (a.cw_w[(size_t)q * a.ld + i] == ws) p = a.cw_p[(size_t)q * a.ld + i];
             if (p >= 0) {
                 const d3 m = o.linear_only ? -o.imp : smul(-1.0, o.imp);
                 double *v = a.cw_val + 3 * (size_t)p;
@@ -1482,12 +1482,12 @@ __global__ void k_check_lists(Geo g, Ell ell, int n, int ld, int ki, int ko, con
     for (unsigned t = 0; t < min(oc, (unsigned)ko); ++t)
         if (!partner_ok(o_nbr[(size_t)t * ld + i] & PARTNER_MASK)) bad(6, (int)t);
     if (cw_w)
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < CW_SLOTS; ++q) {
             const int w = cw_w[(size_t)q * ld + i];
             if (w == -1) continue;
             if (w < S.w_begin || w >= S.w_end || !walls[w].constrained) bad(7, q);
             const int pq = cw_p[(size_t)q * ld + i];
-            if (pq < 0 || (long long)pq >= cw_cap) bad(7, 2 + q);
+            if (pq < 0 || (long long)pq >= cw_cap) bad(7, CW_SLOTS + q);
         }
 }
 
@@ -1617,9 +1617,9 @@ __global__ void k_cw_count(int n, int ld, int ko, const unsigned *cnt, const uns
         const unsigned j = nbr[(size_t)t * ld + i] & PARTNER_MASK;
         if (j >= (unsigned)ld && walls[j - ld].constrained) ++c;
     }
-    if (c > 2) {   // a sphere near more than two constrained walls
+    if (c > (unsigned)CW_SLOTS) {   // a sphere near more constrained walls than it has pair slots
         atomicOr(&flags[3], ERR_CW);
-        c = 2;
+        c = CW_SLOTS;
     }
     out[i] = c;
 }
@@ -1640,7 +1640,7 @@ __global__ void k_cw_emit(int n, int ld, int ko, const unsigned *cnt, const unsi
         const unsigned j = nbr[(size_t)t * ld + i] & PARTNER_MASK;
         if (j >= (unsigned)ld && walls[j - ld].constrained) {
             key[b + q] = ((unsigned long long)(j - ld) << 51) | ck;
-            val[b + q] = ((unsigned)i << 1) | q;
+            val[b + q] = ((unsigned)i << 2) | q;
             ++q;
         }
     }
@@ -1652,7 +1652,7 @@ __global__ void k_cw_finalize(long long np, const unsigned long long *key, const
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const unsigned v = val[p];
-    const int i = (int)(v >> 1), q = (int)(v & 1u);
+    const int i = (int)(v >> 2), q = (int)(v & 3u);
     const int w = (int)(key[p] >> 51);
     cw_w[(size_t)q * ld + i] = w;
     cw_p[(size_t)q * ld + i] = (int)p;

@@ -270,7 +270,8 @@ typedef struct {
     long long max_spheres;     /* sphere capacity of the device layout */
     int max_walls;             /* wall capacity */
     int max_scenes;            /* scene slots */
-    int neighbour_capacity;    /* neighbour-list entries per sphere (<= 64; 0: 32) */
+    int neighbour_capacity;    /* neighbour-list entries per sphere to start with (<= 127; 0: 32); rebuilds
+                                  grow it, up to 127, when a sphere has more partners within the skin */
     unsigned up_iact;          /* shared UpIact */
 } vg_batch_desc;
 

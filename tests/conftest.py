@@ -39,7 +39,7 @@ def load_golden(name):
 
 GOLDEN_CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_bond", "box_bond_bt", "box_cyl",
                 "box_orbital", "box_nbs", "drum_kin", "box_assign", "box_piston", "box_cw3", "box_zoi", "box_hole",
-                "box_pbc_xy", "box_pbc_xyz", "box_pbc_x", "box_pbc_h", "box_nbs_quirks", "box_pbc_nbs"]
+                "box_pbc_xy", "box_pbc_xyz", "box_pbc_x", "box_pbc_h", "box_nbs_quirks", "box_pbc_nbs", "box_crowd"]
 
 
 def step_golden(sim, z, step):

@@ -89,6 +89,10 @@ def make_cases():
     # floor rising and turning about a tilted axis, a side wall pushing in
     cases.append(("box_orbital", orbital_world().to_arrays(up_iact=7, record=1000), 0, 150,
                   [0, 1, 2, 6, 7, 8, 50, 100, 150], None))
+    # crowded lists: two large spheres each touched by about 100 small ones
+    # (more than 64 entries in one sphere's neighbour list)
+    cases.append(("box_crowd", crowd_world().to_arrays(up_iact=7, record=1000), 0, 150,
+                  [0, 1, 2, 6, 7, 8, 50, 100, 150], None))
     # NBS consultant on the same kind of scene
     w = scenes.box_scene(64, seed=12, law="ISwBSDBF", dt=2e-5, steps=400, kn=2e3, e=0.6, mu=0.4, **common)
     cases.append(("box_nbs", w.to_arrays(up_iact=5, record=1000), 1, 100, [0, 1, 5, 6, 100], None))
@@ -248,6 +252,39 @@ def cyl_world(seed=22, vz=-0.25):
         w.statuses[k] = DOStatus(st.name, st.position, (st.velocity[0], st.velocity[1], st.velocity[2] + vz),
                                  st.orientation_x, st.orientation_z, st.angular_velocity)
     return w
+
+
+def crowd_world(seed=24, n_shell=100, r=0.5e-3, big=3e-3, dt=5e-6, steps=400):
+    """Two large spheres (radius 6 r) in a box, each wrapped in a shell of
+    `n_shell` small spheres (Fibonacci points 0.01 r off its surface, at least
+    2.16 r apart) moving onto it at 0.1 m/s, under gravity: each large sphere
+    lists about 100 partners and is touched by all of them at once (more
+    than 64 list entries and history slots)."""
+    from vedo_amd.world import DOModel, DOStatus, DOWorld, IactModel, SystemParameter
+    rng = scenes.SplitMix64(seed)
+    d = big + r + 0.01 * r
+    k = np.arange(n_shell) + 0.5
+    z = 1.0 - 2.0 * k / n_shell
+    phi = np.pi * (1.0 + 5.0 ** 0.5) * k
+    u = np.stack([np.sqrt(1.0 - z * z) * np.cos(phi), np.sqrt(1.0 - z * z) * np.sin(phi), z], 1)
+    gap = np.linalg.norm(u[:, None] - u[None], axis=2) * d + 4.0 * r * np.eye(n_shell)
+    assert gap.min() > 2.0 * r, "the shell's spheres overlap"
+    m = d + r + 2.0 * r   # a shell's reach from its centre, and a margin to the walls
+    centres = [(m, m, m + 0.2 * r), (3.0 * m + 2.0 * r, m + 0.3 * r, m)]
+    models = [DOModel("Big", "Granule", "mobile", "local", 2500.0, 1.0, (0, 0, 0), "Sphere", (big,)),
+              DOModel("Sphere", "Granule", "mobile", "local", 2500.0, 1.0, (0, 0, 0), "Sphere", (r,))]
+    statuses = [DOStatus("Big", c, (0.0, 0.0, 0.0), (1, 0, 0), (0, 0, 1), (0.0, 0.0, 0.0)) for c in centres]
+    for c in centres:
+        v = rng.uniform(3 * n_shell, -0.02, 0.02).reshape(n_shell, 3) - 0.1 * u
+        w = rng.uniform(3 * n_shell, -5.0, 5.0).reshape(n_shell, 3)
+        for q in range(n_shell):
+            statuses.append(DOStatus("Sphere", tuple(np.array(c) + d * u[q]), tuple(v[q]), (1, 0, 0), (0, 0, 1),
+                                     tuple(w[q])))
+    wm, ws = scenes.box_walls(4.0 * m + 2.0 * r, 2e-3)
+    iact = [IactModel("Granule", "Granule", "ISwBSDBF", scenes.spring_dashpot(2e3, 0.8, 0.6, 0.4, False)),
+            IactModel("Wall", "Granule", "ISwBSDBF", scenes.spring_dashpot(2e3, 0.8, 0.6, 0.4, False))]
+    sp = SystemParameter(0.0, steps * dt, dt, 0.0, (0.0, 0.0, -9.81))
+    return DOWorld(sp, models + wm, iact, statuses + ws)
 
 
 def orbital_world(seed=23):

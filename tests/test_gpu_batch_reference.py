@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 UP = 7
 CASES = ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "box_bond", "box_bond_bt", "box_cyl",
-         "box_orbital", "box_piston", "box_hole"]
+         "box_orbital", "box_piston", "box_hole", "box_crowd"]
 EXACT_LAWS = {1, 2, 3}
 
 

@@ -48,7 +48,7 @@ def _check(r, min_pairs, tol=TOL):
     assert r["sphere_ang_impulse_err_max"] <= tol, r
 
 
-@pytest.mark.parametrize("name", ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "drum_kin"])
+@pytest.mark.parametrize("name", ["box_bsdbf", "box_h", "box_hertz", "box_btsdbf", "box_b2f", "drum_kin", "box_crowd"])
 def test_mixed_step_matches_fp64(name):
     from vedo_amd.precision import compare_precisions
     z, wa = load_golden(name)

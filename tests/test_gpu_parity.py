@@ -136,6 +136,29 @@ def test_gpu_matches_reference(name):
     sim.close()
 
 
+def test_gpu_neighbour_lists_grow_past_64_entries():
+    """box_crowd: each large sphere lists about 100 partners and is touched
+    by all of them at once.  The creation-time rebuild grows the lists past
+    64 entries (the step kernel's wide variant takes them), and the device
+    list check (entries, history slots) stays clean.  The states themselves
+    are checked against the reference's dumps by test_gpu_matches_reference."""
+    from vedo_amd.gpu import GpuSimulation
+    z, wa = load_golden("box_crowd")
+    sim = GpuSimulation(wa)
+    sim.step(60)
+    st = sim.stats()
+    assert st["capacity"] > 64 and st["max_fill"] > 64, st
+    chk = sim.check_lists()
+    assert not any(chk["violations"].values()), chk
+    lo, hi, _ = sim.active()
+    for big in (0, 1):   # the two large spheres are DOs 0 and 1
+        touching = int(((lo == big) | (hi == big)).sum())
+        assert touching > 64, (big, touching)
+    print(f"\n  neighbour capacity {st['capacity']}, longest list {st['max_fill']}, "
+          f"large spheres touched by {[int(((lo == b) | (hi == b)).sum()) for b in (0, 1)]}", flush=True)
+    sim.close()
+
+
 def test_gpu_library_loads_and_rejects_bad_scene():
     """The product path fails loudly on an invalid scene instead of falling back."""
     from vedo_amd.gpu import GpuError, GpuSimulation

@@ -1086,11 +1086,14 @@ static int pbc_candidates(vg_scene *h, const Geo &G, int n, const std::vector<Gr
                            ne, h->pbc_first);
         hipLaunchKernelGGL(k_pbc_lists, dim3(nblk(ne, 256)), dim3(256), 0, st, (const unsigned *)h->pbc_ekey[1],
                            (const unsigned *)h->pbc_eval[1], ne, (const uint2 *)h->pbc_pairs,
-                           (const unsigned *)h->pbc_first, h->ld, h->ko, h->ocount, h->onbr, h->flags);
+                           (const unsigned *)h->pbc_first, h->ld, h->ko, h->ki, h->ocount, h->onbr, h->flags);
     }
     HIPCHK(hipGetLastError());
     return 0;
 }
+
+static void free_ell(Ell &e);
+static int alloc_ell(vg_scene *h, Ell &e, int n, int ki);
 
 static int rebuild(vg_scene *h)
 {
@@ -1354,12 +1357,13 @@ static int rebuild(vg_scene *h)
         OuterArgs oa;
         oa.g = G1w; oa.n = n; oa.ld = h->ld; oa.scenes = h->scenes; oa.cstart = h->cstart; oa.cend = h->cend;
         oa.safe = h->safe; oa.walls = h->walls; oa.t = tables(h); oa.count = h->ocount; oa.nbr = h->onbr;
-        oa.ko = h->ko; oa.single_group = h->single_group; oa.flags = h->flags;
+        oa.ko = h->ko; oa.ki = h->ki; oa.single_group = h->single_group; oa.flags = h->flags;
         oa.ovf = h->batch ? h->ovf : nullptr;
         oa.holes = h->holes;
         if (h->batch) HIPCHK(hipMemsetAsync(h->ovf, 0, sizeof(int) * h->scap, st));
+        int need = 0;   // the longest neighbour list the candidates make, where it exceeds the capacity
         for (;;) {
-            HIPCHK(hipMemsetAsync(h->flags + 5, 0, sizeof(int), st));
+            HIPCHK(hipMemsetAsync(h->flags + 5, 0, 2 * sizeof(int), st));
             if (h->pbc || h->nbs) {
                 if (int rc = pbc_candidates(h, G1w, n, grids)) return rc;
             } else if (h->holes) {
@@ -1367,9 +1371,11 @@ static int rebuild(vg_scene *h)
             } else {
                 hipLaunchKernelGGL(k_outer<false>, dim3(nblk(n, 128)), dim3(128), 0, st, oa);
             }
-            int maxc = 0;
-            HIPCHK(hipMemcpyAsync(&maxc, h->flags + 5, sizeof(int), hipMemcpyDeviceToHost, st));
+            int fl[2] = {0, 0};
+            HIPCHK(hipMemcpyAsync(fl, h->flags + 5, sizeof(fl), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
+            const int maxc = fl[0];
+            need = fl[1];
             if (maxc <= h->ko) break;
             if (h->batch) {
                 // a batch keeps its candidate capacity (growing it reallocates
@@ -1400,12 +1406,39 @@ static int rebuild(vg_scene *h)
             oa.ko = h->ko;
         }
         pc.mark("candidates (synced)");
+        // neighbour-list capacity: a sphere with more entries within the skin
+        // than the lists hold grows both list buffers (up to KI_MAX entries),
+        // with a margin for the lists' growth until the next rebuild; the new
+        // lists are written at the new capacity, the previous ones are read
+        // at theirs
+        const int ki_old = h->ki;
+        const bool grow = need > h->ki;
+        if (grow) {
+            if (need > KI_MAX)
+                return fail(h, VG_ECAPACITY, "a sphere has more than " + std::to_string(KI_MAX) +
+                                                 " neighbours within the skin of contact");
+            const int ki_new = std::min(KI_MAX, need + need / 4 + 4);
+            if (getenv("VG_DEBUG")) fprintf(stderr, "[vg] neighbour capacity %d -> %d\n", h->ki, ki_new);
+            Ell &en = h->ell[h->ecur ^ 1];
+            free_ell(en);
+            if (alloc_ell(h, en, h->ld, ki_new)) return VG_ENOMEM;
+            if (h->record) {
+                // per-entry records of record steps: none are read across a rebuild
+                if (h->rec) HIPCHK(hipFree(h->rec));
+                h->rec = nullptr;
+                if (dalloc(h, &h->rec, (size_t)ki_new * 13 * h->ld)) return VG_ENOMEM;
+            }
+            h->ki = ki_new;
+        }
         // neighbour lists with history carry
         InnerArgs ia;
         ia.g = G1w; ia.gold = G0; ia.ell = h->ell[h->ecur ^ 1]; ia.eold = h->ell[h->ecur];
-        ia.n = n; ia.ld = h->ld; ia.ki = h->ki; ia.perm = h->perm; ia.cnt = h->ocount; ia.nbr = h->onbr;
-        ia.ko = h->ko; ia.scenes = h->scenes; ia.models = h->models; ia.walls = h->walls; ia.flags = h->flags;
-        hipLaunchKernelGGL(k_inner_build, dim3(nblk(n, 128)), dim3(128), (size_t)h->ki * 128 * sizeof(unsigned), st,
+        ia.n = n; ia.ld = h->ld; ia.ki = h->ki; ia.ki_old = ki_old; ia.perm = h->perm; ia.cnt = h->ocount;
+        ia.nbr = h->onbr; ia.ko = h->ko; ia.scenes = h->scenes; ia.models = h->models; ia.walls = h->walls;
+        ia.flags = h->flags;
+        // the previous lists' partners of each thread's sphere are staged in LDS
+        const int ib = ki_old > 64 ? 64 : 128;
+        hipLaunchKernelGGL(k_inner_build, dim3(nblk(n, ib)), dim3(ib), (size_t)ki_old * ib * sizeof(unsigned), st,
                            ia);
         if (h->any_constrained) {
             const int rc = build_cw_pairs(h, G1w, n);
@@ -1426,6 +1459,12 @@ static int rebuild(vg_scene *h)
             HIPCHK(hipStreamSynchronize(st));
             (void)hipFree(d_sid);
             (void)hipFree(d_wid);
+        }
+        if (grow) {
+            // the previous lists, read above, become the spare buffer at the new capacity
+            HIPCHK(hipStreamSynchronize(st));
+            free_ell(h->ell[h->ecur]);
+            if (alloc_ell(h, h->ell[h->ecur], h->ld, h->ki)) return VG_ENOMEM;
         }
     }
     h->n_iact = -1;   // counted on request (vg_num_interactions)
@@ -1489,6 +1528,13 @@ static int alloc_geo(vg_scene *h, Geo &g, int n)
     if (dalloc(h, &g.pos, n) || dalloc(h, &g.vel, n) || dalloc(h, &g.avl, n) || dalloc(h, &g.ori, 6 * (size_t)n))
         return VG_ENOMEM;
     return 0;
+}
+
+static void free_ell(Ell &e)
+{
+    for (void *p : {(void *)e.hdr, (void *)e.ent, (void *)e.slot, (void *)e.hd})
+        if (p) (void)hipFree(p);
+    e = Ell();
 }
 
 static int alloc_ell(vg_scene *h, Ell &e, int n, int ki)
@@ -1651,7 +1697,7 @@ extern "C" int vg_create(const vg_scene_desc *d, vg_scene **out)
     }
     int ki = 48;
     const char *ki_env = getenv("VG_NEIGHBOUR_CAPACITY");
-    if (ki_env) ki = std::max(8, std::min(64, atoi(ki_env)));   // the step kernel keeps a 64-bit entry mask
+    if (ki_env) ki = std::max(8, std::min(KI_MAX, atoi(ki_env)));   // a start: rebuilds grow it as needed
     h->up_iact = d->up_iact ? d->up_iact : 1;
     rc = alloc_batch(h, probe.n, std::max(probe.nw, 1), 1, ki, true);
     if (rc) return bail(rc);
@@ -1688,7 +1734,7 @@ extern "C" int vg_batch_create(const vg_batch_desc *bd, vg_scene **out)
     // large layouts: start the candidate capacity high, since growing
     // reallocates [KO][LD] and stalls the device for seconds at 1e7 slots
     h->ko = 512;
-    const int ki = bd->neighbour_capacity > 0 ? std::max(8, std::min(64, bd->neighbour_capacity)) : 32;
+    const int ki = bd->neighbour_capacity > 0 ? std::max(8, std::min(KI_MAX, bd->neighbour_capacity)) : 32;
     int rc = alloc_batch(h, (int)bd->max_spheres, std::max(bd->max_walls, 1), bd->max_scenes, ki, false);
     if (rc) {
         set_err_create(h->err);
@@ -1903,7 +1949,19 @@ static int launch_step(vg_scene *h, bool rec, bool kt)
           {k_step<true, LS_H, false, true>, k_step<true, LS_H, true, true>},
           {k_step<true, LS_ALL, false, true>, k_step<true, LS_ALL, true, true>},
           {k_step<true, LS_EXT, false, false>, k_step<true, LS_EXT, true, false>}}}};
-    hipLaunchKernelGGL(kfn[uniform_scene(h) ? 1 : 0][rec ? 1 : 0][ls][mix ? 1 : 0], grid, blk, 0, h->stream, a);
+    // lists longer than 64 entries: the wide variants, for the general law sets
+    static const KFn kfn_wide[2][2][2][2] = {
+        {{{k_step<false, LS_ALL, false, false, 2>, k_step<false, LS_ALL, true, false, 2>},
+          {k_step<false, LS_EXT, false, false, 2>, k_step<false, LS_EXT, true, false, 2>}},
+         {{k_step<true, LS_ALL, false, false, 2>, k_step<true, LS_ALL, true, false, 2>},
+          {k_step<true, LS_EXT, false, false, 2>, k_step<true, LS_EXT, true, false, 2>}}},
+        {{{k_step<false, LS_ALL, false, true, 2>, k_step<false, LS_ALL, true, true, 2>},
+          {k_step<false, LS_EXT, false, false, 2>, k_step<false, LS_EXT, true, false, 2>}},
+         {{k_step<true, LS_ALL, false, true, 2>, k_step<true, LS_ALL, true, true, 2>},
+          {k_step<true, LS_EXT, false, false, 2>, k_step<true, LS_EXT, true, false, 2>}}}};
+    const KFn fn = h->ki > 64 ? kfn_wide[uniform_scene(h) ? 1 : 0][rec ? 1 : 0][ls == LS_EXT ? 1 : 0][mix ? 1 : 0]
+                              : kfn[uniform_scene(h) ? 1 : 0][rec ? 1 : 0][ls][mix ? 1 : 0];
+    hipLaunchKernelGGL(fn, grid, blk, 0, h->stream, a);
     if (kt) {
         HIPCHK(hipEventRecord(h->kev[2 * h->kev_used + 1], h->stream));
         h->kev_used++;
@@ -1997,8 +2055,8 @@ static int graph_period(vg_scene *h)
     StepArgs a0 = step_args(h);
     a0.stamp = 0;
     // with the kernel specialisation launch_step chooses (law set, precision, one-scene path)
-    const long long extra[6] = {(long long)(h->step_count % 6), h->nw_used, (long long)U, h->precision,
-                                (long long)uniform_scene(h), law_set(h)};
+    const long long extra[7] = {(long long)(h->step_count % 6), h->nw_used, (long long)U, h->precision,
+                                (long long)uniform_scene(h), law_set(h), h->ki};
     std::vector<unsigned char> key(sizeof a0 + sizeof extra);
     std::memcpy(key.data(), &a0, sizeof a0);
     std::memcpy(key.data() + sizeof a0, extra, sizeof extra);
@@ -2094,7 +2152,9 @@ extern "C" int vg_sync(vg_scene *h)
     int flags[8];
     HIPCHK(hipMemcpy(flags, h->flags, sizeof(flags), hipMemcpyDeviceToHost));
     if (flags[3] & ERR_INNER_CAP)
-        return fail(h, VG_ECAPACITY, "neighbour list capacity exceeded (set VG_NEIGHBOUR_CAPACITY)");
+        return fail(h, VG_ECAPACITY,
+                    "a neighbour list outgrew its capacity between rebuilds (VG_NEIGHBOUR_CAPACITY or the batch's "
+                    "neighbour_capacity sets a larger start, at most 127)");
     if (flags[3] & ERR_CW)
         return fail(h, VG_EUNSUPPORTED, "a sphere is a candidate of more than four constrained walls");
     if (h->timing) {

@@ -102,6 +102,11 @@ struct StepArgs {
 
 enum { ERR_INNER_CAP = 1, ERR_OUTER_CAP = 2, ERR_CW = 4 };
 
+// Neighbour-list entries per sphere: the list header keeps the count in 7
+// bits.  Lists longer than 64 entries take the step kernel's wide variant
+// (two 64-bit words for its entry and history-slot sets).
+constexpr int KI_MAX = 127;
+
 // Constrained walls a sphere may have among its candidates (pair slots per sphere).
 constexpr int CW_SLOTS = 4;
 
@@ -134,7 +139,8 @@ struct OuterArgs {
     unsigned *count;         // candidates per sphere [LD]
     unsigned *nbr;           // ELL [KO][LD]
     int ko;
-    int *flags;              // [5]: largest candidate count seen
+    int ki;                  // neighbour-list capacity: spheres with more entries within the skin raise flags[6]
+    int *flags;              // [5]: largest candidate count seen; [6]: largest neighbour count above ki
     int *ovf;                // per scene: a sphere has more candidates than ko (batches; may be null)
     const double4 *holes;    // per wall slot: hole radius, x / y offset, 1 for a holed plate (HOLE kernels)
 };
@@ -163,7 +169,7 @@ struct PbcArgs {
 struct InnerArgs {
     Geo g, gold;             // new order / previous order
     Ell ell, eold;
-    int n, ld, ki;
+    int n, ld, ki, ki_old;   // capacities of the new and the previous lists
     const unsigned *perm;    // new index -> previous index
     const unsigned *cnt, *nbr;
     int ko;
@@ -179,7 +185,7 @@ struct ActiveOut {
     double rec[13];
 };
 
-template <bool REC, int LS, bool MIX, bool UNI> __global__ void k_step(StepArgs a);
+template <bool REC, int LS, bool MIX, bool UNI, int W = 1> __global__ void k_step(StepArgs a);
 __global__ void k_walls(WallDev *walls, int nw, const SceneDev *scenes, double *stime, const double *assigned,
                         const WallDev *ref, const WallKin *kin, const double *cw_imp, int *rflags, int step);
 __global__ void k_cw_count(int n, int ld, int ko, const unsigned *cnt, const unsigned *nbr, const WallDev *walls,
@@ -218,7 +224,8 @@ __global__ void k_pbc_emit(PbcArgs a);
 __global__ void k_pbc_entries(const uint2 *pairs, unsigned np, int ld, unsigned *key, unsigned *val);
 __global__ void k_pbc_segstart(const unsigned *key, unsigned ne, unsigned *first);
 __global__ void k_pbc_lists(const unsigned *key, const unsigned *val, unsigned ne, const uint2 *pairs,
-                            const unsigned *first, int ld, int ko, unsigned *count, unsigned *nbr, int *flags);
+                            const unsigned *first, int ld, int ko, int ki, unsigned *count, unsigned *nbr,
+                            int *flags);
 __global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int b, int e, int ld, int ko,
                              unsigned long long *out);
 __global__ void k_inner_build(InnerArgs a);

@@ -93,12 +93,109 @@ __device__ inline void store_hist(const Ell &e, int sl, int n, int i, const Hist
     hist_st(make_double2(hs.s.x, hs.s.y), p);
     hist_st(make_double2(hs.s.z, AUX ? hs.aux : 0.0), p + 1);
 }
+// A set of a sphere's list positions or history slots, 0 .. 64 W - 1: one
+// 64-bit word per 64.  The words are never indexed by a run-time value, so
+// they stay in registers.
+template <int W>
+struct Bits {
+    unsigned long long w[W];
+    __device__ Bits()
+    {
+#pragma unroll
+        for (int q = 0; q < W; ++q) w[q] = 0ull;
+    }
+    __device__ void set(int k)
+    {
+        if constexpr (W == 1) {
+            w[0] |= 1ull << k;   // k < 64
+            return;
+        }
+#pragma unroll
+        for (int q = 0; q < W; ++q)
+            if ((k >> 6) == q) w[q] |= 1ull << (k & 63);
+    }
+    __device__ bool has(int k) const
+    {
+        bool r = false;
+#pragma unroll
+        for (int q = 0; q < W; ++q)
+            if ((k >> 6) == q) r = (w[q] >> (k & 63)) & 1ull;
+        return r;
+    }
+    __device__ bool any() const
+    {
+        unsigned long long o = 0ull;
+#pragma unroll
+        for (int q = 0; q < W; ++q) o |= w[q];
+        return o != 0ull;
+    }
+    __device__ int count() const
+    {
+        int c = 0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) c += __popcll(w[q]);
+        return c;
+    }
+    // the lowest member, which leaves the set (the set is not empty)
+    __device__ int pop_first()
+    {
+        if constexpr (W == 1) {
+            const int k = __builtin_ctzll(w[0]);
+            w[0] &= w[0] - 1ull;
+            return k;
+        }
+        int k = 0;
+        bool done = false;
+#pragma unroll
+        for (int q = 0; q < W; ++q)
+            if (!done && w[q]) {
+                k = 64 * q + __builtin_ctzll(w[q]);
+                w[q] &= w[q] - 1ull;
+                done = true;
+            }
+        return k;
+    }
+    // the highest member leaves the set (the set is not empty)
+    __device__ void drop_last()
+    {
+        if constexpr (W == 1) {
+            w[0] &= ~(1ull << (63 - __builtin_clzll(w[0])));
+            return;
+        }
+        bool done = false;
+#pragma unroll
+        for (int q = W - 1; q >= 0; --q)
+            if (!done && w[q]) {
+                w[q] &= ~(1ull << (63 - __builtin_clzll(w[q])));
+                done = true;
+            }
+    }
+    __device__ void remove(const Bits &o)
+    {
+#pragma unroll
+        for (int q = 0; q < W; ++q) w[q] &= ~o.w[q];
+    }
+};
+
 // The lowest history slot not in `used`, which then holds it.  A sphere's
-// slots in use never outnumber its list entries (at most 64).
-__device__ inline int take_slot(unsigned long long &used)
+// slots in use never outnumber its list entries (at most KI_MAX < 64 W).
+template <int W>
+__device__ inline int take_slot(Bits<W> &used)
 {
-    const int z = __builtin_ctzll(~used);
-    used |= 1ull << z;
+    if constexpr (W == 1) {
+        const int z = __builtin_ctzll(~used.w[0]);
+        used.w[0] |= 1ull << z;
+        return z;
+    }
+    int z = 0;
+    bool done = false;
+#pragma unroll
+    for (int q = 0; q < W; ++q)
+        if (!done && ~used.w[q]) {
+            z = 64 * q + __builtin_ctzll(~used.w[q]);
+            used.w[q] |= 1ull << (z & 63);
+            done = true;
+        }
     return z;
 }
 
@@ -350,7 +447,7 @@ __device__ inline int block_scan(int v, int *s_wsum, int &total)
 
 // 3 workgroups per CU (12 wavefronts), as the LDS allows: the register
 // budget is capped at 168 VGPRs, the larger law sets spill a few registers
-template <bool REC, int LS, bool MIX, bool UNI>
+template <bool REC, int LS, bool MIX, bool UNI, int W>
 __global__ __launch_bounds__(STEP_NT, STEP_WG_PER_CU) void k_step(StepArgs a)
 {
     constexpr int NT = STEP_NT;   // threads per block: the launch's block size
@@ -361,6 +458,9 @@ __global__ __launch_bounds__(STEP_NT, STEP_WG_PER_CU) void k_step(StepArgs a)
     // of each thread's first QPT queued entries in the result area
     __shared__ double4 s_pos[NT], s_vel[NT], s_avl[NT];
     __shared__ int s_mdl[NT];
+    // queue item: thread << (QB + 1) | list half << QB | entry, QB bits for the entry
+    constexpr int QB = W == 1 ? 6 : 7;
+    static_assert((NT - 1) << (QB + 1) < 65536, "a queue item fits 16 bits");
     __shared__ unsigned short s_item[NT * QPT];
     __shared__ unsigned s_qent[NT * QPT];
     __shared__ unsigned char s_qslot[NT * QPT];
@@ -427,8 +527,9 @@ __global__ __launch_bounds__(STEP_NT, STEP_WG_PER_CU) void k_step(StepArgs a)
 
     // Pass 1: surely separated sphere pairs are settled here (Clean status,
     // no impulse); every other entry is queued for the full solve.
-    unsigned long long todo = 0;
-    unsigned long long used = 0;   // history slots of the entries in contact
+    // W 64-bit words per set: lists of up to 64 entries take one
+    Bits<W> todo;
+    Bits<W> used;   // history slots of the entries in contact
     int nstage = 0;
     // entries PE at a time (their words and slots in one round of loads),
     // then in groups of PF: the partner-position gathers of a group are
@@ -457,7 +558,7 @@ __global__ __launch_bounds__(STEP_NT, STEP_WG_PER_CU) void k_step(StepArgs a)
                 const int k = k1 + g + u;
                 if (k >= cnt) break;
                 const unsigned e = ew[g + u], j = e & PARTNER_MASK;
-                if (e & F_CONTACT) used |= 1ull << es[g + u];
+                if (e & F_CONTACT) used.set(es[g + u]);
                 bool full = true;
                 if (pj[u].w >= 0.0) {   // a sphere partner, unbonded laws
                     d3 dx = ld3(pj[u]) - xi;
@@ -473,7 +574,7 @@ __global__ __launch_bounds__(STEP_NT, STEP_WG_PER_CU) void k_step(StepArgs a)
                     }
                 }
                 if (full) {
-                    todo |= 1ull << k;
+                    todo.set(k);
                     if (nstage < QPT) {
                         s_stage[nstage * NT + t] = e;
                         s_stage_slot[nstage * NT + t] = es[g + u];
@@ -490,12 +591,12 @@ __global__ __launch_bounds__(STEP_NT, STEP_WG_PER_CU) void k_step(StepArgs a)
     // order, so the sums round exactly as a sequential walk of its list.
     d3 I = mk(0, 0, 0), L = mk(0, 0, 0);
     unsigned n_contact = 0;
-    unsigned long long rem = todo;
+    Bits<W> rem = todo;
     bool first_round = true;
     for (;;) {
-        unsigned long long take = rem;
-        for (int q = __popcll(take); q > QPT; --q) take &= ~(1ull << (63 - __builtin_clzll(take)));
-        const int c = __popcll(take);
+        Bits<W> take = rem;
+        for (int q = take.count(); q > QPT; --q) take.drop_last();
+        const int c = take.count();
         int total = 0;
         const int off = block_scan<NT>(c, s_wsum, total);
         {
@@ -503,28 +604,27 @@ __global__ __launch_bounds__(STEP_NT, STEP_WG_PER_CU) void k_step(StepArgs a)
             // words and slots pass 1 staged in order; later rounds read them
             // again.  An entry not in contact gets a free slot for the
             // history it keeps if the solve starts a contact.
-            unsigned long long tk = take;
+            Bits<W> tk = take;
             for (int q = 0; q < c; ++q) {
-                const int k = __builtin_ctzll(tk);
-                tk &= tk - 1;
+                const int k = tk.pop_first();
                 const size_t x = eidx(h, k, a.ki, a.ld, i);
                 const unsigned w = first_round ? s_stage[q * NT + t] : a.ell.ent[x];
                 int sl;
                 if (w & F_CONTACT) sl = first_round ? s_stage_slot[q * NT + t] : a.ell.slot[x];
                 else sl = take_slot(used);
-                s_item[off + q] = (unsigned short)((t << 7) | (h << 6) | k);
+                s_item[off + q] = (unsigned short)((t << (QB + 1)) | (h << QB) | k);
                 s_qent[off + q] = w;
                 s_qslot[off + q] = (unsigned char)sl;
             }
         }
-        rem &= ~take;
+        rem.remove(take);
         first_round = false;
         __syncthreads();
         for (int r0 = 0; r0 < total; r0 += NT) {
             const int q = r0 + t;
             if (q < total) {
                 const unsigned it = s_item[q];
-                const int lt = (int)(it >> 7), hq = (int)((it >> 6) & 1), k = (int)(it & 63);
+                const int lt = (int)(it >> (QB + 1)), hq = (int)((it >> QB) & 1), k = (int)(it & ((1 << QB) - 1));
                 const int io = b0 + lt;
                 const double4 po = s_pos[lt], vo = s_vel[lt], wo = s_avl[lt];
                 int go;
@@ -554,7 +654,7 @@ __global__ __launch_bounds__(STEP_NT, STEP_WG_PER_CU) void k_step(StepArgs a)
             }
             __syncthreads();
         }
-        if (!__syncthreads_or(rem != 0)) break;
+        if (!__syncthreads_or(rem.any())) break;
     }
     if (n_contact) atomicAdd(&s_ncontact, n_contact);
 
@@ -634,6 +734,19 @@ template __global__ void k_step<false, LS_H, true, true>(StepArgs);
 template __global__ void k_step<true, LS_H, true, true>(StepArgs);
 template __global__ void k_step<false, LS_ALL, true, true>(StepArgs);
 template __global__ void k_step<true, LS_ALL, true, true>(StepArgs);
+// wide lists (more than 64 entries per sphere): the general law sets only
+template __global__ void k_step<false, LS_ALL, false, false, 2>(StepArgs);
+template __global__ void k_step<true, LS_ALL, false, false, 2>(StepArgs);
+template __global__ void k_step<false, LS_ALL, true, false, 2>(StepArgs);
+template __global__ void k_step<true, LS_ALL, true, false, 2>(StepArgs);
+template __global__ void k_step<false, LS_ALL, false, true, 2>(StepArgs);
+template __global__ void k_step<true, LS_ALL, false, true, 2>(StepArgs);
+template __global__ void k_step<false, LS_ALL, true, true, 2>(StepArgs);
+template __global__ void k_step<true, LS_ALL, true, true, 2>(StepArgs);
+template __global__ void k_step<false, LS_EXT, false, false, 2>(StepArgs);
+template __global__ void k_step<true, LS_EXT, false, false, 2>(StepArgs);
+template __global__ void k_step<false, LS_EXT, true, false, 2>(StepArgs);
+template __global__ void k_step<true, LS_EXT, true, false, 2>(StepArgs);
 
 // Wall responses after the spheres' (DOFixedQuasiPlate.h:16-19,
 // DOOrbitalQuasiPlate.cpp:18-34) and kinematic prescriptions
@@ -1120,10 +1233,11 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
     unsigned cx, cy, cz;
     cell_of(G, xs, cx, cy, cz);
     const double skin = Sg.skin;
-    unsigned cnt = 0;
+    unsigned cnt = 0, near_cnt = 0;   // candidates; those within the skin (the neighbour list's length)
     auto emit = [&](unsigned j) {
         if (cnt < ko) nbr[(size_t)cnt * ld + s] = j;
         ++cnt;
+        near_cnt += (j & NBR_NEAR) ? 1u : 0u;
     };
     // the contact depth of an accepted pair (CDSphere_Sphere's, the same bits
     // from either side) marks it for the neighbour list: within the skin of
@@ -1203,6 +1317,7 @@ __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
         atomicMax(&a.flags[5], (int)cnt);
         if (a.ovf) atomicOr(&a.ovf[scene], 1);
     }
+    if (near_cnt > (unsigned)a.ki) atomicMax(&a.flags[6], (int)near_cnt);
 }
 template __global__ void k_outer<false>(OuterArgs);
 template __global__ void k_outer<true>(OuterArgs);
@@ -1411,7 +1526,8 @@ __global__ void k_pbc_segstart(const unsigned *key, unsigned ne, unsigned *first
 
 // each sphere's entries, in pair order, into its candidate list
 __global__ void k_pbc_lists(const unsigned *key, const unsigned *val, unsigned ne, const uint2 *pairs,
-                            const unsigned *first, int ld, int ko, unsigned *count, unsigned *nbr, int *flags)
+                            const unsigned *first, int ld, int ko, int ki, unsigned *count, unsigned *nbr,
+                            int *flags)
 {
     const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
@@ -1425,6 +1541,13 @@ __global__ void k_pbc_lists(const unsigned *key, const unsigned *val, unsigned n
     if (e + 1 == ne || key[e + 1] != s) {
         count[s] = t + 1;
         if ((int)(t + 1) > ko) atomicMax(&flags[5], (int)(t + 1));
+        if ((int)(t + 1) > ki) {
+            // the sphere's entries within the skin (its neighbour list's length),
+            // from the pairs this kernel reads (the entries' marks are the pair's)
+            int near_cnt = 0;
+            for (unsigned e2 = first[s]; e2 <= e; ++e2) near_cnt += (pairs[val[e2] >> 1].y & NBR_NEAR) ? 1 : 0;
+            if (near_cnt > ki) atomicMax(&flags[6], near_cnt);
+        }
     }
 }
 
@@ -1466,15 +1589,15 @@ __global__ void k_check_lists(Geo g, Ell ell, int n, int ld, int ki, int ko, con
     const unsigned hdr = ell.hdr[i];
     const int h = hdr >> 7, cnt = hdr & 127;
     if (cnt > ki) bad(2, cnt);
-    unsigned long long used = 0;
+    Bits<2> used;
     for (int k = 0; k < min(cnt, ki); ++k) {
         const size_t x = ((size_t)(h * ki + k)) * (size_t)ld + i;
         if (!partner_ok(ell.ent[x] & PARTNER_MASK)) bad(3, k);
         if (ell.ent[x] & F_CONTACT) {
             // a contact's history slot: within the capacity, no two contacts share one
             const unsigned sl = ell.slot[x];
-            if (sl >= (unsigned)ki || sl >= 64u || (used >> sl & 1ull)) bad(4, k);
-            else used |= 1ull << sl;
+            if (sl >= (unsigned)ki || sl >= (unsigned)KI_MAX || used.has((int)sl)) bad(4, k);
+            else used.set((int)sl);
         }
     }
     const unsigned oc = o_cnt[i];
@@ -1513,7 +1636,7 @@ __global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int b, in
 __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    extern __shared__ unsigned old_oid[];   // [ki][blockDim]
+    extern __shared__ unsigned old_oid[];   // [ki_old][blockDim]
     if (s >= a.n) return;
     const unsigned *__restrict__ oid = a.g.oid;
     const unsigned *__restrict__ nbr = a.nbr;
@@ -1525,7 +1648,7 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
     const int h_old = hdr >> 7, c_old = hdr & 127;
     int n_hist = 0;   // the new list's contacts take history slots 0, 1, ... in list order
     for (int k = 0; k < c_old; ++k) {
-        const unsigned j = ent_old[eidx(h_old, k, a.ki, a.ld, p)] & PARTNER_MASK;
+        const unsigned j = ent_old[eidx(h_old, k, a.ki_old, a.ld, p)] & PARTNER_MASK;
         old_oid[k * blockDim.x + threadIdx.x] = j < (unsigned)a.ld ? a.gold.oid[j] : 0x80000000u + (j - a.ld);
     }
     const unsigned my_oid = oid[s];
@@ -1562,8 +1685,8 @@ __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
             if (jv[u] & NBR_ACTIVE) {
                 for (int k = 0; k < c_old; ++k)
                     if (old_oid[k * blockDim.x + threadIdx.x] == key) {
-                        f = (ent_old[eidx(h_old, k, a.ki, a.ld, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
-                        if (f & F_CONTACT) load_hist(a.eold, a.eold.slot[eidx(h_old, k, a.ki, a.ld, p)], a.ld, p, hs);
+                        f = (ent_old[eidx(h_old, k, a.ki_old, a.ld, p)] & ~PARTNER_MASK & ~F_SLAVE) | F_NOBACKUP;
+                        if (f & F_CONTACT) load_hist(a.eold, a.eold.slot[eidx(h_old, k, a.ki_old, a.ld, p)], a.ld, p, hs);
                         break;
                     }
             }
@@ -1708,10 +1831,10 @@ __global__ void k_restore_history(Geo g, Ell ell, int b, int e, int ld, int ki, 
     const unsigned id = sphere_id[g.oid[i]];
     // history slots of the listed contacts; a restored contact keeps its
     // entry's slot or takes a free one
-    unsigned long long used = 0;
+    Bits<2> used;
     for (int k = 0; k < cnt; ++k) {
         const size_t x = eidx(h, k, ki, ld, i);
-        if (ell.ent[x] & F_CONTACT) used |= 1ull << ell.slot[x];
+        if (ell.ent[x] & F_CONTACT) used.set(ell.slot[x]);
     }
     for (int k = 0; k < cnt; ++k) {
         const size_t x = eidx(h, k, ki, ld, i);

@@ -1328,11 +1328,23 @@ static int rebuild(vg_scene *h)
         // within each scene, so each cell's spheres stay in id order
         int slot_bits = 0;
         while ((1 << slot_bits) <= max_slot + 1) ++slot_bits;
-        const unsigned long long retired_key = (unsigned long long)(max_slot + 1) << 51;
-        const int end_bit = std::min(64, 51 + slot_bits);
+        // cell coordinates are at most nc (the grid's cells per axis without
+        // its margin); below 2^b their Hilbert index lies below 2^(3b) (the
+        // curve fills the corner cube [0, 2^b)^3 first), so the scene slot sits
+        // right above the largest grid's 3b bits and the sort skips the index
+        // bits that are zero in every key; the order is the same
+        int b = 1;
+        for (int k : live) {
+            const GridDev &gd = grids[k];
+            const int m = std::max(gd.nx, std::max(gd.ny, gd.nz)) - 2;   // nc + 1
+            while ((1 << b) < m) ++b;
+        }
+        const int hbits = 3 * b;
+        const unsigned long long retired_key = (unsigned long long)(max_slot + 1) << hbits;
+        const int end_bit = std::min(64, hbits + slot_bits);
         hipLaunchKernelGGL(k_cellkey, dim3(nblk(n_total, 256)), dim3(256), 0, st, G0, n_total,
                            (const SceneDev *)h->scenes, (const ModelDev *)h->models, (const unsigned *)h->inv,
-                           retired_key, h->keys, h->svals);
+                           retired_key, hbits, h->keys, h->svals);
         size_t need = 0;
         (void)rocprim::radix_sort_pairs(nullptr, need, h->keys, h->skeys, h->svals, h->perm, (size_t)n_total, 0,
                                         end_bit, st);

@@ -211,7 +211,7 @@ __global__ void k_qsel_pick(QselTarget *tg, int m, int pass, unsigned *hist, dou
 __global__ void k_quantiles(const double *xs, const double *ys, const double *zs, const long long *idx, int m,
                             double *out);
 __global__ void k_cellkey(Geo g, int n_ranks, const SceneDev *scenes, const ModelDev *models, const unsigned *inv,
-                          unsigned long long retired_key, unsigned long long *keys, unsigned *vals);
+                          unsigned long long retired_key, int hbits, unsigned long long *keys, unsigned *vals);
 __global__ void k_reorder(Geo src, Geo dst, int n, int ld, const unsigned *perm, const SceneDev *scenes,
                           const ModelDev *models, double *safe, int *lin, const double *imp_src, double *imp_dst);
 __global__ void k_fill_int(int *p, long long n, int v);

@@ -1117,10 +1117,11 @@ __device__ inline unsigned long long hilbert3(unsigned x, unsigned y, unsigned z
 }
 
 __global__ void k_cellkey(Geo g, int n_ranks, const SceneDev *scenes, const ModelDev *models, const unsigned *inv,
-                          unsigned long long retired_key, unsigned long long *keys, unsigned *vals)
+                          unsigned long long retired_key, int hbits, unsigned long long *keys, unsigned *vals)
 {
-    // key = scene slot (bits 51..63) | Hilbert index of the cell (17 bits per axis);
-    // ranks run scene by scene in DO-id order, retired scenes last
+    // key = scene slot << hbits | Hilbert index of the cell (below 2^hbits for
+    // every placed scene's grid); ranks run scene by scene in DO-id order,
+    // retired scenes last
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_ranks) return;
     const unsigned i = inv[t];
@@ -1134,7 +1135,7 @@ __global__ void k_cellkey(Geo g, int n_ranks, const SceneDev *scenes, const Mode
     const double4 p = g.pos[i];
     unsigned cx, cy, cz;
     cell_of(S.G, mk(p.x, p.y, p.z), cx, cy, cz);
-    keys[t] = ((unsigned long long)sc << 51) | hilbert3(cx, cy, cz);
+    keys[t] = ((unsigned long long)sc << hbits) | hilbert3(cx, cy, cz);
 }
 
 // Gather state into cell order; per-sphere safe distance

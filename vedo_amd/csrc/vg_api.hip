@@ -1370,10 +1370,6 @@ static int rebuild(vg_scene *h)
         oa.g = G1w; oa.n = n; oa.ld = h->ld; oa.scenes = h->scenes; oa.cstart = h->cstart; oa.cend = h->cend;
         oa.safe = h->safe; oa.walls = h->walls; oa.t = tables(h); oa.count = h->ocount; oa.nbr = h->onbr;
         oa.ko = h->ko; oa.ki = h->ki; oa.single_group = h->single_group; oa.flags = h->flags;
-        // batches of dense scenes: each XCD's L2 serves a contiguous range of
-        // spheres and their partners (one-scene handles keep the hardware's
-        // round-robin, which measured faster on the sparse 1M drum)
-        oa.xcd = h->batch ? 1 : 0;
         oa.ovf = h->batch ? h->ovf : nullptr;
         oa.holes = h->holes;
         if (h->batch) HIPCHK(hipMemsetAsync(h->ovf, 0, sizeof(int) * h->scap, st));
@@ -1452,7 +1448,6 @@ static int rebuild(vg_scene *h)
         ia.n = n; ia.ld = h->ld; ia.ki = h->ki; ia.ki_old = ki_old; ia.perm = h->perm; ia.cnt = h->ocount;
         ia.nbr = h->onbr; ia.ko = h->ko; ia.scenes = h->scenes; ia.models = h->models; ia.walls = h->walls;
         ia.flags = h->flags;
-        ia.xcd = oa.xcd;
         // the previous lists' partners of each thread's sphere are staged in LDS
         const int ib = ki_old > 64 ? 64 : 128;
         hipLaunchKernelGGL(k_inner_build, dim3(nblk(n, ib)), dim3(ib), (size_t)ki_old * ib * sizeof(unsigned), st,

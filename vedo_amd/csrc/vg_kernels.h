@@ -140,7 +140,6 @@ struct OuterArgs {
     unsigned *nbr;           // ELL [KO][LD]
     int ko;
     int ki;                  // neighbour-list capacity: spheres with more entries within the skin raise flags[6]
-    int xcd;                 // workgroups take contiguous per-XCD ranges of spheres (batches)
     int *flags;              // [5]: largest candidate count seen; [6]: largest neighbour count above ki
     int *ovf;                // per scene: a sphere has more candidates than ko (batches; may be null)
     const double4 *holes;    // per wall slot: hole radius, x / y offset, 1 for a holed plate (HOLE kernels)
@@ -171,7 +170,6 @@ struct InnerArgs {
     Geo g, gold;             // new order / previous order
     Ell ell, eold;
     int n, ld, ki, ki_old;   // capacities of the new and the previous lists
-    int xcd;                 // workgroups take contiguous per-XCD ranges of spheres (batches)
     const unsigned *perm;    // new index -> previous index
     const unsigned *cnt, *nbr;
     int ko;

@@ -1201,7 +1201,7 @@ __device__ inline bool accept_wall(const OuterArgs &a, const SceneDev &S, d3 xs,
 template <bool HOLE>
 __global__ __launch_bounds__(128) void k_outer(OuterArgs a)
 {
-    const int s = (a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.n) return;
     const double4 *__restrict__ pos = a.g.pos;
     const double *__restrict__ safe = a.safe;
@@ -1636,7 +1636,7 @@ __global__ void k_count_iact(const unsigned *cnt, const unsigned *nbr, int b, in
 // SimMediator.cpp:690-707); everything else starts from ImpactStatus().
 __global__ __launch_bounds__(128) void k_inner_build(InnerArgs a)
 {
-    const int s = (a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     extern __shared__ unsigned old_oid[];   // [ki_old][blockDim]
     if (s >= a.n) return;
     const unsigned *__restrict__ oid = a.g.oid;
